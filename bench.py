@@ -1,0 +1,214 @@
+"""Benchmark: search QPS on the dense-space-partition (second-candidate) index.
+
+Workload (seeded synthetic data): D=128 vectors from a 64-component Gaussian
+mixture rounded to integers in [0,255]; 100k training vectors, 1M base
+vectors, 10k queries.  Index: cells keyed by the (nearest, second-nearest)
+pair of K=256 coarse centroids over the full 128 dimensions (the reference's
+SCQuery::search_mr_ivf shape), m=8 x 256 PQ codes on the residuals.  Search:
+multi-sequence traversal until T=10k candidates, top-100.
+
+One step = one batched search of all 10k queries with the index and queries
+resident in HBM.  Printed: one JSON line (rank 0).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "second-candidate_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--ntrain", type=int, default=100_000)
+    ap.add_argument("--nq", type=int, default=10_000)
+    ap.add_argument("--kc", type=int, default=256)
+    ap.add_argument("--T", type=int, default=10_000)
+    ap.add_argument("--R", type=int, default=100)
+    ap.add_argument("--w", type=int, default=100_000)
+    ap.add_argument("--iters", type=int, default=10, help="Lloyd iterations (setup)")
+    ap.add_argument("--per-query", type=int, default=200,
+                    help="queries timed one call at a time (drop-in plain form)")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0,
+                    help="target CPU work for the oracle baseline (0 = skip)")
+    ap.add_argument("--recall-sample", type=int, default=100)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print("bench.py measures one GPU; this build runs on a single MI355X "
+              "(--gpus %d is not supported)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import sc_amd as S
+
+    dim, mp, kp, kc = 128, 8, 256, args.kc
+    ctx = S.Ctx(0)
+    mix = S.mix_d128()
+
+    # ---------------- setup (untimed): train, encode, build
+    t0 = time.perf_counter()
+    xt = S.gen_rows(ctx, mix, S.STREAM_TRAIN, 0, args.ntrain, dim)
+    cq, _ = S.kmeans(ctx, xt, kc, args.iters, seed=1, want_labels=False)
+    S.residual(ctx, xt, cq, 1, kc, 0)
+    rh = xt.numpy()
+    pq = np.stack([S.kmeans(ctx, S.Dev.from_numpy(ctx, np.ascontiguousarray(rh[:, s * 16:(s + 1) * 16])),
+                            kp, args.iters, seed=2 + s, want_labels=False)[0].numpy()
+                   for s in range(mp)])
+    del rh, xt
+    dpq = S.Dev.from_numpy(ctx, pq)
+    ctx.sync()
+    t_train = time.perf_counter() - t0
+    log("trained codebooks in %.2fs" % t_train)
+
+    x = S.gen_rows(ctx, mix, S.STREAM_BASE, 0, args.n, dim)
+    ctx.sync()
+    t0 = time.perf_counter()
+    cid, codes, _ = S.encode(ctx, x, cq, dpq, 1, kc, mp, kp, 2, S.PREC_FP32)
+    ctx.sync()
+    t_encode = time.perf_counter() - t0
+    ix = S.Index.build(ctx, cid, codes, dim, cq, 1, kc, dpq, mp, kp, 2)
+    del cid, codes
+    q = S.gen_rows(ctx, mix, S.STREAM_QUERY, 0, args.nq, dim)
+    ctx.sync()
+    log("encoded %d vectors in %.3fs; index built" % (args.n, t_encode))
+
+    ids = S.Dev(ctx, (args.nq, args.R), np.int32)
+    dists = S.Dev(ctx, (args.nq, args.R), np.float32)
+    sums = S.Dev(ctx, (args.nq,), np.int32)
+
+    def step():
+        return ix.search(q, S.KIND_SC, args.R, args.w, args.T, form=S.FORM_PLAIN, ids=ids,
+                         dists=dists, sums=sums, stats=True)[3]
+
+    for _ in range(args.warmup):
+        step()
+    ctx.sync()
+    stats = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        stats.append(step())
+    ctx.sync()
+    elapsed = time.perf_counter() - t0
+    ms_step = elapsed * 1e3 / args.steps
+    qps = args.nq * args.steps / elapsed
+    st = stats[-1]
+    ms_scan = float(np.mean([s.ms_scan for s in stats]))
+    cand = st.candidates
+    scan_bytes = cand * mp + args.nq * args.R * 4
+    achieved = scan_bytes / (ms_scan * 1e-3) / 1e9
+    log("batched: %.1f ms/step, %.0f QPS; stages lut %.3f rank %.3f trav %.3f scan %.3f ms; "
+        "%.0f candidates/query" % (ms_step, qps, st.ms_lut, st.ms_rank, st.ms_traverse,
+                                   st.ms_scan, cand / args.nq))
+
+    # ---------------- drop-in plain form: one query per call
+    pq_n = min(args.per_query, args.nq)
+    qh = q.numpy()
+    one_ids = S.Dev(ctx, (1, args.R), np.int32)
+    one_d = S.Dev(ctx, (1, args.R), np.float32)
+    one_s = S.Dev(ctx, (1,), np.int32)
+    qd1 = [S.Dev.from_numpy(ctx, qh[i:i + 1]) for i in range(pq_n)]
+    ix.search(qd1[0], S.KIND_SC, args.R, args.w, args.T, ids=one_ids, dists=one_d, sums=one_s,
+              stats=False)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for i in range(pq_n):
+        ix.search(qd1[i], S.KIND_SC, args.R, args.w, args.T, ids=one_ids, dists=one_d,
+                  sums=one_s, stats=False)
+        one_ids.numpy()                      # the reference API returns host arrays per call
+    qps_single = pq_n / (time.perf_counter() - t0)
+    log("per-query drop-in calls: %.0f QPS" % qps_single)
+
+    # ---------------- recall@100 on a sample (exact brute-force ground truth)
+    import sco
+    gids = ids.numpy()
+    recall = None
+    if args.recall_sample > 0:
+        ns = min(args.recall_sample, args.nq)
+        xh = x.numpy()
+        gt = sco.bruteforce_nn1(xh, qh[:ns])
+        recall = sco.recall(gt, gids[:ns], args.R)
+        del xh
+        log("recall@%d on %d queries: %.3f" % (args.R, ns, recall))
+
+    # ---------------- CPU baseline: the OpenMP oracle on a bounded sample
+    cpu = None
+    if args.cpu_seconds > 0:
+        e = ix.export()
+        oix = sco.OracleIndex(dim, 1, kc, mp, kp, cq.numpy(), pq, e["L"], e["pid"], e["codes"])
+        threads = sco.nthreads()
+        n_cal = min(64, args.nq)
+        t0 = time.perf_counter()
+        oids, od, osum = oix.search_batch(sco.SC, qh[:n_cal], args.R, args.w, args.T,
+                                          threads=threads)
+        dt = time.perf_counter() - t0
+        ns = int(min(args.nq, max(n_cal, n_cal * args.cpu_seconds / max(dt, 1e-6))))
+        t0 = time.perf_counter()
+        oids, od, osum = oix.search_batch(sco.SC, qh[:ns], args.R, args.w, args.T,
+                                          threads=threads)
+        dt = time.perf_counter() - t0
+        match = bool(np.array_equal(oids, gids[:ns]))
+        cpu = {"value": round(ns / dt, 2), "unit": "queries/s", "cores": threads,
+               "kind": "port",
+               "sample": "%d of the %d queries, same index, OpenMP over queries" % (ns, args.nq),
+               "ids_match_gpu": match}
+        log("cpu oracle: %.1f QPS on %d threads (%d queries), ids match: %s"
+            % (ns / dt, threads, ns, match))
+
+    encode_flops = 2.0 * args.n * (kc * dim + mp * kp * (dim // mp))
+    out = {
+        "metric": "QPS at fixed recall@100, mixed-prec batched vs plain FP32 GPU, 1M×128",
+        "value": round(qps, 1),
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded Philox Gaussian mixture, integer-valued)",
+        "config": {
+            "workload": "dense-partition SC index, K=256 (nearest, second) cells over D=128, "
+                        "PQ 8x256, %d base, %d queries, T=%d, top-%d; form: plain FP32, batched"
+                        % (args.n, args.nq, args.T, args.R),
+            "form": "plain-fp32-batched",
+            "recall_at_100": recall,
+            "qps_plain_fp32_per_query_calls": round(qps_single, 1),
+            "candidates_per_query": round(cand / args.nq, 1),
+            "stage_ms": {"lut": round(st.ms_lut, 4), "rank": round(st.ms_rank, 4),
+                         "traverse": round(st.ms_traverse, 4), "scan_topk": round(st.ms_scan, 4)},
+            "encode_s": round(t_encode, 4),
+            "encode_fp64_tflops": round(encode_flops / t_encode / 1e12, 2),
+            "train_s": round(t_train, 2),
+        },
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None,
+                     "kernel": "k_scan_topk",
+                     "algorithmic_bytes_per_launch": int(scan_bytes)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    ix.close()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

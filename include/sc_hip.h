@@ -1,0 +1,188 @@
+/*
+ * sc_hip.h -- C ABI of the MI355X quantize -> encode -> search library
+ * (libsc_amd.so).  Plain pointers and sizes, int status codes, no exceptions
+ * and no torch types cross this boundary.
+ *
+ * Pointers named d_* are device (HBM) pointers on the context's device;
+ * h_* are host pointers.  Every call is enqueued on the context's stream and
+ * is asynchronous unless it says otherwise.
+ *
+ * Reference interfaces each entry point replaces are cited as path:line
+ * relative to the reference repository root.
+ */
+#ifndef SC_HIP_H
+#define SC_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define SC_OK 0
+#define SC_ERR_ARG (-1)         /* bad argument or shape */
+#define SC_ERR_NOMEM (-2)       /* device or host allocation failed */
+#define SC_ERR_CONFIG (-3)      /* search kind does not match the index
+                                   (the reference prints and returns:
+                                   include/multi_query.h:71-74,
+                                   include/sc_query.h:68-71) */
+#define SC_ERR_HIP (-4)         /* HIP runtime error (see sc_last_error) */
+#define SC_ERR_TOO_LARGE (-5)   /* index too large (src/sc_encoder.cpp:28-33) */
+#define SC_ERR_NO_DEVICE (-6)   /* no gfx950 device */
+#define SC_ERR_HEAP_EMPTY (-7)  /* pq_pop on an empty heap
+                                   (include/sc_utilities.h:274-275) */
+
+/* ---- search kinds ------------------------------------------------------ */
+#define SC_KIND_IVFADC 0  /* PQQuery::search_ivfadc   include/query.h:202-325 */
+#define SC_KIND_IMI 1     /* MultiQuery::search_multi2 include/multi_query.h:62-273 */
+#define SC_KIND_SC 2      /* SCQuery::search_mr_ivf    include/sc_query.h:60-256 */
+
+/* ---- precision / forms ------------------------------------------------- */
+#define SC_PREC_FP32 0    /* the reference's arithmetic everywhere */
+#define SC_PREC_BF16 1    /* MFMA bf16 distance GEMM + exact re-check */
+#define SC_FORM_PLAIN 0   /* FP32 LUT scan */
+#define SC_FORM_MIXED 1   /* FP16 LUT scan + exact FP32 re-rank */
+
+typedef struct sc_ctx sc_ctx;
+typedef struct sc_index sc_index;
+
+/* Last error message of the calling thread ("" when none). */
+const char *sc_last_error(void);
+/* Library build string (arch, flags). */
+const char *sc_version(void);
+
+/* ---- context (no reference counterpart: owns the device and stream) ---- */
+int sc_ctx_create(int device, sc_ctx **out);
+int sc_ctx_destroy(sc_ctx *ctx);
+int sc_ctx_sync(sc_ctx *ctx);
+/* hipStream_t of the context, as void*. */
+void *sc_ctx_stream(sc_ctx *ctx);
+
+/* Device memory for hosts without their own allocator. */
+int sc_malloc(sc_ctx *ctx, size_t bytes, void **d_ptr);
+int sc_free(sc_ctx *ctx, void *d_ptr);
+int sc_memcpy_h2d(sc_ctx *ctx, void *d_dst, const void *h_src, size_t bytes);
+int sc_memcpy_d2h(sc_ctx *ctx, void *h_dst, const void *d_src, size_t bytes);
+int sc_memset(sc_ctx *ctx, void *d_dst, int value, size_t bytes);
+
+/* ---- seeded synthetic data --------------------------------------------- */
+/* Gaussian mixture from Philox4x32-10 counters, integer-only arithmetic so
+ * that any row range regenerates bit-identically on host or device.  Stands
+ * in for the reference's fvecs/bvecs inputs (include/sc_utilities.h:816-886,
+ * 1077-1143). */
+typedef struct {
+    uint32_t seed;
+    int32_t ncomp;
+    int32_t frac_bits;
+    int32_t out_bits;
+    int64_t mu_lo, mu_span;
+    int64_t sig_lo, sig_span;
+    int64_t maxq;
+} sc_mix;
+int sc_gen_rows(sc_ctx *ctx, const sc_mix *mix, uint32_t stream, int64_t row0,
+                int64_t nrows, int32_t dim, float *d_out);
+
+/* ---- encode ------------------------------------------------------------ */
+typedef struct {
+    int64_t rechecked;   /* rows re-evaluated exactly (SC_PREC_BF16) */
+    int64_t fallback;    /* rows whose candidate set overflowed -> full exact */
+} sc_encode_stats;
+
+/* Coarse assignment + residual + PQ codes for n vectors.
+ *   nc = 1: Encoder::encode<T>   include/encoder.h:69-192
+ *           cid [n][mc], nearest per block, lowest index on ties.
+ *   nc = 2: SCEncoder::encode2<T> include/sc_encoder.h:63-195
+ *           cid [n][2*mc]: nearest of each block, then second of each block;
+ *           top-2 by (dist ASC, index DESC).
+ * Codebooks: d_cq [mc][kc][dim/mc], d_pq [mp][kp][dim/mp] (float).
+ * Codes d_codes [n][mp] uint8.  `prec` SC_PREC_FP32 or SC_PREC_BF16; both
+ * produce identical cid and codes.  `st` (host, may be NULL) is filled after
+ * an implicit stream synchronisation. */
+int sc_encode(sc_ctx *ctx, const float *d_x, int64_t n, int32_t dim,
+              const float *d_cq, int32_t mc, int32_t kc, const float *d_pq,
+              int32_t mp, int32_t kp, int32_t nc, int32_t prec,
+              uint16_t *d_cid, uint8_t *d_codes, sc_encode_stats *st);
+
+/* ---- index ------------------------------------------------------------- */
+/* Builds the device-resident inverted file from encoder outputs:
+ * Encoder::distribution (src/encoder.cpp:170-205) / SCEncoder::distribution
+ * (src/sc_encoder.cpp:17-81) + the loaders' flattening (src/query.cpp:82-165,
+ * src/sc_query.cpp:36-123) + PQQuery::pre_compute1 (include/query.h:91-149).
+ * The index copies the codebooks; d_cid/d_codes may be freed afterwards.
+ * Dense cells: kc^mc (nc=1) or kc^2 (nc=2, mc=1). */
+int sc_index_build(sc_ctx *ctx, const uint16_t *d_cid, const uint8_t *d_codes,
+                   int64_t n, int32_t dim, const float *d_cq, int32_t mc,
+                   int32_t kc, const float *d_pq, int32_t mp, int32_t kp,
+                   int32_t nc, sc_index **out);
+/* Loads an index from host arrays already in the loaders' layout:
+ * L inclusive prefix [ncells], pid [n], codes [n][mp] in cell order. */
+int sc_index_load(sc_ctx *ctx, const int32_t *h_L, int64_t ncells,
+                  const int32_t *h_pid, const uint8_t *h_codes, int64_t n,
+                  int32_t dim, const float *h_cq, int32_t mc, int32_t kc,
+                  const float *h_pq, int32_t mp, int32_t kp, int32_t nc,
+                  sc_index **out);
+int sc_index_destroy(sc_index *ix);
+/* Shape: out6 = {n, ncells, mc, kc, mp, kp} */
+int sc_index_shape(const sc_index *ix, int64_t *out6);
+/* Copies the flattened index back (any pointer may be NULL):
+ * L [ncells], pid [n], codes [n][mp], norm_c [mc*kc], norm_r [mp*kp],
+ * dot_cr [kc*mp*kp]. Synchronous. */
+int sc_index_export(sc_ctx *ctx, const sc_index *ix, int32_t *h_L,
+                    int32_t *h_pid, uint8_t *h_codes, float *h_norm_c,
+                    float *h_norm_r, float *h_dot_cr);
+
+/* ---- search ------------------------------------------------------------ */
+typedef struct {
+    int32_t kind;   /* SC_KIND_* */
+    int32_t form;   /* SC_FORM_* */
+    int32_t R;      /* results per query */
+    int32_t w;      /* cell budget (heap capacity; IVF/IMI popped, SC non-empty) */
+    int32_t T;      /* candidate budget */
+    int32_t M;      /* coarse list length (lists are fully sorted; see notes) */
+} sc_search_params;
+
+typedef struct {
+    int64_t queries;
+    int64_t candidates;   /* sum over queries of gathered candidates */
+    int64_t cells;        /* non-empty cells scanned */
+    int64_t visited;      /* cells popped */
+    int64_t reranked;     /* candidates re-ranked exactly (SC_FORM_MIXED) */
+    int64_t heap_fallback;/* queries whose heap exceeded on-chip capacity */
+    int64_t exhausted;    /* queries whose heap emptied before the budget */
+    /* stage times on the context stream (HIP events), milliseconds */
+    double ms_lut, ms_rank, ms_traverse, ms_scan;
+} sc_search_stats;
+
+/* Batched search of nq queries d_q [nq][dim]; outputs d_ids [nq][R],
+ * d_dists [nq][R] sorted by (dist ASC, position ASC); when a query gathers
+ * fewer than R candidates the first `sum` entries keep traversal order and
+ * the rest are -1 / +inf.  d_sums [nq] (may be NULL) = candidates gathered.
+ * Replaces search_ivfadc / search_multi2 / search_mr_ivf called once per
+ * query.  `st` (host, may be NULL) is filled after a synchronisation. */
+int sc_search(sc_ctx *ctx, const sc_index *ix, const float *d_q, int64_t nq,
+              const sc_search_params *p, int32_t *d_ids, float *d_dists,
+              int32_t *d_sums, sc_search_stats *st);
+
+/* ---- k-means (codebook training) --------------------------------------- */
+/* k-means++ seeding + `iters` Lloyd iterations on d_x [n][d]:
+ * PQQuantizer::create_sub_quantizers -> greg_kmeans
+ * (include/quantizer.h:235-272, lib/simple-cluster/include/k-means.h:162-267,
+ * 553-912) with a seeded sampler and FP64 centroid sums.  Outputs
+ * d_centers [k][d], d_labels [n] (may be NULL). */
+int sc_kmeans(sc_ctx *ctx, const float *d_x, int64_t n, int32_t d, int32_t k,
+              int32_t iters, uint32_t seed, int32_t lagged, int32_t prec,
+              float *d_centers, int32_t *d_labels);
+
+/* Residuals for PQ training, in place on d_x [n][dim]:
+ * PQQuantizer::calc_residual_vector (include/quantizer.h:456-510, which=0)
+ * and SCQuantizer::calc_residual_vector (include/sc_quantizer.h:72-133,
+ * which=1, second-nearest). */
+int sc_residual(sc_ctx *ctx, float *d_x, int64_t n, int32_t dim,
+                const float *d_cq, int32_t mc, int32_t kc, int32_t which);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SC_HIP_H */

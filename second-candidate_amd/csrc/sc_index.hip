@@ -1,0 +1,312 @@
+// sc_index.hip -- inverted file construction and the query-independent tables.
+//
+// Keys (S4): IMI/IVF key = sum_j cid_j * kc^(mc-1-j) (vector_base,
+// include/sc_utilities.h:156-166); SC key uid_j = nearest_j*kc + second_j,
+// key = sum_j uid_j * (kc^2)^(mc-1-j) (src/sc_encoder.cpp:39-50), 64-bit.
+// A stable radix sort of (key, id) gives every cell its members in
+// ascending id order, exactly the order Bucket::push_back produces
+// (src/encoder.cpp:182-186), and the dense inclusive prefix L[] the loaders
+// build (src/query.cpp:133-139).
+#include <cstring>
+
+#include <rocprim/device/device_radix_sort.hpp>
+
+#include "sc_internal.h"
+
+namespace {
+
+__global__ void k_keys(const uint16_t *__restrict__ cid, int64_t n, int mc, int kc, int nc,
+                       uint64_t *__restrict__ keys, int32_t *__restrict__ ids) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint64_t key = 0;
+    if (nc == 1) {
+        for (int j = 0; j < mc; ++j) key = key * (uint64_t)kc + cid[(size_t)i * mc + j];
+    } else {
+        const uint16_t *c = cid + (size_t)i * 2 * mc;
+        for (int j = 0; j < mc; ++j)
+            key = key * (uint64_t)kc * (uint64_t)kc + ((uint64_t)c[j] * (uint64_t)kc + c[mc + j]);
+    }
+    keys[i] = key;
+    ids[i] = (int32_t)i;
+}
+
+// L[c] = number of sorted keys <= c (inclusive prefix of cell sizes).
+__global__ void k_dense_prefix(const uint64_t *__restrict__ skeys, int64_t n, int64_t ncells,
+                               int32_t *__restrict__ L) {
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncells) return;
+    int64_t lo = 0, hi = n;  // first index with key > c
+    while (lo < hi) {
+        int64_t mid = (lo + hi) >> 1;
+        if (skeys[mid] <= (uint64_t)c) lo = mid + 1; else hi = mid;
+    }
+    L[c] = (int32_t)lo;
+}
+
+__global__ void k_cell_max(const int32_t *__restrict__ L, int64_t ncells, int32_t *out) {
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int32_t l = 0;
+    if (c < ncells) l = c > 0 ? L[c] - L[c - 1] : L[0];
+    for (int o = 32; o > 0; o >>= 1) l = max(l, __shfl_xor(l, o));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, l);
+}
+
+__global__ void k_gather_codes(const uint8_t *__restrict__ codes, const int32_t *__restrict__ ids,
+                               int64_t n, int mp, uint8_t *__restrict__ out) {
+    int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n * mp) return;
+    int64_t p = e / mp;
+    int s = (int)(e - p * mp);
+    out[e] = codes[(size_t)ids[p] * mp + s];
+}
+
+// PQQuery::pre_compute1 (include/query.h:91-149): sequential FP32 sums with
+// separately rounded products.  This file is compiled with
+// -ffp-contract=off, so `d += a * b` stays a multiply and an add.
+__global__ void k_norms(const float *__restrict__ cb, int64_t rows, int len,
+                        float *__restrict__ out) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= rows) return;
+    const float *v = cb + (size_t)i * len;
+    float d = 0.0f;
+    for (int j = 0; j < len; ++j) { float t = v[j]; d += t * t; }
+    out[i] = d;
+}
+
+// dot_cr[((i*kc + j)*spb + i1)*kp + j1] = 2 * sum_t c_{i,j}[i1*bsp+t] *
+// pq_{i*spb+i1, j1}[t]   (spb = mp/mc sub-quantizers per coarse block)
+__global__ void k_dot_cr(const float *__restrict__ cq, const float *__restrict__ pq, int mc,
+                         int kc, int mp, int kp, int bsc, int bsp, float *__restrict__ out) {
+    int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int spb = mp / mc;
+    const int64_t total = (int64_t)mc * kc * spb * kp;
+    if (e >= total) return;
+    const int j1 = (int)(e % kp);
+    int64_t r = e / kp;
+    const int i1 = (int)(r % spb);
+    r /= spb;                                  // r = i*kc + j
+    const int i = (int)(r / kc);
+    const float *c = cq + (size_t)r * bsc + (size_t)i1 * bsp;
+    const float *p = pq + ((size_t)(i * spb + i1) * kp + j1) * bsp;
+    float d = 0.0f;
+    for (int t = 0; t < bsp; ++t) d += c[t] * p[t];
+    out[e] = 2.0f * d;
+}
+
+int alloc_index(sc_ctx *ctx, int64_t n, int32_t dim, int32_t mc, int32_t kc, int32_t mp,
+                int32_t kp, int32_t nc, int64_t ncells, sc_index **out) {
+    sc_index *ix = new (std::nothrow) sc_index();
+    if (!ix) return SC_ERR_NOMEM;
+    *ix = sc_index{};
+    ix->ctx = ctx;
+    ix->dim = dim; ix->mc = mc; ix->kc = kc; ix->mp = mp; ix->kp = kp; ix->nc = nc;
+    ix->n = n; ix->ncells = ncells;
+    const size_t bsc = dim / mc, bsp = dim / mp;
+    bool ok = hipMalloc(&ix->cq, sizeof(float) * mc * kc * bsc) == hipSuccess &&
+              hipMalloc(&ix->pq, sizeof(float) * mp * kp * bsp) == hipSuccess &&
+              hipMalloc(&ix->norm_c, sizeof(float) * mc * kc) == hipSuccess &&
+              hipMalloc(&ix->norm_r, sizeof(float) * mp * kp) == hipSuccess &&
+              hipMalloc(&ix->dot_cr, sizeof(float) * (size_t)kc * mp * kp) == hipSuccess &&
+              hipMalloc(&ix->L, sizeof(int32_t) * ncells) == hipSuccess &&
+              hipMalloc(&ix->pid, sizeof(int32_t) * (n > 0 ? n : 1)) == hipSuccess &&
+              hipMalloc(&ix->codes, (size_t)(n > 0 ? n : 1) * mp) == hipSuccess;
+    if (!ok) {
+        sc_index_destroy(ix);
+        sc_set_error("index allocation failed (n=%lld cells=%lld)", (long long)n,
+                     (long long)ncells);
+        return SC_ERR_NOMEM;
+    }
+    *out = ix;
+    return SC_OK;
+}
+
+int check_shape(int64_t n, int32_t dim, int32_t mc, int32_t kc, int32_t mp, int32_t kp,
+                int32_t nc, int64_t *ncells) {
+    if (n < 0 || n >= INT32_MAX || dim <= 0 || mc <= 0 || mp <= 0 || dim % mc || dim % mp ||
+        mp % mc || kc < 1 || kc > 65536 || kp < 1 || kp > 256 || (nc != 1 && nc != 2)) {
+        sc_set_error("index: bad shape (n=%lld dim=%d mc=%d kc=%d mp=%d kp=%d nc=%d)",
+                     (long long)n, dim, mc, kc, mp, kp, nc);
+        return SC_ERR_ARG;
+    }
+    double cells = 1.0;
+    for (int j = 0; j < mc * nc; ++j) cells *= kc;
+    if (cells >= (double)INT32_MAX) {
+        sc_set_error("index: kc^(mc*nc) = %.3g cells is too large for a dense index "
+                     "(src/sc_encoder.cpp:28-33)", cells);
+        return SC_ERR_TOO_LARGE;
+    }
+    *ncells = (int64_t)cells;
+    return SC_OK;
+}
+
+int finish_index(sc_index *ix) {
+    sc_ctx *ctx = ix->ctx;
+    SC_TRY(sc_launch_tables(ix));
+    int32_t *dmax;
+    SC_TRY(sc_scratch(ctx, sizeof(int32_t), (void **)&dmax));
+    SC_HIP(hipMemsetAsync(dmax, 0, sizeof(int32_t), ctx->stream));
+    hipLaunchKernelGGL(k_cell_max, dim3((unsigned)sc_cdiv(ix->ncells, 256)), dim3(256), 0,
+                       ctx->stream, ix->L, ix->ncells, dmax);
+    SC_CHECK_LAUNCH();
+    SC_HIP(hipMemcpyAsync(&ix->max_cell, dmax, sizeof(int32_t), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    SC_HIP(hipStreamSynchronize(ctx->stream));
+    return SC_OK;
+}
+
+}  // namespace
+
+int sc_launch_tables(sc_index *ix) {
+    sc_ctx *ctx = ix->ctx;
+    const int bsc = ix->dim / ix->mc, bsp = ix->dim / ix->mp;
+    int64_t rc = (int64_t)ix->mc * ix->kc, rp = (int64_t)ix->mp * ix->kp;
+    hipLaunchKernelGGL(k_norms, dim3((unsigned)sc_cdiv(rc, 256)), dim3(256), 0, ctx->stream,
+                       ix->cq, rc, bsc, ix->norm_c);
+    SC_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_norms, dim3((unsigned)sc_cdiv(rp, 256)), dim3(256), 0, ctx->stream,
+                       ix->pq, rp, bsp, ix->norm_r);
+    SC_CHECK_LAUNCH();
+    int64_t total = (int64_t)ix->kc * ix->mp * ix->kp;
+    hipLaunchKernelGGL(k_dot_cr, dim3((unsigned)sc_cdiv(total, 256)), dim3(256), 0,
+                       ctx->stream, ix->cq, ix->pq, ix->mc, ix->kc, ix->mp, ix->kp, bsc, bsp,
+                       ix->dot_cr);
+    SC_CHECK_LAUNCH();
+    return SC_OK;
+}
+
+extern "C" int sc_index_build(sc_ctx *ctx, const uint16_t *d_cid, const uint8_t *d_codes,
+                              int64_t n, int32_t dim, const float *d_cq, int32_t mc, int32_t kc,
+                              const float *d_pq, int32_t mp, int32_t kp, int32_t nc,
+                              sc_index **out) {
+    SC_TRY(sc_ctx_enter(ctx));
+    if (!out || !d_cq || !d_pq || (n > 0 && (!d_cid || !d_codes))) return SC_ERR_ARG;
+    *out = nullptr;
+    int64_t ncells;
+    SC_TRY(check_shape(n, dim, mc, kc, mp, kp, nc, &ncells));
+    sc_index *ix;
+    SC_TRY(alloc_index(ctx, n, dim, mc, kc, mp, kp, nc, ncells, &ix));
+    const size_t bsc = dim / mc, bsp = dim / mp;
+    auto fail = [&](int rc) { sc_index_destroy(ix); return rc; };
+    if (hipMemcpyAsync(ix->cq, d_cq, sizeof(float) * mc * kc * bsc, hipMemcpyDeviceToDevice,
+                       ctx->stream) != hipSuccess ||
+        hipMemcpyAsync(ix->pq, d_pq, sizeof(float) * mp * kp * bsp, hipMemcpyDeviceToDevice,
+                       ctx->stream) != hipSuccess) {
+        sc_set_error("index: codebook copy failed");
+        return fail(SC_ERR_HIP);
+    }
+    if (n > 0) {
+        int bits = 1;
+        while (bits < 64 && ((uint64_t)1 << bits) < (uint64_t)ncells) ++bits;
+        size_t tmp_bytes = 0;
+        if (rocprim::radix_sort_pairs(nullptr, tmp_bytes, (uint64_t *)nullptr,
+                                      (uint64_t *)nullptr, (int32_t *)nullptr,
+                                      (int32_t *)nullptr, (size_t)n, 0u, (unsigned)bits,
+                                      ctx->stream) != hipSuccess)
+            return fail(SC_ERR_HIP);
+        size_t off_keys2 = (size_t)n * 8, off_ids = off_keys2 + (size_t)n * 8,
+               off_tmp = off_ids + (size_t)n * 4;
+        off_tmp = (off_tmp + 255) & ~(size_t)255;
+        unsigned char *s;
+        int rc = sc_scratch(ctx, off_tmp + tmp_bytes, (void **)&s);
+        if (rc) return fail(rc);
+        uint64_t *keys = (uint64_t *)s, *skeys = (uint64_t *)(s + off_keys2);
+        int32_t *ids = (int32_t *)(s + off_ids);
+        hipLaunchKernelGGL(k_keys, dim3((unsigned)sc_cdiv(n, 256)), dim3(256), 0, ctx->stream,
+                           d_cid, n, mc, kc, nc, keys, ids);
+        if (hipGetLastError() != hipSuccess) return fail(SC_ERR_HIP);
+        if (rocprim::radix_sort_pairs(s + off_tmp, tmp_bytes, keys, skeys, ids, ix->pid,
+                                      (size_t)n, 0u, (unsigned)bits, ctx->stream) != hipSuccess) {
+            sc_set_error("index: radix sort failed");
+            return fail(SC_ERR_HIP);
+        }
+        hipLaunchKernelGGL(k_dense_prefix, dim3((unsigned)sc_cdiv(ncells, 256)), dim3(256), 0,
+                           ctx->stream, skeys, n, ncells, ix->L);
+        hipLaunchKernelGGL(k_gather_codes, dim3((unsigned)sc_cdiv(n * mp, 256)), dim3(256), 0,
+                           ctx->stream, d_codes, ix->pid, n, mp, ix->codes);
+        if (hipGetLastError() != hipSuccess) return fail(SC_ERR_HIP);
+    } else {
+        if (hipMemsetAsync(ix->L, 0, sizeof(int32_t) * ncells, ctx->stream) != hipSuccess)
+            return fail(SC_ERR_HIP);
+    }
+    int rc = finish_index(ix);
+    if (rc) return fail(rc);
+    *out = ix;
+    return SC_OK;
+}
+
+extern "C" int sc_index_load(sc_ctx *ctx, const int32_t *h_L, int64_t ncells,
+                             const int32_t *h_pid, const uint8_t *h_codes, int64_t n,
+                             int32_t dim, const float *h_cq, int32_t mc, int32_t kc,
+                             const float *h_pq, int32_t mp, int32_t kp, int32_t nc,
+                             sc_index **out) {
+    SC_TRY(sc_ctx_enter(ctx));
+    if (!out || !h_L || !h_cq || !h_pq || (n > 0 && (!h_pid || !h_codes))) return SC_ERR_ARG;
+    *out = nullptr;
+    int64_t want;
+    SC_TRY(check_shape(n, dim, mc, kc, mp, kp, nc, &want));
+    if (ncells != want) {
+        sc_set_error("index_load: %lld cells given, shape needs %lld", (long long)ncells,
+                     (long long)want);
+        return SC_ERR_ARG;
+    }
+    if (h_L[ncells - 1] != n) {
+        sc_set_error("index_load: L[last]=%d != n=%lld", h_L[ncells - 1], (long long)n);
+        return SC_ERR_ARG;
+    }
+    sc_index *ix;
+    SC_TRY(alloc_index(ctx, n, dim, mc, kc, mp, kp, nc, ncells, &ix));
+    const size_t bsc = dim / mc, bsp = dim / mp;
+    bool ok =
+        hipMemcpyAsync(ix->cq, h_cq, sizeof(float) * mc * kc * bsc, hipMemcpyHostToDevice,
+                       ctx->stream) == hipSuccess &&
+        hipMemcpyAsync(ix->pq, h_pq, sizeof(float) * mp * kp * bsp, hipMemcpyHostToDevice,
+                       ctx->stream) == hipSuccess &&
+        hipMemcpyAsync(ix->L, h_L, sizeof(int32_t) * ncells, hipMemcpyHostToDevice,
+                       ctx->stream) == hipSuccess &&
+        (n == 0 || (hipMemcpyAsync(ix->pid, h_pid, sizeof(int32_t) * n, hipMemcpyHostToDevice,
+                                   ctx->stream) == hipSuccess &&
+                    hipMemcpyAsync(ix->codes, h_codes, (size_t)n * mp, hipMemcpyHostToDevice,
+                                   ctx->stream) == hipSuccess));
+    if (!ok) { sc_index_destroy(ix); sc_set_error("index_load: copy failed"); return SC_ERR_HIP; }
+    int rc = finish_index(ix);
+    if (rc) { sc_index_destroy(ix); return rc; }
+    *out = ix;
+    return SC_OK;
+}
+
+extern "C" int sc_index_destroy(sc_index *ix) {
+    if (!ix) return SC_OK;
+    if (ix->ctx) {
+        (void)hipSetDevice(ix->ctx->device);
+        (void)hipStreamSynchronize(ix->ctx->stream);
+    }
+    void *bufs[] = {ix->cq, ix->pq, ix->norm_c, ix->norm_r, ix->dot_cr, ix->L, ix->pid, ix->codes};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    delete ix;
+    return SC_OK;
+}
+
+extern "C" int sc_index_shape(const sc_index *ix, int64_t *out6) {
+    if (!ix || !out6) return SC_ERR_ARG;
+    out6[0] = ix->n; out6[1] = ix->ncells; out6[2] = ix->mc;
+    out6[3] = ix->kc; out6[4] = ix->mp; out6[5] = ix->kp;
+    return SC_OK;
+}
+
+extern "C" int sc_index_export(sc_ctx *ctx, const sc_index *ix, int32_t *h_L, int32_t *h_pid,
+                               uint8_t *h_codes, float *h_norm_c, float *h_norm_r,
+                               float *h_dot_cr) {
+    SC_TRY(sc_ctx_enter(ctx));
+    if (!ix) return SC_ERR_ARG;
+    hipStream_t s = ctx->stream;
+    if (h_L) SC_HIP(hipMemcpyAsync(h_L, ix->L, sizeof(int32_t) * ix->ncells, hipMemcpyDeviceToHost, s));
+    if (h_pid && ix->n) SC_HIP(hipMemcpyAsync(h_pid, ix->pid, sizeof(int32_t) * ix->n, hipMemcpyDeviceToHost, s));
+    if (h_codes && ix->n) SC_HIP(hipMemcpyAsync(h_codes, ix->codes, (size_t)ix->n * ix->mp, hipMemcpyDeviceToHost, s));
+    if (h_norm_c) SC_HIP(hipMemcpyAsync(h_norm_c, ix->norm_c, sizeof(float) * ix->mc * ix->kc, hipMemcpyDeviceToHost, s));
+    if (h_norm_r) SC_HIP(hipMemcpyAsync(h_norm_r, ix->norm_r, sizeof(float) * ix->mp * ix->kp, hipMemcpyDeviceToHost, s));
+    if (h_dot_cr) SC_HIP(hipMemcpyAsync(h_dot_cr, ix->dot_cr, sizeof(float) * (size_t)ix->kc * ix->mp * ix->kp, hipMemcpyDeviceToHost, s));
+    SC_HIP(hipStreamSynchronize(s));
+    return SC_OK;
+}

@@ -1,0 +1,90 @@
+// sc_internal.h -- shared definitions of libsc_amd (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/sc_hip.h"
+
+struct sc_ctx {
+    int device;
+    hipStream_t stream;
+    void *scratch;          // growable device scratch for temporaries
+    size_t scratch_bytes;
+    int n_cu;               // compute units
+};
+
+struct sc_index {
+    sc_ctx *ctx;
+    int32_t dim, mc, kc, mp, kp, nc;
+    int64_t n, ncells;
+    float *cq, *pq;                 // codebooks [mc][kc][bsc], [mp][kp][bsp]
+    float *norm_c, *norm_r, *dot_cr;// pre_compute1 tables
+    int32_t *L;                     // inclusive prefix [ncells]
+    int32_t *pid;                   // [n]
+    uint8_t *codes;                 // [n][mp] in cell order
+    int32_t max_cell;               // largest cell size
+};
+
+void sc_set_error(const char *fmt, ...);
+
+#define SC_HIP(call)                                                           \
+    do {                                                                       \
+        hipError_t e_ = (call);                                                \
+        if (e_ != hipSuccess) {                                                \
+            sc_set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),\
+                         __FILE__, __LINE__);                                  \
+            return SC_ERR_HIP;                                                 \
+        }                                                                      \
+    } while (0)
+
+#define SC_CHECK_LAUNCH() SC_HIP(hipGetLastError())
+
+#define SC_TRY(expr)                                                           \
+    do {                                                                       \
+        int rc_ = (expr);                                                      \
+        if (rc_ != SC_OK) return rc_;                                          \
+    } while (0)
+
+// Context scratch: returns a device buffer of at least `bytes` (contents
+// undefined).  Grows by reallocation after a stream synchronisation.
+int sc_scratch(sc_ctx *ctx, size_t bytes, void **ptr);
+
+// Sets the context's device current on the calling thread.
+int sc_ctx_enter(sc_ctx *ctx);
+
+static inline int64_t sc_cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// Philox4x32-10 (Salmon et al., SC'11), device and host.
+__host__ __device__ inline void sc_philox(uint32_t c0, uint32_t c1, uint32_t c2,
+                                          uint32_t c3, uint32_t k0, uint32_t k1,
+                                          uint32_t out[4]) {
+    for (int r = 0; r < 10; ++r) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        c0 = hi1 ^ c1 ^ k0;
+        c1 = lo1;
+        c2 = hi0 ^ c3 ^ k1;
+        c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+// Order-preserving map of a float onto uint32 (ascending floats ->
+// ascending keys; -0.0 and +0.0 share a key).
+__host__ __device__ inline uint32_t sc_float_key(float f) {
+    uint32_t u = __builtin_bit_cast(uint32_t, f);
+    if (u == 0x80000000u) u = 0u;  // -0.0 compares equal to +0.0
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__host__ __device__ inline float sc_key_float(uint32_t k) {
+    uint32_t u = (k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k;
+    return __builtin_bit_cast(float, u);
+}
+
+// Internal launches shared across translation units.
+int sc_launch_tables(sc_index *ix);

@@ -1,0 +1,814 @@
+// sc_search.hip -- batched search: per-query LUTs, coarse ranking, the
+// multi-sequence traversal (exact replay of the reference's array heap), the
+// ADC code scan and the canonical top-R.
+//
+// Stages (one launch each, all on the context stream):
+//   k_lut       PQQuery::pre_compute2 (include/query.h:154-184), index-aligned
+//               stores, sequential FP32 dot products (the defined cblas_sdot).
+//   k_rank      coarse scores (query.h:225-236, multi_query.h:102-122,
+//               sc_query.h:95-107) and a full sort by (score, centroid).
+//   k_traverse  one lane per query: search_ivfadc step 2 (query.h:232-259),
+//               search_multi2 step 2 (multi_query.h:133-205), search_mr_ivf
+//               step 2 (sc_query.h:111-186) with pq_insert*/pq_pop*
+//               (include/sc_utilities.h:229-300, 469-585) replayed exactly.
+//   k_scan_topk one workgroup per query: the ADC sequences of query.h:295-305,
+//               multi_query.h:231-253, sc_query.h:216-229 and the top-R by
+//               (dist ASC, position ASC).
+#include <float.h>
+#include <stdlib.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "sc_internal.h"
+
+namespace {
+
+constexpr int kScanThreads = 256;
+constexpr int kBufCap = 2048;      // streaming top-R buffer (entries)
+constexpr int kRunCells = 256;     // cells staged per scan run
+
+// ----------------------------------------------------------------------------
+// Stage 1: LUTs.  out[q][0 .. mc*kc) = diff_qc, out[q][mc*kc .. +mp*kp) = diff_qr
+// diff = (norm - s) - s,  s = sequential FP32 dot product.
+// ----------------------------------------------------------------------------
+__global__ void k_lut(const float *__restrict__ Q, int64_t nq, int dim,
+                      const float *__restrict__ cq, const float *__restrict__ pq,
+                      const float *__restrict__ norm_c, const float *__restrict__ norm_r, int mc,
+                      int kc, int mp, int kp, float *__restrict__ dqc, float *__restrict__ dqr) {
+    const int per = mc * kc + mp * kp;
+    int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nq * per) return;
+    int64_t q = e / per;
+    int j = (int)(e - q * per);
+    const float *qv = Q + (size_t)q * dim;
+    if (j < mc * kc) {
+        const int bsc = dim / mc, blk = j / kc;
+        const float *a = qv + (size_t)blk * bsc, *b = cq + (size_t)j * bsc;
+        float s = 0.0f;
+        for (int t = 0; t < bsc; ++t) s += a[t] * b[t];
+        dqc[(size_t)q * mc * kc + j] = (norm_c[j] - s) - s;
+    } else {
+        j -= mc * kc;
+        const int bsp = dim / mp, sub = j / kp;
+        const float *a = qv + (size_t)sub * bsp, *b = pq + (size_t)j * bsp;
+        float s = 0.0f;
+        for (int t = 0; t < bsp; ++t) s += a[t] * b[t];
+        dqr[(size_t)q * mp * kp + j] = (norm_r[j] - s) - s;
+    }
+}
+
+// ----------------------------------------------------------------------------
+// Stage 2: coarse scores + sort.  One workgroup per query.
+// qs[q] = {q_sum, q_half0, q_half1, 0}.  sv/si [q][mc*kc]: sorted scores and
+// centroid ids per list (IVF: unsorted scores in centroid order).
+// ----------------------------------------------------------------------------
+__global__ void k_rank(const float *__restrict__ Q, int dim, int kind, int mc, int kc,
+                       const float *__restrict__ dqc, float *__restrict__ qs,
+                       float *__restrict__ sv, int32_t *__restrict__ si, int npow2) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    uint64_t *key = reinterpret_cast<uint64_t *>(smem);
+    __shared__ float s_q[3];
+    const int64_t q = blockIdx.x;
+    const float *qv = Q + (size_t)q * dim;
+    if (threadIdx.x == 0) {
+        float qsum = 0.0f, h[2] = {0.0f, 0.0f};
+        if (kind == SC_KIND_IMI) {
+            const int bsc = dim / mc;
+            for (int i = 0; i < mc; ++i) {
+                float d1 = 0.0f;
+                for (int j = 0; j < bsc; ++j) { float t = qv[i * bsc + j]; d1 += t * t; }
+                qsum += d1;
+                h[i] = d1;
+            }
+        } else {
+            for (int i = 0; i < dim; ++i) { float t = qv[i]; qsum += t * t; }
+        }
+        s_q[0] = qsum; s_q[1] = h[0]; s_q[2] = h[1];
+        qs[q * 4 + 0] = qsum; qs[q * 4 + 1] = h[0]; qs[q * 4 + 2] = h[1]; qs[q * 4 + 3] = 0.0f;
+    }
+    __syncthreads();
+    const float *dq = dqc + (size_t)q * mc * kc;
+    if (kind == SC_KIND_IVFADC) {
+        for (int j = threadIdx.x; j < kc; j += blockDim.x) {
+            sv[(size_t)q * kc + j] = s_q[0] + dq[j];
+            si[(size_t)q * kc + j] = j;
+        }
+        return;
+    }
+    const int lists = (kind == SC_KIND_IMI) ? 2 : 1;
+    for (int h = 0; h < lists; ++h) {
+        const float base = (kind == SC_KIND_IMI) ? s_q[1 + h] : s_q[0];
+        for (int j = threadIdx.x; j < npow2; j += blockDim.x) {
+            if (j < kc) {
+                float v = base + dq[h * kc + j];
+                key[j] = ((uint64_t)sc_float_key(v) << 32) | (uint32_t)j;
+            } else {
+                key[j] = ~(uint64_t)0;
+            }
+        }
+        __syncthreads();
+        for (int size = 2; size <= npow2; size <<= 1) {
+            for (int stride = size >> 1; stride > 0; stride >>= 1) {
+                for (int i = threadIdx.x; i < npow2 / 2; i += blockDim.x) {
+                    int lo = 2 * i - (i & (stride - 1));
+                    int hi = lo + stride;
+                    bool up = ((lo & size) == 0);
+                    uint64_t a = key[lo], b = key[hi];
+                    if ((a > b) == up) { key[lo] = b; key[hi] = a; }
+                }
+                __syncthreads();
+            }
+        }
+        for (int j = threadIdx.x; j < kc; j += blockDim.x) {
+            uint64_t k = key[j];
+            sv[((size_t)q * lists + h) * kc + j] = sc_key_float((uint32_t)(k >> 32));
+            si[((size_t)q * lists + h) * kc + j] = (int32_t)(k & 0xFFFFFFFFu);
+        }
+        __syncthreads();
+    }
+}
+
+// ----------------------------------------------------------------------------
+// Stage 3: traversal.  One lane per query; heap and row lengths in LDS
+// (interleaved across lanes), or in global memory for the fallback launch.
+// The popped cells always form a down-set of the rank grid (a cell is pushed
+// only after both its predecessors were popped), so traversed(a, b) of the
+// reference is exactly b < rowlen[a].
+// qinfo[q*8 + ...]: 0 ncell, 1 sum, 2 visited, 3 empty, 4 exhausted,
+//                   5 overflow (on-chip heap too small), 6 error
+// ----------------------------------------------------------------------------
+struct Heap {
+    float *v;
+    uint32_t *p;
+    int stride;   // element stride
+    int pos;
+    int cap_ref;  // the reference's capacity (w, or kc for IVF)
+    int cap_phys; // storage available
+    bool overflow;
+
+    __device__ __forceinline__ float &V(int i) { return v[(size_t)i * stride]; }
+    __device__ __forceinline__ uint32_t &P(int i) { return p[(size_t)i * stride]; }
+
+    __device__ void insert(float val, uint32_t id) {
+        if (pos >= cap_ref || pos < 0) return;           // silently dropped
+        if (pos >= cap_phys) { overflow = true; return; }
+        int i = pos++;
+        V(i) = val; P(i) = id;
+        int j = i >> 1;
+        while (i > 0) {
+            float vi = V(i), vj = V(j);
+            if (vi < vj) {
+                V(j) = vi; V(i) = vj;
+                uint32_t t = P(i); P(i) = P(j); P(j) = t;
+            } else break;
+            i = j;
+            j = i >> 1;
+        }
+    }
+    __device__ uint32_t pop() {
+        uint32_t top = P(0);
+        int last = --pos;
+        V(0) = V(last);
+        P(0) = P(last);
+        int i = 0, j = 0;
+        while (j + 1 < pos) {
+            float a = V(j), b = V(j + 1);
+            int mx = (a < b) ? j : j + 1;
+            float vm = (a < b) ? a : b, vi = V(i);
+            if (vm < vi) {
+                uint32_t t = P(i); P(i) = P(mx); P(mx) = t;
+                V(i) = vm; V(mx) = vi;
+                i = mx;
+            } else break;
+            j = i << 1;
+        }
+        return top;
+    }
+};
+
+struct TravArgs {
+    int kind, kc, w, T, ccap;
+    const float *sv;
+    const int32_t *si;
+    const int32_t *L;
+    int32_t *cells;
+    uint32_t *cinfo;
+    int32_t *qinfo;
+    const int32_t *qlist;   // fallback: queries to redo (nullptr = all)
+    int64_t nq;
+    float *gheap_v;         // fallback heap storage [nq][w]
+    uint32_t *gheap_p;
+    int32_t *grow;          // fallback row lengths [nq][kc]
+};
+
+__device__ __forceinline__ int32_t cell_len(const int32_t *L, int64_t bid) {
+    return bid > 0 ? L[bid] - L[bid - 1] : L[0];
+}
+
+__global__ void k_traverse(TravArgs a, int hcap, int qpw) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int lane = threadIdx.x;
+    const int64_t slot = (int64_t)blockIdx.x * qpw + lane;
+    if (lane >= qpw || slot >= a.nq) return;
+    const int64_t q = a.qlist ? a.qlist[slot] : slot;
+    const int kc = a.kc;
+
+    Heap h;
+    int32_t *rowlen;
+    int rstride;
+    if (a.gheap_v) {            // fallback: hcap = the reference's capacity
+        h.v = a.gheap_v + (size_t)slot * hcap;
+        h.p = a.gheap_p + (size_t)slot * hcap;
+        h.stride = 1;
+        h.cap_phys = hcap;
+        rowlen = a.grow + (size_t)slot * kc;
+        rstride = 1;
+    } else {
+        float *hv = reinterpret_cast<float *>(smem);
+        uint32_t *hp = reinterpret_cast<uint32_t *>(hv + (size_t)hcap * qpw);
+        int32_t *rl = reinterpret_cast<int32_t *>(hp + (size_t)hcap * qpw);
+        h.v = hv + lane;
+        h.p = hp + lane;
+        h.stride = qpw;
+        h.cap_phys = hcap;
+        rowlen = rl + lane;
+        rstride = qpw;
+    }
+    h.pos = 0;
+    h.overflow = false;
+
+    int32_t *cells = a.cells + (size_t)q * a.ccap;
+    uint32_t *cinfo = a.cinfo + (size_t)q * a.ccap;
+    int count = 0, count2 = 0, sum = 0, empty = 0, exhausted = 0, err = 0;
+
+    if (a.kind == SC_KIND_IVFADC) {
+        const float *v = a.sv + (size_t)q * kc;
+        h.cap_ref = kc;
+        for (int j = 0; j < kc; ++j) h.insert(v[j], (uint32_t)j);
+        for (int i = 0; i < a.w && !h.overflow; ++i) {
+            if (h.pos <= 0) { err = 1; break; }      // pq_pop on an empty heap exits
+            int bid = (int)h.pop();
+            int l = cell_len(a.L, bid);
+            cells[count] = bid;
+            cinfo[count] = (uint32_t)bid;
+            count++;
+            count2++;
+            sum += l;
+            if (sum >= a.T) break;
+        }
+    } else {
+        const bool imi = a.kind == SC_KIND_IMI;
+        const float *v1 = a.sv + (size_t)q * (imi ? 2 : 1) * kc;
+        const float *v2 = imi ? v1 + kc : v1;
+        const int32_t *t1 = a.si + (size_t)q * (imi ? 2 : 1) * kc;
+        const int32_t *t2 = imi ? t1 + kc : t1;
+        h.cap_ref = a.w;
+        for (int r = 0; r < kc; ++r) rowlen[(size_t)r * rstride] = 0;
+        if (imi) {
+            h.insert(v1[0] + v2[0], 0u);
+        } else {
+            rowlen[0] = 1;                           // (0,0) never enters the SC grid
+            float d0 = v1[0] + v1[1];
+            h.insert(d0, (0u << 16) | 1u);
+            h.insert(d0, (1u << 16) | 0u);
+        }
+        while ((imi ? count2 : count) < a.w && sum < a.T) {
+            if (h.overflow) break;
+            if (h.pos <= 0) { exhausted = 1; break; }
+            uint32_t pk = h.pop();
+            int h1 = (int)(pk >> 16), h2 = (int)(pk & 0xFFFFu);
+            int h3 = t1[h1], h4 = t2[h2];
+            int64_t bid = (int64_t)h3 * kc + h4;
+            int l = imi ? cell_len(a.L, bid) : (bid > 0 ? cell_len(a.L, bid) : 0);
+            if (l > 0) {
+                if (count >= a.ccap) { err = 2; break; }
+                cells[count] = (int32_t)bid;
+                cinfo[count] = (uint32_t)h3 | ((uint32_t)h4 << 16);
+                count++;
+                sum += l;
+            } else {
+                empty++;
+            }
+            int32_t &rl = rowlen[(size_t)h1 * rstride];
+            if (rl != h2) err = 3;                   // not a down-set: cannot happen
+            rl = h2 + 1;
+            count2++;
+            if (sum < a.T) {
+                if (h1 < kc - 1 && (h2 == 0 || (h2 - 1) < rowlen[(size_t)(h1 + 1) * rstride]))
+                    h.insert(v1[h1 + 1] + v2[h2], ((uint32_t)(h1 + 1) << 16) | (uint32_t)h2);
+                if (h2 < kc - 1 && (h1 == 0 || (h2 + 1) < rowlen[(size_t)(h1 - 1) * rstride]))
+                    h.insert(v1[h1] + v2[h2 + 1], ((uint32_t)h1 << 16) | (uint32_t)(h2 + 1));
+            }
+        }
+    }
+    int32_t *qi = a.qinfo + (size_t)q * 8;
+    qi[0] = count; qi[1] = sum; qi[2] = count2; qi[3] = empty; qi[4] = exhausted;
+    qi[5] = h.overflow ? 1 : 0; qi[6] = err;
+}
+
+// ----------------------------------------------------------------------------
+// Stage 4: scan + top-R.  One workgroup per query.
+// Candidates are keyed (orderable dist bits << 32 | position); keys are unique
+// so the R smallest keys are exactly the canonical (dist, position) top-R.
+// A streaming buffer keeps every candidate below the running threshold and is
+// compacted to the R best by radix select whenever it nears capacity.
+// ----------------------------------------------------------------------------
+struct ScanArgs {
+    int kind, dim, mc, kc, mp, kp, R, ccap;
+    const float *qs, *dqc, *dqr, *dot_cr;
+    const int32_t *L, *pid;
+    const uint8_t *codes;
+    const int32_t *cells;
+    const uint32_t *cinfo;
+    const int32_t *qinfo;
+    int32_t *ids;
+    float *dists;
+    int32_t *sums;
+};
+
+struct ScanShared {
+    uint64_t key[kBufCap];
+    int32_t idx[kBufCap];        // position in the pid array
+    int32_t c_start[kRunCells];  // cell start in pid/codes
+    int32_t c_pref[kRunCells + 1];
+    uint32_t c_info[kRunCells];
+    uint32_t hist[256];
+    int nb;
+    uint64_t thr;
+    uint64_t sel_prefix;
+    int sel_rem;
+    int run_cells;
+};
+
+// Radix select: the R-th smallest key among key[0..nb).  All threads call.
+__device__ uint64_t select_kth(ScanShared &S, int nb, int R) {
+    if (threadIdx.x == 0) { S.sel_prefix = 0; S.sel_rem = R; }
+    __syncthreads();
+    uint64_t mask = 0;
+    for (int shift = 56; shift >= 0; shift -= 8) {
+        for (int b = threadIdx.x; b < 256; b += blockDim.x) S.hist[b] = 0;
+        __syncthreads();
+        const uint64_t prefix = S.sel_prefix;
+        for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+            uint64_t k = S.key[i];
+            if ((k & mask) == prefix) atomicAdd(&S.hist[(k >> shift) & 255u], 1u);
+        }
+        __syncthreads();
+        if (threadIdx.x < 64) {
+            // wave 0: inclusive scan of 256 bins (4 per lane)
+            const int l = threadIdx.x;
+            uint32_t c0 = S.hist[4 * l], c1 = S.hist[4 * l + 1], c2 = S.hist[4 * l + 2],
+                     c3 = S.hist[4 * l + 3];
+            uint32_t tot = c0 + c1 + c2 + c3, incl = tot;
+            for (int o = 1; o < 64; o <<= 1) {
+                uint32_t t = __shfl_up(incl, o);
+                if (l >= o) incl += t;
+            }
+            uint32_t excl = incl - tot;
+            const uint32_t rem = (uint32_t)S.sel_rem;
+            if (excl < rem && rem <= incl) {
+                uint32_t c[4] = {c0, c1, c2, c3};
+                uint32_t acc = excl;
+                for (int b = 0; b < 4; ++b) {
+                    if (rem <= acc + c[b]) {
+                        S.sel_prefix = prefix | ((uint64_t)(4 * l + b) << shift);
+                        S.sel_rem = (int)(rem - acc);
+                        break;
+                    }
+                    acc += c[b];
+                }
+            }
+        }
+        __syncthreads();
+        mask |= (uint64_t)255 << shift;
+    }
+    return S.sel_prefix;
+}
+
+// Keeps only the R smallest keys (all <= kth) in key/idx[0..R).
+__device__ void compact(ScanShared &S, int R) {
+    const int nb = S.nb;
+    const uint64_t kth = select_kth(S, nb, R);
+    // gather survivors to the front: entries <= kth; stable order not needed
+    __shared__ int s_w;
+    if (threadIdx.x == 0) s_w = 0;
+    __syncthreads();
+    uint64_t mk[kBufCap / kScanThreads];
+    int32_t mi[kBufCap / kScanThreads];
+    int m = 0;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x)
+        if (S.key[i] <= kth) { mk[m] = S.key[i]; mi[m] = S.idx[i]; ++m; }
+    __syncthreads();
+    for (int j = 0; j < m; ++j) {
+        int o = atomicAdd(&s_w, 1);
+        S.key[o] = mk[j];
+        S.idx[o] = mi[j];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) { S.nb = R; S.thr = kth; }
+    __syncthreads();
+}
+
+// Sorts key/idx[0..n) ascending (n <= kBufCap) by bitonic sort.
+__device__ void sort_buffer(ScanShared &S, int n) {
+    int np = 1;
+    while (np < n) np <<= 1;
+    for (int i = n + threadIdx.x; i < np; i += blockDim.x) { S.key[i] = ~(uint64_t)0; S.idx[i] = -1; }
+    __syncthreads();
+    for (int size = 2; size <= np; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int i = threadIdx.x; i < np / 2; i += blockDim.x) {
+                int lo = 2 * i - (i & (stride - 1)), hi = lo + stride;
+                bool up = ((lo & size) == 0);
+                uint64_t a = S.key[lo], b = S.key[hi];
+                if ((a > b) == up) {
+                    S.key[lo] = b; S.key[hi] = a;
+                    int32_t t = S.idx[lo]; S.idx[lo] = S.idx[hi]; S.idx[hi] = t;
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_topk(ScanArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    ScanShared &S = *reinterpret_cast<ScanShared *>(smem);
+    float *lut = reinterpret_cast<float *>(smem + sizeof(ScanShared));   // [mp*kp]
+    const int64_t q = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int ncell = a.qinfo[q * 8 + 0];
+    const int sum = a.qinfo[q * 8 + 1];
+    const int mp = a.mp, kp = a.kp, kc = a.kc;
+    const int lutn = mp * kp;
+    const float *dqr = a.dqr + (size_t)q * lutn;
+    const float *dqc = a.dqc + (size_t)q * a.mc * kc;
+    const float qsum = a.qs[q * 4];
+    const bool imi = a.kind == SC_KIND_IMI;
+    const int bs = kp * mp / a.mc;
+    const int half = mp >> 1;
+    if (tid == 0) { S.nb = 0; S.thr = ~(uint64_t)0; }
+    if (imi)
+        for (int i = tid; i < lutn; i += blockDim.x) lut[i] = dqr[i];
+    int lut_row = -1;     // h3 whose combined row is staged (IVF/SC)
+    int base_pos = 0;     // position of the first candidate of the run
+    const int32_t *cells = a.cells + (size_t)q * a.ccap;
+    const uint32_t *cinfo = a.cinfo + (size_t)q * a.ccap;
+    __syncthreads();
+
+    int c0 = 0;
+    while (c0 < ncell) {
+        // a run: consecutive cells sharing h3 (IVF/SC), or up to kRunCells (IMI)
+        if (tid == 0) {
+            int n = 0, acc = 0;
+            const int h3first = (int)(cinfo[c0] & 0xFFFFu);
+            while (c0 + n < ncell && n < kRunCells) {
+                uint32_t ci = cinfo[c0 + n];
+                if (!imi && (int)(ci & 0xFFFFu) != h3first) break;
+                if (a.kind == SC_KIND_IVFADC && n > 0) break;
+                int64_t bid = cells[c0 + n];
+                int st = bid > 0 ? a.L[bid - 1] : 0;
+                int l = a.L[bid] - st;
+                S.c_start[n] = st;
+                S.c_pref[n] = acc;
+                S.c_info[n] = ci;
+                acc += l;
+                ++n;
+            }
+            S.c_pref[n] = acc;
+            S.run_cells = n;
+        }
+        __syncthreads();
+        const int rn = S.run_cells;
+        const int rtotal = S.c_pref[rn];
+        if (!imi) {
+            const int h3 = (int)(S.c_info[0] & 0xFFFFu);
+            if (h3 != lut_row) {
+                const float *row = a.dot_cr + (size_t)h3 * bs;
+                for (int i = tid; i < lutn; i += blockDim.x) lut[i] = dqr[i] + row[i];
+                lut_row = h3;
+            }
+        }
+        __syncthreads();
+        for (int p0 = 0; p0 < rtotal; p0 += kScanThreads) {
+            if (S.nb > kBufCap - kScanThreads) compact(S, a.R);
+            const int p = p0 + tid;
+            if (p < rtotal) {
+                int lo = 0, hi = rn - 1;          // last cell with c_pref <= p
+                while (lo < hi) {
+                    int mid = (lo + hi + 1) >> 1;
+                    if (S.c_pref[mid] <= p) lo = mid; else hi = mid - 1;
+                }
+                const int off = p - S.c_pref[lo];
+                const int vidx = S.c_start[lo] + off;
+                const uint8_t *cc = a.codes + (size_t)vidx * mp;
+                const uint32_t ci = S.c_info[lo];
+                float d;
+                if (!imi) {
+                    d = qsum + dqc[ci & 0xFFFFu];
+                    for (int k = 0; k < mp; ++k) d += lut[k * kp + cc[k]];
+                } else {
+                    const int h3 = (int)(ci & 0xFFFFu), h4 = (int)(ci >> 16);
+                    d = (qsum + dqc[h3]) + dqc[kc + h4];
+                    const float *r1 = a.dot_cr + (size_t)h3 * bs;
+                    const float *r2 = a.dot_cr + (size_t)(h4 + kc - 1) * bs;
+                    for (int k = 0; k < half; ++k) {
+                        int b = k * kp + cc[k];
+                        d += lut[b];
+                        d += r1[b];
+                    }
+                    for (int k = half; k < mp; ++k) {
+                        int b = k * kp + cc[k];
+                        d += lut[b];
+                        d += r2[b];
+                    }
+                }
+                const uint64_t key = ((uint64_t)sc_float_key(d) << 32) | (uint32_t)(base_pos + p);
+                if (key < S.thr) {
+                    int o = atomicAdd(&S.nb, 1);
+                    S.key[o] = key;
+                    S.idx[o] = vidx;
+                }
+            }
+            __syncthreads();
+        }
+        base_pos += rtotal;
+        c0 += rn;
+        __syncthreads();
+    }
+    // final selection
+    const int R = a.R;
+    int nout;
+    if (sum >= R) {
+        if (S.nb > R) compact(S, R);
+        nout = R;
+        sort_buffer(S, R);
+    }
+    int32_t *ids = a.ids + (size_t)q * R;
+    float *dists = a.dists + (size_t)q * R;
+    if (sum < R) {
+        // fewer than R candidates: traversal (position) order, like the
+        // reference; swap the key halves so the sort runs on position
+        for (int i = tid; i < S.nb; i += blockDim.x) {
+            uint64_t k = S.key[i];
+            S.key[i] = (k << 32) | (k >> 32);
+        }
+        __syncthreads();
+        nout = S.nb;
+        sort_buffer(S, nout);
+    }
+    for (int r = tid; r < R; r += blockDim.x) {
+        if (r < nout) {
+            ids[r] = a.pid[S.idx[r]];
+            const uint64_t k = S.key[r];
+            dists[r] = sc_key_float(sum >= R ? (uint32_t)(k >> 32) : (uint32_t)k);
+        } else {
+            ids[r] = -1;
+            dists[r] = INFINITY;
+        }
+    }
+    if (a.sums && tid == 0) a.sums[q] = sum;
+}
+
+__global__ void k_gather_fallback(const int32_t *__restrict__ qinfo, int64_t nq,
+                                   int32_t *__restrict__ list, int32_t *__restrict__ n) {
+    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    if (qinfo[q * 8 + 5]) list[atomicAdd(n, 1)] = (int32_t)q;
+}
+
+__global__ void k_reduce_stats(const int32_t *__restrict__ qinfo, int64_t nq,
+                               unsigned long long *__restrict__ out) {
+    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long v[6] = {0, 0, 0, 0, 0, 0};
+    if (q < nq) {
+        const int32_t *qi = qinfo + q * 8;
+        v[0] = (unsigned long long)qi[1];
+        v[1] = (unsigned long long)qi[0];
+        v[2] = (unsigned long long)qi[2];
+        v[3] = (unsigned long long)qi[4];
+        v[4] = (unsigned long long)(qi[6] != 0);
+        v[5] = (unsigned long long)qi[5];
+    }
+    for (int i = 0; i < 6; ++i) {
+        unsigned long long t = v[i];
+        for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o);
+        if ((threadIdx.x & 63) == 0 && t) atomicAdd(&out[i], t);
+    }
+}
+
+struct Plan {
+    int64_t chunk;      // queries per pass
+    int ccap;
+    size_t off_qs, off_dqc, off_dqr, off_sv, off_si, off_cells, off_cinfo, off_qinfo,
+        off_list, off_stats, total;
+};
+
+Plan make_plan(const sc_index *ix, const sc_search_params *p, int64_t nq) {
+    Plan P{};
+    const int lists = p->kind == SC_KIND_IMI ? 2 : 1;
+    int64_t ccap = p->kind == SC_KIND_IVFADC ? p->w : std::min<int64_t>(p->w, (int64_t)p->T + 1);
+    if (ccap < 1) ccap = 1;
+    P.ccap = (int)ccap;
+    const size_t per = 16 + 4ull * ix->mc * ix->kc + 4ull * ix->mp * ix->kp +
+                       8ull * lists * ix->kc + 8ull * ccap + 32 + 4;
+    int64_t chunk = (int64_t)((size_t)1 << 30) / (int64_t)per;
+    chunk = std::max<int64_t>(1, std::min<int64_t>(chunk, std::max<int64_t>(nq, 1)));
+    chunk = std::min<int64_t>(chunk, 65536);
+    P.chunk = chunk;
+    size_t o = 0;
+    auto take = [&](size_t bytes) { size_t r = o; o = (o + bytes + 255) & ~(size_t)255; return r; };
+    P.off_qs = take(16ull * chunk);
+    P.off_dqc = take(4ull * ix->mc * ix->kc * chunk);
+    P.off_dqr = take(4ull * ix->mp * ix->kp * chunk);
+    P.off_sv = take(4ull * lists * ix->kc * chunk);
+    P.off_si = take(4ull * lists * ix->kc * chunk);
+    P.off_cells = take(4ull * ccap * chunk);
+    P.off_cinfo = take(4ull * ccap * chunk);
+    P.off_qinfo = take(32ull * chunk);
+    P.off_list = take(4ull * chunk + 4);
+    P.off_stats = take(8ull * 8);
+    P.total = o;
+    return P;
+}
+
+}  // namespace
+
+extern "C" int sc_search(sc_ctx *ctx, const sc_index *ix, const float *d_q, int64_t nq,
+                         const sc_search_params *p, int32_t *d_ids, float *d_dists,
+                         int32_t *d_sums, sc_search_stats *st) {
+    SC_TRY(sc_ctx_enter(ctx));
+    if (!ix || !p || (nq > 0 && (!d_q || !d_ids || !d_dists)) || nq < 0) {
+        sc_set_error("sc_search: bad arguments");
+        return SC_ERR_ARG;
+    }
+    if (p->R < 1 || p->R > kBufCap - kScanThreads || p->w < 1 || p->T < 0 ||
+        (p->form != SC_FORM_PLAIN && p->form != SC_FORM_MIXED)) {
+        sc_set_error("sc_search: bad parameters (R=%d w=%d T=%d form=%d)", p->R, p->w, p->T,
+                     p->form);
+        return SC_ERR_ARG;
+    }
+    const int kind = p->kind;
+    if ((kind == SC_KIND_IVFADC && (ix->mc != 1 || ix->nc != 1)) ||
+        (kind == SC_KIND_IMI && (ix->mc != 2 || ix->nc != 1)) ||
+        (kind == SC_KIND_SC && (ix->mc != 1 || ix->nc != 2)) || kind < 0 || kind > 2) {
+        sc_set_error("sc_search: kind %d does not match the index (mc=%d nc=%d)", kind, ix->mc,
+                     ix->nc);
+        return SC_ERR_CONFIG;
+    }
+    if (kind == SC_KIND_IVFADC && p->w > ix->kc) {
+        sc_set_error("sc_search: w=%d > kc=%d pops an empty heap (include/sc_utilities.h:274)",
+                     p->w, ix->kc);
+        return SC_ERR_HEAP_EMPTY;
+    }
+    if (kind == SC_KIND_SC && ix->kc < 2) return SC_ERR_ARG;
+    if (ix->kc > 8192 || ix->mp * ix->kp > 8192) {
+        sc_set_error("sc_search: kc=%d / mp*kp=%d exceed the on-chip ranking limits", ix->kc,
+                     ix->mp * ix->kp);
+        return SC_ERR_ARG;
+    }
+    if (st) *st = sc_search_stats{};
+    if (nq == 0) return SC_OK;
+
+    const Plan P = make_plan(ix, p, nq);
+    unsigned char *s;
+    SC_TRY(sc_scratch(ctx, P.total, (void **)&s));
+    float *qs = (float *)(s + P.off_qs), *dqc = (float *)(s + P.off_dqc),
+          *dqr = (float *)(s + P.off_dqr), *sv = (float *)(s + P.off_sv);
+    int32_t *si = (int32_t *)(s + P.off_si), *cells = (int32_t *)(s + P.off_cells);
+    uint32_t *cinfo = (uint32_t *)(s + P.off_cinfo);
+    int32_t *qinfo = (int32_t *)(s + P.off_qinfo), *list = (int32_t *)(s + P.off_list);
+    unsigned long long *stats = (unsigned long long *)(s + P.off_stats);
+    if (st) SC_HIP(hipMemsetAsync(stats, 0, 64, ctx->stream));
+
+    const int kc = ix->kc, lists = kind == SC_KIND_IMI ? 2 : 1;
+    int npow2 = 1;
+    while (npow2 < kc) npow2 <<= 1;
+    // traversal launch shape: heap + row lengths per lane in LDS
+    const int hcap_ref = kind == SC_KIND_IVFADC ? kc : p->w;
+    int hcap = std::min(hcap_ref, 256);
+    if (const char *e = getenv("SC_HEAP_ONCHIP")) {   // test hook: smaller on-chip heap
+        int v = atoi(e);
+        if (v >= 1) hcap = std::min(hcap, v);
+    }
+    const size_t lane_bytes = (size_t)hcap * 8 + (size_t)kc * 4;
+    int qpw = (int)std::min<size_t>(64, (48 * 1024) / lane_bytes);
+    if (qpw < 1) { qpw = 1; hcap = 0; }   // row lengths alone do not fit: global heap
+    int64_t fallback_total = 0;
+    // stage timing: five events per chunk on the context stream
+    std::vector<hipEvent_t> ev;
+    auto mark = [&]() {
+        if (!st) return;
+        hipEvent_t e;
+        if (hipEventCreate(&e) == hipSuccess) {
+            (void)hipEventRecord(e, ctx->stream);
+            ev.push_back(e);
+        }
+    };
+
+    for (int64_t q0 = 0; q0 < nq; q0 += P.chunk) {
+        const int64_t cn = std::min<int64_t>(P.chunk, nq - q0);
+        const float *Q = d_q + (size_t)q0 * ix->dim;
+        const int64_t lut_total = cn * (ix->mc * kc + ix->mp * ix->kp);
+        mark();
+        hipLaunchKernelGGL(k_lut, dim3((unsigned)sc_cdiv(lut_total, 256)), dim3(256), 0,
+                           ctx->stream, Q, cn, ix->dim, ix->cq, ix->pq, ix->norm_c, ix->norm_r,
+                           ix->mc, kc, ix->mp, ix->kp, dqc, dqr);
+        SC_CHECK_LAUNCH();
+        mark();
+        hipLaunchKernelGGL(k_rank, dim3((unsigned)cn), dim3(256), (size_t)npow2 * 8, ctx->stream,
+                           Q, ix->dim, kind, ix->mc, kc, dqc, qs, sv, si, npow2);
+        SC_CHECK_LAUNCH();
+        mark();
+        TravArgs ta{kind, kc, p->w, p->T, P.ccap, sv, si, ix->L, cells, cinfo, qinfo,
+                    nullptr, cn, nullptr, nullptr, nullptr};
+        if (hcap > 0) {
+            size_t lds = (size_t)qpw * lane_bytes;
+            hipLaunchKernelGGL(k_traverse, dim3((unsigned)sc_cdiv(cn, qpw)), dim3(qpw), lds,
+                               ctx->stream, ta, hcap, qpw);
+            SC_CHECK_LAUNCH();
+        }
+        // queries whose heap outgrew the on-chip capacity run again with a
+        // global-memory heap of the reference's capacity w
+        int32_t nfb = 0;
+        if (hcap < hcap_ref) {
+            int32_t *dn = list + P.chunk;
+            if (hcap > 0) {
+                SC_HIP(hipMemsetAsync(dn, 0, 4, ctx->stream));
+                hipLaunchKernelGGL(k_gather_fallback, dim3((unsigned)sc_cdiv(cn, 256)), dim3(256),
+                                   0, ctx->stream, qinfo, cn, list, dn);
+                SC_CHECK_LAUNCH();
+                SC_HIP(hipMemcpyAsync(&nfb, dn, 4, hipMemcpyDeviceToHost, ctx->stream));
+                SC_HIP(hipStreamSynchronize(ctx->stream));
+            } else {
+                nfb = (int32_t)cn;
+            }
+            if (nfb > 0) {
+                size_t gb = (size_t)nfb * ((size_t)hcap_ref * 8 + (size_t)kc * 4);
+                void *g = nullptr;
+                if (hipMalloc(&g, gb) != hipSuccess) {
+                    sc_set_error("sc_search: fallback heap allocation (%zu bytes) failed", gb);
+                    return SC_ERR_NOMEM;
+                }
+                TravArgs fa = ta;
+                fa.qlist = hcap > 0 ? list : nullptr;
+                fa.nq = nfb;
+                fa.gheap_v = (float *)g;
+                fa.gheap_p = (uint32_t *)((float *)g + (size_t)nfb * hcap_ref);
+                fa.grow = (int32_t *)(fa.gheap_p + (size_t)nfb * hcap_ref);
+                hipLaunchKernelGGL(k_traverse, dim3((unsigned)sc_cdiv(nfb, 64)), dim3(64), 0,
+                                   ctx->stream, fa, hcap_ref, 64);
+                hipError_t le = hipGetLastError();
+                (void)hipStreamSynchronize(ctx->stream);
+                (void)hipFree(g);
+                if (le != hipSuccess) {
+                    sc_set_error("sc_search: fallback traversal launch failed: %s",
+                                 hipGetErrorString(le));
+                    return SC_ERR_HIP;
+                }
+                fallback_total += nfb;
+            }
+        }
+        mark();
+        ScanArgs sa{kind, ix->dim, ix->mc, kc, ix->mp, ix->kp, p->R, P.ccap, qs, dqc, dqr,
+                    ix->dot_cr, ix->L, ix->pid, ix->codes, cells, cinfo, qinfo,
+                    d_ids + (size_t)q0 * p->R, d_dists + (size_t)q0 * p->R,
+                    d_sums ? d_sums + q0 : nullptr};
+        size_t lds = sizeof(ScanShared) + sizeof(float) * ix->mp * ix->kp;
+        hipLaunchKernelGGL(k_scan_topk, dim3((unsigned)cn), dim3(kScanThreads), lds, ctx->stream,
+                           sa);
+        SC_CHECK_LAUNCH();
+        mark();
+        if (st) {
+            hipLaunchKernelGGL(k_reduce_stats, dim3((unsigned)sc_cdiv(cn, 256)), dim3(256), 0,
+                               ctx->stream, qinfo, cn, stats);
+            SC_CHECK_LAUNCH();
+        }
+    }
+    (void)lists;
+    if (st) {
+        unsigned long long h[8];
+        SC_HIP(hipMemcpyAsync(h, stats, 64, hipMemcpyDeviceToHost, ctx->stream));
+        SC_HIP(hipStreamSynchronize(ctx->stream));
+        st->queries = nq;
+        st->candidates = (int64_t)h[0];
+        st->cells = (int64_t)h[1];
+        st->exhausted = (int64_t)h[3];
+        st->heap_fallback = fallback_total;
+        st->visited = (int64_t)h[2];
+        double acc[4] = {0, 0, 0, 0};
+        for (size_t c = 0; c + 4 < ev.size(); c += 5)
+            for (int k = 0; k < 4; ++k) {
+                float ms = 0.0f;
+                if (hipEventElapsedTime(&ms, ev[c + k], ev[c + k + 1]) == hipSuccess) acc[k] += ms;
+            }
+        st->ms_lut = acc[0]; st->ms_rank = acc[1]; st->ms_traverse = acc[2]; st->ms_scan = acc[3];
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+        if (h[4]) {
+            sc_set_error("sc_search: %llu queries hit an internal traversal error", h[4]);
+            return SC_ERR_HIP;
+        }
+    }
+    return SC_OK;
+}

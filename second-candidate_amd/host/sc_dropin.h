@@ -1,0 +1,836 @@
+// sc_dropin.h -- the reference's C++ class API (namespace SC) on top of the
+// C ABI in include/sc_hip.h.  Header-only; link with libsc_amd.so.
+//
+// Names, signatures, argument meaning, file formats and ownership follow the
+// reference headers cited per class; every computation runs in the HIP
+// library (there is no CPU compute path here).  Errors the reference answers
+// with exit(1)/cerr keep that behaviour: I/O failures print and exit, a
+// search kind that does not match the codebooks prints and returns with the
+// outputs untouched.
+#pragma once
+
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <ctime>
+#include <iostream>
+#include <vector>
+
+#include "../../include/sc_hip.h"
+
+namespace SC {
+
+typedef unsigned short ushort;
+
+// include/sc_utilities.h:801-805
+typedef struct {
+    int N, kc, kp, mc, mp, w, dim, T, L;
+    char db_prefix[256];
+    char db_path[256];
+} PQConfig;
+
+// include/bucket.h:20-24
+typedef struct {
+    int L = 0;
+    std::vector<int> pid;
+    std::vector<unsigned char> codes;
+} Bucket;
+
+namespace detail {
+
+[[noreturn]] inline void die(const char *what) {
+    std::cerr << what << std::endl;
+    std::exit(1);
+}
+
+inline void check(int rc, const char *what) {
+    if (rc != SC_OK) {
+        std::cerr << what << " failed (" << rc << "): " << sc_last_error() << std::endl;
+        std::exit(1);
+    }
+}
+
+// One context per process on device SC_DEVICE (default 0).
+inline sc_ctx *ctx() {
+    static sc_ctx *c = [] {
+        sc_ctx *h = nullptr;
+        const char *e = std::getenv("SC_DEVICE");
+        check(sc_ctx_create(e ? std::atoi(e) : 0, &h), "sc_ctx_create");
+        return h;
+    }();
+    return c;
+}
+
+// RAII device buffer.
+struct Dev {
+    void *p = nullptr;
+    size_t bytes = 0;
+    Dev() = default;
+    explicit Dev(size_t b) { alloc(b); }
+    Dev(const void *host, size_t b) { alloc(b); put(host, b); }
+    ~Dev() { release(); }
+    Dev(const Dev &) = delete;
+    Dev &operator=(const Dev &) = delete;
+    void alloc(size_t b) {
+        release();
+        bytes = b;
+        check(sc_malloc(ctx(), b ? b : 1, &p), "sc_malloc");
+    }
+    void put(const void *host, size_t b) { if (b) check(sc_memcpy_h2d(ctx(), p, host, b), "h2d"); }
+    void get(void *host, size_t b) const { if (b) check(sc_memcpy_d2h(ctx(), host, p, b), "d2h"); }
+    void release() {
+        if (p) sc_free(ctx(), p);
+        p = nullptr;
+    }
+    template <typename T> T *as() const { return static_cast<T *>(p); }
+};
+
+inline size_t file_size(const char *fn) {
+    struct stat s;
+    if (stat(fn, &s) != 0) return 0;
+    return (size_t)s.st_size;
+}
+
+// Rows of `d` values of type T, each preceded by `header` bytes
+// (include/sc_utilities.h:816-886), converted to float.
+template <typename T>
+inline int load_rows(const char *filename, float *&out, int header, int d, bool verbose) {
+    if (d <= 0 || !filename) die("Defective data!");
+    int fd = open(filename, O_RDONLY);
+    if (fd < 0) die("Cannot open the file");
+    size_t size = file_size(filename);
+    const unsigned char *m =
+        (const unsigned char *)mmap(nullptr, size ? size : 1, PROT_READ, MAP_PRIVATE, fd, 0);
+    if (m == MAP_FAILED) die("Cannot map the file");
+    size_t row = (size_t)d * sizeof(T) + header;
+    size_t n = row ? size / row : 0;
+    const size_t cnt = n * (size_t)d;
+    out = static_cast<float *>(::operator new(sizeof(float) * (cnt > 0 ? cnt : 1)));
+    for (size_t i = 0; i < n; ++i) {
+        const unsigned char *r = m + i * row + header;
+        for (int j = 0; j < d; ++j) {
+            T v;
+            std::memcpy(&v, r + (size_t)j * sizeof(T), sizeof(T));
+            out[i * d + j] = static_cast<float>(v);
+        }
+    }
+    munmap((void *)m, size ? size : 1);
+    close(fd);
+    if (verbose) std::cout << "The number of vectors: " << n << std::endl;
+    return (int)n;
+}
+
+}  // namespace detail
+
+// include/sc_utilities.h:896-980: float header [k, m, dim] then k*dim floats.
+// type 0 fills kc/mc (coarse), type 1 kp/mp (product).
+template <typename DataType>
+inline void load_codebook(const char *filename, PQConfig &config, DataType *&codebook, int type,
+                          bool verbose) {
+    int fd = open(filename, O_RDONLY);
+    if (fd < 0) detail::die("Cannot open the file");
+    size_t size = detail::file_size(filename);
+    if (size < 3 * sizeof(float)) detail::die("Wrong data");
+    const float *m = (const float *)mmap(nullptr, size, PROT_READ, MAP_PRIVATE, fd, 0);
+    if (m == MAP_FAILED) detail::die("Cannot map the file");
+    config.dim = (int)m[2];
+    int k = (int)m[0], mm = (int)m[1];
+    if (type == 0) { config.kc = k; config.mc = mm; }
+    else { config.kp = k; config.mp = mm; }
+    if (mm == 0) detail::die("Wrong data");
+    size_t cnt = (size_t)k * config.dim;
+    if ((3 + cnt) * sizeof(float) > size) detail::die("Wrong data");
+    codebook = static_cast<DataType *>(::operator new(sizeof(DataType) * cnt));
+    for (size_t i = 0; i < cnt; ++i) codebook[i] = (DataType)m[3 + i];
+    munmap((void *)m, size);
+    close(fd);
+    if (verbose)
+        std::cout << "Loaded a codebook at " << filename << " with " << mm << " sub codes"
+                  << std::endl;
+}
+
+// `.ctr_` writer (include/quantizer.h:370-431): header, centres, and the one
+// stretch byte the reference's lseek/write leaves at the end.
+inline void write_codebook(const char *fn, const float *centers, int nsc, int part, int dim) {
+    FILE *f = std::fopen(fn, "wb");
+    if (!f) detail::die("Cannot open the file");
+    float hdr[3] = {(float)nsc, (float)part, (float)dim};
+    std::fwrite(hdr, sizeof(float), 3, f);
+    std::fwrite(centers, sizeof(float), (size_t)nsc * dim, f);
+    std::fputc(0, f);
+    std::fclose(f);
+}
+
+// Training knobs with no reference counterpart: the reference's greg_kmeans
+// stops after one Lloyd update (k-means.h:363, 906) and seeds from
+// random_device; the drop-in runs `iterations` Lloyd updates from a seeded
+// k-means++.
+struct KmeansOptions {
+    int iterations = 25;
+    uint32_t seed = 20151001u;
+    int lagged = 0;
+};
+inline KmeansOptions &kmeans_options() {
+    static KmeansOptions o;
+    return o;
+}
+
+// ---------------------------------------------------------------------------
+// include/quantizer.h:46-106
+// ---------------------------------------------------------------------------
+template <typename DataType>
+class PQQuantizer {
+protected:
+    int dim, part, nsc, offset;
+    int N;
+    float *centers;
+    int *labels;
+    float **cq;
+    int k;
+    int *kc, *mc;
+    int residual_which_ = 0;   // 0: nearest (PQQuantizer), 1: second (SCQuantizer)
+
+public:
+    float *data;
+
+    PQQuantizer(int _dim, int _m, int _nsc, int _os, bool verbose)
+        : dim(_dim), part(_m), nsc(_nsc), offset(_os), N(0), centers(nullptr), labels(nullptr),
+          cq(nullptr), k(0), kc(nullptr), mc(nullptr), data(nullptr) {
+        if (dim <= 0 || part <= 0 || dim % part != 0) {
+            if (verbose) std::cerr << "Error at parameters of PQQuantizer" << std::endl;
+            std::exit(EXIT_FAILURE);
+        }
+        centers = static_cast<float *>(::operator new(sizeof(float) * (size_t)nsc * dim));
+    }
+    PQQuantizer(int _dim, int _m, int _nsc, int _os, int _k, bool verbose)
+        : PQQuantizer(_dim, _m, _nsc, _os, verbose) {
+        k = _k;
+        cq = static_cast<float **>(::operator new(sizeof(float *) * (k > 0 ? k : 1)));
+        for (int i = 0; i < k; ++i) cq[i] = nullptr;
+        kc = static_cast<int *>(::operator new(sizeof(int) * (k > 0 ? k : 1)));
+        mc = static_cast<int *>(::operator new(sizeof(int) * (k > 0 ? k : 1)));
+    }
+    virtual ~PQQuantizer() {
+        ::operator delete(centers);
+        ::operator delete(labels);
+        ::operator delete(data);
+        for (int i = 0; i < k; ++i) ::operator delete(cq[i]);
+        ::operator delete(cq);
+        ::operator delete(kc);
+        ::operator delete(mc);
+    }
+
+    // quantizer.h:194-205 (data converted from DataType rows with `offset`
+    // header bytes)
+    inline void load_data(const char *filename, bool verbose) {
+        ::operator delete(data);
+        N = detail::load_rows<DataType>(filename, data, offset, dim, verbose);
+        ::operator delete(labels);
+        labels = static_cast<int *>(::operator new(sizeof(int) * ((size_t)N * part + 1)));
+        if (verbose) std::cout << "Initialized " << N << " vectors" << std::endl;
+    }
+    // quantizer.h:213-219: takes ownership of _data.
+    inline void load_data_mat(float *_data, int _N) {
+        if (_N > 0) N = _N;
+        data = _data;
+        if (!labels) labels = static_cast<int *>(::operator new(sizeof(int) * ((size_t)N * part + 1)));
+    }
+    inline int size() { return N; }
+
+    // quantizer.h:235-272: one k-means per column block, on the GPU.
+    inline void create_sub_quantizers(bool verbose) {
+        const int bs = dim / part;
+        const KmeansOptions &o = kmeans_options();
+        detail::Dev dx((size_t)N * bs * sizeof(float)), dc((size_t)nsc * bs * sizeof(float)),
+            dl((size_t)N * sizeof(int));
+        std::vector<float> strip((size_t)N * bs);
+        for (int i = 0; i < part; ++i) {
+            for (size_t r = 0; r < (size_t)N; ++r)
+                std::memcpy(&strip[r * bs], data + r * dim + (size_t)i * bs, sizeof(float) * bs);
+            dx.put(strip.data(), strip.size() * sizeof(float));
+            if (verbose) std::cout << "Creating codebook " << i << "/" << part << std::endl;
+            detail::check(sc_kmeans(detail::ctx(), dx.as<float>(), N, bs, nsc, o.iterations,
+                                    o.seed + (uint32_t)i, o.lagged, SC_PREC_FP32, dc.as<float>(),
+                                    dl.as<int32_t>()),
+                          "sc_kmeans");
+            dc.get(centers + (size_t)i * nsc * bs, (size_t)nsc * bs * sizeof(float));
+            dl.get(labels + (size_t)i * N, (size_t)N * sizeof(int));
+        }
+    }
+
+    // quantizer.h:279-300 with k-means.h:444-464 (per-part float sum, sqrt).
+    inline double distortion(bool verbose) {
+        (void)verbose;
+        const int bs = dim / part;
+        double e = 0.0;
+        for (int i = 0; i < part; ++i) {
+            float et = 0.0f;
+            const float *c = centers + (size_t)i * nsc * bs;
+            for (size_t r = 0; r < (size_t)N; ++r) {
+                const float *x = data + r * dim + (size_t)i * bs;
+                const float *y = c + (size_t)labels[(size_t)i * N + r] * bs;
+                double acc = 0.0;
+                for (int t = 0; t < bs; ++t) {
+                    double dd = (double)(x[t] - y[t]);
+                    acc += dd * dd;
+                }
+                et += (float)acc;
+            }
+            double es = std::sqrt((double)et);
+            e += es * es;
+        }
+        return std::sqrt(e);
+    }
+
+    inline float *get_center_at(int id, bool verbose) {
+        (void)verbose;
+        return centers + (size_t)id * nsc * (dim / part);
+    }
+    inline int *get_label_at(int id, bool verbose) {
+        (void)verbose;
+        return labels + (size_t)id * N;
+    }
+
+    // quantizer.h:326-362
+    inline void set_params(int _k, int _m, int _nsc) {
+        for (int i = 0; i < k; ++i) ::operator delete(cq[i]);
+        ::operator delete(cq);
+        ::operator delete(kc);
+        ::operator delete(mc);
+        k = _k;
+        cq = static_cast<float **>(::operator new(sizeof(float *) * (k > 0 ? k : 1)));
+        for (int i = 0; i < k; ++i) cq[i] = nullptr;
+        kc = static_cast<int *>(::operator new(sizeof(int) * (k > 0 ? k : 1)));
+        mc = static_cast<int *>(::operator new(sizeof(int) * (k > 0 ? k : 1)));
+        part = _m;
+        nsc = _nsc;
+        ::operator delete(centers);
+        ::operator delete(labels);
+        if (dim <= 0 || part <= 0 || dim % part != 0) {
+            std::cerr << "Error at parameters of PQQuantizer" << std::endl;
+            std::exit(EXIT_FAILURE);
+        }
+        centers = static_cast<float *>(::operator new(sizeof(float) * (size_t)nsc * dim));
+        labels = static_cast<int *>(::operator new(sizeof(int) * ((size_t)N * part + 1)));
+    }
+
+    // quantizer.h:370-431 -> <filename>.ctr_
+    inline void output(const char *filename, bool verbose) {
+        (void)verbose;
+        char fn[512];
+        std::snprintf(fn, sizeof fn, "%s.ctr_", filename);
+        write_codebook(fn, centers, nsc, part, dim);
+    }
+
+    // quantizer.h:437-448
+    inline void load_codebooks(char **filename, bool verbose) {
+        PQConfig config;
+        for (int i = 0; i < k; ++i) {
+            ::operator delete(cq[i]);
+            load_codebook<float>(filename[i], config, cq[i], 0, verbose);
+            kc[i] = config.kc;
+            mc[i] = config.mc;
+        }
+    }
+
+    // quantizer.h:456-510 (nearest) / sc_quantizer.h:72-133 (second), on
+    // the GPU, in place on `data`.
+    inline void calc_residual_vector(bool verbose) {
+        (void)verbose;
+        detail::Dev dx(data, (size_t)N * dim * sizeof(float));
+        for (int k0 = 0; k0 < k; ++k0) {
+            detail::Dev dc(cq[k0], (size_t)kc[k0] * dim * sizeof(float));
+            detail::check(sc_residual(detail::ctx(), dx.as<float>(), N, dim, dc.as<float>(),
+                                      mc[k0], kc[k0], residual_which_),
+                          "sc_residual");
+        }
+        dx.get(data, (size_t)N * dim * sizeof(float));
+    }
+};
+
+// include/sc_quantizer.h:27-37: residuals against the second-nearest centre.
+// (The reference walks `part` blocks of dim/part there, sc_quantizer.h:98-102;
+// the drop-in uses the codebook's own mc blocks.)
+template <typename DataType>
+class SCQuantizer : public PQQuantizer<DataType> {
+public:
+    SCQuantizer(int _dim, int _m, int _nsc, int _os, int _k, bool verbose)
+        : PQQuantizer<DataType>(_dim, _m, _nsc, _os, _k, verbose) {
+        this->residual_which_ = 1;
+    }
+    virtual ~SCQuantizer() {}
+    inline void calc_residual_vector(bool verbose) {
+        PQQuantizer<DataType>::calc_residual_vector(verbose);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// include/encoder.h:30-60, src/encoder.cpp
+// ---------------------------------------------------------------------------
+class Encoder {
+protected:
+    PQConfig config;
+    float *cq, *pq;
+    int size = 0;
+    int non_empty_bucket = 0;
+    int *L, *pid;
+    ushort *cid;
+    unsigned char *codes;
+    std::vector<Bucket> ivf;
+    int old_mp;
+    int nc_ = 1;   // 1: IVF/IMI cells; 2: (nearest, second) cells
+
+    inline void encode_device(const float *x, int nc) {
+        const size_t n = (size_t)config.N;
+        detail::Dev dx(x, n * config.dim * sizeof(float));
+        detail::Dev dcq(cq, (size_t)config.kc * config.dim * sizeof(float));
+        detail::Dev dpq(pq, (size_t)config.kp * config.dim * sizeof(float));
+        detail::Dev dcid(n * config.mc * nc * sizeof(ushort)), dcodes(n * config.mp);
+        detail::check(sc_encode(detail::ctx(), dx.as<float>(), (int64_t)n, config.dim,
+                                dcq.as<float>(), config.mc, config.kc, dpq.as<float>(), config.mp,
+                                config.kp, nc, SC_PREC_FP32, dcid.as<uint16_t>(),
+                                dcodes.as<uint8_t>(), nullptr),
+                      "sc_encode");
+        ::operator delete(cid);
+        ::operator delete(codes);
+        cid = static_cast<ushort *>(::operator new(n * config.mc * nc * sizeof(ushort) + 2));
+        codes = static_cast<unsigned char *>(::operator new(n * config.mp + 1));
+        dcid.get(cid, n * config.mc * nc * sizeof(ushort));
+        dcodes.get(codes, n * config.mp);
+    }
+
+    // Buckets from the device index build (ids ascending per cell).
+    inline void distribute(int nc, int64_t ncells) {
+        const size_t n = (size_t)config.N;
+        detail::Dev dcid(cid, n * config.mc * nc * sizeof(ushort)), dcodes(codes, n * config.mp);
+        detail::Dev dcq(cq, (size_t)config.kc * config.dim * sizeof(float));
+        detail::Dev dpq(pq, (size_t)config.kp * config.dim * sizeof(float));
+        sc_index *ix = nullptr;
+        int rc = sc_index_build(detail::ctx(), dcid.as<uint16_t>(), dcodes.as<uint8_t>(),
+                                (int64_t)n, config.dim, dcq.as<float>(), config.mc, config.kc,
+                                dpq.as<float>(), config.mp, config.kp, nc, &ix);
+        if (rc == SC_ERR_TOO_LARGE) {
+            std::cerr << "The size of this inverted index is too LARGE!" << std::endl;
+            std::exit(EXIT_FAILURE);
+        }
+        detail::check(rc, "sc_index_build");
+        std::vector<int32_t> hL(ncells), hpid(n);
+        std::vector<uint8_t> hcodes(n * config.mp);
+        detail::check(sc_index_export(detail::ctx(), ix, hL.data(), hpid.data(), hcodes.data(),
+                                      nullptr, nullptr, nullptr),
+                      "sc_index_export");
+        sc_index_destroy(ix);
+        ivf.assign((size_t)ncells, Bucket());
+        non_empty_bucket = 0;
+        size_t start = 0;
+        for (int64_t c = 0; c < ncells; ++c) {
+            Bucket &b = ivf[(size_t)c];
+            b.L = hL[(size_t)c] - (int)start;
+            b.pid.assign(hpid.begin() + start, hpid.begin() + hL[(size_t)c]);
+            b.codes.assign(hcodes.begin() + start * config.mp,
+                           hcodes.begin() + (size_t)hL[(size_t)c] * config.mp);
+            if (b.L > 0) ++non_empty_bucket;
+            start = (size_t)hL[(size_t)c];
+        }
+    }
+
+    inline void write_ivf(const char *fname) {
+        FILE *f = std::fopen(fname, "wb");
+        if (!f) detail::die("Cannot open the file");
+        int N = config.N;
+        std::fwrite(&non_empty_bucket, sizeof(int), 1, f);
+        std::fwrite(&N, sizeof(int), 1, f);
+        for (const Bucket &b : ivf) {
+            std::fwrite(&b.L, sizeof(int), 1, f);
+            if (b.L > 0) {
+                std::fwrite(b.pid.data(), sizeof(int), b.pid.size(), f);
+                std::fwrite(b.codes.data(), 1, b.codes.size(), f);
+            }
+        }
+        std::fputc(0, f);   // the reference's lseek stretch byte
+        std::fclose(f);
+    }
+
+public:
+    Encoder() : cq(nullptr), pq(nullptr), L(nullptr), pid(nullptr), cid(nullptr),
+                codes(nullptr), old_mp(0) {
+        std::memset(&config, 0, sizeof config);
+    }
+    virtual ~Encoder() {
+        ::operator delete(cq);
+        ::operator delete(pq);
+        ::operator delete(cid);
+        ::operator delete(codes);
+        ::operator delete(L);
+        ::operator delete(pid);
+    }
+
+    // src/encoder.cpp:44-55
+    void load_codebooks(const char *cq_path, const char *pq_path, bool verbose) {
+        load_codebook<float>(cq_path, config, cq, 0, verbose);
+        load_codebook<float>(pq_path, config, pq, 1, verbose);
+        size = (int)std::pow((double)config.kc, (double)config.mc);
+        std::cout << "--> Settings: (kc,mc,kp,mp)=" << config.kc << " " << config.mc << " "
+                  << config.kp << " " << config.mp << std::endl;
+    }
+
+    // include/encoder.h:69-192
+    template <typename DataType>
+    inline void encode(const char *filename, int offset, bool verbose) {
+        float *x = nullptr;
+        config.N = detail::load_rows<DataType>(filename, x, offset, config.dim, verbose);
+        if (config.mc <= 0 || config.mp <= 0) {
+            std::cerr << "Nothing to do" << std::endl;
+            ::operator delete(x);
+            return;
+        }
+        nc_ = 1;
+        encode_device(x, 1);
+        ::operator delete(x);
+    }
+
+    // src/encoder.cpp:170-205
+    void distribution(bool verbose) {
+        distribute(1, size);
+        if (verbose)
+            std::cout << "The number of non-empty buckets: " << non_empty_bucket << std::endl;
+    }
+
+    // src/encoder.cpp:207-305 -> <path>/<prefix>_ivf.edat_
+    void output(const char *db_path, const char *db_prefix, bool verbose) {
+        (void)verbose;
+        char fname[600];
+        std::snprintf(fname, sizeof fname, "%s/%s_ivf.edat_", db_path, db_prefix);
+        write_ivf(fname);
+    }
+
+    PQConfig get_config() { return config; }
+    void set_mp(int mp) { old_mp = config.mp; config.mp = mp; }
+    void set_size(size_t s) { size = (int)s; }
+    // src/encoder.cpp:152-168
+    void statistic(bool detail_ = false) {
+        (void)detail_;
+        FILE *f = std::fopen("./ivf.txt", "w");
+        if (!f) return;
+        for (size_t i = 0; i < ivf.size(); ++i) {
+            std::fprintf(f, "Bucket %zu:%d\n", i, ivf[i].L);
+            for (int j = 0; j < ivf[i].L; ++j) {
+                std::fprintf(f, "%d:", ivf[i].pid[j]);
+                for (int t = 0; t < config.mp; ++t)
+                    std::fprintf(f, "%d ", (int)ivf[i].codes[(size_t)j * config.mp + t]);
+                std::fprintf(f, "\n");
+            }
+        }
+        std::fclose(f);
+    }
+    // Read-only views for tests and tools.
+    const ushort *cell_ids() const { return cid; }
+    const unsigned char *pq_codes() const { return codes; }
+    const std::vector<Bucket> &buckets() const { return ivf; }
+};
+
+// include/sc_encoder.h:32-54, src/sc_encoder.cpp
+class SCEncoder : public Encoder {
+protected:
+    int nc;
+
+public:
+    SCEncoder() : Encoder(), nc(2) {}
+    explicit SCEncoder(int _nc) : Encoder(), nc(_nc) { std::cout << "nc = " << nc << std::endl; }
+    virtual ~SCEncoder() {}
+
+    // include/sc_encoder.h:63-195: nearest + second per block
+    template <typename DataType>
+    inline void encode2(const char *filename, int offset, bool verbose) {
+        float *x = nullptr;
+        config.N = detail::load_rows<DataType>(filename, x, offset, config.dim, verbose);
+        if (config.mc <= 0 || config.mp <= 0) {
+            std::cerr << "Nothing to do" << std::endl;
+            ::operator delete(x);
+            return;
+        }
+        nc_ = 2;
+        encode_device(x, 2);
+        ::operator delete(x);
+    }
+    // include/sc_encoder.h:205-332: the general-nc encoder is built for nc=2
+    // only (its distribution is broken for nc != 2, src/sc_encoder.cpp:41-59).
+    template <typename DataType>
+    inline void encode(const char *filename, int offset, bool verbose) {
+        if (nc != 2) {
+            std::cerr << "SCEncoder::encode: only nc=2 is supported" << std::endl;
+            return;
+        }
+        encode2<DataType>(filename, offset, verbose);
+    }
+
+    // src/sc_encoder.cpp:17-81
+    void distribution(bool verbose) {
+        double cells = std::pow((double)config.kc, (double)config.mc * nc);
+        if (cells >= (double)INT32_MAX) {
+            std::cerr << "The size of this inverted index is too LARGE!" << std::endl;
+            std::exit(EXIT_FAILURE);
+        }
+        distribute(2, (int64_t)cells);
+        if (verbose)
+            std::cout << "The number of non-empty buckets: " << non_empty_bucket << std::endl;
+    }
+
+    // src/sc_encoder.cpp:83-182 -> <path>/<prefix>_mr<nc>_ivf.edat_
+    void output(const char *db_path, const char *db_prefix, bool verbose) {
+        (void)verbose;
+        char fname[600];
+        std::snprintf(fname, sizeof fname, "%s/%s_mr%d_ivf.edat_", db_path, db_prefix, nc);
+        write_ivf(fname);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// include/query.h:45-80, src/query.cpp
+// ---------------------------------------------------------------------------
+class PQQuery {
+protected:
+    int size = 0;
+    int not_empty = 0;
+    float *cq, *pq;
+    int *L;
+    int *pid;
+    unsigned char *codes;
+    PQConfig config;
+    float *raw_data;
+    int nc_ = 1;
+    sc_index *index_ = nullptr;
+    detail::Dev q_, ids_, dists_, sums_;
+    sc_search_stats last_{};
+
+    // Loads `<file>` (int non_empty, int N, per cell: L, pids, codes) into
+    // the host arrays and the device index (src/query.cpp:82-165).
+    inline int load_ivf(const char *filename, size_t cells, int nc) {
+        int fd = open(filename, O_RDONLY);
+        if (fd < 0) detail::die("Cannot open the file");
+        size_t fsz = detail::file_size(filename);
+        const unsigned char *m =
+            (const unsigned char *)mmap(nullptr, fsz, PROT_READ, MAP_PRIVATE, fd, 0);
+        if (m == MAP_FAILED) detail::die("Cannot map the file");
+        const unsigned char *t = m;
+        std::memcpy(&not_empty, t, 4); t += 4;
+        std::memcpy(&config.N, t, 4); t += 4;
+        ::operator delete(L);
+        ::operator delete(pid);
+        ::operator delete(codes);
+        L = static_cast<int *>(::operator new(sizeof(int) * cells));
+        pid = static_cast<int *>(::operator new(sizeof(int) * ((size_t)config.N + 1)));
+        codes = static_cast<unsigned char *>(::operator new((size_t)config.N * config.mp + 1));
+        size_t base = 0;
+        for (size_t i = 0; i < cells; ++i) {
+            int l;
+            std::memcpy(&l, t, 4); t += 4;
+            L[i] = l + (i > 0 ? L[i - 1] : 0);
+            if (l > 0) {
+                std::memcpy(pid + base, t, (size_t)l * 4); t += (size_t)l * 4;
+                std::memcpy(codes + base * config.mp, t, (size_t)l * config.mp);
+                t += (size_t)l * config.mp;
+                base += (size_t)l;
+            }
+        }
+        munmap((void *)m, fsz);
+        close(fd);
+        if (index_) sc_index_destroy(index_);
+        index_ = nullptr;
+        nc_ = nc;
+        detail::check(sc_index_load(detail::ctx(), L, (int64_t)cells, pid, codes, config.N,
+                                    config.dim, cq, config.mc, config.kc, pq, config.mp,
+                                    config.kp, nc, &index_),
+                      "sc_index_load");
+        return config.N;
+    }
+
+    // One query through the batched device search; fills the reference's
+    // callee-allocated result/dist arrays (length sum, first min(R, sum)
+    // entries are the answer).
+    inline bool run_one(const float *query, int kind, int R, int w, int T, int M, float *&dist,
+                        int *&result, int &sum) {
+        if (!q_.p) q_.alloc(sizeof(float) * config.dim);
+        if (ids_.bytes < sizeof(int) * (size_t)R) {
+            ids_.alloc(sizeof(int) * (size_t)R);
+            dists_.alloc(sizeof(float) * (size_t)R);
+        }
+        if (!sums_.p) sums_.alloc(sizeof(int));
+        q_.put(query, sizeof(float) * config.dim);
+        sc_search_params p{kind, SC_FORM_PLAIN, R, w, T, M};
+        int rc = sc_search(detail::ctx(), index_, q_.as<float>(), 1, &p, ids_.as<int32_t>(),
+                           dists_.as<float>(), sums_.as<int32_t>(), &last_);
+        if (rc == SC_ERR_CONFIG) {
+            std::cerr << sc_last_error() << std::endl;
+            return false;
+        }
+        detail::check(rc, "sc_search");
+        sums_.get(&sum, sizeof(int));
+        const size_t len = (size_t)(sum > R ? sum : R);
+        result = static_cast<int *>(::operator new(sizeof(int) * len));
+        dist = static_cast<float *>(::operator new(sizeof(float) * len));
+        ids_.get(result, sizeof(int) * (size_t)R);
+        dists_.get(dist, sizeof(float) * (size_t)R);
+        return true;
+    }
+
+public:
+    PQQuery() : cq(nullptr), pq(nullptr), L(nullptr), pid(nullptr), codes(nullptr),
+                raw_data(nullptr) {
+        std::memset(&config, 0, sizeof config);
+    }
+    virtual ~PQQuery() {
+        if (index_) sc_index_destroy(index_);
+        ::operator delete(cq);
+        ::operator delete(pq);
+        ::operator delete(L);
+        ::operator delete(pid);
+        ::operator delete(codes);
+        ::operator delete(raw_data);
+    }
+
+    // src/query.cpp:66-75
+    void load_codebooks(const char *cq_path, const char *pq_path, bool verbose) {
+        load_codebook<float>(cq_path, config, cq, 0, verbose);
+        load_codebook<float>(pq_path, config, pq, 1, verbose);
+        size = (int)std::pow((double)config.kc, (double)config.mc);
+        std::cout << "--> Settings: (kc,mc,kp,mp)=" << config.kc << " " << config.mc << " "
+                  << config.kp << " " << config.mp << std::endl;
+    }
+    int load_encoded_data(const char *filename, bool verbose) {
+        (void)verbose;
+        return load_ivf(filename, (size_t)size, 1);
+    }
+    // query.h:83-86 (raw vectors for real distances)
+    template <typename DataType>
+    inline void load_data(const char *filename, int offset, bool verbose) {
+        ::operator delete(raw_data);
+        config.N = detail::load_rows<DataType>(filename, raw_data, offset, config.dim, verbose);
+    }
+    // The tables and LUTs live on the device and are built inside the search
+    // (query.h:91-194); these keep the call sequence of the reference.
+    inline void pre_compute1() {}
+    inline void pre_compute2(float *) {}
+    inline void pre_compute3(float *) {}
+
+    // query.h:202-325
+    inline void search_ivfadc(float *query, float *&v_tmp, float *&dist, int *&result,
+                              int *&buckets, int *&prebuck, int &sum, int R, int w, int T,
+                              bool real_dist, bool verbose) {
+        (void)v_tmp; (void)buckets; (void)prebuck; (void)real_dist; (void)verbose;
+        if (config.mc != 1) {
+            std::cerr << "This search method is for IVFADC only" << std::endl;
+            return;
+        }
+        run_one(query, SC_KIND_IVFADC, R, w, T, config.kc, dist, result, sum);
+    }
+
+    // Batched search (no reference counterpart): nq queries in one call,
+    // results [nq][R] on the host.
+    inline void search_batch(const float *queries, int nq, int kind, int R, int w, int T,
+                             int *result, float *dist, int *sums) {
+        detail::Dev dq(queries, sizeof(float) * (size_t)nq * config.dim);
+        detail::Dev di(sizeof(int) * (size_t)nq * R), dd(sizeof(float) * (size_t)nq * R),
+            ds(sizeof(int) * (size_t)nq);
+        sc_search_params p{kind, SC_FORM_PLAIN, R, w, T, config.kc};
+        detail::check(sc_search(detail::ctx(), index_, dq.as<float>(), nq, &p, di.as<int32_t>(),
+                                dd.as<float>(), ds.as<int32_t>(), &last_),
+                      "sc_search");
+        di.get(result, sizeof(int) * (size_t)nq * R);
+        dd.get(dist, sizeof(float) * (size_t)nq * R);
+        if (sums) ds.get(sums, sizeof(int) * (size_t)nq);
+    }
+
+    int get_size() { return not_empty; }
+    int get_full_size() { return size; }
+    // src/query.cpp:168-182
+    double entropy(size_t sz) {
+        double e = 0.0, Nd = config.N;
+        for (size_t i = 0; i < sz; ++i) {
+            double l = i > 0 ? L[i] - L[i - 1] : L[0];
+            if (l > 0) {
+                double x = Nd / l;
+                e += std::log2(x) / x;
+            }
+        }
+        return e;
+    }
+    const sc_search_stats &last_stats() const { return last_; }
+};
+
+// include/multi_query.h:36-53
+class MultiQuery : public PQQuery {
+public:
+    MultiQuery() : PQQuery() {}
+    virtual ~MultiQuery() {}
+
+    // multi_query.h:62-273; t1/t2 accumulate clock ticks of the LUT and
+    // coarse-sort stages like the reference (stage times from HIP events).
+    inline void search_multi2(float *query, float *&v_tmp, float *&dist, float *&q, int *tmp,
+                              int *&result, int *&hid1, int *&hid2, int *&hid3, int *&hid4,
+                              int *&s1, int *&s2, int *&prebuck, int *&cache, bool *traversed,
+                              int &sum, int R, int w, int T, int M, int &e, double &t1,
+                              double &t2, bool real_dist, bool verbose) {
+        (void)v_tmp; (void)q; (void)tmp; (void)hid1; (void)hid2; (void)hid3; (void)hid4;
+        (void)s1; (void)s2; (void)prebuck; (void)cache; (void)traversed; (void)real_dist;
+        (void)verbose;
+        if (config.mc != 2) {
+            std::cerr << "This search method is for Multi-D-ADC-2 only" << std::endl;
+            return;
+        }
+        if (!run_one(query, SC_KIND_IMI, R, w, T, M, dist, result, sum)) return;
+        e = (int)(last_.visited - last_.cells);
+        t1 += last_.ms_lut * CLOCKS_PER_SEC / 1000.0;
+        t2 += last_.ms_rank * CLOCKS_PER_SEC / 1000.0;
+    }
+};
+
+// include/sc_query.h:21-51, src/sc_query.cpp
+class SCQuery : public PQQuery {
+protected:
+    int nc;
+
+public:
+    SCQuery() : PQQuery(), nc(2) {}
+    explicit SCQuery(int _nc) : PQQuery(), nc(_nc) {}
+    virtual ~SCQuery() {}
+
+    // src/sc_query.cpp:36-123: kc^(mc*nc) cells
+    int load_encoded_data(const char *filename, bool verbose) {
+        (void)verbose;
+        size_t cells = (size_t)std::pow((double)size, (double)nc);
+        return load_ivf(filename, cells, nc);
+    }
+
+    // sc_query.h:60-256 (the top-R comes back sorted by (dist, position))
+    inline void search_mr_ivf(float *query, float *&v_tmp, float *&dist, int *tmp, int *&result,
+                              int *&hid1, int *&hid2, int *&hid3, int *&hid4, int *&s1, int *&s2,
+                              int *&prebuck, int *&cache, bool *traversed, int &sum, int R,
+                              int w, int T, int M, bool real_dist, bool verbose) {
+        (void)v_tmp; (void)tmp; (void)hid1; (void)hid2; (void)hid3; (void)hid4; (void)s1;
+        (void)s2; (void)prebuck; (void)cache; (void)traversed; (void)real_dist; (void)verbose;
+        if (config.mc != 1 || nc != 2) {
+            std::cerr << "This search method is for MultiRank IVFADC only" << std::endl;
+            return;
+        }
+        run_one(query, SC_KIND_SC, R, w, T, M, dist, result, sum);
+    }
+
+    // sc_query.h:265-475 (nc = 3) is not built in this library.
+    inline void search_mr_ivf3(float *, float *&, float *&, int *, int *&, int *&, int *&, int *&,
+                               int *&, int *&, int *&, int *&, int *&, bool *, int &, int, int,
+                               int, int, bool, bool) {
+        std::cerr << "This search method is for MultiRank-3 IVFADC only" << std::endl;
+    }
+};
+
+}  // namespace SC

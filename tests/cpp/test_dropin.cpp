@@ -1,0 +1,261 @@
+// test_dropin.cpp -- the reference's stage sequence (test/test_quantizer.cpp,
+// test/test_sc_encoder.cpp, test/test_sc_query.cpp, test/test_multi_query.cpp,
+// test/test_query.cpp) driven through the drop-in classes of
+// second-candidate_amd/host/sc_dropin.h, checked against the C oracle
+// (oracle/liboracle.so; test infrastructure).  Needs a GPU.
+//
+// usage: test_dropin <scratch dir>     exit 0 = all checks passed
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../oracle/sc_oracle.h"
+#include "../../second-candidate_amd/host/sc_dropin.h"
+
+static int g_fail = 0;
+#define EXPECT(c)                                                                  \
+    do {                                                                           \
+        if (!(c)) {                                                                \
+            std::fprintf(stderr, "FAIL %s:%d: %s\n", __FILE__, __LINE__, #c);      \
+            ++g_fail;                                                              \
+        }                                                                          \
+    } while (0)
+
+static void write_fvecs(const std::string &fn, const std::vector<float> &x, int d) {
+    FILE *f = std::fopen(fn.c_str(), "wb");
+    size_t n = x.size() / d;
+    for (size_t i = 0; i < n; ++i) {
+        std::fwrite(&d, 4, 1, f);
+        std::fwrite(&x[i * d], 4, d, f);
+    }
+    std::fclose(f);
+}
+
+static std::vector<float> gen(uint32_t stream, int64_t n, int d) {
+    sco_mix m = {20151001u, 16, 8, 0, 24 * 256, 208 * 256, 8 * 256, 24 * 256, 255};
+    std::vector<float> x((size_t)n * d);
+    sco_gen_rows(&m, stream, 0, n, d, x.data(), 8);
+    return x;
+}
+
+static std::vector<float> read_ctr(const std::string &fn, int *k, int *m, int *dim) {
+    SC::PQConfig c;
+    float *cb = nullptr;
+    SC::load_codebook<float>(fn.c_str(), c, cb, 0, false);
+    *k = c.kc; *m = c.mc; *dim = c.dim;
+    std::vector<float> v(cb, cb + (size_t)c.kc * c.dim);
+    ::operator delete(cb);
+    return v;
+}
+
+// Trains cq (mc blocks, kc) and pq (8 x 256) on the residuals, like
+// test_quantizer.cpp:58-117, and writes <dir>/<tag>_cq.ctr_, _pq.ctr_.
+static void train(const std::string &dir, const std::string &tag, const std::string &train_fv,
+                  int dim, int mc, int kc, bool second) {
+    SC::kmeans_options().iterations = 6;
+    SC::PQQuantizer<float> *cq = second ? new SC::SCQuantizer<float>(dim, mc, kc, 4, 1, false)
+                                        : new SC::PQQuantizer<float>(dim, mc, kc, 4, 1, false);
+    cq->load_data(train_fv.c_str(), false);
+    EXPECT(cq->size() == 4000);
+    cq->create_sub_quantizers(false);
+    const std::string cfn = dir + "/" + tag + "_cq";
+    cq->output(cfn.c_str(), false);
+    // labels are the argmin of the trained centres (first block)
+    const int bs = dim / mc;
+    const float *c0 = cq->get_center_at(0, false);
+    for (int r = 0; r < 50; ++r) {
+        double best = 1e300;
+        int lab = -1;
+        for (int j = 0; j < kc; ++j) {
+            double d = sco_dist_l2sq(cq->data + (size_t)r * dim, c0 + (size_t)j * bs, bs);
+            if ((float)d <= best) { best = (float)d; lab = j; }
+        }
+        EXPECT(cq->get_label_at(0, false)[r] == lab);
+    }
+    EXPECT(cq->distortion(false) > 0.0);
+    cq->set_params(1, 8, 256);
+    std::string cpath = cfn + ".ctr_";
+    char *names[1] = {&cpath[0]};
+    cq->load_codebooks(names, false);
+    cq->calc_residual_vector(false);
+    cq->create_sub_quantizers(false);
+    const std::string pfn = dir + "/" + tag + "_pq";
+    cq->output(pfn.c_str(), false);
+    delete cq;
+}
+
+struct OracleIx {
+    std::vector<float> cq, pq, nc, nr, dcr;
+    std::vector<int32_t> L, pid;
+    std::vector<uint8_t> codes;
+    sco_index s;
+};
+
+static void oracle_index(OracleIx &o, const std::vector<float> &x, int dim, int mc, int kc,
+                         int nc) {
+    const int64_t n = (int64_t)x.size() / dim;
+    std::vector<uint16_t> cid((size_t)n * mc * nc);
+    std::vector<uint8_t> codes((size_t)n * 8);
+    EXPECT(sco_encode(x.data(), n, dim, o.cq.data(), mc, kc, o.pq.data(), 8, 256, nc, cid.data(),
+                      codes.data(), 8) == 0);
+    std::vector<uint64_t> keys(n);
+    sco_cell_keys(cid.data(), n, mc, kc, nc, keys.data());
+    int64_t cells = 1;
+    for (int j = 0; j < mc * nc; ++j) cells *= kc;
+    o.L.resize(cells);
+    o.pid.resize(n);
+    o.codes.resize((size_t)n * 8);
+    EXPECT(sco_build_dense(keys.data(), n, cells, codes.data(), 8, o.L.data(), o.pid.data(),
+                           o.codes.data()) == 0);
+    o.nc.resize((size_t)mc * kc);
+    o.nr.resize(8 * 256);
+    o.dcr.resize((size_t)kc * 8 * 256);
+    sco_tables(o.cq.data(), o.pq.data(), dim, mc, kc, 8, 256, o.nc.data(), o.nr.data(),
+               o.dcr.data());
+    o.s = sco_index{dim, mc, kc, 8, 256, o.cq.data(), o.pq.data(), o.nc.data(), o.nr.data(),
+                    o.dcr.data(), cells, o.L.data(), o.pid.data(), o.codes.data()};
+}
+
+// The .edat_ file holds exactly the oracle's cells.
+static void check_edat(const std::string &fn, const OracleIx &o, int64_t n) {
+    FILE *f = std::fopen(fn.c_str(), "rb");
+    EXPECT(f != nullptr);
+    if (!f) return;
+    int ne = 0, N = 0;
+    EXPECT(std::fread(&ne, 4, 1, f) == 1 && std::fread(&N, 4, 1, f) == 1);
+    EXPECT(N == n);
+    int nonempty = 0, prev = 0;
+    for (size_t c = 0; c < o.L.size(); ++c) {
+        int l = 0;
+        EXPECT(std::fread(&l, 4, 1, f) == 1);
+        EXPECT(l == o.L[c] - prev);
+        if (l > 0) {
+            ++nonempty;
+            std::vector<int> p(l);
+            std::vector<uint8_t> cs((size_t)l * 8);
+            EXPECT(std::fread(p.data(), 4, l, f) == (size_t)l);
+            EXPECT(std::fread(cs.data(), 1, cs.size(), f) == cs.size());
+            EXPECT(std::memcmp(p.data(), &o.pid[prev], 4 * (size_t)l) == 0);
+            EXPECT(std::memcmp(cs.data(), &o.codes[(size_t)prev * 8], cs.size()) == 0);
+        }
+        prev = o.L[c];
+    }
+    EXPECT(ne == nonempty);
+    EXPECT(std::fgetc(f) == 0 && std::fgetc(f) == EOF);   // the stretch byte
+    std::fclose(f);
+}
+
+int main(int argc, char **argv) {
+    const std::string dir = argc > 1 ? argv[1] : "/tmp";
+    const int dim = 128;
+    std::vector<float> xt = gen(1, 4000, dim), xb = gen(2, 20000, dim), xq = gen(3, 40, dim);
+    write_fvecs(dir + "/train.fvecs", xt, dim);
+    write_fvecs(dir + "/base.fvecs", xb, dim);
+
+    // ---------------- SC: mc=1, kc=32, (nearest, second) cells
+    train(dir, "sc", dir + "/train.fvecs", dim, 1, 32, true);
+    {
+        SC::SCEncoder enc;
+        enc.load_codebooks((dir + "/sc_cq.ctr_").c_str(), (dir + "/sc_pq.ctr_").c_str(), false);
+        enc.encode2<float>((dir + "/base.fvecs").c_str(), 4, false);
+        EXPECT(enc.get_config().N == 20000 && enc.get_config().mc == 1 && enc.get_config().mp == 8);
+        enc.distribution(false);
+        enc.output(dir.c_str(), "base", false);
+        OracleIx o;
+        int k, m, d;
+        o.cq = read_ctr(dir + "/sc_cq.ctr_", &k, &m, &d);
+        EXPECT(k == 32 && m == 1 && d == dim);
+        o.pq = read_ctr(dir + "/sc_pq.ctr_", &k, &m, &d);
+        EXPECT(k == 256 && m == 8);
+        oracle_index(o, xb, dim, 1, 32, 2);
+        check_edat(dir + "/base_mr2_ivf.edat_", o, 20000);
+
+        SC::SCQuery q;
+        q.load_codebooks((dir + "/sc_cq.ctr_").c_str(), (dir + "/sc_pq.ctr_").c_str(), false);
+        EXPECT(q.load_encoded_data((dir + "/base_mr2_ivf.edat_").c_str(), false) == 20000);
+        q.pre_compute1();
+        const int w = 300, T = 1500, R = 100, kc = 32;
+        float *v_tmp = new float[kc + w], *dist = nullptr;
+        int *it = new int[kc], *result = nullptr;
+        int *h1 = new int[w], *h2 = new int[w], *h3 = new int[w], *h4 = new int[w];
+        int *s1 = new int[w], *s2 = new int[w], *pb = new int[w], *cache = new int[w];
+        bool *trav = new bool[kc * kc]();
+        trav[0] = true;
+        for (int i = 0; i < 40; ++i) {
+            int sum = 0;
+            q.search_mr_ivf(&xq[(size_t)i * dim], v_tmp, dist, it, result, h1, h2, h3, h4, s1, s2,
+                            pb, cache, trav, sum, R, w, T, kc, false, false);
+            EXPECT(trav[0] && !trav[1]);   // test_sc_query.cpp:182-183
+            std::vector<int32_t> oid(R);
+            std::vector<float> od(R);
+            sco_stats st;
+            EXPECT(sco_search(&o.s, SCO_SC, &xq[(size_t)i * dim], R, w, T, kc, oid.data(),
+                              od.data(), &st, nullptr, nullptr, nullptr, nullptr, nullptr, 0) == 0);
+            EXPECT(sum == st.sum);
+            for (int r = 0; r < (R < sum ? R : sum); ++r) {
+                EXPECT(result[r] == oid[r]);
+                EXPECT(std::memcmp(&dist[r], &od[r], 4) == 0);
+            }
+            ::operator delete(result);
+            ::operator delete(dist);
+        }
+        std::vector<int> bres(40 * R), bs(40);
+        std::vector<float> bd(40 * R);
+        q.search_batch(xq.data(), 40, SC_KIND_SC, R, w, T, bres.data(), bd.data(), bs.data());
+        for (int i = 0; i < 40; ++i) {
+            std::vector<int32_t> oid(R);
+            std::vector<float> od(R);
+            sco_stats st;
+            sco_search(&o.s, SCO_SC, &xq[(size_t)i * dim], R, w, T, kc, oid.data(), od.data(),
+                       &st, nullptr, nullptr, nullptr, nullptr, nullptr, 0);
+            EXPECT(std::memcmp(&bres[(size_t)i * R], oid.data(), 4 * (size_t)R) == 0);
+        }
+        EXPECT(q.get_full_size() == 32 && q.get_size() > 0 && q.entropy(32 * 32) > 0.0);
+    }
+
+    // ---------------- IMI: mc=2, kc=16 (MultiQuery::search_multi2)
+    train(dir, "imi", dir + "/train.fvecs", dim, 2, 16, false);
+    {
+        SC::Encoder enc;
+        enc.load_codebooks((dir + "/imi_cq.ctr_").c_str(), (dir + "/imi_pq.ctr_").c_str(), false);
+        enc.encode<float>((dir + "/base.fvecs").c_str(), 4, false);
+        enc.distribution(false);
+        enc.output(dir.c_str(), "base", false);
+        OracleIx o;
+        int k, m, d;
+        o.cq = read_ctr(dir + "/imi_cq.ctr_", &k, &m, &d);
+        EXPECT(k == 16 && m == 2);
+        o.pq = read_ctr(dir + "/imi_pq.ctr_", &k, &m, &d);
+        oracle_index(o, xb, dim, 2, 16, 1);
+        check_edat(dir + "/base_ivf.edat_", o, 20000);
+        SC::MultiQuery q;
+        q.load_codebooks((dir + "/imi_cq.ctr_").c_str(), (dir + "/imi_pq.ctr_").c_str(), false);
+        EXPECT(q.load_encoded_data((dir + "/base_ivf.edat_").c_str(), false) == 20000);
+        const int w = 100, T = 2000, R = 50, kc = 16;
+        float *v_tmp = new float[2 * kc + w], *dist = nullptr, *qq = new float[2];
+        int *it = new int[2 * kc], *result = nullptr;
+        int *h1 = new int[w], *h2 = new int[w], *h3 = new int[w], *h4 = new int[w];
+        int *s1 = new int[w], *s2 = new int[w], *pb = new int[w], *cache = new int[w];
+        bool *trav = new bool[kc * kc]();
+        double t1 = 0, t2 = 0;
+        for (int i = 0; i < 40; ++i) {
+            int sum = 0, e = 0;
+            q.search_multi2(&xq[(size_t)i * dim], v_tmp, dist, qq, it, result, h1, h2, h3, h4, s1,
+                            s2, pb, cache, trav, sum, R, w, T, kc, e, t1, t2, false, false);
+            EXPECT(!trav[0]);   // test_multi_query.cpp:191
+            std::vector<int32_t> oid(R);
+            std::vector<float> od(R);
+            sco_stats st;
+            sco_search(&o.s, SCO_IMI, &xq[(size_t)i * dim], R, w, T, kc, oid.data(), od.data(),
+                       &st, nullptr, nullptr, nullptr, nullptr, nullptr, 0);
+            EXPECT(sum == st.sum && e == st.empty);
+            for (int r = 0; r < (R < sum ? R : sum); ++r) EXPECT(result[r] == oid[r]);
+            ::operator delete(result);
+            ::operator delete(dist);
+        }
+        EXPECT(t1 > 0 && t2 > 0);
+    }
+    std::printf("%s: %d failure(s)\n", g_fail ? "FAILED" : "OK", g_fail);
+    return g_fail ? 1 : 0;
+}

@@ -1,0 +1,259 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle.
+
+Integer and index outputs (codes, cell ids, L/pid, neighbour ids) must match
+bit for bit; FP32 outputs (tables, ADC distances) are computed with the same
+operation sequence as the oracle and must match bit for bit as well.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import sc_amd as S
+import sco
+
+pytestmark = pytest.mark.gpu
+
+
+def _bits(a):
+    return np.ascontiguousarray(a).view(np.uint32)
+
+
+# ------------------------------------------------------------------ generator
+def test_generator_matches_oracle(ctx):
+    for mix, omix, dim, row0, n in ((S.mix_d128(), sco.mix_d128(), 128, 0, 5000),
+                                    (S.mix_d128(), sco.mix_d128(), 128, 123456789, 700),
+                                    (S.mix_d960(), sco.mix_d960(), 960, 5, 200)):
+        g = S.gen_rows(ctx, mix, S.STREAM_BASE, row0, n, dim).numpy()
+        o = sco.gen(omix, sco.STREAM_BASE, row0, n, dim)
+        assert np.array_equal(_bits(g), _bits(o))
+
+
+# ------------------------------------------------------------------ encode
+def _codebooks(rng, x, mc, kc, mp, kp, ties=True):
+    n, dim = x.shape
+    bsc, bsp = dim // mc, dim // mp
+    cq = np.empty((mc, kc, bsc), np.float32)
+    for j in range(mc):
+        cq[j] = x[rng.choice(n, kc, replace=False), j * bsc:(j + 1) * bsc]
+        cq[j] += rng.uniform(-0.5, 0.5, (kc, bsc)).astype(np.float32)
+        if ties:
+            cq[j, kc - 1] = cq[j, 1]
+            cq[j, kc - 2] = cq[j, 0]
+    pq = (rng.standard_normal((mp, kp, bsp)) * 8).astype(np.float32)
+    if ties:
+        pq[:, kp - 1] = pq[:, 3]
+    return cq, pq
+
+
+ENC_CASES = [  # dim, mc, kc, mp, kp, nc, n
+    (128, 1, 256, 8, 256, 2, 3001),
+    (128, 2, 64, 8, 256, 1, 2500),
+    (128, 2, 32, 8, 256, 2, 2500),
+    (128, 1, 1, 8, 256, 1, 777),
+    (960, 2, 32, 16, 256, 1, 600),
+    (64, 1, 300, 4, 100, 2, 1000),
+]
+
+
+@pytest.mark.parametrize("case", ENC_CASES)
+def test_encode_exact_matches_oracle(ctx, case):
+    dim, mc, kc, mp, kp, nc, n = case
+    rng = np.random.default_rng(sum(case))
+    mix = sco.mix_d960() if dim == 960 else sco.mix_d128(ncomp=7)
+    x = sco.gen(mix, sco.STREAM_TRAIN, 0, n, dim)
+    if dim != 960:
+        x[n // 2:] += rng.uniform(-0.5, 0.5, (n - n // 2, dim)).astype(np.float32)
+    cq, pq = _codebooks(rng, x, mc, kc, mp, kp, ties=kc > 2)
+    bsc = dim // mc
+    x[:5, :bsc] = cq[0, 1]                        # exact ties on the coarse distance
+    cid, codes, _ = S.encode(ctx, S.Dev.from_numpy(ctx, x), S.Dev.from_numpy(ctx, cq),
+                             S.Dev.from_numpy(ctx, pq), mc, kc, mp, kp, nc)
+    ocid, ocodes = sco.encode(x, cq, pq, mc, kc, mp, kp, nc)
+    assert np.array_equal(cid.numpy(), ocid)
+    assert np.array_equal(codes.numpy(), ocodes)
+
+
+# ------------------------------------------------------------------ index + search
+def _index_pair(ctx, kind, n=6000, dim=64, kc=16, mp=8, kp=64, seed=3):
+    rng = np.random.default_rng(seed)
+    mc = 2 if kind == S.KIND_IMI else 1
+    nc = 2 if kind == S.KIND_SC else 1
+    x = sco.gen(sco.mix_d128(ncomp=11), sco.STREAM_BASE, 0, n, dim)
+    cq, pq = _codebooks(rng, x, mc, kc, mp, kp, ties=False)
+    dx, dcq, dpq = (S.Dev.from_numpy(ctx, a) for a in (x, cq, pq))
+    cid, codes, _ = S.encode(ctx, dx, dcq, dpq, mc, kc, mp, kp, nc)
+    ix = S.Index.build(ctx, cid, codes, dim, dcq, mc, kc, dpq, mp, kp, nc)
+    ocid, ocodes = sco.encode(x, cq, pq, mc, kc, mp, kp, nc)
+    ncells = kc ** (mc * nc)
+    L, pid, cs = sco.build_dense(sco.cell_keys(ocid, mc, kc, nc), ncells, ocodes)
+    oix = sco.OracleIndex(dim, mc, kc, mp, kp, cq, pq, L, pid, cs)
+    return ix, oix
+
+
+@pytest.mark.parametrize("kind", [S.KIND_IVFADC, S.KIND_IMI, S.KIND_SC])
+def test_index_build_and_tables_match_oracle(ctx, kind):
+    ix, oix = _index_pair(ctx, kind)
+    e = ix.export()
+    assert np.array_equal(e["L"], oix.L)
+    assert np.array_equal(e["pid"], oix.pid)
+    assert np.array_equal(e["codes"], oix.codes)
+    assert np.array_equal(_bits(e["norm_c"]), _bits(oix.norm_c))
+    assert np.array_equal(_bits(e["norm_r"]), _bits(oix.norm_r))
+    assert np.array_equal(_bits(e["dot_cr"]), _bits(oix.dot_cr))
+    ix.close()
+
+
+SEARCH_CASES = [  # kind, R, w, T
+    (S.KIND_IVFADC, 100, 16, 3000),
+    (S.KIND_IVFADC, 10, 3, 100000),
+    (S.KIND_IMI, 100, 200, 2000),
+    (S.KIND_IMI, 50, 5, 100000),         # heap capacity 5: inserts dropped
+    (S.KIND_SC, 100, 500, 3000),
+    (S.KIND_SC, 100, 4, 100000),         # cell budget binds
+    (S.KIND_SC, 300, 500, 40),           # fewer candidates than R
+    (S.KIND_SC, 7, 500, 0),              # T = 0: one cell
+]
+
+
+def _compare_search(ctx, ix, oix, kind, R, w, T, nq=150):
+    q = sco.gen(sco.mix_d128(ncomp=11), sco.STREAM_QUERY, 0, nq, ix.dim)
+    ids, dists, sums, st = ix.search(S.Dev.from_numpy(ctx, q), kind, R, w, T)
+    okind = {S.KIND_IVFADC: sco.IVFADC, S.KIND_IMI: sco.IMI, S.KIND_SC: sco.SC}[kind]
+    oids, od, osum = oix.search_batch(okind, q, R, w, T)
+    assert np.array_equal(sums.numpy(), osum)
+    assert np.array_equal(ids.numpy(), oids)
+    assert np.array_equal(_bits(dists.numpy()), _bits(od))
+    assert st.queries == nq and st.candidates == int(osum.sum())
+    return st
+
+
+@pytest.mark.parametrize("case", SEARCH_CASES)
+def test_search_matches_oracle(ctx, case):
+    kind, R, w, T = case
+    ix, oix = _index_pair(ctx, kind)
+    _compare_search(ctx, ix, oix, kind, R, w, T)
+    ix.close()
+
+
+def test_search_heap_fallback_path(ctx, monkeypatch):
+    """A 2-entry on-chip heap forces every multi-sequence query onto the
+    global-memory heap of capacity w; results must not change."""
+    for kind in (S.KIND_IMI, S.KIND_SC):
+        ix, oix = _index_pair(ctx, kind)
+        monkeypatch.setenv("SC_HEAP_ONCHIP", "2")
+        st = _compare_search(ctx, ix, oix, kind, 50, 300, 2500, nq=60)
+        assert st.heap_fallback > 0
+        monkeypatch.delenv("SC_HEAP_ONCHIP")
+        ix.close()
+
+
+def test_search_exhausted_grid(ctx):
+    ix, oix = _index_pair(ctx, S.KIND_SC, n=300, kc=4)
+    st = _compare_search(ctx, ix, oix, S.KIND_SC, 400, 1000, 10 ** 6, nq=20)
+    assert st.exhausted == 20
+    ix.close()
+
+
+def test_search_kind_mismatch_errors(ctx):
+    ix, _ = _index_pair(ctx, S.KIND_SC, n=500)
+    q = S.Dev.from_numpy(ctx, np.zeros((2, 64), np.float32))
+    with pytest.raises(S.ScError) as e:
+        ix.search(q, S.KIND_IMI, 10, 10, 10)
+    assert e.value.rc == -3
+    ix.close()
+    ix, _ = _index_pair(ctx, S.KIND_IVFADC, n=500)
+    with pytest.raises(S.ScError) as e:
+        ix.search(q, S.KIND_IVFADC, 10, ix.kc + 1, 10)
+    assert e.value.rc == -7
+    ix.close()
+
+
+def test_index_load_roundtrip(ctx):
+    ix, oix = _index_pair(ctx, S.KIND_SC)
+    ix2 = S.Index.load(ctx, oix.L, oix.pid, oix.codes, 64, oix.cq, 1, oix.kc, oix.pq, oix.mp,
+                       oix.kp, 2)
+    _compare_search(ctx, ix2, oix, S.KIND_SC, 100, 500, 3000, nq=40)
+    ix.close()
+    ix2.close()
+
+
+# ------------------------------------------------------------------ k-means
+@pytest.mark.parametrize("lagged", [0, 1])
+def test_kmeans_matches_oracle_on_integer_data(ctx, lagged):
+    x = sco.gen(sco.mix_d128(ncomp=9), sco.STREAM_TRAIN, 0, 6000, 16)
+    c, lab = S.kmeans(ctx, S.Dev.from_numpy(ctx, x), 40, 6, seed=11, lagged=lagged)
+    oc, olab = sco.kmeans(x, 40, 6, seed=11, lagged=lagged)
+    assert np.array_equal(lab.numpy(), olab)
+    assert np.array_equal(_bits(c.numpy()), _bits(oc))     # integer data: exact FP64 sums
+
+
+def test_kmeans_empty_cluster_repair(ctx):
+    base = sco.gen(sco.mix_d128(ncomp=5), sco.STREAM_TRAIN, 0, 12, 8)
+    x = np.repeat(base, 150, axis=0)                       # 12 distinct points, k = 16
+    c, lab = S.kmeans(ctx, S.Dev.from_numpy(ctx, x), 16, 4, seed=5)
+    oc, olab = sco.kmeans(x, 16, 4, seed=5)
+    assert np.array_equal(lab.numpy(), olab)
+    assert np.array_equal(_bits(c.numpy()), _bits(oc))
+
+
+def test_kmeans_fp64_error_on_fractional_data(ctx):
+    rng = np.random.default_rng(4)
+    x = sco.gen(sco.mix_d128(ncomp=9), sco.STREAM_TRAIN, 0, 5000, 16)
+    x = x + rng.uniform(-0.5, 0.5, x.shape).astype(np.float32) / 3.0
+    c, lab = S.kmeans(ctx, S.Dev.from_numpy(ctx, x), 32, 3, seed=2)
+    oc, olab = sco.kmeans(x, 32, 3, seed=2)
+    # atomic FP64 sums may differ from the ordered oracle in the last bits
+    rel = np.abs(c.numpy().astype(np.float64) - oc) / np.maximum(1.0, np.abs(oc))
+    assert rel.max() <= 4e-7
+    assert np.mean(lab.numpy() == olab) >= 0.999
+
+
+@pytest.mark.parametrize("which", [0, 1])
+def test_residual_matches_oracle(ctx, which):
+    x = sco.gen(sco.mix_d128(ncomp=6), sco.STREAM_TRAIN, 0, 3000, 32)
+    rng = np.random.default_rng(which)
+    cq = x[rng.choice(3000, 2 * 20, replace=False)].reshape(2, 20, 32)[:, :, :16].copy()
+    d = S.Dev.from_numpy(ctx, x)
+    S.residual(ctx, d, S.Dev.from_numpy(ctx, cq), 2, 20, which)
+    o = sco.residual(x, cq, 2, 20, which)
+    assert np.array_equal(_bits(d.numpy()), _bits(o))
+
+
+# ------------------------------------------------------------------ full size
+def test_full_size_sc_index_properties(ctx):
+    """1M x 128, SC kc=256 (the dense-partition benchmark shape): exact encode
+    on sampled rows, cell invariants, and search on 64 queries vs the oracle."""
+    n, dim, kc, mp, kp = 1_000_000, 128, 256, 8, 256
+    mix = S.mix_d128()
+    xt = S.gen_rows(ctx, mix, S.STREAM_TRAIN, 0, 50_000, dim)
+    cq, _ = S.kmeans(ctx, xt, kc, 4, seed=1, want_labels=False)
+    res = S.gen_rows(ctx, mix, S.STREAM_TRAIN, 0, 50_000, dim)
+    S.residual(ctx, res, cq, 1, kc, 0)
+    rh = res.numpy()
+    pq = np.stack([S.kmeans(ctx, S.Dev.from_numpy(ctx, np.ascontiguousarray(rh[:, s * 16:(s + 1) * 16])),
+                            kp, 4, seed=2 + s, want_labels=False)[0].numpy() for s in range(mp)])
+    dpq = S.Dev.from_numpy(ctx, pq)
+    x = S.gen_rows(ctx, mix, S.STREAM_BASE, 0, n, dim)
+    cid, codes, _ = S.encode(ctx, x, cq, dpq, 1, kc, mp, kp, 2)
+    cidh, codesh = cid.numpy(), codes.numpy()
+    rows = np.random.default_rng(0).choice(n, 3000, replace=False)
+    xr =np.concatenate([sco.gen(sco.mix_d128(), sco.STREAM_BASE, int(r), 1, dim) for r in rows])
+    ocid, ocodes = sco.encode(xr, cq.numpy(), pq, 1, kc, mp, kp, 2)
+    assert np.array_equal(cidh[rows], ocid) and np.array_equal(codesh[rows], ocodes)
+    ix = S.Index.build(ctx, cid, codes, dim, cq, 1, kc, dpq, mp, kp, 2)
+    e = ix.export()
+    keys = sco.cell_keys(cidh, 1, kc, 2)
+    assert e["L"][-1] == n
+    starts = np.concatenate([[0], e["L"][:-1]])
+    assert np.all(np.diff(e["pid"])[np.diff(keys[e["pid"]]) == 0] > 0)   # ids ascend in a cell
+    assert np.array_equal(e["codes"], codesh[e["pid"]])
+    assert np.all(keys[e["pid"]][starts[e["L"] > starts]] == np.nonzero(e["L"] > starts)[0])
+    oix = sco.OracleIndex(dim, 1, kc, mp, kp, cq.numpy(), pq, e["L"], e["pid"], e["codes"])
+    q = sco.gen(sco.mix_d128(), sco.STREAM_QUERY, 0, 64, dim)
+    ids, dists, sums, st = ix.search(S.Dev.from_numpy(ctx, q), S.KIND_SC, 100, 100000, 10000)
+    oids, od, osum = oix.search_batch(sco.SC, q, 100, 100000, 10000)
+    assert np.array_equal(sums.numpy(), osum)
+    assert np.array_equal(ids.numpy(), oids)
+    assert np.array_equal(_bits(dists.numpy()), _bits(od))
+    ix.close()
