@@ -126,12 +126,3 @@ extern "C" int sc_encode(sc_ctx *ctx, const float *d_x, int64_t n, int32_t dim,
     if (nc == 2) return encode_exact<2>(ctx, d_x, n, dim, d_cq, mc, kc, d_pq, mp, kp, d_cid, d_codes);
     return encode_exact<1>(ctx, d_x, n, dim, d_cq, mc, kc, d_pq, mp, kp, d_cid, d_codes);
 }
-
-int sc_encode_bf16(sc_ctx *ctx, const float *x, int64_t n, int dim, const float *cq, int mc,
-                   int kc, const float *pq, int mp, int kp, int nc, uint16_t *cid,
-                   uint8_t *codes, sc_encode_stats *st) {
-    (void)ctx; (void)x; (void)n; (void)dim; (void)cq; (void)mc; (void)kc; (void)pq;
-    (void)mp; (void)kp; (void)nc; (void)cid; (void)codes; (void)st;
-    sc_set_error("sc_encode: SC_PREC_BF16 is not available in this build");
-    return SC_ERR_ARG;
-}

@@ -1,0 +1,355 @@
+// sc_encode_mfma.hip -- mixed-precision assignment (SC_PREC_BF16) with exact
+// results.
+//
+// For every (vector, codeword) pair an approximate distance
+//     d~ = ||x'||^2 + ||c'||^2 - 2 x'.c'
+// is formed from a BF16 MFMA dot product with FP32 accumulation, where
+// x' = x - m and c' = c - m are shifted by a per-block integer mean m (the
+// distance is shift-invariant; the shift shrinks the norms that scale the
+// error) and every operand is split into two BF16 terms (v = hi + lo,
+// |v - hi - lo| <= 2^-16 |v|).  For a row with X = ||x'|| and C = max_k ||c'_k||
+// the error obeys |d~ - d| <= E with
+//     E = 2 * [ 2 X C ((3 * 2^-16 + gamma_n)(1 + 2^-7)) + 2^-22 (X + C)^2 ] + 1e-6,
+//     gamma_n = n u / (1 - n u),  u = 2^-23,  n = 3 * blen + 8
+// (Cauchy-Schwarz bounds sum |x'_i c'_i| by X C; the outer factor 2 covers the
+// FP32 storage of d~, the shift rounding and the FP64 rounding of the exact
+// distance).  The exact top-2 (or top-1) under the reference's order is then
+// contained in {k : d~_k <= d~_(2) + 2E}; those candidates are re-evaluated
+// with the exact S1 arithmetic and ordered with the reference's tie rule, so
+// codes and cell ids equal the FP32 form bit for bit.  A row with more than
+// kCand candidates is evaluated exactly over all codewords.
+//
+// Operand layout of v_mfma_f32_16x16x16bf16_1k (one wave): lane l holds
+// A[l % 16][4 (l / 16) + j] and B[4 (l / 16) + j][l % 16] for j < 4, and
+// receives C[4 (l / 16) + i][l % 16] for i < 4.
+#include <float.h>
+
+#include "sc_internal.h"
+
+namespace {
+
+typedef short s16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kRows = 16;     // vectors per wave
+constexpr int kCand = 16;     // exact re-check slots per row
+
+__device__ __forceinline__ unsigned short bf16_bits(float v) {
+    return __builtin_bit_cast(unsigned short, (__bf16)v);
+}
+__device__ __forceinline__ float bf16_val(unsigned short b) {
+    return (float)__builtin_bit_cast(__bf16, b);
+}
+
+// Per-block codebook preparation: shifted, split BF16 codewords, FP64 norms
+// of the shifted codewords and their maximum per block.
+// mean[b][t] = round(mean_k c[b][k][t]) when `shift`, else 0.
+__global__ void k_cb_mean(const float *__restrict__ cb, int nb, int K, int blen, int shift,
+                          float *__restrict__ mean) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nb * blen) return;
+    int b = e / blen, t = e % blen;
+    double s = 0.0;
+    for (int k = 0; k < K; ++k) s += cb[((size_t)b * K + k) * blen + t];
+    mean[e] = shift ? (float)rint(s / K) : 0.0f;
+}
+
+__global__ void k_cb_split(const float *__restrict__ cb, int nb, int K, int blen,
+                           const float *__restrict__ mean, unsigned short *__restrict__ hi,
+                           unsigned short *__restrict__ lo, double *__restrict__ cn2,
+                           unsigned long long *__restrict__ cmax_bits) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nb * K) return;
+    int b = e / K;
+    const float *c = cb + (size_t)e * blen;
+    double s = 0.0;
+    for (int t = 0; t < blen; ++t) {
+        float v = c[t] - mean[(size_t)b * blen + t];
+        unsigned short h = bf16_bits(v);
+        unsigned short l = bf16_bits(v - bf16_val(h));
+        hi[(size_t)e * blen + t] = h;
+        lo[(size_t)e * blen + t] = l;
+        s += (double)v * (double)v;
+    }
+    cn2[e] = s;
+    // max of a non-negative double via its bit pattern
+    atomicMax(&cmax_bits[b], (unsigned long long)__builtin_bit_cast(unsigned long long, sqrt(s)));
+}
+
+__device__ __forceinline__ double s1(const float *a, const float *b, int d) {
+    double acc = 0.0;
+    for (int t = 0; t < d; ++t) {
+        const double dd = (double)(a[t] - b[t]);
+        acc = __fma_rn(dd, dd, acc);
+    }
+    return acc;
+}
+
+struct CbView {
+    const float *c;              // [K][blen] original codewords
+    const unsigned short *hi, *lo;
+    const double *cn2;           // [K]
+    double cmax;
+    const float *mean;           // [blen]
+    int K, blen;
+};
+
+// (d, k) precedes (e, j): dist ASC, then index DESC when DESC else ASC.
+template <bool DESC>
+__device__ __forceinline__ bool precedes(double d, int k, double e, int j) {
+    if (d < e) return true;
+    if (d > e) return false;
+    return DESC ? (k > j) : (k < j);
+}
+
+// One wave: best one (TOP2=false) or two (TOP2=true) codewords for the 16
+// rows held in xs (row stride `ld`, columns [t0, t0 + blen)).  Writes
+// out1/out2 [16].  Wave-private LDS scratch: cnt [16], ck [16][kCand],
+// cd [16][kCand], rown [16] (FP64 norms of shifted rows).
+template <bool TOP2, bool DESC>
+__device__ void wave_best(const float *xs, int ld, int t0, int nrows, const CbView cb,
+                          int *cnt, int *ck, double *cd, double *rown, int *out1, int *out2,
+                          unsigned long long *st_rechecked, unsigned long long *st_fallback) {
+    const int l = threadIdx.x & 63;
+    const int blen = cb.blen, K = cb.K;
+    // 1. shifted row norms (FP64) and exactness of the BF16 representation
+    bool exact_local = true;
+    if (l < kRows) {
+        double s = 0.0;
+        for (int t = 0; t < blen; ++t) {
+            float v = xs[l * ld + t0 + t] - cb.mean[t];
+            s += (double)v * (double)v;
+            exact_local &= (bf16_val(bf16_bits(v)) == v);
+        }
+        rown[l] = s;
+        cnt[l] = 0;
+    }
+    const bool x_exact = __all(exact_local);
+    __syncthreads();
+    const int r4 = 4 * (l >> 4);          // first of this lane's four result rows
+    const int col = l & 15;
+    double E[4];
+    const double u = 1.0 / 8388608.0;     // 2^-23
+    const double n = 3.0 * blen + 8.0;
+    const double gam = n * u / (1.0 - n * u);
+    for (int i = 0; i < 4; ++i) {
+        const double X = sqrt(rown[r4 + i]), C = cb.cmax;
+        E[i] = 2.0 * (2.0 * X * C * ((3.0 / 65536.0 + gam) * (1.0 + 1.0 / 128.0)) +
+                      (X + C) * (X + C) / 4194304.0) + 1e-6;
+    }
+    // 2. two passes over the codewords: thresholds, then candidates
+    float m1[4], m2[4];
+    for (int i = 0; i < 4; ++i) { m1[i] = INFINITY; m2[i] = INFINITY; }
+    float thr[4];
+    for (int pass = 0; pass < 2; ++pass) {
+        for (int k0 = 0; k0 < K; k0 += 16) {
+            f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+            const int kb = min(k0 + col, K - 1);     // codeword for this lane's B column
+            for (int kk = 0; kk < blen; kk += 16) {
+                s16x4 ah, al, bh, bl;
+                const int ar = l & 15;                // A row
+                for (int j = 0; j < 4; ++j) {
+                    const int t = kk + r4 + j;
+                    float v = xs[ar * ld + t0 + t] - cb.mean[t];
+                    unsigned short h = bf16_bits(v);
+                    ah[j] = (short)h;
+                    al[j] = (short)bf16_bits(v - bf16_val(h));
+                    bh[j] = (short)cb.hi[(size_t)kb * blen + t];
+                    bl[j] = (short)cb.lo[(size_t)kb * blen + t];
+                }
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl, acc, 0, 0, 0);
+                if (!x_exact) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh, acc, 0, 0, 0);
+            }
+            const int k = k0 + col;
+            for (int i = 0; i < 4; ++i) {
+                if (k >= K) continue;
+                const float d = (float)(rown[r4 + i] + cb.cn2[k] - 2.0 * (double)acc[i]);
+                if (pass == 0) {
+                    if (d < m1[i]) { m2[i] = m1[i]; m1[i] = d; }
+                    else if (d < m2[i]) m2[i] = d;
+                } else if ((double)d <= (double)thr[i] && r4 + i < nrows) {
+                    int s = atomicAdd(&cnt[r4 + i], 1);
+                    if (s < kCand) ck[(r4 + i) * kCand + s] = k;
+                }
+            }
+        }
+        if (pass == 0) {
+            for (int i = 0; i < 4; ++i) {
+                for (int o = 1; o < 16; o <<= 1) {
+                    float a1 = __shfl_xor(m1[i], o), a2 = __shfl_xor(m2[i], o);
+                    float lo1 = fminf(m1[i], a1);
+                    float hi1 = fmaxf(m1[i], a1);
+                    m2[i] = fminf(hi1, fminf(m2[i], a2));
+                    m1[i] = lo1;
+                }
+                thr[i] = (float)((double)(TOP2 ? m2[i] : m1[i]) + 2.0 * E[i]);
+                // rounding the threshold up keeps it an upper bound
+                thr[i] = nextafterf(thr[i], INFINITY);
+            }
+        }
+    }
+    __syncthreads();
+    // 3. exact re-check of the candidates
+    for (int slot = l; slot < kRows * kCand; slot += 64) {
+        const int r = slot / kCand, s = slot % kCand;
+        if (r < nrows && s < min(cnt[r], kCand)) {
+            const int k = ck[r * kCand + s];
+            cd[slot] = s1(xs + r * ld + t0, cb.c + (size_t)k * blen, blen);
+        }
+    }
+    __syncthreads();
+    if (l < nrows) {
+        double d1 = INFINITY, d2 = INFINITY;
+        int k1 = -1, k2 = -1;
+        const int c = cnt[l];
+        auto push = [&](double d, int k) {
+            if (precedes<DESC>(d, k, d1, k1)) { d2 = d1; k2 = k1; d1 = d; k1 = k; }
+            else if (precedes<DESC>(d, k, d2, k2)) { d2 = d; k2 = k; }
+        };
+        if (c <= kCand) {
+            for (int s = 0; s < c; ++s) push(cd[l * kCand + s], ck[l * kCand + s]);
+            atomicAdd(st_rechecked, (unsigned long long)c);
+        } else {
+            for (int k = 0; k < K; ++k) push(s1(xs + l * ld + t0, cb.c + (size_t)k * blen, blen), k);
+            atomicAdd(st_fallback, 1ull);
+        }
+        out1[l] = k1;
+        if (out2) out2[l] = k2;
+    }
+    __syncthreads();
+}
+
+template <int NC>
+__global__ __launch_bounds__(64) void k_encode_mfma(
+    const float *__restrict__ x, int64_t n, int dim, const float *__restrict__ cq, int mc,
+    int kc, const float *__restrict__ pq, int mp, int kp, const unsigned short *__restrict__ chi,
+    const unsigned short *__restrict__ clo, const double *__restrict__ cn2,
+    const unsigned long long *__restrict__ cmax, const float *__restrict__ cmean,
+    const unsigned short *__restrict__ phi, const unsigned short *__restrict__ plo,
+    const double *__restrict__ pn2, const unsigned long long *__restrict__ pmax,
+    const float *__restrict__ pmean, uint16_t *__restrict__ cid, uint8_t *__restrict__ codes,
+    unsigned long long *__restrict__ stats) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int ld = dim + 1;
+    float *xs = reinterpret_cast<float *>(smem);                          // [16][dim+1]
+    double *cd = reinterpret_cast<double *>(xs + kRows * ld + (kRows * ld) % 2);
+    double *rown = cd + kRows * kCand;
+    int *cnt = reinterpret_cast<int *>(rown + kRows);
+    int *ck = cnt + kRows;
+    int *o1 = ck + kRows * kCand;
+    int *o2 = o1 + kRows;
+    const int l = threadIdx.x;
+    const int64_t v0 = (int64_t)blockIdx.x * kRows;
+    const int nr = (int)min((int64_t)kRows, n - v0);
+    for (int e = l; e < kRows * dim; e += 64) {
+        int r = e / dim, t = e % dim;
+        xs[r * ld + t] = r < nr ? x[(size_t)(v0 + r) * dim + t] : 0.0f;
+    }
+    __syncthreads();
+    const int bsc = dim / mc, bsp = dim / mp;
+    for (int j = 0; j < mc; ++j) {
+        CbView cv{cq + (size_t)j * kc * bsc, chi + (size_t)j * kc * bsc,
+                  clo + (size_t)j * kc * bsc, cn2 + (size_t)j * kc,
+                  __builtin_bit_cast(double, cmax[j]), cmean + (size_t)j * bsc, kc, bsc};
+        wave_best<NC == 2, NC == 2>(xs, ld, j * bsc, nr, cv, cnt, ck, cd, rown, o1,
+                                    NC == 2 ? o2 : nullptr, &stats[0], &stats[1]);
+        if (l < nr) {
+            if (NC == 2) {
+                cid[(size_t)(v0 + l) * 2 * mc + j] = (uint16_t)o1[l];
+                cid[(size_t)(v0 + l) * 2 * mc + mc + j] = (uint16_t)o2[l];
+            } else {
+                cid[(size_t)(v0 + l) * mc + j] = (uint16_t)o1[l];
+            }
+        }
+        __syncthreads();
+        for (int e = l; e < nr * bsc; e += 64) {   // o1 is written for real rows only
+            int r = e / bsc, t = e % bsc;
+            xs[r * ld + j * bsc + t] -= cv.c[(size_t)max(o1[r], 0) * bsc + t];
+        }
+        __syncthreads();
+    }
+    for (int s = 0; s < mp; ++s) {
+        CbView pv{pq + (size_t)s * kp * bsp, phi + (size_t)s * kp * bsp,
+                  plo + (size_t)s * kp * bsp, pn2 + (size_t)s * kp,
+                  __builtin_bit_cast(double, pmax[s]), pmean + (size_t)s * bsp, kp, bsp};
+        wave_best<false, false>(xs, ld, s * bsp, nr, pv, cnt, ck, cd, rown, o1, nullptr,
+                                &stats[0], &stats[1]);
+        if (l < nr) codes[(size_t)(v0 + l) * mp + s] = (uint8_t)o1[l];
+        __syncthreads();
+    }
+}
+
+int prep_codebook(sc_ctx *ctx, const float *cb, int nb, int K, int blen, int shift,
+                  unsigned char *base, size_t &off, float **mean, unsigned short **hi,
+                  unsigned short **lo, double **cn2, unsigned long long **cmax) {
+    auto take = [&](size_t bytes) { unsigned char *p = base + off; off = (off + bytes + 255) & ~(size_t)255; return p; };
+    *mean = (float *)take(sizeof(float) * nb * blen);
+    *hi = (unsigned short *)take(2ull * nb * K * blen);
+    *lo = (unsigned short *)take(2ull * nb * K * blen);
+    *cn2 = (double *)take(8ull * nb * K);
+    *cmax = (unsigned long long *)take(8ull * nb);
+    if (!base) return SC_OK;   // sizing pass
+    SC_HIP(hipMemsetAsync(*cmax, 0, 8ull * nb, ctx->stream));
+    hipLaunchKernelGGL(k_cb_mean, dim3((unsigned)sc_cdiv((int64_t)nb * blen, 256)), dim3(256), 0,
+                       ctx->stream, cb, nb, K, blen, shift, *mean);
+    hipLaunchKernelGGL(k_cb_split, dim3((unsigned)sc_cdiv((int64_t)nb * K, 256)), dim3(256), 0,
+                       ctx->stream, cb, nb, K, blen, *mean, *hi, *lo, *cn2, *cmax);
+    SC_CHECK_LAUNCH();
+    return SC_OK;
+}
+
+}  // namespace
+
+int sc_encode_bf16(sc_ctx *ctx, const float *x, int64_t n, int dim, const float *cq, int mc,
+                   int kc, const float *pq, int mp, int kp, int nc, uint16_t *cid,
+                   uint8_t *codes, sc_encode_stats *st) {
+    const int bsc = dim / mc, bsp = dim / mp;
+    if (bsc % 16 || bsp % 16 || dim > 2048) {
+        sc_set_error("sc_encode: SC_PREC_BF16 needs block lengths that are multiples of 16 "
+                     "(bsc=%d bsp=%d) and dim <= 2048", bsc, bsp);
+        return SC_ERR_ARG;
+    }
+    float *cmean, *pmean;
+    unsigned short *chi, *clo, *phi, *plo;
+    double *cn2, *pn2;
+    unsigned long long *cmax, *pmax, *stats;
+    size_t off = 0;
+    prep_codebook(ctx, cq, mc, kc, bsc, 1, nullptr, off, &cmean, &chi, &clo, &cn2, &cmax);
+    prep_codebook(ctx, pq, mp, kp, bsp, 0, nullptr, off, &pmean, &phi, &plo, &pn2, &pmax);
+    const size_t total = off + 256;
+    unsigned char *s;
+    SC_TRY(sc_scratch(ctx, total, (void **)&s));
+    off = 0;
+    SC_TRY(prep_codebook(ctx, cq, mc, kc, bsc, 1, s, off, &cmean, &chi, &clo, &cn2, &cmax));
+    SC_TRY(prep_codebook(ctx, pq, mp, kp, bsp, 0, s, off, &pmean, &phi, &plo, &pn2, &pmax));
+    stats = (unsigned long long *)(s + off);
+    SC_HIP(hipMemsetAsync(stats, 0, 16, ctx->stream));
+    const int ld = dim + 1;
+    const size_t lds = sizeof(float) * (kRows * ld + (kRows * ld) % 2) +
+                       sizeof(double) * (kRows * kCand + kRows) +
+                       sizeof(int) * (kRows + kRows * kCand + 2 * kRows);
+    const unsigned blocks = (unsigned)sc_cdiv(n, kRows);
+    if (nc == 2) {
+        SC_HIP(hipFuncSetAttribute((const void *)k_encode_mfma<2>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(k_encode_mfma<2>, dim3(blocks), dim3(64), lds, ctx->stream, x, n, dim,
+                           cq, mc, kc, pq, mp, kp, chi, clo, cn2, cmax, cmean, phi, plo, pn2,
+                           pmax, pmean, cid, codes, stats);
+    } else {
+        SC_HIP(hipFuncSetAttribute((const void *)k_encode_mfma<1>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(k_encode_mfma<1>, dim3(blocks), dim3(64), lds, ctx->stream, x, n, dim,
+                           cq, mc, kc, pq, mp, kp, chi, clo, cn2, cmax, cmean, phi, plo, pn2,
+                           pmax, pmean, cid, codes, stats);
+    }
+    SC_CHECK_LAUNCH();
+    if (st) {
+        unsigned long long h[2];
+        SC_HIP(hipMemcpyAsync(h, stats, 16, hipMemcpyDeviceToHost, ctx->stream));
+        SC_HIP(hipStreamSynchronize(ctx->stream));
+        st->rechecked = (int64_t)h[0];
+        st->fallback = (int64_t)h[1];
+    }
+    return SC_OK;
+}

@@ -325,6 +325,7 @@ struct ScanArgs {
     int32_t *ids;
     float *dists;
     int32_t *sums;
+    const int32_t *qlist;   // rescan: workgroup b handles query qlist[b]
 };
 
 struct ScanShared {
@@ -436,7 +437,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_topk(ScanArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     ScanShared &S = *reinterpret_cast<ScanShared *>(smem);
     float *lut = reinterpret_cast<float *>(smem + sizeof(ScanShared));   // [mp*kp]
-    const int64_t q = blockIdx.x;
+    const int64_t q = a.qlist ? a.qlist[blockIdx.x] : blockIdx.x;
     const int tid = threadIdx.x;
     const int ncell = a.qinfo[q * 8 + 0];
     const int sum = a.qinfo[q * 8 + 1];
@@ -729,10 +730,23 @@ extern "C" int sc_search(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
                                ctx->stream, ta, hcap, qpw);
             SC_CHECK_LAUNCH();
         }
-        // queries whose heap outgrew the on-chip capacity run again with a
-        // global-memory heap of the reference's capacity w
-        int32_t nfb = 0;
+        mark();
+        ScanArgs sa{kind, ix->dim, ix->mc, kc, ix->mp, ix->kp, p->R, P.ccap, qs, dqc, dqr,
+                    ix->dot_cr, ix->L, ix->pid, ix->codes, cells, cinfo, qinfo,
+                    d_ids + (size_t)q0 * p->R, d_dists + (size_t)q0 * p->R,
+                    d_sums ? d_sums + q0 : nullptr, nullptr};
+        const size_t lds = sizeof(ScanShared) + sizeof(float) * ix->mp * ix->kp;
+        if (hcap > 0) {
+            hipLaunchKernelGGL(k_scan_topk, dim3((unsigned)cn), dim3(kScanThreads), lds,
+                               ctx->stream, sa);
+            SC_CHECK_LAUNCH();
+        }
+        mark();
+        // Queries whose heap outgrew the on-chip capacity are traversed again
+        // with a global-memory heap of the reference's capacity and scanned
+        // again; the first scan's output for them is overwritten.
         if (hcap < hcap_ref) {
+            int32_t nfb = 0;
             int32_t *dn = list + P.chunk;
             if (hcap > 0) {
                 SC_HIP(hipMemsetAsync(dn, 0, 4, ctx->stream));
@@ -760,26 +774,22 @@ extern "C" int sc_search(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
                 hipLaunchKernelGGL(k_traverse, dim3((unsigned)sc_cdiv(nfb, 64)), dim3(64), 0,
                                    ctx->stream, fa, hcap_ref, 64);
                 hipError_t le = hipGetLastError();
+                ScanArgs fs = sa;
+                fs.qlist = hcap > 0 ? list : nullptr;
+                if (le == hipSuccess) {
+                    hipLaunchKernelGGL(k_scan_topk, dim3((unsigned)nfb), dim3(kScanThreads), lds,
+                                       ctx->stream, fs);
+                    le = hipGetLastError();
+                }
                 (void)hipStreamSynchronize(ctx->stream);
                 (void)hipFree(g);
                 if (le != hipSuccess) {
-                    sc_set_error("sc_search: fallback traversal launch failed: %s",
-                                 hipGetErrorString(le));
+                    sc_set_error("sc_search: fallback launch failed: %s", hipGetErrorString(le));
                     return SC_ERR_HIP;
                 }
                 fallback_total += nfb;
             }
         }
-        mark();
-        ScanArgs sa{kind, ix->dim, ix->mc, kc, ix->mp, ix->kp, p->R, P.ccap, qs, dqc, dqr,
-                    ix->dot_cr, ix->L, ix->pid, ix->codes, cells, cinfo, qinfo,
-                    d_ids + (size_t)q0 * p->R, d_dists + (size_t)q0 * p->R,
-                    d_sums ? d_sums + q0 : nullptr};
-        size_t lds = sizeof(ScanShared) + sizeof(float) * ix->mp * ix->kp;
-        hipLaunchKernelGGL(k_scan_topk, dim3((unsigned)cn), dim3(kScanThreads), lds, ctx->stream,
-                           sa);
-        SC_CHECK_LAUNCH();
-        mark();
         if (st) {
             hipLaunchKernelGGL(k_reduce_stats, dim3((unsigned)sc_cdiv(cn, 256)), dim3(256), 0,
                                ctx->stream, qinfo, cn, stats);

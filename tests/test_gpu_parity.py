@@ -56,9 +56,12 @@ ENC_CASES = [  # dim, mc, kc, mp, kp, nc, n
 ]
 
 
+@pytest.mark.parametrize("prec", [S.PREC_FP32, S.PREC_BF16])
 @pytest.mark.parametrize("case", ENC_CASES)
-def test_encode_exact_matches_oracle(ctx, case):
+def test_encode_matches_oracle(ctx, case, prec):
     dim, mc, kc, mp, kp, nc, n = case
+    if prec == S.PREC_BF16 and ((dim // mc) % 16 or (dim // mp) % 16):
+        pytest.skip("the BF16 form needs block lengths that are multiples of 16")
     rng = np.random.default_rng(sum(case))
     mix = sco.mix_d960() if dim == 960 else sco.mix_d128(ncomp=7)
     x = sco.gen(mix, sco.STREAM_TRAIN, 0, n, dim)
@@ -67,11 +70,14 @@ def test_encode_exact_matches_oracle(ctx, case):
     cq, pq = _codebooks(rng, x, mc, kc, mp, kp, ties=kc > 2)
     bsc = dim // mc
     x[:5, :bsc] = cq[0, 1]                        # exact ties on the coarse distance
-    cid, codes, _ = S.encode(ctx, S.Dev.from_numpy(ctx, x), S.Dev.from_numpy(ctx, cq),
-                             S.Dev.from_numpy(ctx, pq), mc, kc, mp, kp, nc)
+    cid, codes, st = S.encode(ctx, S.Dev.from_numpy(ctx, x), S.Dev.from_numpy(ctx, cq),
+                              S.Dev.from_numpy(ctx, pq), mc, kc, mp, kp, nc, prec)
     ocid, ocodes = sco.encode(x, cq, pq, mc, kc, mp, kp, nc)
     assert np.array_equal(cid.numpy(), ocid)
     assert np.array_equal(codes.numpy(), ocodes)
+    if prec == S.PREC_BF16:
+        print("bf16 encode: %d candidates re-checked, %d rows fully exact (n=%d)"
+              % (st.rechecked, st.fallback, n))
 
 
 # ------------------------------------------------------------------ index + search
@@ -237,6 +243,11 @@ def test_full_size_sc_index_properties(ctx):
     x = S.gen_rows(ctx, mix, S.STREAM_BASE, 0, n, dim)
     cid, codes, _ = S.encode(ctx, x, cq, dpq, 1, kc, mp, kp, 2)
     cidh, codesh = cid.numpy(), codes.numpy()
+    bcid, bcodes, bst = S.encode(ctx, x, cq, dpq, 1, kc, mp, kp, 2, S.PREC_BF16)
+    assert np.array_equal(bcid.numpy(), cidh)          # BF16 form == FP32 form, all rows
+    assert np.array_equal(bcodes.numpy(), codesh)
+    print("full size bf16: %d re-checked, %d exact rows" % (bst.rechecked, bst.fallback))
+    del bcid, bcodes
     rows = np.random.default_rng(0).choice(n, 3000, replace=False)
     xr =np.concatenate([sco.gen(sco.mix_d128(), sco.STREAM_BASE, int(r), 1, dim) for r in rows])
     ocid, ocodes = sco.encode(xr, cq.numpy(), pq, 1, kc, mp, kp, 2)
