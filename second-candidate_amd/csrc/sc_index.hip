@@ -110,7 +110,9 @@ int alloc_index(sc_ctx *ctx, int64_t n, int32_t dim, int32_t mc, int32_t kc, int
               hipMalloc(&ix->dot_cr, sizeof(float) * (size_t)kc * mp * kp) == hipSuccess &&
               hipMalloc(&ix->L, sizeof(int32_t) * ncells) == hipSuccess &&
               hipMalloc(&ix->pid, sizeof(int32_t) * (n > 0 ? n : 1)) == hipSuccess &&
-              hipMalloc(&ix->codes, (size_t)(n > 0 ? n : 1) * mp) == hipSuccess;
+              hipMalloc(&ix->codes, (size_t)(n > 0 ? n : 1) * mp) == hipSuccess &&
+              hipMalloc(&ix->cqT, sizeof(float) * mc * kc * bsc) == hipSuccess &&
+              hipMalloc(&ix->pqT, sizeof(float) * mp * kp * bsp) == hipSuccess;
     if (!ok) {
         sc_index_destroy(ix);
         sc_set_error("index allocation failed (n=%lld cells=%lld)", (long long)n,
@@ -157,9 +159,25 @@ int finish_index(sc_index *ix) {
 
 }  // namespace
 
+// [nb][K][blen] -> [nb][blen][K]
+__global__ void k_transpose_cb(const float *__restrict__ cb, int nb, int K, int blen,
+                               float *__restrict__ out) {
+    int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)nb * K * blen) return;
+    int64_t b = e / ((int64_t)K * blen);
+    int r = (int)(e - b * K * blen);
+    int k = r / blen, t = r % blen;
+    out[(b * blen + t) * K + k] = cb[e];
+}
+
 int sc_launch_tables(sc_index *ix) {
     sc_ctx *ctx = ix->ctx;
     const int bsc = ix->dim / ix->mc, bsp = ix->dim / ix->mp;
+    hipLaunchKernelGGL(k_transpose_cb, dim3((unsigned)sc_cdiv((int64_t)ix->mc * ix->kc * bsc, 256)),
+                       dim3(256), 0, ctx->stream, ix->cq, ix->mc, ix->kc, bsc, ix->cqT);
+    hipLaunchKernelGGL(k_transpose_cb, dim3((unsigned)sc_cdiv((int64_t)ix->mp * ix->kp * bsp, 256)),
+                       dim3(256), 0, ctx->stream, ix->pq, ix->mp, ix->kp, bsp, ix->pqT);
+    SC_CHECK_LAUNCH();
     int64_t rc = (int64_t)ix->mc * ix->kc, rp = (int64_t)ix->mp * ix->kp;
     hipLaunchKernelGGL(k_norms, dim3((unsigned)sc_cdiv(rc, 256)), dim3(256), 0, ctx->stream,
                        ix->cq, rc, bsc, ix->norm_c);
@@ -281,7 +299,8 @@ extern "C" int sc_index_destroy(sc_index *ix) {
         (void)hipSetDevice(ix->ctx->device);
         (void)hipStreamSynchronize(ix->ctx->stream);
     }
-    void *bufs[] = {ix->cq, ix->pq, ix->norm_c, ix->norm_r, ix->dot_cr, ix->L, ix->pid, ix->codes};
+    void *bufs[] = {ix->cq,  ix->pq,  ix->norm_c, ix->norm_r, ix->dot_cr,
+                    ix->L,   ix->pid, ix->codes,  ix->cqT,    ix->pqT};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     delete ix;

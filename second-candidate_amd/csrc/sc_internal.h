@@ -19,6 +19,7 @@ struct sc_index {
     int32_t dim, mc, kc, mp, kp, nc;
     int64_t n, ncells;
     float *cq, *pq;                 // codebooks [mc][kc][bsc], [mp][kp][bsp]
+    float *cqT, *pqT;               // transposed [mc][bsc][kc], [mp][bsp][kp] (LUT loads)
     float *norm_c, *norm_r, *dot_cr;// pre_compute1 tables
     int32_t *L;                     // inclusive prefix [ncells]
     int32_t *pid;                   // [n]

@@ -42,18 +42,20 @@ __global__ void k_lut(const float *__restrict__ Q, int64_t nq, int dim,
     int64_t q = e / per;
     int j = (int)(e - q * per);
     const float *qv = Q + (size_t)q * dim;
+    // cq/pq are the transposed codebooks [blocks][blen][K]: neighbouring
+    // threads read neighbouring codewords; each dot still runs t = 0, 1, ...
     if (j < mc * kc) {
-        const int bsc = dim / mc, blk = j / kc;
-        const float *a = qv + (size_t)blk * bsc, *b = cq + (size_t)j * bsc;
+        const int bsc = dim / mc, blk = j / kc, k = j - blk * kc;
+        const float *a = qv + (size_t)blk * bsc, *b = cq + (size_t)blk * bsc * kc + k;
         float s = 0.0f;
-        for (int t = 0; t < bsc; ++t) s += a[t] * b[t];
+        for (int t = 0; t < bsc; ++t) s += a[t] * b[(size_t)t * kc];
         dqc[(size_t)q * mc * kc + j] = (norm_c[j] - s) - s;
     } else {
         j -= mc * kc;
-        const int bsp = dim / mp, sub = j / kp;
-        const float *a = qv + (size_t)sub * bsp, *b = pq + (size_t)j * bsp;
+        const int bsp = dim / mp, sub = j / kp, c = j - sub * kp;
+        const float *a = qv + (size_t)sub * bsp, *b = pq + (size_t)sub * bsp * kp + c;
         float s = 0.0f;
-        for (int t = 0; t < bsp; ++t) s += a[t] * b[t];
+        for (int t = 0; t < bsp; ++t) s += a[t] * b[(size_t)t * kp];
         dqr[(size_t)q * mp * kp + j] = (norm_r[j] - s) - s;
     }
 }
@@ -433,19 +435,15 @@ __device__ void sort_buffer(ScanShared &S, int n) {
     }
 }
 
-__global__ __launch_bounds__(kScanThreads) void k_scan_topk(ScanArgs a) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    ScanShared &S = *reinterpret_cast<ScanShared *>(smem);
-    float *lut = reinterpret_cast<float *>(smem + sizeof(ScanShared));   // [mp*kp]
-    const int64_t q = a.qlist ? a.qlist[blockIdx.x] : blockIdx.x;
+// Scan + top-R of query q over its cell list (cells/cinfo, ncell cells, sum
+// candidates); dqr/dqc/cells/cinfo may live in LDS or global memory.  All
+// threads of the workgroup call it; `lut` is LDS of mp*kp floats.
+__device__ void scan_query(const ScanArgs &a, ScanShared &S, float *lut, int64_t q,
+                           const float *dqr, const float *dqc, float qsum,
+                           const int32_t *cells, const uint32_t *cinfo, int ncell, int sum) {
     const int tid = threadIdx.x;
-    const int ncell = a.qinfo[q * 8 + 0];
-    const int sum = a.qinfo[q * 8 + 1];
     const int mp = a.mp, kp = a.kp, kc = a.kc;
     const int lutn = mp * kp;
-    const float *dqr = a.dqr + (size_t)q * lutn;
-    const float *dqc = a.dqc + (size_t)q * a.mc * kc;
-    const float qsum = a.qs[q * 4];
     const bool imi = a.kind == SC_KIND_IMI;
     const int bs = kp * mp / a.mc;
     const int half = mp >> 1;
@@ -454,8 +452,6 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_topk(ScanArgs a) {
         for (int i = tid; i < lutn; i += blockDim.x) lut[i] = dqr[i];
     int lut_row = -1;     // h3 whose combined row is staged (IVF/SC)
     int base_pos = 0;     // position of the first candidate of the run
-    const int32_t *cells = a.cells + (size_t)q * a.ccap;
-    const uint32_t *cinfo = a.cinfo + (size_t)q * a.ccap;
     __syncthreads();
 
     int c0 = 0;
@@ -508,7 +504,17 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_topk(ScanArgs a) {
                 float d;
                 if (!imi) {
                     d = qsum + dqc[ci & 0xFFFFu];
-                    for (int k = 0; k < mp; ++k) d += lut[k * kp + cc[k]];
+                    if ((mp & 3) == 0) {      // 4 codes per 32-bit load, same add order
+                        const uint32_t *cw = reinterpret_cast<const uint32_t *>(cc);
+                        for (int w4 = 0; w4 < (mp >> 2); ++w4) {
+                            const uint32_t wv = cw[w4];
+#pragma unroll
+                            for (int b = 0; b < 4; ++b)
+                                d += lut[(w4 * 4 + b) * kp + ((wv >> (8 * b)) & 255u)];
+                        }
+                    } else {
+                        for (int k = 0; k < mp; ++k) d += lut[k * kp + cc[k]];
+                    }
                 } else {
                     const int h3 = (int)(ci & 0xFFFFu), h4 = (int)(ci >> 16);
                     d = (qsum + dqc[h3]) + dqc[kc + h4];
@@ -570,6 +576,244 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_topk(ScanArgs a) {
         }
     }
     if (a.sums && tid == 0) a.sums[q] = sum;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_topk(ScanArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    ScanShared &S = *reinterpret_cast<ScanShared *>(smem);
+    float *lut = reinterpret_cast<float *>(smem + sizeof(ScanShared));   // [mp*kp]
+    const int64_t q = a.qlist ? a.qlist[blockIdx.x] : blockIdx.x;
+    scan_query(a, S, lut, q, a.dqr + (size_t)q * a.mp * a.kp, a.dqc + (size_t)q * a.mc * a.kc,
+               a.qs[q * 4], a.cells + (size_t)q * a.ccap, a.cinfo + (size_t)q * a.ccap,
+               a.qinfo[q * 8 + 0], a.qinfo[q * 8 + 1]);
+}
+
+// ----------------------------------------------------------------------------
+// Fused persistent search (IVF and SC kinds): one workgroup runs all stages of
+// one query in LDS -- LUTs, coarse ranking, traversal (thread 0, exact heap
+// replay), scan and top-R -- then takes the next query from a work counter.
+// A query whose heap or cell list outgrows its LDS capacity is flagged and
+// finished by the staged kernels afterwards.
+// ----------------------------------------------------------------------------
+struct FusedArgs {
+    ScanArgs s;
+    const float *Q, *cq, *pq, *norm_c, *norm_r;
+    int64_t nq;
+    int w, T, hcap, ccap, npow2;
+    int *counter;
+    int32_t *flags;     // [nq] 1 = finish with the staged kernels
+    int *nflag;
+    int32_t *qinfo;     // [nq][8] statistics (same fields as the staged path)
+};
+
+struct FusedLayout {
+    size_t a, b, c, d, e, f, total;
+};
+
+__host__ __device__ inline FusedLayout fused_layout(int dim, int mc, int kc, int mp, int kp,
+                                                    int hcap, int ccap, int npow2) {
+    auto up = [](size_t v) { return (v + 15) & ~(size_t)15; };
+    FusedLayout L{};
+    size_t heap = (size_t)hcap * 8 + (size_t)kc * 4;
+    size_t a = sizeof(ScanShared);
+    if ((size_t)npow2 * 8 > a) a = (size_t)npow2 * 8;
+    if (heap > a) a = heap;
+    L.a = 0;
+    L.b = up(a);                                   // cells + cinfo [ccap] each
+    L.c = up(L.b + (size_t)ccap * 8);              // sorted v [kc] + idx [kc]
+    L.d = up(L.c + (size_t)kc * 8);                // dqc [mc*kc] + dqr [mp*kp]
+    L.e = up(L.d + 4 * ((size_t)mc * kc + (size_t)mp * kp));   // lut [mp*kp]
+    L.f = up(L.e + 4 * (size_t)mp * kp);           // query [dim]
+    L.total = up(L.f + 4 * (size_t)dim);
+    return L;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_search_fused(FusedArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const ScanArgs &sa = a.s;
+    const int dim = sa.dim, mc = sa.mc, kc = sa.kc, mp = sa.mp, kp = sa.kp;
+    const FusedLayout Lay = fused_layout(dim, mc, kc, mp, kp, a.hcap, a.ccap, a.npow2);
+    ScanShared &S = *reinterpret_cast<ScanShared *>(smem + Lay.a);
+    uint64_t *keys = reinterpret_cast<uint64_t *>(smem + Lay.a);
+    int32_t *cells = reinterpret_cast<int32_t *>(smem + Lay.b);
+    uint32_t *cinfo = reinterpret_cast<uint32_t *>(cells + a.ccap);
+    float *sv = reinterpret_cast<float *>(smem + Lay.c);
+    int32_t *si = reinterpret_cast<int32_t *>(sv + kc);
+    float *dqc = reinterpret_cast<float *>(smem + Lay.d);
+    float *dqr = dqc + mc * kc;
+    float *lut = reinterpret_cast<float *>(smem + Lay.e);
+    float *qv = reinterpret_cast<float *>(smem + Lay.f);
+    __shared__ int s_q, s_ncell, s_sum, s_flag;
+    __shared__ float s_qsum;
+    const int tid = threadIdx.x;
+    const bool sc = sa.kind == SC_KIND_SC;
+    const int bsc = dim / mc, bsp = dim / mp;
+
+    for (;;) {
+        if (tid == 0) s_q = atomicAdd(a.counter, 1);
+        __syncthreads();
+        const int64_t q = s_q;
+        if (q >= a.nq) break;
+        for (int t = tid; t < dim; t += blockDim.x) qv[t] = a.Q[(size_t)q * dim + t];
+        __syncthreads();
+        // LUTs (query.h:154-184) and q_sum (query.h:225-230, sc_query.h:95-100)
+        if (tid == 0) {
+            float s = 0.0f;
+            for (int i = 0; i < dim; ++i) { float t = qv[i]; s += t * t; }
+            s_qsum = s;
+        }
+        for (int j = tid; j < mc * kc + mp * kp; j += blockDim.x) {
+            if (j < mc * kc) {     // a.cq / a.pq: transposed codebooks
+                const int blk = j / kc, k = j - blk * kc;
+                const float *b = a.cq + (size_t)blk * bsc * kc + k;
+                const float *x = qv + (size_t)blk * bsc;
+                float s = 0.0f;
+                for (int t = 0; t < bsc; ++t) s += x[t] * b[(size_t)t * kc];
+                dqc[j] = (a.norm_c[j] - s) - s;
+            } else {
+                const int r = j - mc * kc, sub = r / kp, c = r - sub * kp;
+                const float *b = a.pq + (size_t)sub * bsp * kp + c;
+                const float *x = qv + (size_t)sub * bsp;
+                float s = 0.0f;
+                for (int t = 0; t < bsp; ++t) s += x[t] * b[(size_t)t * kp];
+                dqr[r] = (a.norm_r[r] - s) - s;
+            }
+        }
+        __syncthreads();
+        const float qsum = s_qsum;
+        // coarse scores; SC sorts them by (score, centroid)
+        if (sc) {
+            for (int j = tid; j < a.npow2; j += blockDim.x)
+                keys[j] = j < kc ? (((uint64_t)sc_float_key(qsum + dqc[j]) << 32) | (uint32_t)j)
+                                 : ~(uint64_t)0;
+            __syncthreads();
+            for (int size = 2; size <= a.npow2; size <<= 1)
+                for (int stride = size >> 1; stride > 0; stride >>= 1) {
+                    for (int i = tid; i < a.npow2 / 2; i += blockDim.x) {
+                        int lo = 2 * i - (i & (stride - 1)), hi = lo + stride;
+                        bool upw = ((lo & size) == 0);
+                        uint64_t x = keys[lo], y = keys[hi];
+                        if ((x > y) == upw) { keys[lo] = y; keys[hi] = x; }
+                    }
+                    __syncthreads();
+                }
+            for (int j = tid; j < kc; j += blockDim.x) {
+                sv[j] = sc_key_float((uint32_t)(keys[j] >> 32));
+                si[j] = (int32_t)(keys[j] & 0xFFFFFFFFu);
+            }
+        } else {
+            for (int j = tid; j < kc; j += blockDim.x) { sv[j] = qsum + dqc[j]; si[j] = j; }
+        }
+        __syncthreads();
+        // traversal: thread 0, heap and row lengths in LDS (region A)
+        if (tid == 0) {
+            Heap h;
+            float *hv = reinterpret_cast<float *>(smem + Lay.a);
+            h.v = hv;
+            h.p = reinterpret_cast<uint32_t *>(hv + a.hcap);
+            int32_t *rowlen = reinterpret_cast<int32_t *>(h.p + a.hcap);
+            h.stride = 1;
+            h.cap_phys = a.hcap;
+            h.pos = 0;
+            h.overflow = false;
+            int count = 0, count2 = 0, sum = 0, empty = 0, exhausted = 0, flag = 0;
+            if (!sc) {
+                h.cap_ref = kc;
+                for (int j = 0; j < kc; ++j) h.insert(sv[j], (uint32_t)j);
+                for (int i = 0; i < a.w && !h.overflow; ++i) {
+                    if (h.pos <= 0) { flag = 1; break; }
+                    int bid = (int)h.pop();
+                    if (count >= a.ccap) { flag = 1; break; }
+                    cells[count] = bid;
+                    cinfo[count] = (uint32_t)bid;
+                    count++;
+                    count2++;
+                    sum += cell_len(sa.L, bid);
+                    if (sum >= a.T) break;
+                }
+            } else {
+                h.cap_ref = a.w;
+                for (int r = 0; r < kc; ++r) rowlen[r] = 0;
+                rowlen[0] = 1;
+                const float d0 = sv[0] + sv[1];
+                h.insert(d0, 1u);
+                h.insert(d0, 1u << 16);
+                while (count < a.w && sum < a.T) {
+                    if (h.overflow) break;
+                    if (h.pos <= 0) { exhausted = 1; break; }
+                    const uint32_t pk = h.pop();
+                    const int h1 = (int)(pk >> 16), h2 = (int)(pk & 0xFFFFu);
+                    const int h3 = si[h1], h4 = si[h2];
+                    const int64_t bid = (int64_t)h3 * kc + h4;
+                    const int l = bid > 0 ? cell_len(sa.L, bid) : 0;
+                    if (l > 0) {
+                        if (count >= a.ccap) { flag = 1; break; }
+                        cells[count] = (int32_t)bid;
+                        cinfo[count] = (uint32_t)h3 | ((uint32_t)h4 << 16);
+                        count++;
+                        sum += l;
+                    } else {
+                        empty++;
+                    }
+                    rowlen[h1] = h2 + 1;
+                    count2++;
+                    if (sum < a.T) {
+                        if (h1 < kc - 1 && (h2 == 0 || (h2 - 1) < rowlen[h1 + 1]))
+                            h.insert(sv[h1 + 1] + sv[h2], ((uint32_t)(h1 + 1) << 16) | (uint32_t)h2);
+                        if (h2 < kc - 1 && (h1 == 0 || (h2 + 1) < rowlen[h1 - 1]))
+                            h.insert(sv[h1] + sv[h2 + 1], ((uint32_t)h1 << 16) | (uint32_t)(h2 + 1));
+                    }
+                }
+            }
+            if (h.overflow) flag = 1;
+            s_ncell = count;
+            s_sum = sum;
+            s_flag = flag;
+            int32_t *qi = a.qinfo + (size_t)q * 8;   // a flagged query is counted by its redo
+            qi[0] = flag ? 0 : count; qi[1] = flag ? 0 : sum; qi[2] = flag ? 0 : count2;
+            qi[3] = flag ? 0 : empty; qi[4] = flag ? 0 : exhausted;
+            qi[5] = flag; qi[6] = 0;
+            a.flags[q] = flag;
+            if (flag) atomicAdd(a.nflag, 1);
+        }
+        __syncthreads();
+        if (!s_flag)
+            scan_query(sa, S, lut, q, dqr, dqc, qsum, cells, cinfo, s_ncell, s_sum);
+        __syncthreads();
+    }
+}
+
+__global__ void k_zero_ints(int *p, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+__global__ void k_gather_rows(const float *__restrict__ Q, int dim, const int32_t *__restrict__ list,
+                              int n, float *__restrict__ out) {
+    int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)n * dim) return;
+    int64_t r = e / dim;
+    out[e] = Q[(size_t)list[r] * dim + (e - r * dim)];
+}
+
+__global__ void k_list_flags(const int32_t *__restrict__ flags, int64_t nq, int32_t *__restrict__ list,
+                             int *__restrict__ n) {
+    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < nq && flags[q]) list[atomicAdd(n, 1)] = (int32_t)q;
+}
+
+__global__ void k_scatter_results(const int32_t *__restrict__ list, int n, int R,
+                                  const int32_t *__restrict__ ids, const float *__restrict__ d,
+                                  const int32_t *__restrict__ sums, int32_t *__restrict__ out_ids,
+                                  float *__restrict__ out_d, int32_t *__restrict__ out_sums) {
+    int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)n * R) return;
+    int64_t r = e / R;
+    int c = (int)(e - r * R);
+    const int64_t q = list[r];
+    out_ids[q * R + c] = ids[e];
+    out_d[q * R + c] = d[e];
+    if (c == 0 && out_sums) out_sums[q] = sums[r];
 }
 
 __global__ void k_gather_fallback(const int32_t *__restrict__ qinfo, int64_t nq,
@@ -636,6 +880,140 @@ Plan make_plan(const sc_index *ix, const sc_search_params *p, int64_t nq) {
 
 }  // namespace
 
+static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int64_t nq,
+                         const sc_search_params *p, int32_t *d_ids, float *d_dists,
+                         int32_t *d_sums, sc_search_stats *st);
+
+// Fused persistent path for IVF/SC.  Returns SC_ERR_TOO_LARGE when its LDS
+// layout does not fit (the caller then uses the staged kernels).
+static int search_fused(sc_ctx *ctx, const sc_index *ix, const float *d_q, int64_t nq,
+                        const sc_search_params *p, int32_t *d_ids, float *d_dists,
+                        int32_t *d_sums, sc_search_stats *st) {
+    const int kind = p->kind, kc = ix->kc;
+    int npow2 = 1;
+    while (npow2 < kc) npow2 <<= 1;
+    const int hcap_ref = kind == SC_KIND_IVFADC ? kc : p->w;
+    int hcap = std::min(hcap_ref, 1024);
+    if (const char *e = getenv("SC_HEAP_ONCHIP")) {
+        int v = atoi(e);
+        if (v >= 1) hcap = std::min(hcap, v);
+    }
+    const int64_t ccap_ref =
+        kind == SC_KIND_IVFADC ? p->w : std::min<int64_t>(p->w, (int64_t)p->T + 1);
+    const int ccap = (int)std::max<int64_t>(1, std::min<int64_t>(ccap_ref, 1024));
+    const FusedLayout Lay = fused_layout(ix->dim, ix->mc, kc, ix->mp, ix->kp, hcap, ccap, npow2);
+    if (Lay.total > 150 * 1024) return SC_ERR_TOO_LARGE;
+
+    size_t o_cnt = 0, o_flags = 256, o_qinfo = o_flags + ((4 * (size_t)nq + 255) & ~(size_t)255),
+           o_stats = o_qinfo + 32 * (size_t)nq, total = o_stats + 64;
+    unsigned char *s;
+    SC_TRY(sc_scratch(ctx, total, (void **)&s));
+    int *counter = (int *)(s + o_cnt);            // [0] work counter, [1] flagged
+    int32_t *flags = (int32_t *)(s + o_flags);
+    int32_t *qinfo = (int32_t *)(s + o_qinfo);
+    unsigned long long *stats = (unsigned long long *)(s + o_stats);
+    hipStream_t sm = ctx->stream;
+
+    hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, sm, counter, 2);
+    SC_CHECK_LAUNCH();
+    SC_HIP(hipFuncSetAttribute((const void *)k_search_fused,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)Lay.total));
+    int occ = (int)std::max<size_t>(1, std::min<size_t>(4, (160 * 1024) / Lay.total));
+    int64_t grid = (int64_t)ctx->n_cu * occ;
+    if (const char *e = getenv("SC_FUSED_GRID")) {   // tuning hook: workgroups in flight
+        int v = atoi(e);
+        if (v >= 1) grid = v;
+    }
+    grid = std::min<int64_t>(grid, nq);
+    FusedArgs fa;
+    fa.s = ScanArgs{kind,      ix->dim,   ix->mc,  kc,      ix->mp,  ix->kp, p->R,
+                    ccap,      nullptr,   nullptr, nullptr, ix->dot_cr, ix->L, ix->pid,
+                    ix->codes, nullptr,   nullptr, nullptr, d_ids,   d_dists, d_sums, nullptr};
+    fa.Q = d_q; fa.cq = ix->cqT; fa.pq = ix->pqT; fa.norm_c = ix->norm_c; fa.norm_r = ix->norm_r;
+    fa.nq = nq; fa.w = p->w; fa.T = p->T; fa.hcap = hcap; fa.ccap = ccap; fa.npow2 = npow2;
+    fa.counter = counter; fa.flags = flags; fa.nflag = counter + 1; fa.qinfo = qinfo;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (st) {
+        SC_HIP(hipEventCreate(&e0));
+        SC_HIP(hipEventCreate(&e1));
+        SC_HIP(hipEventRecord(e0, sm));
+    }
+    hipLaunchKernelGGL(k_search_fused, dim3((unsigned)grid), dim3(kScanThreads), Lay.total, sm,
+                       fa);
+    SC_CHECK_LAUNCH();
+    if (st) {
+        SC_HIP(hipEventRecord(e1, sm));
+        SC_HIP(hipMemsetAsync(stats, 0, 64, sm));
+        hipLaunchKernelGGL(k_reduce_stats, dim3((unsigned)sc_cdiv(nq, 256)), dim3(256), 0, sm,
+                           qinfo, nq, stats);
+        SC_CHECK_LAUNCH();
+    }
+    int nflag = 0;
+    unsigned long long h[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    SC_HIP(hipMemcpyAsync(&nflag, counter + 1, 4, hipMemcpyDeviceToHost, sm));
+    if (st) SC_HIP(hipMemcpyAsync(h, stats, 64, hipMemcpyDeviceToHost, sm));
+    SC_HIP(hipStreamSynchronize(sm));
+    if (st) {
+        float ms = 0.0f;
+        SC_HIP(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        st->queries = nq;
+        st->candidates = (int64_t)h[0];
+        st->cells = (int64_t)h[1];
+        st->visited = (int64_t)h[2];
+        st->exhausted = (int64_t)h[3];
+        st->ms_scan = ms;
+    }
+    if (nflag > 0) {
+        // finish the flagged queries with the staged kernels
+        int32_t *list = nullptr, *sub_ids = nullptr, *sub_sums = nullptr;
+        float *sub_q = nullptr, *sub_d = nullptr;
+        int *dn = nullptr;
+        bool ok = hipMalloc(&list, 4 * (size_t)nflag) == hipSuccess &&
+                  hipMalloc(&dn, 4) == hipSuccess &&
+                  hipMalloc(&sub_q, 4 * (size_t)nflag * ix->dim) == hipSuccess &&
+                  hipMalloc(&sub_ids, 4 * (size_t)nflag * p->R) == hipSuccess &&
+                  hipMalloc(&sub_d, 4 * (size_t)nflag * p->R) == hipSuccess &&
+                  hipMalloc(&sub_sums, 4 * (size_t)nflag) == hipSuccess;
+        int rc = ok ? SC_OK : SC_ERR_NOMEM;
+        if (ok) {
+            hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, sm, dn, 1);
+            hipLaunchKernelGGL(k_list_flags, dim3((unsigned)sc_cdiv(nq, 256)), dim3(256), 0, sm,
+                               flags, nq, list, dn);
+            hipLaunchKernelGGL(k_gather_rows, dim3((unsigned)sc_cdiv((int64_t)nflag * ix->dim, 256)),
+                               dim3(256), 0, sm, d_q, ix->dim, list, nflag, sub_q);
+            rc = hipGetLastError() == hipSuccess ? SC_OK : SC_ERR_HIP;
+        }
+        sc_search_stats sub{};
+        if (rc == SC_OK)
+            rc = search_staged(ctx, ix, sub_q, nflag, p, sub_ids, sub_d, sub_sums, st ? &sub : nullptr);
+        if (rc == SC_OK) {
+            hipLaunchKernelGGL(k_scatter_results, dim3((unsigned)sc_cdiv((int64_t)nflag * p->R, 256)),
+                               dim3(256), 0, sm, list, nflag, p->R, sub_ids, sub_d, sub_sums,
+                               d_ids, d_dists, d_sums);
+            rc = hipGetLastError() == hipSuccess ? SC_OK : SC_ERR_HIP;
+            (void)hipStreamSynchronize(sm);
+        }
+        for (void *b : {(void *)list, (void *)dn, (void *)sub_q, (void *)sub_ids, (void *)sub_d,
+                        (void *)sub_sums})
+            if (b) (void)hipFree(b);
+        if (rc != SC_OK) {
+            if (rc == SC_ERR_NOMEM) sc_set_error("sc_search: fallback allocation failed");
+            return rc;
+        }
+        if (st) {
+            st->candidates += sub.candidates;
+            st->cells += sub.cells;
+            st->visited += sub.visited;
+            st->exhausted += sub.exhausted;
+            st->heap_fallback = nflag;
+            st->ms_traverse = sub.ms_lut + sub.ms_rank + sub.ms_traverse + sub.ms_scan;
+        }
+    }
+    return SC_OK;
+}
+
 extern "C" int sc_search(sc_ctx *ctx, const sc_index *ix, const float *d_q, int64_t nq,
                          const sc_search_params *p, int32_t *d_ids, float *d_dists,
                          int32_t *d_sums, sc_search_stats *st) {
@@ -671,7 +1049,18 @@ extern "C" int sc_search(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
     }
     if (st) *st = sc_search_stats{};
     if (nq == 0) return SC_OK;
+    if (kind != SC_KIND_IMI && !getenv("SC_NO_FUSED")) {
+        int rc = search_fused(ctx, ix, d_q, nq, p, d_ids, d_dists, d_sums, st);
+        if (rc != SC_ERR_TOO_LARGE) return rc;      // otherwise: staged kernels
+        if (st) *st = sc_search_stats{};
+    }
+    return search_staged(ctx, ix, d_q, nq, p, d_ids, d_dists, d_sums, st);
+}
 
+static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int64_t nq,
+                         const sc_search_params *p, int32_t *d_ids, float *d_dists,
+                         int32_t *d_sums, sc_search_stats *st) {
+    const int kind = p->kind;
     const Plan P = make_plan(ix, p, nq);
     unsigned char *s;
     SC_TRY(sc_scratch(ctx, P.total, (void **)&s));
@@ -714,7 +1103,7 @@ extern "C" int sc_search(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
         const int64_t lut_total = cn * (ix->mc * kc + ix->mp * ix->kp);
         mark();
         hipLaunchKernelGGL(k_lut, dim3((unsigned)sc_cdiv(lut_total, 256)), dim3(256), 0,
-                           ctx->stream, Q, cn, ix->dim, ix->cq, ix->pq, ix->norm_c, ix->norm_r,
+                           ctx->stream, Q, cn, ix->dim, ix->cqT, ix->pqT, ix->norm_c, ix->norm_r,
                            ix->mc, kc, ix->mp, ix->kp, dqc, dqr);
         SC_CHECK_LAUNCH();
         mark();

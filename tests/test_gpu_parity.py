@@ -69,7 +69,8 @@ def test_encode_matches_oracle(ctx, case, prec):
         x[n // 2:] += rng.uniform(-0.5, 0.5, (n - n // 2, dim)).astype(np.float32)
     cq, pq = _codebooks(rng, x, mc, kc, mp, kp, ties=kc > 2)
     bsc = dim // mc
-    x[:5, :bsc] = cq[0, 1]                        # exact ties on the coarse distance
+    if kc > 1:
+        x[:5, :bsc] = cq[0, 1]                    # exact ties on the coarse distance
     cid, codes, st = S.encode(ctx, S.Dev.from_numpy(ctx, x), S.Dev.from_numpy(ctx, cq),
                               S.Dev.from_numpy(ctx, pq), mc, kc, mp, kp, nc, prec)
     ocid, ocodes = sco.encode(x, cq, pq, mc, kc, mp, kp, nc)
