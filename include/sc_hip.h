@@ -165,6 +165,27 @@ int sc_search(sc_ctx *ctx, const sc_index *ix, const float *d_q, int64_t nq,
               const sc_search_params *p, int32_t *d_ids, float *d_dists,
               int32_t *d_sums, sc_search_stats *st);
 
+/* ---- batched search plans (no reference counterpart) ------------------- */
+/* A plan binds one batch shape (nq queries at d_q, outputs at d_ids/d_dists/
+ * d_sums) and captures the persistent fused search of that batch (work-queue
+ * reset + one persistent launch) in a hipGraph that sc_plan_execute replays.
+ * The buffers' contents may change between executions; their addresses may
+ * not.  Kinds whose traversal does not fit on chip (IMI) run the staged
+ * kernels instead of a graph.  Queries the graph flags as exceeding on-chip
+ * capacity are finished by the staged kernels inside sc_plan_execute, which
+ * synchronises the context stream.  This batches what the reference does one
+ * query per call (include/sc_query.h:60-256, include/multi_query.h:62-273). */
+typedef struct sc_plan sc_plan;
+int sc_plan_create(sc_ctx *ctx, const sc_index *ix, const sc_search_params *p,
+                   const float *d_q, int64_t nq, int32_t *d_ids, float *d_dists,
+                   int32_t *d_sums, sc_plan **out);
+int sc_plan_execute(sc_plan *plan, sc_search_stats *st);
+/* Times the fused kernel over launch shapes (threads per workgroup x
+ * workgroups in flight) on the plan's own batch, keeps the fastest and
+ * re-captures the graph.  out3 (may be NULL) = {threads, grid, best_us}. */
+int sc_plan_autotune(sc_plan *plan, int32_t *out3);
+int sc_plan_destroy(sc_plan *plan);
+
 /* ---- k-means (codebook training) --------------------------------------- */
 /* k-means++ seeding + `iters` Lloyd iterations on d_x [n][d]:
  * PQQuantizer::create_sub_quantizers -> greg_kmeans

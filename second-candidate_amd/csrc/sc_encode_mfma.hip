@@ -98,7 +98,7 @@ struct CbView {
 template <bool DESC>
 __device__ __forceinline__ bool precedes(double d, int k, double e, int j) {
     if (d < e) return true;
-    if (d > e) return false;
+    if (!(d == e)) return false;   // d > e, or a NaN: the reference's `>`/`>=` never take it
     return DESC ? (k > j) : (k < j);
 }
 

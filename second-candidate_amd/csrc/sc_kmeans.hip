@@ -181,7 +181,8 @@ __global__ void k_lonely(const float *__restrict__ x, int64_t n, int d, const fl
     unsigned long long key = 0;
     if (i < n) {
         float t = (float)s1(x + (size_t)i * d, c + (size_t)labels[i] * d, d);
-        key = ((unsigned long long)sc_float_key(t) << 32) | (uint32_t)~(uint32_t)i;
+        // find_lonely's `dfst < d` never takes a NaN distance
+        if (t == t) key = ((unsigned long long)sc_float_key(t) << 32) | (uint32_t)~(uint32_t)i;
     }
     for (int o = 32; o > 0; o >>= 1) {
         unsigned long long u = __shfl_xor(key, o);
@@ -325,7 +326,12 @@ extern "C" int sc_kmeans(sc_ctx *ctx, const float *d_x, int64_t n, int32_t d, in
             int32_t old = 0;
             SC_HIP(hipMemcpyAsync(&hb, best, 8, hipMemcpyDeviceToHost, s));
             SC_HIP(hipStreamSynchronize(s));
-            const int64_t fst = (int64_t)(uint32_t)~(uint32_t)(hb & 0xFFFFFFFFull);
+            // no finite distance at all: find_lonely keeps its initial fst = 0
+            const int64_t fst = hb ? (int64_t)(uint32_t)~(uint32_t)(hb & 0xFFFFFFFFull) : 0;
+            if (fst >= n) {
+                sc_set_error("sc_kmeans: farthest-point search returned %lld", (long long)fst);
+                return SC_ERR_HIP;
+            }
             SC_HIP(hipMemcpyAsync(&old, labels + fst, 4, hipMemcpyDeviceToHost, s));
             SC_HIP(hipStreamSynchronize(s));
             hipLaunchKernelGGL(k_move_center, dim3(1), dim3(256), 0, s, d_x, d, fst, j, old,

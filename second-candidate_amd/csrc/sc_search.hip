@@ -18,6 +18,7 @@
 #include <stdlib.h>
 
 #include <algorithm>
+#include <new>
 #include <vector>
 
 #include "sc_internal.h"
@@ -25,6 +26,7 @@
 namespace {
 
 constexpr int kScanThreads = 256;
+constexpr int kMaxFusedThreads = 512;   // fused workgroups run 256 or 512 threads
 constexpr int kBufCap = 2048;      // streaming top-R buffer (entries)
 constexpr int kRunCells = 256;     // cells staged per scan run
 
@@ -488,8 +490,8 @@ __device__ void scan_query(const ScanArgs &a, ScanShared &S, float *lut, int64_t
             }
         }
         __syncthreads();
-        for (int p0 = 0; p0 < rtotal; p0 += kScanThreads) {
-            if (S.nb > kBufCap - kScanThreads) compact(S, a.R);
+        for (int p0 = 0; p0 < rtotal; p0 += blockDim.x) {
+            if (S.nb > kBufCap - (int)blockDim.x) compact(S, a.R);
             const int p = p0 + tid;
             if (p < rtotal) {
                 int lo = 0, hi = rn - 1;          // last cell with c_pref <= p
@@ -628,7 +630,7 @@ __host__ __device__ inline FusedLayout fused_layout(int dim, int mc, int kc, int
     return L;
 }
 
-__global__ __launch_bounds__(kScanThreads) void k_search_fused(FusedArgs a) {
+__global__ __launch_bounds__(kMaxFusedThreads) void k_search_fused(FusedArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     const ScanArgs &sa = a.s;
     const int dim = sa.dim, mc = sa.mc, kc = sa.kc, mp = sa.mp, kp = sa.kp;
@@ -883,6 +885,9 @@ Plan make_plan(const sc_index *ix, const sc_search_params *p, int64_t nq) {
 static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int64_t nq,
                          const sc_search_params *p, int32_t *d_ids, float *d_dists,
                          int32_t *d_sums, sc_search_stats *st);
+static int finish_flagged(sc_ctx *ctx, const sc_index *ix, const sc_search_params *p,
+                          const float *d_q, int64_t nq, const int32_t *flags, int nflag,
+                          int32_t *d_ids, float *d_dists, int32_t *d_sums, sc_search_stats *st);
 
 // Fused persistent path for IVF/SC.  Returns SC_ERR_TOO_LARGE when its LDS
 // layout does not fit (the caller then uses the staged kernels).
@@ -965,6 +970,15 @@ static int search_fused(sc_ctx *ctx, const sc_index *ix, const float *d_q, int64
         st->exhausted = (int64_t)h[3];
         st->ms_scan = ms;
     }
+    return finish_flagged(ctx, ix, p, d_q, nq, flags, nflag, d_ids, d_dists, d_sums, st);
+}
+
+// Finishes the queries the fused kernel flagged (on-chip capacity exceeded)
+// with the staged kernels and scatters their results into place.
+static int finish_flagged(sc_ctx *ctx, const sc_index *ix, const sc_search_params *p,
+                          const float *d_q, int64_t nq, const int32_t *flags, int nflag,
+                          int32_t *d_ids, float *d_dists, int32_t *d_sums, sc_search_stats *st) {
+    hipStream_t sm = ctx->stream;
     if (nflag > 0) {
         // finish the flagged queries with the staged kernels
         int32_t *list = nullptr, *sub_ids = nullptr, *sub_sums = nullptr;
@@ -1022,7 +1036,7 @@ extern "C" int sc_search(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
         sc_set_error("sc_search: bad arguments");
         return SC_ERR_ARG;
     }
-    if (p->R < 1 || p->R > kBufCap - kScanThreads || p->w < 1 || p->T < 0 ||
+    if (p->R < 1 || p->R > kBufCap - kMaxFusedThreads || p->w < 1 || p->T < 0 ||
         (p->form != SC_FORM_PLAIN && p->form != SC_FORM_MIXED)) {
         sc_set_error("sc_search: bad parameters (R=%d w=%d T=%d form=%d)", p->R, p->w, p->T,
                      p->form);
@@ -1209,5 +1223,195 @@ static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
             return SC_ERR_HIP;
         }
     }
+    return SC_OK;
+}
+
+// ----------------------------------------------------------------------------
+// Plans: the fused persistent search of a fixed batch captured in a hipGraph.
+// ----------------------------------------------------------------------------
+struct sc_plan {
+    sc_ctx *ctx;
+    const sc_index *ix;
+    sc_search_params p;
+    const float *d_q;
+    int64_t nq;
+    int32_t *d_ids, *d_sums;
+    float *d_dists;
+    bool fused;
+    FusedArgs fa;
+    size_t lds;
+    int threads, grid;
+    int *counter;                  // [0] work counter, [1] flagged
+    int32_t *flags, *qinfo;
+    unsigned long long *stats;
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+};
+
+static int plan_capture(sc_plan *P) {
+    if (P->exec) { (void)hipGraphExecDestroy(P->exec); P->exec = nullptr; }
+    if (P->graph) { (void)hipGraphDestroy(P->graph); P->graph = nullptr; }
+    hipStream_t sm = P->ctx->stream;
+    SC_HIP(hipStreamBeginCapture(sm, hipStreamCaptureModeThreadLocal));
+    // the work counter is cleared by a kernel, not a memset node
+    hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, sm, P->counter, 2);
+    hipLaunchKernelGGL(k_search_fused, dim3((unsigned)P->grid), dim3(P->threads), P->lds, sm,
+                       P->fa);
+    hipError_t le = hipGetLastError();
+    hipError_t ce = hipStreamEndCapture(sm, &P->graph);
+    if (le != hipSuccess || ce != hipSuccess) {
+        sc_set_error("sc_plan: graph capture failed: %s", hipGetErrorString(le != hipSuccess ? le : ce));
+        return SC_ERR_HIP;
+    }
+    SC_HIP(hipGraphInstantiate(&P->exec, P->graph, nullptr, nullptr, 0));
+    return SC_OK;
+}
+
+extern "C" int sc_plan_create(sc_ctx *ctx, const sc_index *ix, const sc_search_params *p,
+                              const float *d_q, int64_t nq, int32_t *d_ids, float *d_dists,
+                              int32_t *d_sums, sc_plan **out) {
+    SC_TRY(sc_ctx_enter(ctx));
+    if (!out || !p || nq < 1 || !d_q || !d_ids || !d_dists || !d_sums) {
+        sc_set_error("sc_plan_create: bad arguments");
+        return SC_ERR_ARG;
+    }
+    *out = nullptr;
+    // validate with an empty search of the same parameters
+    SC_TRY(sc_search(ctx, ix, d_q, 0, p, d_ids, d_dists, d_sums, nullptr));
+    sc_plan *P = new (std::nothrow) sc_plan();
+    if (!P) return SC_ERR_NOMEM;
+    *P = sc_plan{};
+    P->ctx = ctx; P->ix = ix; P->p = *p; P->d_q = d_q; P->nq = nq;
+    P->d_ids = d_ids; P->d_dists = d_dists; P->d_sums = d_sums;
+    const int kind = p->kind, kc = ix->kc;
+    int npow2 = 1;
+    while (npow2 < kc) npow2 <<= 1;
+    const int hcap = std::min(kind == SC_KIND_IVFADC ? kc : p->w, 1024);
+    const int64_t ccap_ref =
+        kind == SC_KIND_IVFADC ? p->w : std::min<int64_t>(p->w, (int64_t)p->T + 1);
+    const int ccap = (int)std::max<int64_t>(1, std::min<int64_t>(ccap_ref, 1024));
+    const FusedLayout Lay = fused_layout(ix->dim, ix->mc, kc, ix->mp, ix->kp, hcap, ccap, npow2);
+    P->fused = kind != SC_KIND_IMI && Lay.total <= 150 * 1024;
+    if (!P->fused) { *out = P; return SC_OK; }
+    P->lds = Lay.total;
+    bool ok = hipMalloc(&P->counter, 256) == hipSuccess &&
+              hipMalloc(&P->flags, 4 * (size_t)nq) == hipSuccess &&
+              hipMalloc(&P->qinfo, 32 * (size_t)nq) == hipSuccess &&
+              hipMalloc(&P->stats, 64) == hipSuccess;
+    if (!ok) { sc_plan_destroy(P); sc_set_error("sc_plan_create: allocation failed"); return SC_ERR_NOMEM; }
+    FusedArgs &fa = P->fa;
+    fa.s = ScanArgs{kind,      ix->dim, ix->mc,  kc,      ix->mp,     ix->kp, p->R,
+                    ccap,      nullptr, nullptr, nullptr, ix->dot_cr, ix->L,  ix->pid,
+                    ix->codes, nullptr, nullptr, nullptr, d_ids,      d_dists, d_sums, nullptr};
+    fa.Q = d_q; fa.cq = ix->cqT; fa.pq = ix->pqT; fa.norm_c = ix->norm_c; fa.norm_r = ix->norm_r;
+    fa.nq = nq; fa.w = p->w; fa.T = p->T; fa.hcap = hcap; fa.ccap = ccap; fa.npow2 = npow2;
+    fa.counter = P->counter; fa.flags = P->flags; fa.nflag = P->counter + 1; fa.qinfo = P->qinfo;
+    if (hipFuncSetAttribute((const void *)k_search_fused, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)P->lds) != hipSuccess) {
+        sc_plan_destroy(P);
+        return SC_ERR_HIP;
+    }
+    P->threads = kScanThreads;
+    const int occ = (int)std::max<size_t>(1, std::min<size_t>(4, (160 * 1024) / P->lds));
+    P->grid = (int)std::min<int64_t>((int64_t)ctx->n_cu * occ, nq);
+    int rc = plan_capture(P);
+    if (rc) { sc_plan_destroy(P); return rc; }
+    *out = P;
+    return SC_OK;
+}
+
+extern "C" int sc_plan_execute(sc_plan *P, sc_search_stats *st) {
+    if (!P) return SC_ERR_ARG;
+    SC_TRY(sc_ctx_enter(P->ctx));
+    if (!P->fused)
+        return sc_search(P->ctx, P->ix, P->d_q, P->nq, &P->p, P->d_ids, P->d_dists, P->d_sums, st);
+    if (st) *st = sc_search_stats{};
+    hipStream_t sm = P->ctx->stream;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (st) {
+        SC_HIP(hipEventCreate(&e0));
+        SC_HIP(hipEventCreate(&e1));
+        SC_HIP(hipEventRecord(e0, sm));
+    }
+    SC_HIP(hipGraphLaunch(P->exec, sm));
+    if (st) {
+        SC_HIP(hipEventRecord(e1, sm));
+        SC_HIP(hipMemsetAsync(P->stats, 0, 64, sm));
+        hipLaunchKernelGGL(k_reduce_stats, dim3((unsigned)sc_cdiv(P->nq, 256)), dim3(256), 0, sm,
+                           P->qinfo, P->nq, P->stats);
+        SC_CHECK_LAUNCH();
+    }
+    int nflag = 0;
+    unsigned long long h[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    SC_HIP(hipMemcpyAsync(&nflag, P->counter + 1, 4, hipMemcpyDeviceToHost, sm));
+    if (st) SC_HIP(hipMemcpyAsync(h, P->stats, 64, hipMemcpyDeviceToHost, sm));
+    SC_HIP(hipStreamSynchronize(sm));
+    if (st) {
+        float ms = 0.0f;
+        SC_HIP(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        st->queries = P->nq;
+        st->candidates = (int64_t)h[0];
+        st->cells = (int64_t)h[1];
+        st->visited = (int64_t)h[2];
+        st->exhausted = (int64_t)h[3];
+        st->ms_scan = ms;
+    }
+    return finish_flagged(P->ctx, P->ix, &P->p, P->d_q, P->nq, P->flags, nflag, P->d_ids,
+                          P->d_dists, P->d_sums, st);
+}
+
+extern "C" int sc_plan_autotune(sc_plan *P, int32_t *out3) {
+    if (!P) return SC_ERR_ARG;
+    SC_TRY(sc_ctx_enter(P->ctx));
+    if (!P->fused) {
+        if (out3) { out3[0] = 0; out3[1] = 0; out3[2] = 0; }
+        return SC_OK;
+    }
+    hipStream_t sm = P->ctx->stream;
+    const int occ = (int)std::max<size_t>(1, std::min<size_t>(4, (160 * 1024) / P->lds));
+    hipEvent_t e0, e1;
+    SC_HIP(hipEventCreate(&e0));
+    SC_HIP(hipEventCreate(&e1));
+    float best = INFINITY;
+    int bt = P->threads, bg = P->grid;
+    for (int threads : {256, 512}) {
+        for (int per_cu = 1; per_cu <= occ; ++per_cu) {
+            const int grid = (int)std::min<int64_t>((int64_t)P->ctx->n_cu * per_cu, P->nq);
+            float tot = 0.0f;
+            for (int rep = 0; rep < 3; ++rep) {
+                hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, sm, P->counter, 2);
+                SC_HIP(hipEventRecord(e0, sm));
+                hipLaunchKernelGGL(k_search_fused, dim3((unsigned)grid), dim3(threads), P->lds, sm,
+                                   P->fa);
+                SC_CHECK_LAUNCH();
+                SC_HIP(hipEventRecord(e1, sm));
+                SC_HIP(hipEventSynchronize(e1));
+                float ms = 0.0f;
+                SC_HIP(hipEventElapsedTime(&ms, e0, e1));
+                if (rep > 0) tot += ms;        // first run warms caches
+            }
+            if (tot < best) { best = tot; bt = threads; bg = grid; }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    P->threads = bt;
+    P->grid = bg;
+    SC_TRY(plan_capture(P));
+    if (out3) { out3[0] = bt; out3[1] = bg; out3[2] = (int32_t)(best / 2.0f * 1000.0f); }
+    return SC_OK;
+}
+
+extern "C" int sc_plan_destroy(sc_plan *P) {
+    if (!P) return SC_OK;
+    (void)hipSetDevice(P->ctx->device);
+    (void)hipStreamSynchronize(P->ctx->stream);
+    if (P->exec) (void)hipGraphExecDestroy(P->exec);
+    if (P->graph) (void)hipGraphDestroy(P->graph);
+    for (void *b : {(void *)P->counter, (void *)P->flags, (void *)P->qinfo, (void *)P->stats})
+        if (b) (void)hipFree(b);
+    delete P;
     return SC_OK;
 }

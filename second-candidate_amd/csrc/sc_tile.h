@@ -27,7 +27,7 @@ struct Best2 {
 template <bool DESC>
 __device__ __forceinline__ bool precedes(double d, int k, double e, int j) {
     if (d < e) return true;
-    if (d > e) return false;
+    if (!(d == e)) return false;   // d > e, or a NaN: the reference's `>`/`>=` never take it
     return DESC ? (k > j) : (k < j);
 }
 
