@@ -93,6 +93,11 @@ _SIGS = {
     "sc_index_export": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P]),
     "sc_search": (C.c_int, [_P, _P, _P, C.c_int64, C.POINTER(SearchParams), _P, _P, _P,
                             C.POINTER(SearchStats)]),
+    "sc_plan_create": (C.c_int, [_P, _P, C.POINTER(SearchParams), _P, C.c_int64, _P, _P, _P,
+                                 C.POINTER(_P)]),
+    "sc_plan_execute": (C.c_int, [_P, C.POINTER(SearchStats)]),
+    "sc_plan_autotune": (C.c_int, [_P, _P]),
+    "sc_plan_destroy": (C.c_int, [_P]),
     "sc_kmeans": (C.c_int, [_P, _P, C.c_int64, C.c_int32, C.c_int32, C.c_int32,
                             C.c_uint32, C.c_int32, C.c_int32, _P, _P]),
     "sc_residual": (C.c_int, [_P, _P, C.c_int64, C.c_int32, _P, C.c_int32, C.c_int32,
@@ -270,6 +275,45 @@ class Index:
     def close(self):
         if self.h:
             lib().sc_index_destroy(self.h)
+            self.h = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Plan:
+    """A batched search captured in a hipGraph (sc_plan_*): fixed queries and
+    output buffers, replayed by execute()."""
+
+    def __init__(self, index, q, kind, R, w, T, M=0, form=FORM_PLAIN):
+        self.index, self.q = index, q
+        nq = q.shape[0]
+        ctx = index.ctx
+        self.ids = Dev(ctx, (nq, R), np.int32)
+        self.dists = Dev(ctx, (nq, R), np.float32)
+        self.sums = Dev(ctx, (nq,), np.int32)
+        self.params = SearchParams(kind, form, R, w, T, M or index.kc)
+        self.h = _P()
+        _check("sc_plan_create", lib().sc_plan_create(
+            ctx.h, index.h, C.byref(self.params), q.ptr, nq, self.ids.ptr, self.dists.ptr,
+            self.sums.ptr, C.byref(self.h)))
+
+    def execute(self, stats=True):
+        st = SearchStats()
+        _check("sc_plan_execute", lib().sc_plan_execute(self.h, C.byref(st) if stats else None))
+        return st if stats else None
+
+    def autotune(self):
+        out = np.zeros(3, np.int32)
+        _check("sc_plan_autotune", lib().sc_plan_autotune(self.h, out.ctypes.data))
+        return {"threads": int(out[0]), "grid": int(out[1]), "us": int(out[2])}
+
+    def close(self):
+        if self.h:
+            lib().sc_plan_destroy(self.h)
             self.h = _P()
 
     def __del__(self):
