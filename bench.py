@@ -110,7 +110,7 @@ def main():
     st = stats[-1]
     ms_scan = float(np.mean([s.ms_scan for s in stats]))
     cand = st.candidates
-    fused = st.ms_lut == 0.0 and st.ms_rank == 0.0
+    fused = st.ms_rank == 0.0 and st.ms_traverse == 0.0
     # algorithmic bytes of the dominant kernel: the codes of every candidate,
     # plus (fused: the query vectors) and the returned ids and distances
     scan_bytes = cand * mp + args.nq * args.R * 8 + (args.nq * dim * 4 if fused else 0)

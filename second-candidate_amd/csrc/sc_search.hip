@@ -29,6 +29,7 @@ constexpr int kScanThreads = 256;
 constexpr int kMaxFusedThreads = 512;   // fused workgroups run 256 or 512 threads
 constexpr int kBufCap = 2048;      // streaming top-R buffer (entries)
 constexpr int kRunCells = 256;     // cells staged per scan run
+constexpr int kSpec = 16;          // traversal pops run ahead of the L[] loads
 
 // ----------------------------------------------------------------------------
 // Stage 1: LUTs.  out[q][0 .. mc*kc) = diff_qc, out[q][mc*kc .. +mp*kp) = diff_qr
@@ -59,6 +60,69 @@ __global__ void k_lut(const float *__restrict__ Q, int64_t nq, int dim,
         float s = 0.0f;
         for (int t = 0; t < bsp; ++t) s += a[t] * b[(size_t)t * kp];
         dqr[(size_t)q * mp * kp + j] = (norm_r[j] - s) - s;
+    }
+}
+
+// Tiled form of k_lut: a workgroup computes a 64-query x 64-codeword tile of
+// one block's dot products; each thread owns 4 x 4 of them and runs every dot
+// in index order with a separately rounded multiply and add, exactly like
+// k_lut.  Queries are staged transposed in LDS ([t][64]), codewords come from
+// the transposed codebook ([t][K], contiguous in the codeword).
+// grid.x: query tiles, grid.y: output tiles of 64 over mc*kc + mp*kp.
+constexpr int kLutQ = 64, kLutC = 64;
+__global__ __launch_bounds__(256) void k_lut_tiled(
+    const float *__restrict__ Q, int64_t nq, int dim, const float *__restrict__ cqT,
+    const float *__restrict__ pqT, const float *__restrict__ norm_c,
+    const float *__restrict__ norm_r, int mc, int kc, int mp, int kp, float *__restrict__ dqc,
+    float *__restrict__ dqr) {
+    __shared__ float qs[32][kLutQ];      // 32 dims at a time
+    __shared__ float cs[32][kLutC];
+    const int tid = threadIdx.x, tq = tid % 16, tc = tid / 16;
+    const int64_t q0 = (int64_t)blockIdx.x * kLutQ;
+    const int o0 = blockIdx.y * kLutC;            // first output of the tile
+    const int nco = mc * kc;
+    // outputs of one tile never straddle a block: kc and kp are multiples of
+    // 64 on this path (checked by the caller)
+    const bool coarse = o0 < nco;
+    const int blk = coarse ? o0 / kc : (o0 - nco) / kp;
+    const int first = coarse ? o0 - blk * kc : (o0 - nco) - blk * kp;
+    const int K = coarse ? kc : kp;
+    const int blen = coarse ? dim / mc : dim / mp;
+    const float *cb = (coarse ? cqT : pqT) + (size_t)blk * blen * K + first;
+    float acc[4][4];
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) acc[i][j] = 0.0f;
+    for (int t0 = 0; t0 < blen; t0 += 32) {
+        const int tt = min(32, blen - t0);
+        __syncthreads();
+        for (int e = tid; e < 32 * kLutQ; e += 256) {
+            const int t = e / kLutQ, qq = e % kLutQ;
+            qs[t][qq] = (t < tt && q0 + qq < nq) ? Q[(size_t)(q0 + qq) * dim + blk * blen + t0 + t] : 0.0f;
+            cs[t][qq] = t < tt ? cb[(size_t)(t0 + t) * K + qq] : 0.0f;
+        }
+        __syncthreads();
+        for (int t = 0; t < tt; ++t) {
+            const float4 a = *reinterpret_cast<const float4 *>(&qs[t][tq * 4]);
+            const float4 b = *reinterpret_cast<const float4 *>(&cs[t][tc * 4]);
+            const float av[4] = {a.x, a.y, a.z, a.w}, bv[4] = {b.x, b.y, b.z, b.w};
+            for (int i = 0; i < 4; ++i)
+                for (int j = 0; j < 4; ++j) acc[i][j] += av[i] * bv[j];
+        }
+    }
+    for (int i = 0; i < 4; ++i) {
+        const int64_t q = q0 + tq * 4 + i;
+        if (q >= nq) continue;
+        for (int j = 0; j < 4; ++j) {
+            const int o = first + tc * 4 + j;          // codeword within the block
+            const float s = acc[i][j];
+            if (coarse) {
+                const int idx = blk * kc + o;
+                dqc[(size_t)q * nco + idx] = (norm_c[idx] - s) - s;
+            } else {
+                const int idx = blk * kp + o;
+                dqr[(size_t)q * mp * kp + idx] = (norm_r[idx] - s) - s;
+            }
+        }
     }
 }
 
@@ -204,10 +268,14 @@ struct TravArgs {
     float *gheap_v;         // fallback heap storage [nq][w]
     uint32_t *gheap_p;
     int32_t *grow;          // fallback row lengths [nq][kc]
+    int32_t *clen;          // [nq][ccap] cell lengths (cells holds cell starts)
 };
 
 __device__ __forceinline__ int32_t cell_len(const int32_t *L, int64_t bid) {
     return bid > 0 ? L[bid] - L[bid - 1] : L[0];
+}
+__device__ __forceinline__ int32_t cell_start(const int32_t *L, int64_t bid) {
+    return bid > 0 ? L[bid - 1] : 0;
 }
 
 __global__ void k_traverse(TravArgs a, int hcap, int qpw) {
@@ -242,7 +310,8 @@ __global__ void k_traverse(TravArgs a, int hcap, int qpw) {
     h.pos = 0;
     h.overflow = false;
 
-    int32_t *cells = a.cells + (size_t)q * a.ccap;
+    int32_t *cells = a.cells + (size_t)q * a.ccap;      // cell starts
+    int32_t *clen = a.clen + (size_t)q * a.ccap;
     uint32_t *cinfo = a.cinfo + (size_t)q * a.ccap;
     int count = 0, count2 = 0, sum = 0, empty = 0, exhausted = 0, err = 0;
 
@@ -254,7 +323,8 @@ __global__ void k_traverse(TravArgs a, int hcap, int qpw) {
             if (h.pos <= 0) { err = 1; break; }      // pq_pop on an empty heap exits
             int bid = (int)h.pop();
             int l = cell_len(a.L, bid);
-            cells[count] = bid;
+            cells[count] = cell_start(a.L, bid);
+            clen[count] = l;
             cinfo[count] = (uint32_t)bid;
             count++;
             count2++;
@@ -287,7 +357,8 @@ __global__ void k_traverse(TravArgs a, int hcap, int qpw) {
             int l = imi ? cell_len(a.L, bid) : (bid > 0 ? cell_len(a.L, bid) : 0);
             if (l > 0) {
                 if (count >= a.ccap) { err = 2; break; }
-                cells[count] = (int32_t)bid;
+                cells[count] = cell_start(a.L, bid);
+                clen[count] = l;
                 cinfo[count] = (uint32_t)h3 | ((uint32_t)h4 << 16);
                 count++;
                 sum += l;
@@ -330,6 +401,7 @@ struct ScanArgs {
     float *dists;
     int32_t *sums;
     const int32_t *qlist;   // rescan: workgroup b handles query qlist[b]
+    const int32_t *clen;    // [nq][ccap] cell lengths (cells holds cell starts)
 };
 
 struct ScanShared {
@@ -437,18 +509,128 @@ __device__ void sort_buffer(ScanShared &S, int n) {
     }
 }
 
-// Scan + top-R of query q over its cell list (cells/cinfo, ncell cells, sum
-// candidates); dqr/dqc/cells/cinfo may live in LDS or global memory.  All
-// threads of the workgroup call it; `lut` is LDS of mp*kp floats.
+// Scan + top-R of query q over its cell list: cstart/clen/cinfo [ncell]
+// (start of each cell in pid/codes, its length, and h3 | h4 << 16), `sum`
+// candidates in all.  dqr/dqc and the cell arrays may live in LDS or global
+// memory.  All threads of the workgroup call it; `lut` is LDS of mp*kp floats.
+//
+// Each round covers kRound candidates: a thread takes U = kRound / blockDim
+// of them (positions tid, tid + blockDim, ...), walking a cell pointer that
+// only moves forward, so their code loads are independent and in flight
+// together.  Additions keep the reference's order (k ascending).
+constexpr int kRound = 1024;
+
+template <int MPW>   // 32-bit words of codes per candidate (mp = 4 * MPW)
+__device__ __forceinline__ void load_codes(const uint8_t *codes, int vidx, uint32_t *w) {
+    if (MPW == 2) {
+        const uint2 v = *reinterpret_cast<const uint2 *>(codes + (size_t)vidx * 8);
+        w[0] = v.x; w[1] = v.y;
+    } else if (MPW == 4) {
+        const uint4 v = *reinterpret_cast<const uint4 *>(codes + (size_t)vidx * 16);
+        w[0] = v.x; w[1] = v.y; w[2] = v.z; w[3] = v.w;
+    } else {
+        const uint32_t *p = reinterpret_cast<const uint32_t *>(codes + (size_t)vidx * 4 * MPW);
+        for (int i = 0; i < MPW; ++i) w[i] = p[i];
+    }
+}
+
+template <int MPW>
+__device__ void scan_run_packed(const ScanArgs &a, ScanShared &S, const float *lut, float d0,
+                                int rn, int rtotal, int base_pos) {
+    const int tid = threadIdx.x, NT = blockDim.x, U = kRound / NT;
+    const int kp = a.kp;
+    int cur = 0;
+    for (int p0 = 0; p0 < rtotal; p0 += kRound) {
+        if (S.nb > kBufCap - kRound) compact(S, a.R);
+        uint32_t cw[4][MPW];
+        int vx[4];
+        for (int u = 0; u < U; ++u) {
+            const int p = p0 + u * NT + tid;
+            vx[u] = -1;
+            if (p < rtotal) {
+                while (cur + 1 < rn && S.c_pref[cur + 1] <= p) ++cur;
+                vx[u] = S.c_start[cur] + (p - S.c_pref[cur]);
+                load_codes<MPW>(a.codes, vx[u], cw[u]);
+            }
+        }
+        for (int u = 0; u < U; ++u) {
+            if (vx[u] < 0) continue;
+            float d = d0;
+#pragma unroll
+            for (int w4 = 0; w4 < MPW; ++w4) {
+                const uint32_t wv = cw[u][w4];
+#pragma unroll
+                for (int b = 0; b < 4; ++b) d += lut[(w4 * 4 + b) * kp + ((wv >> (8 * b)) & 255u)];
+            }
+            const int p = p0 + u * NT + tid;
+            const uint64_t key = ((uint64_t)sc_float_key(d) << 32) | (uint32_t)(base_pos + p);
+            if (key < S.thr) {
+                const int o = atomicAdd(&S.nb, 1);
+                S.key[o] = key;
+                S.idx[o] = vx[u];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Any mp; IMI sequences read dot_cr rows from global memory.
+__device__ void scan_run_generic(const ScanArgs &a, ScanShared &S, const float *lut,
+                                 const float *dqc, float qsum, int rn, int rtotal, int base_pos) {
+    const int tid = threadIdx.x, NT = blockDim.x, U = kRound / NT;
+    const int mp = a.mp, kp = a.kp, kc = a.kc, half = mp >> 1;
+    const bool imi = a.kind == SC_KIND_IMI;
+    const int bs = kp * mp / a.mc;
+    int cur = 0;
+    for (int p0 = 0; p0 < rtotal; p0 += kRound) {
+        if (S.nb > kBufCap - kRound) compact(S, a.R);
+        for (int u = 0; u < U; ++u) {
+            const int p = p0 + u * NT + tid;
+            if (p >= rtotal) continue;
+            while (cur + 1 < rn && S.c_pref[cur + 1] <= p) ++cur;
+            const int vidx = S.c_start[cur] + (p - S.c_pref[cur]);
+            const uint8_t *cc = a.codes + (size_t)vidx * mp;
+            const uint32_t ci = S.c_info[cur];
+            float d;
+            if (!imi) {
+                d = qsum + dqc[ci & 0xFFFFu];
+                for (int k = 0; k < mp; ++k) d += lut[k * kp + cc[k]];
+            } else {
+                const int h3 = (int)(ci & 0xFFFFu), h4 = (int)(ci >> 16);
+                d = (qsum + dqc[h3]) + dqc[kc + h4];
+                const float *r1 = a.dot_cr + (size_t)h3 * bs;
+                const float *r2 = a.dot_cr + (size_t)(h4 + kc - 1) * bs;
+                for (int k = 0; k < half; ++k) {
+                    int b = k * kp + cc[k];
+                    d += lut[b];
+                    d += r1[b];
+                }
+                for (int k = half; k < mp; ++k) {
+                    int b = k * kp + cc[k];
+                    d += lut[b];
+                    d += r2[b];
+                }
+            }
+            const uint64_t key = ((uint64_t)sc_float_key(d) << 32) | (uint32_t)(base_pos + p);
+            if (key < S.thr) {
+                const int o = atomicAdd(&S.nb, 1);
+                S.key[o] = key;
+                S.idx[o] = vidx;
+            }
+        }
+        __syncthreads();
+    }
+}
+
 __device__ void scan_query(const ScanArgs &a, ScanShared &S, float *lut, int64_t q,
                            const float *dqr, const float *dqc, float qsum,
-                           const int32_t *cells, const uint32_t *cinfo, int ncell, int sum) {
+                           const int32_t *cstart, const int32_t *clen, const uint32_t *cinfo,
+                           int ncell, int sum) {
     const int tid = threadIdx.x;
-    const int mp = a.mp, kp = a.kp, kc = a.kc;
+    const int mp = a.mp, kp = a.kp;
     const int lutn = mp * kp;
     const bool imi = a.kind == SC_KIND_IMI;
     const int bs = kp * mp / a.mc;
-    const int half = mp >> 1;
     if (tid == 0) { S.nb = 0; S.thr = ~(uint64_t)0; }
     if (imi)
         for (int i = tid; i < lutn; i += blockDim.x) lut[i] = dqr[i];
@@ -461,18 +643,14 @@ __device__ void scan_query(const ScanArgs &a, ScanShared &S, float *lut, int64_t
         // a run: consecutive cells sharing h3 (IVF/SC), or up to kRunCells (IMI)
         if (tid == 0) {
             int n = 0, acc = 0;
-            const int h3first = (int)(cinfo[c0] & 0xFFFFu);
+            const uint32_t h3first = cinfo[c0] & 0xFFFFu;
             while (c0 + n < ncell && n < kRunCells) {
-                uint32_t ci = cinfo[c0 + n];
-                if (!imi && (int)(ci & 0xFFFFu) != h3first) break;
-                if (a.kind == SC_KIND_IVFADC && n > 0) break;
-                int64_t bid = cells[c0 + n];
-                int st = bid > 0 ? a.L[bid - 1] : 0;
-                int l = a.L[bid] - st;
-                S.c_start[n] = st;
+                const uint32_t ci = cinfo[c0 + n];
+                if (!imi && (ci & 0xFFFFu) != h3first) break;
+                S.c_start[n] = cstart[c0 + n];
                 S.c_pref[n] = acc;
                 S.c_info[n] = ci;
-                acc += l;
+                acc += clen[c0 + n];
                 ++n;
             }
             S.c_pref[n] = acc;
@@ -481,8 +659,10 @@ __device__ void scan_query(const ScanArgs &a, ScanShared &S, float *lut, int64_t
         __syncthreads();
         const int rn = S.run_cells;
         const int rtotal = S.c_pref[rn];
+        float d0 = 0.0f;
         if (!imi) {
             const int h3 = (int)(S.c_info[0] & 0xFFFFu);
+            d0 = qsum + dqc[h3];
             if (h3 != lut_row) {
                 const float *row = a.dot_cr + (size_t)h3 * bs;
                 for (int i = tid; i < lutn; i += blockDim.x) lut[i] = dqr[i] + row[i];
@@ -490,58 +670,12 @@ __device__ void scan_query(const ScanArgs &a, ScanShared &S, float *lut, int64_t
             }
         }
         __syncthreads();
-        for (int p0 = 0; p0 < rtotal; p0 += blockDim.x) {
-            if (S.nb > kBufCap - (int)blockDim.x) compact(S, a.R);
-            const int p = p0 + tid;
-            if (p < rtotal) {
-                int lo = 0, hi = rn - 1;          // last cell with c_pref <= p
-                while (lo < hi) {
-                    int mid = (lo + hi + 1) >> 1;
-                    if (S.c_pref[mid] <= p) lo = mid; else hi = mid - 1;
-                }
-                const int off = p - S.c_pref[lo];
-                const int vidx = S.c_start[lo] + off;
-                const uint8_t *cc = a.codes + (size_t)vidx * mp;
-                const uint32_t ci = S.c_info[lo];
-                float d;
-                if (!imi) {
-                    d = qsum + dqc[ci & 0xFFFFu];
-                    if ((mp & 3) == 0) {      // 4 codes per 32-bit load, same add order
-                        const uint32_t *cw = reinterpret_cast<const uint32_t *>(cc);
-                        for (int w4 = 0; w4 < (mp >> 2); ++w4) {
-                            const uint32_t wv = cw[w4];
-#pragma unroll
-                            for (int b = 0; b < 4; ++b)
-                                d += lut[(w4 * 4 + b) * kp + ((wv >> (8 * b)) & 255u)];
-                        }
-                    } else {
-                        for (int k = 0; k < mp; ++k) d += lut[k * kp + cc[k]];
-                    }
-                } else {
-                    const int h3 = (int)(ci & 0xFFFFu), h4 = (int)(ci >> 16);
-                    d = (qsum + dqc[h3]) + dqc[kc + h4];
-                    const float *r1 = a.dot_cr + (size_t)h3 * bs;
-                    const float *r2 = a.dot_cr + (size_t)(h4 + kc - 1) * bs;
-                    for (int k = 0; k < half; ++k) {
-                        int b = k * kp + cc[k];
-                        d += lut[b];
-                        d += r1[b];
-                    }
-                    for (int k = half; k < mp; ++k) {
-                        int b = k * kp + cc[k];
-                        d += lut[b];
-                        d += r2[b];
-                    }
-                }
-                const uint64_t key = ((uint64_t)sc_float_key(d) << 32) | (uint32_t)(base_pos + p);
-                if (key < S.thr) {
-                    int o = atomicAdd(&S.nb, 1);
-                    S.key[o] = key;
-                    S.idx[o] = vidx;
-                }
-            }
-            __syncthreads();
-        }
+        if (!imi && mp == 8)
+            scan_run_packed<2>(a, S, lut, d0, rn, rtotal, base_pos);
+        else if (!imi && mp == 16)
+            scan_run_packed<4>(a, S, lut, d0, rn, rtotal, base_pos);
+        else
+            scan_run_generic(a, S, lut, dqc, qsum, rn, rtotal, base_pos);
         base_pos += rtotal;
         c0 += rn;
         __syncthreads();
@@ -586,7 +720,8 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_topk(ScanArgs a) {
     float *lut = reinterpret_cast<float *>(smem + sizeof(ScanShared));   // [mp*kp]
     const int64_t q = a.qlist ? a.qlist[blockIdx.x] : blockIdx.x;
     scan_query(a, S, lut, q, a.dqr + (size_t)q * a.mp * a.kp, a.dqc + (size_t)q * a.mc * a.kc,
-               a.qs[q * 4], a.cells + (size_t)q * a.ccap, a.cinfo + (size_t)q * a.ccap,
+               a.qs[q * 4], a.cells + (size_t)q * a.ccap, a.clen + (size_t)q * a.ccap,
+               a.cinfo + (size_t)q * a.ccap,
                a.qinfo[q * 8 + 0], a.qinfo[q * 8 + 1]);
 }
 
@@ -606,6 +741,7 @@ struct FusedArgs {
     int32_t *flags;     // [nq] 1 = finish with the staged kernels
     int *nflag;
     int32_t *qinfo;     // [nq][8] statistics (same fields as the staged path)
+    const float *lut_dqc, *lut_dqr;   // [nq][mc*kc], [nq][mp*kp] or null: compute here
 };
 
 struct FusedLayout {
@@ -621,8 +757,8 @@ __host__ __device__ inline FusedLayout fused_layout(int dim, int mc, int kc, int
     if ((size_t)npow2 * 8 > a) a = (size_t)npow2 * 8;
     if (heap > a) a = heap;
     L.a = 0;
-    L.b = up(a);                                   // cells + cinfo [ccap] each
-    L.c = up(L.b + (size_t)ccap * 8);              // sorted v [kc] + idx [kc]
+    L.b = up(a);                                   // starts, lengths, info [ccap] each
+    L.c = up(L.b + (size_t)ccap * 12);             // sorted v [kc] + idx [kc]
     L.d = up(L.c + (size_t)kc * 8);                // dqc [mc*kc] + dqr [mp*kp]
     L.e = up(L.d + 4 * ((size_t)mc * kc + (size_t)mp * kp));   // lut [mp*kp]
     L.f = up(L.e + 4 * (size_t)mp * kp);           // query [dim]
@@ -637,8 +773,9 @@ __global__ __launch_bounds__(kMaxFusedThreads) void k_search_fused(FusedArgs a) 
     const FusedLayout Lay = fused_layout(dim, mc, kc, mp, kp, a.hcap, a.ccap, a.npow2);
     ScanShared &S = *reinterpret_cast<ScanShared *>(smem + Lay.a);
     uint64_t *keys = reinterpret_cast<uint64_t *>(smem + Lay.a);
-    int32_t *cells = reinterpret_cast<int32_t *>(smem + Lay.b);
-    uint32_t *cinfo = reinterpret_cast<uint32_t *>(cells + a.ccap);
+    int32_t *cells = reinterpret_cast<int32_t *>(smem + Lay.b);     // cell starts
+    int32_t *clen = cells + a.ccap;
+    uint32_t *cinfo = reinterpret_cast<uint32_t *>(clen + a.ccap);
     float *sv = reinterpret_cast<float *>(smem + Lay.c);
     int32_t *si = reinterpret_cast<int32_t *>(sv + kc);
     float *dqc = reinterpret_cast<float *>(smem + Lay.d);
@@ -664,7 +801,10 @@ __global__ __launch_bounds__(kMaxFusedThreads) void k_search_fused(FusedArgs a) 
             for (int i = 0; i < dim; ++i) { float t = qv[i]; s += t * t; }
             s_qsum = s;
         }
-        for (int j = tid; j < mc * kc + mp * kp; j += blockDim.x) {
+        if (a.lut_dqc) {   // precomputed by k_lut_tiled
+            for (int j = tid; j < mc * kc; j += blockDim.x) dqc[j] = a.lut_dqc[(size_t)q * mc * kc + j];
+            for (int j = tid; j < mp * kp; j += blockDim.x) dqr[j] = a.lut_dqr[(size_t)q * mp * kp + j];
+        } else for (int j = tid; j < mc * kc + mp * kp; j += blockDim.x) {
             if (j < mc * kc) {     // a.cq / a.pq: transposed codebooks
                 const int blk = j / kc, k = j - blk * kc;
                 const float *b = a.cq + (size_t)blk * bsc * kc + k;
@@ -722,16 +862,26 @@ __global__ __launch_bounds__(kMaxFusedThreads) void k_search_fused(FusedArgs a) 
             if (!sc) {
                 h.cap_ref = kc;
                 for (int j = 0; j < kc; ++j) h.insert(sv[j], (uint32_t)j);
-                for (int i = 0; i < a.w && !h.overflow; ++i) {
-                    if (h.pos <= 0) { flag = 1; break; }
-                    int bid = (int)h.pop();
-                    if (count >= a.ccap) { flag = 1; break; }
-                    cells[count] = bid;
-                    cinfo[count] = (uint32_t)bid;
-                    count++;
-                    count2++;
-                    sum += cell_len(sa.L, bid);
-                    if (sum >= a.T) break;
+                bool done = h.overflow;
+                for (int i0 = 0; i0 < a.w && !done; i0 += kSpec) {
+                    int sbid[kSpec];
+                    const int nsp = min(kSpec, a.w - i0);
+                    for (int i = 0; i < nsp; ++i) sbid[i] = h.pos > 0 ? (int)h.pop() : -1;
+                    int32_t lhi[kSpec], llo[kSpec];
+                    for (int i = 0; i < nsp; ++i) {
+                        lhi[i] = sbid[i] >= 0 ? sa.L[sbid[i]] : 0;
+                        llo[i] = sbid[i] > 0 ? sa.L[sbid[i] - 1] : 0;
+                    }
+                    for (int i = 0; i < nsp; ++i) {
+                        if (sbid[i] < 0 || count >= a.ccap) { flag = 1; done = true; break; }
+                        cells[count] = llo[i];
+                        clen[count] = lhi[i] - llo[i];
+                        cinfo[count] = (uint32_t)sbid[i];
+                        count++;
+                        count2++;
+                        sum += lhi[i] - llo[i];
+                        if (sum >= a.T) { done = true; break; }
+                    }
                 }
             } else {
                 h.cap_ref = a.w;
@@ -740,31 +890,53 @@ __global__ __launch_bounds__(kMaxFusedThreads) void k_search_fused(FusedArgs a) 
                 const float d0 = sv[0] + sv[1];
                 h.insert(d0, 1u);
                 h.insert(d0, 1u << 16);
-                while (count < a.w && sum < a.T) {
-                    if (h.overflow) break;
-                    if (h.pos <= 0) { exhausted = 1; break; }
-                    const uint32_t pk = h.pop();
-                    const int h1 = (int)(pk >> 16), h2 = (int)(pk & 0xFFFFu);
-                    const int h3 = si[h1], h4 = si[h2];
-                    const int64_t bid = (int64_t)h3 * kc + h4;
-                    const int l = bid > 0 ? cell_len(sa.L, bid) : 0;
-                    if (l > 0) {
-                        if (count >= a.ccap) { flag = 1; break; }
-                        cells[count] = (int32_t)bid;
-                        cinfo[count] = (uint32_t)h3 | ((uint32_t)h4 << 16);
-                        count++;
-                        sum += l;
-                    } else {
-                        empty++;
-                    }
-                    rowlen[h1] = h2 + 1;
-                    count2++;
-                    if (sum < a.T) {
+                // The pops and pushes do not depend on the cell sizes until the
+                // loop stops, so pops run ahead in batches of kSpec, their L[]
+                // entries load together, and a sequential commit stops exactly
+                // where the reference does (pushes past that point are moot).
+                bool done = (count >= a.w || sum >= a.T);
+                while (!done) {
+                    int64_t sbid[kSpec];
+                    uint32_t sinfo[kSpec];
+                    int nsp = 0;
+                    bool dry = false;
+                    while (nsp < kSpec && !h.overflow) {
+                        if (h.pos <= 0) { dry = true; break; }
+                        const uint32_t pk = h.pop();
+                        const int h1 = (int)(pk >> 16), h2 = (int)(pk & 0xFFFFu);
+                        const int h3 = si[h1], h4 = si[h2];
+                        sbid[nsp] = (int64_t)h3 * kc + h4;
+                        sinfo[nsp] = (uint32_t)h3 | ((uint32_t)h4 << 16);
+                        ++nsp;
+                        rowlen[h1] = h2 + 1;
                         if (h1 < kc - 1 && (h2 == 0 || (h2 - 1) < rowlen[h1 + 1]))
                             h.insert(sv[h1 + 1] + sv[h2], ((uint32_t)(h1 + 1) << 16) | (uint32_t)h2);
                         if (h2 < kc - 1 && (h1 == 0 || (h2 + 1) < rowlen[h1 - 1]))
                             h.insert(sv[h1] + sv[h2 + 1], ((uint32_t)h1 << 16) | (uint32_t)(h2 + 1));
                     }
+                    int32_t lhi[kSpec], llo[kSpec];
+                    for (int i = 0; i < nsp; ++i) {
+                        lhi[i] = sbid[i] > 0 ? sa.L[sbid[i]] : 0;
+                        llo[i] = sbid[i] > 0 ? sa.L[sbid[i] - 1] : 0;
+                    }
+                    for (int i = 0; i < nsp && !done; ++i) {
+                        const int l = lhi[i] - llo[i];
+                        if (l > 0) {
+                            if (count >= a.ccap) { flag = 1; done = true; break; }
+                            cells[count] = llo[i];
+                            clen[count] = l;
+                            cinfo[count] = sinfo[i];
+                            count++;
+                            sum += l;
+                        } else {
+                            empty++;
+                        }
+                        count2++;
+                        if (count >= a.w || sum >= a.T) done = true;
+                    }
+                    if (done) break;
+                    if (h.overflow) break;                 // flagged below
+                    if (dry) { exhausted = 1; break; }
                 }
             }
             if (h.overflow) flag = 1;
@@ -780,7 +952,7 @@ __global__ __launch_bounds__(kMaxFusedThreads) void k_search_fused(FusedArgs a) 
         }
         __syncthreads();
         if (!s_flag)
-            scan_query(sa, S, lut, q, dqr, dqc, qsum, cells, cinfo, s_ncell, s_sum);
+            scan_query(sa, S, lut, q, dqr, dqc, qsum, cells, clen, cinfo, s_ncell, s_sum);
         __syncthreads();
     }
 }
@@ -848,7 +1020,7 @@ __global__ void k_reduce_stats(const int32_t *__restrict__ qinfo, int64_t nq,
 struct Plan {
     int64_t chunk;      // queries per pass
     int ccap;
-    size_t off_qs, off_dqc, off_dqr, off_sv, off_si, off_cells, off_cinfo, off_qinfo,
+    size_t off_qs, off_dqc, off_dqr, off_sv, off_si, off_cells, off_clen, off_cinfo, off_qinfo,
         off_list, off_stats, total;
 };
 
@@ -859,7 +1031,7 @@ Plan make_plan(const sc_index *ix, const sc_search_params *p, int64_t nq) {
     if (ccap < 1) ccap = 1;
     P.ccap = (int)ccap;
     const size_t per = 16 + 4ull * ix->mc * ix->kc + 4ull * ix->mp * ix->kp +
-                       8ull * lists * ix->kc + 8ull * ccap + 32 + 4;
+                       8ull * lists * ix->kc + 12ull * ccap + 32 + 4;
     int64_t chunk = (int64_t)((size_t)1 << 30) / (int64_t)per;
     chunk = std::max<int64_t>(1, std::min<int64_t>(chunk, std::max<int64_t>(nq, 1)));
     chunk = std::min<int64_t>(chunk, 65536);
@@ -872,6 +1044,7 @@ Plan make_plan(const sc_index *ix, const sc_search_params *p, int64_t nq) {
     P.off_sv = take(4ull * lists * ix->kc * chunk);
     P.off_si = take(4ull * lists * ix->kc * chunk);
     P.off_cells = take(4ull * ccap * chunk);
+    P.off_clen = take(4ull * ccap * chunk);
     P.off_cinfo = take(4ull * ccap * chunk);
     P.off_qinfo = take(32ull * chunk);
     P.off_list = take(4ull * chunk + 4);
@@ -909,8 +1082,11 @@ static int search_fused(sc_ctx *ctx, const sc_index *ix, const float *d_q, int64
     const FusedLayout Lay = fused_layout(ix->dim, ix->mc, kc, ix->mp, ix->kp, hcap, ccap, npow2);
     if (Lay.total > 150 * 1024) return SC_ERR_TOO_LARGE;
 
+    const bool tiled_lut = kc % kLutC == 0 && ix->kp % kLutC == 0;
+    const size_t lut_per = (size_t)ix->mc * kc + (size_t)ix->mp * ix->kp;
     size_t o_cnt = 0, o_flags = 256, o_qinfo = o_flags + ((4 * (size_t)nq + 255) & ~(size_t)255),
-           o_stats = o_qinfo + 32 * (size_t)nq, total = o_stats + 64;
+           o_stats = o_qinfo + 32 * (size_t)nq, o_lut = o_stats + 256,
+           total = o_lut + (tiled_lut ? 4 * lut_per * (size_t)nq : 0);
     unsigned char *s;
     SC_TRY(sc_scratch(ctx, total, (void **)&s));
     int *counter = (int *)(s + o_cnt);            // [0] work counter, [1] flagged
@@ -937,12 +1113,26 @@ static int search_fused(sc_ctx *ctx, const sc_index *ix, const float *d_q, int64
     fa.Q = d_q; fa.cq = ix->cqT; fa.pq = ix->pqT; fa.norm_c = ix->norm_c; fa.norm_r = ix->norm_r;
     fa.nq = nq; fa.w = p->w; fa.T = p->T; fa.hcap = hcap; fa.ccap = ccap; fa.npow2 = npow2;
     fa.counter = counter; fa.flags = flags; fa.nflag = counter + 1; fa.qinfo = qinfo;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
+    fa.lut_dqc = fa.lut_dqr = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr, el = nullptr;
     if (st) {
         SC_HIP(hipEventCreate(&e0));
         SC_HIP(hipEventCreate(&e1));
-        SC_HIP(hipEventRecord(e0, sm));
+        SC_HIP(hipEventCreate(&el));
+        SC_HIP(hipEventRecord(el, sm));
     }
+    if (tiled_lut) {
+        float *lq = (float *)(s + o_lut);
+        fa.lut_dqc = lq;
+        fa.lut_dqr = lq + (size_t)nq * ix->mc * kc;
+        hipLaunchKernelGGL(k_lut_tiled,
+                           dim3((unsigned)sc_cdiv(nq, kLutQ), (unsigned)(lut_per / kLutC)),
+                           dim3(256), 0, sm, d_q, nq, ix->dim, ix->cqT, ix->pqT, ix->norm_c,
+                           ix->norm_r, ix->mc, kc, ix->mp, ix->kp, lq,
+                           lq + (size_t)nq * ix->mc * kc);
+        SC_CHECK_LAUNCH();
+    }
+    if (st) SC_HIP(hipEventRecord(e0, sm));
     hipLaunchKernelGGL(k_search_fused, dim3((unsigned)grid), dim3(kScanThreads), Lay.total, sm,
                        fa);
     SC_CHECK_LAUNCH();
@@ -959,15 +1149,18 @@ static int search_fused(sc_ctx *ctx, const sc_index *ix, const float *d_q, int64
     if (st) SC_HIP(hipMemcpyAsync(h, stats, 64, hipMemcpyDeviceToHost, sm));
     SC_HIP(hipStreamSynchronize(sm));
     if (st) {
-        float ms = 0.0f;
+        float ms = 0.0f, ml = 0.0f;
         SC_HIP(hipEventElapsedTime(&ms, e0, e1));
+        SC_HIP(hipEventElapsedTime(&ml, el, e0));
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
+        (void)hipEventDestroy(el);
         st->queries = nq;
         st->candidates = (int64_t)h[0];
         st->cells = (int64_t)h[1];
         st->visited = (int64_t)h[2];
         st->exhausted = (int64_t)h[3];
+        st->ms_lut = ml;
         st->ms_scan = ms;
     }
     return finish_flagged(ctx, ix, p, d_q, nq, flags, nflag, d_ids, d_dists, d_sums, st);
@@ -1036,7 +1229,7 @@ extern "C" int sc_search(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
         sc_set_error("sc_search: bad arguments");
         return SC_ERR_ARG;
     }
-    if (p->R < 1 || p->R > kBufCap - kMaxFusedThreads || p->w < 1 || p->T < 0 ||
+    if (p->R < 1 || p->R > kBufCap - kRound || p->w < 1 || p->T < 0 ||
         (p->form != SC_FORM_PLAIN && p->form != SC_FORM_MIXED)) {
         sc_set_error("sc_search: bad parameters (R=%d w=%d T=%d form=%d)", p->R, p->w, p->T,
                      p->form);
@@ -1081,6 +1274,7 @@ static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
     float *qs = (float *)(s + P.off_qs), *dqc = (float *)(s + P.off_dqc),
           *dqr = (float *)(s + P.off_dqr), *sv = (float *)(s + P.off_sv);
     int32_t *si = (int32_t *)(s + P.off_si), *cells = (int32_t *)(s + P.off_cells);
+    int32_t *clen = (int32_t *)(s + P.off_clen);
     uint32_t *cinfo = (uint32_t *)(s + P.off_cinfo);
     int32_t *qinfo = (int32_t *)(s + P.off_qinfo), *list = (int32_t *)(s + P.off_list);
     unsigned long long *stats = (unsigned long long *)(s + P.off_stats);
@@ -1126,7 +1320,7 @@ static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
         SC_CHECK_LAUNCH();
         mark();
         TravArgs ta{kind, kc, p->w, p->T, P.ccap, sv, si, ix->L, cells, cinfo, qinfo,
-                    nullptr, cn, nullptr, nullptr, nullptr};
+                    nullptr, cn, nullptr, nullptr, nullptr, clen};
         if (hcap > 0) {
             size_t lds = (size_t)qpw * lane_bytes;
             hipLaunchKernelGGL(k_traverse, dim3((unsigned)sc_cdiv(cn, qpw)), dim3(qpw), lds,
@@ -1137,7 +1331,7 @@ static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
         ScanArgs sa{kind, ix->dim, ix->mc, kc, ix->mp, ix->kp, p->R, P.ccap, qs, dqc, dqr,
                     ix->dot_cr, ix->L, ix->pid, ix->codes, cells, cinfo, qinfo,
                     d_ids + (size_t)q0 * p->R, d_dists + (size_t)q0 * p->R,
-                    d_sums ? d_sums + q0 : nullptr, nullptr};
+                    d_sums ? d_sums + q0 : nullptr, nullptr, clen};
         const size_t lds = sizeof(ScanShared) + sizeof(float) * ix->mp * ix->kp;
         if (hcap > 0) {
             hipLaunchKernelGGL(k_scan_topk, dim3((unsigned)cn), dim3(kScanThreads), lds,
@@ -1244,9 +1438,20 @@ struct sc_plan {
     int *counter;                  // [0] work counter, [1] flagged
     int32_t *flags, *qinfo;
     unsigned long long *stats;
+    float *lut;                    // [nq][mc*kc + mp*kp] (tiled LUT kernel) or null
     hipGraph_t graph;
     hipGraphExec_t exec;
 };
+
+static void plan_lut(sc_plan *P) {
+    if (!P->lut) return;
+    const sc_index *ix = P->ix;
+    const int lut_per = ix->mc * ix->kc + ix->mp * ix->kp;
+    hipLaunchKernelGGL(k_lut_tiled, dim3((unsigned)sc_cdiv(P->nq, kLutQ), (unsigned)(lut_per / kLutC)),
+                       dim3(256), 0, P->ctx->stream, P->d_q, P->nq, ix->dim, ix->cqT, ix->pqT,
+                       ix->norm_c, ix->norm_r, ix->mc, ix->kc, ix->mp, ix->kp, P->lut,
+                       P->lut + (size_t)P->nq * ix->mc * ix->kc);
+}
 
 static int plan_capture(sc_plan *P) {
     if (P->exec) { (void)hipGraphExecDestroy(P->exec); P->exec = nullptr; }
@@ -1255,6 +1460,7 @@ static int plan_capture(sc_plan *P) {
     SC_HIP(hipStreamBeginCapture(sm, hipStreamCaptureModeThreadLocal));
     // the work counter is cleared by a kernel, not a memset node
     hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, sm, P->counter, 2);
+    plan_lut(P);
     hipLaunchKernelGGL(k_search_fused, dim3((unsigned)P->grid), dim3(P->threads), P->lds, sm,
                        P->fa);
     hipError_t le = hipGetLastError();
@@ -1298,6 +1504,8 @@ extern "C" int sc_plan_create(sc_ctx *ctx, const sc_index *ix, const sc_search_p
               hipMalloc(&P->flags, 4 * (size_t)nq) == hipSuccess &&
               hipMalloc(&P->qinfo, 32 * (size_t)nq) == hipSuccess &&
               hipMalloc(&P->stats, 64) == hipSuccess;
+    if (ok && kc % kLutC == 0 && ix->kp % kLutC == 0)
+        ok = hipMalloc(&P->lut, 4 * (size_t)nq * (ix->mc * kc + ix->mp * ix->kp)) == hipSuccess;
     if (!ok) { sc_plan_destroy(P); sc_set_error("sc_plan_create: allocation failed"); return SC_ERR_NOMEM; }
     FusedArgs &fa = P->fa;
     fa.s = ScanArgs{kind,      ix->dim, ix->mc,  kc,      ix->mp,     ix->kp, p->R,
@@ -1306,6 +1514,8 @@ extern "C" int sc_plan_create(sc_ctx *ctx, const sc_index *ix, const sc_search_p
     fa.Q = d_q; fa.cq = ix->cqT; fa.pq = ix->pqT; fa.norm_c = ix->norm_c; fa.norm_r = ix->norm_r;
     fa.nq = nq; fa.w = p->w; fa.T = p->T; fa.hcap = hcap; fa.ccap = ccap; fa.npow2 = npow2;
     fa.counter = P->counter; fa.flags = P->flags; fa.nflag = P->counter + 1; fa.qinfo = P->qinfo;
+    fa.lut_dqc = P->lut;
+    fa.lut_dqr = P->lut ? P->lut + (size_t)nq * ix->mc * kc : nullptr;
     if (hipFuncSetAttribute((const void *)k_search_fused, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)P->lds) != hipSuccess) {
         sc_plan_destroy(P);
@@ -1376,6 +1586,7 @@ extern "C" int sc_plan_autotune(sc_plan *P, int32_t *out3) {
     SC_HIP(hipEventCreate(&e1));
     float best = INFINITY;
     int bt = P->threads, bg = P->grid;
+    plan_lut(P);                       // the fused kernel reads this batch's LUTs
     for (int threads : {256, 512}) {
         for (int per_cu = 1; per_cu <= occ; ++per_cu) {
             const int grid = (int)std::min<int64_t>((int64_t)P->ctx->n_cu * per_cu, P->nq);
@@ -1410,7 +1621,8 @@ extern "C" int sc_plan_destroy(sc_plan *P) {
     (void)hipStreamSynchronize(P->ctx->stream);
     if (P->exec) (void)hipGraphExecDestroy(P->exec);
     if (P->graph) (void)hipGraphDestroy(P->graph);
-    for (void *b : {(void *)P->counter, (void *)P->flags, (void *)P->qinfo, (void *)P->stats})
+    for (void *b : {(void *)P->counter, (void *)P->flags, (void *)P->qinfo, (void *)P->stats,
+                    (void *)P->lut})
         if (b) (void)hipFree(b);
     delete P;
     return SC_OK;
