@@ -48,6 +48,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="target CPU work for the oracle baseline (0 = skip)")
     ap.add_argument("--recall-sample", type=int, default=100)
+    ap.add_argument("--plan", type=int, default=1,
+                    help="1: time the autotuned hipGraph plan (sc_plan_*); 0: sc_search calls")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -92,9 +94,30 @@ def main():
     dists = S.Dev(ctx, (args.nq, args.R), np.float32)
     sums = S.Dev(ctx, (args.nq,), np.int32)
 
-    def step():
-        return ix.search(q, S.KIND_SC, args.R, args.w, args.T, form=S.FORM_PLAIN, ids=ids,
-                         dists=dists, sums=sums, stats=True)[3]
+    # eager calls with statistics: the fused kernel's own duration (HIP events
+    # on the context stream around that launch) and its per-phase times
+    eager = [ix.search(q, S.KIND_SC, args.R, args.w, args.T, form=S.FORM_PLAIN, ids=ids,
+                       dists=dists, sums=sums, stats=True)[3] for _ in range(4)][1:]
+    st0 = eager[-1]
+    ms_kernel = float(np.mean([e.ms_scan for e in eager]))
+    phases = [round(float(v), 4) for v in st0.fused_phase_ms]
+    log("eager call: kernel %.3f ms, lut %.3f ms; fused phases per workgroup (ms): "
+        "lut %.3f sort %.3f traverse %.3f scan %.3f select %.3f"
+        % (st0.ms_scan, st0.ms_lut, *phases))
+
+    plan, tune = None, None
+    if args.plan:
+        plan = S.Plan(ix, q, S.KIND_SC, args.R, args.w, args.T, form=S.FORM_PLAIN)
+        tune = plan.autotune()
+        log("autotuned plan: %s" % tune)
+        ids, dists, sums = plan.ids, plan.dists, plan.sums
+
+        def step():
+            return plan.execute(stats=True)
+    else:
+        def step():
+            return ix.search(q, S.KIND_SC, args.R, args.w, args.T, form=S.FORM_PLAIN, ids=ids,
+                             dists=dists, sums=sums, stats=True)[3]
 
     for _ in range(args.warmup):
         step()
@@ -108,16 +131,16 @@ def main():
     ms_step = elapsed * 1e3 / args.steps
     qps = args.nq * args.steps / elapsed
     st = stats[-1]
-    ms_scan = float(np.mean([s.ms_scan for s in stats]))
     cand = st.candidates
-    fused = st.ms_rank == 0.0 and st.ms_traverse == 0.0
+    fused = st0.ms_rank == 0.0 and st0.ms_traverse == 0.0
     # algorithmic bytes of the dominant kernel: the codes of every candidate,
     # plus (fused: the query vectors) and the returned ids and distances
     scan_bytes = cand * mp + args.nq * args.R * 8 + (args.nq * dim * 4 if fused else 0)
-    achieved = scan_bytes / (ms_scan * 1e-3) / 1e9
+    achieved = scan_bytes / (ms_kernel * 1e-3) / 1e9
     log("batched: %.1f ms/step, %.0f QPS; stages lut %.3f rank %.3f trav %.3f scan %.3f ms; "
-        "%.0f candidates/query" % (ms_step, qps, st.ms_lut, st.ms_rank, st.ms_traverse,
-                                   st.ms_scan, cand / args.nq))
+        "%.0f candidates, %.1f cells, %.1f visited per query"
+        % (ms_step, qps, st.ms_lut, st.ms_rank, st.ms_traverse, st.ms_scan, cand / args.nq,
+           st.cells / args.nq, st.visited / args.nq))
 
     # ---------------- drop-in plain form: one query per call
     pq_n = min(args.per_query, args.nq)
@@ -125,17 +148,19 @@ def main():
     one_ids = S.Dev(ctx, (1, args.R), np.int32)
     one_d = S.Dev(ctx, (1, args.R), np.float32)
     one_s = S.Dev(ctx, (1,), np.int32)
-    qd1 = [S.Dev.from_numpy(ctx, qh[i:i + 1]) for i in range(pq_n)]
-    ix.search(qd1[0], S.KIND_SC, args.R, args.w, args.T, ids=one_ids, dists=one_d, sums=one_s,
-              stats=False)
-    ctx.sync()
-    t0 = time.perf_counter()
-    for i in range(pq_n):
-        ix.search(qd1[i], S.KIND_SC, args.R, args.w, args.T, ids=one_ids, dists=one_d,
+    qps_single = None
+    if pq_n > 0:
+        qd1 = [S.Dev.from_numpy(ctx, qh[i:i + 1]) for i in range(pq_n)]
+        ix.search(qd1[0], S.KIND_SC, args.R, args.w, args.T, ids=one_ids, dists=one_d,
                   sums=one_s, stats=False)
-        one_ids.numpy()                      # the reference API returns host arrays per call
-    qps_single = pq_n / (time.perf_counter() - t0)
-    log("per-query drop-in calls: %.0f QPS" % qps_single)
+        ctx.sync()
+        t0 = time.perf_counter()
+        for i in range(pq_n):
+            ix.search(qd1[i], S.KIND_SC, args.R, args.w, args.T, ids=one_ids, dists=one_d,
+                      sums=one_s, stats=False)
+            one_ids.numpy()                  # the reference API returns host arrays per call
+        qps_single = pq_n / (time.perf_counter() - t0)
+        log("per-query drop-in calls: %.0f QPS" % qps_single)
 
     # ---------------- recall@100 on a sample (exact brute-force ground truth)
     import sco
@@ -193,10 +218,13 @@ def main():
                         % (args.n, args.nq, args.T, args.R),
             "form": "plain-fp32-batched",
             "recall_at_100": recall,
-            "qps_plain_fp32_per_query_calls": round(qps_single, 1),
+            "qps_plain_fp32_per_query_calls": round(qps_single, 1) if qps_single else None,
             "candidates_per_query": round(cand / args.nq, 1),
-            "stage_ms": {"lut": round(st.ms_lut, 4), "rank": round(st.ms_rank, 4),
-                         "traverse": round(st.ms_traverse, 4), "scan_topk": round(st.ms_scan, 4)},
+            "stage_ms": {"lut": round(st0.ms_lut, 4), "rank": round(st0.ms_rank, 4),
+                         "traverse": round(st0.ms_traverse, 4), "scan_topk": round(ms_kernel, 4)},
+            "fused_phase_ms_per_workgroup": dict(zip(
+                ["lut", "sort", "traverse", "scan", "select"], phases)),
+            "launch": "hipGraph plan, autotuned %s" % tune if plan else "sc_search calls",
             "encode_s": round(t_encode, 4),
             "encode_fp64_tflops": round(encode_flops / t_encode / 1e12, 2),
             "train_s": round(t_train, 2),
@@ -209,6 +237,8 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+    if plan:
+        plan.close()
     ix.close()
     ctx.close()
 

@@ -153,6 +153,10 @@ typedef struct {
     int64_t exhausted;    /* queries whose heap emptied before the budget */
     /* stage times on the context stream (HIP events), milliseconds */
     double ms_lut, ms_rank, ms_traverse, ms_scan;
+    /* fused kernel only: time per phase summed over workgroups, divided by
+     * the number of workgroups (milliseconds): 0 LUT load, 1 coarse sort,
+     * 2 traversal, 3 scan, 4 final selection and output */
+    double fused_phase_ms[5];
 } sc_search_stats;
 
 /* Batched search of nq queries d_q [nq][dim]; outputs d_ids [nq][R],

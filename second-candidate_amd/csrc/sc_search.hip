@@ -625,7 +625,7 @@ __device__ void scan_run_generic(const ScanArgs &a, ScanShared &S, const float *
 __device__ void scan_query(const ScanArgs &a, ScanShared &S, float *lut, int64_t q,
                            const float *dqr, const float *dqc, float qsum,
                            const int32_t *cstart, const int32_t *clen, const uint32_t *cinfo,
-                           int ncell, int sum) {
+                           int ncell, int sum, uint64_t *t_scanned = nullptr) {
     const int tid = threadIdx.x;
     const int mp = a.mp, kp = a.kp;
     const int lutn = mp * kp;
@@ -680,6 +680,7 @@ __device__ void scan_query(const ScanArgs &a, ScanShared &S, float *lut, int64_t
         c0 += rn;
         __syncthreads();
     }
+    if (t_scanned && tid == 0) *t_scanned = wall_clock64();
     // final selection
     const int R = a.R;
     int nout;
@@ -731,69 +732,392 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_topk(ScanArgs a) {
 // replay), scan and top-R -- then takes the next query from a work counter.
 // A query whose heap or cell list outgrows its LDS capacity is flagged and
 // finished by the staged kernels afterwards.
+//
+// The scan visits the traversed cells grouped by coarse centroid h3 (a
+// counting sort of the cell list), so the combined table diff_qr + dot_cr[h3]
+// (include/sc_query.h:216-229, include/query.h:295-305) is built once per
+// distinct h3 instead of once per cell.  Every candidate keeps the position it
+// has in the reference's traversal order, so the keys (dist, position) and the
+// canonical top-R do not depend on the visiting order.
 // ----------------------------------------------------------------------------
 struct FusedArgs {
     ScanArgs s;
     const float *Q, *cq, *pq, *norm_c, *norm_r;
     int64_t nq;
-    int w, T, hcap, ccap, npow2;
+    int w, T, hcap, ccap, npow2, bufcap;
     int *counter;
     int32_t *flags;     // [nq] 1 = finish with the staged kernels
     int *nflag;
     int32_t *qinfo;     // [nq][8] statistics (same fields as the staged path)
     const float *lut_dqc, *lut_dqr;   // [nq][mc*kc], [nq][mp*kp] or null: compute here
+    unsigned long long *phase;        // [6]: wall-clock ticks per phase, workgroups
 };
 
 struct FusedLayout {
     size_t a, b, c, d, e, f, total;
 };
 
+// Region A: top-R buffer (bufcap keys + ids) | heap + row lengths | coarse
+// sort keys (kc > 256); B: per-cell start, length, info, position, permuted
+// prefix [ccap + 1], permutation (u16); C: sorted scores + centroids [kc]
+// (after the traversal: per-centroid counts and the group list); D: diff_qc
+// (+ diff_qr when computed here); E: combined table [mp*kp]; F: query [dim].
 __host__ __device__ inline FusedLayout fused_layout(int dim, int mc, int kc, int mp, int kp,
-                                                    int hcap, int ccap, int npow2) {
+                                                    int hcap, int ccap, int npow2, int bufcap,
+                                                    bool dqr_lds) {
     auto up = [](size_t v) { return (v + 15) & ~(size_t)15; };
     FusedLayout L{};
-    size_t heap = (size_t)hcap * 8 + (size_t)kc * 4;
-    size_t a = sizeof(ScanShared);
-    if ((size_t)npow2 * 8 > a) a = (size_t)npow2 * 8;
+    size_t a = (size_t)bufcap * 12;
+    const size_t heap = (size_t)hcap * 8 + (size_t)kc * 4;
     if (heap > a) a = heap;
+    if (npow2 > 256 && (size_t)npow2 * 8 > a) a = (size_t)npow2 * 8;
     L.a = 0;
-    L.b = up(a);                                   // starts, lengths, info [ccap] each
-    L.c = up(L.b + (size_t)ccap * 12);             // sorted v [kc] + idx [kc]
-    L.d = up(L.c + (size_t)kc * 8);                // dqc [mc*kc] + dqr [mp*kp]
-    L.e = up(L.d + 4 * ((size_t)mc * kc + (size_t)mp * kp));   // lut [mp*kp]
-    L.f = up(L.e + 4 * (size_t)mp * kp);           // query [dim]
+    L.b = up(a);
+    L.c = up(L.b + (size_t)ccap * 18 + ((size_t)ccap + 1) * 4);
+    L.d = up(L.c + (size_t)kc * 8);
+    L.e = up(L.d + 4 * ((size_t)mc * kc + (dqr_lds ? (size_t)mp * kp : 0)));
+    L.f = up(L.e + 4 * (size_t)mp * kp);
     L.total = up(L.f + 4 * (size_t)dim);
     return L;
 }
+
+// Small shared state of the streaming top-R buffer.
+struct TopShared {
+    uint32_t hist[256];
+    uint64_t thr;          // keys >= thr cannot enter the top R any more
+    uint64_t sel_prefix, sel_and, sel_or;
+    int nb;                // entries in the buffer
+    int sel_rem, sel_done, wcount;
+    int wsum[kMaxFusedThreads / 64];
+};
+
+__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t x, int m) {
+    const uint32_t lo = __shfl_xor((uint32_t)x, m), hi = __shfl_xor((uint32_t)(x >> 32), m);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Appends the entries of the lanes with `take` at a counter, one LDS atomic
+// per wave.  Every lane of the wave must call it.
+__device__ __forceinline__ void wave_append(int *counter, uint64_t *key, int32_t *idx, bool take,
+                                            uint64_t k, int32_t v) {
+    const uint64_t bal = __ballot(take);
+    if (bal == 0) return;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((long long)bal) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(counter, __popcll(bal));
+    base = __shfl(base, leader);
+    if (take) {
+        const int o = base + __popcll(bal & ((1ull << lane) - 1ull));
+        key[o] = k;
+        idx[o] = v;
+    }
+}
+
+// Threshold thr with exactly R of the (unique) keys key[0..nb) <= thr,
+// R <= nb.  Radix select on 8-bit digits that starts below the bits all keys
+// share and stops as soon as the digit bin holding the R-th key is taken
+// whole.  All threads call it.
+__device__ uint64_t select_threshold(const uint64_t *key, int nb, int R, TopShared &T) {
+    const int tid = threadIdx.x, NT = blockDim.x, lane = tid & 63;
+    uint64_t ka = ~0ull, ko = 0;
+    for (int i = tid; i < nb; i += NT) { ka &= key[i]; ko |= key[i]; }
+    for (int s = 32; s > 0; s >>= 1) { ka &= shfl_xor_u64(ka, s); ko |= shfl_xor_u64(ko, s); }
+    if (tid == 0) { T.sel_and = ~0ull; T.sel_or = 0; T.sel_rem = R; T.sel_done = 0; }
+    __syncthreads();
+    if (lane == 0) { atomicAnd(&T.sel_and, ka); atomicOr(&T.sel_or, ko); }
+    __syncthreads();
+    const uint64_t kand = T.sel_and, diff = kand ^ T.sel_or;
+    if (diff == 0) return kand;                      // a single key (R == nb == 1)
+    const int hb = 63 - __clzll((long long)diff);
+    int shift = hb - 7;
+    uint64_t mask = hb == 63 ? 0ull : (~0ull << (hb + 1));
+    uint64_t prefix = kand & mask;
+    for (;;) {
+        const int sh = shift < 0 ? 0 : shift;
+        const int width = shift < 0 ? 8 + shift : 8;
+        const uint32_t dmask = (1u << width) - 1u;
+        for (int b = tid; b < 256; b += NT) T.hist[b] = 0;
+        __syncthreads();
+        for (int i = tid; i < nb; i += NT) {
+            const uint64_t k = key[i];
+            if ((k & mask) == prefix) atomicAdd(&T.hist[(uint32_t)(k >> sh) & dmask], 1u);
+        }
+        __syncthreads();
+        if (tid < 64) {
+            const int l = tid;
+            const uint32_t c0 = T.hist[4 * l], c1 = T.hist[4 * l + 1], c2 = T.hist[4 * l + 2],
+                           c3 = T.hist[4 * l + 3];
+            const uint32_t tot = c0 + c1 + c2 + c3;
+            uint32_t incl = tot;
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t t = __shfl_up(incl, o);
+                if (l >= o) incl += t;
+            }
+            const uint32_t excl = incl - tot, rem = (uint32_t)T.sel_rem;
+            if (excl < rem && rem <= incl) {
+                const uint32_t c[4] = {c0, c1, c2, c3};
+                uint32_t acc = excl;
+                for (int b = 0; b < 4; ++b) {
+                    if (rem <= acc + c[b]) {
+                        const uint64_t np = prefix | ((uint64_t)(4 * l + b) << sh);
+                        if (rem - acc == c[b]) {      // the whole bin: done
+                            T.sel_prefix = np | (sh > 0 ? ((1ull << sh) - 1ull) : 0ull);
+                            T.sel_done = 1;
+                        } else {
+                            T.sel_prefix = np;
+                            T.sel_rem = (int)(rem - acc);
+                        }
+                        break;
+                    }
+                    acc += c[b];
+                }
+            }
+        }
+        __syncthreads();
+        prefix = T.sel_prefix;
+        if (T.sel_done || sh == 0) return prefix;
+        mask |= (uint64_t)dmask << sh;
+        shift -= 8;
+    }
+}
+
+// Keeps exactly the R smallest keys of key/idx[0..T.nb) at the front.  The
+// survivors move left chunk by chunk of blockDim.x entries: a chunk is read
+// whole before any of its survivors is written, and every write lands below
+// the chunk's end, so nothing unread is overwritten.  All threads call it.
+__device__ void compact_top(TopShared &T, uint64_t *key, int32_t *idx, int R) {
+    const int tid = threadIdx.x, NT = blockDim.x;
+    __syncthreads();
+    const int nb = T.nb;
+    const uint64_t th = select_threshold(key, nb, R, T);
+    if (tid == 0) T.wcount = 0;
+    for (int c0 = 0; c0 < nb; c0 += NT) {
+        const int i = c0 + tid;
+        const uint64_t k = i < nb ? key[i] : ~0ull;
+        const int32_t v = i < nb ? idx[i] : -1;
+        __syncthreads();
+        wave_append(&T.wcount, key, idx, i < nb && k <= th, k, v);
+    }
+    __syncthreads();
+    if (tid == 0) { T.nb = R; T.thr = th; }
+    __syncthreads();
+}
+
+// Bitonic sort of 64 * E keys held E per lane (element lane * E + e), one wave.
+template <int E>
+__device__ __forceinline__ void wave_bitonic(uint64_t (&k)[E], int32_t (&v)[E]) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int size = 2; size <= 64 * E; size <<= 1) {
+#pragma unroll
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            if (stride >= E) {
+                const int pl = stride / E;
+#pragma unroll
+                for (int e = 0; e < E; ++e) {
+                    const int i = lane * E + e;
+                    const uint64_t ok = shfl_xor_u64(k[e], pl);
+                    const int32_t ov = __shfl_xor(v[e], pl);
+                    const bool upw = (i & size) == 0, lower = (i & stride) == 0;
+                    const bool sw = (lower == upw) ? (ok < k[e]) : (ok > k[e]);
+                    if (sw) { k[e] = ok; v[e] = ov; }
+                }
+            } else {
+#pragma unroll
+                for (int e = 0; e < E; ++e) {
+                    if (e & stride) continue;
+                    const int e2 = e | stride, i = lane * E + e;
+                    const bool upw = (i & size) == 0;
+                    if ((k[e] > k[e2]) == upw) {
+                        const uint64_t tk = k[e]; k[e] = k[e2]; k[e2] = tk;
+                        const int32_t tv = v[e]; v[e] = v[e2]; v[e2] = tv;
+                    }
+                }
+            }
+        }
+    }
+}
+
+template <int E>
+__device__ void wave_sort_lds(uint64_t *key, int32_t *idx, int n) {   // wave 0 only
+    const int lane = threadIdx.x & 63;
+    uint64_t k[E];
+    int32_t v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int i = lane * E + e;
+        k[e] = i < n ? key[i] : ~0ull;
+        v[e] = i < n ? idx[i] : -1;
+    }
+    wave_bitonic<E>(k, v);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int i = lane * E + e;
+        if (i < n) { key[i] = k[e]; idx[i] = v[e]; }
+    }
+}
+
+// Sorts key/idx[0..n) ascending; the buffer holds at least next_pow2(n)
+// entries.  All threads call it.
+__device__ void sort_top(uint64_t *key, int32_t *idx, int n) {
+    const int tid = threadIdx.x, NT = blockDim.x;
+    __syncthreads();
+    if (n <= 256) {
+        if (tid < 64) {
+            if (n <= 64) wave_sort_lds<1>(key, idx, n);
+            else if (n <= 128) wave_sort_lds<2>(key, idx, n);
+            else wave_sort_lds<4>(key, idx, n);
+        }
+        __syncthreads();
+        return;
+    }
+    int np = 1;
+    while (np < n) np <<= 1;
+    for (int i = n + tid; i < np; i += NT) { key[i] = ~0ull; idx[i] = -1; }
+    __syncthreads();
+    for (int size = 2; size <= np; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int i = tid; i < np / 2; i += NT) {
+                const int lo = 2 * i - (i & (stride - 1)), hi = lo + stride;
+                const bool upw = (lo & size) == 0;
+                const uint64_t x = key[lo], y = key[hi];
+                if ((x > y) == upw) {
+                    key[lo] = y; key[hi] = x;
+                    const int32_t t = idx[lo]; idx[lo] = idx[hi]; idx[hi] = t;
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// Cells of one scan pass: permuted cell i (of the group) covers positions
+// [gp[i], gp[i+1]) of the pass; its vectors start at cstart[order[i]] and its
+// first candidate has traversal position cpos[order[i]].
+struct GroupCells {
+    const int32_t *gp, *cstart, *cpos;
+    const uint16_t *order;
+};
+
+// One group: candidates at pass positions [p0, p1), first cell i0, table
+// `lut` (mp x kp, combined), start value d0 = q_sum + diff_qc[h3].
+// MPW = 32-bit words of codes per candidate (mp = 4 * MPW); 0 = any mp.
+template <int MPW>
+__device__ void scan_group(const ScanArgs &a, TopShared &T, uint64_t *key, int32_t *idx,
+                           int bufcap, const float *lut, float d0, int p0, int p1, int i0,
+                           const GroupCells &G) {
+    const int tid = threadIdx.x, NT = blockDim.x, U = kRound / NT;
+    const int mp = a.mp, kp = a.kp;
+    int cur = i0, g0 = G.gp[i0], g1 = G.gp[i0 + 1];
+    int cs = 0, cp = 0;
+    {
+        const int c = G.order[i0];
+        cs = G.cstart[c];
+        cp = G.cpos[c];
+    }
+    for (int r0 = p0; r0 < p1; r0 += kRound) {
+        if (T.nb > bufcap - kRound) compact_top(T, key, idx, a.R);
+        const uint64_t thr = T.thr;
+        uint32_t cw[4][MPW > 0 ? MPW : 1];
+        int vx[4], ps[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            vx[u] = -1;
+            ps[u] = 0;
+            const int p = r0 + u * NT + tid;
+            if (u < U && p < p1) {
+                while (p >= g1) {
+                    ++cur;
+                    g0 = g1;
+                    g1 = G.gp[cur + 1];
+                    const int c = G.order[cur];
+                    cs = G.cstart[c];
+                    cp = G.cpos[c];
+                }
+                vx[u] = cs + (p - g0);
+                ps[u] = cp + (p - g0);
+                if (MPW > 0) load_codes<(MPW > 0 ? MPW : 1)>(a.codes, vx[u], cw[u]);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (u >= U) continue;    // U is the same for every thread
+            bool take = false;
+            uint64_t k = 0;
+            if (vx[u] >= 0) {
+                float d = d0;
+                if (MPW > 0) {
+#pragma unroll
+                    for (int w4 = 0; w4 < (MPW > 0 ? MPW : 1); ++w4) {
+                        const uint32_t wv = cw[u][w4];
+#pragma unroll
+                        for (int b = 0; b < 4; ++b)
+                            d += lut[(w4 * 4 + b) * kp + ((wv >> (8 * b)) & 255u)];
+                    }
+                } else {
+                    const uint8_t *cc = a.codes + (size_t)vx[u] * mp;
+                    for (int j = 0; j < mp; ++j) d += lut[j * kp + cc[j]];
+                }
+                k = ((uint64_t)sc_float_key(d) << 32) | (uint32_t)ps[u];
+                take = k < thr;
+            }
+            wave_append(&T.nb, key, idx, take, k, vx[u]);
+        }
+        __syncthreads();
+    }
+}
+
+constexpr int kLutRegs = 8;    // table entries per thread kept in registers
 
 __global__ __launch_bounds__(kMaxFusedThreads) void k_search_fused(FusedArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     const ScanArgs &sa = a.s;
     const int dim = sa.dim, mc = sa.mc, kc = sa.kc, mp = sa.mp, kp = sa.kp;
-    const FusedLayout Lay = fused_layout(dim, mc, kc, mp, kp, a.hcap, a.ccap, a.npow2);
-    ScanShared &S = *reinterpret_cast<ScanShared *>(smem + Lay.a);
-    uint64_t *keys = reinterpret_cast<uint64_t *>(smem + Lay.a);
+    const bool dqr_lds = a.lut_dqr == nullptr;
+    const FusedLayout Lay =
+        fused_layout(dim, mc, kc, mp, kp, a.hcap, a.ccap, a.npow2, a.bufcap, dqr_lds);
+    uint64_t *bkey = reinterpret_cast<uint64_t *>(smem + Lay.a);
+    int32_t *bidx = reinterpret_cast<int32_t *>(bkey + a.bufcap);
+    uint64_t *keys = bkey;                                           // coarse sort, kc > 256
     int32_t *cells = reinterpret_cast<int32_t *>(smem + Lay.b);     // cell starts
     int32_t *clen = cells + a.ccap;
     uint32_t *cinfo = reinterpret_cast<uint32_t *>(clen + a.ccap);
+    int32_t *cpos = reinterpret_cast<int32_t *>(cinfo + a.ccap);
+    int32_t *gp = cpos + a.ccap;                                     // [ccap + 1]
+    uint16_t *order = reinterpret_cast<uint16_t *>(gp + a.ccap + 1);
     float *sv = reinterpret_cast<float *>(smem + Lay.c);
     int32_t *si = reinterpret_cast<int32_t *>(sv + kc);
+    int32_t *ghist = reinterpret_cast<int32_t *>(sv);                // after the traversal
+    uint32_t *gdesc = reinterpret_cast<uint32_t *>(si);
     float *dqc = reinterpret_cast<float *>(smem + Lay.d);
-    float *dqr = dqc + mc * kc;
+    float *dqr = dqc + mc * kc;                                      // dqr_lds only
     float *lut = reinterpret_cast<float *>(smem + Lay.e);
     float *qv = reinterpret_cast<float *>(smem + Lay.f);
-    __shared__ int s_q, s_ncell, s_sum, s_flag;
+    __shared__ TopShared T;
+    __shared__ int64_t s_sbid[kSpec];        // traversal batch (thread 0)
+    __shared__ uint32_t s_sinfo[kSpec];
+    __shared__ int s_q, s_ncell, s_sum, s_flag, s_ngroup;
     __shared__ float s_qsum;
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, NT = blockDim.x, lane = tid & 63, wid = tid >> 6;
     const bool sc = sa.kind == SC_KIND_SC;
     const int bsc = dim / mc, bsp = dim / mp;
+    const int lutn = mp * kp, bs = kp * mp / mc;
+    const bool lut_regs = lutn <= kLutRegs * NT;
 
+    // thread 0 keeps per-phase wall-clock ticks (100 MHz) of this workgroup
+    __shared__ uint64_t s_tph[6];   // 5 phases + last mark
+    if (tid == 0) for (int i = 0; i < 6; ++i) s_tph[i] = 0;
+    auto mark = [&](int ph) {
+        if (tid == 0) { const uint64_t t = wall_clock64(); s_tph[ph] += t - s_tph[5]; s_tph[5] = t; }
+    };
     for (;;) {
         if (tid == 0) s_q = atomicAdd(a.counter, 1);
         __syncthreads();
         const int64_t q = s_q;
         if (q >= a.nq) break;
-        for (int t = tid; t < dim; t += blockDim.x) qv[t] = a.Q[(size_t)q * dim + t];
+        if (tid == 0) s_tph[5] = wall_clock64();
+        for (int t = tid; t < dim; t += NT) qv[t] = a.Q[(size_t)q * dim + t];
         __syncthreads();
         // LUTs (query.h:154-184) and q_sum (query.h:225-230, sc_query.h:95-100)
         if (tid == 0) {
@@ -801,10 +1125,9 @@ __global__ __launch_bounds__(kMaxFusedThreads) void k_search_fused(FusedArgs a) 
             for (int i = 0; i < dim; ++i) { float t = qv[i]; s += t * t; }
             s_qsum = s;
         }
-        if (a.lut_dqc) {   // precomputed by k_lut_tiled
-            for (int j = tid; j < mc * kc; j += blockDim.x) dqc[j] = a.lut_dqc[(size_t)q * mc * kc + j];
-            for (int j = tid; j < mp * kp; j += blockDim.x) dqr[j] = a.lut_dqr[(size_t)q * mp * kp + j];
-        } else for (int j = tid; j < mc * kc + mp * kp; j += blockDim.x) {
+        if (!dqr_lds) {   // precomputed by k_lut_tiled
+            for (int j = tid; j < mc * kc; j += NT) dqc[j] = a.lut_dqc[(size_t)q * mc * kc + j];
+        } else for (int j = tid; j < mc * kc + lutn; j += NT) {
             if (j < mc * kc) {     // a.cq / a.pq: transposed codebooks
                 const int blk = j / kc, k = j - blk * kc;
                 const float *b = a.cq + (size_t)blk * bsc * kc + k;
@@ -822,16 +1145,44 @@ __global__ __launch_bounds__(kMaxFusedThreads) void k_search_fused(FusedArgs a) 
             }
         }
         __syncthreads();
+        mark(0);
         const float qsum = s_qsum;
         // coarse scores; SC sorts them by (score, centroid)
-        if (sc) {
-            for (int j = tid; j < a.npow2; j += blockDim.x)
+        if (sc && a.npow2 <= 256) {
+            if (tid < 64) {
+                auto run = [&](auto tag) {
+                    constexpr int E = decltype(tag)::value;
+                    uint64_t k[E];
+                    int32_t v[E];
+#pragma unroll
+                    for (int e = 0; e < E; ++e) {
+                        const int j = lane * E + e;
+                        k[e] = j < kc ? (((uint64_t)sc_float_key(qsum + dqc[j]) << 32) | (uint32_t)j)
+                                      : ~0ull;
+                        v[e] = 0;
+                    }
+                    wave_bitonic<E>(k, v);
+#pragma unroll
+                    for (int e = 0; e < E; ++e) {
+                        const int j = lane * E + e;
+                        if (j < kc) {
+                            sv[j] = sc_key_float((uint32_t)(k[e] >> 32));
+                            si[j] = (int32_t)(k[e] & 0xFFFFFFFFu);
+                        }
+                    }
+                };
+                if (a.npow2 <= 64) run(std::integral_constant<int, 1>{});
+                else if (a.npow2 <= 128) run(std::integral_constant<int, 2>{});
+                else run(std::integral_constant<int, 4>{});
+            }
+        } else if (sc) {
+            for (int j = tid; j < a.npow2; j += NT)
                 keys[j] = j < kc ? (((uint64_t)sc_float_key(qsum + dqc[j]) << 32) | (uint32_t)j)
                                  : ~(uint64_t)0;
             __syncthreads();
             for (int size = 2; size <= a.npow2; size <<= 1)
                 for (int stride = size >> 1; stride > 0; stride >>= 1) {
-                    for (int i = tid; i < a.npow2 / 2; i += blockDim.x) {
+                    for (int i = tid; i < a.npow2 / 2; i += NT) {
                         int lo = 2 * i - (i & (stride - 1)), hi = lo + stride;
                         bool upw = ((lo & size) == 0);
                         uint64_t x = keys[lo], y = keys[hi];
@@ -839,14 +1190,15 @@ __global__ __launch_bounds__(kMaxFusedThreads) void k_search_fused(FusedArgs a) 
                     }
                     __syncthreads();
                 }
-            for (int j = tid; j < kc; j += blockDim.x) {
+            for (int j = tid; j < kc; j += NT) {
                 sv[j] = sc_key_float((uint32_t)(keys[j] >> 32));
                 si[j] = (int32_t)(keys[j] & 0xFFFFFFFFu);
             }
         } else {
-            for (int j = tid; j < kc; j += blockDim.x) { sv[j] = qsum + dqc[j]; si[j] = j; }
+            for (int j = tid; j < kc; j += NT) { sv[j] = qsum + dqc[j]; si[j] = j; }
         }
         __syncthreads();
+        mark(1);
         // traversal: thread 0, heap and row lengths in LDS (region A)
         if (tid == 0) {
             Heap h;
@@ -864,23 +1216,28 @@ __global__ __launch_bounds__(kMaxFusedThreads) void k_search_fused(FusedArgs a) 
                 for (int j = 0; j < kc; ++j) h.insert(sv[j], (uint32_t)j);
                 bool done = h.overflow;
                 for (int i0 = 0; i0 < a.w && !done; i0 += kSpec) {
-                    int sbid[kSpec];
                     const int nsp = min(kSpec, a.w - i0);
-                    for (int i = 0; i < nsp; ++i) sbid[i] = h.pos > 0 ? (int)h.pop() : -1;
+                    for (int i = 0; i < nsp; ++i) s_sbid[i] = h.pos > 0 ? (int)h.pop() : -1;
                     int32_t lhi[kSpec], llo[kSpec];
-                    for (int i = 0; i < nsp; ++i) {
-                        lhi[i] = sbid[i] >= 0 ? sa.L[sbid[i]] : 0;
-                        llo[i] = sbid[i] > 0 ? sa.L[sbid[i] - 1] : 0;
+#pragma unroll
+                    for (int i = 0; i < kSpec; ++i) {
+                        const int64_t b = i < nsp ? s_sbid[i] : -1;
+                        lhi[i] = b >= 0 ? sa.L[b] : 0;
+                        llo[i] = b > 0 ? sa.L[b - 1] : 0;
                     }
-                    for (int i = 0; i < nsp; ++i) {
-                        if (sbid[i] < 0 || count >= a.ccap) { flag = 1; done = true; break; }
+#pragma unroll
+                    for (int i = 0; i < kSpec; ++i) {
+                        if (i >= nsp || done) continue;
+                        const int64_t b = s_sbid[i];
+                        if (b < 0 || count >= a.ccap) { flag = 1; done = true; continue; }
                         cells[count] = llo[i];
                         clen[count] = lhi[i] - llo[i];
-                        cinfo[count] = (uint32_t)sbid[i];
+                        cinfo[count] = (uint32_t)b;
+                        cpos[count] = sum;
                         count++;
                         count2++;
                         sum += lhi[i] - llo[i];
-                        if (sum >= a.T) { done = true; break; }
+                        if (sum >= a.T) done = true;
                     }
                 }
             } else {
@@ -896,8 +1253,6 @@ __global__ __launch_bounds__(kMaxFusedThreads) void k_search_fused(FusedArgs a) 
                 // where the reference does (pushes past that point are moot).
                 bool done = (count >= a.w || sum >= a.T);
                 while (!done) {
-                    int64_t sbid[kSpec];
-                    uint32_t sinfo[kSpec];
                     int nsp = 0;
                     bool dry = false;
                     while (nsp < kSpec && !h.overflow) {
@@ -905,8 +1260,8 @@ __global__ __launch_bounds__(kMaxFusedThreads) void k_search_fused(FusedArgs a) 
                         const uint32_t pk = h.pop();
                         const int h1 = (int)(pk >> 16), h2 = (int)(pk & 0xFFFFu);
                         const int h3 = si[h1], h4 = si[h2];
-                        sbid[nsp] = (int64_t)h3 * kc + h4;
-                        sinfo[nsp] = (uint32_t)h3 | ((uint32_t)h4 << 16);
+                        s_sbid[nsp] = (int64_t)h3 * kc + h4;
+                        s_sinfo[nsp] = (uint32_t)h3 | ((uint32_t)h4 << 16);
                         ++nsp;
                         rowlen[h1] = h2 + 1;
                         if (h1 < kc - 1 && (h2 == 0 || (h2 - 1) < rowlen[h1 + 1]))
@@ -915,17 +1270,22 @@ __global__ __launch_bounds__(kMaxFusedThreads) void k_search_fused(FusedArgs a) 
                             h.insert(sv[h1] + sv[h2 + 1], ((uint32_t)h1 << 16) | (uint32_t)(h2 + 1));
                     }
                     int32_t lhi[kSpec], llo[kSpec];
-                    for (int i = 0; i < nsp; ++i) {
-                        lhi[i] = sbid[i] > 0 ? sa.L[sbid[i]] : 0;
-                        llo[i] = sbid[i] > 0 ? sa.L[sbid[i] - 1] : 0;
+#pragma unroll
+                    for (int i = 0; i < kSpec; ++i) {
+                        const int64_t b = i < nsp ? s_sbid[i] : 0;
+                        lhi[i] = b > 0 ? sa.L[b] : 0;
+                        llo[i] = b > 0 ? sa.L[b - 1] : 0;
                     }
-                    for (int i = 0; i < nsp && !done; ++i) {
+#pragma unroll
+                    for (int i = 0; i < kSpec; ++i) {
+                        if (i >= nsp || done) continue;
                         const int l = lhi[i] - llo[i];
                         if (l > 0) {
-                            if (count >= a.ccap) { flag = 1; done = true; break; }
+                            if (count >= a.ccap) { flag = 1; done = true; continue; }
                             cells[count] = llo[i];
                             clen[count] = l;
-                            cinfo[count] = sinfo[i];
+                            cinfo[count] = s_sinfo[i];
+                            cpos[count] = sum;
                             count++;
                             sum += l;
                         } else {
@@ -951,9 +1311,147 @@ __global__ __launch_bounds__(kMaxFusedThreads) void k_search_fused(FusedArgs a) 
             if (flag) atomicAdd(a.nflag, 1);
         }
         __syncthreads();
-        if (!s_flag)
-            scan_query(sa, S, lut, q, dqr, dqc, qsum, cells, clen, cinfo, s_ncell, s_sum);
+        mark(2);
+        if (!s_flag) {
+            const int ncell = s_ncell, sum = s_sum;
+            // group the cells by h3: counts, offsets and group list (wave 0),
+            // permutation, then the pass positions gp of the permuted cells
+            for (int j = tid; j < kc; j += NT) ghist[j] = 0;
+            __syncthreads();
+            for (int c = tid; c < ncell; c += NT) atomicAdd(&ghist[cinfo[c] & 0xFFFFu], 1);
+            __syncthreads();
+            if (tid < 64) {
+                const int per = (kc + 63) / 64, b0 = tid * per, b1 = min(kc, b0 + per);
+                int cnt = 0, ne = 0;
+                for (int b = b0; b < b1; ++b) { const int h = ghist[b]; cnt += h; ne += h > 0; }
+                int ic = cnt, in = ne;
+                for (int o = 1; o < 64; o <<= 1) {
+                    const int t1 = __shfl_up(ic, o), t2 = __shfl_up(in, o);
+                    if (tid >= o) { ic += t1; in += t2; }
+                }
+                int oc = ic - cnt, on = in - ne;
+                for (int b = b0; b < b1; ++b) {
+                    const int h = ghist[b];
+                    if (h > 0) gdesc[on++] = (uint32_t)b | ((uint32_t)oc << 16);
+                    ghist[b] = oc;
+                    oc += h;
+                }
+                if (tid == 63) s_ngroup = in;
+            }
+            __syncthreads();
+            for (int c = tid; c < ncell; c += NT) {
+                const int slot = atomicAdd(&ghist[cinfo[c] & 0xFFFFu], 1);
+                order[slot] = (uint16_t)c;
+            }
+            __syncthreads();
+            {
+                const int per = (ncell + NT - 1) / NT, i0 = min(ncell, tid * per),
+                          i1 = min(ncell, i0 + per);
+                int s = 0;
+                for (int i = i0; i < i1; ++i) s += clen[order[i]];
+                int inc = s;
+                for (int o = 1; o < 64; o <<= 1) {
+                    const int t = __shfl_up(inc, o);
+                    if (lane >= o) inc += t;
+                }
+                if (lane == 63) T.wsum[wid] = inc;
+                if (tid == 0) { gp[ncell] = sum; T.nb = 0; T.thr = ~0ull; }
+                __syncthreads();
+                int run = inc - s;
+                for (int w = 0; w < wid; ++w) run += T.wsum[w];
+                for (int i = i0; i < i1; ++i) { gp[i] = run; run += clen[order[i]]; }
+            }
+            __syncthreads();
+            const int ng = s_ngroup;
+            const GroupCells G{gp, cells, cpos, order};
+            const float *dsrc = dqr_lds ? dqr : a.lut_dqr + (size_t)q * lutn;
+            float dq[kLutRegs], rw[kLutRegs];
+            if (lut_regs) {
+#pragma unroll
+                for (int j = 0; j < kLutRegs; ++j) {
+                    const int i = tid + j * NT;
+                    dq[j] = i < lutn ? dsrc[i] : 0.0f;
+                    rw[j] = 0.0f;
+                }
+                if (ng > 0) {
+                    const float *row = sa.dot_cr + (size_t)(gdesc[0] & 0xFFFFu) * bs;
+#pragma unroll
+                    for (int j = 0; j < kLutRegs; ++j) {
+                        const int i = tid + j * NT;
+                        if (i < lutn) rw[j] = row[i];
+                    }
+                }
+            }
+            for (int g = 0; g < ng; ++g) {
+                const uint32_t gd = gdesc[g];
+                const int h3 = (int)(gd & 0xFFFFu), i0 = (int)(gd >> 16);
+                const int i1 = g + 1 < ng ? (int)(gdesc[g + 1] >> 16) : ncell;
+                // the combined table diff_qr + dot_cr[h3]
+                if (lut_regs) {
+#pragma unroll
+                    for (int j = 0; j < kLutRegs; ++j) {
+                        const int i = tid + j * NT;
+                        if (i < lutn) lut[i] = dq[j] + rw[j];
+                    }
+                } else {
+                    const float *row = sa.dot_cr + (size_t)h3 * bs;
+                    for (int i = tid; i < lutn; i += NT) lut[i] = dsrc[i] + row[i];
+                }
+                const float d0 = qsum + dqc[h3];
+                __syncthreads();
+                if (lut_regs && g + 1 < ng) {   // next group's dot_cr row, in flight
+                    const float *row = sa.dot_cr + (size_t)(gdesc[g + 1] & 0xFFFFu) * bs;
+#pragma unroll
+                    for (int j = 0; j < kLutRegs; ++j) {
+                        const int i = tid + j * NT;
+                        if (i < lutn) rw[j] = row[i];
+                    }
+                }
+                const int p0 = gp[i0], p1 = gp[i1];
+                if (mp == 8)
+                    scan_group<2>(sa, T, bkey, bidx, a.bufcap, lut, d0, p0, p1, i0, G);
+                else if (mp == 16)
+                    scan_group<4>(sa, T, bkey, bidx, a.bufcap, lut, d0, p0, p1, i0, G);
+                else
+                    scan_group<0>(sa, T, bkey, bidx, a.bufcap, lut, d0, p0, p1, i0, G);
+                __syncthreads();
+            }
+            mark(3);
+            // final selection: canonical top-R, or traversal order when the
+            // query gathered fewer than R candidates
+            const int R = sa.R;
+            int nout;
+            if (sum >= R) {
+                if (T.nb > R) compact_top(T, bkey, bidx, R);
+                nout = R;
+            } else {
+                nout = T.nb;
+                for (int i = tid; i < nout; i += NT) {
+                    const uint64_t k = bkey[i];
+                    bkey[i] = (k << 32) | (k >> 32);
+                }
+            }
+            sort_top(bkey, bidx, nout);
+            int32_t *ids = sa.ids + (size_t)q * R;
+            float *dists = sa.dists + (size_t)q * R;
+            for (int r = tid; r < R; r += NT) {
+                if (r < nout) {
+                    ids[r] = sa.pid[bidx[r]];
+                    const uint64_t k = bkey[r];
+                    dists[r] = sc_key_float(sum >= R ? (uint32_t)(k >> 32) : (uint32_t)k);
+                } else {
+                    ids[r] = -1;
+                    dists[r] = INFINITY;
+                }
+            }
+            if (sa.sums && tid == 0) sa.sums[q] = sum;
+        }
         __syncthreads();
+        mark(4);
+    }
+    if (tid == 0 && a.phase) {
+        for (int i = 0; i < 5; ++i) atomicAdd(&a.phase[i], (unsigned long long)s_tph[i]);
+        atomicAdd(&a.phase[5], 1ull);
     }
 }
 
@@ -1062,30 +1560,52 @@ static int finish_flagged(sc_ctx *ctx, const sc_index *ix, const sc_search_param
                           const float *d_q, int64_t nq, const int32_t *flags, int nflag,
                           int32_t *d_ids, float *d_dists, int32_t *d_sums, sc_search_stats *st);
 
+// On-chip capacities and LDS layout of the fused kernel for one search.
+struct FusedConfig {
+    int hcap, ccap, npow2, bufcap;
+    bool tiled_lut;     // LUTs precomputed by k_lut_tiled (else in the kernel)
+    FusedLayout lay;
+    int occ;            // workgroups per CU that the LDS allows
+};
+
+static FusedConfig fused_config(const sc_index *ix, const sc_search_params *p) {
+    FusedConfig f{};
+    const int kind = p->kind, kc = ix->kc;
+    f.npow2 = 1;
+    while (f.npow2 < kc) f.npow2 <<= 1;
+    f.hcap = std::min(kind == SC_KIND_IVFADC ? kc : p->w, 1024);
+    if (const char *e = getenv("SC_HEAP_ONCHIP")) {   // test hook: force the heap fallback
+        int v = atoi(e);
+        if (v >= 1) f.hcap = std::min(f.hcap, v);
+    }
+    const int64_t ccap_ref =
+        kind == SC_KIND_IVFADC ? p->w : std::min<int64_t>(p->w, (int64_t)p->T + 1);
+    f.ccap = (int)std::max<int64_t>(1, std::min<int64_t>(ccap_ref, 1024));
+    // the buffer takes one scan round (kRound) above the R it keeps; at most
+    // 8 entries per thread of a 256-thread workgroup (R <= 1024)
+    f.bufcap = (p->R + kRound + 63) & ~63;
+    f.tiled_lut = kc % kLutC == 0 && ix->kp % kLutC == 0;
+    f.lay = fused_layout(ix->dim, ix->mc, kc, ix->mp, ix->kp, f.hcap, f.ccap, f.npow2, f.bufcap,
+                         !f.tiled_lut);
+    f.occ = (int)std::max<size_t>(1, std::min<size_t>(8, (160 * 1024) / f.lay.total));
+    return f;
+}
+
 // Fused persistent path for IVF/SC.  Returns SC_ERR_TOO_LARGE when its LDS
 // layout does not fit (the caller then uses the staged kernels).
 static int search_fused(sc_ctx *ctx, const sc_index *ix, const float *d_q, int64_t nq,
                         const sc_search_params *p, int32_t *d_ids, float *d_dists,
                         int32_t *d_sums, sc_search_stats *st) {
     const int kind = p->kind, kc = ix->kc;
-    int npow2 = 1;
-    while (npow2 < kc) npow2 <<= 1;
-    const int hcap_ref = kind == SC_KIND_IVFADC ? kc : p->w;
-    int hcap = std::min(hcap_ref, 1024);
-    if (const char *e = getenv("SC_HEAP_ONCHIP")) {
-        int v = atoi(e);
-        if (v >= 1) hcap = std::min(hcap, v);
-    }
-    const int64_t ccap_ref =
-        kind == SC_KIND_IVFADC ? p->w : std::min<int64_t>(p->w, (int64_t)p->T + 1);
-    const int ccap = (int)std::max<int64_t>(1, std::min<int64_t>(ccap_ref, 1024));
-    const FusedLayout Lay = fused_layout(ix->dim, ix->mc, kc, ix->mp, ix->kp, hcap, ccap, npow2);
+    const FusedConfig F = fused_config(ix, p);
+    const FusedLayout &Lay = F.lay;
+    const int hcap = F.hcap, ccap = F.ccap, npow2 = F.npow2;
     if (Lay.total > 150 * 1024) return SC_ERR_TOO_LARGE;
 
-    const bool tiled_lut = kc % kLutC == 0 && ix->kp % kLutC == 0;
+    const bool tiled_lut = F.tiled_lut;
     const size_t lut_per = (size_t)ix->mc * kc + (size_t)ix->mp * ix->kp;
     size_t o_cnt = 0, o_flags = 256, o_qinfo = o_flags + ((4 * (size_t)nq + 255) & ~(size_t)255),
-           o_stats = o_qinfo + 32 * (size_t)nq, o_lut = o_stats + 256,
+           o_stats = o_qinfo + 32 * (size_t)nq, o_lut = o_stats + 256,   // stats: 8 + 6 u64
            total = o_lut + (tiled_lut ? 4 * lut_per * (size_t)nq : 0);
     unsigned char *s;
     SC_TRY(sc_scratch(ctx, total, (void **)&s));
@@ -1099,8 +1619,7 @@ static int search_fused(sc_ctx *ctx, const sc_index *ix, const float *d_q, int64
     SC_CHECK_LAUNCH();
     SC_HIP(hipFuncSetAttribute((const void *)k_search_fused,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)Lay.total));
-    int occ = (int)std::max<size_t>(1, std::min<size_t>(4, (160 * 1024) / Lay.total));
-    int64_t grid = (int64_t)ctx->n_cu * occ;
+    int64_t grid = (int64_t)ctx->n_cu * F.occ;
     if (const char *e = getenv("SC_FUSED_GRID")) {   // tuning hook: workgroups in flight
         int v = atoi(e);
         if (v >= 1) grid = v;
@@ -1112,8 +1631,11 @@ static int search_fused(sc_ctx *ctx, const sc_index *ix, const float *d_q, int64
                     ix->codes, nullptr,   nullptr, nullptr, d_ids,   d_dists, d_sums, nullptr};
     fa.Q = d_q; fa.cq = ix->cqT; fa.pq = ix->pqT; fa.norm_c = ix->norm_c; fa.norm_r = ix->norm_r;
     fa.nq = nq; fa.w = p->w; fa.T = p->T; fa.hcap = hcap; fa.ccap = ccap; fa.npow2 = npow2;
+    fa.bufcap = F.bufcap;
     fa.counter = counter; fa.flags = flags; fa.nflag = counter + 1; fa.qinfo = qinfo;
     fa.lut_dqc = fa.lut_dqr = nullptr;
+    fa.phase = st ? stats + 8 : nullptr;     // 6 counters after the 8 statistics
+    if (st) SC_HIP(hipMemsetAsync(stats + 8, 0, 48, sm));
     hipEvent_t e0 = nullptr, e1 = nullptr, el = nullptr;
     if (st) {
         SC_HIP(hipEventCreate(&e0));
@@ -1144,11 +1666,13 @@ static int search_fused(sc_ctx *ctx, const sc_index *ix, const float *d_q, int64
         SC_CHECK_LAUNCH();
     }
     int nflag = 0;
-    unsigned long long h[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    unsigned long long h[14] = {0};
     SC_HIP(hipMemcpyAsync(&nflag, counter + 1, 4, hipMemcpyDeviceToHost, sm));
-    if (st) SC_HIP(hipMemcpyAsync(h, stats, 64, hipMemcpyDeviceToHost, sm));
+    if (st) SC_HIP(hipMemcpyAsync(h, stats, 112, hipMemcpyDeviceToHost, sm));
     SC_HIP(hipStreamSynchronize(sm));
     if (st) {
+        const double wgs = h[13] ? (double)h[13] : 1.0;
+        for (int i = 0; i < 5; ++i) st->fused_phase_ms[i] = (double)h[8 + i] / wgs / 1e5;
         float ms = 0.0f, ml = 0.0f;
         SC_HIP(hipEventElapsedTime(&ms, e0, e1));
         SC_HIP(hipEventElapsedTime(&ml, el, e0));
@@ -1434,7 +1958,7 @@ struct sc_plan {
     bool fused;
     FusedArgs fa;
     size_t lds;
-    int threads, grid;
+    int threads, grid, occ;
     int *counter;                  // [0] work counter, [1] flagged
     int32_t *flags, *qinfo;
     unsigned long long *stats;
@@ -1460,6 +1984,7 @@ static int plan_capture(sc_plan *P) {
     SC_HIP(hipStreamBeginCapture(sm, hipStreamCaptureModeThreadLocal));
     // the work counter is cleared by a kernel, not a memset node
     hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, sm, P->counter, 2);
+    hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, sm, (int *)(P->stats + 8), 12);
     plan_lut(P);
     hipLaunchKernelGGL(k_search_fused, dim3((unsigned)P->grid), dim3(P->threads), P->lds, sm,
                        P->fa);
@@ -1490,21 +2015,17 @@ extern "C" int sc_plan_create(sc_ctx *ctx, const sc_index *ix, const sc_search_p
     P->ctx = ctx; P->ix = ix; P->p = *p; P->d_q = d_q; P->nq = nq;
     P->d_ids = d_ids; P->d_dists = d_dists; P->d_sums = d_sums;
     const int kind = p->kind, kc = ix->kc;
-    int npow2 = 1;
-    while (npow2 < kc) npow2 <<= 1;
-    const int hcap = std::min(kind == SC_KIND_IVFADC ? kc : p->w, 1024);
-    const int64_t ccap_ref =
-        kind == SC_KIND_IVFADC ? p->w : std::min<int64_t>(p->w, (int64_t)p->T + 1);
-    const int ccap = (int)std::max<int64_t>(1, std::min<int64_t>(ccap_ref, 1024));
-    const FusedLayout Lay = fused_layout(ix->dim, ix->mc, kc, ix->mp, ix->kp, hcap, ccap, npow2);
+    const FusedConfig F = fused_config(ix, p);
+    const FusedLayout &Lay = F.lay;
+    const int hcap = F.hcap, ccap = F.ccap, npow2 = F.npow2;
     P->fused = kind != SC_KIND_IMI && Lay.total <= 150 * 1024;
     if (!P->fused) { *out = P; return SC_OK; }
     P->lds = Lay.total;
     bool ok = hipMalloc(&P->counter, 256) == hipSuccess &&
               hipMalloc(&P->flags, 4 * (size_t)nq) == hipSuccess &&
               hipMalloc(&P->qinfo, 32 * (size_t)nq) == hipSuccess &&
-              hipMalloc(&P->stats, 64) == hipSuccess;
-    if (ok && kc % kLutC == 0 && ix->kp % kLutC == 0)
+              hipMalloc(&P->stats, 8 * 14) == hipSuccess;   // 8 statistics + 6 phase counters
+    if (ok && F.tiled_lut)
         ok = hipMalloc(&P->lut, 4 * (size_t)nq * (ix->mc * kc + ix->mp * ix->kp)) == hipSuccess;
     if (!ok) { sc_plan_destroy(P); sc_set_error("sc_plan_create: allocation failed"); return SC_ERR_NOMEM; }
     FusedArgs &fa = P->fa;
@@ -1513,17 +2034,19 @@ extern "C" int sc_plan_create(sc_ctx *ctx, const sc_index *ix, const sc_search_p
                     ix->codes, nullptr, nullptr, nullptr, d_ids,      d_dists, d_sums, nullptr};
     fa.Q = d_q; fa.cq = ix->cqT; fa.pq = ix->pqT; fa.norm_c = ix->norm_c; fa.norm_r = ix->norm_r;
     fa.nq = nq; fa.w = p->w; fa.T = p->T; fa.hcap = hcap; fa.ccap = ccap; fa.npow2 = npow2;
+    fa.bufcap = F.bufcap;
     fa.counter = P->counter; fa.flags = P->flags; fa.nflag = P->counter + 1; fa.qinfo = P->qinfo;
     fa.lut_dqc = P->lut;
     fa.lut_dqr = P->lut ? P->lut + (size_t)nq * ix->mc * kc : nullptr;
+    fa.phase = P->stats + 8;
     if (hipFuncSetAttribute((const void *)k_search_fused, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)P->lds) != hipSuccess) {
         sc_plan_destroy(P);
         return SC_ERR_HIP;
     }
     P->threads = kScanThreads;
-    const int occ = (int)std::max<size_t>(1, std::min<size_t>(4, (160 * 1024) / P->lds));
-    P->grid = (int)std::min<int64_t>((int64_t)ctx->n_cu * occ, nq);
+    P->occ = F.occ;
+    P->grid = (int)std::min<int64_t>((int64_t)ctx->n_cu * F.occ, nq);
     int rc = plan_capture(P);
     if (rc) { sc_plan_destroy(P); return rc; }
     *out = P;
@@ -1552,11 +2075,13 @@ extern "C" int sc_plan_execute(sc_plan *P, sc_search_stats *st) {
         SC_CHECK_LAUNCH();
     }
     int nflag = 0;
-    unsigned long long h[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    unsigned long long h[14] = {0};
     SC_HIP(hipMemcpyAsync(&nflag, P->counter + 1, 4, hipMemcpyDeviceToHost, sm));
-    if (st) SC_HIP(hipMemcpyAsync(h, P->stats, 64, hipMemcpyDeviceToHost, sm));
+    if (st) SC_HIP(hipMemcpyAsync(h, P->stats, 112, hipMemcpyDeviceToHost, sm));
     SC_HIP(hipStreamSynchronize(sm));
     if (st) {
+        const double wgs = h[13] ? (double)h[13] : 1.0;
+        for (int i = 0; i < 5; ++i) st->fused_phase_ms[i] = (double)h[8 + i] / wgs / 1e5;
         float ms = 0.0f;
         SC_HIP(hipEventElapsedTime(&ms, e0, e1));
         (void)hipEventDestroy(e0);
@@ -1580,7 +2105,7 @@ extern "C" int sc_plan_autotune(sc_plan *P, int32_t *out3) {
         return SC_OK;
     }
     hipStream_t sm = P->ctx->stream;
-    const int occ = (int)std::max<size_t>(1, std::min<size_t>(4, (160 * 1024) / P->lds));
+    const int occ = P->occ;
     hipEvent_t e0, e1;
     SC_HIP(hipEventCreate(&e0));
     SC_HIP(hipEventCreate(&e1));

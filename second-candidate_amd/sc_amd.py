@@ -49,7 +49,7 @@ class SearchStats(C.Structure):
                 ("reranked", C.c_int64), ("heap_fallback", C.c_int64),
                 ("exhausted", C.c_int64), ("ms_lut", C.c_double),
                 ("ms_rank", C.c_double), ("ms_traverse", C.c_double),
-                ("ms_scan", C.c_double)]
+                ("ms_scan", C.c_double), ("fused_phase_ms", C.c_double * 5)]
 
 
 def mix_d128(seed=20151001, ncomp=64):
@@ -181,6 +181,16 @@ class Dev:
             _check("sc_memcpy_d2h", lib().sc_memcpy_d2h(self.ctx.h, out.ctypes.data, self.ptr,
                                                         self.nbytes))
         return out
+
+    def copy_from(self, arr):
+        """Overwrites the buffer with a host array of the same size (synchronous)."""
+        arr = np.ascontiguousarray(arr, dtype=self.dtype)
+        if arr.nbytes != self.nbytes:
+            raise ValueError("copy_from: %d bytes into a %d-byte buffer" % (arr.nbytes, self.nbytes))
+        if self.nbytes:
+            _check("sc_memcpy_h2d", lib().sc_memcpy_h2d(self.ctx.h, self.ptr, arr.ctypes.data,
+                                                        self.nbytes))
+        return self
 
     def offset(self, elems):
         return _P(self.ptr.value + int(elems) * self.dtype.itemsize)

@@ -143,6 +143,38 @@ def test_search_matches_oracle(ctx, case):
     ix.close()
 
 
+@pytest.mark.parametrize("kind,kc,kp", [(S.KIND_SC, 16, 64), (S.KIND_SC, 64, 64),
+                                        (S.KIND_IVFADC, 64, 64), (S.KIND_IMI, 16, 64)])
+def test_search_plan_replays_match_oracle(ctx, kind, kc, kp):
+    """A captured plan replayed several times, with eager searches and new
+    query contents between the replays, and again after autotuning; every
+    replay must equal the oracle (kc = kp = 64 takes the tiled LUT kernel
+    inside the graph; IMI runs the staged kernels)."""
+    ix, oix = _index_pair(ctx, kind, kc=kc, kp=kp)
+    okind = {S.KIND_IVFADC: sco.IVFADC, S.KIND_IMI: sco.IMI, S.KIND_SC: sco.SC}[kind]
+    R, w, T, nq = 50, (12 if kind == S.KIND_IVFADC else 300), 2500, 130
+    qs = [sco.gen(sco.mix_d128(ncomp=11), sco.STREAM_QUERY, i * nq, nq, ix.dim) for i in range(3)]
+    dq = S.Dev.from_numpy(ctx, qs[0])
+    plan = S.Plan(ix, dq, kind, R, w, T)
+    other = S.Dev.from_numpy(ctx, qs[2][:40])
+    for rep in range(5):
+        if rep == 3:
+            plan.autotune()
+        qh = qs[rep % 2]
+        dq.copy_from(qh)
+        st = plan.execute(stats=True)
+        oids, od, osum = oix.search_batch(okind, qh, R, w, T)
+        assert np.array_equal(plan.sums.numpy(), osum), rep
+        assert np.array_equal(plan.ids.numpy(), oids), rep
+        assert np.array_equal(_bits(plan.dists.numpy()), _bits(od)), rep
+        assert st.queries == nq and st.candidates == int(osum.sum())
+        # eager work between replays
+        ix.search(other, kind, R, w, T)
+        ctx.sync()
+    plan.close()
+    ix.close()
+
+
 def test_search_heap_fallback_path(ctx, monkeypatch):
     """A 2-entry on-chip heap forces every multi-sequence query onto the
     global-memory heap of capacity w; results must not change."""
