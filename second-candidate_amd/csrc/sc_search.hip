@@ -541,7 +541,9 @@ __device__ void scan_run_packed(const ScanArgs &a, ScanShared &S, const float *l
     const int kp = a.kp;
     int cur = 0;
     for (int p0 = 0; p0 < rtotal; p0 += kRound) {
-        if (S.nb > kBufCap - kRound) compact(S, a.R);
+        const bool full = S.nb > kBufCap - kRound;   // read by all before any append
+        __syncthreads();
+        if (full) compact(S, a.R);
         uint32_t cw[4][MPW];
         int vx[4];
         for (int u = 0; u < U; ++u) {
@@ -583,7 +585,9 @@ __device__ void scan_run_generic(const ScanArgs &a, ScanShared &S, const float *
     const int bs = kp * mp / a.mc;
     int cur = 0;
     for (int p0 = 0; p0 < rtotal; p0 += kRound) {
-        if (S.nb > kBufCap - kRound) compact(S, a.R);
+        const bool full = S.nb > kBufCap - kRound;   // read by all before any append
+        __syncthreads();
+        if (full) compact(S, a.R);
         for (int u = 0; u < U; ++u) {
             const int p = p0 + u * NT + tid;
             if (p >= rtotal) continue;
@@ -848,13 +852,14 @@ __device__ uint64_t select_threshold(const uint64_t *key, int nb, int R, TopShar
             const int l = tid;
             const uint32_t c0 = T.hist[4 * l], c1 = T.hist[4 * l + 1], c2 = T.hist[4 * l + 2],
                            c3 = T.hist[4 * l + 3];
+            const uint32_t rem = (uint32_t)T.sel_rem;   // read before any lane updates it
             const uint32_t tot = c0 + c1 + c2 + c3;
             uint32_t incl = tot;
             for (int o = 1; o < 64; o <<= 1) {
                 const uint32_t t = __shfl_up(incl, o);
                 if (l >= o) incl += t;
             }
-            const uint32_t excl = incl - tot, rem = (uint32_t)T.sel_rem;
+            const uint32_t excl = incl - tot;
             if (excl < rem && rem <= incl) {
                 const uint32_t c[4] = {c0, c1, c2, c3};
                 uint32_t acc = excl;
@@ -1017,7 +1022,11 @@ __device__ void scan_group(const ScanArgs &a, TopShared &T, uint64_t *key, int32
         cp = G.cpos[c];
     }
     for (int r0 = p0; r0 < p1; r0 += kRound) {
-        if (T.nb > bufcap - kRound) compact_top(T, key, idx, a.R);
+        // every thread reads the fill before any wave appends this round, so
+        // all of them take the same compaction decision
+        const bool full = T.nb > bufcap - kRound;
+        __syncthreads();
+        if (full) compact_top(T, key, idx, a.R);
         const uint64_t thr = T.thr;
         uint32_t cw[4][MPW > 0 ? MPW : 1];
         int vx[4], ps[4];
