@@ -50,7 +50,23 @@ def main():
     ap.add_argument("--recall-sample", type=int, default=100)
     ap.add_argument("--plan", type=int, default=1,
                     help="1: time the autotuned hipGraph plan (sc_plan_*); 0: sc_search calls")
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="only run a plain device copy of this many MiB (HBM counter scale)")
     args = ap.parse_args()
+
+    if args.copy_calibration_mb > 0:
+        # A plain copy kernel of known size, to scale the FETCH_SIZE/WRITE_SIZE
+        # counters of the search kernel: reads and writes exactly this many bytes.
+        import torch
+        n = args.copy_calibration_mb << 20
+        src = torch.ones(n // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": n, "launches": 3}), flush=True)
+        return
 
     if args.gpus != 1:
         print("bench.py measures one GPU; this build runs on a single MI355X "
@@ -224,7 +240,9 @@ def main():
                          "traverse": round(st0.ms_traverse, 4), "scan_topk": round(ms_kernel, 4)},
             "fused_phase_ms_per_workgroup": dict(zip(
                 ["lut", "sort", "traverse", "scan", "select"], phases)),
-            "launch": "hipGraph plan, autotuned %s" % tune if plan else "sc_search calls",
+            "launch": ("hipGraph plan, autotuned: %d threads x %d workgroups, register variant "
+                       "%d waves/SIMD" % (tune["threads"], tune["grid"], tune["variant"]))
+                      if plan else "sc_search calls",
             "encode_s": round(t_encode, 4),
             "encode_fp64_tflops": round(encode_flops / t_encode / 1e12, 2),
             "train_s": round(t_train, 2),

@@ -184,10 +184,11 @@ int sc_plan_create(sc_ctx *ctx, const sc_index *ix, const sc_search_params *p,
                    const float *d_q, int64_t nq, int32_t *d_ids, float *d_dists,
                    int32_t *d_sums, sc_plan **out);
 int sc_plan_execute(sc_plan *plan, sc_search_stats *st);
-/* Times the fused kernel over launch shapes (threads per workgroup x
- * workgroups in flight) on the plan's own batch, keeps the fastest and
- * re-captures the graph.  out3 (may be NULL) = {threads, grid, best_us}. */
-int sc_plan_autotune(sc_plan *plan, int32_t *out3);
+/* Times the fused kernel over launch shapes (register variant x workgroups in
+ * flight) on the plan's own batch, keeps the fastest and re-captures the
+ * graph.  out4 (may be NULL) = {threads, grid, best_us, variant}; the variant
+ * is the number of waves per SIMD its register allocation allows. */
+int sc_plan_autotune(sc_plan *plan, int32_t *out4);
 int sc_plan_destroy(sc_plan *plan);
 
 /* ---- k-means (codebook training) --------------------------------------- */

@@ -44,12 +44,17 @@ __global__ void k_dense_prefix(const uint64_t *__restrict__ skeys, int64_t n, in
     L[c] = (int32_t)lo;
 }
 
+// out[0] = largest cell, out[1] = number of non-empty cells
 __global__ void k_cell_max(const int32_t *__restrict__ L, int64_t ncells, int32_t *out) {
     int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int32_t l = 0;
     if (c < ncells) l = c > 0 ? L[c] - L[c - 1] : L[0];
+    const int ne = __popcll(__ballot(l > 0));
     for (int o = 32; o > 0; o >>= 1) l = max(l, __shfl_xor(l, o));
-    if ((threadIdx.x & 63) == 0) atomicMax(out, l);
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(out, l);
+        atomicAdd(out + 1, ne);
+    }
 }
 
 __global__ void k_gather_codes(const uint8_t *__restrict__ codes, const int32_t *__restrict__ ids,
@@ -145,15 +150,16 @@ int check_shape(int64_t n, int32_t dim, int32_t mc, int32_t kc, int32_t mp, int3
 int finish_index(sc_index *ix) {
     sc_ctx *ctx = ix->ctx;
     SC_TRY(sc_launch_tables(ix));
-    int32_t *dmax;
-    SC_TRY(sc_scratch(ctx, sizeof(int32_t), (void **)&dmax));
-    SC_HIP(hipMemsetAsync(dmax, 0, sizeof(int32_t), ctx->stream));
+    int32_t *dmax, h[2] = {0, 0};
+    SC_TRY(sc_scratch(ctx, 2 * sizeof(int32_t), (void **)&dmax));
+    SC_HIP(hipMemsetAsync(dmax, 0, 2 * sizeof(int32_t), ctx->stream));
     hipLaunchKernelGGL(k_cell_max, dim3((unsigned)sc_cdiv(ix->ncells, 256)), dim3(256), 0,
                        ctx->stream, ix->L, ix->ncells, dmax);
     SC_CHECK_LAUNCH();
-    SC_HIP(hipMemcpyAsync(&ix->max_cell, dmax, sizeof(int32_t), hipMemcpyDeviceToHost,
-                          ctx->stream));
+    SC_HIP(hipMemcpyAsync(h, dmax, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
     SC_HIP(hipStreamSynchronize(ctx->stream));
+    ix->max_cell = h[0];
+    ix->nonempty = h[1];
     return SC_OK;
 }
 

@@ -25,6 +25,7 @@ struct sc_index {
     int32_t *pid;                   // [n]
     uint8_t *codes;                 // [n][mp] in cell order
     int32_t max_cell;               // largest cell size
+    int64_t nonempty;               // non-empty cells
 };
 
 void sc_set_error(const char *fmt, ...);

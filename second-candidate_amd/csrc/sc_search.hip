@@ -1077,9 +1077,23 @@ __device__ void scan_group(const ScanArgs &a, TopShared &T, uint64_t *key, int32
     }
 }
 
-constexpr int kLutRegs = 8;    // table entries per thread kept in registers
+constexpr int kLutRegs = 8;
 
-__global__ __launch_bounds__(kMaxFusedThreads) void k_search_fused(FusedArgs a) {
+// The traversal runs on one lane; left alone, the compiler treats its loaded
+// values as wave-uniform and moves each into a scalar register right after its
+// load, waiting for the loads one by one.  Passing the address through a
+// vector register keeps a batch of loads in flight together.
+__device__ __forceinline__ int64_t lane_value(int64_t v) {
+    asm volatile("" : "+v"(v));
+    return v;
+}    // table entries per thread kept in registers
+
+// OCC = waves per SIMD the register allocation must allow (3, 4 or 5): more
+// workgroups in flight hide the serial phases of one query behind the others
+// at the price of register spills; the plan autotuner picks the variant.
+template <int OCC>
+__global__ __launch_bounds__(kMaxFusedThreads) __attribute__((amdgpu_waves_per_eu(OCC, 8)))
+void k_search_fused(FusedArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     const ScanArgs &sa = a.s;
     const int dim = sa.dim, mc = sa.mc, kc = sa.kc, mp = sa.mp, kp = sa.kp;
@@ -1230,7 +1244,7 @@ __global__ __launch_bounds__(kMaxFusedThreads) void k_search_fused(FusedArgs a) 
                     int32_t lhi[kSpec], llo[kSpec];
 #pragma unroll
                     for (int i = 0; i < kSpec; ++i) {
-                        const int64_t b = i < nsp ? s_sbid[i] : -1;
+                        const int64_t b = lane_value(i < nsp ? (int64_t)s_sbid[i] : -1);
                         lhi[i] = b >= 0 ? sa.L[b] : 0;
                         llo[i] = b > 0 ? sa.L[b - 1] : 0;
                     }
@@ -1281,7 +1295,7 @@ __global__ __launch_bounds__(kMaxFusedThreads) void k_search_fused(FusedArgs a) 
                     int32_t lhi[kSpec], llo[kSpec];
 #pragma unroll
                     for (int i = 0; i < kSpec; ++i) {
-                        const int64_t b = i < nsp ? s_sbid[i] : 0;
+                        const int64_t b = lane_value(i < nsp ? s_sbid[i] : 0);
                         lhi[i] = b > 0 ? sa.L[b] : 0;
                         llo[i] = b > 0 ? sa.L[b - 1] : 0;
                     }
@@ -1464,6 +1478,19 @@ __global__ __launch_bounds__(kMaxFusedThreads) void k_search_fused(FusedArgs a) 
     }
 }
 
+using FusedKernel = void (*)(FusedArgs);
+FusedKernel fused_kernel(int occ) {
+    return occ >= 5 ? k_search_fused<5> : occ == 4 ? k_search_fused<4> : k_search_fused<3>;
+}
+
+hipError_t launch_fused(int occ, int grid, int threads, size_t lds, hipStream_t sm,
+                        const FusedArgs &fa) {
+    FusedArgs arg = fa;
+    void *args[] = {&arg};
+    return hipLaunchKernel((const void *)fused_kernel(occ), dim3((unsigned)grid),
+                           dim3((unsigned)threads), args, lds, sm);
+}
+
 __global__ void k_zero_ints(int *p, int n) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;
@@ -1575,6 +1602,7 @@ struct FusedConfig {
     bool tiled_lut;     // LUTs precomputed by k_lut_tiled (else in the kernel)
     FusedLayout lay;
     int occ;            // workgroups per CU that the LDS allows
+    int variant;        // kernel register variant (waves per SIMD: 3, 4, 5)
 };
 
 static FusedConfig fused_config(const sc_index *ix, const sc_search_params *p) {
@@ -1589,14 +1617,28 @@ static FusedConfig fused_config(const sc_index *ix, const sc_search_params *p) {
     }
     const int64_t ccap_ref =
         kind == SC_KIND_IVFADC ? p->w : std::min<int64_t>(p->w, (int64_t)p->T + 1);
-    f.ccap = (int)std::max<int64_t>(1, std::min<int64_t>(ccap_ref, 1024));
+    // Cells a query gathers before reaching T: sized from the mean non-empty
+    // cell with a wide margin (queries that need more are finished by the
+    // staged kernels); smaller lists leave LDS for more workgroups per CU.
+    int64_t est = 1024;
+    if (kind == SC_KIND_SC && ix->nonempty > 0 && ix->n > 0) {
+        const double mean = (double)ix->n / (double)ix->nonempty;
+        est = std::max<int64_t>(128, (int64_t)(8.0 * (double)p->T / mean) + 64);
+    }
+    if (const char *e = getenv("SC_FUSED_CELLS")) {   // test hook: cell list capacity
+        int v = atoi(e);
+        if (v >= 1) est = v;
+    }
+    f.ccap = (int)std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(ccap_ref, est), 1024));
     // the buffer takes one scan round (kRound) above the R it keeps; at most
     // 8 entries per thread of a 256-thread workgroup (R <= 1024)
     f.bufcap = (p->R + kRound + 63) & ~63;
     f.tiled_lut = kc % kLutC == 0 && ix->kp % kLutC == 0;
     f.lay = fused_layout(ix->dim, ix->mc, kc, ix->mp, ix->kp, f.hcap, f.ccap, f.npow2, f.bufcap,
                          !f.tiled_lut);
-    f.occ = (int)std::max<size_t>(1, std::min<size_t>(8, (160 * 1024) / f.lay.total));
+    // 2 KiB covers the kernel's static LDS
+    f.occ = (int)std::max<size_t>(1, std::min<size_t>(8, (160 * 1024) / (f.lay.total + 2048)));
+    f.variant = std::max(3, std::min(f.occ, 5));
     return f;
 }
 
@@ -1626,9 +1668,9 @@ static int search_fused(sc_ctx *ctx, const sc_index *ix, const float *d_q, int64
 
     hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, sm, counter, 2);
     SC_CHECK_LAUNCH();
-    SC_HIP(hipFuncSetAttribute((const void *)k_search_fused,
+    SC_HIP(hipFuncSetAttribute((const void *)fused_kernel(F.variant),
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)Lay.total));
-    int64_t grid = (int64_t)ctx->n_cu * F.occ;
+    int64_t grid = (int64_t)ctx->n_cu * std::min(F.occ, F.variant);
     if (const char *e = getenv("SC_FUSED_GRID")) {   // tuning hook: workgroups in flight
         int v = atoi(e);
         if (v >= 1) grid = v;
@@ -1664,9 +1706,7 @@ static int search_fused(sc_ctx *ctx, const sc_index *ix, const float *d_q, int64
         SC_CHECK_LAUNCH();
     }
     if (st) SC_HIP(hipEventRecord(e0, sm));
-    hipLaunchKernelGGL(k_search_fused, dim3((unsigned)grid), dim3(kScanThreads), Lay.total, sm,
-                       fa);
-    SC_CHECK_LAUNCH();
+    SC_HIP(launch_fused(F.variant, (int)grid, kScanThreads, Lay.total, sm, fa));
     if (st) {
         SC_HIP(hipEventRecord(e1, sm));
         SC_HIP(hipMemsetAsync(stats, 0, 64, sm));
@@ -1967,7 +2007,7 @@ struct sc_plan {
     bool fused;
     FusedArgs fa;
     size_t lds;
-    int threads, grid, occ;
+    int threads, grid, occ, variant;
     int *counter;                  // [0] work counter, [1] flagged
     int32_t *flags, *qinfo;
     unsigned long long *stats;
@@ -1995,9 +2035,8 @@ static int plan_capture(sc_plan *P) {
     hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, sm, P->counter, 2);
     hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, sm, (int *)(P->stats + 8), 12);
     plan_lut(P);
-    hipLaunchKernelGGL(k_search_fused, dim3((unsigned)P->grid), dim3(P->threads), P->lds, sm,
-                       P->fa);
-    hipError_t le = hipGetLastError();
+    hipError_t le = launch_fused(P->variant, P->grid, P->threads, P->lds, sm, P->fa);
+    if (le == hipSuccess) le = hipGetLastError();
     hipError_t ce = hipStreamEndCapture(sm, &P->graph);
     if (le != hipSuccess || ce != hipSuccess) {
         sc_set_error("sc_plan: graph capture failed: %s", hipGetErrorString(le != hipSuccess ? le : ce));
@@ -2048,14 +2087,17 @@ extern "C" int sc_plan_create(sc_ctx *ctx, const sc_index *ix, const sc_search_p
     fa.lut_dqc = P->lut;
     fa.lut_dqr = P->lut ? P->lut + (size_t)nq * ix->mc * kc : nullptr;
     fa.phase = P->stats + 8;
-    if (hipFuncSetAttribute((const void *)k_search_fused, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)P->lds) != hipSuccess) {
-        sc_plan_destroy(P);
-        return SC_ERR_HIP;
-    }
+    for (int v = 3; v <= 5; ++v)
+        if (hipFuncSetAttribute((const void *)fused_kernel(v),
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)P->lds) != hipSuccess) {
+            sc_plan_destroy(P);
+            return SC_ERR_HIP;
+        }
     P->threads = kScanThreads;
     P->occ = F.occ;
-    P->grid = (int)std::min<int64_t>((int64_t)ctx->n_cu * F.occ, nq);
+    P->variant = F.variant;
+    P->grid = (int)std::min<int64_t>((int64_t)ctx->n_cu * std::min(F.occ, F.variant), nq);
     int rc = plan_capture(P);
     if (rc) { sc_plan_destroy(P); return rc; }
     *out = P;
@@ -2106,11 +2148,11 @@ extern "C" int sc_plan_execute(sc_plan *P, sc_search_stats *st) {
                           P->d_dists, P->d_sums, st);
 }
 
-extern "C" int sc_plan_autotune(sc_plan *P, int32_t *out3) {
+extern "C" int sc_plan_autotune(sc_plan *P, int32_t *out4) {
     if (!P) return SC_ERR_ARG;
     SC_TRY(sc_ctx_enter(P->ctx));
     if (!P->fused) {
-        if (out3) { out3[0] = 0; out3[1] = 0; out3[2] = 0; }
+        if (out4) { out4[0] = 0; out4[1] = 0; out4[2] = 0; out4[3] = 0; }
         return SC_OK;
     }
     hipStream_t sm = P->ctx->stream;
@@ -2119,33 +2161,35 @@ extern "C" int sc_plan_autotune(sc_plan *P, int32_t *out3) {
     SC_HIP(hipEventCreate(&e0));
     SC_HIP(hipEventCreate(&e1));
     float best = INFINITY;
-    int bt = P->threads, bg = P->grid;
+    int bt = P->threads, bg = P->grid, bv = P->variant;
     plan_lut(P);                       // the fused kernel reads this batch's LUTs
-    for (int threads : {256, 512}) {
-        for (int per_cu = 1; per_cu <= occ; ++per_cu) {
+    // register variants x workgroups per CU (256 threads); a variant beyond
+    // what the LDS allows per CU only adds spills
+    for (int v = 3; v <= std::max(3, std::min(occ, 5)); ++v) {
+        for (int per_cu = std::max(1, std::min(v, occ) - 1); per_cu <= std::min(v, occ); ++per_cu) {
+            const int threads = kScanThreads;
             const int grid = (int)std::min<int64_t>((int64_t)P->ctx->n_cu * per_cu, P->nq);
             float tot = 0.0f;
             for (int rep = 0; rep < 3; ++rep) {
                 hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, sm, P->counter, 2);
                 SC_HIP(hipEventRecord(e0, sm));
-                hipLaunchKernelGGL(k_search_fused, dim3((unsigned)grid), dim3(threads), P->lds, sm,
-                                   P->fa);
-                SC_CHECK_LAUNCH();
+                SC_HIP(launch_fused(v, grid, threads, P->lds, sm, P->fa));
                 SC_HIP(hipEventRecord(e1, sm));
                 SC_HIP(hipEventSynchronize(e1));
                 float ms = 0.0f;
                 SC_HIP(hipEventElapsedTime(&ms, e0, e1));
                 if (rep > 0) tot += ms;        // first run warms caches
             }
-            if (tot < best) { best = tot; bt = threads; bg = grid; }
+            if (tot < best) { best = tot; bt = threads; bg = grid; bv = v; }
         }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     P->threads = bt;
     P->grid = bg;
+    P->variant = bv;
     SC_TRY(plan_capture(P));
-    if (out3) { out3[0] = bt; out3[1] = bg; out3[2] = (int32_t)(best / 2.0f * 1000.0f); }
+    if (out4) { out4[0] = bt; out4[1] = bg; out4[2] = (int32_t)(best / 2.0f * 1000.0f); out4[3] = bv; }
     return SC_OK;
 }
 

@@ -317,9 +317,10 @@ class Plan:
         return st if stats else None
 
     def autotune(self):
-        out = np.zeros(3, np.int32)
+        out = np.zeros(4, np.int32)
         _check("sc_plan_autotune", lib().sc_plan_autotune(self.h, out.ctypes.data))
-        return {"threads": int(out[0]), "grid": int(out[1]), "us": int(out[2])}
+        return {"threads": int(out[0]), "grid": int(out[1]), "us": int(out[2]),
+                "variant": int(out[3])}
 
     def close(self):
         if self.h:

@@ -187,6 +187,16 @@ def test_search_heap_fallback_path(ctx, monkeypatch):
         ix.close()
 
 
+def test_search_cell_list_overflow_path(ctx, monkeypatch):
+    """A 4-cell on-chip cell list flags most fused queries; the staged kernels
+    finish them and the results must not change."""
+    ix, oix = _index_pair(ctx, S.KIND_SC)
+    monkeypatch.setenv("SC_FUSED_CELLS", "4")
+    _compare_search(ctx, ix, oix, S.KIND_SC, 50, 300, 2500, nq=60)
+    monkeypatch.delenv("SC_FUSED_CELLS")
+    ix.close()
+
+
 def test_search_exhausted_grid(ctx):
     ix, oix = _index_pair(ctx, S.KIND_SC, n=300, kc=4)
     st = _compare_search(ctx, ix, oix, S.KIND_SC, 400, 1000, 10 ** 6, nq=20)
