@@ -214,6 +214,17 @@ def main():
         log("cpu oracle: %.1f QPS on %d threads (%d queries), ids match: %s"
             % (ns / dt, threads, ns, match))
 
+    # HBM bytes per launch of the same kernel and workload, measured with
+    # rocprofv3 FETCH_SIZE/WRITE_SIZE and scaled by a plain copy of known size
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "search_fused_sc1M_traffic.json")
+    if fused and os.path.exists(tpath):
+        t = json.load(open(tpath))
+        w = t.get("workload", {})
+        if (w.get("n"), w.get("nq"), w.get("T"), w.get("R"), w.get("kc")) == \
+                (args.n, args.nq, args.T, args.R, kc):
+            traffic = int(t["traffic_bytes"])
+
     encode_flops = 2.0 * args.n * (kc * dim + mp * kp * (dim // mp))
     out = {
         "metric": "QPS at fixed recall@100, mixed-prec batched vs plain FP32 GPU, 1M×128",
@@ -249,9 +260,10 @@ def main():
         },
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None,
+                     "traffic": traffic,
                      "kernel": "k_search_fused" if fused else "k_scan_topk",
-                     "algorithmic_bytes_per_launch": int(scan_bytes)},
+                     "algorithmic_bytes_per_launch": int(scan_bytes),
+                     "traffic_source": "profiles/search_fused_sc1M_hbm_counters.md" if traffic else None},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
