@@ -50,6 +50,8 @@ def main():
     ap.add_argument("--recall-sample", type=int, default=100)
     ap.add_argument("--plan", type=int, default=1,
                     help="1: time the autotuned hipGraph plan (sc_plan_*); 0: sc_search calls")
+    ap.add_argument("--mixed", type=int, default=1,
+                    help="1: also time the mixed-precision form (FP16 table + exact re-rank)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="only run a plain device copy of this many MiB (HBM counter scale)")
     args = ap.parse_args()
@@ -158,6 +160,33 @@ def main():
         % (ms_step, qps, st.ms_lut, st.ms_rank, st.ms_traverse, st.ms_scan, cand / args.nq,
            st.cells / args.nq, st.visited / args.nq))
 
+    # ---------------- mixed-precision form, batched (same plan machinery)
+    mixed = None
+    if args.mixed and args.plan:
+        mplan = S.Plan(ix, q, S.KIND_SC, args.R, args.w, args.T, form=S.FORM_MIXED)
+        mtune = mplan.autotune()
+        for _ in range(args.warmup):
+            mplan.execute(stats=False)
+        ctx.sync()
+        mstats = []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            mstats.append(mplan.execute(stats=True))
+        ctx.sync()
+        mel = time.perf_counter() - t0
+        same = bool(np.array_equal(mplan.ids.numpy(), ids.numpy()) and
+                    np.array_equal(mplan.dists.numpy().view(np.uint32),
+                                   dists.numpy().view(np.uint32)))
+        mixed = {"qps": round(args.nq * args.steps / mel, 1),
+                 "ms_per_step": round(mel * 1e3 / args.steps, 3),
+                 "reranked_per_query": round(mstats[-1].reranked / args.nq, 1),
+                 "identical_to_plain": same,
+                 "launch": "%d threads x %d workgroups, variant %d" % (
+                     mtune["threads"], mtune["grid"], mtune["variant"])}
+        log("mixed batched: %.1f ms/step, %.0f QPS, %.1f re-ranked/query, identical: %s"
+            % (mixed["ms_per_step"], mixed["qps"], mixed["reranked_per_query"], same))
+        mplan.close()
+
     # ---------------- drop-in plain form: one query per call
     pq_n = min(args.per_query, args.nq)
     qh = q.numpy()
@@ -246,6 +275,7 @@ def main():
             "form": "plain-fp32-batched",
             "recall_at_100": recall,
             "qps_plain_fp32_per_query_calls": round(qps_single, 1) if qps_single else None,
+            "mixed_precision_batched": mixed,
             "candidates_per_query": round(cand / args.nq, 1),
             "stage_ms": {"lut": round(st0.ms_lut, 4), "rank": round(st0.ms_rank, 4),
                          "traverse": round(st0.ms_traverse, 4), "scan_topk": round(ms_kernel, 4)},

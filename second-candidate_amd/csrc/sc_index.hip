@@ -11,6 +11,8 @@
 
 #include <rocprim/device/device_radix_sort.hpp>
 
+#include <algorithm>
+
 #include "sc_internal.h"
 
 namespace {
@@ -42,6 +44,17 @@ __global__ void k_dense_prefix(const uint64_t *__restrict__ skeys, int64_t n, in
         if (skeys[mid] <= (uint64_t)c) lo = mid + 1; else hi = mid;
     }
     L[c] = (int32_t)lo;
+}
+
+// out = max |x| over x[0..n) as float bits (non-negative floats order like
+// their bit patterns)
+__global__ void k_absmax(const float *__restrict__ x, int64_t n, uint32_t *out) {
+    uint32_t m = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        m = max(m, __float_as_uint(fabsf(x[i])));
+    for (int o = 32; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, m);
 }
 
 // out[0] = largest cell, out[1] = number of non-empty cells
@@ -150,16 +163,21 @@ int check_shape(int64_t n, int32_t dim, int32_t mc, int32_t kc, int32_t mp, int3
 int finish_index(sc_index *ix) {
     sc_ctx *ctx = ix->ctx;
     SC_TRY(sc_launch_tables(ix));
-    int32_t *dmax, h[2] = {0, 0};
-    SC_TRY(sc_scratch(ctx, 2 * sizeof(int32_t), (void **)&dmax));
-    SC_HIP(hipMemsetAsync(dmax, 0, 2 * sizeof(int32_t), ctx->stream));
+    int32_t *dmax, h[3] = {0, 0, 0};
+    SC_TRY(sc_scratch(ctx, 3 * sizeof(int32_t), (void **)&dmax));
+    SC_HIP(hipMemsetAsync(dmax, 0, 3 * sizeof(int32_t), ctx->stream));
     hipLaunchKernelGGL(k_cell_max, dim3((unsigned)sc_cdiv(ix->ncells, 256)), dim3(256), 0,
                        ctx->stream, ix->L, ix->ncells, dmax);
     SC_CHECK_LAUNCH();
-    SC_HIP(hipMemcpyAsync(h, dmax, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    const int64_t ndot = (int64_t)ix->kc * ix->mp * ix->kp;
+    hipLaunchKernelGGL(k_absmax, dim3((unsigned)std::min<int64_t>(1024, sc_cdiv(ndot, 256))),
+                       dim3(256), 0, ctx->stream, ix->dot_cr, ndot, (uint32_t *)(dmax + 2));
+    SC_CHECK_LAUNCH();
+    SC_HIP(hipMemcpyAsync(h, dmax, 3 * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
     SC_HIP(hipStreamSynchronize(ctx->stream));
     ix->max_cell = h[0];
     ix->nonempty = h[1];
+    ix->dot_absmax = __builtin_bit_cast(float, (uint32_t)h[2]);
     return SC_OK;
 }
 

@@ -26,6 +26,7 @@ struct sc_index {
     uint8_t *codes;                 // [n][mp] in cell order
     int32_t max_cell;               // largest cell size
     int64_t nonempty;               // non-empty cells
+    float dot_absmax;               // max |dot_cr| (error bound of the mixed form)
 };
 
 void sc_set_error(const char *fmt, ...);

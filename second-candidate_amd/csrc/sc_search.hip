@@ -379,7 +379,7 @@ __global__ void k_traverse(TravArgs a, int hcap, int qpw) {
     }
     int32_t *qi = a.qinfo + (size_t)q * 8;
     qi[0] = count; qi[1] = sum; qi[2] = count2; qi[3] = empty; qi[4] = exhausted;
-    qi[5] = h.overflow ? 1 : 0; qi[6] = err;
+    qi[5] = h.overflow ? 1 : 0; qi[6] = err; qi[7] = 0;
 }
 
 // ----------------------------------------------------------------------------
@@ -755,6 +755,8 @@ struct FusedArgs {
     int32_t *qinfo;     // [nq][8] statistics (same fields as the staged path)
     const float *lut_dqc, *lut_dqr;   // [nq][mc*kc], [nq][mp*kp] or null: compute here
     unsigned long long *phase;        // [6]: wall-clock ticks per phase, workgroups
+    float dot_absmax;                 // max |dot_cr| (mixed form error bound)
+    float margin_scale;               // mixed form: multiplies the error bound (1)
 };
 
 struct FusedLayout {
@@ -793,6 +795,8 @@ struct TopShared {
     int nb;                // entries in the buffer
     int sel_rem, sel_done, wcount;
     int wsum[kMaxFusedThreads / 64];
+    uint32_t amax_r, amax_c;   // mixed form: max |diff_qr|, max |diff_qc| (float bits)
+    int ovf, cnt;              // mixed form: margin set outgrew the buffer; a count
 };
 
 __device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t x, int m) {
@@ -909,6 +913,47 @@ __device__ void compact_top(TopShared &T, uint64_t *key, int32_t *idx, int R) {
     __syncthreads();
 }
 
+// Mixed form: keeps every key whose approximate distance is within twoE of
+// the R-th smallest (at least R keys).  When more than lim keys would stay it
+// keeps exactly R and sets T.ovf: the query is then redone exactly.
+__device__ void compact_top_margin(TopShared &T, uint64_t *key, int32_t *idx, int R, float twoE,
+                                   int lim) {
+    const int tid = threadIdx.x, NT = blockDim.x;
+    __syncthreads();
+    const int nb = T.nb;
+    const uint64_t kth = select_threshold(key, nb, R, T);
+    // one float step above the rounded sum keeps the bound on the safe side
+    uint32_t lk = sc_float_key(sc_key_float((uint32_t)(kth >> 32)) + twoE);
+    if (lk != 0xFFFFFFFFu) ++lk;
+    uint64_t th = ((uint64_t)lk << 32) | 0xFFFFFFFFull;
+    if (th < kth) th = kth;
+    if (tid == 0) T.cnt = 0;
+    __syncthreads();
+    int c = 0;
+    for (int i = tid; i < nb; i += NT) c += key[i] <= th ? 1 : 0;
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+    if ((tid & 63) == 0) atomicAdd(&T.cnt, c);
+    __syncthreads();
+    const int kept = T.cnt;
+    const bool over = kept > lim;
+    if (over) th = kth;
+    if (tid == 0) T.wcount = 0;
+    for (int c0 = 0; c0 < nb; c0 += NT) {
+        const int i = c0 + tid;
+        const uint64_t k = i < nb ? key[i] : ~0ull;
+        const int32_t v = i < nb ? idx[i] : -1;
+        __syncthreads();
+        wave_append(&T.wcount, key, idx, i < nb && k <= th, k, v);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        T.nb = over ? R : kept;
+        T.thr = over ? kth : (th == ~0ull ? th : th + 1);
+        if (over) T.ovf = 1;
+    }
+    __syncthreads();
+}
+
 // Bitonic sort of 64 * E keys held E per lane (element lane * E + e), one wave.
 template <int E>
 __device__ __forceinline__ void wave_bitonic(uint64_t (&k)[E], int32_t (&v)[E]) {
@@ -1008,10 +1053,13 @@ struct GroupCells {
 // One group: candidates at pass positions [p0, p1), first cell i0, table
 // `lut` (mp x kp, combined), start value d0 = q_sum + diff_qc[h3].
 // MPW = 32-bit words of codes per candidate (mp = 4 * MPW); 0 = any mp.
-template <int MPW>
+// MIXED: the table holds FP16 values of (diff_qr + dot_cr) / S; the
+// approximate distance d0 + S * sum is kept within a margin of twoE.
+template <int MPW, bool MIXED>
 __device__ void scan_group(const ScanArgs &a, TopShared &T, uint64_t *key, int32_t *idx,
                            int bufcap, const float *lut, float d0, int p0, int p1, int i0,
-                           const GroupCells &G) {
+                           const GroupCells &G, float S, float twoE) {
+    const _Float16 *lut16 = reinterpret_cast<const _Float16 *>(lut);
     const int tid = threadIdx.x, NT = blockDim.x, U = kRound / NT;
     const int mp = a.mp, kp = a.kp;
     int cur = i0, g0 = G.gp[i0], g1 = G.gp[i0 + 1];
@@ -1026,7 +1074,10 @@ __device__ void scan_group(const ScanArgs &a, TopShared &T, uint64_t *key, int32
         // all of them take the same compaction decision
         const bool full = T.nb > bufcap - kRound;
         __syncthreads();
-        if (full) compact_top(T, key, idx, a.R);
+        if (full) {
+            if (MIXED) compact_top_margin(T, key, idx, a.R, twoE, bufcap - kRound);
+            else compact_top(T, key, idx, a.R);
+        }
         const uint64_t thr = T.thr;
         uint32_t cw[4][MPW > 0 ? MPW : 1];
         int vx[4], ps[4];
@@ -1056,7 +1107,22 @@ __device__ void scan_group(const ScanArgs &a, TopShared &T, uint64_t *key, int32
             uint64_t k = 0;
             if (vx[u] >= 0) {
                 float d = d0;
-                if (MPW > 0) {
+                if (MIXED) {
+                    float acc = 0.0f;
+                    if (MPW > 0) {
+#pragma unroll
+                        for (int w4 = 0; w4 < (MPW > 0 ? MPW : 1); ++w4) {
+                            const uint32_t wv = cw[u][w4];
+#pragma unroll
+                            for (int b = 0; b < 4; ++b)
+                                acc += (float)lut16[(w4 * 4 + b) * kp + ((wv >> (8 * b)) & 255u)];
+                        }
+                    } else {
+                        const uint8_t *cc = a.codes + (size_t)vx[u] * mp;
+                        for (int j = 0; j < mp; ++j) acc += (float)lut16[j * kp + cc[j]];
+                    }
+                    d = d0 + S * acc;
+                } else if (MPW > 0) {
 #pragma unroll
                     for (int w4 = 0; w4 < (MPW > 0 ? MPW : 1); ++w4) {
                         const uint32_t wv = cw[u][w4];
@@ -1091,7 +1157,10 @@ __device__ __forceinline__ int64_t lane_value(int64_t v) {
 // OCC = waves per SIMD the register allocation must allow (3, 4 or 5): more
 // workgroups in flight hide the serial phases of one query behind the others
 // at the price of register spills; the plan autotuner picks the variant.
-template <int OCC>
+// MIXED = the mixed-precision form: FP16 table scan with a rigorous error
+// bound, then an exact FP32 re-rank of every candidate the bound cannot
+// exclude, so ids and distances equal the plain form's.
+template <int OCC, bool MIXED>
 __global__ __launch_bounds__(kMaxFusedThreads) __attribute__((amdgpu_waves_per_eu(OCC, 8)))
 void k_search_fused(FusedArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -1135,7 +1204,12 @@ void k_search_fused(FusedArgs a) {
         if (tid == 0) { const uint64_t t = wall_clock64(); s_tph[ph] += t - s_tph[5]; s_tph[5] = t; }
     };
     for (;;) {
-        if (tid == 0) s_q = atomicAdd(a.counter, 1);
+        if (tid == 0) {
+            s_q = atomicAdd(a.counter, 1);
+            T.amax_r = 0u;
+            T.amax_c = 0u;
+            T.ovf = 0;
+        }
         __syncthreads();
         const int64_t q = s_q;
         if (q >= a.nq) break;
@@ -1170,6 +1244,37 @@ void k_search_fused(FusedArgs a) {
         __syncthreads();
         mark(0);
         const float qsum = s_qsum;
+        // Mixed form: error bound E of one approximate distance.  With
+        // B >= |diff_qr + dot_cr| for every table entry, S = 2^e >= B / 2^14
+        // (FP16 range) and M >= |every partial sum|, the FP16 entries are off
+        // by <= 2^-11 B (+ S 2^-25 when subnormal), the FP32 sum of mp of them
+        // by <= mp^2 2^-24 B, and the exact and approximate sums round by
+        // <= (mp + 1) 2^-24 M; E is 1.25x the total.  Every candidate of the
+        // exact top R has an approximate distance within 2E of the R-th
+        // smallest approximate one.
+        float S = 1.0f, twoE = 0.0f;
+        if (MIXED) {
+            const float *dsrc0 = dqr_lds ? dqr : a.lut_dqr + (size_t)q * lutn;
+            uint32_t mr = 0u, mcq = 0u;
+            for (int i = tid; i < lutn; i += NT) mr = max(mr, __float_as_uint(fabsf(dsrc0[i])));
+            for (int j = tid; j < mc * kc; j += NT) mcq = max(mcq, __float_as_uint(fabsf(dqc[j])));
+            for (int o = 32; o > 0; o >>= 1) {
+                mr = max(mr, (uint32_t)__shfl_xor(mr, o));
+                mcq = max(mcq, (uint32_t)__shfl_xor(mcq, o));
+            }
+            if (lane == 0) { atomicMax(&T.amax_r, mr); atomicMax(&T.amax_c, mcq); }
+            __syncthreads();
+            const float B = (__uint_as_float(T.amax_r) + a.dot_absmax) * (1.0f + 0x1p-20f);
+            int ex = 0;
+            if (B > 0.0f) (void)frexpf(B * 0x1p-14f, &ex);
+            S = ldexpf(1.0f, ex);
+            const float fmp = (float)mp;
+            const float M = fabsf(qsum) + __uint_as_float(T.amax_c) + fmp * B;
+            const float E = ((fmp + 1.0f) * 0x1p-24f * M + fmp * fmp * 0x1p-24f * B +
+                             fmp * 0x1p-11f * B + fmp * S * 0x1p-25f) * 1.25f * a.margin_scale;
+            twoE = 2.0f * E;
+        }
+        const float invS = 1.0f / S;
         // coarse scores; SC sorts them by (score, centroid)
         if (sc && a.npow2 <= 256) {
             if (tid < 64) {
@@ -1329,7 +1434,7 @@ void k_search_fused(FusedArgs a) {
             int32_t *qi = a.qinfo + (size_t)q * 8;   // a flagged query is counted by its redo
             qi[0] = flag ? 0 : count; qi[1] = flag ? 0 : sum; qi[2] = flag ? 0 : count2;
             qi[3] = flag ? 0 : empty; qi[4] = flag ? 0 : exhausted;
-            qi[5] = flag; qi[6] = 0;
+            qi[5] = flag; qi[6] = 0; qi[7] = 0;
             a.flags[q] = flag;
             if (flag) atomicAdd(a.nflag, 1);
         }
@@ -1410,15 +1515,24 @@ void k_search_fused(FusedArgs a) {
                 const int h3 = (int)(gd & 0xFFFFu), i0 = (int)(gd >> 16);
                 const int i1 = g + 1 < ng ? (int)(gdesc[g + 1] >> 16) : ncell;
                 // the combined table diff_qr + dot_cr[h3]
+                _Float16 *lut16 = reinterpret_cast<_Float16 *>(lut);
                 if (lut_regs) {
 #pragma unroll
                     for (int j = 0; j < kLutRegs; ++j) {
                         const int i = tid + j * NT;
-                        if (i < lutn) lut[i] = dq[j] + rw[j];
+                        if (i < lutn) {
+                            const float t = dq[j] + rw[j];
+                            if (MIXED) lut16[i] = (_Float16)(t * invS);
+                            else lut[i] = t;
+                        }
                     }
                 } else {
                     const float *row = sa.dot_cr + (size_t)h3 * bs;
-                    for (int i = tid; i < lutn; i += NT) lut[i] = dsrc[i] + row[i];
+                    for (int i = tid; i < lutn; i += NT) {
+                        const float t = dsrc[i] + row[i];
+                        if (MIXED) lut16[i] = (_Float16)(t * invS);
+                        else lut[i] = t;
+                    }
                 }
                 const float d0 = qsum + dqc[h3];
                 __syncthreads();
@@ -1432,17 +1546,65 @@ void k_search_fused(FusedArgs a) {
                 }
                 const int p0 = gp[i0], p1 = gp[i1];
                 if (mp == 8)
-                    scan_group<2>(sa, T, bkey, bidx, a.bufcap, lut, d0, p0, p1, i0, G);
+                    scan_group<2, MIXED>(sa, T, bkey, bidx, a.bufcap, lut, d0, p0, p1, i0, G, S,
+                                         twoE);
                 else if (mp == 16)
-                    scan_group<4>(sa, T, bkey, bidx, a.bufcap, lut, d0, p0, p1, i0, G);
+                    scan_group<4, MIXED>(sa, T, bkey, bidx, a.bufcap, lut, d0, p0, p1, i0, G, S,
+                                         twoE);
                 else
-                    scan_group<0>(sa, T, bkey, bidx, a.bufcap, lut, d0, p0, p1, i0, G);
+                    scan_group<0, MIXED>(sa, T, bkey, bidx, a.bufcap, lut, d0, p0, p1, i0, G, S,
+                                         twoE);
                 __syncthreads();
             }
             mark(3);
             // final selection: canonical top-R, or traversal order when the
             // query gathered fewer than R candidates
             const int R = sa.R;
+            if (MIXED) {
+                // keep what the bound cannot exclude, then give every kept
+                // candidate its exact distance: q_sum + diff_qc[h3], then
+                // + (diff_qr + dot_cr) per sub-quantizer in order, the same
+                // operations as the plain form's table and scan
+                if (sum >= R && T.nb > R) compact_top_margin(T, bkey, bidx, R, twoE, a.bufcap);
+                const int m = T.nb;
+                if (!T.ovf) {
+                    const float *dsrc1 = dqr_lds ? dqr : a.lut_dqr + (size_t)q * lutn;
+                    for (int i = tid; i < m; i += NT) {
+                        const uint64_t k = bkey[i];
+                        const int pos = (int)(uint32_t)k;
+                        int lo = 0, hi = ncell;   // last cell with cpos <= pos
+                        while (hi - lo > 1) {
+                            const int mid = (lo + hi) >> 1;
+                            if (cpos[mid] <= pos) lo = mid; else hi = mid;
+                        }
+                        const int h3 = (int)(cinfo[lo] & 0xFFFFu);
+                        const uint8_t *cc = sa.codes + (size_t)bidx[i] * mp;
+                        const float *row = sa.dot_cr + (size_t)h3 * bs;
+                        float d = qsum + dqc[h3];
+                        for (int j = 0; j < mp; ++j) {
+                            const int e = j * kp + cc[j];
+                            d += dsrc1[e] + row[e];
+                        }
+                        bkey[i] = ((uint64_t)sc_float_key(d) << 32) | (uint32_t)pos;
+                    }
+                    if (tid == 0) a.qinfo[(size_t)q * 8 + 7] = m;
+                }
+                __syncthreads();
+            }
+            if (MIXED && T.ovf) {
+                // the kept set outgrew the buffer: finished exactly afterwards
+                if (tid == 0) {
+                    int32_t *qi = a.qinfo + (size_t)q * 8;
+                    qi[0] = qi[1] = qi[2] = qi[3] = qi[4] = 0;
+                    qi[5] = 1;
+                    qi[7] = 0;
+                    a.flags[q] = 1;
+                    atomicAdd(a.nflag, 1);
+                }
+                __syncthreads();
+                mark(4);
+                continue;
+            }
             int nout;
             if (sum >= R) {
                 if (T.nb > R) compact_top(T, bkey, bidx, R);
@@ -1479,15 +1641,19 @@ void k_search_fused(FusedArgs a) {
 }
 
 using FusedKernel = void (*)(FusedArgs);
-FusedKernel fused_kernel(int occ) {
-    return occ >= 5 ? k_search_fused<5> : occ == 4 ? k_search_fused<4> : k_search_fused<3>;
+FusedKernel fused_kernel(int occ, bool mixed) {
+    if (mixed)
+        return occ >= 5 ? k_search_fused<5, true>
+                        : occ == 4 ? k_search_fused<4, true> : k_search_fused<3, true>;
+    return occ >= 5 ? k_search_fused<5, false>
+                    : occ == 4 ? k_search_fused<4, false> : k_search_fused<3, false>;
 }
 
-hipError_t launch_fused(int occ, int grid, int threads, size_t lds, hipStream_t sm,
+hipError_t launch_fused(int occ, bool mixed, int grid, int threads, size_t lds, hipStream_t sm,
                         const FusedArgs &fa) {
     FusedArgs arg = fa;
     void *args[] = {&arg};
-    return hipLaunchKernel((const void *)fused_kernel(occ), dim3((unsigned)grid),
+    return hipLaunchKernel((const void *)fused_kernel(occ, mixed), dim3((unsigned)grid),
                            dim3((unsigned)threads), args, lds, sm);
 }
 
@@ -1534,7 +1700,7 @@ __global__ void k_gather_fallback(const int32_t *__restrict__ qinfo, int64_t nq,
 __global__ void k_reduce_stats(const int32_t *__restrict__ qinfo, int64_t nq,
                                unsigned long long *__restrict__ out) {
     int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    unsigned long long v[6] = {0, 0, 0, 0, 0, 0};
+    unsigned long long v[7] = {0, 0, 0, 0, 0, 0, 0};
     if (q < nq) {
         const int32_t *qi = qinfo + q * 8;
         v[0] = (unsigned long long)qi[1];
@@ -1543,8 +1709,9 @@ __global__ void k_reduce_stats(const int32_t *__restrict__ qinfo, int64_t nq,
         v[3] = (unsigned long long)qi[4];
         v[4] = (unsigned long long)(qi[6] != 0);
         v[5] = (unsigned long long)qi[5];
+        v[6] = (unsigned long long)qi[7];
     }
-    for (int i = 0; i < 6; ++i) {
+    for (int i = 0; i < 7; ++i) {
         unsigned long long t = v[i];
         for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o);
         if ((threadIdx.x & 63) == 0 && t) atomicAdd(&out[i], t);
@@ -1596,6 +1763,15 @@ static int finish_flagged(sc_ctx *ctx, const sc_index *ix, const sc_search_param
                           const float *d_q, int64_t nq, const int32_t *flags, int nflag,
                           int32_t *d_ids, float *d_dists, int32_t *d_sums, sc_search_stats *st);
 
+// Mixed form: factor on its error bound (test hook; 1 otherwise).
+static float mixed_margin_scale() {
+    if (const char *e = getenv("SC_MIXED_MARGIN")) {
+        const float v = (float)atof(e);
+        if (v >= 1.0f) return v;
+    }
+    return 1.0f;
+}
+
 // On-chip capacities and LDS layout of the fused kernel for one search.
 struct FusedConfig {
     int hcap, ccap, npow2, bufcap;
@@ -1632,7 +1808,8 @@ static FusedConfig fused_config(const sc_index *ix, const sc_search_params *p) {
     f.ccap = (int)std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(ccap_ref, est), 1024));
     // the buffer takes one scan round (kRound) above the R it keeps; at most
     // 8 entries per thread of a 256-thread workgroup (R <= 1024)
-    f.bufcap = (p->R + kRound + 63) & ~63;
+    // (the mixed form keeps up to 3R within its error margin)
+    f.bufcap = ((p->form == SC_FORM_MIXED ? 3 * p->R : p->R) + kRound + 63) & ~63;
     f.tiled_lut = kc % kLutC == 0 && ix->kp % kLutC == 0;
     f.lay = fused_layout(ix->dim, ix->mc, kc, ix->mp, ix->kp, f.hcap, f.ccap, f.npow2, f.bufcap,
                          !f.tiled_lut);
@@ -1668,7 +1845,8 @@ static int search_fused(sc_ctx *ctx, const sc_index *ix, const float *d_q, int64
 
     hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, sm, counter, 2);
     SC_CHECK_LAUNCH();
-    SC_HIP(hipFuncSetAttribute((const void *)fused_kernel(F.variant),
+    const bool mixed = p->form == SC_FORM_MIXED;
+    SC_HIP(hipFuncSetAttribute((const void *)fused_kernel(F.variant, mixed),
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)Lay.total));
     int64_t grid = (int64_t)ctx->n_cu * std::min(F.occ, F.variant);
     if (const char *e = getenv("SC_FUSED_GRID")) {   // tuning hook: workgroups in flight
@@ -1686,6 +1864,8 @@ static int search_fused(sc_ctx *ctx, const sc_index *ix, const float *d_q, int64
     fa.counter = counter; fa.flags = flags; fa.nflag = counter + 1; fa.qinfo = qinfo;
     fa.lut_dqc = fa.lut_dqr = nullptr;
     fa.phase = st ? stats + 8 : nullptr;     // 6 counters after the 8 statistics
+    fa.dot_absmax = ix->dot_absmax;
+    fa.margin_scale = mixed_margin_scale();
     if (st) SC_HIP(hipMemsetAsync(stats + 8, 0, 48, sm));
     hipEvent_t e0 = nullptr, e1 = nullptr, el = nullptr;
     if (st) {
@@ -1706,7 +1886,7 @@ static int search_fused(sc_ctx *ctx, const sc_index *ix, const float *d_q, int64
         SC_CHECK_LAUNCH();
     }
     if (st) SC_HIP(hipEventRecord(e0, sm));
-    SC_HIP(launch_fused(F.variant, (int)grid, kScanThreads, Lay.total, sm, fa));
+    SC_HIP(launch_fused(F.variant, mixed, (int)grid, kScanThreads, Lay.total, sm, fa));
     if (st) {
         SC_HIP(hipEventRecord(e1, sm));
         SC_HIP(hipMemsetAsync(stats, 0, 64, sm));
@@ -1733,6 +1913,7 @@ static int search_fused(sc_ctx *ctx, const sc_index *ix, const float *d_q, int64
         st->cells = (int64_t)h[1];
         st->visited = (int64_t)h[2];
         st->exhausted = (int64_t)h[3];
+        st->reranked = (int64_t)h[6];
         st->ms_lut = ml;
         st->ms_scan = ms;
     }
@@ -2008,6 +2189,7 @@ struct sc_plan {
     FusedArgs fa;
     size_t lds;
     int threads, grid, occ, variant;
+    bool mixed;                    // SC_FORM_MIXED kernels
     int *counter;                  // [0] work counter, [1] flagged
     int32_t *flags, *qinfo;
     unsigned long long *stats;
@@ -2035,7 +2217,7 @@ static int plan_capture(sc_plan *P) {
     hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, sm, P->counter, 2);
     hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, sm, (int *)(P->stats + 8), 12);
     plan_lut(P);
-    hipError_t le = launch_fused(P->variant, P->grid, P->threads, P->lds, sm, P->fa);
+    hipError_t le = launch_fused(P->variant, P->mixed, P->grid, P->threads, P->lds, sm, P->fa);
     if (le == hipSuccess) le = hipGetLastError();
     hipError_t ce = hipStreamEndCapture(sm, &P->graph);
     if (le != hipSuccess || ce != hipSuccess) {
@@ -2087,8 +2269,11 @@ extern "C" int sc_plan_create(sc_ctx *ctx, const sc_index *ix, const sc_search_p
     fa.lut_dqc = P->lut;
     fa.lut_dqr = P->lut ? P->lut + (size_t)nq * ix->mc * kc : nullptr;
     fa.phase = P->stats + 8;
+    fa.dot_absmax = ix->dot_absmax;
+    fa.margin_scale = mixed_margin_scale();
+    P->mixed = p->form == SC_FORM_MIXED;
     for (int v = 3; v <= 5; ++v)
-        if (hipFuncSetAttribute((const void *)fused_kernel(v),
+        if (hipFuncSetAttribute((const void *)fused_kernel(v, P->mixed),
                                 hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)P->lds) != hipSuccess) {
             sc_plan_destroy(P);
@@ -2142,6 +2327,7 @@ extern "C" int sc_plan_execute(sc_plan *P, sc_search_stats *st) {
         st->cells = (int64_t)h[1];
         st->visited = (int64_t)h[2];
         st->exhausted = (int64_t)h[3];
+        st->reranked = (int64_t)h[6];
         st->ms_scan = ms;
     }
     return finish_flagged(P->ctx, P->ix, &P->p, P->d_q, P->nq, P->flags, nflag, P->d_ids,
@@ -2173,7 +2359,7 @@ extern "C" int sc_plan_autotune(sc_plan *P, int32_t *out4) {
             for (int rep = 0; rep < 3; ++rep) {
                 hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, sm, P->counter, 2);
                 SC_HIP(hipEventRecord(e0, sm));
-                SC_HIP(launch_fused(v, grid, threads, P->lds, sm, P->fa));
+                SC_HIP(launch_fused(v, P->mixed, grid, threads, P->lds, sm, P->fa));
                 SC_HIP(hipEventRecord(e1, sm));
                 SC_HIP(hipEventSynchronize(e1));
                 float ms = 0.0f;

@@ -123,9 +123,9 @@ SEARCH_CASES = [  # kind, R, w, T
 ]
 
 
-def _compare_search(ctx, ix, oix, kind, R, w, T, nq=150):
+def _compare_search(ctx, ix, oix, kind, R, w, T, nq=150, form=S.FORM_PLAIN):
     q = sco.gen(sco.mix_d128(ncomp=11), sco.STREAM_QUERY, 0, nq, ix.dim)
-    ids, dists, sums, st = ix.search(S.Dev.from_numpy(ctx, q), kind, R, w, T)
+    ids, dists, sums, st = ix.search(S.Dev.from_numpy(ctx, q), kind, R, w, T, form=form)
     okind = {S.KIND_IVFADC: sco.IVFADC, S.KIND_IMI: sco.IMI, S.KIND_SC: sco.SC}[kind]
     oids, od, osum = oix.search_batch(okind, q, R, w, T)
     assert np.array_equal(sums.numpy(), osum)
@@ -143,9 +143,34 @@ def test_search_matches_oracle(ctx, case):
     ix.close()
 
 
-@pytest.mark.parametrize("kind,kc,kp", [(S.KIND_SC, 16, 64), (S.KIND_SC, 64, 64),
-                                        (S.KIND_IVFADC, 64, 64), (S.KIND_IMI, 16, 64)])
-def test_search_plan_replays_match_oracle(ctx, kind, kc, kp):
+@pytest.mark.parametrize("case", [c for c in SEARCH_CASES if c[0] != S.KIND_IMI])
+@pytest.mark.parametrize("kc,kp", [(16, 64), (64, 64)])
+def test_search_mixed_form_matches_oracle(ctx, case, kc, kp):
+    """SC_FORM_MIXED (FP16 table scan + exact re-rank) returns the plain form's
+    ids and distances bit for bit; kc = kp = 64 takes the precomputed-LUT path."""
+    kind, R, w, T = case
+    ix, oix = _index_pair(ctx, kind, kc=kc, kp=kp)
+    st = _compare_search(ctx, ix, oix, kind, R, w, T, form=S.FORM_MIXED)
+    assert st.reranked > 0 or st.candidates == 0     # the exact re-rank ran
+    ix.close()
+
+
+def test_search_mixed_form_overflow_redo(ctx, monkeypatch):
+    """An inflated error margin makes the kept set outgrow the buffer; those
+    queries are redone exactly and the results must not change."""
+    ix, oix = _index_pair(ctx, S.KIND_SC, kc=64, kp=64)
+    monkeypatch.setenv("SC_MIXED_MARGIN", "1e9")
+    _compare_search(ctx, ix, oix, S.KIND_SC, 50, 300, 2500, nq=40, form=S.FORM_MIXED)
+    monkeypatch.delenv("SC_MIXED_MARGIN")
+    ix.close()
+
+
+@pytest.mark.parametrize("kind,kc,kp,form", [(S.KIND_SC, 16, 64, S.FORM_PLAIN),
+                                             (S.KIND_SC, 64, 64, S.FORM_PLAIN),
+                                             (S.KIND_IVFADC, 64, 64, S.FORM_PLAIN),
+                                             (S.KIND_IMI, 16, 64, S.FORM_PLAIN),
+                                             (S.KIND_SC, 64, 64, S.FORM_MIXED)])
+def test_search_plan_replays_match_oracle(ctx, kind, kc, kp, form):
     """A captured plan replayed several times, with eager searches and new
     query contents between the replays, and again after autotuning; every
     replay must equal the oracle (kc = kp = 64 takes the tiled LUT kernel
@@ -155,7 +180,7 @@ def test_search_plan_replays_match_oracle(ctx, kind, kc, kp):
     R, w, T, nq = 50, (12 if kind == S.KIND_IVFADC else 300), 2500, 130
     qs = [sco.gen(sco.mix_d128(ncomp=11), sco.STREAM_QUERY, i * nq, nq, ix.dim) for i in range(3)]
     dq = S.Dev.from_numpy(ctx, qs[0])
-    plan = S.Plan(ix, dq, kind, R, w, T)
+    plan = S.Plan(ix, dq, kind, R, w, T, form=form)
     other = S.Dev.from_numpy(ctx, qs[2][:40])
     for rep in range(5):
         if rep == 3:
