@@ -1,0 +1,48 @@
+"""Builds the host emulation of the fused search kernel (test infrastructure).
+
+The device code of second-candidate_amd/csrc/sc_search.hip (its anonymous
+namespace up to the host-side plan code) is compiled as C++ against shim.h and
+linked with the oracle library and driver.inc.  Usage:
+    python make_emu.py OUT_BINARY SANITIZER   (SANITIZER: address | thread)
+"""
+import os
+import re
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+SRC = os.path.join(ROOT, "second-candidate_amd", "csrc", "sc_search.hip")
+ORACLE = os.path.join(ROOT, "oracle")
+
+
+def generate(path):
+    src = open(SRC).read()
+    # launch helpers use the HIP runtime; the emulation calls the kernel directly
+    src = re.sub(r"using FusedKernel = .*?(?=__global__ void k_zero_ints)", "", src, flags=re.S)
+    a, b = src.index("namespace {"), src.index("struct Plan {")
+    body = src[a:b]
+    body = body.replace("extern __shared__ __align__(16) unsigned char smem[];",
+                        "unsigned char *smem = g_smem;")
+    body = body.replace('asm volatile("" : "+v"(v));', "")
+    out = ['#include "shim.h"', "#include <vector>", "extern unsigned char *g_smem;", body,
+           "} // namespace",
+           "void emu_run_fused(const FusedArgs &a, bool mixed) {",
+           "    if (mixed) k_search_fused<3, true>(a); else k_search_fused<3, false>(a);",
+           "}", open(os.path.join(HERE, "driver.inc")).read()]
+    open(path, "w").write("\n".join(out))
+
+
+def build(out, sanitizer, cxx=None):
+    cxx = cxx or os.environ.get("EMU_CXX", "/opt/rocm/llvm/bin/clang++")
+    cpp = out + ".cpp"
+    generate(cpp)
+    cmd = [cxx, "-std=c++17", "-O1", "-g", "-fsanitize=" + sanitizer, "-fno-omit-frame-pointer",
+           "-I" + HERE, "-I" + ORACLE, cpp, "-o", out, "-L" + ORACLE, "-loracle",
+           "-Wl,-rpath," + ORACLE, "-lpthread"]
+    subprocess.run(cmd, check=True)
+    return out
+
+
+if __name__ == "__main__":
+    build(sys.argv[1], sys.argv[2])
