@@ -50,6 +50,8 @@ def main():
     ap.add_argument("--recall-sample", type=int, default=100)
     ap.add_argument("--plan", type=int, default=1,
                     help="1: time the autotuned hipGraph plan (sc_plan_*); 0: sc_search calls")
+    ap.add_argument("--sweep", type=str, default="1000,2000,5000,10000,20000",
+                    help="T values of the QPS/recall curve ('' = skip)")
     ap.add_argument("--mixed", type=int, default=1,
                     help="1: also time the mixed-precision form (FP16 table + exact re-rank)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
@@ -102,6 +104,16 @@ def main():
     cid, codes, _ = S.encode(ctx, x, cq, dpq, 1, kc, mp, kp, 2, S.PREC_FP32)
     ctx.sync()
     t_encode = time.perf_counter() - t0
+    # the mixed-precision assignment (BF16 MFMA distances + exact re-check)
+    t0 = time.perf_counter()
+    cid16, codes16, est16 = S.encode(ctx, x, cq, dpq, 1, kc, mp, kp, 2, S.PREC_BF16)
+    ctx.sync()
+    t_encode16 = time.perf_counter() - t0
+    enc16_same = bool(np.array_equal(cid16.numpy(), cid.numpy()) and
+                      np.array_equal(codes16.numpy(), codes.numpy()))
+    log("encode: fp32 exact %.4fs, bf16+re-check %.4fs (identical: %s, %.1f re-checked/row)"
+        % (t_encode, t_encode16, enc16_same, est16.rechecked / args.n))
+    del cid16, codes16
     ix = S.Index.build(ctx, cid, codes, dim, cq, 1, kc, dpq, mp, kp, 2)
     del cid, codes
     q = S.gen_rows(ctx, mix, S.STREAM_QUERY, 0, args.nq, dim)
@@ -151,9 +163,11 @@ def main():
     st = stats[-1]
     cand = st.candidates
     fused = st0.ms_rank == 0.0 and st0.ms_traverse == 0.0
-    # algorithmic bytes of the dominant kernel: the codes of every candidate,
-    # plus (fused: the query vectors) and the returned ids and distances
-    scan_bytes = cand * mp + args.nq * args.R * 8 + (args.nq * dim * 4 if fused else 0)
+    # algorithmic bytes of the dominant kernel: (mp + 4) bytes per candidate
+    # (its codes and its id, as the reference's scan copies both,
+    # include/sc_query.h:197-239), the query vectors and the R ids and
+    # distances written per query
+    scan_bytes = cand * (mp + 4) + args.nq * dim * 4 + args.nq * args.R * 8
     achieved = scan_bytes / (ms_kernel * 1e-3) / 1e9
     log("batched: %.1f ms/step, %.0f QPS; stages lut %.3f rank %.3f trav %.3f scan %.3f ms; "
         "%.0f candidates, %.1f cells, %.1f visited per query"
@@ -218,6 +232,29 @@ def main():
         recall = sco.recall(gt, gids[:ns], args.R)
         del xh
         log("recall@%d on %d queries: %.3f" % (args.R, ns, recall))
+
+    # ---------------- QPS / recall curve over T (same R, both forms, batched)
+    curve = []
+    if args.sweep and args.plan and args.recall_sample > 0:
+        for tv in [int(v) for v in args.sweep.split(",") if v]:
+            row = {"T": tv}
+            for form, name in ((S.FORM_PLAIN, "plain"), (S.FORM_MIXED, "mixed")):
+                pl = S.Plan(ix, q, S.KIND_SC, args.R, args.w, tv, form=form)
+                pl.autotune()
+                pl.execute(stats=False)
+                ctx.sync()
+                t0 = time.perf_counter()
+                for _ in range(args.steps):
+                    pl.execute(stats=False)
+                ctx.sync()
+                row["qps_" + name] = round(args.nq * args.steps / (time.perf_counter() - t0), 1)
+                if form == S.FORM_PLAIN:
+                    row["recall_at_%d" % args.R] = round(
+                        sco.recall(gt, pl.ids.numpy()[:ns], args.R), 4)
+                pl.close()
+            curve.append(row)
+            log("T=%d: recall %.3f, plain %.0f QPS, mixed %.0f QPS"
+                % (tv, row["recall_at_%d" % args.R], row["qps_plain"], row["qps_mixed"]))
 
     # ---------------- CPU baseline: the OpenMP oracle on a bounded sample
     cpu = None
@@ -284,7 +321,10 @@ def main():
             "launch": ("hipGraph plan, autotuned: %d threads x %d workgroups, register variant "
                        "%d waves/SIMD" % (tune["threads"], tune["grid"], tune["variant"]))
                       if plan else "sc_search calls",
+            "qps_recall_curve": curve,
             "encode_s": round(t_encode, 4),
+            "encode_bf16_s": round(t_encode16, 4),
+            "encode_bf16_identical": enc16_same,
             "encode_fp64_tflops": round(encode_flops / t_encode / 1e12, 2),
             "train_s": round(t_train, 2),
         },

@@ -218,39 +218,44 @@ struct Heap {
     __device__ __forceinline__ float &V(int i) { return v[(size_t)i * stride]; }
     __device__ __forceinline__ uint32_t &P(int i) { return p[(size_t)i * stride]; }
 
-    __device__ void insert(float val, uint32_t id) {
+    // Sift-up and sift-down move the entry being placed through a hole kept
+    // in registers: one round trip to LDS per level (parent, or both
+    // children with their ids) instead of a read-compare-swap chain.  The
+    // comparisons and tie rules are exactly those of the array heap of
+    // include/sc_utilities.h:469-585, so the pop order is the same.
+    __device__ __forceinline__ void insert(float val, uint32_t id) {
         if (pos >= cap_ref || pos < 0) return;           // silently dropped
         if (pos >= cap_phys) { overflow = true; return; }
         int i = pos++;
-        V(i) = val; P(i) = id;
-        int j = i >> 1;
         while (i > 0) {
-            float vi = V(i), vj = V(j);
-            if (vi < vj) {
-                V(j) = vi; V(i) = vj;
-                uint32_t t = P(i); P(i) = P(j); P(j) = t;
-            } else break;
-            i = j;
-            j = i >> 1;
+            const int j = i >> 1;
+            const float vj = V(j);
+            const uint32_t pj = P(j);
+            if (val < vj) { V(i) = vj; P(i) = pj; i = j; } else break;
         }
+        V(i) = val;
+        P(i) = id;
     }
-    __device__ uint32_t pop() {
-        uint32_t top = P(0);
-        int last = --pos;
-        V(0) = V(last);
-        P(0) = P(last);
+    __device__ __forceinline__ uint32_t pop() {
+        const uint32_t top = P(0);
+        const int last = --pos;
+        const float mv = V(last);
+        const uint32_t mpid = P(last);
         int i = 0, j = 0;
-        while (j + 1 < pos) {
-            float a = V(j), b = V(j + 1);
-            int mx = (a < b) ? j : j + 1;
-            float vm = (a < b) ? a : b, vi = V(i);
-            if (vm < vi) {
-                uint32_t t = P(i); P(i) = P(mx); P(mx) = t;
-                V(i) = vm; V(mx) = vi;
-                i = mx;
+        while (j + 1 < pos) {      // a lone last child is not compared
+            // at the first level the left "child" is the hole itself
+            const float a = j == i ? mv : V(j), b = V(j + 1);
+            const uint32_t pa = P(j), pb = P(j + 1);
+            const bool left = a < b;
+            if ((left ? a : b) < mv) {
+                V(i) = left ? a : b;
+                P(i) = left ? pa : pb;
+                i = left ? j : j + 1;
             } else break;
             j = i << 1;
         }
+        V(i) = mv;
+        P(i) = mpid;
         return top;
     }
 };
@@ -826,7 +831,7 @@ __device__ __forceinline__ void wave_append(int *counter, uint64_t *key, int32_t
 // R <= nb.  Radix select on 8-bit digits that starts below the bits all keys
 // share and stops as soon as the digit bin holding the R-th key is taken
 // whole.  All threads call it.
-__device__ uint64_t select_threshold(const uint64_t *key, int nb, int R, TopShared &T) {
+__device__ __forceinline__ uint64_t select_threshold(const uint64_t *key, int nb, int R, TopShared &T) {
     const int tid = threadIdx.x, NT = blockDim.x, lane = tid & 63;
     uint64_t ka = ~0ull, ko = 0;
     for (int i = tid; i < nb; i += NT) { ka &= key[i]; ko |= key[i]; }
@@ -895,7 +900,7 @@ __device__ uint64_t select_threshold(const uint64_t *key, int nb, int R, TopShar
 // survivors move left chunk by chunk of blockDim.x entries: a chunk is read
 // whole before any of its survivors is written, and every write lands below
 // the chunk's end, so nothing unread is overwritten.  All threads call it.
-__device__ void compact_top(TopShared &T, uint64_t *key, int32_t *idx, int R) {
+__device__ __forceinline__ void compact_top(TopShared &T, uint64_t *key, int32_t *idx, int R) {
     const int tid = threadIdx.x, NT = blockDim.x;
     __syncthreads();
     const int nb = T.nb;
@@ -916,7 +921,7 @@ __device__ void compact_top(TopShared &T, uint64_t *key, int32_t *idx, int R) {
 // Mixed form: keeps every key whose approximate distance is within twoE of
 // the R-th smallest (at least R keys).  When more than lim keys would stay it
 // keeps exactly R and sets T.ovf: the query is then redone exactly.
-__device__ void compact_top_margin(TopShared &T, uint64_t *key, int32_t *idx, int R, float twoE,
+__device__ __forceinline__ void compact_top_margin(TopShared &T, uint64_t *key, int32_t *idx, int R, float twoE,
                                    int lim) {
     const int tid = threadIdx.x, NT = blockDim.x;
     __syncthreads();
@@ -990,7 +995,7 @@ __device__ __forceinline__ void wave_bitonic(uint64_t (&k)[E], int32_t (&v)[E]) 
 }
 
 template <int E>
-__device__ void wave_sort_lds(uint64_t *key, int32_t *idx, int n) {   // wave 0 only
+__device__ __forceinline__ void wave_sort_lds(uint64_t *key, int32_t *idx, int n) {   // wave 0 only
     const int lane = threadIdx.x & 63;
     uint64_t k[E];
     int32_t v[E];
@@ -1010,7 +1015,7 @@ __device__ void wave_sort_lds(uint64_t *key, int32_t *idx, int n) {   // wave 0 
 
 // Sorts key/idx[0..n) ascending; the buffer holds at least next_pow2(n)
 // entries.  All threads call it.
-__device__ void sort_top(uint64_t *key, int32_t *idx, int n) {
+__device__ __forceinline__ void sort_top(uint64_t *key, int32_t *idx, int n) {
     const int tid = threadIdx.x, NT = blockDim.x;
     __syncthreads();
     if (n <= 256) {
@@ -1055,8 +1060,10 @@ struct GroupCells {
 // MPW = 32-bit words of codes per candidate (mp = 4 * MPW); 0 = any mp.
 // MIXED: the table holds FP16 values of (diff_qr + dot_cr) / S; the
 // approximate distance d0 + S * sum is kept within a margin of twoE.
+// (helpers are force-inlined: the kernel's register-variant attribute keeps
+// the inliner from inlining ordinary calls, and calls spill registers)
 template <int MPW, bool MIXED>
-__device__ void scan_group(const ScanArgs &a, TopShared &T, uint64_t *key, int32_t *idx,
+__device__ __forceinline__ void scan_group(const ScanArgs &a, TopShared &T, uint64_t *key, int32_t *idx,
                            int bufcap, const float *lut, float d0, int p0, int p1, int i0,
                            const GroupCells &G, float S, float twoE) {
     const _Float16 *lut16 = reinterpret_cast<const _Float16 *>(lut);

@@ -92,6 +92,8 @@ template <class T> inline T __shfl_up(T v, int d) {
 }
 inline int __builtin_amdgcn_readfirstlane_emu(int v) { return shfl_any(v, 0); }
 #define __builtin_amdgcn_readfirstlane(x) __builtin_amdgcn_readfirstlane_emu((int)(x))
+inline int __builtin_amdgcn_readlane_emu(int v, int l) { return shfl_any(v, l); }
+#define __builtin_amdgcn_readlane(x, l) __builtin_amdgcn_readlane_emu((int)(x), (int)(l))
 template <class T, class U> inline T atomicAdd(T *p, U v) { return __atomic_fetch_add(p, (T)v, __ATOMIC_SEQ_CST); }
 template <class T, class U> inline T atomicAnd(T *p, U v) { return __atomic_fetch_and(p, (T)v, __ATOMIC_SEQ_CST); }
 template <class T, class U> inline T atomicOr(T *p, U v) { return __atomic_fetch_or(p, (T)v, __ATOMIC_SEQ_CST); }
