@@ -30,6 +30,7 @@ constexpr int kMaxFusedThreads = 512;   // fused workgroups run 256 or 512 threa
 constexpr int kBufCap = 2048;      // streaming top-R buffer (entries)
 constexpr int kRunCells = 256;     // cells staged per scan run
 constexpr int kSpec = 16;          // traversal pops run ahead of the L[] loads
+constexpr int kWin = 16;           // SC: cell sizes of the kWin x kWin nearest rank pairs
 
 // ----------------------------------------------------------------------------
 // Stage 1: LUTs.  out[q][0 .. mc*kc) = diff_qc, out[q][mc*kc .. +mp*kp) = diff_qr
@@ -762,6 +763,7 @@ struct FusedArgs {
     unsigned long long *phase;        // [6]: wall-clock ticks per phase, workgroups
     float dot_absmax;                 // max |dot_cr| (mixed form error bound)
     float margin_scale;               // mixed form: multiplies the error bound (1)
+    int nvec;                         // vectors in the index (code loads stay below it)
 };
 
 struct FusedLayout {
@@ -779,7 +781,9 @@ __host__ __device__ inline FusedLayout fused_layout(int dim, int mc, int kc, int
     auto up = [](size_t v) { return (v + 15) & ~(size_t)15; };
     FusedLayout L{};
     size_t a = (size_t)bufcap * 12;
-    const size_t heap = (size_t)hcap * 8 + (size_t)kc * 4;
+    // heap (value, id) pairs, SC row lengths, then the SC window of
+    // (start, length) per rank pair
+    const size_t heap = (size_t)hcap * 8 + (size_t)kc * 4 + (size_t)kWin * kWin * 8;
     if (heap > a) a = heap;
     if (npow2 > 256 && (size_t)npow2 * 8 > a) a = (size_t)npow2 * 8;
     L.a = 0;
@@ -1065,9 +1069,10 @@ struct GroupCells {
 template <int MPW, bool MIXED>
 __device__ __forceinline__ void scan_group(const ScanArgs &a, TopShared &T, uint64_t *key, int32_t *idx,
                            int bufcap, const float *lut, float d0, int p0, int p1, int i0,
-                           const GroupCells &G, float S, float twoE) {
+                           const GroupCells &G, float S, float twoE, int nvec) {
+    constexpr int W = MPW > 0 ? MPW : 1;
     const _Float16 *lut16 = reinterpret_cast<const _Float16 *>(lut);
-    const int tid = threadIdx.x, NT = blockDim.x, U = kRound / NT;
+    const int tid = threadIdx.x, NT = blockDim.x, U = kRound / NT, lane = tid & 63;
     const int mp = a.mp, kp = a.kp;
     int cur = i0, g0 = G.gp[i0], g1 = G.gp[i0 + 1];
     int cs = 0, cp = 0;
@@ -1076,18 +1081,11 @@ __device__ __forceinline__ void scan_group(const ScanArgs &a, TopShared &T, uint
         cs = G.cstart[c];
         cp = G.cpos[c];
     }
-    for (int r0 = p0; r0 < p1; r0 += kRound) {
-        // every thread reads the fill before any wave appends this round, so
-        // all of them take the same compaction decision
-        const bool full = T.nb > bufcap - kRound;
-        __syncthreads();
-        if (full) {
-            if (MIXED) compact_top_margin(T, key, idx, a.R, twoE, bufcap - kRound);
-            else compact_top(T, key, idx, a.R);
-        }
-        const uint64_t thr = T.thr;
-        uint32_t cw[4][MPW > 0 ? MPW : 1];
-        int vx[4], ps[4];
+    uint32_t cw[4][W];
+    int vx[4], ps[4];
+    // positions r0 + u * NT + tid of a round: cell pointer, vector index
+    // (kept inside the codes array) and codes, whose loads stay in flight
+    auto fetch = [&](int r0) {
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             vx[u] = -1;
@@ -1102,23 +1100,38 @@ __device__ __forceinline__ void scan_group(const ScanArgs &a, TopShared &T, uint
                     cs = G.cstart[c];
                     cp = G.cpos[c];
                 }
-                vx[u] = cs + (p - g0);
+                vx[u] = min(cs + (p - g0), nvec - 1);
                 ps[u] = cp + (p - g0);
-                if (MPW > 0) load_codes<(MPW > 0 ? MPW : 1)>(a.codes, vx[u], cw[u]);
+                if (MPW > 0) load_codes<W>(a.codes, vx[u], cw[u]);
             }
         }
+    };
+    fetch(p0);
+    for (int r0 = p0; r0 < p1; r0 += kRound) {
+        // every thread reads the fill before any wave appends this round, so
+        // all of them take the same compaction decision
+        const bool full = T.nb > bufcap - kRound;
+        __syncthreads();
+        if (full) {
+            if (MIXED) compact_top_margin(T, key, idx, a.R, twoE, bufcap - kRound);
+            else compact_top(T, key, idx, a.R);
+        }
+        const uint64_t thr = T.thr;
+        uint64_t k[4];
+        int32_t v[4];
+        bool take[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            if (u >= U) continue;    // U is the same for every thread
-            bool take = false;
-            uint64_t k = 0;
-            if (vx[u] >= 0) {
+            take[u] = false;
+            k[u] = 0;
+            v[u] = vx[u];
+            if (u < U && vx[u] >= 0) {
                 float d = d0;
                 if (MIXED) {
                     float acc = 0.0f;
                     if (MPW > 0) {
 #pragma unroll
-                        for (int w4 = 0; w4 < (MPW > 0 ? MPW : 1); ++w4) {
+                        for (int w4 = 0; w4 < W; ++w4) {
                             const uint32_t wv = cw[u][w4];
 #pragma unroll
                             for (int b = 0; b < 4; ++b)
@@ -1131,7 +1144,7 @@ __device__ __forceinline__ void scan_group(const ScanArgs &a, TopShared &T, uint
                     d = d0 + S * acc;
                 } else if (MPW > 0) {
 #pragma unroll
-                    for (int w4 = 0; w4 < (MPW > 0 ? MPW : 1); ++w4) {
+                    for (int w4 = 0; w4 < W; ++w4) {
                         const uint32_t wv = cw[u][w4];
 #pragma unroll
                         for (int b = 0; b < 4; ++b)
@@ -1141,16 +1154,40 @@ __device__ __forceinline__ void scan_group(const ScanArgs &a, TopShared &T, uint
                     const uint8_t *cc = a.codes + (size_t)vx[u] * mp;
                     for (int j = 0; j < mp; ++j) d += lut[j * kp + cc[j]];
                 }
-                k = ((uint64_t)sc_float_key(d) << 32) | (uint32_t)ps[u];
-                take = k < thr;
+                k[u] = ((uint64_t)sc_float_key(d) << 32) | (uint32_t)ps[u];
+                take[u] = k[u] < thr;
             }
-            wave_append(&T.nb, key, idx, take, k, vx[u]);
+        }
+        // the next round's codes load while this round appends and waits
+        if (r0 + kRound < p1) fetch(r0 + kRound);
+        // the round's survivors: one LDS atomic per wave
+        uint64_t bal[4];
+        int tot = 0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            bal[u] = __ballot(take[u]);
+            tot += __popcll(bal[u]);
+        }
+        if (tot > 0) {
+            int base = 0;
+            if (lane == 0) base = atomicAdd(&T.nb, tot);
+            base = __shfl(base, 0);
+            const uint64_t lt = (1ull << lane) - 1ull;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (take[u]) {
+                    const int o = base + __popcll(bal[u] & lt);
+                    key[o] = k[u];
+                    idx[o] = v[u];
+                }
+                base += __popcll(bal[u]);
+            }
         }
         __syncthreads();
     }
 }
 
-constexpr int kLutRegs = 8;
+constexpr int kLutRegs = 8;    // table entries per thread kept in registers
 
 // The traversal runs on one lane; left alone, the compiler treats its loaded
 // values as wave-uniform and moves each into a scalar register right after its
@@ -1159,7 +1196,7 @@ constexpr int kLutRegs = 8;
 __device__ __forceinline__ int64_t lane_value(int64_t v) {
     asm volatile("" : "+v"(v));
     return v;
-}    // table entries per thread kept in registers
+}
 
 // OCC = waves per SIMD the register allocation must allow (3, 4 or 5): more
 // workgroups in flight hide the serial phases of one query behind the others
@@ -1335,12 +1372,28 @@ void k_search_fused(FusedArgs a) {
         __syncthreads();
         mark(1);
         // traversal: thread 0, heap and row lengths in LDS (region A)
+        float *hv = reinterpret_cast<float *>(smem + Lay.a);
+        int32_t *rowlen = reinterpret_cast<int32_t *>(hv + 2 * a.hcap);
+        int32_t *win = rowlen + kc;    // SC: [kWin][kWin] (start, length) by rank pair
+        const int wk = min(kWin, kc);
+        if (sc) {
+            // all threads: row lengths, and the sizes of the cells of the
+            // nearest rank pairs in one round of loads, so the traversal
+            // knows them without waiting (cell 0 = (0, 0) is never a pair)
+            for (int r = tid; r < kc; r += NT) rowlen[r] = r == 0 ? 1 : 0;
+            for (int i = tid; i < wk * wk; i += NT) {
+                const int ra = i / wk, rb = i - ra * wk;
+                const int64_t b = (int64_t)si[ra] * kc + si[rb];
+                const int32_t hi = b > 0 ? sa.L[b] : 0, lo = b > 0 ? sa.L[b - 1] : 0;
+                win[2 * i] = lo;
+                win[2 * i + 1] = hi - lo;
+            }
+            __syncthreads();
+        }
         if (tid == 0) {
             Heap h;
-            float *hv = reinterpret_cast<float *>(smem + Lay.a);
             h.v = hv;
             h.p = reinterpret_cast<uint32_t *>(hv + a.hcap);
-            int32_t *rowlen = reinterpret_cast<int32_t *>(h.p + a.hcap);
             h.stride = 1;
             h.cap_phys = a.hcap;
             h.pos = 0;
@@ -1377,25 +1430,27 @@ void k_search_fused(FusedArgs a) {
                 }
             } else {
                 h.cap_ref = a.w;
-                for (int r = 0; r < kc; ++r) rowlen[r] = 0;
-                rowlen[0] = 1;
                 const float d0 = sv[0] + sv[1];
                 h.insert(d0, 1u);
                 h.insert(d0, 1u << 16);
                 // The pops and pushes do not depend on the cell sizes until the
-                // loop stops, so pops run ahead in batches of kSpec, their L[]
-                // entries load together, and a sequential commit stops exactly
-                // where the reference does (pushes past that point are moot).
+                // loop stops.  Pops of rank pairs inside the window know their
+                // cell sizes at once; while every pop of a batch does, the
+                // batch ends exactly where the loop stops.  Past the window,
+                // pops run ahead in batches of kSpec, their L[] entries load
+                // together, and a sequential commit stops exactly where the
+                // reference does (pushes past that point are moot).
                 bool done = (count >= a.w || sum >= a.T);
                 while (!done) {
-                    int nsp = 0;
-                    bool dry = false;
+                    int nsp = 0, kcount = count, ksum = sum;
+                    bool dry = false, known = true;
                     while (nsp < kSpec && !h.overflow) {
                         if (h.pos <= 0) { dry = true; break; }
                         const uint32_t pk = h.pop();
                         const int h1 = (int)(pk >> 16), h2 = (int)(pk & 0xFFFFu);
                         const int h3 = si[h1], h4 = si[h2];
-                        s_sbid[nsp] = (int64_t)h3 * kc + h4;
+                        const bool inw = h1 < wk && h2 < wk;
+                        s_sbid[nsp] = inw ? -1 - (int64_t)(h1 * wk + h2) : (int64_t)h3 * kc + h4;
                         s_sinfo[nsp] = (uint32_t)h3 | ((uint32_t)h4 << 16);
                         ++nsp;
                         rowlen[h1] = h2 + 1;
@@ -1403,13 +1458,25 @@ void k_search_fused(FusedArgs a) {
                             h.insert(sv[h1 + 1] + sv[h2], ((uint32_t)(h1 + 1) << 16) | (uint32_t)h2);
                         if (h2 < kc - 1 && (h1 == 0 || (h2 + 1) < rowlen[h1 - 1]))
                             h.insert(sv[h1] + sv[h2 + 1], ((uint32_t)h1 << 16) | (uint32_t)(h2 + 1));
+                        if (!inw) known = false;
+                        if (known) {
+                            const int l = win[2 * (h1 * wk + h2) + 1];
+                            kcount += l > 0;
+                            ksum += l;
+                            if (kcount >= a.w || ksum >= a.T) break;   // the loop stops here
+                        }
                     }
                     int32_t lhi[kSpec], llo[kSpec];
 #pragma unroll
                     for (int i = 0; i < kSpec; ++i) {
                         const int64_t b = lane_value(i < nsp ? s_sbid[i] : 0);
-                        lhi[i] = b > 0 ? sa.L[b] : 0;
-                        llo[i] = b > 0 ? sa.L[b - 1] : 0;
+                        if (b < 0) {
+                            llo[i] = win[2 * (-1 - b)];
+                            lhi[i] = llo[i] + win[2 * (-1 - b) + 1];
+                        } else {
+                            lhi[i] = b > 0 ? sa.L[b] : 0;
+                            llo[i] = b > 0 ? sa.L[b - 1] : 0;
+                        }
                     }
 #pragma unroll
                     for (int i = 0; i < kSpec; ++i) {
@@ -1449,29 +1516,58 @@ void k_search_fused(FusedArgs a) {
         mark(2);
         if (!s_flag) {
             const int ncell = s_ncell, sum = s_sum;
-            // group the cells by h3: counts, offsets and group list (wave 0),
-            // permutation, then the pass positions gp of the permuted cells
-            for (int j = tid; j < kc; j += NT) ghist[j] = 0;
+            // group the cells by h3, the groups in order of first appearance
+            // in the traversal (the nearest cells come first, so the top-R
+            // threshold tightens early and the buffer compacts less often):
+            // sizes and first cells; group ranks (wave 0); group offsets
+            // (wave 0); permutation; then the pass positions gp of the
+            // permuted cells.  The scan order does not change results: every
+            // candidate keeps its traversal position.
+            int32_t *gfirst = reinterpret_cast<int32_t *>(smem + Lay.a);   // region A is free
+            int32_t *gsize = gfirst + kc;                                  // by group rank
+            for (int j = tid; j < kc; j += NT) { ghist[j] = 0; gfirst[j] = ncell; }
             __syncthreads();
-            for (int c = tid; c < ncell; c += NT) atomicAdd(&ghist[cinfo[c] & 0xFFFFu], 1);
+            for (int c = tid; c < ncell; c += NT) {
+                const int h = (int)(cinfo[c] & 0xFFFFu);
+                atomicAdd(&ghist[h], 1);
+                atomicMin(&gfirst[h], c);
+            }
             __syncthreads();
             if (tid < 64) {
-                const int per = (kc + 63) / 64, b0 = tid * per, b1 = min(kc, b0 + per);
-                int cnt = 0, ne = 0;
-                for (int b = b0; b < b1; ++b) { const int h = ghist[b]; cnt += h; ne += h > 0; }
-                int ic = cnt, in = ne;
+                const int per = (ncell + 63) / 64, c0 = min(ncell, tid * per),
+                          c1 = min(ncell, c0 + per);
+                int nl = 0;
+                for (int c = c0; c < c1; ++c) nl += gfirst[cinfo[c] & 0xFFFFu] == c;
+                int inc = nl;
                 for (int o = 1; o < 64; o <<= 1) {
-                    const int t1 = __shfl_up(ic, o), t2 = __shfl_up(in, o);
-                    if (tid >= o) { ic += t1; in += t2; }
+                    const int t = __shfl_up(inc, o);
+                    if (tid >= o) inc += t;
                 }
-                int oc = ic - cnt, on = in - ne;
-                for (int b = b0; b < b1; ++b) {
-                    const int h = ghist[b];
-                    if (h > 0) gdesc[on++] = (uint32_t)b | ((uint32_t)oc << 16);
-                    ghist[b] = oc;
-                    oc += h;
+                int r = inc - nl;
+                for (int c = c0; c < c1; ++c) {
+                    const int h = (int)(cinfo[c] & 0xFFFFu);
+                    if (gfirst[h] == c) { gdesc[r] = (uint32_t)h; gsize[r] = ghist[h]; ++r; }
                 }
-                if (tid == 63) s_ngroup = in;
+                if (tid == 63) s_ngroup = inc;
+            }
+            __syncthreads();
+            if (tid < 64) {
+                const int ngr = s_ngroup, per = (ngr + 63) / 64, r0 = min(ngr, tid * per),
+                          r1 = min(ngr, r0 + per);
+                int cnt = 0;
+                for (int r = r0; r < r1; ++r) cnt += gsize[r];
+                int inc = cnt;
+                for (int o = 1; o < 64; o <<= 1) {
+                    const int t = __shfl_up(inc, o);
+                    if (tid >= o) inc += t;
+                }
+                int oc = inc - cnt;
+                for (int r = r0; r < r1; ++r) {
+                    const uint32_t h = gdesc[r];
+                    gdesc[r] = h | ((uint32_t)oc << 16);
+                    ghist[h] = oc;
+                    oc += gsize[r];
+                }
             }
             __syncthreads();
             for (int c = tid; c < ncell; c += NT) {
@@ -1554,13 +1650,13 @@ void k_search_fused(FusedArgs a) {
                 const int p0 = gp[i0], p1 = gp[i1];
                 if (mp == 8)
                     scan_group<2, MIXED>(sa, T, bkey, bidx, a.bufcap, lut, d0, p0, p1, i0, G, S,
-                                         twoE);
+                                         twoE, a.nvec);
                 else if (mp == 16)
                     scan_group<4, MIXED>(sa, T, bkey, bidx, a.bufcap, lut, d0, p0, p1, i0, G, S,
-                                         twoE);
+                                         twoE, a.nvec);
                 else
                     scan_group<0, MIXED>(sa, T, bkey, bidx, a.bufcap, lut, d0, p0, p1, i0, G, S,
-                                         twoE);
+                                         twoE, a.nvec);
                 __syncthreads();
             }
             mark(3);
@@ -1873,6 +1969,7 @@ static int search_fused(sc_ctx *ctx, const sc_index *ix, const float *d_q, int64
     fa.phase = st ? stats + 8 : nullptr;     // 6 counters after the 8 statistics
     fa.dot_absmax = ix->dot_absmax;
     fa.margin_scale = mixed_margin_scale();
+    fa.nvec = (int)ix->n;
     if (st) SC_HIP(hipMemsetAsync(stats + 8, 0, 48, sm));
     hipEvent_t e0 = nullptr, e1 = nullptr, el = nullptr;
     if (st) {
@@ -2278,6 +2375,7 @@ extern "C" int sc_plan_create(sc_ctx *ctx, const sc_index *ix, const sc_search_p
     fa.phase = P->stats + 8;
     fa.dot_absmax = ix->dot_absmax;
     fa.margin_scale = mixed_margin_scale();
+    fa.nvec = (int)ix->n;
     P->mixed = p->form == SC_FORM_MIXED;
     for (int v = 3; v <= 5; ++v)
         if (hipFuncSetAttribute((const void *)fused_kernel(v, P->mixed),

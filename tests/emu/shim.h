@@ -40,6 +40,8 @@ inline float4 make_float4(float a, float b, float c, float d) { return {a, b, c,
 inline uint2 make_uint2(uint32_t a, uint32_t b) { return {a, b}; }
 inline float __uint_as_float(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
 inline uint32_t __float_as_uint(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
+inline float __int_as_float(int32_t u) { float f; memcpy(&f, &u, 4); return f; }
+inline int32_t __float_as_int(float f) { int32_t u; memcpy(&u, &f, 4); return u; }
 inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
 inline int __ffsll(long long x) { return __builtin_ffsll(x); }
 inline int __clzll(long long x) { return x == 0 ? 64 : __builtin_clzll((unsigned long long)x); }
@@ -98,6 +100,7 @@ template <class T, class U> inline T atomicAdd(T *p, U v) { return __atomic_fetc
 template <class T, class U> inline T atomicAnd(T *p, U v) { return __atomic_fetch_and(p, (T)v, __ATOMIC_SEQ_CST); }
 template <class T, class U> inline T atomicOr(T *p, U v) { return __atomic_fetch_or(p, (T)v, __ATOMIC_SEQ_CST); }
 template <class T, class U> inline T atomicMax(T *p, U v) { T o = __atomic_load_n(p, __ATOMIC_SEQ_CST); while (o < (T)v && !__atomic_compare_exchange_n(p, &o, (T)v, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST)) {} return o; }
+template <class T, class U> inline T atomicMin(T *p, U v) { T o = __atomic_load_n(p, __ATOMIC_SEQ_CST); while (o > (T)v && !__atomic_compare_exchange_n(p, &o, (T)v, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST)) {} return o; }
 
 #define SC_KIND_IVFADC 0
 #define SC_KIND_IMI 1

@@ -24,12 +24,14 @@ CXX = os.environ.get("EMU_CXX", "/opt/rocm/llvm/bin/clang++")
 pytestmark = pytest.mark.skipif(not shutil.which(CXX) and not os.path.exists(CXX),
                                 reason="no clang++ for the host emulation")
 
-CASES = [  # kind n kc R w T nq threads trained precomputed_lut mixed
+CASES = [  # kind n kc R w T nq threads trained precomputed_lut mixed [margin ncomp]
     ("sc plain, trained codebooks", "2 20000 32 100 300 1500 4 256 1 0 0"),
     ("sc mixed, precomputed LUTs", "2 20000 64 100 300 1500 4 256 0 1 1"),
     ("ivf plain, precomputed LUTs", "0 20000 64 100 12 3000 4 256 0 1 0"),
     ("sc mixed, forced exact redo", "2 20000 32 100 300 1500 3 256 1 0 1 1000000"),
     ("sc plain, 512 threads", "2 20000 32 100 300 1500 3 512 0 0 0"),
+    ("sc plain, traversal past the rank window", "2 20000 64 100 3000 6000 3 256 1 0 0"),
+    ("drop-in data: 16-component mixture, 40 queries", "2 20000 32 100 300 1500 40 256 1 0 0 1 16"),
 ]
 
 
