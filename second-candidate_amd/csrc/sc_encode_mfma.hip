@@ -7,14 +7,18 @@
 // x' = x - m and c' = c - m are shifted by a per-block integer mean m (the
 // distance is shift-invariant; the shift shrinks the norms that scale the
 // error) and every operand is split into two BF16 terms (v = hi + lo,
-// |v - hi - lo| <= 2^-16 |v|).  For a row with X = ||x'|| and C = max_k ||c'_k||
+// |v - hi - lo| <= 2^-16 |v|).  The codeword terms are laid out once in MFMA
+// operand order (k_cb_frag), so a lane loads its B operand with one 16-byte
+// load; a wave keeps 128 codewords' accumulators in registers at a time.  For a row with X = ||x'|| and C = max_k ||c'_k||
 // the error obeys |d~ - d| <= E with
 //     E = 2 * [ 2 X C ((3 * 2^-16 + gamma_n)(1 + 2^-7)) + 2^-22 (X + C)^2 ] + 1e-6,
 //     gamma_n = n u / (1 - n u),  u = 2^-23,  n = 3 * blen + 8
 // (Cauchy-Schwarz bounds sum |x'_i c'_i| by X C; the outer factor 2 covers the
 // FP32 storage of d~, the shift rounding and the FP64 rounding of the exact
 // distance).  The exact top-2 (or top-1) under the reference's order is then
-// contained in {k : d~_k <= d~_(2) + 2E}; those candidates are re-evaluated
+// contained in {k : d~_k <= d~_(2) + 2E}; the kernel collects a superset of
+// it (each group of codewords against the running second smallest, which
+// never falls below the final one), and those candidates are re-evaluated
 // with the exact S1 arithmetic and ordered with the reference's tie rule, so
 // codes and cell ids equal the FP32 form bit for bit.  A row with more than
 // kCand candidates is evaluated exactly over all codewords.
@@ -33,6 +37,7 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kRows = 16;     // vectors per wave
 constexpr int kCand = 16;     // exact re-check slots per row
+constexpr int kGroup = 8;     // codeword tiles (of 16) per register group
 
 __device__ __forceinline__ unsigned short bf16_bits(float v) {
     return __builtin_bit_cast(unsigned short, (__bf16)v);
@@ -54,9 +59,8 @@ __global__ void k_cb_mean(const float *__restrict__ cb, int nb, int K, int blen,
     mean[e] = shift ? (float)rint(s / K) : 0.0f;
 }
 
-__global__ void k_cb_split(const float *__restrict__ cb, int nb, int K, int blen,
-                           const float *__restrict__ mean, unsigned short *__restrict__ hi,
-                           unsigned short *__restrict__ lo, double *__restrict__ cn2,
+__global__ void k_cb_norms(const float *__restrict__ cb, int nb, int K, int blen,
+                           const float *__restrict__ mean, double *__restrict__ cn2,
                            unsigned long long *__restrict__ cmax_bits) {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nb * K) return;
@@ -65,15 +69,36 @@ __global__ void k_cb_split(const float *__restrict__ cb, int nb, int K, int blen
     double s = 0.0;
     for (int t = 0; t < blen; ++t) {
         float v = c[t] - mean[(size_t)b * blen + t];
-        unsigned short h = bf16_bits(v);
-        unsigned short l = bf16_bits(v - bf16_val(h));
-        hi[(size_t)e * blen + t] = h;
-        lo[(size_t)e * blen + t] = l;
         s += (double)v * (double)v;
     }
     cn2[e] = s;
     // max of a non-negative double via its bit pattern
     atomicMax(&cmax_bits[b], (unsigned long long)__builtin_bit_cast(unsigned long long, sqrt(s)));
+}
+
+// Codebook fragments in MFMA operand order, one 16-byte load per lane per
+// MFMA pair: for block b, tile kt of 16 codewords, dims step kk of 16 and
+// lane l, the BF16 hi parts then the BF16 lo parts of c'[k][t + j], j < 4,
+// with k = 16 kt + l % 16 and t = 16 kk + 4 (l / 16) (zero past K).
+// Layout [nb][KT][KK][64] with KT = ceil(K / 16), KK = blen / 16.
+__global__ void k_cb_frag(const float *__restrict__ cb, int nb, int K, int blen,
+                          const float *__restrict__ mean, uint4 *__restrict__ frag) {
+    const int KT = (K + 15) / 16, KK = blen / 16;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)nb * KT * KK * 64) return;
+    const int l = (int)(e % 64);
+    const int64_t f = e / 64;
+    const int kk = (int)(f % KK), kt = (int)((f / KK) % KT), b = (int)(f / ((int64_t)KK * KT));
+    const int k = 16 * kt + (l & 15), t = 16 * kk + 4 * (l >> 4);
+    unsigned short h[4], lo[4];
+    for (int j = 0; j < 4; ++j) {
+        const float v = k < K ? cb[((size_t)b * K + k) * blen + t + j] - mean[(size_t)b * blen + t + j]
+                              : 0.0f;
+        h[j] = bf16_bits(v);
+        lo[j] = bf16_bits(v - bf16_val(h[j]));
+    }
+    frag[e] = make_uint4(h[0] | ((uint32_t)h[1] << 16), h[2] | ((uint32_t)h[3] << 16),
+                         lo[0] | ((uint32_t)lo[1] << 16), lo[2] | ((uint32_t)lo[3] << 16));
 }
 
 __device__ __forceinline__ double s1(const float *a, const float *b, int d) {
@@ -87,7 +112,7 @@ __device__ __forceinline__ double s1(const float *a, const float *b, int d) {
 
 struct CbView {
     const float *c;              // [K][blen] original codewords
-    const unsigned short *hi, *lo;
+    const uint4 *frag;           // [KT][KK][64] BF16 hi/lo fragments (k_cb_frag)
     const double *cn2;           // [K]
     double cmax;
     const float *mean;           // [blen]
@@ -137,55 +162,92 @@ __device__ void wave_best(const float *xs, int ld, int t0, int nrows, const CbVi
         E[i] = 2.0 * (2.0 * X * C * ((3.0 / 65536.0 + gam) * (1.0 + 1.0 / 128.0)) +
                       (X + C) * (X + C) / 4194304.0) + 1e-6;
     }
-    // 2. two passes over the codewords: thresholds, then candidates
+    // 2. approximate distances, codewords in groups of kGroup tiles of 16
+    //    accumulated in registers over the dims.  After each group the row's
+    //    running second (or first) smallest d~ plus 2E is the threshold, and
+    //    the group's codewords at or below it become candidates.  A running
+    //    threshold never falls below the final one, so the candidates
+    //    contain every codeword within 2E of the final second smallest.
+    const int KK = blen / 16, KT = (K + 15) / 16;
+    const int ar = l & 15;                // A row
     float m1[4], m2[4];
     for (int i = 0; i < 4; ++i) { m1[i] = INFINITY; m2[i] = INFINITY; }
-    float thr[4];
-    for (int pass = 0; pass < 2; ++pass) {
-        for (int k0 = 0; k0 < K; k0 += 16) {
-            f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-            const int kb = min(k0 + col, K - 1);     // codeword for this lane's B column
-            for (int kk = 0; kk < blen; kk += 16) {
-                s16x4 ah, al, bh, bl;
-                const int ar = l & 15;                // A row
-                for (int j = 0; j < 4; ++j) {
-                    const int t = kk + r4 + j;
-                    float v = xs[ar * ld + t0 + t] - cb.mean[t];
-                    unsigned short h = bf16_bits(v);
-                    ah[j] = (short)h;
-                    al[j] = (short)bf16_bits(v - bf16_val(h));
-                    bh[j] = (short)cb.hi[(size_t)kb * blen + t];
-                    bl[j] = (short)cb.lo[(size_t)kb * blen + t];
-                }
-                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl, acc, 0, 0, 0);
-                if (!x_exact) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh, acc, 0, 0, 0);
+    for (int kt0 = 0; kt0 < KT; kt0 += kGroup) {
+        const int gn = min(kGroup, KT - kt0);
+        f32x4 acc[kGroup];
+#pragma unroll
+        for (int g = 0; g < kGroup; ++g) acc[g] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        const uint4 *fp = cb.frag + (size_t)kt0 * KK * 64 + l;
+        for (int kk = 0; kk < KK; ++kk) {
+            s16x4 ah, al;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int t = 16 * kk + r4 + j;
+                const float v = xs[ar * ld + t0 + t] - cb.mean[t];
+                const unsigned short h = bf16_bits(v);
+                ah[j] = (short)h;
+                al[j] = (short)bf16_bits(v - bf16_val(h));
             }
-            const int k = k0 + col;
-            for (int i = 0; i < 4; ++i) {
-                if (k >= K) continue;
-                const float d = (float)(rown[r4 + i] + cb.cn2[k] - 2.0 * (double)acc[i]);
-                if (pass == 0) {
-                    if (d < m1[i]) { m2[i] = m1[i]; m1[i] = d; }
-                    else if (d < m2[i]) m2[i] = d;
-                } else if ((double)d <= (double)thr[i] && r4 + i < nrows) {
-                    int s = atomicAdd(&cnt[r4 + i], 1);
-                    if (s < kCand) ck[(r4 + i) * kCand + s] = k;
+#pragma unroll
+            for (int g = 0; g < kGroup; ++g) {
+                if (g < gn) {
+                    const uint4 f = fp[((size_t)g * KK + kk) * 64];
+                    const s16x4 bh = {(short)(f.x & 0xFFFFu), (short)(f.x >> 16),
+                                      (short)(f.y & 0xFFFFu), (short)(f.y >> 16)};
+                    const s16x4 bl = {(short)(f.z & 0xFFFFu), (short)(f.z >> 16),
+                                      (short)(f.w & 0xFFFFu), (short)(f.w >> 16)};
+                    acc[g] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, acc[g], 0, 0, 0);
+                    acc[g] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl, acc[g], 0, 0, 0);
+                    if (!x_exact)
+                        acc[g] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh, acc[g], 0, 0, 0);
                 }
             }
         }
-        if (pass == 0) {
+        // d~ in place (+inf past K), the two smallest of this lane's group
+        float g1[4], g2[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) { g1[i] = INFINITY; g2[i] = INFINITY; }
+#pragma unroll
+        for (int g = 0; g < kGroup; ++g) {
+            const int k = 16 * (kt0 + g) + col;
+            const double c2 = k < K ? cb.cn2[k] : 0.0;
+#pragma unroll
             for (int i = 0; i < 4; ++i) {
-                for (int o = 1; o < 16; o <<= 1) {
-                    float a1 = __shfl_xor(m1[i], o), a2 = __shfl_xor(m2[i], o);
-                    float lo1 = fminf(m1[i], a1);
-                    float hi1 = fmaxf(m1[i], a1);
-                    m2[i] = fminf(hi1, fminf(m2[i], a2));
-                    m1[i] = lo1;
+                const float d =
+                    k < K ? (float)(rown[r4 + i] + c2 - 2.0 * (double)acc[g][i]) : INFINITY;
+                acc[g][i] = d;
+                if (d < g1[i]) { g2[i] = g1[i]; g1[i] = d; }
+                else if (d < g2[i]) g2[i] = d;
+            }
+        }
+        // the group's two smallest over the row's 16 lanes (disjoint sets),
+        // merged into the row's running pair, then the threshold
+        float thr[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            for (int o = 1; o < 16; o <<= 1) {
+                float a1 = __shfl_xor(g1[i], o), a2 = __shfl_xor(g2[i], o);
+                float lo1 = fminf(g1[i], a1);
+                float hi1 = fmaxf(g1[i], a1);
+                g2[i] = fminf(hi1, fminf(g2[i], a2));
+                g1[i] = lo1;
+            }
+            const float lo1 = fminf(m1[i], g1[i]), hi1 = fmaxf(m1[i], g1[i]);
+            m2[i] = fminf(hi1, fminf(m2[i], g2[i]));
+            m1[i] = lo1;
+            thr[i] = (float)((double)(TOP2 ? m2[i] : m1[i]) + 2.0 * E[i]);
+            // rounding the threshold up keeps it an upper bound
+            thr[i] = nextafterf(thr[i], INFINITY);
+        }
+#pragma unroll
+        for (int g = 0; g < kGroup; ++g) {
+            const int k = 16 * (kt0 + g) + col;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                if (k < K && (double)acc[g][i] <= (double)thr[i] && r4 + i < nrows) {
+                    int s = atomicAdd(&cnt[r4 + i], 1);
+                    if (s < kCand) ck[(r4 + i) * kCand + s] = k;
                 }
-                thr[i] = (float)((double)(TOP2 ? m2[i] : m1[i]) + 2.0 * E[i]);
-                // rounding the threshold up keeps it an upper bound
-                thr[i] = nextafterf(thr[i], INFINITY);
             }
         }
     }
@@ -220,13 +282,15 @@ __device__ void wave_best(const float *xs, int ld, int t0, int nrows, const CbVi
     __syncthreads();
 }
 
+// One wave per 16 vectors; 3 waves per SIMD keep the register tiles
+// without spills.
 template <int NC>
-__global__ __launch_bounds__(64) void k_encode_mfma(
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_encode_mfma(
     const float *__restrict__ x, int64_t n, int dim, const float *__restrict__ cq, int mc,
-    int kc, const float *__restrict__ pq, int mp, int kp, const unsigned short *__restrict__ chi,
-    const unsigned short *__restrict__ clo, const double *__restrict__ cn2,
+    int kc, const float *__restrict__ pq, int mp, int kp, const uint4 *__restrict__ cfrag,
+    const double *__restrict__ cn2,
     const unsigned long long *__restrict__ cmax, const float *__restrict__ cmean,
-    const unsigned short *__restrict__ phi, const unsigned short *__restrict__ plo,
+    const uint4 *__restrict__ pfrag,
     const double *__restrict__ pn2, const unsigned long long *__restrict__ pmax,
     const float *__restrict__ pmean, uint16_t *__restrict__ cid, uint8_t *__restrict__ codes,
     unsigned long long *__restrict__ stats) {
@@ -249,8 +313,8 @@ __global__ __launch_bounds__(64) void k_encode_mfma(
     __syncthreads();
     const int bsc = dim / mc, bsp = dim / mp;
     for (int j = 0; j < mc; ++j) {
-        CbView cv{cq + (size_t)j * kc * bsc, chi + (size_t)j * kc * bsc,
-                  clo + (size_t)j * kc * bsc, cn2 + (size_t)j * kc,
+        CbView cv{cq + (size_t)j * kc * bsc, cfrag + (size_t)j * ((kc + 15) / 16) * (bsc / 16) * 64,
+                  cn2 + (size_t)j * kc,
                   __builtin_bit_cast(double, cmax[j]), cmean + (size_t)j * bsc, kc, bsc};
         wave_best<NC == 2, NC == 2>(xs, ld, j * bsc, nr, cv, cnt, ck, cd, rown, o1,
                                     NC == 2 ? o2 : nullptr, &stats[0], &stats[1]);
@@ -270,8 +334,8 @@ __global__ __launch_bounds__(64) void k_encode_mfma(
         __syncthreads();
     }
     for (int s = 0; s < mp; ++s) {
-        CbView pv{pq + (size_t)s * kp * bsp, phi + (size_t)s * kp * bsp,
-                  plo + (size_t)s * kp * bsp, pn2 + (size_t)s * kp,
+        CbView pv{pq + (size_t)s * kp * bsp, pfrag + (size_t)s * ((kp + 15) / 16) * (bsp / 16) * 64,
+                  pn2 + (size_t)s * kp,
                   __builtin_bit_cast(double, pmax[s]), pmean + (size_t)s * bsp, kp, bsp};
         wave_best<false, false>(xs, ld, s * bsp, nr, pv, cnt, ck, cd, rown, o1, nullptr,
                                 &stats[0], &stats[1]);
@@ -281,20 +345,22 @@ __global__ __launch_bounds__(64) void k_encode_mfma(
 }
 
 int prep_codebook(sc_ctx *ctx, const float *cb, int nb, int K, int blen, int shift,
-                  unsigned char *base, size_t &off, float **mean, unsigned short **hi,
-                  unsigned short **lo, double **cn2, unsigned long long **cmax) {
+                  unsigned char *base, size_t &off, float **mean, uint4 **frag, double **cn2,
+                  unsigned long long **cmax) {
     auto take = [&](size_t bytes) { unsigned char *p = base + off; off = (off + bytes + 255) & ~(size_t)255; return p; };
+    const int64_t nfrag = (int64_t)nb * ((K + 15) / 16) * (blen / 16) * 64;
     *mean = (float *)take(sizeof(float) * nb * blen);
-    *hi = (unsigned short *)take(2ull * nb * K * blen);
-    *lo = (unsigned short *)take(2ull * nb * K * blen);
+    *frag = (uint4 *)take(sizeof(uint4) * (size_t)nfrag);
     *cn2 = (double *)take(8ull * nb * K);
     *cmax = (unsigned long long *)take(8ull * nb);
     if (!base) return SC_OK;   // sizing pass
     SC_HIP(hipMemsetAsync(*cmax, 0, 8ull * nb, ctx->stream));
     hipLaunchKernelGGL(k_cb_mean, dim3((unsigned)sc_cdiv((int64_t)nb * blen, 256)), dim3(256), 0,
                        ctx->stream, cb, nb, K, blen, shift, *mean);
-    hipLaunchKernelGGL(k_cb_split, dim3((unsigned)sc_cdiv((int64_t)nb * K, 256)), dim3(256), 0,
-                       ctx->stream, cb, nb, K, blen, *mean, *hi, *lo, *cn2, *cmax);
+    hipLaunchKernelGGL(k_cb_norms, dim3((unsigned)sc_cdiv((int64_t)nb * K, 256)), dim3(256), 0,
+                       ctx->stream, cb, nb, K, blen, *mean, *cn2, *cmax);
+    hipLaunchKernelGGL(k_cb_frag, dim3((unsigned)sc_cdiv(nfrag, 256)), dim3(256), 0, ctx->stream,
+                       cb, nb, K, blen, *mean, *frag);
     SC_CHECK_LAUNCH();
     return SC_OK;
 }
@@ -311,18 +377,18 @@ int sc_encode_bf16(sc_ctx *ctx, const float *x, int64_t n, int dim, const float 
         return SC_ERR_ARG;
     }
     float *cmean, *pmean;
-    unsigned short *chi, *clo, *phi, *plo;
+    uint4 *cfrag, *pfrag;
     double *cn2, *pn2;
     unsigned long long *cmax, *pmax, *stats;
     size_t off = 0;
-    prep_codebook(ctx, cq, mc, kc, bsc, 1, nullptr, off, &cmean, &chi, &clo, &cn2, &cmax);
-    prep_codebook(ctx, pq, mp, kp, bsp, 0, nullptr, off, &pmean, &phi, &plo, &pn2, &pmax);
+    prep_codebook(ctx, cq, mc, kc, bsc, 1, nullptr, off, &cmean, &cfrag, &cn2, &cmax);
+    prep_codebook(ctx, pq, mp, kp, bsp, 0, nullptr, off, &pmean, &pfrag, &pn2, &pmax);
     const size_t total = off + 256;
     unsigned char *s;
     SC_TRY(sc_scratch(ctx, total, (void **)&s));
     off = 0;
-    SC_TRY(prep_codebook(ctx, cq, mc, kc, bsc, 1, s, off, &cmean, &chi, &clo, &cn2, &cmax));
-    SC_TRY(prep_codebook(ctx, pq, mp, kp, bsp, 0, s, off, &pmean, &phi, &plo, &pn2, &pmax));
+    SC_TRY(prep_codebook(ctx, cq, mc, kc, bsc, 1, s, off, &cmean, &cfrag, &cn2, &cmax));
+    SC_TRY(prep_codebook(ctx, pq, mp, kp, bsp, 0, s, off, &pmean, &pfrag, &pn2, &pmax));
     stats = (unsigned long long *)(s + off);
     SC_HIP(hipMemsetAsync(stats, 0, 16, ctx->stream));
     const int ld = dim + 1;
@@ -334,13 +400,13 @@ int sc_encode_bf16(sc_ctx *ctx, const float *x, int64_t n, int dim, const float 
         SC_HIP(hipFuncSetAttribute((const void *)k_encode_mfma<2>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         hipLaunchKernelGGL(k_encode_mfma<2>, dim3(blocks), dim3(64), lds, ctx->stream, x, n, dim,
-                           cq, mc, kc, pq, mp, kp, chi, clo, cn2, cmax, cmean, phi, plo, pn2,
+                           cq, mc, kc, pq, mp, kp, cfrag, cn2, cmax, cmean, pfrag, pn2,
                            pmax, pmean, cid, codes, stats);
     } else {
         SC_HIP(hipFuncSetAttribute((const void *)k_encode_mfma<1>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         hipLaunchKernelGGL(k_encode_mfma<1>, dim3(blocks), dim3(64), lds, ctx->stream, x, n, dim,
-                           cq, mc, kc, pq, mp, kp, chi, clo, cn2, cmax, cmean, phi, plo, pn2,
+                           cq, mc, kc, pq, mp, kp, cfrag, cn2, cmax, cmean, pfrag, pn2,
                            pmax, pmean, cid, codes, stats);
     }
     SC_CHECK_LAUNCH();

@@ -3,7 +3,8 @@
 The device code of second-candidate_amd/csrc/sc_search.hip (its anonymous
 namespace up to the host-side plan code) is compiled as C++ against shim.h and
 linked with the oracle library and driver.inc.  Usage:
-    python make_emu.py OUT_BINARY SANITIZER   (SANITIZER: address | thread)
+    python make_emu.py OUT_BINARY SANITIZER [encode]   (SANITIZER: address | thread)
+With `encode`, the MFMA encode kernel of sc_encode_mfma.hip instead.
 """
 import os
 import re
@@ -33,10 +34,27 @@ def generate(path):
     open(path, "w").write("\n".join(out))
 
 
-def build(out, sanitizer, cxx=None):
+ENC_SRC = os.path.join(ROOT, "second-candidate_amd", "csrc", "sc_encode_mfma.hip")
+
+
+def generate_encode(path):
+    """The device code of sc_encode_mfma.hip (its anonymous namespace up to
+    the host-side codebook preparation) with the MFMA builtin emulated."""
+    src = open(ENC_SRC).read()
+    a, b = src.index("namespace {"), src.index("int prep_codebook(")
+    body = src[a:b]
+    body = body.replace("extern __shared__ __align__(16) unsigned char smem[];",
+                        "unsigned char *smem = g_smem;")
+    body = body.replace("__builtin_amdgcn_mfma_f32_16x16x16bf16_1k", "emu_mfma_bf16")
+    out = ['#include "shim.h"', "#include <vector>", "extern unsigned char *g_smem;", body,
+           "} // namespace", open(os.path.join(HERE, "driver_encode.inc")).read()]
+    open(path, "w").write("\n".join(out))
+
+
+def build(out, sanitizer, cxx=None, encode=False):
     cxx = cxx or os.environ.get("EMU_CXX", "/opt/rocm/llvm/bin/clang++")
     cpp = out + ".cpp"
-    generate(cpp)
+    (generate_encode if encode else generate)(cpp)
     cmd = [cxx, "-std=c++17", "-O1", "-g", "-fsanitize=" + sanitizer, "-fno-omit-frame-pointer",
            "-I" + HERE, "-I" + ORACLE, cpp, "-o", out, "-L" + ORACLE, "-loracle",
            "-Wl,-rpath," + ORACLE, "-lpthread"]
@@ -45,4 +63,4 @@ def build(out, sanitizer, cxx=None):
 
 
 if __name__ == "__main__":
-    build(sys.argv[1], sys.argv[2])
+    build(sys.argv[1], sys.argv[2], encode=len(sys.argv) > 3 and sys.argv[3] == "encode")

@@ -1,5 +1,5 @@
-// shim.h -- host emulation of the HIP device features the fused search kernel
-// uses, so that its device code compiles as ordinary C++ and runs one
+// shim.h -- host emulation of the HIP device features the fused search and
+// MFMA encode kernels use, so that its device code compiles as ordinary C++ and runs one
 // workgroup as std::threads under AddressSanitizer or ThreadSanitizer:
 //   * __syncthreads is a pthread barrier over the workgroup's threads;
 //   * wave intrinsics (ballot, shuffles, readfirstlane) exchange values through
@@ -101,6 +101,32 @@ template <class T, class U> inline T atomicAnd(T *p, U v) { return __atomic_fetc
 template <class T, class U> inline T atomicOr(T *p, U v) { return __atomic_fetch_or(p, (T)v, __ATOMIC_SEQ_CST); }
 template <class T, class U> inline T atomicMax(T *p, U v) { T o = __atomic_load_n(p, __ATOMIC_SEQ_CST); while (o < (T)v && !__atomic_compare_exchange_n(p, &o, (T)v, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST)) {} return o; }
 template <class T, class U> inline T atomicMin(T *p, U v) { T o = __atomic_load_n(p, __ATOMIC_SEQ_CST); while (o > (T)v && !__atomic_compare_exchange_n(p, &o, (T)v, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST)) {} return o; }
+
+inline int __all(int pred) { return __ballot(pred) == __ballot(1); }
+inline uint4 make_uint4(uint32_t a, uint32_t b, uint32_t c, uint32_t d) { return {a, b, c, d}; }
+inline double __fma_rn(double a, double b, double c) { return std::fma(a, b, c); }
+
+// v_mfma_f32_16x16x16bf16_1k of one wave: lane l holds A[l % 16][4 (l / 16) + j]
+// and B[4 (l / 16) + j][l % 16] for j < 4 and receives C[4 (l / 16) + i][l % 16];
+// products of BF16 values are exact in FP32, the sums are FP32 in k order.
+extern short g_mfma_a[16][64][4], g_mfma_b[16][64][4];
+inline float emu_bf16_float(short b) { return __uint_as_float((uint32_t)(uint16_t)b << 16); }
+template <class A, class C> inline C emu_mfma_bf16(A a, A b, C c, int, int, int) {
+    const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+    for (int j = 0; j < 4; ++j) { g_mfma_a[w][l][j] = a[j]; g_mfma_b[w][l][j] = b[j]; }
+    g_wave_bar[w].wait("mfma");
+    C r = c;
+    for (int i = 0; i < 4; ++i) {
+        const int row = 4 * (l / 16) + i, col = l % 16;
+        float s = r[i];
+        for (int k = 0; k < 16; ++k)
+            s += emu_bf16_float(g_mfma_a[w][row + 16 * (k / 4)][k % 4]) *
+                 emu_bf16_float(g_mfma_b[w][col + 16 * (k / 4)][k % 4]);
+        r[i] = s;
+    }
+    g_wave_bar[w].wait("mfma2");
+    return r;
+}
 
 #define SC_KIND_IVFADC 0
 #define SC_KIND_IMI 1
