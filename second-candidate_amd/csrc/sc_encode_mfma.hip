@@ -9,13 +9,17 @@
 // error) and every operand is split into two BF16 terms (v = hi + lo,
 // |v - hi - lo| <= 2^-16 |v|).  The codeword terms are laid out once in MFMA
 // operand order (k_cb_frag), so a lane loads its B operand with one 16-byte
-// load; a wave keeps 128 codewords' accumulators in registers at a time.  For a row with X = ||x'|| and C = max_k ||c'_k||
-// the error obeys |d~ - d| <= E with
-//     E = 2 * [ 2 X C ((3 * 2^-16 + gamma_n)(1 + 2^-7)) + 2^-22 (X + C)^2 ] + 1e-6,
+// load; a wave keeps 128 codewords' accumulators in registers at a time.
+// d~ is combined in FP32: (float(||x'||^2) + float(||c'||^2)) - 2 x'.c'.
+// For a row with X = ||x'|| and C = max_k ||c'_k|| the error obeys
+// |d~ - d| <= E with
+//     E = 2 * [ 2 X C ((3 * 2^-16 + gamma_n)(1 + 2^-7)) + 2^-22 (X + C)^2 ]
+//         + 2^-22 (X + C)^2 + 1e-6,
 //     gamma_n = n u / (1 - n u),  u = 2^-23,  n = 3 * blen + 8
 // (Cauchy-Schwarz bounds sum |x'_i c'_i| by X C; the outer factor 2 covers the
-// FP32 storage of d~, the shift rounding and the FP64 rounding of the exact
-// distance).  The exact top-2 (or top-1) under the reference's order is then
+// shift rounding and the FP64 rounding of the exact distance; the last
+// 2^-22 (X + C)^2 covers the three FP32 roundings of the combination, at most
+// 2^-24 (2 X^2 + 2 C^2 + (X + C)^2)).  The exact top-2 (or top-1) under the reference's order is then
 // contained in {k : d~_k <= d~_(2) + 2E}; the kernel collects a superset of
 // it (each group of codewords against the running second smallest, which
 // never falls below the final one), and those candidates are re-evaluated
@@ -101,9 +105,23 @@ __global__ void k_cb_frag(const float *__restrict__ cb, int nb, int K, int blen,
                          lo[0] | ((uint32_t)lo[1] << 16), lo[2] | ((uint32_t)lo[3] << 16));
 }
 
+// S1 in FP64: exact squares, FMA-accumulated in t order.  b (a codeword in
+// global memory) is read 16 values at a time so its loads overlap; the
+// accumulation order stays sequential.
 __device__ __forceinline__ double s1(const float *a, const float *b, int d) {
     double acc = 0.0;
-    for (int t = 0; t < d; ++t) {
+    int t = 0;
+    for (; t + 16 <= d; t += 16) {
+        float bv[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) bv[j] = b[t + j];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const double dd = (double)(a[t + j] - bv[j]);
+            acc = __fma_rn(dd, dd, acc);
+        }
+    }
+    for (; t < d; ++t) {
         const double dd = (double)(a[t] - b[t]);
         acc = __fma_rn(dd, dd, acc);
     }
@@ -130,19 +148,24 @@ __device__ __forceinline__ bool precedes(double d, int k, double e, int j) {
 // One wave: best one (TOP2=false) or two (TOP2=true) codewords for the 16
 // rows held in xs (row stride `ld`, columns [t0, t0 + blen)).  Writes
 // out1/out2 [16].  Wave-private LDS scratch: cnt [16], ck [16][kCand],
-// cd [16][kCand], rown [16] (FP64 norms of shifted rows).
+// cd [16][kCand], rown [16] (FP64 norms of shifted rows), sm [blen] (the
+// block's shift), wst [2] (the workgroup's re-checked candidates and fully
+// exact rows).
 template <bool TOP2, bool DESC>
 __device__ void wave_best(const float *xs, int ld, int t0, int nrows, const CbView cb,
-                          int *cnt, int *ck, double *cd, double *rown, int *out1, int *out2,
-                          unsigned long long *st_rechecked, unsigned long long *st_fallback) {
+                          int *cnt, int *ck, double *cd, double *rown, float *sm, int *out1,
+                          int *out2, unsigned *wst) {
     const int l = threadIdx.x & 63;
     const int blen = cb.blen, K = cb.K;
+    for (int t = l; t < blen; t += 64) sm[t] = cb.mean[t];
+    __syncthreads();
     // 1. shifted row norms (FP64) and exactness of the BF16 representation
     bool exact_local = true;
     if (l < kRows) {
         double s = 0.0;
+#pragma unroll 8
         for (int t = 0; t < blen; ++t) {
-            float v = xs[l * ld + t0 + t] - cb.mean[t];
+            float v = xs[l * ld + t0 + t] - sm[t];
             s += (double)v * (double)v;
             exact_local &= (bf16_val(bf16_bits(v)) == v);
         }
@@ -160,7 +183,8 @@ __device__ void wave_best(const float *xs, int ld, int t0, int nrows, const CbVi
     for (int i = 0; i < 4; ++i) {
         const double X = sqrt(rown[r4 + i]), C = cb.cmax;
         E[i] = 2.0 * (2.0 * X * C * ((3.0 / 65536.0 + gam) * (1.0 + 1.0 / 128.0)) +
-                      (X + C) * (X + C) / 4194304.0) + 1e-6;
+                      (X + C) * (X + C) / 4194304.0) +
+               (X + C) * (X + C) / 4194304.0 + 1e-6;
     }
     // 2. approximate distances, codewords in groups of kGroup tiles of 16
     //    accumulated in registers over the dims.  After each group the row's
@@ -170,8 +194,8 @@ __device__ void wave_best(const float *xs, int ld, int t0, int nrows, const CbVi
     //    contain every codeword within 2E of the final second smallest.
     const int KK = blen / 16, KT = (K + 15) / 16;
     const int ar = l & 15;                // A row
-    float m1[4], m2[4];
-    for (int i = 0; i < 4; ++i) { m1[i] = INFINITY; m2[i] = INFINITY; }
+    float m1[4], m2[4], rf[4];
+    for (int i = 0; i < 4; ++i) { m1[i] = INFINITY; m2[i] = INFINITY; rf[i] = (float)rown[r4 + i]; }
     for (int kt0 = 0; kt0 < KT; kt0 += kGroup) {
         const int gn = min(kGroup, KT - kt0);
         f32x4 acc[kGroup];
@@ -183,7 +207,7 @@ __device__ void wave_best(const float *xs, int ld, int t0, int nrows, const CbVi
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const int t = 16 * kk + r4 + j;
-                const float v = xs[ar * ld + t0 + t] - cb.mean[t];
+                const float v = xs[ar * ld + t0 + t] - sm[t];
                 const unsigned short h = bf16_bits(v);
                 ah[j] = (short)h;
                 al[j] = (short)bf16_bits(v - bf16_val(h));
@@ -210,11 +234,10 @@ __device__ void wave_best(const float *xs, int ld, int t0, int nrows, const CbVi
 #pragma unroll
         for (int g = 0; g < kGroup; ++g) {
             const int k = 16 * (kt0 + g) + col;
-            const double c2 = k < K ? cb.cn2[k] : 0.0;
+            const float c2 = k < K ? (float)cb.cn2[k] : 0.0f;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const float d =
-                    k < K ? (float)(rown[r4 + i] + c2 - 2.0 * (double)acc[g][i]) : INFINITY;
+                const float d = k < K ? (rf[i] + c2) - 2.0f * acc[g][i] : INFINITY;
                 acc[g][i] = d;
                 if (d < g1[i]) { g2[i] = g1[i]; g1[i] = d; }
                 else if (d < g2[i]) g2[i] = d;
@@ -244,7 +267,7 @@ __device__ void wave_best(const float *xs, int ld, int t0, int nrows, const CbVi
             const int k = 16 * (kt0 + g) + col;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                if (k < K && (double)acc[g][i] <= (double)thr[i] && r4 + i < nrows) {
+                if (k < K && acc[g][i] <= thr[i] && r4 + i < nrows) {
                     int s = atomicAdd(&cnt[r4 + i], 1);
                     if (s < kCand) ck[(r4 + i) * kCand + s] = k;
                 }
@@ -252,13 +275,20 @@ __device__ void wave_best(const float *xs, int ld, int t0, int nrows, const CbVi
         }
     }
     __syncthreads();
-    // 3. exact re-check of the candidates
-    for (int slot = l; slot < kRows * kCand; slot += 64) {
-        const int r = slot / kCand, s = slot % kCand;
-        if (r < nrows && s < min(cnt[r], kCand)) {
-            const int k = ck[r * kCand + s];
-            cd[slot] = s1(xs + r * ld + t0, cb.c + (size_t)k * blen, blen);
+    // 3. exact re-check of the candidates, numbered densely over the rows
+    //    (rows with more than kCand are evaluated in full below)
+    int total = 0;
+    for (int r = 0; r < nrows; ++r) total += cnt[r] <= kCand ? cnt[r] : 0;
+    for (int p = l; p < total; p += 64) {
+        int r = 0, s = p;
+        while (true) {
+            const int c = cnt[r] <= kCand ? cnt[r] : 0;
+            if (s < c) break;
+            s -= c;
+            ++r;
         }
+        const int k = ck[r * kCand + s];
+        cd[r * kCand + s] = s1(xs + r * ld + t0, cb.c + (size_t)k * blen, blen);
     }
     __syncthreads();
     if (l < nrows) {
@@ -271,15 +301,24 @@ __device__ void wave_best(const float *xs, int ld, int t0, int nrows, const CbVi
         };
         if (c <= kCand) {
             for (int s = 0; s < c; ++s) push(cd[l * kCand + s], ck[l * kCand + s]);
-            atomicAdd(st_rechecked, (unsigned long long)c);
+            atomicAdd(&wst[0], (unsigned)c);
         } else {
             for (int k = 0; k < K; ++k) push(s1(xs + l * ld + t0, cb.c + (size_t)k * blen, blen), k);
-            atomicAdd(st_fallback, 1ull);
+            atomicAdd(&wst[1], 1u);
         }
         out1[l] = k1;
         if (out2) out2[l] = k2;
     }
     __syncthreads();
+}
+
+// LDS of one workgroup: rows [16][dim + 1] (float), re-check distances
+// [16][kCand] and row norms [16] (double), counts [16], candidates
+// [16][kCand], two result rows [16] (int), the block shift [dim] (float).
+__host__ __device__ inline size_t encode_lds(int dim) {
+    const int ld = dim + 1;
+    return sizeof(float) * (kRows * ld + (kRows * ld) % 2) + sizeof(double) * (kRows * kCand + kRows) +
+           sizeof(int) * (kRows + kRows * kCand + 2 * kRows) + sizeof(float) * dim;
 }
 
 // One wave per 16 vectors; 3 waves per SIMD keep the register tiles
@@ -303,21 +342,29 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8))) void
     int *ck = cnt + kRows;
     int *o1 = ck + kRows * kCand;
     int *o2 = o1 + kRows;
+    float *sm = reinterpret_cast<float *>(o2 + kRows);                   // [dim]
     const int l = threadIdx.x;
     const int64_t v0 = (int64_t)blockIdx.x * kRows;
     const int nr = (int)min((int64_t)kRows, n - v0);
-    for (int e = l; e < kRows * dim; e += 64) {
-        int r = e / dim, t = e % dim;
-        xs[r * ld + t] = r < nr ? x[(size_t)(v0 + r) * dim + t] : 0.0f;
+    const int d4 = dim / 4;                  // dim is a multiple of 16 here
+#pragma unroll 4
+    for (int e = l; e < kRows * d4; e += 64) {
+        const int r = e / d4, c = e - r * d4;
+        const float4 v = r < nr ? reinterpret_cast<const float4 *>(x + (size_t)(v0 + r) * dim)[c]
+                                : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        float *dst = xs + r * ld + 4 * c;
+        dst[0] = v.x; dst[1] = v.y; dst[2] = v.z; dst[3] = v.w;
     }
+    __shared__ unsigned s_wst[2];   // re-checked candidates, fully exact rows
+    if (l == 0) { s_wst[0] = 0u; s_wst[1] = 0u; }
     __syncthreads();
     const int bsc = dim / mc, bsp = dim / mp;
     for (int j = 0; j < mc; ++j) {
         CbView cv{cq + (size_t)j * kc * bsc, cfrag + (size_t)j * ((kc + 15) / 16) * (bsc / 16) * 64,
                   cn2 + (size_t)j * kc,
                   __builtin_bit_cast(double, cmax[j]), cmean + (size_t)j * bsc, kc, bsc};
-        wave_best<NC == 2, NC == 2>(xs, ld, j * bsc, nr, cv, cnt, ck, cd, rown, o1,
-                                    NC == 2 ? o2 : nullptr, &stats[0], &stats[1]);
+        wave_best<NC == 2, NC == 2>(xs, ld, j * bsc, nr, cv, cnt, ck, cd, rown, sm, o1,
+                                    NC == 2 ? o2 : nullptr, s_wst);
         if (l < nr) {
             if (NC == 2) {
                 cid[(size_t)(v0 + l) * 2 * mc + j] = (uint16_t)o1[l];
@@ -327,6 +374,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8))) void
             }
         }
         __syncthreads();
+#pragma unroll 8
         for (int e = l; e < nr * bsc; e += 64) {   // o1 is written for real rows only
             int r = e / bsc, t = e % bsc;
             xs[r * ld + j * bsc + t] -= cv.c[(size_t)max(o1[r], 0) * bsc + t];
@@ -337,10 +385,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8))) void
         CbView pv{pq + (size_t)s * kp * bsp, pfrag + (size_t)s * ((kp + 15) / 16) * (bsp / 16) * 64,
                   pn2 + (size_t)s * kp,
                   __builtin_bit_cast(double, pmax[s]), pmean + (size_t)s * bsp, kp, bsp};
-        wave_best<false, false>(xs, ld, s * bsp, nr, pv, cnt, ck, cd, rown, o1, nullptr,
-                                &stats[0], &stats[1]);
+        wave_best<false, false>(xs, ld, s * bsp, nr, pv, cnt, ck, cd, rown, sm, o1, nullptr,
+                                s_wst);
         if (l < nr) codes[(size_t)(v0 + l) * mp + s] = (uint8_t)o1[l];
         __syncthreads();
+    }
+    if (l == 0) {   // one update of the global statistics per workgroup
+        if (s_wst[0]) atomicAdd(&stats[0], (unsigned long long)s_wst[0]);
+        if (s_wst[1]) atomicAdd(&stats[1], (unsigned long long)s_wst[1]);
     }
 }
 
@@ -391,10 +443,7 @@ int sc_encode_bf16(sc_ctx *ctx, const float *x, int64_t n, int dim, const float 
     SC_TRY(prep_codebook(ctx, pq, mp, kp, bsp, 0, s, off, &pmean, &pfrag, &pn2, &pmax));
     stats = (unsigned long long *)(s + off);
     SC_HIP(hipMemsetAsync(stats, 0, 16, ctx->stream));
-    const int ld = dim + 1;
-    const size_t lds = sizeof(float) * (kRows * ld + (kRows * ld) % 2) +
-                       sizeof(double) * (kRows * kCand + kRows) +
-                       sizeof(int) * (kRows + kRows * kCand + 2 * kRows);
+    const size_t lds = encode_lds(dim);
     const unsigned blocks = (unsigned)sc_cdiv(n, kRows);
     if (nc == 2) {
         SC_HIP(hipFuncSetAttribute((const void *)k_encode_mfma<2>,
