@@ -100,15 +100,19 @@ def main():
 
     x = S.gen_rows(ctx, mix, S.STREAM_BASE, 0, args.n, dim)
     ctx.sync()
-    t0 = time.perf_counter()
-    cid, codes, _ = S.encode(ctx, x, cq, dpq, 1, kc, mp, kp, 2, S.PREC_FP32)
-    ctx.sync()
-    t_encode = time.perf_counter() - t0
+    def timed_encode(prec, reps=3):
+        """Encodes all base vectors `reps` times; the fastest run's time."""
+        best, res = None, None
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            res = S.encode(ctx, x, cq, dpq, 1, kc, mp, kp, 2, prec)
+            ctx.sync()
+            dt = time.perf_counter() - t0
+            best = dt if best is None else min(best, dt)
+        return best, res
+    t_encode, (cid, codes, _) = timed_encode(S.PREC_FP32)
     # the mixed-precision assignment (BF16 MFMA distances + exact re-check)
-    t0 = time.perf_counter()
-    cid16, codes16, est16 = S.encode(ctx, x, cq, dpq, 1, kc, mp, kp, 2, S.PREC_BF16)
-    ctx.sync()
-    t_encode16 = time.perf_counter() - t0
+    t_encode16, (cid16, codes16, est16) = timed_encode(S.PREC_BF16)
     enc16_same = bool(np.array_equal(cid16.numpy(), cid.numpy()) and
                       np.array_equal(codes16.numpy(), codes.numpy()))
     log("encode: fp32 exact %.4fs, bf16+re-check %.4fs (identical: %s, %.1f re-checked/row)"
@@ -292,6 +296,12 @@ def main():
             traffic = int(t["traffic_bytes"])
 
     encode_flops = 2.0 * args.n * (kc * dim + mp * kp * (dim // mp))
+    # MFMAs the BF16 encode issues: per 16 vectors, (K/16 codeword tiles) x
+    # (block length / 16) 16x16x16 MFMAs, two per tile step for the coarse
+    # block (integer rows are exact in BF16), three for the PQ blocks (the
+    # residual rows are split into two BF16 terms); 8192 flops each
+    mfma_issued = (args.n / 16.0) * ((kc // 16) * (dim // 16) * 2 +
+                                     mp * (kp // 16) * max(1, (dim // mp) // 16) * 3) * 8192.0
     out = {
         "metric": "QPS at fixed recall@100, mixed-prec batched vs plain FP32 GPU, 1M×128",
         "value": round(qps, 1),
@@ -326,6 +336,13 @@ def main():
             "encode_bf16_s": round(t_encode16, 4),
             "encode_bf16_identical": enc16_same,
             "encode_fp64_tflops": round(encode_flops / t_encode / 1e12, 2),
+            "encode_bf16_mfma": {
+                "algorithmic_tflops": round(encode_flops / t_encode16 / 1e12, 2),
+                "issued_tflops": round(mfma_issued / t_encode16 / 1e12, 2),
+                "peak_tflops": 2500.0,
+                "frac_issued": round(mfma_issued / t_encode16 / 2.5e15, 4),
+                "rechecked_per_vector": round(est16.rechecked / args.n, 2),
+                "full_exact_rows": int(est16.fallback)},
             "train_s": round(t_train, 2),
         },
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,

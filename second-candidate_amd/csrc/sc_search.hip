@@ -763,7 +763,6 @@ struct FusedArgs {
     unsigned long long *phase;        // [6]: wall-clock ticks per phase, workgroups
     float dot_absmax;                 // max |dot_cr| (mixed form error bound)
     float margin_scale;               // mixed form: multiplies the error bound (1)
-    int nvec;                         // vectors in the index (code loads stay below it)
 };
 
 struct FusedLayout {
@@ -1069,10 +1068,9 @@ struct GroupCells {
 template <int MPW, bool MIXED>
 __device__ __forceinline__ void scan_group(const ScanArgs &a, TopShared &T, uint64_t *key, int32_t *idx,
                            int bufcap, const float *lut, float d0, int p0, int p1, int i0,
-                           const GroupCells &G, float S, float twoE, int nvec) {
-    constexpr int W = MPW > 0 ? MPW : 1;
+                           const GroupCells &G, float S, float twoE) {
     const _Float16 *lut16 = reinterpret_cast<const _Float16 *>(lut);
-    const int tid = threadIdx.x, NT = blockDim.x, U = kRound / NT, lane = tid & 63;
+    const int tid = threadIdx.x, NT = blockDim.x, U = kRound / NT;
     const int mp = a.mp, kp = a.kp;
     int cur = i0, g0 = G.gp[i0], g1 = G.gp[i0 + 1];
     int cs = 0, cp = 0;
@@ -1081,11 +1079,18 @@ __device__ __forceinline__ void scan_group(const ScanArgs &a, TopShared &T, uint
         cs = G.cstart[c];
         cp = G.cpos[c];
     }
-    uint32_t cw[4][W];
-    int vx[4], ps[4];
-    // positions r0 + u * NT + tid of a round: cell pointer, vector index
-    // (kept inside the codes array) and codes, whose loads stay in flight
-    auto fetch = [&](int r0) {
+    for (int r0 = p0; r0 < p1; r0 += kRound) {
+        // every thread reads the fill before any wave appends this round, so
+        // all of them take the same compaction decision
+        const bool full = T.nb > bufcap - kRound;
+        __syncthreads();
+        if (full) {
+            if (MIXED) compact_top_margin(T, key, idx, a.R, twoE, bufcap - kRound);
+            else compact_top(T, key, idx, a.R);
+        }
+        const uint64_t thr = T.thr;
+        uint32_t cw[4][MPW > 0 ? MPW : 1];
+        int vx[4], ps[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             vx[u] = -1;
@@ -1100,38 +1105,23 @@ __device__ __forceinline__ void scan_group(const ScanArgs &a, TopShared &T, uint
                     cs = G.cstart[c];
                     cp = G.cpos[c];
                 }
-                vx[u] = min(cs + (p - g0), nvec - 1);
+                vx[u] = cs + (p - g0);
                 ps[u] = cp + (p - g0);
-                if (MPW > 0) load_codes<W>(a.codes, vx[u], cw[u]);
+                if (MPW > 0) load_codes<(MPW > 0 ? MPW : 1)>(a.codes, vx[u], cw[u]);
             }
         }
-    };
-    fetch(p0);
-    for (int r0 = p0; r0 < p1; r0 += kRound) {
-        // every thread reads the fill before any wave appends this round, so
-        // all of them take the same compaction decision
-        const bool full = T.nb > bufcap - kRound;
-        __syncthreads();
-        if (full) {
-            if (MIXED) compact_top_margin(T, key, idx, a.R, twoE, bufcap - kRound);
-            else compact_top(T, key, idx, a.R);
-        }
-        const uint64_t thr = T.thr;
-        uint64_t k[4];
-        int32_t v[4];
-        bool take[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            take[u] = false;
-            k[u] = 0;
-            v[u] = vx[u];
-            if (u < U && vx[u] >= 0) {
+            if (u >= U) continue;    // U is the same for every thread
+            bool take = false;
+            uint64_t k = 0;
+            if (vx[u] >= 0) {
                 float d = d0;
                 if (MIXED) {
                     float acc = 0.0f;
                     if (MPW > 0) {
 #pragma unroll
-                        for (int w4 = 0; w4 < W; ++w4) {
+                        for (int w4 = 0; w4 < (MPW > 0 ? MPW : 1); ++w4) {
                             const uint32_t wv = cw[u][w4];
 #pragma unroll
                             for (int b = 0; b < 4; ++b)
@@ -1144,7 +1134,7 @@ __device__ __forceinline__ void scan_group(const ScanArgs &a, TopShared &T, uint
                     d = d0 + S * acc;
                 } else if (MPW > 0) {
 #pragma unroll
-                    for (int w4 = 0; w4 < W; ++w4) {
+                    for (int w4 = 0; w4 < (MPW > 0 ? MPW : 1); ++w4) {
                         const uint32_t wv = cw[u][w4];
 #pragma unroll
                         for (int b = 0; b < 4; ++b)
@@ -1154,34 +1144,10 @@ __device__ __forceinline__ void scan_group(const ScanArgs &a, TopShared &T, uint
                     const uint8_t *cc = a.codes + (size_t)vx[u] * mp;
                     for (int j = 0; j < mp; ++j) d += lut[j * kp + cc[j]];
                 }
-                k[u] = ((uint64_t)sc_float_key(d) << 32) | (uint32_t)ps[u];
-                take[u] = k[u] < thr;
+                k = ((uint64_t)sc_float_key(d) << 32) | (uint32_t)ps[u];
+                take = k < thr;
             }
-        }
-        // the next round's codes load while this round appends and waits
-        if (r0 + kRound < p1) fetch(r0 + kRound);
-        // the round's survivors: one LDS atomic per wave
-        uint64_t bal[4];
-        int tot = 0;
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            bal[u] = __ballot(take[u]);
-            tot += __popcll(bal[u]);
-        }
-        if (tot > 0) {
-            int base = 0;
-            if (lane == 0) base = atomicAdd(&T.nb, tot);
-            base = __shfl(base, 0);
-            const uint64_t lt = (1ull << lane) - 1ull;
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                if (take[u]) {
-                    const int o = base + __popcll(bal[u] & lt);
-                    key[o] = k[u];
-                    idx[o] = v[u];
-                }
-                base += __popcll(bal[u]);
-            }
+            wave_append(&T.nb, key, idx, take, k, vx[u]);
         }
         __syncthreads();
     }
@@ -1650,13 +1616,13 @@ void k_search_fused(FusedArgs a) {
                 const int p0 = gp[i0], p1 = gp[i1];
                 if (mp == 8)
                     scan_group<2, MIXED>(sa, T, bkey, bidx, a.bufcap, lut, d0, p0, p1, i0, G, S,
-                                         twoE, a.nvec);
+                                         twoE);
                 else if (mp == 16)
                     scan_group<4, MIXED>(sa, T, bkey, bidx, a.bufcap, lut, d0, p0, p1, i0, G, S,
-                                         twoE, a.nvec);
+                                         twoE);
                 else
                     scan_group<0, MIXED>(sa, T, bkey, bidx, a.bufcap, lut, d0, p0, p1, i0, G, S,
-                                         twoE, a.nvec);
+                                         twoE);
                 __syncthreads();
             }
             mark(3);
@@ -1969,7 +1935,6 @@ static int search_fused(sc_ctx *ctx, const sc_index *ix, const float *d_q, int64
     fa.phase = st ? stats + 8 : nullptr;     // 6 counters after the 8 statistics
     fa.dot_absmax = ix->dot_absmax;
     fa.margin_scale = mixed_margin_scale();
-    fa.nvec = (int)ix->n;
     if (st) SC_HIP(hipMemsetAsync(stats + 8, 0, 48, sm));
     hipEvent_t e0 = nullptr, e1 = nullptr, el = nullptr;
     if (st) {
@@ -2375,7 +2340,6 @@ extern "C" int sc_plan_create(sc_ctx *ctx, const sc_index *ix, const sc_search_p
     fa.phase = P->stats + 8;
     fa.dot_absmax = ix->dot_absmax;
     fa.margin_scale = mixed_margin_scale();
-    fa.nvec = (int)ix->n;
     P->mixed = p->form == SC_FORM_MIXED;
     for (int v = 3; v <= 5; ++v)
         if (hipFuncSetAttribute((const void *)fused_kernel(v, P->mixed),
