@@ -761,7 +761,6 @@ struct FusedArgs {
     int32_t *qinfo;     // [nq][8] statistics (same fields as the staged path)
     const float *lut_dqc, *lut_dqr;   // [nq][mc*kc], [nq][mp*kp] or null: compute here
     unsigned long long *phase;        // [6]: wall-clock ticks per phase, workgroups
-    float dot_absmax;                 // max |dot_cr| (mixed form error bound)
     float margin_scale;               // mixed form: multiplies the error bound (1)
 };
 
@@ -773,10 +772,11 @@ struct FusedLayout {
 // sort keys (kc > 256); B: per-cell start, length, info, position, permuted
 // prefix [ccap + 1], permutation (u16); C: sorted scores + centroids [kc]
 // (after the traversal: per-centroid counts and the group list); D: diff_qc
-// (+ diff_qr when computed here); E: combined table [mp*kp]; F: query [dim].
+// (+ diff_qr when computed here); E: combined table [mp*kp] (FP32, or FP16
+// in the mixed form); F: query [dim].
 __host__ __device__ inline FusedLayout fused_layout(int dim, int mc, int kc, int mp, int kp,
                                                     int hcap, int ccap, int npow2, int bufcap,
-                                                    bool dqr_lds) {
+                                                    bool dqr_lds, bool half_table) {
     auto up = [](size_t v) { return (v + 15) & ~(size_t)15; };
     FusedLayout L{};
     size_t a = (size_t)bufcap * 12;
@@ -790,7 +790,7 @@ __host__ __device__ inline FusedLayout fused_layout(int dim, int mc, int kc, int
     L.c = up(L.b + (size_t)ccap * 18 + ((size_t)ccap + 1) * 4);
     L.d = up(L.c + (size_t)kc * 8);
     L.e = up(L.d + 4 * ((size_t)mc * kc + (dqr_lds ? (size_t)mp * kp : 0)));
-    L.f = up(L.e + 4 * (size_t)mp * kp);
+    L.f = up(L.e + (half_table ? 2 : 4) * (size_t)mp * kp);
     L.total = up(L.f + 4 * (size_t)dim);
     return L;
 }
@@ -1178,7 +1178,7 @@ void k_search_fused(FusedArgs a) {
     const int dim = sa.dim, mc = sa.mc, kc = sa.kc, mp = sa.mp, kp = sa.kp;
     const bool dqr_lds = a.lut_dqr == nullptr;
     const FusedLayout Lay =
-        fused_layout(dim, mc, kc, mp, kp, a.hcap, a.ccap, a.npow2, a.bufcap, dqr_lds);
+        fused_layout(dim, mc, kc, mp, kp, a.hcap, a.ccap, a.npow2, a.bufcap, dqr_lds, MIXED);
     uint64_t *bkey = reinterpret_cast<uint64_t *>(smem + Lay.a);
     int32_t *bidx = reinterpret_cast<int32_t *>(bkey + a.bufcap);
     uint64_t *keys = bkey;                                           // coarse sort, kc > 256
@@ -1254,37 +1254,12 @@ void k_search_fused(FusedArgs a) {
         __syncthreads();
         mark(0);
         const float qsum = s_qsum;
-        // Mixed form: error bound E of one approximate distance.  With
-        // B >= |diff_qr + dot_cr| for every table entry, S = 2^e >= B / 2^14
-        // (FP16 range) and M >= |every partial sum|, the FP16 entries are off
-        // by <= 2^-11 B (+ S 2^-25 when subnormal), the FP32 sum of mp of them
-        // by <= mp^2 2^-24 B, and the exact and approximate sums round by
-        // <= (mp + 1) 2^-24 M; E is 1.25x the total.  Every candidate of the
-        // exact top R has an approximate distance within 2E of the R-th
-        // smallest approximate one.
-        float S = 1.0f, twoE = 0.0f;
-        if (MIXED) {
-            const float *dsrc0 = dqr_lds ? dqr : a.lut_dqr + (size_t)q * lutn;
-            uint32_t mr = 0u, mcq = 0u;
-            for (int i = tid; i < lutn; i += NT) mr = max(mr, __float_as_uint(fabsf(dsrc0[i])));
+        if (MIXED) {   // max |diff_qc| for the error bound (read after later barriers)
+            uint32_t mcq = 0u;
             for (int j = tid; j < mc * kc; j += NT) mcq = max(mcq, __float_as_uint(fabsf(dqc[j])));
-            for (int o = 32; o > 0; o >>= 1) {
-                mr = max(mr, (uint32_t)__shfl_xor(mr, o));
-                mcq = max(mcq, (uint32_t)__shfl_xor(mcq, o));
-            }
-            if (lane == 0) { atomicMax(&T.amax_r, mr); atomicMax(&T.amax_c, mcq); }
-            __syncthreads();
-            const float B = (__uint_as_float(T.amax_r) + a.dot_absmax) * (1.0f + 0x1p-20f);
-            int ex = 0;
-            if (B > 0.0f) (void)frexpf(B * 0x1p-14f, &ex);
-            S = ldexpf(1.0f, ex);
-            const float fmp = (float)mp;
-            const float M = fabsf(qsum) + __uint_as_float(T.amax_c) + fmp * B;
-            const float E = ((fmp + 1.0f) * 0x1p-24f * M + fmp * fmp * 0x1p-24f * B +
-                             fmp * 0x1p-11f * B + fmp * S * 0x1p-25f) * 1.25f * a.margin_scale;
-            twoE = 2.0f * E;
+            for (int o = 32; o > 0; o >>= 1) mcq = max(mcq, (uint32_t)__shfl_xor(mcq, o));
+            if (lane == 0) atomicMax(&T.amax_c, mcq);
         }
-        const float invS = 1.0f / S;
         // coarse scores; SC sorts them by (score, centroid)
         if (sc && a.npow2 <= 256) {
             if (tid < 64) {
@@ -1562,6 +1537,38 @@ void k_search_fused(FusedArgs a) {
             const int ng = s_ngroup;
             const GroupCells G{gp, cells, cpos, order};
             const float *dsrc = dqr_lds ? dqr : a.lut_dqr + (size_t)q * lutn;
+            // Mixed form: error bound E of one approximate distance.  With
+            // B >= |diff_qr + dot_cr[h3]| for every entry of the query's
+            // combined tables (the maximum over its groups' tables, computed
+            // with the same FP32 additions as the tables), S = 2^e >= B / 2^14
+            // (FP16 range) and M >= |every partial sum|, the FP16 entries are
+            // off by <= 2^-11 B (+ S 2^-25 when subnormal), the FP32 sum of mp
+            // of them by <= mp^2 2^-24 B, and the exact and approximate sums
+            // round by <= (mp + 1) 2^-24 M; E is 1.25x the total.  Every
+            // candidate of the exact top R has an approximate distance within
+            // 2E of the R-th smallest approximate one.
+            float S = 1.0f, twoE = 0.0f;
+            if (MIXED) {
+                uint32_t mb = 0u;
+                for (int g = 0; g < ng; ++g) {
+                    const float *row = sa.dot_cr + (size_t)(gdesc[g] & 0xFFFFu) * bs;
+                    for (int i = tid; i < lutn; i += NT)
+                        mb = max(mb, __float_as_uint(fabsf(dsrc[i] + row[i])));
+                }
+                for (int o = 32; o > 0; o >>= 1) mb = max(mb, (uint32_t)__shfl_xor(mb, o));
+                if (lane == 0) atomicMax(&T.amax_r, mb);
+                __syncthreads();
+                const float B = __uint_as_float(T.amax_r) * (1.0f + 0x1p-20f);
+                int ex = 0;
+                if (B > 0.0f) (void)frexpf(B * 0x1p-14f, &ex);
+                S = ldexpf(1.0f, ex);
+                const float fmp = (float)mp;
+                const float M = fabsf(qsum) + __uint_as_float(T.amax_c) + fmp * B;
+                const float E = ((fmp + 1.0f) * 0x1p-24f * M + fmp * fmp * 0x1p-24f * B +
+                                 fmp * 0x1p-11f * B + fmp * S * 0x1p-25f) * 1.25f * a.margin_scale;
+                twoE = 2.0f * E;
+            }
+            const float invS = 1.0f / S;
             float dq[kLutRegs], rw[kLutRegs];
             if (lut_regs) {
 #pragma unroll
@@ -1876,12 +1883,16 @@ static FusedConfig fused_config(const sc_index *ix, const sc_search_params *p) {
     }
     f.ccap = (int)std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(ccap_ref, est), 1024));
     // the buffer takes one scan round (kRound) above the R it keeps; at most
-    // 8 entries per thread of a 256-thread workgroup (R <= 1024)
-    // (the mixed form keeps up to 3R within its error margin)
-    f.bufcap = ((p->form == SC_FORM_MIXED ? 3 * p->R : p->R) + kRound + 63) & ~63;
+    // 8 entries per thread of a 256-thread workgroup (R <= 1024).  The mixed
+    // form keeps every candidate within its error margin, up to 5R (3R + 512
+    // when smaller) before a query is handed to the exact staged kernels; at
+    // R = 100 that still leaves LDS for five workgroups per CU (the FP16 table
+    // takes half the FP32 one's room).
+    const int keep = p->form == SC_FORM_MIXED ? std::min(5 * p->R, 3 * p->R + 512) : p->R;
+    f.bufcap = (keep + kRound + 63) & ~63;
     f.tiled_lut = kc % kLutC == 0 && ix->kp % kLutC == 0;
     f.lay = fused_layout(ix->dim, ix->mc, kc, ix->mp, ix->kp, f.hcap, f.ccap, f.npow2, f.bufcap,
-                         !f.tiled_lut);
+                         !f.tiled_lut, p->form == SC_FORM_MIXED);
     // 2 KiB covers the kernel's static LDS
     f.occ = (int)std::max<size_t>(1, std::min<size_t>(8, (160 * 1024) / (f.lay.total + 2048)));
     f.variant = std::max(3, std::min(f.occ, 5));
@@ -1933,7 +1944,6 @@ static int search_fused(sc_ctx *ctx, const sc_index *ix, const float *d_q, int64
     fa.counter = counter; fa.flags = flags; fa.nflag = counter + 1; fa.qinfo = qinfo;
     fa.lut_dqc = fa.lut_dqr = nullptr;
     fa.phase = st ? stats + 8 : nullptr;     // 6 counters after the 8 statistics
-    fa.dot_absmax = ix->dot_absmax;
     fa.margin_scale = mixed_margin_scale();
     if (st) SC_HIP(hipMemsetAsync(stats + 8, 0, 48, sm));
     hipEvent_t e0 = nullptr, e1 = nullptr, el = nullptr;
@@ -2338,7 +2348,6 @@ extern "C" int sc_plan_create(sc_ctx *ctx, const sc_index *ix, const sc_search_p
     fa.lut_dqc = P->lut;
     fa.lut_dqr = P->lut ? P->lut + (size_t)nq * ix->mc * kc : nullptr;
     fa.phase = P->stats + 8;
-    fa.dot_absmax = ix->dot_absmax;
     fa.margin_scale = mixed_margin_scale();
     P->mixed = p->form == SC_FORM_MIXED;
     for (int v = 3; v <= 5; ++v)
