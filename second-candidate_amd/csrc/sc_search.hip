@@ -762,13 +762,14 @@ struct FusedArgs {
     const float *lut_dqc, *lut_dqr;   // [nq][mc*kc], [nq][mp*kp] or null: compute here
     unsigned long long *phase;        // [6]: wall-clock ticks per phase, workgroups
     float margin_scale;               // mixed form: multiplies the error bound (1)
+    int nvec;                         // vectors in the index
 };
 
 struct FusedLayout {
     size_t a, b, c, d, e, f, total;
 };
 
-// Region A: top-R buffer (bufcap keys + ids) | heap + row lengths | coarse
+// Region A: top-R buffer (bufcap keys) | heap + row lengths | coarse
 // sort keys (kc > 256); B: per-cell start, length, info, position, permuted
 // prefix [ccap + 1], permutation (u16); C: sorted scores + centroids [kc]
 // (after the traversal: per-centroid counts and the group list); D: diff_qc
@@ -779,7 +780,8 @@ __host__ __device__ inline FusedLayout fused_layout(int dim, int mc, int kc, int
                                                     bool dqr_lds, bool half_table) {
     auto up = [](size_t v) { return (v + 15) & ~(size_t)15; };
     FusedLayout L{};
-    size_t a = (size_t)bufcap * 12;
+    size_t a = (size_t)bufcap * 8;
+    if ((size_t)kc * 8 > a) a = (size_t)kc * 8;   // the grouping's two counts per centroid
     // heap (value, id) pairs, SC row lengths, then the SC window of
     // (start, length) per rank pair
     const size_t heap = (size_t)hcap * 8 + (size_t)kc * 4 + (size_t)kWin * kWin * 8;
@@ -812,10 +814,9 @@ __device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t x, int m) {
     return ((uint64_t)hi << 32) | lo;
 }
 
-// Appends the entries of the lanes with `take` at a counter, one LDS atomic
-// per wave.  Every lane of the wave must call it.
-__device__ __forceinline__ void wave_append(int *counter, uint64_t *key, int32_t *idx, bool take,
-                                            uint64_t k, int32_t v) {
+// Appends the keys of the lanes with `take` at a counter, one LDS atomic per
+// wave.  Every lane of the wave must call it.
+__device__ __forceinline__ void wave_append(int *counter, uint64_t *key, bool take, uint64_t k) {
     const uint64_t bal = __ballot(take);
     if (bal == 0) return;
     const int lane = threadIdx.x & 63;
@@ -826,7 +827,6 @@ __device__ __forceinline__ void wave_append(int *counter, uint64_t *key, int32_t
     if (take) {
         const int o = base + __popcll(bal & ((1ull << lane) - 1ull));
         key[o] = k;
-        idx[o] = v;
     }
 }
 
@@ -899,11 +899,11 @@ __device__ __forceinline__ uint64_t select_threshold(const uint64_t *key, int nb
     }
 }
 
-// Keeps exactly the R smallest keys of key/idx[0..T.nb) at the front.  The
+// Keeps exactly the R smallest keys of key[0..T.nb) at the front.  The
 // survivors move left chunk by chunk of blockDim.x entries: a chunk is read
 // whole before any of its survivors is written, and every write lands below
 // the chunk's end, so nothing unread is overwritten.  All threads call it.
-__device__ __forceinline__ void compact_top(TopShared &T, uint64_t *key, int32_t *idx, int R) {
+__device__ __forceinline__ void compact_top(TopShared &T, uint64_t *key, int R) {
     const int tid = threadIdx.x, NT = blockDim.x;
     __syncthreads();
     const int nb = T.nb;
@@ -912,9 +912,8 @@ __device__ __forceinline__ void compact_top(TopShared &T, uint64_t *key, int32_t
     for (int c0 = 0; c0 < nb; c0 += NT) {
         const int i = c0 + tid;
         const uint64_t k = i < nb ? key[i] : ~0ull;
-        const int32_t v = i < nb ? idx[i] : -1;
         __syncthreads();
-        wave_append(&T.wcount, key, idx, i < nb && k <= th, k, v);
+        wave_append(&T.wcount, key, i < nb && k <= th, k);
     }
     __syncthreads();
     if (tid == 0) { T.nb = R; T.thr = th; }
@@ -924,7 +923,7 @@ __device__ __forceinline__ void compact_top(TopShared &T, uint64_t *key, int32_t
 // Mixed form: keeps every key whose approximate distance is within twoE of
 // the R-th smallest (at least R keys).  When more than lim keys would stay it
 // keeps exactly R and sets T.ovf: the query is then redone exactly.
-__device__ __forceinline__ void compact_top_margin(TopShared &T, uint64_t *key, int32_t *idx, int R, float twoE,
+__device__ __forceinline__ void compact_top_margin(TopShared &T, uint64_t *key, int R, float twoE,
                                    int lim) {
     const int tid = threadIdx.x, NT = blockDim.x;
     __syncthreads();
@@ -949,9 +948,8 @@ __device__ __forceinline__ void compact_top_margin(TopShared &T, uint64_t *key, 
     for (int c0 = 0; c0 < nb; c0 += NT) {
         const int i = c0 + tid;
         const uint64_t k = i < nb ? key[i] : ~0ull;
-        const int32_t v = i < nb ? idx[i] : -1;
         __syncthreads();
-        wave_append(&T.wcount, key, idx, i < nb && k <= th, k, v);
+        wave_append(&T.wcount, key, i < nb && k <= th, k);
     }
     __syncthreads();
     if (tid == 0) {
@@ -998,7 +996,7 @@ __device__ __forceinline__ void wave_bitonic(uint64_t (&k)[E], int32_t (&v)[E]) 
 }
 
 template <int E>
-__device__ __forceinline__ void wave_sort_lds(uint64_t *key, int32_t *idx, int n) {   // wave 0 only
+__device__ __forceinline__ void wave_sort_lds(uint64_t *key, int n) {   // wave 0 only
     const int lane = threadIdx.x & 63;
     uint64_t k[E];
     int32_t v[E];
@@ -1006,33 +1004,33 @@ __device__ __forceinline__ void wave_sort_lds(uint64_t *key, int32_t *idx, int n
     for (int e = 0; e < E; ++e) {
         const int i = lane * E + e;
         k[e] = i < n ? key[i] : ~0ull;
-        v[e] = i < n ? idx[i] : -1;
+        v[e] = 0;
     }
     wave_bitonic<E>(k, v);
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         const int i = lane * E + e;
-        if (i < n) { key[i] = k[e]; idx[i] = v[e]; }
+        if (i < n) key[i] = k[e];
     }
 }
 
-// Sorts key/idx[0..n) ascending; the buffer holds at least next_pow2(n)
+// Sorts key[0..n) ascending; the buffer holds at least next_pow2(n)
 // entries.  All threads call it.
-__device__ __forceinline__ void sort_top(uint64_t *key, int32_t *idx, int n) {
+__device__ __forceinline__ void sort_top(uint64_t *key, int n) {
     const int tid = threadIdx.x, NT = blockDim.x;
     __syncthreads();
     if (n <= 256) {
         if (tid < 64) {
-            if (n <= 64) wave_sort_lds<1>(key, idx, n);
-            else if (n <= 128) wave_sort_lds<2>(key, idx, n);
-            else wave_sort_lds<4>(key, idx, n);
+            if (n <= 64) wave_sort_lds<1>(key, n);
+            else if (n <= 128) wave_sort_lds<2>(key, n);
+            else wave_sort_lds<4>(key, n);
         }
         __syncthreads();
         return;
     }
     int np = 1;
     while (np < n) np <<= 1;
-    for (int i = n + tid; i < np; i += NT) { key[i] = ~0ull; idx[i] = -1; }
+    for (int i = n + tid; i < np; i += NT) key[i] = ~0ull;
     __syncthreads();
     for (int size = 2; size <= np; size <<= 1) {
         for (int stride = size >> 1; stride > 0; stride >>= 1) {
@@ -1040,10 +1038,7 @@ __device__ __forceinline__ void sort_top(uint64_t *key, int32_t *idx, int n) {
                 const int lo = 2 * i - (i & (stride - 1)), hi = lo + stride;
                 const bool upw = (lo & size) == 0;
                 const uint64_t x = key[lo], y = key[hi];
-                if ((x > y) == upw) {
-                    key[lo] = y; key[hi] = x;
-                    const int32_t t = idx[lo]; idx[lo] = idx[hi]; idx[hi] = t;
-                }
+                if ((x > y) == upw) { key[lo] = y; key[hi] = x; }
             }
             __syncthreads();
         }
@@ -1066,8 +1061,7 @@ struct GroupCells {
 // (helpers are force-inlined: the kernel's register-variant attribute keeps
 // the inliner from inlining ordinary calls, and calls spill registers)
 template <int MPW, bool MIXED>
-__device__ __forceinline__ void scan_group(const ScanArgs &a, TopShared &T, uint64_t *key, int32_t *idx,
-                           int bufcap, const float *lut, float d0, int p0, int p1, int i0,
+__device__ __forceinline__ void scan_group(const ScanArgs &a, TopShared &T, uint64_t *key, int bufcap, const float *lut, float d0, int p0, int p1, int i0,
                            const GroupCells &G, float S, float twoE) {
     const _Float16 *lut16 = reinterpret_cast<const _Float16 *>(lut);
     const int tid = threadIdx.x, NT = blockDim.x, U = kRound / NT;
@@ -1085,8 +1079,8 @@ __device__ __forceinline__ void scan_group(const ScanArgs &a, TopShared &T, uint
         const bool full = T.nb > bufcap - kRound;
         __syncthreads();
         if (full) {
-            if (MIXED) compact_top_margin(T, key, idx, a.R, twoE, bufcap - kRound);
-            else compact_top(T, key, idx, a.R);
+            if (MIXED) compact_top_margin(T, key, a.R, twoE, bufcap - kRound);
+            else compact_top(T, key, a.R);
         }
         const uint64_t thr = T.thr;
         uint32_t cw[4][MPW > 0 ? MPW : 1];
@@ -1147,7 +1141,7 @@ __device__ __forceinline__ void scan_group(const ScanArgs &a, TopShared &T, uint
                 k = ((uint64_t)sc_float_key(d) << 32) | (uint32_t)ps[u];
                 take = k < thr;
             }
-            wave_append(&T.nb, key, idx, take, k, vx[u]);
+            wave_append(&T.nb, key, take, k);
         }
         __syncthreads();
     }
@@ -1179,8 +1173,7 @@ void k_search_fused(FusedArgs a) {
     const bool dqr_lds = a.lut_dqr == nullptr;
     const FusedLayout Lay =
         fused_layout(dim, mc, kc, mp, kp, a.hcap, a.ccap, a.npow2, a.bufcap, dqr_lds, MIXED);
-    uint64_t *bkey = reinterpret_cast<uint64_t *>(smem + Lay.a);
-    int32_t *bidx = reinterpret_cast<int32_t *>(bkey + a.bufcap);
+    uint64_t *bkey = reinterpret_cast<uint64_t *>(smem + Lay.a);   // (dist key << 32) | position
     uint64_t *keys = bkey;                                           // coarse sort, kc > 256
     int32_t *cells = reinterpret_cast<int32_t *>(smem + Lay.b);     // cell starts
     int32_t *clen = cells + a.ccap;
@@ -1622,13 +1615,13 @@ void k_search_fused(FusedArgs a) {
                 }
                 const int p0 = gp[i0], p1 = gp[i1];
                 if (mp == 8)
-                    scan_group<2, MIXED>(sa, T, bkey, bidx, a.bufcap, lut, d0, p0, p1, i0, G, S,
+                    scan_group<2, MIXED>(sa, T, bkey, a.bufcap, lut, d0, p0, p1, i0, G, S,
                                          twoE);
                 else if (mp == 16)
-                    scan_group<4, MIXED>(sa, T, bkey, bidx, a.bufcap, lut, d0, p0, p1, i0, G, S,
+                    scan_group<4, MIXED>(sa, T, bkey, a.bufcap, lut, d0, p0, p1, i0, G, S,
                                          twoE);
                 else
-                    scan_group<0, MIXED>(sa, T, bkey, bidx, a.bufcap, lut, d0, p0, p1, i0, G, S,
+                    scan_group<0, MIXED>(sa, T, bkey, a.bufcap, lut, d0, p0, p1, i0, G, S,
                                          twoE);
                 __syncthreads();
             }
@@ -1641,7 +1634,7 @@ void k_search_fused(FusedArgs a) {
                 // candidate its exact distance: q_sum + diff_qc[h3], then
                 // + (diff_qr + dot_cr) per sub-quantizer in order, the same
                 // operations as the plain form's table and scan
-                if (sum >= R && T.nb > R) compact_top_margin(T, bkey, bidx, R, twoE, a.bufcap);
+                if (sum >= R && T.nb > R) compact_top_margin(T, bkey, R, twoE, a.bufcap);
                 const int m = T.nb;
                 if (!T.ovf) {
                     const float *dsrc1 = dqr_lds ? dqr : a.lut_dqr + (size_t)q * lutn;
@@ -1654,7 +1647,8 @@ void k_search_fused(FusedArgs a) {
                             if (cpos[mid] <= pos) lo = mid; else hi = mid;
                         }
                         const int h3 = (int)(cinfo[lo] & 0xFFFFu);
-                        const uint8_t *cc = sa.codes + (size_t)bidx[i] * mp;
+                        const int v = min(max(cells[lo] + (pos - cpos[lo]), 0), a.nvec - 1);
+                        const uint8_t *cc = sa.codes + (size_t)v * mp;
                         const float *row = sa.dot_cr + (size_t)h3 * bs;
                         float d = qsum + dqc[h3];
                         for (int j = 0; j < mp; ++j) {
@@ -1683,7 +1677,7 @@ void k_search_fused(FusedArgs a) {
             }
             int nout;
             if (sum >= R) {
-                if (T.nb > R) compact_top(T, bkey, bidx, R);
+                if (T.nb > R) compact_top(T, bkey, R);
                 nout = R;
             } else {
                 nout = T.nb;
@@ -1692,13 +1686,22 @@ void k_search_fused(FusedArgs a) {
                     bkey[i] = (k << 32) | (k >> 32);
                 }
             }
-            sort_top(bkey, bidx, nout);
+            sort_top(bkey, nout);
             int32_t *ids = sa.ids + (size_t)q * R;
             float *dists = sa.dists + (size_t)q * R;
             for (int r = tid; r < R; r += NT) {
                 if (r < nout) {
-                    ids[r] = sa.pid[bidx[r]];
                     const uint64_t k = bkey[r];
+                    // the vector at traversal position pos: last cell with
+                    // cpos <= pos, then the offset into it
+                    const int pos = (int)(sum >= R ? (uint32_t)k : (uint32_t)(k >> 32));
+                    int lo = 0, hi = ncell;
+                    while (hi - lo > 1) {
+                        const int mid = (lo + hi) >> 1;
+                        if (cpos[mid] <= pos) lo = mid; else hi = mid;
+                    }
+                    const int v = min(max(cells[lo] + (pos - cpos[lo]), 0), a.nvec - 1);
+                    ids[r] = sa.pid[v];
                     dists[r] = sc_key_float(sum >= R ? (uint32_t)(k >> 32) : (uint32_t)k);
                 } else {
                     ids[r] = -1;
@@ -1884,11 +1887,14 @@ static FusedConfig fused_config(const sc_index *ix, const sc_search_params *p) {
     f.ccap = (int)std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(ccap_ref, est), 1024));
     // the buffer takes one scan round (kRound) above the R it keeps; at most
     // 8 entries per thread of a 256-thread workgroup (R <= 1024).  The mixed
-    // form keeps every candidate within its error margin, up to 5R (3R + 512
-    // when smaller) before a query is handed to the exact staged kernels; at
-    // R = 100 that still leaves LDS for five workgroups per CU (the FP16 table
-    // takes half the FP32 one's room).
-    const int keep = p->form == SC_FORM_MIXED ? std::min(5 * p->R, 3 * p->R + 512) : p->R;
+    // form keeps every candidate within its error margin, up to 5R + 512
+    // (3R + 1024 when smaller) before a query is handed to the exact staged
+    // kernels.  512 entries of slack above what a compaction keeps let most
+    // queries compact once or twice; at R = 100 both forms leave LDS for five
+    // workgroups per CU (8-byte keys; the FP16 table takes half the FP32
+    // one's room).
+    const int keep = p->form == SC_FORM_MIXED ? std::min(5 * p->R + 512, 3 * p->R + 1024)
+                                              : p->R + 512;
     f.bufcap = (keep + kRound + 63) & ~63;
     f.tiled_lut = kc % kLutC == 0 && ix->kp % kLutC == 0;
     f.lay = fused_layout(ix->dim, ix->mc, kc, ix->mp, ix->kp, f.hcap, f.ccap, f.npow2, f.bufcap,
@@ -1945,6 +1951,7 @@ static int search_fused(sc_ctx *ctx, const sc_index *ix, const float *d_q, int64
     fa.lut_dqc = fa.lut_dqr = nullptr;
     fa.phase = st ? stats + 8 : nullptr;     // 6 counters after the 8 statistics
     fa.margin_scale = mixed_margin_scale();
+    fa.nvec = (int)ix->n;
     if (st) SC_HIP(hipMemsetAsync(stats + 8, 0, 48, sm));
     hipEvent_t e0 = nullptr, e1 = nullptr, el = nullptr;
     if (st) {
@@ -2349,6 +2356,7 @@ extern "C" int sc_plan_create(sc_ctx *ctx, const sc_index *ix, const sc_search_p
     fa.lut_dqr = P->lut ? P->lut + (size_t)nq * ix->mc * kc : nullptr;
     fa.phase = P->stats + 8;
     fa.margin_scale = mixed_margin_scale();
+    fa.nvec = (int)ix->n;
     P->mixed = p->form == SC_FORM_MIXED;
     for (int v = 3; v <= 5; ++v)
         if (hipFuncSetAttribute((const void *)fused_kernel(v, P->mixed),
