@@ -44,6 +44,12 @@ extern "C" {
 #define SC_PREC_BF16 1    /* MFMA bf16 distance GEMM + exact re-check */
 #define SC_FORM_PLAIN 0   /* FP32 LUT scan */
 #define SC_FORM_MIXED 1   /* FP16 LUT scan + exact FP32 re-rank */
+#define SC_FORM_REAL 2    /* the reference's real_dist = true: every gathered
+                             candidate ranked by its exact distance to its raw
+                             vector (include/query.h:307-312,
+                             include/multi_query.h:255-259,
+                             include/sc_query.h:232-237); needs
+                             sc_index_attach_raw */
 
 typedef struct sc_ctx sc_ctx;
 typedef struct sc_index sc_index;
@@ -124,6 +130,11 @@ int sc_index_load(sc_ctx *ctx, const int32_t *h_L, int64_t ncells,
                   const float *h_pq, int32_t mp, int32_t kp, int32_t nc,
                   sc_index **out);
 int sc_index_destroy(sc_index *ix);
+/* Gives the index the raw base vectors d_raw [n][dim] (float, in vector id
+ * order) for SC_FORM_REAL searches: PQQuery::load_data
+ * (include/query.h:83-86).  The index keeps the pointer, not a copy; the
+ * buffer must outlive the searches that use it.  d_raw = NULL detaches. */
+int sc_index_attach_raw(sc_index *ix, const float *d_raw, int64_t n);
 /* Shape: out6 = {n, ncells, mc, kc, mp, kp} */
 int sc_index_shape(const sc_index *ix, int64_t *out6);
 /* Copies the flattened index back (any pointer may be NULL):

@@ -594,6 +594,44 @@ done:
     return rc;
 }
 
+/* The searches with real_dist = true (include/query.h:307-312,
+ * include/multi_query.h:255-259, include/sc_query.h:232-237): the same
+ * traversal and candidates as sco_search, each candidate's distance the S1
+ * distance between the query and its raw vector raw[id] (stored as float),
+ * then the same top-R (canonical order; traversal order below R). */
+int sco_search_real(const sco_index *ix, int32_t kind, const float *q, const float *raw,
+                    int32_t R, int32_t w, int32_t T, int32_t *ids, float *dists,
+                    sco_stats *st) {
+    sco_stats s0;
+    const int64_t n = ix->L[ix->ncells - 1];   /* vectors in the index */
+    int32_t *all = (int32_t *)malloc(sizeof(int32_t) * ((size_t)n + 1));
+    int32_t *tid = (int32_t *)malloc(sizeof(int32_t) * (size_t)R);
+    float *td = (float *)malloc(sizeof(float) * (size_t)R);
+    if (!all || !tid || !td) { free(all); free(tid); free(td); return SCO_ERR_NOMEM; }
+    int rc = sco_search(ix, kind, q, R, w, T, ix->kc, tid, td, &s0, NULL, NULL, NULL, all,
+                        NULL, n);
+    if (rc == 0) {
+        const int32_t sum = s0.sum;
+        dpos *cand = (dpos *)malloc(sizeof(dpos) * ((size_t)sum + 1));
+        if (!cand) rc = SCO_ERR_NOMEM;
+        else {
+            for (int32_t p = 0; p < sum; ++p) {
+                cand[p].d = (float)sco_dist_l2sq(q, raw + (size_t)all[p] * ix->dim, ix->dim);
+                cand[p].pos = p;
+            }
+            if (sum >= R) qsort(cand, (size_t)sum, sizeof(dpos), cmp_dpos);
+            for (int32_t r = 0; r < R; ++r) {
+                if (r < sum) { ids[r] = all[cand[r].pos]; dists[r] = cand[r].d; }
+                else { ids[r] = -1; dists[r] = INFINITY; }
+            }
+            if (st) *st = s0;
+            free(cand);
+        }
+    }
+    free(all); free(tid); free(td);
+    return rc;
+}
+
 int sco_search_batch(const sco_index *ix, int32_t kind, const float *q,
                      int64_t nq, int32_t R, int32_t w, int32_t T, int32_t M,
                      int32_t *ids, float *dists, sco_stats *st, int32_t nthreads) {

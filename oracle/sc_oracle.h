@@ -111,6 +111,12 @@ int sco_search(const sco_index *ix, int32_t kind, const float *q, int32_t R,
                sco_stats *st, int32_t *cells, int32_t *s1, int32_t *s2,
                int32_t *all_ids, float *all_dists, int64_t cap_all);
 
+/* real_dist = true: the candidates of sco_search ranked by their S1 distance
+ * to the raw vectors raw [n][dim] (by id). */
+int sco_search_real(const sco_index *ix, int32_t kind, const float *q, const float *raw,
+                    int32_t R, int32_t w, int32_t T, int32_t *ids, float *dists,
+                    sco_stats *st);
+
 /* Many queries (OpenMP over queries), ids/dists [nq][R], stats [nq] (NULL ok) */
 int sco_search_batch(const sco_index *ix, int32_t kind, const float *q,
                      int64_t nq, int32_t R, int32_t w, int32_t T, int32_t M,

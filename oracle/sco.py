@@ -78,6 +78,8 @@ def _load():
     lib.sco_search_batch.argtypes = [C.POINTER(Index), C.c_int32, _f32p, C.c_int64,
                                      C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                      _i32p, _f32p, C.c_void_p, C.c_int32]
+    lib.sco_search_real.argtypes = [C.POINTER(Index), C.c_int32, _f32p, _f32p, C.c_int32,
+                                    C.c_int32, C.c_int32, _i32p, _f32p, C.c_void_p]
     lib.sco_heap4_insert.argtypes = [C.POINTER(Heap4), C.c_float] + [C.c_int32] * 4
     lib.sco_heap4_pop.argtypes = [C.POINTER(Heap4)] + [C.POINTER(C.c_int32)] * 4
     lib.sco_topr.argtypes = [_f32p, C.c_int32, C.c_int32, _i32p]
@@ -237,6 +239,19 @@ class OracleIndex:
         if rc:
             raise ValueError("sco_search rc=%d" % rc)
         return ids, dists, st, extra
+
+    def search_real(self, kind, q, raw, R, w, T):
+        """The search with real_dist = true: candidates ranked by their S1
+        distance to the raw vectors raw [n][dim] (by id)."""
+        ids = np.zeros(R, np.int32)
+        dists = np.zeros(R, np.float32)
+        st = Stats()
+        rc = lib().sco_search_real(C.byref(self.s), kind, np.ascontiguousarray(q, np.float32),
+                                   np.ascontiguousarray(raw, np.float32), R, w, T, ids, dists,
+                                   C.cast(C.pointer(st), C.c_void_p))
+        if rc:
+            raise ValueError("sco_search_real rc=%d" % rc)
+        return ids, dists, st
 
     def search_batch(self, kind, Q, R, w, T, M=None, threads=None):
         M = self.kc if M is None else M

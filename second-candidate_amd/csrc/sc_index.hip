@@ -317,6 +317,16 @@ extern "C" int sc_index_load(sc_ctx *ctx, const int32_t *h_L, int64_t ncells,
     return SC_OK;
 }
 
+extern "C" int sc_index_attach_raw(sc_index *ix, const float *d_raw, int64_t n) {
+    if (!ix || (d_raw && n != ix->n)) {
+        sc_set_error("sc_index_attach_raw: the raw vectors must match the index (n=%lld)",
+                     ix ? (long long)ix->n : 0ll);
+        return SC_ERR_ARG;
+    }
+    ix->raw = d_raw;
+    return SC_OK;
+}
+
 extern "C" int sc_index_destroy(sc_index *ix) {
     if (!ix) return SC_OK;
     if (ix->ctx) {

@@ -26,7 +26,8 @@ struct sc_index {
     uint8_t *codes;                 // [n][mp] in cell order
     int32_t max_cell;               // largest cell size
     int64_t nonempty;               // non-empty cells
-    float dot_absmax;               // max |dot_cr| (error bound of the mixed form)
+    float dot_absmax;               // max |dot_cr| (an index statistic)
+    const float *raw;               // raw base vectors [n][dim] by id (not owned) or null
 };
 
 void sc_set_error(const char *fmt, ...);

@@ -408,6 +408,8 @@ struct ScanArgs {
     int32_t *sums;
     const int32_t *qlist;   // rescan: workgroup b handles query qlist[b]
     const int32_t *clen;    // [nq][ccap] cell lengths (cells holds cell starts)
+    const float *Q;         // SC_FORM_REAL: the queries [nq][dim]
+    const float *raw;       // SC_FORM_REAL: raw base vectors [n][dim] by id, else null
 };
 
 struct ScanShared {
@@ -632,6 +634,55 @@ __device__ void scan_run_generic(const ScanArgs &a, ScanShared &S, const float *
     }
 }
 
+// SC_FORM_REAL (include/query.h:307-312, include/multi_query.h:255-259,
+// include/sc_query.h:232-237): a candidate's distance is S1 between the query
+// and its raw vector (lib/simple-cluster/include/utilities.h:194-230: the
+// float difference, squared and summed in double in dimension order, stored
+// as float; the square is exact, so an FMA gives the same sums).  The raw
+// vector is read 16 values at a time so its loads overlap.
+__device__ void scan_run_real(const ScanArgs &a, ScanShared &S, const float *qv, int rn,
+                              int rtotal, int base_pos) {
+    const int tid = threadIdx.x, NT = blockDim.x, U = kRound / NT;
+    const int dim = a.dim;
+    int cur = 0;
+    for (int p0 = 0; p0 < rtotal; p0 += kRound) {
+        const bool full = S.nb > kBufCap - kRound;   // read by all before any append
+        __syncthreads();
+        if (full) compact(S, a.R);
+        for (int u = 0; u < U; ++u) {
+            const int p = p0 + u * NT + tid;
+            if (p >= rtotal) continue;
+            while (cur + 1 < rn && S.c_pref[cur + 1] <= p) ++cur;
+            const int vidx = S.c_start[cur] + (p - S.c_pref[cur]);
+            const float *y = a.raw + (size_t)a.pid[vidx] * dim;
+            double acc = 0.0;
+            int t = 0;
+            for (; t + 16 <= dim; t += 16) {
+                float yv[16];
+#pragma unroll
+                for (int j = 0; j < 16; ++j) yv[j] = y[t + j];
+#pragma unroll
+                for (int j = 0; j < 16; ++j) {
+                    const double d = (double)(qv[t + j] - yv[j]);
+                    acc = __fma_rn(d, d, acc);
+                }
+            }
+            for (; t < dim; ++t) {
+                const double d = (double)(qv[t] - y[t]);
+                acc = __fma_rn(d, d, acc);
+            }
+            const uint64_t key =
+                ((uint64_t)sc_float_key((float)acc) << 32) | (uint32_t)(base_pos + p);
+            if (key < S.thr) {
+                const int o = atomicAdd(&S.nb, 1);
+                S.key[o] = key;
+                S.idx[o] = vidx;
+            }
+        }
+        __syncthreads();
+    }
+}
+
 __device__ void scan_query(const ScanArgs &a, ScanShared &S, float *lut, int64_t q,
                            const float *dqr, const float *dqc, float qsum,
                            const int32_t *cstart, const int32_t *clen, const uint32_t *cinfo,
@@ -640,9 +691,12 @@ __device__ void scan_query(const ScanArgs &a, ScanShared &S, float *lut, int64_t
     const int mp = a.mp, kp = a.kp;
     const int lutn = mp * kp;
     const bool imi = a.kind == SC_KIND_IMI;
+    const bool real = a.raw != nullptr;   // lut holds the query vector then
     const int bs = kp * mp / a.mc;
     if (tid == 0) { S.nb = 0; S.thr = ~(uint64_t)0; }
-    if (imi)
+    if (real)
+        for (int i = tid; i < a.dim; i += blockDim.x) lut[i] = a.Q[(size_t)q * a.dim + i];
+    else if (imi)
         for (int i = tid; i < lutn; i += blockDim.x) lut[i] = dqr[i];
     int lut_row = -1;     // h3 whose combined row is staged (IVF/SC)
     int base_pos = 0;     // position of the first candidate of the run
@@ -670,7 +724,7 @@ __device__ void scan_query(const ScanArgs &a, ScanShared &S, float *lut, int64_t
         const int rn = S.run_cells;
         const int rtotal = S.c_pref[rn];
         float d0 = 0.0f;
-        if (!imi) {
+        if (!imi && !real) {
             const int h3 = (int)(S.c_info[0] & 0xFFFFu);
             d0 = qsum + dqc[h3];
             if (h3 != lut_row) {
@@ -680,7 +734,9 @@ __device__ void scan_query(const ScanArgs &a, ScanShared &S, float *lut, int64_t
             }
         }
         __syncthreads();
-        if (!imi && mp == 8)
+        if (real)
+            scan_run_real(a, S, lut, rn, rtotal, base_pos);
+        else if (!imi && mp == 8)
             scan_run_packed<2>(a, S, lut, d0, rn, rtotal, base_pos);
         else if (!imi && mp == 16)
             scan_run_packed<4>(a, S, lut, d0, rn, rtotal, base_pos);
@@ -2070,7 +2126,7 @@ extern "C" int sc_search(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
         return SC_ERR_ARG;
     }
     if (p->R < 1 || p->R > kBufCap - kRound || p->w < 1 || p->T < 0 ||
-        (p->form != SC_FORM_PLAIN && p->form != SC_FORM_MIXED)) {
+        (p->form != SC_FORM_PLAIN && p->form != SC_FORM_MIXED && p->form != SC_FORM_REAL)) {
         sc_set_error("sc_search: bad parameters (R=%d w=%d T=%d form=%d)", p->R, p->w, p->T,
                      p->form);
         return SC_ERR_ARG;
@@ -2094,9 +2150,15 @@ extern "C" int sc_search(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
                      ix->mp * ix->kp);
         return SC_ERR_ARG;
     }
+    if (p->form == SC_FORM_REAL && !ix->raw) {
+        sc_set_error("sc_search: SC_FORM_REAL needs the raw vectors (sc_index_attach_raw)");
+        return SC_ERR_ARG;
+    }
     if (st) *st = sc_search_stats{};
     if (nq == 0) return SC_OK;
-    if (kind != SC_KIND_IMI && !getenv("SC_NO_FUSED")) {
+    // real distances run on the staged kernels: the raw vectors are read per
+    // candidate, so the scan is bound by those reads, not by the on-chip stages
+    if (kind != SC_KIND_IMI && p->form != SC_FORM_REAL && !getenv("SC_NO_FUSED")) {
         int rc = search_fused(ctx, ix, d_q, nq, p, d_ids, d_dists, d_sums, st);
         if (rc != SC_ERR_TOO_LARGE) return rc;      // otherwise: staged kernels
         if (st) *st = sc_search_stats{};
@@ -2171,8 +2233,11 @@ static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
         ScanArgs sa{kind, ix->dim, ix->mc, kc, ix->mp, ix->kp, p->R, P.ccap, qs, dqc, dqr,
                     ix->dot_cr, ix->L, ix->pid, ix->codes, cells, cinfo, qinfo,
                     d_ids + (size_t)q0 * p->R, d_dists + (size_t)q0 * p->R,
-                    d_sums ? d_sums + q0 : nullptr, nullptr, clen};
-        const size_t lds = sizeof(ScanShared) + sizeof(float) * ix->mp * ix->kp;
+                    d_sums ? d_sums + q0 : nullptr, nullptr, clen,
+                    Q, p->form == SC_FORM_REAL ? ix->raw : nullptr};
+        const size_t lds = sizeof(ScanShared) +
+                           sizeof(float) * std::max<size_t>((size_t)ix->mp * ix->kp,
+                                                            p->form == SC_FORM_REAL ? ix->dim : 0);
         if (hcap > 0) {
             hipLaunchKernelGGL(k_scan_topk, dim3((unsigned)cn), dim3(kScanThreads), lds,
                                ctx->stream, sa);
@@ -2334,7 +2399,7 @@ extern "C" int sc_plan_create(sc_ctx *ctx, const sc_index *ix, const sc_search_p
     const FusedConfig F = fused_config(ix, p);
     const FusedLayout &Lay = F.lay;
     const int hcap = F.hcap, ccap = F.ccap, npow2 = F.npow2;
-    P->fused = kind != SC_KIND_IMI && Lay.total <= 150 * 1024;
+    P->fused = kind != SC_KIND_IMI && p->form != SC_FORM_REAL && Lay.total <= 150 * 1024;
     if (!P->fused) { *out = P; return SC_OK; }
     P->lds = Lay.total;
     bool ok = hipMalloc(&P->counter, 256) == hipSuccess &&

@@ -610,6 +610,15 @@ protected:
     int nc_ = 1;
     sc_index *index_ = nullptr;
     detail::Dev q_, ids_, dists_, sums_;
+    detail::Dev raw_dev_;         // raw vectors on the device (load_data)
+    int64_t raw_n_ = 0;
+
+    // Gives the index the raw vectors when both are loaded and match (a
+    // real_dist search reports the mismatch otherwise).
+    inline void attach_raw() {
+        if (index_ && raw_dev_.p && sc_index_attach_raw(index_, raw_dev_.as<float>(), raw_n_) != SC_OK)
+            std::cerr << sc_last_error() << std::endl;
+    }
     sc_search_stats last_{};
 
     // Loads `<file>` (int non_empty, int N, per cell: L, pids, codes) into
@@ -651,6 +660,7 @@ protected:
                                     config.dim, cq, config.mc, config.kc, pq, config.mp,
                                     config.kp, nc, &index_),
                       "sc_index_load");
+        attach_raw();
         return config.N;
     }
 
@@ -658,7 +668,7 @@ protected:
     // callee-allocated result/dist arrays (length sum, first min(R, sum)
     // entries are the answer).
     inline bool run_one(const float *query, int kind, int R, int w, int T, int M, float *&dist,
-                        int *&result, int &sum) {
+                        int *&result, int &sum, bool real_dist = false) {
         if (!q_.p) q_.alloc(sizeof(float) * config.dim);
         if (ids_.bytes < sizeof(int) * (size_t)R) {
             ids_.alloc(sizeof(int) * (size_t)R);
@@ -666,7 +676,7 @@ protected:
         }
         if (!sums_.p) sums_.alloc(sizeof(int));
         q_.put(query, sizeof(float) * config.dim);
-        sc_search_params p{kind, SC_FORM_PLAIN, R, w, T, M};
+        sc_search_params p{kind, real_dist ? SC_FORM_REAL : SC_FORM_PLAIN, R, w, T, M};
         int rc = sc_search(detail::ctx(), index_, q_.as<float>(), 1, &p, ids_.as<int32_t>(),
                            dists_.as<float>(), sums_.as<int32_t>(), &last_);
         if (rc == SC_ERR_CONFIG) {
@@ -710,11 +720,15 @@ public:
         (void)verbose;
         return load_ivf(filename, (size_t)size, 1);
     }
-    // query.h:83-86 (raw vectors for real distances)
+    // query.h:83-86 (raw vectors for real distances), also kept on the device
     template <typename DataType>
     inline void load_data(const char *filename, int offset, bool verbose) {
         ::operator delete(raw_data);
         config.N = detail::load_rows<DataType>(filename, raw_data, offset, config.dim, verbose);
+        raw_n_ = config.N;
+        raw_dev_.alloc(sizeof(float) * (size_t)raw_n_ * config.dim);
+        raw_dev_.put(raw_data, sizeof(float) * (size_t)raw_n_ * config.dim);
+        attach_raw();
     }
     // The tables and LUTs live on the device and are built inside the search
     // (query.h:91-194); these keep the call sequence of the reference.
@@ -726,12 +740,12 @@ public:
     inline void search_ivfadc(float *query, float *&v_tmp, float *&dist, int *&result,
                               int *&buckets, int *&prebuck, int &sum, int R, int w, int T,
                               bool real_dist, bool verbose) {
-        (void)v_tmp; (void)buckets; (void)prebuck; (void)real_dist; (void)verbose;
+        (void)v_tmp; (void)buckets; (void)prebuck; (void)verbose;
         if (config.mc != 1) {
             std::cerr << "This search method is for IVFADC only" << std::endl;
             return;
         }
-        run_one(query, SC_KIND_IVFADC, R, w, T, config.kc, dist, result, sum);
+        run_one(query, SC_KIND_IVFADC, R, w, T, config.kc, dist, result, sum, real_dist);
     }
 
     // Batched search (no reference counterpart): nq queries in one call,
@@ -781,13 +795,13 @@ public:
                               int &sum, int R, int w, int T, int M, int &e, double &t1,
                               double &t2, bool real_dist, bool verbose) {
         (void)v_tmp; (void)q; (void)tmp; (void)hid1; (void)hid2; (void)hid3; (void)hid4;
-        (void)s1; (void)s2; (void)prebuck; (void)cache; (void)traversed; (void)real_dist;
+        (void)s1; (void)s2; (void)prebuck; (void)cache; (void)traversed;
         (void)verbose;
         if (config.mc != 2) {
             std::cerr << "This search method is for Multi-D-ADC-2 only" << std::endl;
             return;
         }
-        if (!run_one(query, SC_KIND_IMI, R, w, T, M, dist, result, sum)) return;
+        if (!run_one(query, SC_KIND_IMI, R, w, T, M, dist, result, sum, real_dist)) return;
         e = (int)(last_.visited - last_.cells);
         t1 += last_.ms_lut * CLOCKS_PER_SEC / 1000.0;
         t2 += last_.ms_rank * CLOCKS_PER_SEC / 1000.0;
@@ -817,12 +831,12 @@ public:
                               int *&prebuck, int *&cache, bool *traversed, int &sum, int R,
                               int w, int T, int M, bool real_dist, bool verbose) {
         (void)v_tmp; (void)tmp; (void)hid1; (void)hid2; (void)hid3; (void)hid4; (void)s1;
-        (void)s2; (void)prebuck; (void)cache; (void)traversed; (void)real_dist; (void)verbose;
+        (void)s2; (void)prebuck; (void)cache; (void)traversed; (void)verbose;
         if (config.mc != 1 || nc != 2) {
             std::cerr << "This search method is for MultiRank IVFADC only" << std::endl;
             return;
         }
-        run_one(query, SC_KIND_SC, R, w, T, M, dist, result, sum);
+        run_one(query, SC_KIND_SC, R, w, T, M, dist, result, sum, real_dist);
     }
 
     // sc_query.h:265-475 (nc = 3) is not built in this library.

@@ -17,7 +17,7 @@ HEADER = os.path.join(os.path.dirname(HERE), "include", "sc_hip.h")
 SC_OK = 0
 KIND_IVFADC, KIND_IMI, KIND_SC = 0, 1, 2
 PREC_FP32, PREC_BF16 = 0, 1
-FORM_PLAIN, FORM_MIXED = 0, 1
+FORM_PLAIN, FORM_MIXED, FORM_REAL = 0, 1, 2
 
 
 class ScError(RuntimeError):
@@ -89,6 +89,7 @@ _SIGS = {
                                 C.c_int32, C.c_int32, _P, C.c_int32, C.c_int32, C.c_int32,
                                 C.POINTER(_P)]),
     "sc_index_destroy": (C.c_int, [_P]),
+    "sc_index_attach_raw": (C.c_int, [_P, _P, C.c_int64]),
     "sc_index_shape": (C.c_int, [_P, _P]),
     "sc_index_export": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P]),
     "sc_search": (C.c_int, [_P, _P, _P, C.c_int64, C.POINTER(SearchParams), _P, _P, _P,
@@ -249,6 +250,13 @@ class Index:
             ctx.h, L.ctypes.data, L.size, pid.ctypes.data, codes.ctypes.data, pid.size, dim,
             cq.ctypes.data, mc, kc, pq.ctypes.data, mp, kp, nc, C.byref(h)))
         return cls(ctx, h, dim, mc, kc, mp, kp, nc)
+
+    def attach_raw(self, raw):
+        """Raw base vectors (Dev [n][dim], by id) for FORM_REAL searches; the
+        index keeps a reference to `raw`."""
+        _check("sc_index_attach_raw", lib().sc_index_attach_raw(
+            self.h, raw.ptr if raw is not None else None, raw.shape[0] if raw is not None else 0))
+        self.raw = raw
 
     def shape(self):
         out = np.zeros(6, np.int64)

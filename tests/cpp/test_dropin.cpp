@@ -212,6 +212,26 @@ int main(int argc, char **argv) {
             EXPECT(std::memcmp(&bres[(size_t)i * R], oid.data(), 4 * (size_t)R) == 0);
         }
         EXPECT(q.get_full_size() == 32 && q.get_size() > 0 && q.entropy(32 * 32) > 0.0);
+
+        // real_dist = true: raw vectors loaded after the index (query.h:83-86)
+        q.load_data<float>((dir + "/base.fvecs").c_str(), 4, false);   // 4-byte row header
+        for (int i = 0; i < 8; ++i) {
+            int sum = 0;
+            q.search_mr_ivf(&xq[(size_t)i * dim], v_tmp, dist, it, result, h1, h2, h3, h4, s1, s2,
+                            pb, cache, trav, sum, R, w, T, kc, true, false);
+            std::vector<int32_t> oid(R);
+            std::vector<float> od(R);
+            sco_stats st;
+            EXPECT(sco_search_real(&o.s, SCO_SC, &xq[(size_t)i * dim], xb.data(), R, w, T,
+                                   oid.data(), od.data(), &st) == 0);
+            EXPECT(sum == st.sum);
+            for (int r = 0; r < (R < sum ? R : sum); ++r) {
+                EXPECT(result[r] == oid[r]);
+                EXPECT(std::memcmp(&dist[r], &od[r], 4) == 0);
+            }
+            ::operator delete(result);
+            ::operator delete(dist);
+        }
     }
 
     // ---------------- IMI: mc=2, kc=16 (MultiQuery::search_multi2)

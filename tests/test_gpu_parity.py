@@ -99,6 +99,32 @@ def _index_pair(ctx, kind, n=6000, dim=64, kc=16, mp=8, kp=64, seed=3):
 
 
 @pytest.mark.parametrize("kind", [S.KIND_IVFADC, S.KIND_IMI, S.KIND_SC])
+def test_search_real_distances_match_oracle(ctx, kind):
+    """SC_FORM_REAL (real_dist = true): candidates ranked by the exact S1
+    distance to the raw vectors, bit-exact ids and distances against the
+    oracle, including queries with fewer candidates than R."""
+    rng = np.random.default_rng(11)
+    ix, oix = _index_pair(ctx, kind)
+    n = oix.pid.size
+    raw = sco.gen(sco.mix_d128(ncomp=11), sco.STREAM_BASE, 0, n, ix.dim)
+    raw += rng.uniform(-0.5, 0.5, raw.shape).astype(np.float32)
+    draw = S.Dev.from_numpy(ctx, raw)
+    q = sco.gen(sco.mix_d128(ncomp=11), sco.STREAM_QUERY, 0, 24, ix.dim)
+    dq = S.Dev.from_numpy(ctx, q)
+    with pytest.raises(S.ScError):
+        ix.search(dq, kind, 50, 8, 800, form=S.FORM_REAL)   # no raw vectors yet
+    ix.attach_raw(draw)
+    for R, w, T in ((50, 8, 800), (400, 3, 150)):
+        ids, dists, sums, _ = ix.search(dq, kind, R, w, T, form=S.FORM_REAL)
+        gi, gd, gs = ids.numpy(), dists.numpy(), sums.numpy()
+        for i in range(q.shape[0]):
+            oi, od, ost = oix.search_real(kind, q[i], raw, R, w, T)
+            assert gs[i] == ost.sum
+            assert np.array_equal(gi[i], oi), (R, i)
+            assert np.array_equal(gd[i].view(np.uint32), od.view(np.uint32)), (R, i)
+
+
+@pytest.mark.parametrize("kind", [S.KIND_IVFADC, S.KIND_IMI, S.KIND_SC])
 def test_index_build_and_tables_match_oracle(ctx, kind):
     ix, oix = _index_pair(ctx, kind)
     e = ix.export()

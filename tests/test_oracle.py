@@ -410,6 +410,45 @@ def test_search_topr_is_canonical_selection():
             assert np.array_equal(dists, ex["all_dists"][sel])
 
 
+def _s1_sequential(q, y):
+    """S1 restated independently: float differences, squared in float64 and
+    summed in dimension order (cumsum is sequential), stored as float32."""
+    t = (q.astype(np.float32)[None, :] - y.astype(np.float32)).astype(np.float64)
+    return np.cumsum(t * t, axis=1)[:, -1].astype(np.float32)
+
+
+def test_search_real_distances():
+    """real_dist = true: the traced search's candidates, in traversal order,
+    ranked by their exact distance to the raw vectors; R above the candidate
+    count keeps traversal order; a query equal to a gathered vector finds it
+    at distance 0 first."""
+    rng = np.random.default_rng(5)
+    for kind in (sco.IVFADC, sco.IMI, sco.SC):
+        ix, x, _, _ = _make_index(kind)
+        raw = x + rng.uniform(-0.5, 0.5, x.shape).astype(np.float32)   # non-integer rows
+        q = sco.gen(sco.mix_d128(ncomp=9), sco.STREAM_QUERY, 0, 4, ix.dim)
+        for qi in range(4):
+            for R, T in ((30, 800), (4000, 300)):
+                w = min(ix.kc, 20)
+                ids, dists, st = ix.search_real(kind, q[qi], raw, R, w, T)
+                _, _, st2, ex = ix.search(kind, q[qi], R=R, w=w, T=T, trace=True)
+                cand = ex["all_ids"]
+                assert st.sum == st2.sum == cand.size
+                d = _s1_sequential(q[qi], raw[cand])
+                if cand.size >= R:
+                    sel = np.lexsort((np.arange(cand.size), d))[:R]
+                else:
+                    sel = np.arange(cand.size)
+                m = min(R, cand.size)
+                assert np.array_equal(ids[:m], cand[sel]) and np.array_equal(dists[:m], d[sel])
+                assert (ids[m:] == -1).all() and np.isinf(dists[m:]).all()
+        # a gathered vector used as the query is found first, at distance 0
+        _, _, _, ex = ix.search(kind, q[0], R=30, w=min(ix.kc, 20), T=800, trace=True)
+        v = int(ex["all_ids"][7])
+        ids, dists, _ = ix.search_real(kind, raw[v], raw, 30, min(ix.kc, 20), 800)
+        assert ids[0] == v and dists[0] == 0.0
+
+
 def test_exhaustive_ivf_is_full_adc_scan():
     """Config-1 mapping: kc=1 coarse cell, w=1, T >= N scans every vector."""
     rng = np.random.default_rng(8)
