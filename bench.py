@@ -54,6 +54,8 @@ def main():
                     help="T values of the QPS/recall curve ('' = skip)")
     ap.add_argument("--mixed", type=int, default=1,
                     help="1: also time the mixed-precision form (FP16 table + exact re-rank)")
+    ap.add_argument("--real", type=int, default=1,
+                    help="1: also time the real-distance form (exact distances from the raw vectors)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="only run a plain device copy of this many MiB (HBM counter scale)")
     args = ap.parse_args()
@@ -237,6 +239,40 @@ def main():
         del xh
         log("recall@%d on %d queries: %.3f" % (args.R, ns, recall))
 
+    # ---------------- real-distance form (real_dist = true): the candidates are
+    # ranked by their exact distance to the query, read from the raw vectors
+    real = None
+    if args.real:
+        ix.attach_raw(x)
+        rids = S.Dev(ctx, (args.nq, args.R), np.int32)
+        rd = S.Dev(ctx, (args.nq, args.R), np.float32)
+        rs = S.Dev(ctx, (args.nq,), np.int32)
+        for _ in range(max(1, args.warmup)):
+            ix.search(q, S.KIND_SC, args.R, args.w, args.T, form=S.FORM_REAL, ids=rids, dists=rd,
+                      sums=rs, stats=False)
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ix.search(q, S.KIND_SC, args.R, args.w, args.T, form=S.FORM_REAL, ids=rids, dists=rd,
+                      sums=rs, stats=False)
+        ctx.sync()
+        rel = time.perf_counter() - t0
+        rcand = float(rs.numpy().astype(np.int64).sum())
+        # per candidate: its raw row (dim floats) and its id
+        rbytes = rcand * (dim * 4 + 4)
+        real = {"qps": round(args.nq * args.steps / rel, 1),
+                "ms_per_step": round(rel * 1e3 / args.steps, 3),
+                "candidates_per_query": round(rcand / args.nq, 1),
+                "raw_gb_per_s": round(rbytes * args.steps / rel / 1e9, 1),
+                "recall_at_%d" % args.R: (round(sco.recall(gt, rids.numpy()[:ns], args.R), 4)
+                                          if recall is not None else None),
+                "launch": "sc_search calls (staged kernels)"}
+        log("real-distance form: %.1f ms/step, %.0f QPS, %.0f GB/s of raw rows, recall %s"
+            % (real["ms_per_step"], real["qps"], real["raw_gb_per_s"],
+               real["recall_at_%d" % args.R]))
+        ix.attach_raw(None)
+        del rids, rd, rs
+
     # ---------------- QPS / recall curve over T (same R, both forms, batched)
     curve = []
     if args.sweep and args.plan and args.recall_sample > 0:
@@ -323,6 +359,7 @@ def main():
             "recall_at_100": recall,
             "qps_plain_fp32_per_query_calls": round(qps_single, 1) if qps_single else None,
             "mixed_precision_batched": mixed,
+            "real_distance": real,
             "candidates_per_query": round(cand / args.nq, 1),
             "stage_ms": {"lut": round(st0.ms_lut, 4), "rank": round(st0.ms_rank, 4),
                          "traverse": round(st0.ms_traverse, 4), "scan_topk": round(ms_kernel, 4)},
