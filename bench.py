@@ -47,7 +47,9 @@ def main():
                     help="queries timed one call at a time (drop-in plain form)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="target CPU work for the oracle baseline (0 = skip)")
-    ap.add_argument("--recall-sample", type=int, default=100)
+    ap.add_argument("--recall-sample", type=int, default=100,
+                    help="queries whose GPU ground truth is checked against the oracle "
+                         "(0 = no recall)")
     ap.add_argument("--plan", type=int, default=1,
                     help="1: time the autotuned hipGraph plan (sc_plan_*); 0: sc_search calls")
     ap.add_argument("--sweep", type=str, default="1000,2000,5000,10000,20000",
@@ -227,17 +229,27 @@ def main():
         qps_single = pq_n / (time.perf_counter() - t0)
         log("per-query drop-in calls: %.0f QPS" % qps_single)
 
-    # ---------------- recall@100 on a sample (exact brute-force ground truth)
+    # ---------------- recall@100 over all queries: exact nearest neighbours on
+    # the GPU (sc_groundtruth_nn1), checked against the oracle's brute force on
+    # a sample of the queries
     import sco
     gids = ids.numpy()
     recall = None
+    gt_info = None
     if args.recall_sample > 0:
-        ns = min(args.recall_sample, args.nq)
+        ns = args.nq
+        t0 = time.perf_counter()
+        gt = S.groundtruth_nn1(ctx, x, q)[0].numpy()
+        t_gt = time.perf_counter() - t0
+        nchk = min(args.recall_sample, args.nq)
         xh = x.numpy()
-        gt = sco.bruteforce_nn1(xh, qh[:ns])
-        recall = sco.recall(gt, gids[:ns], args.R)
+        gt_ok = bool(np.array_equal(gt[:nchk], sco.bruteforce_nn1(xh, qh[:nchk])))
         del xh
-        log("recall@%d on %d queries: %.3f" % (args.R, ns, recall))
+        recall = sco.recall(gt, gids, args.R)
+        gt_info = {"queries": int(args.nq), "seconds": round(t_gt, 4),
+                   "oracle_checked_queries": int(nchk), "matches_oracle": gt_ok}
+        log("recall@%d on %d queries: %.4f (GPU ground truth in %.3fs, matches the oracle on "
+            "%d queries: %s)" % (args.R, ns, recall, t_gt, nchk, gt_ok))
 
     # ---------------- real-distance form (real_dist = true): the candidates are
     # ranked by their exact distance to the query, read from the raw vectors
@@ -357,6 +369,7 @@ def main():
                         % (args.n, args.nq, args.T, args.R),
             "form": "plain-fp32-batched",
             "recall_at_100": recall,
+            "ground_truth": gt_info,
             "qps_plain_fp32_per_query_calls": round(qps_single, 1) if qps_single else None,
             "mixed_precision_batched": mixed,
             "real_distance": real,

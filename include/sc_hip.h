@@ -219,6 +219,17 @@ int sc_kmeans(sc_ctx *ctx, const float *d_x, int64_t n, int32_t d, int32_t k,
 int sc_residual(sc_ctx *ctx, float *d_x, int64_t n, int32_t dim,
                 const float *d_cq, int32_t mc, int32_t kc, int32_t which);
 
+/* ---- evaluation ---------------------------------------------------------- */
+/* Nearest base vector of each query by the exact S1 distance, the lowest id
+ * on ties (the first strict minimum of a scan in id order): the first column
+ * of the ground-truth files Evaluation::load_groundtruth reads and
+ * calc_recall matches results against (include/evaluation.h:34-92; the
+ * reference ships those files, it does not compute them).  d_base [n][dim],
+ * d_q [nq][dim] (float); d_ids [nq]; d_dists [nq] (double, may be NULL).
+ * Synchronises the context stream. */
+int sc_groundtruth_nn1(sc_ctx *ctx, const float *d_base, int64_t n, const float *d_q,
+                       int64_t nq, int32_t dim, int32_t *d_ids, double *d_dists);
+
 #ifdef __cplusplus
 }
 #endif

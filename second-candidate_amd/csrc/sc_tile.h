@@ -46,10 +46,12 @@ __device__ __forceinline__ void push2(Best2 &b, double d, int k) {
 // Distances of the TV vectors held in xs (rows [t0, t0+blen) of the
 // transposed tile) to K centroids cb[K][blen] (global), reduced to the best
 // two (or best one) per vector under the tie order.  Results land in
-// out_k1/out_k2 (LDS, TV entries).  Must be called by all threads.
+// out_k1/out_k2 (LDS, TV entries) and, when out_d1 is given, the best
+// distance in out_d1.  Must be called by all threads.
 template <int TV, int TK, bool DESC, bool ROUNDF = false>
 __device__ void tile_best(const float *xs, int t0, int blen, const float *__restrict__ cb,
-                          int K, float *cs, Best2 *red, int *out_k1, int *out_k2) {
+                          int K, float *cs, Best2 *red, int *out_k1, int *out_k2,
+                          double *out_d1 = nullptr) {
     constexpr int NT = (TV / 4) * (TK / 4);
     const int tid = threadIdx.x;
     const int tv = tid % (TV / 4), tk = tid / (TV / 4);
@@ -113,6 +115,7 @@ __device__ void tile_best(const float *xs, int t0, int blen, const float *__rest
         }
         out_k1[tid] = m.k1;
         if (out_k2) out_k2[tid] = m.k2;
+        if (out_d1) out_d1[tid] = m.d1;
     }
     __syncthreads();
 }

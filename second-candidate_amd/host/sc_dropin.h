@@ -847,4 +847,65 @@ public:
     }
 };
 
+// include/evaluation.h:20-140
+class Evaluation {
+public:
+    // evaluation.h:34-47: the first of the d ints of each row (4-byte row
+    // header) of a ground-truth file; gt is callee-allocated (::operator new)
+    inline void load_groundtruth(const char *filename, int *&gt, int d, int &N, bool verbose) {
+        if (d <= 0 || !filename) detail::die("Defective data!");
+        int fd = open(filename, O_RDONLY);
+        if (fd < 0) detail::die("Cannot open the file");
+        const size_t size = detail::file_size(filename);
+        const size_t row = (size_t)d * 4 + 4;
+        N = (int)(size / row);
+        gt = static_cast<int *>(::operator new(sizeof(int) * ((size_t)N + 1)));
+        for (int i = 0; i < N; ++i) {
+            if (pread(fd, &gt[i], 4, (off_t)((size_t)i * row + 4)) != 4) detail::die("Cannot read the file");
+        }
+        close(fd);
+        if (verbose) std::cout << "The number of vectors: " << N << std::endl;
+    }
+    // evaluation.h:58-92: queries whose ground truth is among their R results
+    inline void calc_recall(int *result, int *groundtruth, int R, int N, int &recall,
+                            bool verbose) {
+        recall = 0;
+        for (int i = 0; i < N; ++i) {
+            const int *r = result + (size_t)i * R;
+            bool found = false;
+            for (int j = 0; j < R && !found; ++j) found = r[j] == groundtruth[i];
+            if (found) recall++;
+        }
+        if (verbose) std::cout << "Recall@" << R << " = " << recall << std::endl;
+    }
+    // evaluation.h:103-140: found in any of the S result sets
+    inline void calc_comb_recall(int **results, int *groundtruth, int R, int N, int S,
+                                 int &recall, bool verbose) {
+        recall = 0;
+        for (int i = 0; i < N; ++i) {
+            bool found = false;
+            for (int s = 0; s < S && !found; ++s)
+                for (int j = 0; j < R && !found; ++j)
+                    found = results[s][(size_t)i * R + j] == groundtruth[i];
+            if (found) recall++;
+        }
+        if (verbose) std::cout << "Recall@" << R << " = " << recall << std::endl;
+    }
+    // No reference counterpart (the reference reads ground truth from files):
+    // the nearest base vector of each query by the exact distance, lowest id
+    // on ties, computed on the GPU (sc_groundtruth_nn1).  gt is
+    // callee-allocated (::operator new).
+    inline void compute_groundtruth(const float *base, int n, const float *queries, int nq,
+                                    int dim, int *&gt) {
+        detail::Dev b(base, sizeof(float) * (size_t)n * dim);
+        detail::Dev q(queries, sizeof(float) * (size_t)nq * dim);
+        detail::Dev ids(sizeof(int32_t) * (size_t)(nq > 0 ? nq : 1));
+        detail::check(sc_groundtruth_nn1(detail::ctx(), b.as<float>(), n, q.as<float>(), nq, dim,
+                                         ids.as<int32_t>(), nullptr),
+                      "sc_groundtruth_nn1");
+        gt = static_cast<int *>(::operator new(sizeof(int) * ((size_t)nq + 1)));
+        ids.get(gt, sizeof(int32_t) * (size_t)nq);
+    }
+};
+
 }  // namespace SC

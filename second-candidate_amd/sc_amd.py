@@ -90,6 +90,7 @@ _SIGS = {
                                 C.POINTER(_P)]),
     "sc_index_destroy": (C.c_int, [_P]),
     "sc_index_attach_raw": (C.c_int, [_P, _P, C.c_int64]),
+    "sc_groundtruth_nn1": (C.c_int, [_P, _P, C.c_int64, _P, C.c_int64, C.c_int32, _P, _P]),
     "sc_index_shape": (C.c_int, [_P, _P]),
     "sc_index_export": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P]),
     "sc_search": (C.c_int, [_P, _P, _P, C.c_int64, C.POINTER(SearchParams), _P, _P, _P,
@@ -340,6 +341,17 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+
+def groundtruth_nn1(ctx, base, q):
+    """Nearest base vector of each query (Dev [nq] int32) by the exact S1
+    distance, lowest id on ties, and that distance (Dev [nq] float64)."""
+    nq, dim = q.shape
+    ids = Dev(ctx, (nq,), np.int32)
+    dists = Dev(ctx, (nq,), np.float64)
+    _check("sc_groundtruth_nn1", lib().sc_groundtruth_nn1(ctx.h, base.ptr, base.shape[0], q.ptr,
+                                                          nq, dim, ids.ptr, dists.ptr))
+    return ids, dists
 
 
 def kmeans(ctx, x, k, iters, seed, lagged=0, prec=PREC_FP32, want_labels=True):

@@ -276,6 +276,40 @@ int main(int argc, char **argv) {
         }
         EXPECT(t1 > 0 && t2 > 0);
     }
+    // ---------------- Evaluation (include/evaluation.h:34-140)
+    {
+        SC::Evaluation e;
+        int *gt = nullptr;
+        e.compute_groundtruth(xb.data(), 20000, xq.data(), 40, dim, gt);
+        std::vector<int32_t> ogt(40);
+        sco_bruteforce_nn1(xb.data(), 20000, xq.data(), 40, dim, ogt.data(), 8);
+        EXPECT(std::memcmp(gt, ogt.data(), 4 * 40) == 0);
+        // a ground-truth file of 5 ints per row, the first the neighbour
+        const std::string gfn = dir + "/gt.ivecs";
+        FILE *f = std::fopen(gfn.c_str(), "wb");
+        for (int i = 0; i < 40; ++i) {
+            int d5 = 5, row[5] = {ogt[i], 7, 8, 9, 10};
+            std::fwrite(&d5, 4, 1, f);
+            std::fwrite(row, 4, 5, f);
+        }
+        std::fclose(f);
+        int *lg = nullptr, N = 0;
+        e.load_groundtruth(gfn.c_str(), lg, 5, N, false);
+        EXPECT(N == 40 && std::memcmp(lg, ogt.data(), 4 * 40) == 0);
+        // results: the neighbour at rank 3 for even queries, absent for odd
+        const int R = 5;
+        std::vector<int> res(40 * R, -1), res2(40 * R, -1);
+        for (int i = 0; i < 40; i += 2) res[(size_t)i * R + 3] = gt[i];
+        for (int i = 1; i < 40; i += 4) res2[(size_t)i * R] = gt[i];
+        int rec = -1;
+        e.calc_recall(res.data(), gt, R, 40, rec, false);
+        EXPECT(rec == 20);
+        int *sets[2] = {res.data(), res2.data()};
+        e.calc_comb_recall(sets, gt, R, 40, 2, rec, false);
+        EXPECT(rec == 30);
+        ::operator delete(gt);
+        ::operator delete(lg);
+    }
     std::printf("%s: %d failure(s)\n", g_fail ? "FAILED" : "OK", g_fail);
     return g_fail ? 1 : 0;
 }

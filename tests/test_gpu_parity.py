@@ -124,6 +124,34 @@ def test_search_real_distances_match_oracle(ctx, kind):
             assert np.array_equal(gd[i].view(np.uint32), od.view(np.uint32)), (R, i)
 
 
+@pytest.mark.parametrize("dim,n,nq", [(128, 5000, 300), (128, 70, 1), (960, 1500, 37),
+                                      (48, 20000, 129)])
+def test_groundtruth_nn1_matches_oracle(ctx, dim, n, nq):
+    """sc_groundtruth_nn1 against the oracle's brute force: ids bit-exact
+    (duplicated rows resolve to the lower id), distances equal to the FP64
+    S1 value; a query equal to a base row finds it at distance 0."""
+    rng = np.random.default_rng(dim + n)
+    mix = sco.mix_d960(ncomp=8) if dim == 960 else sco.mix_d128(ncomp=8)
+    base = sco.gen(mix, sco.STREAM_BASE, 0, n, dim)
+    base += rng.uniform(-0.5, 0.5, base.shape).astype(np.float32)
+    q = sco.gen(mix, sco.STREAM_QUERY, 0, nq, dim)
+    if n > 100:
+        base[n - 1] = base[17]
+        base[n // 2] = base[n // 3]
+        q[0] = base[n - 1]                     # equal distance 0 at ids 17 and n-1
+        if nq > 1:
+            q[nq - 1] = base[n // 2]
+    ids, dists = S.groundtruth_nn1(ctx, S.Dev.from_numpy(ctx, base), S.Dev.from_numpy(ctx, q))
+    gi, gd = ids.numpy(), dists.numpy()
+    assert np.array_equal(gi, sco.bruteforce_nn1(base, q))
+    for i in range(nq):
+        assert gd[i] == sco.dist(q[i], base[gi[i]])
+    if n > 100:
+        assert gi[0] == 17 and gd[0] == 0.0
+        if nq > 1:
+            assert gi[nq - 1] == n // 3
+
+
 @pytest.mark.parametrize("kind", [S.KIND_IVFADC, S.KIND_IMI, S.KIND_SC])
 def test_index_build_and_tables_match_oracle(ctx, kind):
     ix, oix = _index_pair(ctx, kind)

@@ -449,6 +449,32 @@ def test_search_real_distances():
         assert ids[0] == v and dists[0] == 0.0
 
 
+def test_bruteforce_nn1_and_recall():
+    """Ground truth: the first strict minimum of the float64 S1 distance in id
+    order (duplicated rows resolve to the lower id; a query equal to a row
+    has distance 0); recall@R counts queries whose ground truth is among
+    their first R results."""
+    rng = np.random.default_rng(21)
+    base = sco.gen(sco.mix_d128(ncomp=6), sco.STREAM_BASE, 0, 900, 48)
+    base += rng.uniform(-0.5, 0.5, base.shape).astype(np.float32)
+    base[700] = base[123]                      # exact duplicate, later id
+    base[800] = base[50]
+    q = sco.gen(sco.mix_d128(ncomp=6), sco.STREAM_QUERY, 0, 40, 48)
+    q[3] = base[700]                           # ties at distance 0: 123 wins
+    q[4] = base[50]
+    gt = sco.bruteforce_nn1(base, q)
+    for i in range(q.shape[0]):
+        t = (q[i][None, :] - base).astype(np.float64)
+        d = np.cumsum(t * t, axis=1)[:, -1]
+        assert gt[i] == int(np.flatnonzero(d == d.min())[0])
+    assert gt[3] == 123 and gt[4] == 50
+    ids = np.tile(np.arange(10, dtype=np.int32), (40, 1))
+    ids[:, 2] = gt
+    ids[::2, 2] = -5                           # half the queries miss
+    assert sco.recall(gt, ids, 3) == 0.5
+    assert sco.recall(gt, ids, 2) == float(np.mean(gt < 2))
+
+
 def test_exhaustive_ivf_is_full_adc_scan():
     """Config-1 mapping: kc=1 coarse cell, w=1, T >= N scans every vector."""
     rng = np.random.default_rng(8)
