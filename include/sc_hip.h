@@ -38,6 +38,7 @@ extern "C" {
 #define SC_KIND_IVFADC 0  /* PQQuery::search_ivfadc   include/query.h:202-325 */
 #define SC_KIND_IMI 1     /* MultiQuery::search_multi2 include/multi_query.h:62-273 */
 #define SC_KIND_SC 2      /* SCQuery::search_mr_ivf    include/sc_query.h:60-256 */
+#define SC_KIND_SC3 3     /* SCQuery::search_mr_ivf3   include/sc_query.h:265-475 */
 
 /* ---- precision / forms ------------------------------------------------- */
 #define SC_PREC_FP32 0    /* the reference's arithmetic everywhere */
@@ -111,13 +112,27 @@ int sc_encode(sc_ctx *ctx, const float *d_x, int64_t n, int32_t dim,
               int32_t mp, int32_t kp, int32_t nc, int32_t prec,
               uint16_t *d_cid, uint8_t *d_codes, sc_encode_stats *st);
 
+/* SCEncoder::encode<T> with general nc (include/sc_encoder.h:205-332), nc in
+ * 1..3: every distance is stored in a float, the nc nearest coarse centroids
+ * of each block are taken by float distance with ties to the lower index
+ * (the reference's nth_element_id + sort_id, whose libstdc++ tie order this
+ * replaces), cid [n][mc][nc]; residual against the nearest; PQ argmin of
+ * float distances, the lower index on ties.  Exact (FP64 accumulation of
+ * the float differences' squares, then rounded to float). */
+int sc_encode_general(sc_ctx *ctx, const float *d_x, int64_t n, int32_t dim,
+                      const float *d_cq, int32_t mc, int32_t kc, const float *d_pq,
+                      int32_t mp, int32_t kp, int32_t nc, uint16_t *d_cid, uint8_t *d_codes);
+
 /* ---- index ------------------------------------------------------------- */
 /* Builds the device-resident inverted file from encoder outputs:
  * Encoder::distribution (src/encoder.cpp:170-205) / SCEncoder::distribution
  * (src/sc_encoder.cpp:17-81) + the loaders' flattening (src/query.cpp:82-165,
  * src/sc_query.cpp:36-123) + PQQuery::pre_compute1 (include/query.h:91-149).
  * The index copies the codebooks; d_cid/d_codes may be freed afterwards.
- * Dense cells: kc^mc (nc=1) or kc^2 (nc=2, mc=1). */
+ * Dense cells: kc^mc (nc=1), kc^2 (nc=2, mc=1) or kc^3 (nc=3, mc=1, kc <= 1024;
+ * d_cid in sc_encode_general's layout [n][3]; SCEncoder::distribution with
+ * the per-vector cid stride mc*nc that src/sc_encoder.cpp:41-59 gets wrong
+ * for nc != 2). */
 int sc_index_build(sc_ctx *ctx, const uint16_t *d_cid, const uint8_t *d_codes,
                    int64_t n, int32_t dim, const float *d_cq, int32_t mc,
                    int32_t kc, const float *d_pq, int32_t mp, int32_t kp,

@@ -5,8 +5,8 @@
 // (lib/simple-cluster/include/utilities.h:194-230), Encoder::encode<float>
 // (include/encoder.h:69-192), Encoder::distribution (src/encoder.cpp:170-205),
 // vector_base (include/sc_utilities.h:156-185) and the array heaps
-// pq_insert/pq_pop, pq_insert4/pq_pop4 (include/sc_utilities.h:229-300,
-// 469-585).  The reference's query classes and SCEncoder include <cblas.h>
+// pq_insert/pq_pop, pq_insert4/pq_pop4, pq_insert6/pq_pop6
+// (include/sc_utilities.h:229-300, 469-675).  The reference's query classes and SCEncoder include <cblas.h>
 // (OpenBLAS fork, not vendored) and are not built.
 //
 // Built by oracle/Makefile into oracle/_ref/libscref.so straight from the
@@ -105,6 +105,25 @@ int scref_heap1_pop(float *v, int32_t *id, int32_t *pos) {
   int r = SC::pq_pop(v, id, p, false);
   *pos = p;
   return r;
+}
+
+void scref_heap6_insert(float *v, int32_t *h1, int32_t *h2, int32_t *h3, int32_t *h4,
+                        int32_t *h5, int32_t *h6, float data, int a, int b, int c, int d,
+                        int e, int f, int32_t *pos, int cap) {
+  int p = *pos;
+  SC::pq_insert6(v, h1, h2, h3, h4, h5, h6, data, a, b, c, d, e, f, p, cap, false);
+  *pos = p;
+}
+
+int scref_heap6_pop(float *v, int32_t *h1, int32_t *h2, int32_t *h3, int32_t *h4,
+                    int32_t *h5, int32_t *h6, int32_t *out6, int32_t *pos) {
+  int p = *pos;
+  if (p <= 0) return 0;
+  int a, b, c, d, e, f;
+  SC::pq_pop6(v, h1, h2, h3, h4, h5, h6, a, b, c, d, e, f, p, false);
+  out6[0] = a; out6[1] = b; out6[2] = c; out6[3] = d; out6[4] = e; out6[5] = f;
+  *pos = p;
+  return 1;
 }
 
 }  // extern "C"

@@ -174,6 +174,64 @@ int sco_encode(const float *x, int64_t n, int32_t dim, const float *cq,
     return 0;
 }
 
+/* SCEncoder::encode, general nc (include/sc_encoder.h:205-332): each
+ * coarse distance is stored in a float (d_tmp[k], :276-277) and the nc
+ * nearest are taken by nth_element_id + sort_id (:281-282) -- here the nc
+ * smallest by (float dist ASC, index ASC), the canonical order for the
+ * libstdc++ tie order; cid [N][mc][nc] (:285-287); residual against the
+ * nearest; PQ argmin over float distances with strict `>` from FLT_MAX
+ * (:306-318), so the lowest index wins a tie. */
+int sco_encode_general(const float *x, int64_t n, int32_t dim, const float *cq,
+                       int32_t mc, int32_t kc, const float *pq, int32_t mp, int32_t kp,
+                       int32_t nc, uint16_t *cid, uint8_t *codes, int32_t nthreads) {
+    if (mc <= 0 || mp <= 0 || dim % mc || dim % mp || kc < 1 || kp < 1 || kp > 256 ||
+        kc > 65536 || nc < 1 || nc > 3 || nc > kc || dim > 4096)
+        return SCO_ERR_ARG;
+    const int32_t bsc = dim / mc, bsp = dim / mp;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(nthreads > 0 ? nthreads : 1)
+#endif
+    for (int64_t i = 0; i < n; ++i) {
+        float r[4096];
+        const float *v = x + (size_t)i * dim;
+        for (int32_t j = 0; j < mc; ++j) {
+            const float *xb = v + (size_t)j * bsc;
+            const float *cb = cq + (size_t)j * kc * bsc;
+            float bd[3];
+            int32_t bk[3];
+            int32_t nb = 0;
+            for (int32_t k = 0; k < kc; ++k) {
+                const float d = (float)sco_dist_l2sq(xb, cb + (size_t)k * bsc, bsc);
+                /* insert (d, k) into the sorted list of the nc best; k ascends, so
+                 * an equal distance goes after the ones already held */
+                int32_t p = nb;
+                while (p > 0 && d < bd[p - 1]) --p;
+                if (p >= nc) continue;
+                int32_t e = nb < nc ? nb : nc - 1;
+                for (; e > p; --e) { bd[e] = bd[e - 1]; bk[e] = bk[e - 1]; }
+                bd[p] = d;
+                bk[p] = k;
+                if (nb < nc) ++nb;
+            }
+            for (int32_t t = 0; t < nc; ++t)
+                cid[((size_t)i * mc + j) * nc + t] = (uint16_t)bk[t];
+            const float *c = cb + (size_t)bk[0] * bsc;
+            for (int32_t t = 0; t < bsc; ++t) r[j * bsc + t] = xb[t] - c[t];
+        }
+        for (int32_t s = 0; s < mp; ++s) {
+            const float *pb = pq + (size_t)s * kp * bsp;
+            float d = FLT_MAX;
+            int32_t mn = 0;
+            for (int32_t k = 0; k < kp; ++k) {
+                const float t = (float)sco_dist_l2sq(r + (size_t)s * bsp, pb + (size_t)k * bsp, bsp);
+                if (d > t) { d = t; mn = k; }
+            }
+            codes[(size_t)i * mp + s] = (uint8_t)mn;
+        }
+    }
+    return 0;
+}
+
 /* S4: IMI key vector_base(cid, mc, kc) (include/sc_utilities.h:156-166,
  * src/encoder.cpp:180); SC key uid_j = nearest_j*kc + second_j,
  * key = sum uid_j (kc^2)^(mc-1-j) (src/sc_encoder.cpp:39-50), in 64 bits. */
@@ -183,6 +241,15 @@ void sco_cell_keys(const uint16_t *cid, int64_t n, int32_t mc, int32_t kc,
         uint64_t key = 0;
         if (nc == 1) {
             for (int32_t j = 0; j < mc; ++j) key = key * (uint64_t)kc + cid[(size_t)i * mc + j];
+        } else if (nc == 3) {
+            /* general encoder layout [N][mc][3]; uid_j = c0 kc^2 + c1 kc + c2
+             * (src/sc_encoder.cpp:39-45 with the per-vector stride mc*nc) */
+            const uint16_t *c = cid + (size_t)i * 3 * mc;
+            for (int32_t j = 0; j < mc; ++j) {
+                uint64_t uid = ((uint64_t)c[3 * j] * (uint64_t)kc + c[3 * j + 1]) * (uint64_t)kc +
+                               c[3 * j + 2];
+                key = key * (uint64_t)kc * (uint64_t)kc * (uint64_t)kc + uid;
+            }
         } else {
             const uint16_t *c = cid + (size_t)i * 2 * mc;
             for (int32_t j = 0; j < mc; ++j) {
@@ -403,6 +470,8 @@ int sco_search(const sco_index *ix, int32_t kind, const float *q, int32_t R,
     if (kind == SCO_IVFADC && mc != 1) return SCO_ERR_CONFIG;
     if (kind == SCO_IMI && mc != 2) return SCO_ERR_CONFIG;
     if (kind == SCO_SC && mc != 1) return SCO_ERR_CONFIG;
+    if (kind == SCO_SC3 && (mc != 1 || kc > 1024 || ix->ncells < (int64_t)kc * kc * kc))
+        return SCO_ERR_CONFIG;
     if (kind == SCO_IVFADC && w > kc) return SCO_ERR_ARG; /* pq_pop on empty -> exit */
 
     float *diff_qc = (float *)malloc(sizeof(float) * (size_t)mc * kc);
@@ -417,7 +486,8 @@ int sco_search(const sco_index *ix, int32_t kind, const float *q, int32_t R,
     int32_t *s1 = (int32_t *)malloc(sizeof(int32_t) * (size_t)w);
     int32_t *s2 = (int32_t *)malloc(sizeof(int32_t) * (size_t)w);
     unsigned char *trav = NULL;
-    if (kind != SCO_IVFADC) trav = (unsigned char *)calloc((size_t)kc * kc, 1);
+    if (kind != SCO_IVFADC)
+        trav = (unsigned char *)calloc((size_t)kc * kc * (kind == SCO_SC3 ? (size_t)kc : 1), 1);
     int rc = 0;
     if (!diff_qc || !diff_qr || !v || !tmp || !stmp || !hv || !hb || !prebuck ||
         !s1 || !s2 || (kind != SCO_IVFADC && !trav)) { rc = SCO_ERR_NOMEM; goto done; }
@@ -486,6 +556,40 @@ int sco_search(const sco_index *ix, int32_t kind, const float *q, int32_t R,
                     sco_heap4_insert(&h, v2[h1 + 1] + v3[h2], h1 + 1, h2, t1[h1 + 1], t2[h2]);
                 if (h2 < kc - 1 && (h1 == 0 || trav[i - kc + 1]))
                     sco_heap4_insert(&h, v2[h1] + v3[h2 + 1], h1, h2 + 1, t1[h1], t2[h2 + 1]);
+            }
+        }
+    } else if (kind == SCO_SC3) {
+        /* sc_query.h:296-402: (nearest, second, third) cells over one coarse
+         * list; the heap holds rank triples, capacity w */
+        for (int32_t j = 0; j < kc; ++j) { v[j] = q_sum + diff_qc[j]; tmp[j] = j; }
+        sort_scores(v, tmp, kc, stmp);
+        const int64_t k1 = kc, k2 = (int64_t)kc * kc, size2 = k2 * kc;
+        sco_heap4 h = {hv, hb, hb + hcap, hb + 2 * hcap, hb + 3 * hcap, 0, w};
+        sco_heap4_insert(&h, 3.0f * v[0], 0, 0, 0, 0);
+        int32_t h1 = 0, h2 = 0, h3 = 0, unused = 0;
+        int64_t i = 0;
+        while (count < w && sum < T && count2 < size2) {
+            if (h.pos > 0) {
+                sco_heap4_pop(&h, &h1, &h2, &h3, &unused);
+                int64_t bid = (int64_t)tmp[h1] * k2 + (int64_t)tmp[h2] * k1 + tmp[h3];
+                int32_t l = bid > 0 ? ix->L[bid] - ix->L[bid - 1] : 0;
+                if (l > 0) {
+                    prebuck[count] = (int32_t)bid; s1[count] = tmp[h1]; s2[count] = 0;
+                    count++;
+                    sum += l;
+                } else e++;
+                i = k2 * h1 + k1 * h2 + h3;
+                trav[i] = 1;
+                count2++;
+            } else { exhausted = 1; break; }
+            if (sum < T) {
+                /* a triple enters once its other predecessors have been popped */
+                if (h1 < kc - 1 && (h2 == 0 || trav[i + k2 - k1]) && (h3 == 0 || trav[i + k2 - 1]))
+                    sco_heap4_insert(&h, v[h1 + 1] + v[h2] + v[h3], h1 + 1, h2, h3, 0);
+                if (h2 < kc - 1 && (h3 == 0 || trav[i + k1 - 1]) && (h1 == 0 || trav[i + k1 - k2]))
+                    sco_heap4_insert(&h, v[h1] + v[h2 + 1] + v[h3], h1, h2 + 1, h3, 0);
+                if (h3 < kc - 1 && (h2 == 0 || trav[i - k1 + 1]) && (h1 == 0 || trav[i - k2 + 1]))
+                    sco_heap4_insert(&h, v[h1] + v[h2] + v[h3 + 1], h1, h2, h3 + 1, 0);
             }
         }
     } else {

@@ -58,7 +58,15 @@ int sco_encode(const float *x, int64_t n, int32_t dim,
                const float *pq, int32_t mp, int32_t kp, int32_t nc,
                uint16_t *cid, uint8_t *codes, int32_t nthreads);
 
-/* Cell key per vector (64-bit, S4). */
+/* SCEncoder::encode with general nc (1..3): float-rounded distances, the nc
+ * nearest by (float dist ASC, index ASC); cid [N][mc][nc]. */
+int sco_encode_general(const float *x, int64_t n, int32_t dim,
+                       const float *cq, int32_t mc, int32_t kc,
+                       const float *pq, int32_t mp, int32_t kp, int32_t nc,
+                       uint16_t *cid, uint8_t *codes, int32_t nthreads);
+
+/* Cell key per vector (64-bit, S4); nc = 3 reads the general encoder's
+ * layout [N][mc][3]. */
 void sco_cell_keys(const uint16_t *cid, int64_t n, int32_t mc, int32_t kc,
                    int32_t nc, uint64_t *keys);
 
@@ -82,7 +90,7 @@ void sco_lut(const float *q, const float *cq, const float *pq,
 typedef struct {
     int32_t dim, mc, kc, mp, kp;
     const float *cq, *pq, *norm_c, *norm_r, *dot_cr;
-    int64_t ncells;      /* dense cells: kc^mc (IVF/IMI) or kc^2 (SC) */
+    int64_t ncells;      /* dense cells: kc^mc (IVF/IMI), kc^2 (SC) or kc^3 (SC3) */
     const int32_t *L;    /* inclusive prefix, [ncells] */
     const int32_t *pid;  /* [N] */
     const uint8_t *codes;/* [N][mp], cell order */
@@ -100,6 +108,7 @@ typedef struct {
 #define SCO_IVFADC 0  /* PQQuery::search_ivfadc   (include/query.h:202-325) */
 #define SCO_IMI    1  /* MultiQuery::search_multi2 (include/multi_query.h:62-273) */
 #define SCO_SC     2  /* SCQuery::search_mr_ivf    (include/sc_query.h:60-256) */
+#define SCO_SC3    3  /* SCQuery::search_mr_ivf3   (include/sc_query.h:265-475) */
 
 /* One query.  ids/dists: [R] (padded with -1 / +inf when sum < R, in which
  * case the first `sum` entries keep traversal order like the reference).

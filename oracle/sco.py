@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 REF_PATH = os.path.join(HERE, "_ref", "libscref.so")
 
-IVFADC, IMI, SC = 0, 1, 2
+IVFADC, IMI, SC, SC3 = 0, 1, 2, 3
 
 _f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
 _i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
@@ -65,6 +65,7 @@ def _load():
     lib.sco_encode.argtypes = [_f32p, C.c_int64, C.c_int32, _f32p, C.c_int32,
                                C.c_int32, _f32p, C.c_int32, C.c_int32, C.c_int32,
                                _u16p, _u8p, C.c_int32]
+    lib.sco_encode_general.argtypes = lib.sco_encode.argtypes
     lib.sco_cell_keys.argtypes = [_u16p, C.c_int64, C.c_int32, C.c_int32,
                                   C.c_int32, _u64p]
     lib.sco_build_dense.argtypes = [_u64p, C.c_int64, C.c_int64, _u8p, C.c_int32,
@@ -157,6 +158,20 @@ def encode(x, cq, pq, mc, kc, mp, kp, nc):
                           codes, nthreads())
     if rc:
         raise ValueError("sco_encode rc=%d" % rc)
+    return cid, codes
+
+
+def encode_general(x, cq, pq, mc, kc, mp, kp, nc):
+    """SCEncoder::encode (general nc): cid [n][mc*nc] (block-major)."""
+    x = np.ascontiguousarray(x, np.float32)
+    n, dim = x.shape
+    cid = np.zeros((n, mc * nc), np.uint16)
+    codes = np.zeros((n, mp), np.uint8)
+    rc = lib().sco_encode_general(x, n, dim, np.ascontiguousarray(cq, np.float32), mc, kc,
+                                  np.ascontiguousarray(pq, np.float32), mp, kp, nc, cid,
+                                  codes, nthreads())
+    if rc:
+        raise ValueError("sco_encode_general rc=%d" % rc)
     return cid, codes
 
 
@@ -340,6 +355,9 @@ def ref():
         r.scref_heap1_insert.argtypes = [_f32p, _i32p, C.c_float, C.c_int,
                                          C.POINTER(C.c_int32), C.c_int]
         r.scref_heap1_pop.argtypes = [_f32p, _i32p, C.POINTER(C.c_int32)]
+        r.scref_heap6_insert.argtypes = [_f32p] + [_i32p] * 6 + [C.c_float] + [C.c_int] * 6 + \
+            [C.POINTER(C.c_int32), C.c_int]
+        r.scref_heap6_pop.argtypes = [_f32p] + [_i32p] * 6 + [_i32p, C.POINTER(C.c_int32)]
         _REF = r
     return _REF
 

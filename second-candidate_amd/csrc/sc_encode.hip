@@ -100,7 +100,99 @@ int encode_exact(sc_ctx *ctx, const float *x, int64_t n, int dim, const float *c
     return launch_exact<NC, 16, 256>(ctx, x, n, dim, cq, mc, kc, pq, mp, kp, cid, codes);
 }
 
+// SCEncoder::encode with general nc (include/sc_encoder.h:205-332): the NC
+// nearest coarse centroids of each block by float-rounded distance (ties to
+// the lower index), cid [n][mc][NC], residual against the nearest, PQ
+// argmin of float-rounded distances.
+template <int NC, int TV, int TK>
+__global__ __launch_bounds__((TV / 4) * (TK / 4)) void k_encode_general(
+    const float *__restrict__ x, int64_t n, int dim, const float *__restrict__ cq, int mc,
+    int kc, const float *__restrict__ pq, int mp, int kp, uint16_t *__restrict__ cid,
+    uint8_t *__restrict__ codes) {
+    constexpr int NT = (TV / 4) * (TK / 4);
+    extern __shared__ __align__(16) unsigned char smem[];
+    float *xs = reinterpret_cast<float *>(smem);                       // [dim][TV]
+    float *cs = xs + (size_t)dim * TV;                                  // [kTT][TK]
+    TopF<NC> *red = reinterpret_cast<TopF<NC> *>(cs + kTT * TK);        // [TK/4][TV]
+    int *kb = reinterpret_cast<int *>(red + (TK / 4) * TV);             // [TV][NC]
+    TopF<1> *red1 = reinterpret_cast<TopF<1> *>(red);
+
+    const int tid = threadIdx.x;
+    const int64_t v0 = (int64_t)blockIdx.x * TV;
+    const int nv = (int)min((int64_t)TV, n - v0);
+    for (int e = tid; e < TV * dim; e += NT) {
+        int v = e / dim, t = e % dim;
+        xs[(size_t)t * TV + v] = v < nv ? x[(size_t)(v0 + v) * dim + t] : 0.0f;
+    }
+    __syncthreads();
+
+    const int bsc = dim / mc, bsp = dim / mp;
+    for (int j = 0; j < mc; ++j) {
+        const float *cb = cq + (size_t)j * kc * bsc;
+        tile_top<TV, TK, NC>(xs, j * bsc, bsc, cb, kc, cs, red, kb);
+        if (tid < nv)
+            for (int t = 0; t < NC; ++t)
+                cid[((size_t)(v0 + tid) * mc + j) * NC + t] = (uint16_t)kb[tid * NC + t];
+        for (int e = tid; e < TV * bsc; e += NT) {
+            int v = e % TV, t = e / TV;
+            float *p = xs + (size_t)(j * bsc + t) * TV + v;
+            *p = *p - cb[(size_t)max(kb[v * NC], 0) * bsc + t];
+        }
+        __syncthreads();
+    }
+    for (int s = 0; s < mp; ++s) {
+        tile_top<TV, TK, 1>(xs, s * bsp, bsp, pq + (size_t)s * kp * bsp, kp, cs, red1, kb);
+        if (tid < nv) codes[(size_t)(v0 + tid) * mp + s] = (uint8_t)max(kb[tid], 0);
+        __syncthreads();
+    }
+}
+
+template <int NC, int TV, int TK>
+int launch_general(sc_ctx *ctx, const float *x, int64_t n, int dim, const float *cq, int mc,
+                   int kc, const float *pq, int mp, int kp, uint16_t *cid, uint8_t *codes) {
+    constexpr int NT = (TV / 4) * (TK / 4);
+    size_t lds = (size_t)dim * TV * 4 + kTT * TK * 4 + (TK / 4) * TV * sizeof(TopF<NC>) +
+                 (size_t)TV * NC * 4;
+    if (lds > 160 * 1024) {
+        sc_set_error("sc_encode_general: dim %d too large for the exact tile", dim);
+        return SC_ERR_ARG;
+    }
+    auto kern = k_encode_general<NC, TV, TK>;
+    SC_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)lds));
+    hipLaunchKernelGGL(kern, dim3((unsigned)sc_cdiv(n, TV)), dim3(NT), lds, ctx->stream, x, n, dim,
+                       cq, mc, kc, pq, mp, kp, cid, codes);
+    SC_CHECK_LAUNCH();
+    return SC_OK;
+}
+
+template <int NC>
+int encode_general(sc_ctx *ctx, const float *x, int64_t n, int dim, const float *cq, int mc,
+                   int kc, const float *pq, int mp, int kp, uint16_t *cid, uint8_t *codes) {
+    if (dim <= 256)
+        return launch_general<NC, 64, 64>(ctx, x, n, dim, cq, mc, kc, pq, mp, kp, cid, codes);
+    return launch_general<NC, 16, 256>(ctx, x, n, dim, cq, mc, kc, pq, mp, kp, cid, codes);
+}
+
 }  // namespace
+
+extern "C" int sc_encode_general(sc_ctx *ctx, const float *d_x, int64_t n, int32_t dim,
+                                 const float *d_cq, int32_t mc, int32_t kc, const float *d_pq,
+                                 int32_t mp, int32_t kp, int32_t nc, uint16_t *d_cid,
+                                 uint8_t *d_codes) {
+    SC_TRY(sc_ctx_enter(ctx));
+    if (!d_x || !d_cq || !d_pq || !d_cid || !d_codes || n < 0 || dim <= 0 || mc <= 0 ||
+        mp <= 0 || dim % mc || dim % mp || kc < 1 || kc > 65536 || kp < 1 || kp > 256 ||
+        nc < 1 || nc > 3 || nc > kc || dim > 4096) {
+        sc_set_error("sc_encode_general: bad arguments (dim=%d mc=%d kc=%d mp=%d kp=%d nc=%d)",
+                     dim, mc, kc, mp, kp, nc);
+        return SC_ERR_ARG;
+    }
+    if (n == 0) return SC_OK;
+    if (nc == 3) return encode_general<3>(ctx, d_x, n, dim, d_cq, mc, kc, d_pq, mp, kp, d_cid, d_codes);
+    if (nc == 2) return encode_general<2>(ctx, d_x, n, dim, d_cq, mc, kc, d_pq, mp, kp, d_cid, d_codes);
+    return encode_general<1>(ctx, d_x, n, dim, d_cq, mc, kc, d_pq, mp, kp, d_cid, d_codes);
+}
 
 int sc_encode_bf16(sc_ctx *ctx, const float *x, int64_t n, int dim, const float *cq, int mc,
                    int kc, const float *pq, int mp, int kp, int nc, uint16_t *cid,

@@ -24,6 +24,12 @@ __global__ void k_keys(const uint16_t *__restrict__ cid, int64_t n, int mc, int 
     uint64_t key = 0;
     if (nc == 1) {
         for (int j = 0; j < mc; ++j) key = key * (uint64_t)kc + cid[(size_t)i * mc + j];
+    } else if (nc == 3) {
+        // SCEncoder::encode layout [n][mc][3]; uid = c0 kc^2 + c1 kc + c2
+        const uint16_t *c = cid + (size_t)i * 3 * mc;
+        for (int j = 0; j < mc; ++j)
+            key = key * (uint64_t)kc * (uint64_t)kc * (uint64_t)kc +
+                  (((uint64_t)c[3 * j] * (uint64_t)kc + c[3 * j + 1]) * (uint64_t)kc + c[3 * j + 2]);
     } else {
         const uint16_t *c = cid + (size_t)i * 2 * mc;
         for (int j = 0; j < mc; ++j)
@@ -144,7 +150,8 @@ int alloc_index(sc_ctx *ctx, int64_t n, int32_t dim, int32_t mc, int32_t kc, int
 int check_shape(int64_t n, int32_t dim, int32_t mc, int32_t kc, int32_t mp, int32_t kp,
                 int32_t nc, int64_t *ncells) {
     if (n < 0 || n >= INT32_MAX || dim <= 0 || mc <= 0 || mp <= 0 || dim % mc || dim % mp ||
-        mp % mc || kc < 1 || kc > 65536 || kp < 1 || kp > 256 || (nc != 1 && nc != 2)) {
+        mp % mc || kc < 1 || kc > 65536 || kp < 1 || kp > 256 || nc < 1 || nc > 3 ||
+        (nc == 3 && (mc != 1 || kc > 1024))) {
         sc_set_error("index: bad shape (n=%lld dim=%d mc=%d kc=%d mp=%d kp=%d nc=%d)",
                      (long long)n, dim, mc, kc, mp, kp, nc);
         return SC_ERR_ARG;

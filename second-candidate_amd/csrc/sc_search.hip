@@ -388,6 +388,93 @@ __global__ void k_traverse(TravArgs a, int hcap, int qpw) {
     qi[5] = h.overflow ? 1 : 0; qi[6] = err; qi[7] = 0;
 }
 
+// Three-candidate traversal (SCQuery::search_mr_ivf3, include/sc_query.h:296-402):
+// rank triples (h1, h2, h3) of one coarse list, cell (t[h1], t[h2], t[h3]),
+// heap of the reference's capacity w in global memory, one lane per query.
+// A triple enters the heap once its other predecessors have been popped, so
+// the popped set is a down-set of the rank grid and the reference's kc^3
+// `traversed` flags reduce to a depth per (h1, h2) column: (h1, h2, h3) has
+// been popped iff h3 < depth[h1][h2].
+struct Trav3Args {
+    int kc, w, T, ccap;
+    const float *sv;
+    const int32_t *si;
+    const int32_t *L;
+    int32_t *cells;         // [nq][ccap] cell starts
+    int32_t *clen;          // [nq][ccap] cell lengths
+    uint32_t *cinfo;        // [nq][ccap] nearest centroid of the cell
+    int32_t *qinfo;         // [nq][8]
+    int64_t q0, nq;         // the queries [q0, q0 + nq) of this launch
+    float *hv;              // [nq][w] heap scores
+    uint32_t *hp;           // [nq][w] heap triples (10 bits per rank)
+    uint16_t *depth;        // [nq][kc][kc], zero on entry
+};
+
+__global__ void k_traverse3(Trav3Args a) {
+    const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= a.nq) return;
+    const int64_t q = a.q0 + slot;
+    const int kc = a.kc;
+    const float *v = a.sv + (size_t)q * kc;
+    const int32_t *t = a.si + (size_t)q * kc;
+    uint16_t *D = a.depth + (size_t)slot * kc * kc;
+    auto popped = [&](int x, int y, int z) { return z < (int)D[(size_t)x * kc + y]; };
+    auto pack = [](int x, int y, int z) {
+        return (uint32_t)x | ((uint32_t)y << 10) | ((uint32_t)z << 20);
+    };
+    Heap h;
+    h.v = a.hv + (size_t)slot * a.w;
+    h.p = a.hp + (size_t)slot * a.w;
+    h.stride = 1;
+    h.pos = 0;
+    h.cap_ref = a.w;
+    h.cap_phys = a.w;
+    h.overflow = false;
+    int32_t *cells = a.cells + (size_t)q * a.ccap;
+    int32_t *clen = a.clen + (size_t)q * a.ccap;
+    uint32_t *cinfo = a.cinfo + (size_t)q * a.ccap;
+    const int64_t grid = (int64_t)kc * kc * kc;
+    int count = 0, sum = 0, empty = 0, exhausted = 0, err = 0;
+    int64_t count2 = 0;
+    h.insert(3.0f * v[0], pack(0, 0, 0));
+    while (count < a.w && sum < a.T && count2 < grid) {
+        if (h.pos <= 0) { exhausted = 1; break; }
+        const uint32_t pk = h.pop();
+        const int h1 = (int)(pk & 1023u), h2 = (int)((pk >> 10) & 1023u), h3 = (int)(pk >> 20);
+        const int c1 = t[h1];
+        const int64_t bid = ((int64_t)c1 * kc + t[h2]) * kc + t[h3];
+        const int l = bid > 0 ? cell_len(a.L, bid) : 0;
+        if (l > 0) {
+            if (count >= a.ccap) { err = 2; break; }
+            cells[count] = cell_start(a.L, bid);
+            clen[count] = l;
+            cinfo[count] = (uint32_t)c1;
+            count++;
+            sum += l;
+        } else {
+            empty++;
+        }
+        uint16_t &dd = D[(size_t)h1 * kc + h2];
+        if ((int)dd != h3) err = 3;                  // not a down-set: cannot happen
+        dd = (uint16_t)(h3 + 1);
+        count2++;
+        if (sum < a.T) {
+            if (h1 < kc - 1 && (h2 == 0 || popped(h1 + 1, h2 - 1, h3)) &&
+                (h3 == 0 || popped(h1 + 1, h2, h3 - 1)))
+                h.insert((v[h1 + 1] + v[h2]) + v[h3], pack(h1 + 1, h2, h3));
+            if (h2 < kc - 1 && (h3 == 0 || popped(h1, h2 + 1, h3 - 1)) &&
+                (h1 == 0 || popped(h1 - 1, h2 + 1, h3)))
+                h.insert((v[h1] + v[h2 + 1]) + v[h3], pack(h1, h2 + 1, h3));
+            if (h3 < kc - 1 && (h2 == 0 || popped(h1, h2 - 1, h3 + 1)) &&
+                (h1 == 0 || popped(h1 - 1, h2, h3 + 1)))
+                h.insert((v[h1] + v[h2]) + v[h3 + 1], pack(h1, h2, h3 + 1));
+        }
+    }
+    int32_t *qi = a.qinfo + (size_t)q * 8;
+    qi[0] = count; qi[1] = sum; qi[2] = (int32_t)count2; qi[3] = empty; qi[4] = exhausted;
+    qi[5] = 0; qi[6] = err; qi[7] = 0;
+}
+
 // ----------------------------------------------------------------------------
 // Stage 4: scan + top-R.  One workgroup per query.
 // Candidates are keyed (orderable dist bits << 32 | position); keys are unique
@@ -2134,7 +2221,9 @@ extern "C" int sc_search(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
     const int kind = p->kind;
     if ((kind == SC_KIND_IVFADC && (ix->mc != 1 || ix->nc != 1)) ||
         (kind == SC_KIND_IMI && (ix->mc != 2 || ix->nc != 1)) ||
-        (kind == SC_KIND_SC && (ix->mc != 1 || ix->nc != 2)) || kind < 0 || kind > 2) {
+        (kind == SC_KIND_SC && (ix->mc != 1 || ix->nc != 2)) ||
+        (kind == SC_KIND_SC3 && (ix->mc != 1 || ix->nc != 3 || ix->kc > 1024)) || kind < 0 ||
+        kind > 3) {
         sc_set_error("sc_search: kind %d does not match the index (mc=%d nc=%d)", kind, ix->mc,
                      ix->nc);
         return SC_ERR_CONFIG;
@@ -2144,7 +2233,8 @@ extern "C" int sc_search(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
                      p->w, ix->kc);
         return SC_ERR_HEAP_EMPTY;
     }
-    if (kind == SC_KIND_SC && ix->kc < 2) return SC_ERR_ARG;
+    if ((kind == SC_KIND_SC && ix->kc < 2) || (kind == SC_KIND_SC3 && ix->kc < 3))
+        return SC_ERR_ARG;
     if (ix->kc > 8192 || ix->mp * ix->kp > 8192) {
         sc_set_error("sc_search: kc=%d / mp*kp=%d exceed the on-chip ranking limits", ix->kc,
                      ix->mp * ix->kp);
@@ -2158,7 +2248,8 @@ extern "C" int sc_search(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
     if (nq == 0) return SC_OK;
     // real distances run on the staged kernels: the raw vectors are read per
     // candidate, so the scan is bound by those reads, not by the on-chip stages
-    if (kind != SC_KIND_IMI && p->form != SC_FORM_REAL && !getenv("SC_NO_FUSED")) {
+    if (kind != SC_KIND_IMI && kind != SC_KIND_SC3 && p->form != SC_FORM_REAL &&
+        !getenv("SC_NO_FUSED")) {
         int rc = search_fused(ctx, ix, d_q, nq, p, d_ids, d_dists, d_sums, st);
         if (rc != SC_ERR_TOO_LARGE) return rc;      // otherwise: staged kernels
         if (st) *st = sc_search_stats{};
@@ -2166,10 +2257,48 @@ extern "C" int sc_search(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
     return search_staged(ctx, ix, d_q, nq, p, d_ids, d_dists, d_sums, st);
 }
 
+// The three-candidate traversal of a chunk's cn queries, in groups whose
+// global heaps (the reference's capacity w) and depth maps fit 1 GiB.
+static int traverse3(sc_ctx *ctx, const sc_search_params *p, int kc, const float *sv,
+                     const int32_t *si, const int32_t *L, int32_t *cells, int32_t *clen,
+                     uint32_t *cinfo, int32_t *qinfo, int ccap, int64_t cn) {
+    const size_t per = (size_t)p->w * 8 + (size_t)kc * kc * 2;
+    const int64_t group =
+        std::max<int64_t>(1, std::min<int64_t>(cn, (int64_t)(((size_t)1 << 30) / per)));
+    void *g = nullptr;
+    if (hipMalloc(&g, per * (size_t)group + 256) != hipSuccess) {
+        sc_set_error("sc_search: traversal heaps (%zu bytes) could not be allocated",
+                     per * (size_t)group);
+        return SC_ERR_NOMEM;
+    }
+    float *hv = (float *)g;
+    uint32_t *hp = (uint32_t *)(hv + (size_t)group * p->w);
+    uint16_t *depth = (uint16_t *)(hp + (size_t)group * p->w);
+    int rc = SC_OK;
+    for (int64_t g0 = 0; g0 < cn && rc == SC_OK; g0 += group) {
+        const int64_t gn = std::min<int64_t>(group, cn - g0);
+        if (hipMemsetAsync(depth, 0, (size_t)gn * kc * kc * 2, ctx->stream) != hipSuccess) {
+            rc = SC_ERR_HIP;
+            break;
+        }
+        Trav3Args ta{kc, p->w, p->T, ccap, sv, si, L, cells, clen, cinfo, qinfo, g0, gn, hv, hp,
+                     depth};
+        hipLaunchKernelGGL(k_traverse3, dim3((unsigned)sc_cdiv(gn, 64)), dim3(64), 0, ctx->stream,
+                           ta);
+        if (hipGetLastError() != hipSuccess) rc = SC_ERR_HIP;
+    }
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(g);
+    if (rc != SC_OK) sc_set_error("sc_search: three-candidate traversal launch failed");
+    return rc;
+}
+
 static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int64_t nq,
                          const sc_search_params *p, int32_t *d_ids, float *d_dists,
                          int32_t *d_sums, sc_search_stats *st) {
     const int kind = p->kind;
+    const bool three = kind == SC_KIND_SC3;
+    const int scan_kind = three ? SC_KIND_SC : kind;   // same ranking and scan as SC
     const Plan P = make_plan(ix, p, nq);
     unsigned char *s;
     SC_TRY(sc_scratch(ctx, P.total, (void **)&s));
@@ -2218,19 +2347,21 @@ static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
         SC_CHECK_LAUNCH();
         mark();
         hipLaunchKernelGGL(k_rank, dim3((unsigned)cn), dim3(256), (size_t)npow2 * 8, ctx->stream,
-                           Q, ix->dim, kind, ix->mc, kc, dqc, qs, sv, si, npow2);
+                           Q, ix->dim, scan_kind, ix->mc, kc, dqc, qs, sv, si, npow2);
         SC_CHECK_LAUNCH();
         mark();
         TravArgs ta{kind, kc, p->w, p->T, P.ccap, sv, si, ix->L, cells, cinfo, qinfo,
                     nullptr, cn, nullptr, nullptr, nullptr, clen};
-        if (hcap > 0) {
+        if (three) {
+            SC_TRY(traverse3(ctx, p, kc, sv, si, ix->L, cells, clen, cinfo, qinfo, P.ccap, cn));
+        } else if (hcap > 0) {
             size_t lds = (size_t)qpw * lane_bytes;
             hipLaunchKernelGGL(k_traverse, dim3((unsigned)sc_cdiv(cn, qpw)), dim3(qpw), lds,
                                ctx->stream, ta, hcap, qpw);
             SC_CHECK_LAUNCH();
         }
         mark();
-        ScanArgs sa{kind, ix->dim, ix->mc, kc, ix->mp, ix->kp, p->R, P.ccap, qs, dqc, dqr,
+        ScanArgs sa{scan_kind, ix->dim, ix->mc, kc, ix->mp, ix->kp, p->R, P.ccap, qs, dqc, dqr,
                     ix->dot_cr, ix->L, ix->pid, ix->codes, cells, cinfo, qinfo,
                     d_ids + (size_t)q0 * p->R, d_dists + (size_t)q0 * p->R,
                     d_sums ? d_sums + q0 : nullptr, nullptr, clen,
@@ -2238,7 +2369,7 @@ static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
         const size_t lds = sizeof(ScanShared) +
                            sizeof(float) * std::max<size_t>((size_t)ix->mp * ix->kp,
                                                             p->form == SC_FORM_REAL ? ix->dim : 0);
-        if (hcap > 0) {
+        if (hcap > 0 || three) {
             hipLaunchKernelGGL(k_scan_topk, dim3((unsigned)cn), dim3(kScanThreads), lds,
                                ctx->stream, sa);
             SC_CHECK_LAUNCH();
@@ -2247,7 +2378,7 @@ static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
         // Queries whose heap outgrew the on-chip capacity are traversed again
         // with a global-memory heap of the reference's capacity and scanned
         // again; the first scan's output for them is overwritten.
-        if (hcap < hcap_ref) {
+        if (!three && hcap < hcap_ref) {
             int32_t nfb = 0;
             int32_t *dn = list + P.chunk;
             if (hcap > 0) {
@@ -2399,7 +2530,8 @@ extern "C" int sc_plan_create(sc_ctx *ctx, const sc_index *ix, const sc_search_p
     const FusedConfig F = fused_config(ix, p);
     const FusedLayout &Lay = F.lay;
     const int hcap = F.hcap, ccap = F.ccap, npow2 = F.npow2;
-    P->fused = kind != SC_KIND_IMI && p->form != SC_FORM_REAL && Lay.total <= 150 * 1024;
+    P->fused = kind != SC_KIND_IMI && kind != SC_KIND_SC3 && p->form != SC_FORM_REAL &&
+               Lay.total <= 150 * 1024;
     if (!P->fused) { *out = P; return SC_OK; }
     P->lds = Lay.total;
     bool ok = hipMalloc(&P->counter, 256) == hipSuccess &&

@@ -120,4 +120,97 @@ __device__ void tile_best(const float *xs, int t0, int blen, const float *__rest
     __syncthreads();
 }
 
+// The NB best (float-rounded distance, index) pairs under (dist ASC, index
+// ASC); empty slots (k < 0) come after every candidate.
+template <int NB>
+struct TopF {
+    float d[NB];
+    int k[NB];
+};
+
+template <int NB>
+__device__ __forceinline__ void topf_push(TopF<NB> &t, float d, int k) {
+    int p = NB;
+    while (p > 0 && (t.k[p - 1] < 0 || d < t.d[p - 1] || (d == t.d[p - 1] && k < t.k[p - 1]))) --p;
+    if (p >= NB) return;
+    for (int e = NB - 1; e > p; --e) { t.d[e] = t.d[e - 1]; t.k[e] = t.k[e - 1]; }
+    t.d[p] = d;
+    t.k[p] = k;
+}
+
+// tile_best's walk with every distance rounded to float (SCEncoder::encode
+// stores them in float arrays, include/sc_encoder.h:276-277, 306-314) and the
+// NB best kept per vector; out_k [TV][NB] (LDS).  Must be called by all
+// threads.
+template <int TV, int TK, int NB>
+__device__ void tile_top(const float *xs, int t0, int blen, const float *__restrict__ cb, int K,
+                         float *cs, TopF<NB> *red, int *out_k) {
+    constexpr int NT = (TV / 4) * (TK / 4);
+    const int tid = threadIdx.x;
+    const int tv = tid % (TV / 4), tk = tid / (TV / 4);
+    TopF<NB> b[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int e = 0; e < NB; ++e) { b[i].d[e] = INFINITY; b[i].k[e] = -1; }
+
+    for (int k0 = 0; k0 < K; k0 += TK) {
+        double acc[4][4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
+        for (int s0 = 0; s0 < blen; s0 += kTT) {
+            const int tt = min(kTT, blen - s0);
+            __syncthreads();
+            for (int e = tid; e < TK * kTT; e += NT) {
+                int kl = e / kTT, tl = e % kTT, k = k0 + kl;
+                float v = 0.0f;
+                if (k < K && tl < tt) v = cb[(size_t)k * blen + s0 + tl];
+                cs[tl * TK + kl] = v;
+            }
+            __syncthreads();
+            for (int tl = 0; tl < tt; ++tl) {
+                const float4 xv = *reinterpret_cast<const float4 *>(xs + (size_t)(t0 + s0 + tl) * TV + tv * 4);
+                const float4 cv = *reinterpret_cast<const float4 *>(cs + tl * TK + tk * 4);
+                const float xa[4] = {xv.x, xv.y, xv.z, xv.w};
+                const float ca[4] = {cv.x, cv.y, cv.z, cv.w};
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const double dd = (double)(xa[i] - ca[j]);
+                        acc[i][j] = __fma_rn(dd, dd, acc[i][j]);
+                    }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int k = k0 + tk * 4 + j;
+            if (k < K) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) topf_push<NB>(b[i], (float)acc[i][j], k);
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 4; ++i) red[tk * TV + tv * 4 + i] = b[i];
+    __syncthreads();
+    if (tid < TV) {
+        TopF<NB> m;
+#pragma unroll
+        for (int e = 0; e < NB; ++e) { m.d[e] = INFINITY; m.k[e] = -1; }
+        for (int p = 0; p < TK / 4; ++p) {
+            const TopF<NB> o = red[p * TV + tid];
+#pragma unroll
+            for (int e = 0; e < NB; ++e)
+                if (o.k[e] >= 0) topf_push<NB>(m, o.d[e], o.k[e]);
+        }
+#pragma unroll
+        for (int e = 0; e < NB; ++e) out_k[tid * NB + e] = m.k[e];
+    }
+    __syncthreads();
+}
+
 }  // namespace sc_tile

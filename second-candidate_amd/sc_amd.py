@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libsc_amd.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "sc_hip.h")
 
 SC_OK = 0
-KIND_IVFADC, KIND_IMI, KIND_SC = 0, 1, 2
+KIND_IVFADC, KIND_IMI, KIND_SC, KIND_SC3 = 0, 1, 2, 3
 PREC_FP32, PREC_BF16 = 0, 1
 FORM_PLAIN, FORM_MIXED, FORM_REAL = 0, 1, 2
 
@@ -90,6 +90,8 @@ _SIGS = {
                                 C.POINTER(_P)]),
     "sc_index_destroy": (C.c_int, [_P]),
     "sc_index_attach_raw": (C.c_int, [_P, _P, C.c_int64]),
+    "sc_encode_general": (C.c_int, [_P, _P, C.c_int64, C.c_int32, _P, C.c_int32, C.c_int32, _P,
+                                    C.c_int32, C.c_int32, C.c_int32, _P, _P]),
     "sc_groundtruth_nn1": (C.c_int, [_P, _P, C.c_int64, _P, C.c_int64, C.c_int32, _P, _P]),
     "sc_index_shape": (C.c_int, [_P, _P]),
     "sc_index_export": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P]),
@@ -341,6 +343,17 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+
+def encode_general(ctx, x, cq, pq, mc, kc, mp, kp, nc):
+    """SCEncoder::encode with general nc (1..3): cid Dev [n][mc*nc]
+    (block-major), codes Dev [n][mp]."""
+    n, dim = x.shape
+    cid = Dev(ctx, (n, mc * nc), np.uint16)
+    codes = Dev(ctx, (n, mp), np.uint8)
+    _check("sc_encode_general", lib().sc_encode_general(ctx.h, x.ptr, n, dim, cq.ptr, mc, kc,
+                                                        pq.ptr, mp, kp, nc, cid.ptr, codes.ptr))
+    return cid, codes
 
 
 def groundtruth_nn1(ctx, base, q):

@@ -3,8 +3,9 @@
 The device code of second-candidate_amd/csrc/sc_search.hip (its anonymous
 namespace up to the host-side plan code) is compiled as C++ against shim.h and
 linked with the oracle library and driver.inc.  Usage:
-    python make_emu.py OUT_BINARY SANITIZER [encode]   (SANITIZER: address | thread)
-With `encode`, the MFMA encode kernel of sc_encode_mfma.hip instead.
+    python make_emu.py OUT_BINARY SANITIZER [encode|trav3]   (SANITIZER: address | thread)
+With `encode`, the MFMA encode kernel of sc_encode_mfma.hip instead; with
+`trav3`, the three-candidate traversal kernel with its own driver.
 """
 import os
 import re
@@ -17,7 +18,7 @@ SRC = os.path.join(ROOT, "second-candidate_amd", "csrc", "sc_search.hip")
 ORACLE = os.path.join(ROOT, "oracle")
 
 
-def generate(path):
+def generate(path, driver="driver.inc"):
     src = open(SRC).read()
     # launch helpers use the HIP runtime; the emulation calls the kernel directly
     src = re.sub(r"using FusedKernel = .*?(?=__global__ void k_zero_ints)", "", src, flags=re.S)
@@ -30,7 +31,7 @@ def generate(path):
            "} // namespace",
            "void emu_run_fused(const FusedArgs &a, bool mixed) {",
            "    if (mixed) k_search_fused<3, true>(a); else k_search_fused<3, false>(a);",
-           "}", open(os.path.join(HERE, "driver.inc")).read()]
+           "}", open(os.path.join(HERE, driver)).read()]
     open(path, "w").write("\n".join(out))
 
 
@@ -51,10 +52,15 @@ def generate_encode(path):
     open(path, "w").write("\n".join(out))
 
 
-def build(out, sanitizer, cxx=None, encode=False):
+def build(out, sanitizer, cxx=None, encode=False, trav3=False):
     cxx = cxx or os.environ.get("EMU_CXX", "/opt/rocm/llvm/bin/clang++")
     cpp = out + ".cpp"
-    (generate_encode if encode else generate)(cpp)
+    if encode:
+        generate_encode(cpp)
+    elif trav3:
+        generate(cpp, "driver_trav3.inc")
+    else:
+        generate(cpp)
     cmd = [cxx, "-std=c++17", "-O1", "-g", "-fsanitize=" + sanitizer, "-fno-omit-frame-pointer",
            "-I" + HERE, "-I" + ORACLE, cpp, "-o", out, "-L" + ORACLE, "-loracle",
            "-Wl,-rpath," + ORACLE, "-lpthread"]
@@ -63,4 +69,5 @@ def build(out, sanitizer, cxx=None, encode=False):
 
 
 if __name__ == "__main__":
-    build(sys.argv[1], sys.argv[2], encode=len(sys.argv) > 3 and sys.argv[3] == "encode")
+    mode = sys.argv[3] if len(sys.argv) > 3 else ""
+    build(sys.argv[1], sys.argv[2], encode=mode == "encode", trav3=mode == "trav3")

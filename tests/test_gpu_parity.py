@@ -85,20 +85,24 @@ def test_encode_matches_oracle(ctx, case, prec):
 def _index_pair(ctx, kind, n=6000, dim=64, kc=16, mp=8, kp=64, seed=3):
     rng = np.random.default_rng(seed)
     mc = 2 if kind == S.KIND_IMI else 1
-    nc = 2 if kind == S.KIND_SC else 1
+    nc = {S.KIND_SC: 2, S.KIND_SC3: 3}.get(kind, 1)
     x = sco.gen(sco.mix_d128(ncomp=11), sco.STREAM_BASE, 0, n, dim)
     cq, pq = _codebooks(rng, x, mc, kc, mp, kp, ties=False)
     dx, dcq, dpq = (S.Dev.from_numpy(ctx, a) for a in (x, cq, pq))
-    cid, codes, _ = S.encode(ctx, dx, dcq, dpq, mc, kc, mp, kp, nc)
+    if kind == S.KIND_SC3:      # SCEncoder::encode, general nc
+        cid, codes = S.encode_general(ctx, dx, dcq, dpq, mc, kc, mp, kp, nc)
+        ocid, ocodes = sco.encode_general(x, cq, pq, mc, kc, mp, kp, nc)
+    else:
+        cid, codes, _ = S.encode(ctx, dx, dcq, dpq, mc, kc, mp, kp, nc)
+        ocid, ocodes = sco.encode(x, cq, pq, mc, kc, mp, kp, nc)
     ix = S.Index.build(ctx, cid, codes, dim, dcq, mc, kc, dpq, mp, kp, nc)
-    ocid, ocodes = sco.encode(x, cq, pq, mc, kc, mp, kp, nc)
     ncells = kc ** (mc * nc)
     L, pid, cs = sco.build_dense(sco.cell_keys(ocid, mc, kc, nc), ncells, ocodes)
     oix = sco.OracleIndex(dim, mc, kc, mp, kp, cq, pq, L, pid, cs)
     return ix, oix
 
 
-@pytest.mark.parametrize("kind", [S.KIND_IVFADC, S.KIND_IMI, S.KIND_SC])
+@pytest.mark.parametrize("kind", [S.KIND_IVFADC, S.KIND_IMI, S.KIND_SC, S.KIND_SC3])
 def test_search_real_distances_match_oracle(ctx, kind):
     """SC_FORM_REAL (real_dist = true): candidates ranked by the exact S1
     distance to the raw vectors, bit-exact ids and distances against the
@@ -180,7 +184,8 @@ SEARCH_CASES = [  # kind, R, w, T
 def _compare_search(ctx, ix, oix, kind, R, w, T, nq=150, form=S.FORM_PLAIN):
     q = sco.gen(sco.mix_d128(ncomp=11), sco.STREAM_QUERY, 0, nq, ix.dim)
     ids, dists, sums, st = ix.search(S.Dev.from_numpy(ctx, q), kind, R, w, T, form=form)
-    okind = {S.KIND_IVFADC: sco.IVFADC, S.KIND_IMI: sco.IMI, S.KIND_SC: sco.SC}[kind]
+    okind = {S.KIND_IVFADC: sco.IVFADC, S.KIND_IMI: sco.IMI, S.KIND_SC: sco.SC,
+             S.KIND_SC3: sco.SC3}[kind]
     oids, od, osum = oix.search_batch(okind, q, R, w, T)
     assert np.array_equal(sums.numpy(), osum)
     assert np.array_equal(ids.numpy(), oids)
@@ -194,6 +199,64 @@ def test_search_matches_oracle(ctx, case):
     kind, R, w, T = case
     ix, oix = _index_pair(ctx, kind)
     _compare_search(ctx, ix, oix, kind, R, w, T)
+    ix.close()
+
+
+# ------------------------------------------------------------------ three candidates (nc = 3)
+ENC_GENERAL_CASES = [  # dim, mc, kc, mp, kp, nc, n
+    (128, 1, 256, 8, 256, 3, 3001),
+    (128, 2, 32, 8, 256, 3, 1500),
+    (64, 1, 300, 4, 100, 2, 1000),
+    (960, 2, 32, 16, 256, 1, 500),
+    (32, 1, 5, 4, 16, 3, 700),
+]
+
+
+@pytest.mark.parametrize("case", ENC_GENERAL_CASES)
+def test_encode_general_matches_oracle(ctx, case):
+    """SCEncoder::encode with general nc: the nc nearest by float-rounded
+    distance (ties to the lower index) and the PQ codes, bit-exact."""
+    dim, mc, kc, mp, kp, nc, n = case
+    rng = np.random.default_rng(sum(case) + 1)
+    mix = sco.mix_d960() if dim == 960 else sco.mix_d128(ncomp=7)
+    x = sco.gen(mix, sco.STREAM_TRAIN, 0, n, dim)
+    if dim != 960:
+        x[n // 2:] += rng.uniform(-0.5, 0.5, (n - n // 2, dim)).astype(np.float32)
+    cq, pq = _codebooks(rng, x, mc, kc, mp, kp, ties=kc > 2)
+    x[:5, :dim // mc] = cq[0, 1]                  # exact ties on the coarse distance
+    cid, codes = S.encode_general(ctx, S.Dev.from_numpy(ctx, x), S.Dev.from_numpy(ctx, cq),
+                                  S.Dev.from_numpy(ctx, pq), mc, kc, mp, kp, nc)
+    ocid, ocodes = sco.encode_general(x, cq, pq, mc, kc, mp, kp, nc)
+    assert np.array_equal(cid.numpy(), ocid)
+    assert np.array_equal(codes.numpy(), ocodes)
+
+
+SC3_CASES = [  # kc, R, w, T
+    (12, 100, 500, 3000),
+    (12, 100, 4, 100000),        # cell budget (and heap capacity) binds: inserts dropped
+    (12, 300, 500, 40),          # fewer candidates than R
+    (12, 7, 500, 0),             # T = 0: no cell
+    (4, 50, 5000, 100000),       # the whole 4^3 grid popped
+    (32, 100, 2000, 2500),
+]
+
+
+@pytest.mark.parametrize("case", SC3_CASES)
+def test_search_sc3_matches_oracle(ctx, case):
+    """SCQuery::search_mr_ivf3: the (nearest, second, third) index built on
+    the GPU equals the oracle's; ids, distances and candidate counts of the
+    plain and mixed forms equal the oracle's search."""
+    kc, R, w, T = case
+    ix, oix = _index_pair(ctx, S.KIND_SC3, kc=kc)
+    e = ix.export()
+    assert np.array_equal(e["L"], oix.L) and np.array_equal(e["pid"], oix.pid)
+    assert np.array_equal(e["codes"], oix.codes)
+    st = _compare_search(ctx, ix, oix, S.KIND_SC3, R, w, T, nq=80)
+    if kc == 4 and T == 100000:
+        assert st.visited == 80 * 64
+    _compare_search(ctx, ix, oix, S.KIND_SC3, R, w, T, nq=80, form=S.FORM_MIXED)
+    with pytest.raises(S.ScError):
+        ix.search(S.Dev.from_numpy(ctx, np.zeros((1, ix.dim), np.float32)), S.KIND_SC, R, w, T)
     ix.close()
 
 
@@ -230,7 +293,8 @@ def test_search_plan_replays_match_oracle(ctx, kind, kc, kp, form):
     replay must equal the oracle (kc = kp = 64 takes the tiled LUT kernel
     inside the graph; IMI runs the staged kernels)."""
     ix, oix = _index_pair(ctx, kind, kc=kc, kp=kp)
-    okind = {S.KIND_IVFADC: sco.IVFADC, S.KIND_IMI: sco.IMI, S.KIND_SC: sco.SC}[kind]
+    okind = {S.KIND_IVFADC: sco.IVFADC, S.KIND_IMI: sco.IMI, S.KIND_SC: sco.SC,
+             S.KIND_SC3: sco.SC3}[kind]
     R, w, T, nq = 50, (12 if kind == S.KIND_IVFADC else 300), 2500, 130
     qs = [sco.gen(sco.mix_d128(ncomp=11), sco.STREAM_QUERY, i * nq, nq, ix.dim) for i in range(3)]
     dq = S.Dev.from_numpy(ctx, qs[0])

@@ -54,3 +54,32 @@ def test_fused_kernel_emulation(emu, name, args):
     assert r.returncode == 0, log[-4000:]
     nq = int(args.split()[6])
     assert ("emu: 0 of %d queries mismatch" % nq) in log, log[-4000:]
+
+
+TRAV3_CASES = [  # n kc w T nq
+    ("budget T binds", "5000 12 300 1500 16"),
+    ("heap capacity 3: inserts dropped", "4000 8 3 100000 16"),
+    ("whole 4^3 grid popped", "1500 4 5000 100000 8"),
+    ("cell budget binds", "6000 12 40 300 32"),
+    ("kc 16", "8000 16 2000 4000 24"),
+]
+
+
+@pytest.fixture(scope="module")
+def trav3(tmp_path_factory):
+    out = str(tmp_path_factory.mktemp("emu3") / "trav3")
+    make_emu.build(out, "address", CXX, trav3=True)
+    return out
+
+
+@pytest.mark.parametrize("name,args", TRAV3_CASES, ids=[c[0] for c in TRAV3_CASES])
+def test_traverse3_kernel_emulation(trav3, name, args):
+    """The three-candidate traversal kernel, compiled for the host under
+    AddressSanitizer, against the oracle's traced search query by query."""
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0")
+    r = subprocess.run([trav3] + args.split(), capture_output=True, text=True, timeout=600,
+                       env=env)
+    log = r.stdout + r.stderr
+    assert "ERROR: AddressSanitizer" not in log, log[-4000:]
+    assert r.returncode == 0, log[-4000:]
+    assert ("OK: 0 of %s queries differ" % args.split()[4]) in log, log[-4000:]

@@ -272,10 +272,13 @@ def _make_index(kind, n=3000, dim=32, kc=8, mp=4, kp=16, seed=7):
     mc = 2 if kind == sco.IMI else 1
     x = sco.gen(sco.mix_d128(ncomp=9), sco.STREAM_BASE, 0, n, dim)
     cq, pq = _codebooks(rng, x, mc, kc, mp, kp, dup=False)
-    nc = 2 if kind == sco.SC else 1
-    cid, codes = sco.encode(x, cq, pq, mc, kc, mp, kp, nc)
+    nc = {sco.SC: 2, sco.SC3: 3}.get(kind, 1)
+    if kind == sco.SC3:
+        cid, codes = sco.encode_general(x, cq, pq, mc, kc, mp, kp, nc)
+    else:
+        cid, codes = sco.encode(x, cq, pq, mc, kc, mp, kp, nc)
     keys = sco.cell_keys(cid, mc, kc, nc)
-    ncells = kc * kc if kind != sco.IVFADC else kc
+    ncells = {sco.IVFADC: kc, sco.SC3: kc ** 3}.get(kind, kc * kc)
     L, pid, cs = sco.build_dense(keys, ncells, codes)
     ix = sco.OracleIndex(dim, mc, kc, mp, kp, cq, pq, L, pid, cs)
     return ix, x, cid, codes
@@ -284,7 +287,7 @@ def _make_index(kind, n=3000, dim=32, kc=8, mp=4, kp=16, seed=7):
 def test_adc_matches_fp64_reconstruction():
     """ADC value = ||q - c - r||^2 for the candidate's centroid c and PQ
     reconstruction r (FP64), within FP32 rounding; q = c + r gives ~0."""
-    for kind in (sco.IVFADC, sco.IMI, sco.SC):
+    for kind in (sco.IVFADC, sco.IMI, sco.SC, sco.SC3):
         ix, x, cid, codes = _make_index(kind)
         q = sco.gen(sco.mix_d128(ncomp=9), sco.STREAM_QUERY, 0, 3, ix.dim)
         for qi in range(3):
@@ -397,8 +400,109 @@ def test_traversal_replay_with_reference_heap(kind):
             assert st.sum == total
 
 
+def _replay_traversal3(ix, q, w, T):
+    """Direct replay of search_mr_ivf3's multi-sequence step
+    (sc_query.h:296-402) over rank triples, driven by the reference's
+    compiled pq_insert6/pq_pop6; returns the non-empty cells in order, the
+    candidate total and the number of triples popped."""
+    import ctypes as C
+    R = sco.ref()
+    dqc, dqr = sco.lut(q, ix.cq, ix.pq, ix.norm_c, ix.norm_r, ix.dim, ix.mc, ix.kc, ix.mp, ix.kp)
+    kc = ix.kc
+    Lc = ix.L
+    length = lambda b: int(Lc[b] - Lc[b - 1]) if b > 0 else 0
+    hv = np.zeros(w, np.float32)
+    hid = [np.zeros(w, np.int32) for _ in range(6)]
+    pos = C.c_int32(0)
+    acc = np.float32(0)
+    for t in q:
+        acc = np.float32(acc + np.float32(t * t))
+    v = np.float32(acc + dqc)
+    o = np.lexsort((np.arange(kc), v))
+    vs = v[o]
+    ins = lambda s, a, b, c: R.scref_heap6_insert(hv, *hid, s, a, b, c, int(o[a]), int(o[b]),
+                                                  int(o[c]), C.byref(pos), w)
+    out = np.zeros(6, np.int32)
+    trav = np.zeros((kc, kc, kc), bool)
+    ins(np.float32(np.float32(3.0) * vs[0]), 0, 0, 0)
+    cells, total, visited = [], 0, 0
+    while len(cells) < w and total < T and visited < kc ** 3:
+        if not R.scref_heap6_pop(hv, *hid, out, C.byref(pos)):
+            break
+        h1, h2, h3, h4, h5, h6 = (int(z) for z in out)
+        bid = (h4 * kc + h5) * kc + h6
+        if length(bid) > 0:
+            cells.append(bid)
+            total += length(bid)
+        trav[h1, h2, h3] = True
+        visited += 1
+        if total < T:
+            if h1 < kc - 1 and (h2 == 0 or trav[h1 + 1, h2 - 1, h3]) and \
+                    (h3 == 0 or trav[h1 + 1, h2, h3 - 1]):
+                ins(np.float32(np.float32(vs[h1 + 1] + vs[h2]) + vs[h3]), h1 + 1, h2, h3)
+            if h2 < kc - 1 and (h3 == 0 or trav[h1, h2 + 1, h3 - 1]) and \
+                    (h1 == 0 or trav[h1 - 1, h2 + 1, h3]):
+                ins(np.float32(np.float32(vs[h1] + vs[h2 + 1]) + vs[h3]), h1, h2 + 1, h3)
+            if h3 < kc - 1 and (h2 == 0 or trav[h1, h2 - 1, h3 + 1]) and \
+                    (h1 == 0 or trav[h1 - 1, h2, h3 + 1]):
+                ins(np.float32(np.float32(vs[h1] + vs[h2]) + vs[h3 + 1]), h1, h2, h3 + 1)
+    # the popped set is a down-set of the rank grid (closed under lowering
+    # any coordinate), which is what lets a depth map stand for it
+    a, b, c = np.nonzero(trav)
+    for x, y, z in zip(a, b, c):
+        assert trav[:x + 1, :y + 1, :z + 1].all()
+    return cells, total, visited
+
+
+@ref_missing
+@pytest.mark.ref
+def test_traversal3_replay_with_reference_heap():
+    """The three-candidate traversal of the oracle equals the replay driven by
+    the reference's own heap, including a grid walked to exhaustion (kc=4:
+    every one of the 64 triples popped) and heap drops (w small)."""
+    for kc, n in ((8, 4000), (4, 1500)):
+        ix, x, _, _ = _make_index(sco.SC3, n=n, kc=kc)
+        q = sco.gen(sco.mix_d128(ncomp=9), sco.STREAM_QUERY, 0, 8, ix.dim)
+        for qi in range(8):
+            for w, T in ((3, 10 ** 9), (40, 300), (200, 1200), (5000, 10 ** 9)):
+                ids, dists, st, ex = ix.search(sco.SC3, q[qi], R=20, w=w, T=T, trace=True)
+                cells, total, visited = _replay_traversal3(ix, q[qi], w, T)
+                assert list(ex["cells"]) == cells
+                assert st.sum == total and st.nvisited == visited
+                if kc == 4 and w == 5000 and T == 10 ** 9:
+                    assert visited == 64 and total == n
+
+
+def test_encode_general_bruteforce():
+    """SCEncoder::encode (general nc): the nc nearest by float-rounded S1
+    distance, ties to the lower index; residual against the nearest; PQ
+    argmin of float-rounded distances, first index on ties."""
+    rng = np.random.default_rng(31)
+    for mc, nc in ((1, 3), (2, 3), (1, 1), (2, 2)):
+        kc, mp, kp, dim, n = 12, 4, 16, 32, 400
+        x = sco.gen(sco.mix_d128(ncomp=5), sco.STREAM_TRAIN, 0, n, dim)
+        x[::3] += rng.uniform(-0.5, 0.5, x[::3].shape).astype(np.float32)
+        cq, pq = _codebooks(rng, x, mc, kc, mp, kp)      # duplicated centroids: exact ties
+        cid, codes = sco.encode_general(x, cq, pq, mc, kc, mp, kp, nc)
+        bsc, bsp = dim // mc, dim // mp
+        cqb = cq.reshape(mc, kc, bsc)
+        pqb = pq.reshape(mp, kp, bsp)
+        for i in range(n):
+            r = np.empty(dim, np.float32)
+            for j in range(mc):
+                xb = x[i, j * bsc:(j + 1) * bsc]
+                d = _s1_sequential(xb, cqb[j])
+                top = np.lexsort((np.arange(kc), d))[:nc]
+                assert list(cid[i, j * nc:(j + 1) * nc]) == list(top)
+                r[j * bsc:(j + 1) * bsc] = xb - cqb[j, top[0]]
+            for s_ in range(mp):
+                d = _s1_sequential(r[s_ * bsp:(s_ + 1) * bsp], pqb[s_])
+                assert codes[i, s_] == int(np.flatnonzero(d == d.min())[0])
+        assert (cid.reshape(n, mc, nc)[:, :, 0] != cid.reshape(n, mc, nc)[:, :, -1]).all() or nc == 1
+
+
 def test_search_topr_is_canonical_selection():
-    for kind in (sco.IVFADC, sco.IMI, sco.SC):
+    for kind in (sco.IVFADC, sco.IMI, sco.SC, sco.SC3):
         ix, x, _, _ = _make_index(kind)
         q = sco.gen(sco.mix_d128(ncomp=9), sco.STREAM_QUERY, 0, 5, ix.dim)
         for qi in range(5):
