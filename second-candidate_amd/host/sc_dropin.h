@@ -387,19 +387,28 @@ protected:
     unsigned char *codes;
     std::vector<Bucket> ivf;
     int old_mp;
-    int nc_ = 1;   // 1: IVF/IMI cells; 2: (nearest, second) cells
+    int nc_ = 1;   // coarse candidates per block the last encode wrote
 
-    inline void encode_device(const float *x, int nc) {
+    // general: SCEncoder::encode's float-rounded top-nc (sc_encode_general);
+    // otherwise Encoder::encode (nc = 1) / SCEncoder::encode2 (nc = 2)
+    inline void encode_device(const float *x, int nc, bool general = false) {
         const size_t n = (size_t)config.N;
         detail::Dev dx(x, n * config.dim * sizeof(float));
         detail::Dev dcq(cq, (size_t)config.kc * config.dim * sizeof(float));
         detail::Dev dpq(pq, (size_t)config.kp * config.dim * sizeof(float));
         detail::Dev dcid(n * config.mc * nc * sizeof(ushort)), dcodes(n * config.mp);
-        detail::check(sc_encode(detail::ctx(), dx.as<float>(), (int64_t)n, config.dim,
-                                dcq.as<float>(), config.mc, config.kc, dpq.as<float>(), config.mp,
-                                config.kp, nc, SC_PREC_FP32, dcid.as<uint16_t>(),
-                                dcodes.as<uint8_t>(), nullptr),
-                      "sc_encode");
+        if (general)
+            detail::check(sc_encode_general(detail::ctx(), dx.as<float>(), (int64_t)n, config.dim,
+                                            dcq.as<float>(), config.mc, config.kc, dpq.as<float>(),
+                                            config.mp, config.kp, nc, dcid.as<uint16_t>(),
+                                            dcodes.as<uint8_t>()),
+                          "sc_encode_general");
+        else
+            detail::check(sc_encode(detail::ctx(), dx.as<float>(), (int64_t)n, config.dim,
+                                    dcq.as<float>(), config.mc, config.kc, dpq.as<float>(),
+                                    config.mp, config.kp, nc, SC_PREC_FP32, dcid.as<uint16_t>(),
+                                    dcodes.as<uint8_t>(), nullptr),
+                          "sc_encode");
         ::operator delete(cid);
         ::operator delete(codes);
         cid = static_cast<ushort *>(::operator new(n * config.mc * nc * sizeof(ushort) + 2));
@@ -562,25 +571,36 @@ public:
         encode_device(x, 2);
         ::operator delete(x);
     }
-    // include/sc_encoder.h:205-332: the general-nc encoder is built for nc=2
-    // only (its distribution is broken for nc != 2, src/sc_encoder.cpp:41-59).
+    // include/sc_encoder.h:205-332: the nc nearest per block by float-rounded
+    // distance (nc = 1..3), cid [N][mc][nc]
     template <typename DataType>
     inline void encode(const char *filename, int offset, bool verbose) {
-        if (nc != 2) {
-            std::cerr << "SCEncoder::encode: only nc=2 is supported" << std::endl;
+        if (nc < 1 || nc > 3) {
+            std::cerr << "SCEncoder::encode: nc must be 1, 2 or 3" << std::endl;
             return;
         }
-        encode2<DataType>(filename, offset, verbose);
+        float *x = nullptr;
+        config.N = detail::load_rows<DataType>(filename, x, offset, config.dim, verbose);
+        if (config.mc <= 0 || config.mp <= 0) {
+            std::cerr << "Nothing to do" << std::endl;
+            ::operator delete(x);
+            return;
+        }
+        nc_ = nc;
+        encode_device(x, nc, true);
+        ::operator delete(x);
     }
 
-    // src/sc_encoder.cpp:17-81
+    // src/sc_encoder.cpp:17-81, with the per-vector cid stride mc*nc (the
+    // reference strides 2*mc, which only holds for nc = 2); nc = 3 needs
+    // mc = 1 and kc <= 1024
     void distribution(bool verbose) {
-        double cells = std::pow((double)config.kc, (double)config.mc * nc);
+        double cells = std::pow((double)config.kc, (double)config.mc * nc_);
         if (cells >= (double)INT32_MAX) {
             std::cerr << "The size of this inverted index is too LARGE!" << std::endl;
             std::exit(EXIT_FAILURE);
         }
-        distribute(2, (int64_t)cells);
+        distribute(nc_, (int64_t)cells);
         if (verbose)
             std::cout << "The number of non-empty buckets: " << non_empty_bucket << std::endl;
     }
@@ -839,11 +859,20 @@ public:
         run_one(query, SC_KIND_SC, R, w, T, M, dist, result, sum, real_dist);
     }
 
-    // sc_query.h:265-475 (nc = 3) is not built in this library.
-    inline void search_mr_ivf3(float *, float *&, float *&, int *, int *&, int *&, int *&, int *&,
-                               int *&, int *&, int *&, int *&, int *&, bool *, int &, int, int,
-                               int, int, bool, bool) {
-        std::cerr << "This search method is for MultiRank-3 IVFADC only" << std::endl;
+    // sc_query.h:265-475: (nearest, second, third) cells (the top-R comes back
+    // sorted by (dist, position))
+    inline void search_mr_ivf3(float *query, float *&v_tmp, float *&dist, int *tmp, int *&result,
+                               int *&hid1, int *&hid2, int *&hid3, int *&hid4, int *&hid5,
+                               int *&hid6, int *&s1, int *&prebuck, int *&cache, bool *traversed,
+                               int &sum, int R, int w, int T, int M, bool real_dist,
+                               bool verbose) {
+        (void)v_tmp; (void)tmp; (void)hid1; (void)hid2; (void)hid3; (void)hid4; (void)hid5;
+        (void)hid6; (void)s1; (void)prebuck; (void)cache; (void)traversed; (void)verbose;
+        if (config.mc != 1 || nc != 3) {
+            std::cerr << "This search method is for MultiRank IVFADC-3 only" << std::endl;
+            return;
+        }
+        run_one(query, SC_KIND_SC3, R, w, T, M, dist, result, sum, real_dist);
     }
 };
 

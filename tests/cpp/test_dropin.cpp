@@ -97,8 +97,12 @@ static void oracle_index(OracleIx &o, const std::vector<float> &x, int dim, int 
     const int64_t n = (int64_t)x.size() / dim;
     std::vector<uint16_t> cid((size_t)n * mc * nc);
     std::vector<uint8_t> codes((size_t)n * 8);
-    EXPECT(sco_encode(x.data(), n, dim, o.cq.data(), mc, kc, o.pq.data(), 8, 256, nc, cid.data(),
-                      codes.data(), 8) == 0);
+    if (nc == 3)   // SCEncoder::encode, general nc
+        EXPECT(sco_encode_general(x.data(), n, dim, o.cq.data(), mc, kc, o.pq.data(), 8, 256, nc,
+                                  cid.data(), codes.data(), 8) == 0);
+    else
+        EXPECT(sco_encode(x.data(), n, dim, o.cq.data(), mc, kc, o.pq.data(), 8, 256, nc,
+                          cid.data(), codes.data(), 8) == 0);
     std::vector<uint64_t> keys(n);
     sco_cell_keys(cid.data(), n, mc, kc, nc, keys.data());
     int64_t cells = 1;
@@ -225,6 +229,48 @@ int main(int argc, char **argv) {
             EXPECT(sco_search_real(&o.s, SCO_SC, &xq[(size_t)i * dim], xb.data(), R, w, T,
                                    oid.data(), od.data(), &st) == 0);
             EXPECT(sum == st.sum);
+            for (int r = 0; r < (R < sum ? R : sum); ++r) {
+                EXPECT(result[r] == oid[r]);
+                EXPECT(std::memcmp(&dist[r], &od[r], 4) == 0);
+            }
+            ::operator delete(result);
+            ::operator delete(dist);
+        }
+    }
+
+    // ---------------- SC, three candidates: nc=3 over the same codebooks
+    {
+        SC::SCEncoder enc(3);
+        enc.load_codebooks((dir + "/sc_cq.ctr_").c_str(), (dir + "/sc_pq.ctr_").c_str(), false);
+        enc.encode<float>((dir + "/base.fvecs").c_str(), 4, false);
+        enc.distribution(false);
+        enc.output(dir.c_str(), "base", false);
+        OracleIx o;
+        int k, m, d;
+        o.cq = read_ctr(dir + "/sc_cq.ctr_", &k, &m, &d);
+        o.pq = read_ctr(dir + "/sc_pq.ctr_", &k, &m, &d);
+        oracle_index(o, xb, dim, 1, 32, 3);
+        check_edat(dir + "/base_mr3_ivf.edat_", o, 20000);
+        SC::SCQuery q(3);
+        q.load_codebooks((dir + "/sc_cq.ctr_").c_str(), (dir + "/sc_pq.ctr_").c_str(), false);
+        EXPECT(q.load_encoded_data((dir + "/base_mr3_ivf.edat_").c_str(), false) == 20000);
+        q.pre_compute1();
+        const int w = 3000, T = 1500, R = 100, kc = 32;
+        float *v_tmp = new float[kc + w], *dist = nullptr;
+        int *it = new int[kc], *result = nullptr;
+        int *h[6], *s1 = new int[w], *pb = new int[w], *cache = new int[w];
+        for (int t = 0; t < 6; ++t) h[t] = new int[w];
+        bool *trav = new bool[kc * kc * kc]();
+        for (int i = 0; i < 40; ++i) {
+            int sum = 0;
+            q.search_mr_ivf3(&xq[(size_t)i * dim], v_tmp, dist, it, result, h[0], h[1], h[2], h[3],
+                             h[4], h[5], s1, pb, cache, trav, sum, R, w, T, kc, false, false);
+            std::vector<int32_t> oid(R);
+            std::vector<float> od(R);
+            sco_stats st;
+            EXPECT(sco_search(&o.s, SCO_SC3, &xq[(size_t)i * dim], R, w, T, kc, oid.data(),
+                              od.data(), &st, nullptr, nullptr, nullptr, nullptr, nullptr, 0) == 0);
+            EXPECT(sum == st.sum && sum > 0);
             for (int r = 0; r < (R < sum ? R : sum); ++r) {
                 EXPECT(result[r] == oid[r]);
                 EXPECT(std::memcmp(&dist[r], &od[r], 4) == 0);
