@@ -6,7 +6,8 @@
 // (include/encoder.h:69-192), Encoder::distribution (src/encoder.cpp:170-205),
 // vector_base (include/sc_utilities.h:156-185) and the array heaps
 // pq_insert/pq_pop, pq_insert4/pq_pop4, pq_insert6/pq_pop6
-// (include/sc_utilities.h:229-300, 469-675).  The reference's query classes and SCEncoder include <cblas.h>
+// (include/sc_utilities.h:229-300, 469-675) and SimpleCluster::distortion
+// (lib/simple-cluster/include/k-means.h:444-464).  The reference's query classes and SCEncoder include <cblas.h>
 // (OpenBLAS fork, not vendored) and are not built.
 //
 // Built by oracle/Makefile into oracle/_ref/libscref.so straight from the
@@ -16,6 +17,7 @@
 #include <vector>
 
 #include "encoder.h"
+#include "k-means.h"
 #include "sc_utilities.h"
 #include "utilities.h"
 
@@ -105,6 +107,14 @@ int scref_heap1_pop(float *v, int32_t *id, int32_t *pos) {
   int r = SC::pq_pop(v, id, p, false);
   *pos = p;
   return r;
+}
+
+float scref_distortion(const float *data, const float *centers, const int *labels, int d,
+                       int N, int k) {
+  return SimpleCluster::distortion<float>(const_cast<float *>(data),
+                                          const_cast<float *>(centers),
+                                          const_cast<int *>(labels),
+                                          SimpleCluster::DistanceType::NORM_L2, d, N, k, false);
 }
 
 void scref_heap6_insert(float *v, int32_t *h1, int32_t *h2, int32_t *h3, int32_t *h4,

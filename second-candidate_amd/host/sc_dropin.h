@@ -267,7 +267,10 @@ public:
         }
     }
 
-    // quantizer.h:279-300 with k-means.h:444-464 (per-part float sum, sqrt).
+    // quantizer.h:279-300 with SimpleCluster::distortion (k-means.h:444-464):
+    // per part, a float running sum of the S1 distances (each `+=` adds in
+    // double and rounds to float) and its float square root; then the double
+    // square root of the sum of the parts' squares.
     inline double distortion(bool verbose) {
         (void)verbose;
         const int bs = dim / part;
@@ -283,9 +286,9 @@ public:
                     double dd = (double)(x[t] - y[t]);
                     acc += dd * dd;
                 }
-                et += (float)acc;
+                et = (float)((double)et + acc);
             }
-            double es = std::sqrt((double)et);
+            const double es = std::sqrt(et);   // float square root, widened
             e += es * es;
         }
         return std::sqrt(e);
