@@ -58,6 +58,9 @@ def main():
                     help="1: also time the mixed-precision form (FP16 table + exact re-rank)")
     ap.add_argument("--real", type=int, default=1,
                     help="1: also time the real-distance form (exact distances from the raw vectors)")
+    ap.add_argument("--imi", type=int, default=1,
+                    help="1: also time the inverted multi-index workload (two 64-dim halves, "
+                         "K=1024 each, PQ 8x256, same base and queries)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="only run a plain device copy of this many MiB (HBM counter scale)")
     args = ap.parse_args()
@@ -308,6 +311,63 @@ def main():
             log("T=%d: recall %.3f, plain %.0f QPS, mixed %.0f QPS"
                 % (tv, row["recall_at_%d" % args.R], row["qps_plain"], row["qps_mixed"]))
 
+    # ---------------- inverted multi-index: two halves with K=1024 centroids
+    # each, PQ 8x256 on the residuals, T=10k (the staged kernels)
+    imi = None
+    if args.imi:
+        kci = 1024
+        t0 = time.perf_counter()
+        xt = S.gen_rows(ctx, mix, S.STREAM_TRAIN, 0, args.ntrain, dim)
+        th = xt.numpy()
+        cqi = np.stack([S.kmeans(ctx, S.Dev.from_numpy(ctx, np.ascontiguousarray(th[:, h * 64:(h + 1) * 64])),
+                                 kci, args.iters, seed=11 + h, want_labels=False)[0].numpy()
+                        for h in range(2)])
+        dcqi = S.Dev.from_numpy(ctx, cqi)
+        S.residual(ctx, xt, dcqi, 2, kci, 0)
+        rh = xt.numpy()
+        pqi = np.stack([S.kmeans(ctx, S.Dev.from_numpy(ctx, np.ascontiguousarray(rh[:, s * 16:(s + 1) * 16])),
+                                 kp, args.iters, seed=21 + s, want_labels=False)[0].numpy()
+                        for s in range(mp)])
+        dpqi = S.Dev.from_numpy(ctx, pqi)
+        del xt, th, rh
+        t_imi_train = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        icid, icodes, _ = S.encode(ctx, x, dcqi, dpqi, 2, kci, mp, kp, 1)
+        ctx.sync()
+        t_imi_enc = time.perf_counter() - t0
+        iix = S.Index.build(ctx, icid, icodes, dim, dcqi, 2, kci, dpqi, mp, kp, 1)
+        del icid, icodes
+        iids = S.Dev(ctx, (args.nq, args.R), np.int32)
+        idd = S.Dev(ctx, (args.nq, args.R), np.float32)
+        isums = S.Dev(ctx, (args.nq,), np.int32)
+        ist = iix.search(q, S.KIND_IMI, args.R, args.w, args.T, ids=iids, dists=idd, sums=isums,
+                         stats=True)[3]
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            iix.search(q, S.KIND_IMI, args.R, args.w, args.T, ids=iids, dists=idd, sums=isums,
+                       stats=False)
+        ctx.sync()
+        iel = time.perf_counter() - t0
+        imi = {"workload": "IMI, two 64-dim halves with K=1024 centroids each, PQ 8x256, "
+                           "%d base, %d queries, w=%d, T=%d, top-%d" % (args.n, args.nq, args.w,
+                                                                       args.T, args.R),
+               "qps": round(args.nq * args.steps / iel, 1),
+               "ms_per_step": round(iel * 1e3 / args.steps, 3),
+               "candidates_per_query": round(ist.candidates / args.nq, 1),
+               "cells_popped_per_query": round(ist.visited / args.nq, 1),
+               "heap_fallback_queries": int(ist.heap_fallback),
+               "stage_ms": {"lut": round(ist.ms_lut, 3), "rank": round(ist.ms_rank, 3),
+                            "traverse": round(ist.ms_traverse, 3), "scan": round(ist.ms_scan, 3)},
+               "encode_s": round(t_imi_enc, 4), "train_s": round(t_imi_train, 2),
+               "recall_at_%d" % args.R: (round(sco.recall(gt, iids.numpy(), args.R), 4)
+                                         if recall is not None else None)}
+        log("imi K=1024: %.1f ms/step, %.0f QPS, %.0f candidates and %.0f cells popped per query, "
+            "stages %s, recall %s" % (imi["ms_per_step"], imi["qps"], imi["candidates_per_query"],
+                                      imi["cells_popped_per_query"], imi["stage_ms"],
+                                      imi["recall_at_%d" % args.R]))
+        iix.close()
+
     # ---------------- CPU baseline: the OpenMP oracle on a bounded sample
     cpu = None
     if args.cpu_seconds > 0:
@@ -373,6 +433,7 @@ def main():
             "qps_plain_fp32_per_query_calls": round(qps_single, 1) if qps_single else None,
             "mixed_precision_batched": mixed,
             "real_distance": real,
+            "imi_k1024": imi,
             "candidates_per_query": round(cand / args.nq, 1),
             "stage_ms": {"lut": round(st0.ms_lut, 4), "rank": round(st0.ms_rank, 4),
                          "traverse": round(st0.ms_traverse, 4), "scan_topk": round(ms_kernel, 4)},

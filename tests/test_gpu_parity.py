@@ -454,3 +454,43 @@ def test_full_size_sc_index_properties(ctx):
     assert np.array_equal(ids.numpy(), oids)
     assert np.array_equal(_bits(dists.numpy()), _bits(od))
     ix.close()
+
+
+def test_full_size_imi_search(ctx):
+    """1M x 128 inverted multi-index, two 64-dim halves with K=1024 each, PQ
+    8x256 (the multi-index benchmark shape): encode on sampled rows, index
+    invariants, and search (w=100k, T=10k, top-100) on 48 queries vs the
+    oracle."""
+    n, dim, kc, mp, kp = 1_000_000, 128, 1024, 8, 256
+    mix = S.mix_d128()
+    xt = S.gen_rows(ctx, mix, S.STREAM_TRAIN, 0, 50_000, dim)
+    th = xt.numpy()
+    cq = np.stack([S.kmeans(ctx, S.Dev.from_numpy(ctx, np.ascontiguousarray(th[:, h * 64:(h + 1) * 64])),
+                            kc, 3, seed=11 + h, want_labels=False)[0].numpy() for h in range(2)])
+    dcq = S.Dev.from_numpy(ctx, cq)
+    S.residual(ctx, xt, dcq, 2, kc, 0)
+    rh = xt.numpy()
+    pq = np.stack([S.kmeans(ctx, S.Dev.from_numpy(ctx, np.ascontiguousarray(rh[:, s * 16:(s + 1) * 16])),
+                            kp, 3, seed=21 + s, want_labels=False)[0].numpy() for s in range(mp)])
+    dpq = S.Dev.from_numpy(ctx, pq)
+    x = S.gen_rows(ctx, mix, S.STREAM_BASE, 0, n, dim)
+    cid, codes, _ = S.encode(ctx, x, dcq, dpq, 2, kc, mp, kp, 1)
+    cidh, codesh = cid.numpy(), codes.numpy()
+    rows = np.random.default_rng(1).choice(n, 2000, replace=False)
+    xr = np.concatenate([sco.gen(sco.mix_d128(), sco.STREAM_BASE, int(r), 1, dim) for r in rows])
+    ocid, ocodes = sco.encode(xr, cq, pq, 2, kc, mp, kp, 1)
+    assert np.array_equal(cidh[rows], ocid) and np.array_equal(codesh[rows], ocodes)
+    ix = S.Index.build(ctx, cid, codes, dim, dcq, 2, kc, dpq, mp, kp, 1)
+    e = ix.export()
+    assert e["L"].size == kc * kc and e["L"][-1] == n
+    keys = sco.cell_keys(cidh, 2, kc, 1)
+    assert np.array_equal(np.bincount(keys.astype(np.int64), minlength=kc * kc),
+                          np.diff(np.concatenate([[0], e["L"]])))
+    oix = sco.OracleIndex(dim, 2, kc, mp, kp, cq, pq, e["L"], e["pid"], e["codes"])
+    q = sco.gen(sco.mix_d128(), sco.STREAM_QUERY, 0, 48, dim)
+    ids, dists, sums, st = ix.search(S.Dev.from_numpy(ctx, q), S.KIND_IMI, 100, 100000, 10000)
+    oids, od, osum = oix.search_batch(sco.IMI, q, 100, 100000, 10000)
+    assert np.array_equal(sums.numpy(), osum)
+    assert np.array_equal(ids.numpy(), oids)
+    assert np.array_equal(_bits(dists.numpy()), _bits(od))
+    ix.close()
