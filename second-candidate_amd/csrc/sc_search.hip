@@ -388,6 +388,137 @@ __global__ void k_traverse(TravArgs a, int hcap, int qpw) {
     qi[5] = h.overflow ? 1 : 0; qi[6] = err; qi[7] = 0;
 }
 
+// Multi-sequence traversal of IMI / SC (include/multi_query.h:133-205,
+// include/sc_query.h:111-186) with one 64-thread workgroup per query: the
+// query's ranked lists, row lengths and heap sit in LDS; thread 0 replays the
+// heap and runs up to kSpec pops ahead (the pops and pushes do not depend on
+// the cell sizes until the loop stops); the batch's cell sizes are loaded by
+// kSpec lanes at once, and thread 0 commits the pops in order, stopping
+// exactly where the reference does.  A heap outgrowing hcap is flagged
+// (qinfo[5]) for the global-heap lane kernel, as in k_traverse.
+// LDS: 5 kc ints/floats + hcap (score, pair) entries.
+constexpr int kTravThreads = 64;
+
+__global__ __launch_bounds__(kTravThreads) void k_traverse_wave(TravArgs a, int hcap) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int kc = a.kc, tid = threadIdx.x;
+    const bool imi = a.kind == SC_KIND_IMI;
+    const int lists = imi ? 2 : 1;
+    float *sv = reinterpret_cast<float *>(smem);                   // [lists][kc]
+    int32_t *st = reinterpret_cast<int32_t *>(sv + lists * kc);    // [lists][kc]
+    int32_t *rowlen = st + lists * kc;                              // [kc]
+    float *hv = reinterpret_cast<float *>(rowlen + kc);             // [hcap]
+    uint32_t *hp = reinterpret_cast<uint32_t *>(hv + hcap);         // [hcap]
+    __shared__ int64_t s_bid[kSpec];
+    __shared__ uint32_t s_info[kSpec];
+    __shared__ int32_t s_len[kSpec];
+    __shared__ int s_n, s_done;
+
+    const int64_t slot = blockIdx.x;
+    if (slot >= a.nq) return;
+    const int64_t q = a.qlist ? a.qlist[slot] : slot;
+    for (int i = tid; i < lists * kc; i += kTravThreads) {
+        sv[i] = a.sv[(size_t)q * lists * kc + i];
+        st[i] = a.si[(size_t)q * lists * kc + i];
+    }
+    for (int i = tid; i < kc; i += kTravThreads) rowlen[i] = 0;
+    if (tid == 0) s_done = 0;
+    __syncthreads();
+    const float *v1 = sv, *v2 = imi ? sv + kc : sv;
+    const int32_t *t1 = st, *t2 = imi ? st + kc : st;
+
+    int32_t *cells = a.cells + (size_t)q * a.ccap;
+    int32_t *clen = a.clen + (size_t)q * a.ccap;
+    uint32_t *cinfo = a.cinfo + (size_t)q * a.ccap;
+    // thread 0's state
+    Heap h;
+    h.v = hv;
+    h.p = hp;
+    h.stride = 1;
+    h.pos = 0;
+    h.cap_ref = a.w;
+    h.cap_phys = hcap;
+    h.overflow = false;
+    int count = 0, count2 = 0, sum = 0, empty = 0, exhausted = 0, err = 0, flag = 0;
+    bool dry = false;                                // heap empty after the batch (thread 0)
+    if (tid == 0) {
+        if (imi) {
+            h.insert(v1[0] + v2[0], 0u);
+        } else {
+            rowlen[0] = 1;                           // (0,0) never enters the SC grid
+            const float d0 = v1[0] + v1[1];
+            h.insert(d0, (0u << 16) | 1u);
+            h.insert(d0, (1u << 16) | 0u);
+        }
+        if (!((imi ? count2 : count) < a.w && sum < a.T)) s_done = 1;
+    }
+    __syncthreads();
+    while (!s_done) {
+        if (tid == 0) {
+            // run ahead: up to kSpec pops with their pushes
+            int n = 0;
+            dry = false;
+            while (n < kSpec && !h.overflow) {
+                if (h.pos <= 0) { dry = true; break; }
+                const uint32_t pk = h.pop();
+                const int h1 = (int)(pk >> 16), h2 = (int)(pk & 0xFFFFu);
+                const int c1 = t1[h1], c2 = t2[h2];
+                const int64_t bid = (int64_t)c1 * kc + c2;
+                s_bid[n] = imi ? bid : (bid > 0 ? bid : -1);   // SC: cell 0 counts as empty
+                s_info[n] = (uint32_t)c1 | ((uint32_t)c2 << 16);
+                ++n;
+                if (rowlen[h1] != h2) err = 3;           // not a down-set: cannot happen
+                rowlen[h1] = h2 + 1;
+                if (h1 < kc - 1 && (h2 == 0 || (h2 - 1) < rowlen[h1 + 1]))
+                    h.insert(v1[h1 + 1] + v2[h2], ((uint32_t)(h1 + 1) << 16) | (uint32_t)h2);
+                if (h2 < kc - 1 && (h1 == 0 || (h2 + 1) < rowlen[h1 - 1]))
+                    h.insert(v1[h1] + v2[h2 + 1], ((uint32_t)h1 << 16) | (uint32_t)(h2 + 1));
+            }
+            s_n = n;
+        }
+        __syncthreads();
+        const int n = s_n;
+        if (tid < n) {
+            const int64_t b = s_bid[tid];
+            s_len[tid] = b >= 0 ? cell_len(a.L, b) : 0;
+            s_bid[tid] = b >= 0 ? cell_start(a.L, b) : 0;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            // commit in order; the loop test precedes every pop
+            int i = 0;
+            bool stop = false;
+            for (; i < n; ++i) {
+                if (!((imi ? count2 : count) < a.w && sum < a.T)) { stop = true; break; }
+                const int l = s_len[i];
+                if (l > 0) {
+                    if (count >= a.ccap) { err = 2; stop = true; break; }
+                    cells[count] = (int32_t)s_bid[i];
+                    clen[count] = l;
+                    cinfo[count] = s_info[i];
+                    count++;
+                    sum += l;
+                } else {
+                    empty++;
+                }
+                count2++;
+            }
+            if (!stop && !((imi ? count2 : count) < a.w && sum < a.T)) stop = true;
+            if (!stop) {
+                if (dry) { exhausted = 1; stop = true; }             // the heap is empty
+                else if (h.overflow) { flag = 1; stop = true; }      // later pops are invalid
+            }
+            s_done = stop ? 1 : 0;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        int32_t *qi = a.qinfo + (size_t)q * 8;
+        qi[0] = count; qi[1] = sum; qi[2] = count2; qi[3] = empty; qi[4] = exhausted;
+        qi[5] = flag; qi[6] = err; qi[7] = 0;
+    }
+}
+
 // Three-candidate traversal (SCQuery::search_mr_ivf3, include/sc_query.h:296-402):
 // rank triples (h1, h2, h3) of one coarse list, cell (t[h1], t[h2], t[h3]),
 // heap of the reference's capacity w in global memory, one lane per query.
@@ -1955,7 +2086,9 @@ Plan make_plan(const sc_index *ix, const sc_search_params *p, int64_t nq) {
     P.ccap = (int)ccap;
     const size_t per = 16 + 4ull * ix->mc * ix->kc + 4ull * ix->mp * ix->kp +
                        8ull * lists * ix->kc + 12ull * ccap + 32 + 4;
-    int64_t chunk = (int64_t)((size_t)1 << 30) / (int64_t)per;
+    // up to 4 GiB of per-query scratch (cell lists sized for the budget T),
+    // so that a batch of ten thousand queries runs as one pass
+    int64_t chunk = (int64_t)((size_t)4 << 30) / (int64_t)per;
     chunk = std::max<int64_t>(1, std::min<int64_t>(chunk, std::max<int64_t>(nq, 1)));
     chunk = std::min<int64_t>(chunk, 65536);
     P.chunk = chunk;
@@ -2324,6 +2457,20 @@ static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
     const size_t lane_bytes = (size_t)hcap * 8 + (size_t)kc * 4;
     int qpw = (int)std::min<size_t>(64, (48 * 1024) / lane_bytes);
     if (qpw < 1) { qpw = 1; hcap = 0; }   // row lengths alone do not fit: global heap
+    // IMI and SC: one workgroup per query (ranked lists, row lengths and a heap
+    // of up to 1024 entries in at most 32 KiB of LDS) when the lists fit
+    const size_t wave_fixed = (size_t)lists * kc * 8 + (size_t)kc * 4;
+    int hcap_w = 0;
+    if (kind != SC_KIND_IVFADC && !three && wave_fixed + 64 * 8 <= 32 * 1024) {
+        hcap_w = (int)std::min<size_t>(std::min<size_t>((size_t)hcap_ref, 1024),
+                                       (32 * 1024 - wave_fixed) / 8);
+        if (const char *e = getenv("SC_HEAP_ONCHIP")) {
+            int v = atoi(e);
+            if (v >= 1) hcap_w = std::min(hcap_w, v);
+        }
+    }
+    const bool wave = hcap_w > 0;
+    const int hcap_eff = wave ? hcap_w : hcap;     // on-chip heap of the traversal used
     int64_t fallback_total = 0;
     // stage timing: five events per chunk on the context stream
     std::vector<hipEvent_t> ev;
@@ -2354,6 +2501,11 @@ static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
                     nullptr, cn, nullptr, nullptr, nullptr, clen};
         if (three) {
             SC_TRY(traverse3(ctx, p, kc, sv, si, ix->L, cells, clen, cinfo, qinfo, P.ccap, cn));
+        } else if (wave) {
+            const size_t lds = wave_fixed + (size_t)hcap_w * 8;
+            hipLaunchKernelGGL(k_traverse_wave, dim3((unsigned)cn), dim3(kTravThreads), lds,
+                               ctx->stream, ta, hcap_w);
+            SC_CHECK_LAUNCH();
         } else if (hcap > 0) {
             size_t lds = (size_t)qpw * lane_bytes;
             hipLaunchKernelGGL(k_traverse, dim3((unsigned)sc_cdiv(cn, qpw)), dim3(qpw), lds,
@@ -2369,7 +2521,7 @@ static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
         const size_t lds = sizeof(ScanShared) +
                            sizeof(float) * std::max<size_t>((size_t)ix->mp * ix->kp,
                                                             p->form == SC_FORM_REAL ? ix->dim : 0);
-        if (hcap > 0 || three) {
+        if (hcap_eff > 0 || three) {
             hipLaunchKernelGGL(k_scan_topk, dim3((unsigned)cn), dim3(kScanThreads), lds,
                                ctx->stream, sa);
             SC_CHECK_LAUNCH();
@@ -2378,10 +2530,10 @@ static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
         // Queries whose heap outgrew the on-chip capacity are traversed again
         // with a global-memory heap of the reference's capacity and scanned
         // again; the first scan's output for them is overwritten.
-        if (!three && hcap < hcap_ref) {
+        if (!three && hcap_eff < hcap_ref) {
             int32_t nfb = 0;
             int32_t *dn = list + P.chunk;
-            if (hcap > 0) {
+            if (hcap_eff > 0) {
                 SC_HIP(hipMemsetAsync(dn, 0, 4, ctx->stream));
                 hipLaunchKernelGGL(k_gather_fallback, dim3((unsigned)sc_cdiv(cn, 256)), dim3(256),
                                    0, ctx->stream, qinfo, cn, list, dn);
@@ -2399,7 +2551,7 @@ static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
                     return SC_ERR_NOMEM;
                 }
                 TravArgs fa = ta;
-                fa.qlist = hcap > 0 ? list : nullptr;
+                fa.qlist = hcap_eff > 0 ? list : nullptr;
                 fa.nq = nfb;
                 fa.gheap_v = (float *)g;
                 fa.gheap_p = (uint32_t *)((float *)g + (size_t)nfb * hcap_ref);
@@ -2408,7 +2560,7 @@ static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
                                    ctx->stream, fa, hcap_ref, 64);
                 hipError_t le = hipGetLastError();
                 ScanArgs fs = sa;
-                fs.qlist = hcap > 0 ? list : nullptr;
+                fs.qlist = hcap_eff > 0 ? list : nullptr;
                 if (le == hipSuccess) {
                     hipLaunchKernelGGL(k_scan_topk, dim3((unsigned)nfb), dim3(kScanThreads), lds,
                                        ctx->stream, fs);

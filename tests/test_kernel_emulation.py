@@ -83,3 +83,38 @@ def test_traverse3_kernel_emulation(trav3, name, args):
     assert "ERROR: AddressSanitizer" not in log, log[-4000:]
     assert r.returncode == 0, log[-4000:]
     assert ("OK: 0 of %s queries differ" % args.split()[4]) in log, log[-4000:]
+
+
+TRAVW_CASES = [  # kind n kc w T nq [hcap]
+    ("imi", "1 8000 32 300 1500 8"),
+    ("imi heap capacity 5: inserts dropped", "1 8000 32 5 100000 8"),
+    ("imi whole 8x8 grid", "1 3000 8 1000 100000 6"),
+    ("sc", "2 8000 32 300 1500 8"),
+    ("sc cell budget binds", "2 8000 32 4 100000 8"),
+    ("imi on-chip heap 3: flagged", "1 8000 32 300 1500 8 3"),
+    ("sc on-chip heap 4: flagged", "2 8000 32 300 1500 8 4"),
+]
+
+
+@pytest.fixture(scope="module", params=["thread", "address"])
+def travw(request, tmp_path_factory):
+    out = str(tmp_path_factory.mktemp("emuw") / ("travw_" + request.param))
+    make_emu.build(out, request.param, CXX, travw=True)
+    return request.param, out
+
+
+@pytest.mark.parametrize("name,args", TRAVW_CASES, ids=[c[0] for c in TRAVW_CASES])
+def test_traverse_wave_kernel_emulation(travw, name, args):
+    """The one-workgroup-per-query traversal kernel (64 host threads per
+    workgroup) under ThreadSanitizer and AddressSanitizer, against the
+    oracle's traced search; queries whose heap outgrows the on-chip capacity
+    must be flagged, never wrong."""
+    san, binary = travw
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0", TSAN_OPTIONS="report_signal_unsafe=0")
+    r = subprocess.run([binary] + args.split(), capture_output=True, text=True, timeout=600,
+                       env=env)
+    log = r.stdout + r.stderr
+    assert "WARNING: ThreadSanitizer" not in log, log[-4000:]
+    assert "ERROR: AddressSanitizer" not in log, log[-4000:]
+    assert r.returncode == 0, log[-4000:]
+    assert ("OK: 0 of %s queries differ" % args.split()[5]) in log, log[-4000:]
