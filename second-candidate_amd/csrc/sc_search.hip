@@ -396,19 +396,34 @@ __global__ void k_traverse(TravArgs a, int hcap, int qpw) {
 // kSpec lanes at once, and thread 0 commits the pops in order, stopping
 // exactly where the reference does.  A heap outgrowing hcap is flagged
 // (qinfo[5]) for the global-heap lane kernel, as in k_traverse.
-// LDS: 5 kc ints/floats + hcap (score, pair) entries.
+// LDS: per list kc scores (float) and centroids (u16), kc row lengths (u16),
+// hcap (score, pair) entries -- small, so that many queries are in flight.
 constexpr int kTravThreads = 64;
+
+struct TravWaveLayout {
+    size_t st, rowlen, hv, hp, total;
+};
+__host__ __device__ inline TravWaveLayout trav_wave_layout(int lists, int kc, int hcap) {
+    TravWaveLayout L;
+    L.st = (size_t)lists * kc * 4;
+    L.rowlen = L.st + (size_t)lists * kc * 2;
+    L.hv = (L.rowlen + (size_t)kc * 2 + 7) & ~(size_t)7;
+    L.hp = L.hv + (size_t)hcap * 4;
+    L.total = L.hp + (size_t)hcap * 4;
+    return L;
+}
 
 __global__ __launch_bounds__(kTravThreads) void k_traverse_wave(TravArgs a, int hcap) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int kc = a.kc, tid = threadIdx.x;
     const bool imi = a.kind == SC_KIND_IMI;
     const int lists = imi ? 2 : 1;
-    float *sv = reinterpret_cast<float *>(smem);                   // [lists][kc]
-    int32_t *st = reinterpret_cast<int32_t *>(sv + lists * kc);    // [lists][kc]
-    int32_t *rowlen = st + lists * kc;                              // [kc]
-    float *hv = reinterpret_cast<float *>(rowlen + kc);             // [hcap]
-    uint32_t *hp = reinterpret_cast<uint32_t *>(hv + hcap);         // [hcap]
+    const TravWaveLayout lay = trav_wave_layout(lists, kc, hcap);
+    float *sv = reinterpret_cast<float *>(smem);                       // [lists][kc]
+    uint16_t *st = reinterpret_cast<uint16_t *>(smem + lay.st);       // [lists][kc]
+    uint16_t *rowlen = reinterpret_cast<uint16_t *>(smem + lay.rowlen);   // [kc]
+    float *hv = reinterpret_cast<float *>(smem + lay.hv);             // [hcap]
+    uint32_t *hp = reinterpret_cast<uint32_t *>(smem + lay.hp);       // [hcap]
     __shared__ int64_t s_bid[kSpec];
     __shared__ uint32_t s_info[kSpec];
     __shared__ int32_t s_len[kSpec];
@@ -419,13 +434,13 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse_wave(TravArgs a, int 
     const int64_t q = a.qlist ? a.qlist[slot] : slot;
     for (int i = tid; i < lists * kc; i += kTravThreads) {
         sv[i] = a.sv[(size_t)q * lists * kc + i];
-        st[i] = a.si[(size_t)q * lists * kc + i];
+        st[i] = (uint16_t)a.si[(size_t)q * lists * kc + i];
     }
     for (int i = tid; i < kc; i += kTravThreads) rowlen[i] = 0;
     if (tid == 0) s_done = 0;
     __syncthreads();
     const float *v1 = sv, *v2 = imi ? sv + kc : sv;
-    const int32_t *t1 = st, *t2 = imi ? st + kc : st;
+    const uint16_t *t1 = st, *t2 = imi ? st + kc : st;
 
     int32_t *cells = a.cells + (size_t)q * a.ccap;
     int32_t *clen = a.clen + (size_t)q * a.ccap;
@@ -467,8 +482,8 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse_wave(TravArgs a, int 
                 s_bid[n] = imi ? bid : (bid > 0 ? bid : -1);   // SC: cell 0 counts as empty
                 s_info[n] = (uint32_t)c1 | ((uint32_t)c2 << 16);
                 ++n;
-                if (rowlen[h1] != h2) err = 3;           // not a down-set: cannot happen
-                rowlen[h1] = h2 + 1;
+                if ((int)rowlen[h1] != h2) err = 3;      // not a down-set: cannot happen
+                rowlen[h1] = (uint16_t)(h2 + 1);
                 if (h1 < kc - 1 && (h2 == 0 || (h2 - 1) < rowlen[h1 + 1]))
                     h.insert(v1[h1 + 1] + v2[h2], ((uint32_t)(h1 + 1) << 16) | (uint32_t)h2);
                 if (h2 < kc - 1 && (h1 == 0 || (h2 + 1) < rowlen[h1 - 1]))
@@ -2458,12 +2473,11 @@ static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
     int qpw = (int)std::min<size_t>(64, (48 * 1024) / lane_bytes);
     if (qpw < 1) { qpw = 1; hcap = 0; }   // row lengths alone do not fit: global heap
     // IMI and SC: one workgroup per query (ranked lists, row lengths and a heap
-    // of up to 1024 entries in at most 32 KiB of LDS) when the lists fit
-    const size_t wave_fixed = (size_t)lists * kc * 8 + (size_t)kc * 4;
+    // of up to 256 entries in LDS; a larger frontier is redone by the
+    // global-heap lane kernel) when the lists take at most 24 KiB
     int hcap_w = 0;
-    if (kind != SC_KIND_IVFADC && !three && wave_fixed + 64 * 8 <= 32 * 1024) {
-        hcap_w = (int)std::min<size_t>(std::min<size_t>((size_t)hcap_ref, 1024),
-                                       (32 * 1024 - wave_fixed) / 8);
+    if (kind != SC_KIND_IVFADC && !three && trav_wave_layout(lists, kc, 0).total <= 24 * 1024) {
+        hcap_w = std::min(hcap_ref, 256);
         if (const char *e = getenv("SC_HEAP_ONCHIP")) {
             int v = atoi(e);
             if (v >= 1) hcap_w = std::min(hcap_w, v);
@@ -2502,7 +2516,7 @@ static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
         if (three) {
             SC_TRY(traverse3(ctx, p, kc, sv, si, ix->L, cells, clen, cinfo, qinfo, P.ccap, cn));
         } else if (wave) {
-            const size_t lds = wave_fixed + (size_t)hcap_w * 8;
+            const size_t lds = trav_wave_layout(lists, kc, hcap_w).total;
             hipLaunchKernelGGL(k_traverse_wave, dim3((unsigned)cn), dim3(kTravThreads), lds,
                                ctx->stream, ta, hcap_w);
             SC_CHECK_LAUNCH();
