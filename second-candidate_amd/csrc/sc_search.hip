@@ -824,7 +824,9 @@ __device__ void scan_run_generic(const ScanArgs &a, ScanShared &S, const float *
     const int mp = a.mp, kp = a.kp, kc = a.kc, half = mp >> 1;
     const bool imi = a.kind == SC_KIND_IMI;
     const int bs = kp * mp / a.mc;
-    int cur = 0;
+    int cur = 0, held = -1;       // the cell whose start term and table rows are held
+    float d0 = 0.0f;
+    const float *r1 = nullptr, *r2 = nullptr;
     for (int p0 = 0; p0 < rtotal; p0 += kRound) {
         const bool full = S.nb > kBufCap - kRound;   // read by all before any append
         __syncthreads();
@@ -834,17 +836,23 @@ __device__ void scan_run_generic(const ScanArgs &a, ScanShared &S, const float *
             if (p >= rtotal) continue;
             while (cur + 1 < rn && S.c_pref[cur + 1] <= p) ++cur;
             const int vidx = S.c_start[cur] + (p - S.c_pref[cur]);
+            if (cur != held) {      // a thread's positions ascend: once per cell it meets
+                const uint32_t ci = S.c_info[cur];
+                const int h3 = (int)(ci & 0xFFFFu), h4 = (int)(ci >> 16);
+                if (!imi) {
+                    d0 = qsum + dqc[h3];
+                } else {
+                    d0 = (qsum + dqc[h3]) + dqc[kc + h4];
+                    r1 = a.dot_cr + (size_t)h3 * bs;
+                    r2 = a.dot_cr + (size_t)(h4 + kc - 1) * bs;
+                }
+                held = cur;
+            }
             const uint8_t *cc = a.codes + (size_t)vidx * mp;
-            const uint32_t ci = S.c_info[cur];
-            float d;
+            float d = d0;
             if (!imi) {
-                d = qsum + dqc[ci & 0xFFFFu];
                 for (int k = 0; k < mp; ++k) d += lut[k * kp + cc[k]];
             } else {
-                const int h3 = (int)(ci & 0xFFFFu), h4 = (int)(ci >> 16);
-                d = (qsum + dqc[h3]) + dqc[kc + h4];
-                const float *r1 = a.dot_cr + (size_t)h3 * bs;
-                const float *r2 = a.dot_cr + (size_t)(h4 + kc - 1) * bs;
                 for (int k = 0; k < half; ++k) {
                     int b = k * kp + cc[k];
                     d += lut[b];
@@ -1014,7 +1022,11 @@ __device__ void scan_query(const ScanArgs &a, ScanShared &S, float *lut, int64_t
     if (a.sums && tid == 0) a.sums[q] = sum;
 }
 
-__global__ __launch_bounds__(kScanThreads) void k_scan_topk(ScanArgs a) {
+// Four waves per SIMD: the register budget that lets the four workgroups per
+// CU the LDS allows run together (unconstrained, the compiler's allocation
+// holds it to two; a few cold-path values spill).
+__global__ __launch_bounds__(kScanThreads) __attribute__((amdgpu_waves_per_eu(4, 8)))
+void k_scan_topk(ScanArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     ScanShared &S = *reinterpret_cast<ScanShared *>(smem);
     float *lut = reinterpret_cast<float *>(smem + sizeof(ScanShared));   // [mp*kp]
