@@ -395,10 +395,13 @@ __global__ void k_traverse(TravArgs a, int hcap, int qpw) {
 // the cell sizes until the loop stops); the batch's cell sizes are loaded by
 // kSpec lanes at once, and thread 0 commits the pops in order, stopping
 // exactly where the reference does.  A heap outgrowing hcap is flagged
-// (qinfo[5]) for the global-heap lane kernel, as in k_traverse.
-// LDS: per list kc scores (float) and centroids (u16), kc row lengths (u16),
+// (qinfo[5]) for the global-heap lane kernel, as in k_traverse, and so is a
+// query whose traversal reaches rank kTravRanks of a list: only the first
+// kTravRanks ranks of each list are staged.
+// LDS: per list kr scores (float) and centroids (u16), kr row lengths (u16),
 // hcap (score, pair) entries -- small, so that many queries are in flight.
 constexpr int kTravThreads = 64;
+constexpr int kTravRanks = 256;
 
 struct TravWaveLayout {
     size_t st, rowlen, hv, hp, total;
@@ -416,12 +419,13 @@ __host__ __device__ inline TravWaveLayout trav_wave_layout(int lists, int kc, in
 __global__ __launch_bounds__(kTravThreads) void k_traverse_wave(TravArgs a, int hcap) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int kc = a.kc, tid = threadIdx.x;
+    const int kr = min(kc, kTravRanks);              // ranks staged per list
     const bool imi = a.kind == SC_KIND_IMI;
     const int lists = imi ? 2 : 1;
-    const TravWaveLayout lay = trav_wave_layout(lists, kc, hcap);
-    float *sv = reinterpret_cast<float *>(smem);                       // [lists][kc]
-    uint16_t *st = reinterpret_cast<uint16_t *>(smem + lay.st);       // [lists][kc]
-    uint16_t *rowlen = reinterpret_cast<uint16_t *>(smem + lay.rowlen);   // [kc]
+    const TravWaveLayout lay = trav_wave_layout(lists, kr, hcap);
+    float *sv = reinterpret_cast<float *>(smem);                       // [lists][kr]
+    uint16_t *st = reinterpret_cast<uint16_t *>(smem + lay.st);       // [lists][kr]
+    uint16_t *rowlen = reinterpret_cast<uint16_t *>(smem + lay.rowlen);   // [kr]
     float *hv = reinterpret_cast<float *>(smem + lay.hv);             // [hcap]
     uint32_t *hp = reinterpret_cast<uint32_t *>(smem + lay.hp);       // [hcap]
     __shared__ int64_t s_bid[kSpec];
@@ -432,15 +436,16 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse_wave(TravArgs a, int 
     const int64_t slot = blockIdx.x;
     if (slot >= a.nq) return;
     const int64_t q = a.qlist ? a.qlist[slot] : slot;
-    for (int i = tid; i < lists * kc; i += kTravThreads) {
-        sv[i] = a.sv[(size_t)q * lists * kc + i];
-        st[i] = (uint16_t)a.si[(size_t)q * lists * kc + i];
+    for (int i = tid; i < lists * kr; i += kTravThreads) {
+        const int l = i / kr, r = i - l * kr;
+        sv[i] = a.sv[((size_t)q * lists + l) * kc + r];
+        st[i] = (uint16_t)a.si[((size_t)q * lists + l) * kc + r];
     }
-    for (int i = tid; i < kc; i += kTravThreads) rowlen[i] = 0;
+    for (int i = tid; i < kr; i += kTravThreads) rowlen[i] = 0;
     if (tid == 0) s_done = 0;
     __syncthreads();
-    const float *v1 = sv, *v2 = imi ? sv + kc : sv;
-    const uint16_t *t1 = st, *t2 = imi ? st + kc : st;
+    const float *v1 = sv, *v2 = imi ? sv + kr : sv;
+    const uint16_t *t1 = st, *t2 = imi ? st + kr : st;
 
     int32_t *cells = a.cells + (size_t)q * a.ccap;
     int32_t *clen = a.clen + (size_t)q * a.ccap;
@@ -484,10 +489,17 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse_wave(TravArgs a, int 
                 ++n;
                 if ((int)rowlen[h1] != h2) err = 3;      // not a down-set: cannot happen
                 rowlen[h1] = (uint16_t)(h2 + 1);
-                if (h1 < kc - 1 && (h2 == 0 || (h2 - 1) < rowlen[h1 + 1]))
-                    h.insert(v1[h1 + 1] + v2[h2], ((uint32_t)(h1 + 1) << 16) | (uint32_t)h2);
-                if (h2 < kc - 1 && (h1 == 0 || (h2 + 1) < rowlen[h1 - 1]))
-                    h.insert(v1[h1] + v2[h2 + 1], ((uint32_t)h1 << 16) | (uint32_t)(h2 + 1));
+                // a push past the staged ranks cannot be replayed here: it is
+                // treated like a heap overflow (the pops after it are invalid)
+                if (h1 < kc - 1) {
+                    if (h1 + 1 >= kr) h.overflow = true;
+                    else if (h2 == 0 || (h2 - 1) < rowlen[h1 + 1])
+                        h.insert(v1[h1 + 1] + v2[h2], ((uint32_t)(h1 + 1) << 16) | (uint32_t)h2);
+                }
+                if (h2 < kc - 1 && (h1 == 0 || (h2 + 1) < rowlen[h1 - 1])) {
+                    if (h2 + 1 >= kr) h.overflow = true;
+                    else h.insert(v1[h1] + v2[h2 + 1], ((uint32_t)h1 << 16) | (uint32_t)(h2 + 1));
+                }
             }
             s_n = n;
         }
@@ -2484,11 +2496,12 @@ static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
     const size_t lane_bytes = (size_t)hcap * 8 + (size_t)kc * 4;
     int qpw = (int)std::min<size_t>(64, (48 * 1024) / lane_bytes);
     if (qpw < 1) { qpw = 1; hcap = 0; }   // row lengths alone do not fit: global heap
-    // IMI and SC: one workgroup per query (ranked lists, row lengths and a heap
-    // of up to 256 entries in LDS; a larger frontier is redone by the
-    // global-heap lane kernel) when the lists take at most 24 KiB
+    // IMI and SC: one workgroup per query (the first 256 ranks of each list,
+    // their row lengths and a heap of up to 256 entries in LDS; a query that
+    // needs more is redone by the global-heap lane kernel)
     int hcap_w = 0;
-    if (kind != SC_KIND_IVFADC && !three && trav_wave_layout(lists, kc, 0).total <= 24 * 1024) {
+    const int kr = std::min(kc, kTravRanks);
+    if (kind != SC_KIND_IVFADC && !three && kc <= 65535) {
         hcap_w = std::min(hcap_ref, 256);
         if (const char *e = getenv("SC_HEAP_ONCHIP")) {
             int v = atoi(e);
@@ -2514,9 +2527,16 @@ static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
         const float *Q = d_q + (size_t)q0 * ix->dim;
         const int64_t lut_total = cn * (ix->mc * kc + ix->mp * ix->kp);
         mark();
-        hipLaunchKernelGGL(k_lut, dim3((unsigned)sc_cdiv(lut_total, 256)), dim3(256), 0,
-                           ctx->stream, Q, cn, ix->dim, ix->cqT, ix->pqT, ix->norm_c, ix->norm_r,
-                           ix->mc, kc, ix->mp, ix->kp, dqc, dqr);
+        if (kc % kLutC == 0 && ix->kp % kLutC == 0)   // same values, tiled
+            hipLaunchKernelGGL(k_lut_tiled,
+                               dim3((unsigned)sc_cdiv(cn, kLutQ),
+                                    (unsigned)((ix->mc * kc + ix->mp * ix->kp) / kLutC)),
+                               dim3(256), 0, ctx->stream, Q, cn, ix->dim, ix->cqT, ix->pqT,
+                               ix->norm_c, ix->norm_r, ix->mc, kc, ix->mp, ix->kp, dqc, dqr);
+        else
+            hipLaunchKernelGGL(k_lut, dim3((unsigned)sc_cdiv(lut_total, 256)), dim3(256), 0,
+                               ctx->stream, Q, cn, ix->dim, ix->cqT, ix->pqT, ix->norm_c,
+                               ix->norm_r, ix->mc, kc, ix->mp, ix->kp, dqc, dqr);
         SC_CHECK_LAUNCH();
         mark();
         hipLaunchKernelGGL(k_rank, dim3((unsigned)cn), dim3(256), (size_t)npow2 * 8, ctx->stream,
@@ -2528,7 +2548,7 @@ static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
         if (three) {
             SC_TRY(traverse3(ctx, p, kc, sv, si, ix->L, cells, clen, cinfo, qinfo, P.ccap, cn));
         } else if (wave) {
-            const size_t lds = trav_wave_layout(lists, kc, hcap_w).total;
+            const size_t lds = trav_wave_layout(lists, kr, hcap_w).total;
             hipLaunchKernelGGL(k_traverse_wave, dim3((unsigned)cn), dim3(kTravThreads), lds,
                                ctx->stream, ta, hcap_w);
             SC_CHECK_LAUNCH();

@@ -340,6 +340,18 @@ def test_search_cell_list_overflow_path(ctx, monkeypatch):
     ix.close()
 
 
+@pytest.mark.parametrize("kind", [S.KIND_IMI, S.KIND_SC])
+def test_search_past_staged_ranks(ctx, kind, monkeypatch):
+    """kc = 300: the one-workgroup traversal of the staged kernels stages 256
+    ranks per list; the queries that walk the whole grid go past them and are
+    redone by the global-heap kernel; every result must equal the oracle's."""
+    monkeypatch.setenv("SC_NO_FUSED", "1")          # SC through the staged kernels too
+    ix, oix = _index_pair(ctx, kind, n=4000, kc=300)
+    _compare_search(ctx, ix, oix, kind, 50, 100000, 3000, nq=24)
+    _compare_search(ctx, ix, oix, kind, 50, 100000, 100000, nq=6)
+    ix.close()
+
+
 def test_search_exhausted_grid(ctx):
     ix, oix = _index_pair(ctx, S.KIND_SC, n=300, kc=4)
     st = _compare_search(ctx, ix, oix, S.KIND_SC, 400, 1000, 10 ** 6, nq=20)

@@ -93,6 +93,8 @@ TRAVW_CASES = [  # kind n kc w T nq [hcap]
     ("sc cell budget binds", "2 8000 32 4 100000 8"),
     ("imi on-chip heap 3: flagged", "1 8000 32 300 1500 8 3"),
     ("sc on-chip heap 4: flagged", "2 8000 32 300 1500 8 4"),
+    ("imi kc 300, within the staged ranks", "1 8000 300 300 1500 4"),
+    ("imi kc 300, past the staged ranks: flagged", "1 3000 300 100000 100000 2"),
 ]
 
 
