@@ -506,3 +506,54 @@ def test_full_size_imi_search(ctx):
     assert np.array_equal(ids.numpy(), oids)
     assert np.array_equal(_bits(dists.numpy()), _bits(od))
     ix.close()
+
+
+def test_large_k_sc_index_mixed_batch_sizes(ctx):
+    """K=4096 (nearest, second) cells over D=128 (the large dense-partition
+    shape, here with 1M base vectors): BF16 and FP32 encodes identical,
+    encode on sampled rows vs the oracle, and searches served in batches of
+    1, 64 and 4096 queries (eager calls and a hipGraph plan) vs the oracle."""
+    n, dim, kc, mp, kp = 1_000_000, 128, 4096, 8, 256
+    mix = S.mix_d128()
+    xt = S.gen_rows(ctx, mix, S.STREAM_TRAIN, 0, 100_000, dim)
+    cq, _ = S.kmeans(ctx, xt, kc, 3, seed=5, want_labels=False)
+    S.residual(ctx, xt, cq, 1, kc, 0)
+    rh = xt.numpy()
+    pq = np.stack([S.kmeans(ctx, S.Dev.from_numpy(ctx, np.ascontiguousarray(rh[:, s * 16:(s + 1) * 16])),
+                            kp, 3, seed=31 + s, want_labels=False)[0].numpy() for s in range(mp)])
+    dpq = S.Dev.from_numpy(ctx, pq)
+    x = S.gen_rows(ctx, mix, S.STREAM_BASE, 0, n, dim)
+    cid, codes, _ = S.encode(ctx, x, cq, dpq, 1, kc, mp, kp, 2, S.PREC_BF16)
+    fcid, fcodes, _ = S.encode(ctx, x, cq, dpq, 1, kc, mp, kp, 2, S.PREC_FP32)
+    cidh, codesh = cid.numpy(), codes.numpy()
+    assert np.array_equal(cidh, fcid.numpy()) and np.array_equal(codesh, fcodes.numpy())
+    del fcid, fcodes
+    rows = np.random.default_rng(2).choice(n, 600, replace=False)
+    xr = np.concatenate([sco.gen(sco.mix_d128(), sco.STREAM_BASE, int(r), 1, dim) for r in rows])
+    ocid, ocodes = sco.encode(xr, cq.numpy(), pq, 1, kc, mp, kp, 2)
+    assert np.array_equal(cidh[rows], ocid) and np.array_equal(codesh[rows], ocodes)
+    ix = S.Index.build(ctx, cid, codes, dim, cq, 1, kc, dpq, mp, kp, 2)
+    e = ix.export()
+    assert e["L"].size == kc * kc and e["L"][-1] == n
+    oix = sco.OracleIndex(dim, 1, kc, mp, kp, cq.numpy(), pq, e["L"], e["pid"], e["codes"])
+    q = sco.gen(sco.mix_d128(), sco.STREAM_QUERY, 0, 4096, dim)
+    chk = np.random.default_rng(3).choice(4096, 40, replace=False)
+    oids, od, osum = oix.search_batch(sco.SC, q[chk], 100, 100000, 10000)
+    for nb in (1, 64, 4096):
+        starts = sorted({int(c) // nb * nb for c in (chk[:8] if nb == 1 else chk)})
+        for start in starts:
+            dq = S.Dev.from_numpy(ctx, np.ascontiguousarray(q[start:start + nb]))
+            ids, dists, sums, _ = ix.search(dq, S.KIND_SC, 100, 100000, 10000, stats=False)
+            gi, gd, gs = ids.numpy(), dists.numpy(), sums.numpy()
+            for k, c in enumerate(chk):
+                if start <= c < start + nb:
+                    assert gs[c - start] == osum[k]
+                    assert np.array_equal(gi[c - start], oids[k])
+                    assert np.array_equal(_bits(gd[c - start]), _bits(od[k]))
+    plan = S.Plan(ix, S.Dev.from_numpy(ctx, q), S.KIND_SC, 100, 100000, 10000)
+    plan.autotune()
+    plan.execute(stats=False)
+    pi, pd = plan.ids.numpy(), plan.dists.numpy()
+    assert np.array_equal(pi[chk], oids) and np.array_equal(_bits(pd[chk]), _bits(od))
+    plan.close()
+    ix.close()
