@@ -1076,6 +1076,7 @@ struct FusedArgs {
     unsigned long long *phase;        // [6]: wall-clock ticks per phase, workgroups
     float margin_scale;               // mixed form: multiplies the error bound (1)
     int nvec;                         // vectors in the index
+    int pop_budget;                   // SC: cells popped before a query is handed on
 };
 
 struct FusedLayout {
@@ -1742,6 +1743,9 @@ void k_search_fused(FusedArgs a) {
                         }
                         count2++;
                         if (count >= a.w || sum >= a.T) done = true;
+                        // a traversal this long (mostly empty cells) runs faster in
+                        // the staged kernels, which keep more queries in flight
+                        else if (count2 >= a.pop_budget) { flag = 1; done = true; }
                     }
                     if (done) break;
                     if (h.overflow) break;                 // flagged below
@@ -2157,6 +2161,19 @@ static int finish_flagged(sc_ctx *ctx, const sc_index *ix, const sc_search_param
                           const float *d_q, int64_t nq, const int32_t *flags, int nflag,
                           int32_t *d_ids, float *d_dists, int32_t *d_sums, sc_search_stats *st);
 
+// SC traversals that pop this many cells leave the fused kernel for the
+// staged kernels (SC_POP_BUDGET overrides it; a test hook).  Typical queries
+// pop tens of cells; a query that must walk far through empty cells pops
+// thousands, and the staged traversal keeps about six times more of them in
+// flight.
+static int fused_pop_budget() {
+    if (const char *e = getenv("SC_POP_BUDGET")) {
+        int v = atoi(e);
+        if (v >= 1) return v;
+    }
+    return 256;
+}
+
 // Mixed form: factor on its error bound (test hook; 1 otherwise).
 static float mixed_margin_scale() {
     if (const char *e = getenv("SC_MIXED_MARGIN")) {
@@ -2267,6 +2284,7 @@ static int search_fused(sc_ctx *ctx, const sc_index *ix, const float *d_q, int64
     fa.phase = st ? stats + 8 : nullptr;     // 6 counters after the 8 statistics
     fa.margin_scale = mixed_margin_scale();
     fa.nvec = (int)ix->n;
+    fa.pop_budget = fused_pop_budget();
     if (st) SC_HIP(hipMemsetAsync(stats + 8, 0, 48, sm));
     hipEvent_t e0 = nullptr, e1 = nullptr, el = nullptr;
     if (st) {
@@ -2752,6 +2770,7 @@ extern "C" int sc_plan_create(sc_ctx *ctx, const sc_index *ix, const sc_search_p
     fa.phase = P->stats + 8;
     fa.margin_scale = mixed_margin_scale();
     fa.nvec = (int)ix->n;
+    fa.pop_budget = fused_pop_budget();
     P->mixed = p->form == SC_FORM_MIXED;
     for (int v = 3; v <= 5; ++v)
         if (hipFuncSetAttribute((const void *)fused_kernel(v, P->mixed),

@@ -262,9 +262,11 @@ def test_search_sc3_matches_oracle(ctx, case):
 
 @pytest.mark.parametrize("case", [c for c in SEARCH_CASES if c[0] != S.KIND_IMI])
 @pytest.mark.parametrize("kc,kp", [(16, 64), (64, 64)])
-def test_search_mixed_form_matches_oracle(ctx, case, kc, kp):
+def test_search_mixed_form_matches_oracle(ctx, case, kc, kp, monkeypatch):
     """SC_FORM_MIXED (FP16 table scan + exact re-rank) returns the plain form's
-    ids and distances bit for bit; kc = kp = 64 takes the precomputed-LUT path."""
+    ids and distances bit for bit; kc = kp = 64 takes the precomputed-LUT path.
+    The pop budget is lifted so that long traversals stay in the fused kernel."""
+    monkeypatch.setenv("SC_POP_BUDGET", "1000000")
     kind, R, w, T = case
     ix, oix = _index_pair(ctx, kind, kc=kc, kp=kp)
     st = _compare_search(ctx, ix, oix, kind, R, w, T, form=S.FORM_MIXED)
@@ -349,6 +351,18 @@ def test_search_past_staged_ranks(ctx, kind, monkeypatch):
     ix, oix = _index_pair(ctx, kind, n=4000, kc=300)
     _compare_search(ctx, ix, oix, kind, 50, 100000, 3000, nq=24)
     _compare_search(ctx, ix, oix, kind, 50, 100000, 100000, nq=6)
+    ix.close()
+
+
+@pytest.mark.parametrize("form", [S.FORM_PLAIN, S.FORM_MIXED])
+def test_search_pop_budget_handoff(ctx, monkeypatch, form):
+    """SC queries whose traversal pops more cells than the fused kernel's
+    budget are finished by the staged kernels; a budget of 3 hands most of
+    them on, and every result must still equal the oracle's."""
+    monkeypatch.setenv("SC_POP_BUDGET", "3")
+    ix, oix = _index_pair(ctx, S.KIND_SC)
+    st = _compare_search(ctx, ix, oix, S.KIND_SC, 100, 500, 3000, form=form)
+    assert st.visited > 3 * 150 * 0.5       # most queries popped past the budget
     ix.close()
 
 

@@ -32,6 +32,7 @@ CASES = [  # kind n kc R w T nq threads trained precomputed_lut mixed [margin nc
     ("sc plain, 512 threads", "2 20000 32 100 300 1500 3 512 0 0 0"),
     ("sc plain, traversal past the rank window", "2 20000 64 100 3000 6000 3 256 1 0 0"),
     ("drop-in data: 16-component mixture, 40 queries", "2 20000 32 100 300 1500 40 256 1 0 0 1 16"),
+    ("sc plain, pop budget 5: long traversals handed on", "2 20000 32 100 300 1500 4 256 1 0 0 1 64 5"),
 ]
 
 
