@@ -52,7 +52,7 @@ def main():
                          "(0 = no recall)")
     ap.add_argument("--plan", type=int, default=1,
                     help="1: time the autotuned hipGraph plan (sc_plan_*); 0: sc_search calls")
-    ap.add_argument("--sweep", type=str, default="1000,2000,5000,10000,20000",
+    ap.add_argument("--sweep", type=str, default="1000,2000,5000,10000,15000,20000",
                     help="T values of the QPS/recall curve ('' = skip)")
     ap.add_argument("--mixed", type=int, default=1,
                     help="1: also time the mixed-precision form (FP16 table + exact re-rank)")

@@ -2163,7 +2163,8 @@ static int finish_flagged(sc_ctx *ctx, const sc_index *ix, const sc_search_param
 
 // SC traversals that pop this many cells leave the fused kernel for the
 // staged kernels (SC_POP_BUDGET overrides it; a test hook).  Typical queries
-// pop tens of cells; a query that must walk far through empty cells pops
+// pop tens of cells (at most ~80 on the 1M x 128, K=256 index up to 15k
+// candidates); a query that must walk far through empty cells pops
 // thousands, and the staged traversal keeps about six times more of them in
 // flight.
 static int fused_pop_budget() {
@@ -2171,7 +2172,7 @@ static int fused_pop_budget() {
         int v = atoi(e);
         if (v >= 1) return v;
     }
-    return 256;
+    return 128;
 }
 
 // Mixed form: factor on its error bound (test hook; 1 otherwise).
