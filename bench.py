@@ -58,6 +58,9 @@ def main():
                     help="1: also time the mixed-precision form (FP16 table + exact re-rank)")
     ap.add_argument("--real", type=int, default=1,
                     help="1: also time the real-distance form (exact distances from the raw vectors)")
+    ap.add_argument("--kmeans", type=int, default=1,
+                    help="1: also time k-means training at K=4096 on 1M x 64 (exact and mixed "
+                         "Lloyd assignment)")
     ap.add_argument("--imi", type=int, default=1,
                     help="1: also time the inverted multi-index workload (two 64-dim halves, "
                          "K=1024 each, PQ 8x256, same base and queries)")
@@ -368,6 +371,35 @@ def main():
                                       imi["recall_at_%d" % args.R]))
         iix.close()
 
+    # ---------------- k-means at K=4096 (the large-index training shape, on a 1M-point
+    # half-vector sample): seeding alone (iters=0), then 3 Lloyd iterations in the
+    # exact form and in the mixed form (BF16 MFMA assignment, exact re-check)
+    km = None
+    if args.kmeans:
+        xk = S.gen_rows(ctx, mix, S.STREAM_TRAIN, 0, 1_000_000, dim)
+        xh64 = S.Dev.from_numpy(ctx, np.ascontiguousarray(xk.numpy()[:, :64]))
+        del xk
+        def timed_kmeans(iters, prec):
+            t0 = time.perf_counter()
+            c, lab = S.kmeans(ctx, xh64, 4096, iters, seed=3, prec=prec)
+            ctx.sync()
+            return time.perf_counter() - t0, c, lab
+        timed_kmeans(0, S.PREC_FP32)                       # warm-up
+        t_seed = timed_kmeans(0, S.PREC_FP32)[0]
+        t_ex, c_ex, l_ex = timed_kmeans(3, S.PREC_FP32)
+        t_mx, c_mx, l_mx = timed_kmeans(3, S.PREC_BF16)
+        same = bool(np.array_equal(l_ex.numpy(), l_mx.numpy()) and
+                    np.array_equal(c_ex.numpy().view(np.uint32), c_mx.numpy().view(np.uint32)))
+        lloyd_flops = 3 * 2.0 * 1_000_000 * 4096 * 64
+        km = {"workload": "k-means, 1M points x 64 dims, K=4096, 3 Lloyd iterations",
+              "seeding_s": round(t_seed, 3),
+              "lloyd_fp32_s": round(t_ex - t_seed, 4), "lloyd_bf16_s": round(t_mx - t_seed, 4),
+              "identical": same,
+              "lloyd_bf16_algorithmic_tflops": round(lloyd_flops / max(t_mx - t_seed, 1e-9) / 1e12, 2)}
+        log("k-means K=4096: seeding %.3fs; 3 Lloyd iterations exact %.3fs, mixed %.3fs "
+            "(identical: %s)" % (t_seed, km["lloyd_fp32_s"], km["lloyd_bf16_s"], same))
+        del xh64, c_ex, l_ex, c_mx, l_mx
+
     # ---------------- CPU baseline: the OpenMP oracle on a bounded sample
     cpu = None
     if args.cpu_seconds > 0:
@@ -434,6 +466,7 @@ def main():
             "mixed_precision_batched": mixed,
             "real_distance": real,
             "imi_k1024": imi,
+            "kmeans_k4096": km,
             "candidates_per_query": round(cand / args.nq, 1),
             "stage_ms": {"lut": round(st0.ms_lut, 4), "rank": round(st0.ms_rank, 4),
                          "traverse": round(st0.ms_traverse, 4), "scan_topk": round(ms_kernel, 4)},

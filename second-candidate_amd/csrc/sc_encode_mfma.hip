@@ -147,11 +147,14 @@ __device__ __forceinline__ bool precedes(double d, int k, double e, int j) {
 
 // One wave: best one (TOP2=false) or two (TOP2=true) codewords for the 16
 // rows held in xs (row stride `ld`, columns [t0, t0 + blen)).  Writes
-// out1/out2 [16].  Wave-private LDS scratch: cnt [16], ck [16][kCand],
+// out1/out2 [16].  ROUNDF orders by the distance rounded to float (the
+// k-means labels, k-means.h:597-612): the candidate threshold then also
+// covers every codeword whose rounded distance can equal the best's
+// (within 2^-22 of it), and the re-check rounds before it compares.  Wave-private LDS scratch: cnt [16], ck [16][kCand],
 // cd [16][kCand], rown [16] (FP64 norms of shifted rows), sm [blen] (the
 // block's shift), wst [2] (the workgroup's re-checked candidates and fully
 // exact rows).
-template <bool TOP2, bool DESC>
+template <bool TOP2, bool DESC, bool ROUNDF = false>
 __device__ void wave_best(const float *xs, int ld, int t0, int nrows, const CbView cb,
                           int *cnt, int *ck, double *cd, double *rown, float *sm, int *out1,
                           int *out2, unsigned *wst) {
@@ -258,7 +261,9 @@ __device__ void wave_best(const float *xs, int ld, int t0, int nrows, const CbVi
             const float lo1 = fminf(m1[i], g1[i]), hi1 = fmaxf(m1[i], g1[i]);
             m2[i] = fminf(hi1, fminf(m2[i], g2[i]));
             m1[i] = lo1;
-            thr[i] = (float)((double)(TOP2 ? m2[i] : m1[i]) + 2.0 * E[i]);
+            const double mref = (double)(TOP2 ? m2[i] : m1[i]);
+            thr[i] = (float)(mref + 2.0 * E[i] +
+                             (ROUNDF ? (fabs(mref) + E[i]) * (1.0 / 4194304.0) : 0.0));
             // rounding the threshold up keeps it an upper bound
             thr[i] = nextafterf(thr[i], INFINITY);
         }
@@ -288,7 +293,8 @@ __device__ void wave_best(const float *xs, int ld, int t0, int nrows, const CbVi
             ++r;
         }
         const int k = ck[r * kCand + s];
-        cd[r * kCand + s] = s1(xs + r * ld + t0, cb.c + (size_t)k * blen, blen);
+        const double e = s1(xs + r * ld + t0, cb.c + (size_t)k * blen, blen);
+        cd[r * kCand + s] = ROUNDF ? (double)(float)e : e;
     }
     __syncthreads();
     if (l < nrows) {
@@ -303,7 +309,10 @@ __device__ void wave_best(const float *xs, int ld, int t0, int nrows, const CbVi
             for (int s = 0; s < c; ++s) push(cd[l * kCand + s], ck[l * kCand + s]);
             atomicAdd(&wst[0], (unsigned)c);
         } else {
-            for (int k = 0; k < K; ++k) push(s1(xs + l * ld + t0, cb.c + (size_t)k * blen, blen), k);
+            for (int k = 0; k < K; ++k) {
+                const double e = s1(xs + l * ld + t0, cb.c + (size_t)k * blen, blen);
+                push(ROUNDF ? (double)(float)e : e, k);
+            }
             atomicAdd(&wst[1], 1u);
         }
         out1[l] = k1;
@@ -396,6 +405,48 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8))) void
     }
 }
 
+// Lloyd assignment in the mixed form (SC_PREC_BF16 k-means): the label of
+// each point is its nearest centre by the distance rounded to float, the
+// later index on ties (k-means.h:597-612, 800-810), from BF16 MFMA distances
+// re-checked exactly -- the labels of the exact form.
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_assign_mfma(
+    const float *__restrict__ x, int64_t n, int d, const float *__restrict__ c, int k,
+    const uint4 *__restrict__ cfrag, const double *__restrict__ cn2,
+    const unsigned long long *__restrict__ cmax, const float *__restrict__ cmean,
+    int32_t *__restrict__ labels, unsigned long long *__restrict__ stats) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int ld = d + 1;
+    float *xs = reinterpret_cast<float *>(smem);                          // [16][d+1]
+    double *cd = reinterpret_cast<double *>(xs + kRows * ld + (kRows * ld) % 2);
+    double *rown = cd + kRows * kCand;
+    int *cnt = reinterpret_cast<int *>(rown + kRows);
+    int *ck = cnt + kRows;
+    int *o1 = ck + kRows * kCand;
+    float *sm = reinterpret_cast<float *>(o1 + 2 * kRows);               // [d]
+    const int l = threadIdx.x;
+    const int64_t v0 = (int64_t)blockIdx.x * kRows;
+    const int nr = (int)min((int64_t)kRows, n - v0);
+    const int d4 = d / 4;                    // d is a multiple of 16 here
+#pragma unroll 4
+    for (int e = l; e < kRows * d4; e += 64) {
+        const int r = e / d4, cc = e - r * d4;
+        const float4 v = r < nr ? reinterpret_cast<const float4 *>(x + (size_t)(v0 + r) * d)[cc]
+                                : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        float *dst = xs + r * ld + 4 * cc;
+        dst[0] = v.x; dst[1] = v.y; dst[2] = v.z; dst[3] = v.w;
+    }
+    __shared__ unsigned s_wst[2];
+    if (l == 0) { s_wst[0] = 0u; s_wst[1] = 0u; }
+    __syncthreads();
+    CbView cv{c, cfrag, cn2, __builtin_bit_cast(double, cmax[0]), cmean, k, d};
+    wave_best<false, true, true>(xs, ld, 0, nr, cv, cnt, ck, cd, rown, sm, o1, nullptr, s_wst);
+    if (l < nr) labels[v0 + l] = o1[l];
+    if (l == 0) {
+        if (s_wst[0]) atomicAdd(&stats[0], (unsigned long long)s_wst[0]);
+        if (s_wst[1]) atomicAdd(&stats[1], (unsigned long long)s_wst[1]);
+    }
+}
+
 int prep_codebook(sc_ctx *ctx, const float *cb, int nb, int K, int blen, int shift,
                   unsigned char *base, size_t &off, float **mean, uint4 **frag, double **cn2,
                   unsigned long long **cmax) {
@@ -418,6 +469,36 @@ int prep_codebook(sc_ctx *ctx, const float *cb, int nb, int K, int blen, int shi
 }
 
 }  // namespace
+
+// Scratch of kmeans_assign_bf16: the prepared centres and two statistics.
+size_t kmeans_bf16_scratch(int k, int d) {
+    float *mean;
+    uint4 *frag;
+    double *cn2;
+    unsigned long long *cmax;
+    size_t off = 0;
+    prep_codebook(nullptr, nullptr, 1, k, d, 1, nullptr, off, &mean, &frag, &cn2, &cmax);
+    return off + 256;
+}
+
+int kmeans_assign_bf16(sc_ctx *ctx, const float *x, int64_t n, int d, const float *c, int k,
+                       int32_t *labels, unsigned char *scratch) {
+    float *mean;
+    uint4 *frag;
+    double *cn2;
+    unsigned long long *cmax;
+    size_t off = 0;
+    SC_TRY(prep_codebook(ctx, c, 1, k, d, 1, scratch, off, &mean, &frag, &cn2, &cmax));
+    unsigned long long *stats = (unsigned long long *)(scratch + off);
+    SC_HIP(hipMemsetAsync(stats, 0, 16, ctx->stream));
+    const size_t lds = encode_lds(d);
+    SC_HIP(hipFuncSetAttribute((const void *)k_assign_mfma,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(k_assign_mfma, dim3((unsigned)sc_cdiv(n, kRows)), dim3(64), lds,
+                       ctx->stream, x, n, d, c, k, frag, cn2, cmax, mean, labels, stats);
+    SC_CHECK_LAUNCH();
+    return SC_OK;
+}
 
 int sc_encode_bf16(sc_ctx *ctx, const float *x, int64_t n, int dim, const float *cq, int mc,
                    int kc, const float *pq, int mp, int kp, int nc, uint16_t *cid,

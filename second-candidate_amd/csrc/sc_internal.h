@@ -54,6 +54,13 @@ void sc_set_error(const char *fmt, ...);
 // undefined).  Grows by reallocation after a stream synchronisation.
 int sc_scratch(sc_ctx *ctx, size_t bytes, void **ptr);
 
+// Mixed-form Lloyd assignment (sc_encode_mfma.hip): labels [n] of the points
+// x [n][d] (d a multiple of 16, at most 2048) against the centres c [k][d];
+// `scratch` holds kmeans_bf16_scratch(k, d) bytes of device memory.
+size_t kmeans_bf16_scratch(int k, int d);
+int kmeans_assign_bf16(sc_ctx *ctx, const float *x, int64_t n, int d, const float *c, int k,
+                       int32_t *labels, unsigned char *scratch);
+
 // Sets the context's device current on the calling thread.
 int sc_ctx_enter(sc_ctx *ctx);
 

@@ -173,6 +173,20 @@ __global__ __launch_bounds__((TV / 4) * (TK / 4)) void k_assign(
     }
 }
 
+// The Lloyd sums and counts of given labels (the mixed form assigns first):
+// FP64 sums, exact for integer data in any order, like k_assign's.
+__global__ void k_accumulate(const float *__restrict__ x, int64_t n, int d,
+                             const int32_t *__restrict__ labels, double *__restrict__ sums,
+                             int32_t *__restrict__ counts) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n * d) return;
+    const int64_t i = e / d;
+    const int t = (int)(e - i * d);
+    const int j = labels[i];
+    atomicAdd(&sums[(size_t)j * d + t], (double)x[e]);
+    if (t == 0) atomicAdd(&counts[j], 1);
+}
+
 // argmax over i of (float)S1(x_i, c[label_i]) (first index on ties):
 // per-block best packed as (orderable float key << 32 | ~index).
 __global__ void k_lonely(const float *__restrict__ x, int64_t n, int d, const float *__restrict__ c,
@@ -258,8 +272,10 @@ extern "C" int sc_kmeans(sc_ctx *ctx, const float *d_x, int64_t n, int32_t d, in
                      iters);
         return SC_ERR_ARG;
     }
-    if (prec != SC_PREC_FP32) {
-        sc_set_error("sc_kmeans: only SC_PREC_FP32 is available in this build");
+    const bool mixed = prec == SC_PREC_BF16;
+    if ((prec != SC_PREC_FP32 && !mixed) || (mixed && (d % 16 || d > 2048))) {
+        sc_set_error("sc_kmeans: precision %d not available for d=%d (SC_PREC_BF16 needs d a "
+                     "multiple of 16, at most 2048)", prec, d);
         return SC_ERR_ARG;
     }
     hipStream_t s = ctx->stream;
@@ -275,7 +291,8 @@ extern "C" int sc_kmeans(sc_ctx *ctx, const float *d_x, int64_t n, int32_t d, in
     size_t o_d2 = 0, o_bs = o_d2 + 8 * (size_t)n, o_pk = o_bs + 8 * (size_t)nb,
            o_sum = o_pk + 8 * (size_t)k, o_cnt = o_sum + 8 * (size_t)k * d,
            o_lab = o_cnt + 4 * (size_t)k + 8, o_best = o_lab + 4 * (size_t)n + 8,
-           total = o_best + 16;
+           o_mix = (o_best + 16 + 255) & ~(size_t)255,
+           total = o_mix + (mixed ? kmeans_bf16_scratch(k, d) : 0);
     unsigned char *S;
     SC_TRY(sc_scratch(ctx, total, (void **)&S));
     double *d2 = (double *)(S + o_d2), *sums = (double *)(S + o_sum);
@@ -311,8 +328,14 @@ extern "C" int sc_kmeans(sc_ctx *ctx, const float *d_x, int64_t n, int32_t d, in
     for (int it = 0; it < iters; ++it) {
         SC_HIP(hipMemsetAsync(sums, 0, 8 * (size_t)k * d, s));
         SC_HIP(hipMemsetAsync(counts, 0, 4 * (size_t)k, s));
-        hipLaunchKernelGGL(kern, dim3((unsigned)sc_cdiv(n, TV)), dim3((TV / 4) * (TK / 4)), lds, s,
-                           d_x, n, d, d_centers, k, labels, sums, counts);
+        if (mixed) {   // BF16 MFMA distances, exact re-check: the same labels
+            SC_TRY(kmeans_assign_bf16(ctx, d_x, n, d, d_centers, k, labels, S + o_mix));
+            hipLaunchKernelGGL(k_accumulate, dim3((unsigned)sc_cdiv(n * d, 256)), dim3(256), 0, s,
+                               d_x, n, d, labels, sums, counts);
+        } else {
+            hipLaunchKernelGGL(kern, dim3((unsigned)sc_cdiv(n, TV)), dim3((TV / 4) * (TK / 4)),
+                               lds, s, d_x, n, d, d_centers, k, labels, sums, counts);
+        }
         SC_CHECK_LAUNCH();
         SC_HIP(hipMemcpyAsync(hcnt.data(), counts, 4 * (size_t)k, hipMemcpyDeviceToHost, s));
         SC_HIP(hipStreamSynchronize(s));

@@ -397,10 +397,13 @@ def test_index_load_roundtrip(ctx):
 
 
 # ------------------------------------------------------------------ k-means
+@pytest.mark.parametrize("prec", [S.PREC_FP32, S.PREC_BF16])
 @pytest.mark.parametrize("lagged", [0, 1])
-def test_kmeans_matches_oracle_on_integer_data(ctx, lagged):
+def test_kmeans_matches_oracle_on_integer_data(ctx, lagged, prec):
+    """Labels and centres equal the oracle's, for the exact form and for the
+    mixed form (BF16 MFMA assignment distances, exact re-check)."""
     x = sco.gen(sco.mix_d128(ncomp=9), sco.STREAM_TRAIN, 0, 6000, 16)
-    c, lab = S.kmeans(ctx, S.Dev.from_numpy(ctx, x), 40, 6, seed=11, lagged=lagged)
+    c, lab = S.kmeans(ctx, S.Dev.from_numpy(ctx, x), 40, 6, seed=11, lagged=lagged, prec=prec)
     oc, olab = sco.kmeans(x, 40, 6, seed=11, lagged=lagged)
     assert np.array_equal(lab.numpy(), olab)
     assert np.array_equal(_bits(c.numpy()), _bits(oc))     # integer data: exact FP64 sums
@@ -571,3 +574,21 @@ def test_large_k_sc_index_mixed_batch_sizes(ctx):
     assert np.array_equal(pi[chk], oids) and np.array_equal(_bits(pd[chk]), _bits(od))
     plan.close()
     ix.close()
+
+
+@pytest.mark.parametrize("d,k,n", [(64, 1024, 30000), (128, 256, 20000), (16, 256, 50000)])
+def test_kmeans_mixed_form_equals_exact(ctx, d, k, n):
+    """SC_PREC_BF16 k-means (BF16 MFMA Lloyd assignment with an exact
+    re-check of every centre within the error bound) gives the exact form's
+    labels and centres bit for bit on integer data, also with many exact
+    distance ties (duplicated points), and the oracle's on a sample."""
+    x = sco.gen(sco.mix_d128(ncomp=17), sco.STREAM_TRAIN, 0, n, 128)[:, :d].copy()
+    x[n // 2:n // 2 + 500] = x[:500]            # duplicated points: equal distances
+    dx = S.Dev.from_numpy(ctx, x)
+    c1, l1 = S.kmeans(ctx, dx, k, 4, seed=7, prec=S.PREC_FP32)
+    c2, l2 = S.kmeans(ctx, dx, k, 4, seed=7, prec=S.PREC_BF16)
+    assert np.array_equal(l1.numpy(), l2.numpy())
+    assert np.array_equal(_bits(c1.numpy()), _bits(c2.numpy()))
+    if n <= 30000 and k <= 256:
+        oc, olab = sco.kmeans(x, k, 4, seed=7)
+        assert np.array_equal(l2.numpy(), olab)
