@@ -47,16 +47,39 @@ __global__ void k_first_seed(int64_t n, uint32_t seed, int64_t *pick) {
     if (threadIdx.x == 0 && blockIdx.x == 0) pick[0] = (int64_t)(pick_u64(seed, 0, 0) % (uint64_t)n);
 }
 
-// D2_i = min(D2_i, S1(x_i, x_pick[slot]))  (init: D2_i = S1(x_i, x_pick[0]))
-__global__ void k_update_d2(const float *__restrict__ x, int64_t n, int d,
-                            const int64_t *__restrict__ picks, int slot, int init,
-                            double *__restrict__ d2, uint64_t *__restrict__ bsum) {
+// D2_i = min(D2_i, S1(x_i, x_pick[slot]))  (init: D2_i = S1(x_i, x_pick[0])).
+// The workgroup's 256 points are staged 32 dimensions at a time (each row
+// slice is one 128-byte read), and every thread accumulates its own point's
+// S1 from LDS in dimension order.
+constexpr int kD2Dims = 32;
+__global__ __launch_bounds__(256) void k_update_d2(const float *__restrict__ x, int64_t n, int d,
+                                                   const int64_t *__restrict__ picks, int slot,
+                                                   int init, double *__restrict__ d2,
+                                                   uint64_t *__restrict__ bsum) {
     __shared__ uint64_t red[256];
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ float tile[256][kD2Dims + 1];
+    __shared__ float cs[kD2Dims];
+    const int tid = threadIdx.x;
+    const int64_t i0 = (int64_t)blockIdx.x * 256, i = i0 + tid;
+    const float *c = x + (size_t)picks[slot] * d;
+    double acc = 0.0;
+    for (int t0 = 0; t0 < d; t0 += kD2Dims) {
+        const int tt = min(kD2Dims, d - t0);
+        __syncthreads();
+        for (int e = tid; e < 256 * kD2Dims; e += 256) {
+            const int r = e / kD2Dims, col = e % kD2Dims;
+            tile[r][col] = (col < tt && i0 + r < n) ? x[(size_t)(i0 + r) * d + t0 + col] : 0.0f;
+        }
+        if (tid < kD2Dims) cs[tid] = tid < tt ? c[t0 + tid] : 0.0f;
+        __syncthreads();
+        for (int j = 0; j < tt; ++j) {
+            const double dd = (double)(tile[tid][j] - cs[j]);
+            acc = __fma_rn(dd, dd, acc);
+        }
+    }
     uint64_t w = 0;
     if (i < n) {
-        const float *c = x + (size_t)picks[slot] * d;
-        double t = s1(x + (size_t)i * d, c, d);
+        const double t = acc;
         double v = init ? t : (d2[i] > t ? t : d2[i]);
         d2[i] = v;
         w = weight(v);
