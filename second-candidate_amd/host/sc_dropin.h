@@ -178,6 +178,9 @@ struct KmeansOptions {
     int iterations = 25;
     uint32_t seed = 20151001u;
     int lagged = 0;
+    // SC_PREC_BF16: Lloyd assignment on BF16 MFMA with an exact re-check
+    // (identical labels and centres; sub-vector lengths multiple of 16)
+    int prec = SC_PREC_FP32;
 };
 inline KmeansOptions &kmeans_options() {
     static KmeansOptions o;
@@ -259,7 +262,7 @@ public:
             dx.put(strip.data(), strip.size() * sizeof(float));
             if (verbose) std::cout << "Creating codebook " << i << "/" << part << std::endl;
             detail::check(sc_kmeans(detail::ctx(), dx.as<float>(), N, bs, nsc, o.iterations,
-                                    o.seed + (uint32_t)i, o.lagged, SC_PREC_FP32, dc.as<float>(),
+                                    o.seed + (uint32_t)i, o.lagged, o.prec, dc.as<float>(),
                                     dl.as<int32_t>()),
                           "sc_kmeans");
             dc.get(centers + (size_t)i * nsc * bs, (size_t)nsc * bs * sizeof(float));
