@@ -592,3 +592,41 @@ def test_kmeans_mixed_form_equals_exact(ctx, d, k, n):
     if n <= 30000 and k <= 256:
         oc, olab = sco.kmeans(x, k, 4, seed=7)
         assert np.array_equal(l2.numpy(), olab)
+
+
+def test_d960_imi_search(ctx):
+    """D=960 inverted multi-index (two 480-dim halves with K=1024 each, PQ
+    16x256, the high-dimensional shape) on 60k base vectors: encode on
+    sampled rows and searches at budgets from 1k to 30k candidates (one call
+    per budget) vs the oracle."""
+    n, dim, kc, mp, kp = 60_000, 960, 1024, 16, 256
+    mix = S.mix_d960()
+    xt = S.gen_rows(ctx, mix, S.STREAM_TRAIN, 0, 20_000, dim)
+    th = xt.numpy()
+    cq = np.stack([S.kmeans(ctx, S.Dev.from_numpy(ctx, np.ascontiguousarray(th[:, h * 480:(h + 1) * 480])),
+                            kc, 2, seed=41 + h, want_labels=False)[0].numpy() for h in range(2)])
+    dcq = S.Dev.from_numpy(ctx, cq)
+    S.residual(ctx, xt, dcq, 2, kc, 0)
+    rh = xt.numpy()
+    pq = np.stack([S.kmeans(ctx, S.Dev.from_numpy(ctx, np.ascontiguousarray(rh[:, s * 60:(s + 1) * 60])),
+                            kp, 2, seed=51 + s, want_labels=False)[0].numpy() for s in range(mp)])
+    dpq = S.Dev.from_numpy(ctx, pq)
+    x = S.gen_rows(ctx, mix, S.STREAM_BASE, 0, n, dim)
+    cid, codes, _ = S.encode(ctx, x, dcq, dpq, 2, kc, mp, kp, 1)
+    cidh, codesh = cid.numpy(), codes.numpy()
+    rows = np.random.default_rng(4).choice(n, 300, replace=False)
+    xr = np.concatenate([sco.gen(sco.mix_d960(), sco.STREAM_BASE, int(r), 1, dim) for r in rows])
+    ocid, ocodes = sco.encode(xr, cq, pq, 2, kc, mp, kp, 1)
+    assert np.array_equal(cidh[rows], ocid) and np.array_equal(codesh[rows], ocodes)
+    ix = S.Index.build(ctx, cid, codes, dim, dcq, 2, kc, dpq, mp, kp, 1)
+    e = ix.export()
+    oix = sco.OracleIndex(dim, 2, kc, mp, kp, cq, pq, e["L"], e["pid"], e["codes"])
+    q = sco.gen(sco.mix_d960(), sco.STREAM_QUERY, 0, 24, dim)
+    dq = S.Dev.from_numpy(ctx, q)
+    for T in (1000, 12000, 30000):
+        ids, dists, sums, _ = ix.search(dq, S.KIND_IMI, 100, 200000, T)
+        oids, od, osum = oix.search_batch(sco.IMI, q, 100, 200000, T)
+        assert np.array_equal(sums.numpy(), osum), T
+        assert np.array_equal(ids.numpy(), oids), T
+        assert np.array_equal(_bits(dists.numpy()), _bits(od)), T
+    ix.close()
