@@ -630,3 +630,30 @@ def test_d960_imi_search(ctx):
         assert np.array_equal(ids.numpy(), oids), T
         assert np.array_equal(_bits(dists.numpy()), _bits(od)), T
     ix.close()
+
+
+def test_exhaustive_pq_config(ctx):
+    """Exhaustive PQ ADC (the reference's CPU configuration: 100k base vectors,
+    PQ 8x256 trained on 50k, top-100) mapped onto search_ivfadc with one
+    zero coarse centroid, w=1 and T >= N: every vector is a candidate; ids
+    and distances vs the oracle."""
+    n, dim, mp, kp = 100_000, 128, 8, 256
+    mix = S.mix_d128()
+    xt = S.gen_rows(ctx, mix, S.STREAM_TRAIN, 0, 50_000, dim).numpy()
+    pq = np.stack([S.kmeans(ctx, S.Dev.from_numpy(ctx, np.ascontiguousarray(xt[:, s * 16:(s + 1) * 16])),
+                            kp, 5, seed=61 + s, want_labels=False)[0].numpy() for s in range(mp)])
+    cq = np.zeros((1, 1, dim), np.float32)
+    dcq, dpq = S.Dev.from_numpy(ctx, cq), S.Dev.from_numpy(ctx, pq)
+    x = S.gen_rows(ctx, mix, S.STREAM_BASE, 0, n, dim)
+    cid, codes, _ = S.encode(ctx, x, dcq, dpq, 1, 1, mp, kp, 1)
+    ix = S.Index.build(ctx, cid, codes, dim, dcq, 1, 1, dpq, mp, kp, 1)
+    e = ix.export()
+    assert e["L"][0] == n and np.array_equal(e["pid"], np.arange(n))
+    oix = sco.OracleIndex(dim, 1, 1, mp, kp, cq, pq, e["L"], e["pid"], e["codes"])
+    q = sco.gen(sco.mix_d128(), sco.STREAM_QUERY, 0, 64, dim)
+    ids, dists, sums, _ = ix.search(S.Dev.from_numpy(ctx, q), S.KIND_IVFADC, 100, 1, n)
+    oids, od, osum = oix.search_batch(sco.IVFADC, q, 100, 1, n)
+    assert (sums.numpy() == n).all() and np.array_equal(sums.numpy(), osum)
+    assert np.array_equal(ids.numpy(), oids)
+    assert np.array_equal(_bits(dists.numpy()), _bits(od))
+    ix.close()
