@@ -194,6 +194,15 @@ typedef struct {
 int sc_search(sc_ctx *ctx, const sc_index *ix, const float *d_q, int64_t nq,
               const sc_search_params *p, int32_t *d_ids, float *d_dists,
               int32_t *d_sums, sc_search_stats *st);
+/* The same search with a candidate budget per query (no reference
+ * counterpart: the reference takes one T per call, include/sc_query.h:60-66;
+ * a batch here can mix budgets, e.g. drawn from [1k, 30k]): d_T [nq] int32 on
+ * the device; p->T is the largest budget, and each d_T[q] is clamped to
+ * [0, p->T].  Query q's results equal a sc_search of that query alone with
+ * T = d_T[q]. */
+int sc_search_budgets(sc_ctx *ctx, const sc_index *ix, const float *d_q, int64_t nq,
+                      const sc_search_params *p, const int32_t *d_T, int32_t *d_ids,
+                      float *d_dists, int32_t *d_sums, sc_search_stats *st);
 
 /* ---- batched search plans (no reference counterpart) ------------------- */
 /* A plan binds one batch shape (nq queries at d_q, outputs at d_ids/d_dists/
@@ -209,6 +218,11 @@ typedef struct sc_plan sc_plan;
 int sc_plan_create(sc_ctx *ctx, const sc_index *ix, const sc_search_params *p,
                    const float *d_q, int64_t nq, int32_t *d_ids, float *d_dists,
                    int32_t *d_sums, sc_plan **out);
+/* A plan whose queries carry their own budgets d_T [nq] (as in
+ * sc_search_budgets); the budgets' contents may change between executions. */
+int sc_plan_create_budgets(sc_ctx *ctx, const sc_index *ix, const sc_search_params *p,
+                           const float *d_q, const int32_t *d_T, int64_t nq, int32_t *d_ids,
+                           float *d_dists, int32_t *d_sums, sc_plan **out);
 int sc_plan_execute(sc_plan *plan, sc_search_stats *st);
 /* Times the fused kernel over launch shapes (register variant x workgroups in
  * flight) on the plan's own batch, keeps the fastest and re-captures the

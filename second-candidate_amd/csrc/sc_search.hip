@@ -275,7 +275,14 @@ struct TravArgs {
     uint32_t *gheap_p;
     int32_t *grow;          // fallback row lengths [nq][kc]
     int32_t *clen;          // [nq][ccap] cell lengths (cells holds cell starts)
+    const int32_t *Tq;      // [nq] per-query budgets (at most T) or null
 };
+
+// The candidate budget of query q: its own when per-query budgets are given
+// (clamped to [0, T]), else T.
+__device__ __forceinline__ int query_budget(const int32_t *Tq, int64_t q, int T) {
+    return Tq ? min(max(Tq[q], 0), T) : T;
+}
 
 __device__ __forceinline__ int32_t cell_len(const int32_t *L, int64_t bid) {
     return bid > 0 ? L[bid] - L[bid - 1] : L[0];
@@ -291,6 +298,7 @@ __global__ void k_traverse(TravArgs a, int hcap, int qpw) {
     if (lane >= qpw || slot >= a.nq) return;
     const int64_t q = a.qlist ? a.qlist[slot] : slot;
     const int kc = a.kc;
+    const int T = query_budget(a.Tq, q, a.T);
 
     Heap h;
     int32_t *rowlen;
@@ -335,7 +343,7 @@ __global__ void k_traverse(TravArgs a, int hcap, int qpw) {
             count++;
             count2++;
             sum += l;
-            if (sum >= a.T) break;
+            if (sum >= T) break;
         }
     } else {
         const bool imi = a.kind == SC_KIND_IMI;
@@ -353,7 +361,7 @@ __global__ void k_traverse(TravArgs a, int hcap, int qpw) {
             h.insert(d0, (0u << 16) | 1u);
             h.insert(d0, (1u << 16) | 0u);
         }
-        while ((imi ? count2 : count) < a.w && sum < a.T) {
+        while ((imi ? count2 : count) < a.w && sum < T) {
             if (h.overflow) break;
             if (h.pos <= 0) { exhausted = 1; break; }
             uint32_t pk = h.pop();
@@ -375,7 +383,7 @@ __global__ void k_traverse(TravArgs a, int hcap, int qpw) {
             if (rl != h2) err = 3;                   // not a down-set: cannot happen
             rl = h2 + 1;
             count2++;
-            if (sum < a.T) {
+            if (sum < T) {
                 if (h1 < kc - 1 && (h2 == 0 || (h2 - 1) < rowlen[(size_t)(h1 + 1) * rstride]))
                     h.insert(v1[h1 + 1] + v2[h2], ((uint32_t)(h1 + 1) << 16) | (uint32_t)h2);
                 if (h2 < kc - 1 && (h1 == 0 || (h2 + 1) < rowlen[(size_t)(h1 - 1) * rstride]))
@@ -436,6 +444,7 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse_wave(TravArgs a, int 
     const int64_t slot = blockIdx.x;
     if (slot >= a.nq) return;
     const int64_t q = a.qlist ? a.qlist[slot] : slot;
+    const int T = query_budget(a.Tq, q, a.T);
     for (int i = tid; i < lists * kr; i += kTravThreads) {
         const int l = i / kr, r = i - l * kr;
         sv[i] = a.sv[((size_t)q * lists + l) * kc + r];
@@ -470,7 +479,7 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse_wave(TravArgs a, int 
             h.insert(d0, (0u << 16) | 1u);
             h.insert(d0, (1u << 16) | 0u);
         }
-        if (!((imi ? count2 : count) < a.w && sum < a.T)) s_done = 1;
+        if (!((imi ? count2 : count) < a.w && sum < T)) s_done = 1;
     }
     __syncthreads();
     while (!s_done) {
@@ -516,7 +525,7 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse_wave(TravArgs a, int 
             int i = 0;
             bool stop = false;
             for (; i < n; ++i) {
-                if (!((imi ? count2 : count) < a.w && sum < a.T)) { stop = true; break; }
+                if (!((imi ? count2 : count) < a.w && sum < T)) { stop = true; break; }
                 const int l = s_len[i];
                 if (l > 0) {
                     if (count >= a.ccap) { err = 2; stop = true; break; }
@@ -530,7 +539,7 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse_wave(TravArgs a, int 
                 }
                 count2++;
             }
-            if (!stop && !((imi ? count2 : count) < a.w && sum < a.T)) stop = true;
+            if (!stop && !((imi ? count2 : count) < a.w && sum < T)) stop = true;
             if (!stop) {
                 if (dry) { exhausted = 1; stop = true; }             // the heap is empty
                 else if (h.overflow) { flag = 1; stop = true; }      // later pops are invalid
@@ -566,6 +575,7 @@ struct Trav3Args {
     float *hv;              // [nq][w] heap scores
     uint32_t *hp;           // [nq][w] heap triples (10 bits per rank)
     uint16_t *depth;        // [nq][kc][kc], zero on entry
+    const int32_t *Tq;      // per-query budgets (indexed like the queries) or null
 };
 
 __global__ void k_traverse3(Trav3Args a) {
@@ -573,6 +583,7 @@ __global__ void k_traverse3(Trav3Args a) {
     if (slot >= a.nq) return;
     const int64_t q = a.q0 + slot;
     const int kc = a.kc;
+    const int T = query_budget(a.Tq, q, a.T);
     const float *v = a.sv + (size_t)q * kc;
     const int32_t *t = a.si + (size_t)q * kc;
     uint16_t *D = a.depth + (size_t)slot * kc * kc;
@@ -595,7 +606,7 @@ __global__ void k_traverse3(Trav3Args a) {
     int count = 0, sum = 0, empty = 0, exhausted = 0, err = 0;
     int64_t count2 = 0;
     h.insert(3.0f * v[0], pack(0, 0, 0));
-    while (count < a.w && sum < a.T && count2 < grid) {
+    while (count < a.w && sum < T && count2 < grid) {
         if (h.pos <= 0) { exhausted = 1; break; }
         const uint32_t pk = h.pop();
         const int h1 = (int)(pk & 1023u), h2 = (int)((pk >> 10) & 1023u), h3 = (int)(pk >> 20);
@@ -616,7 +627,7 @@ __global__ void k_traverse3(Trav3Args a) {
         if ((int)dd != h3) err = 3;                  // not a down-set: cannot happen
         dd = (uint16_t)(h3 + 1);
         count2++;
-        if (sum < a.T) {
+        if (sum < T) {
             if (h1 < kc - 1 && (h2 == 0 || popped(h1 + 1, h2 - 1, h3)) &&
                 (h3 == 0 || popped(h1 + 1, h2, h3 - 1)))
                 h.insert((v[h1 + 1] + v[h2]) + v[h3], pack(h1 + 1, h2, h3));
@@ -1077,6 +1088,7 @@ struct FusedArgs {
     float margin_scale;               // mixed form: multiplies the error bound (1)
     int nvec;                         // vectors in the index
     int pop_budget;                   // SC: cells popped before a query is handed on
+    const int32_t *Tq;                // [nq] per-query candidate budgets or null
 };
 
 struct FusedLayout {
@@ -1530,6 +1542,7 @@ void k_search_fused(FusedArgs a) {
         __syncthreads();
         const int64_t q = s_q;
         if (q >= a.nq) break;
+        const int Tb = query_budget(a.Tq, q, a.T);   // this query's candidate budget
         if (tid == 0) s_tph[5] = wall_clock64();
         for (int t = tid; t < dim; t += NT) qv[t] = a.Q[(size_t)q * dim + t];
         __syncthreads();
@@ -1673,7 +1686,7 @@ void k_search_fused(FusedArgs a) {
                         count++;
                         count2++;
                         sum += lhi[i] - llo[i];
-                        if (sum >= a.T) done = true;
+                        if (sum >= Tb) done = true;
                     }
                 }
             } else {
@@ -1688,7 +1701,7 @@ void k_search_fused(FusedArgs a) {
                 // pops run ahead in batches of kSpec, their L[] entries load
                 // together, and a sequential commit stops exactly where the
                 // reference does (pushes past that point are moot).
-                bool done = (count >= a.w || sum >= a.T);
+                bool done = (count >= a.w || sum >= Tb);
                 while (!done) {
                     int nsp = 0, kcount = count, ksum = sum;
                     bool dry = false, known = true;
@@ -1711,7 +1724,7 @@ void k_search_fused(FusedArgs a) {
                             const int l = win[2 * (h1 * wk + h2) + 1];
                             kcount += l > 0;
                             ksum += l;
-                            if (kcount >= a.w || ksum >= a.T) break;   // the loop stops here
+                            if (kcount >= a.w || ksum >= Tb) break;    // the loop stops here
                         }
                     }
                     int32_t lhi[kSpec], llo[kSpec];
@@ -1742,7 +1755,7 @@ void k_search_fused(FusedArgs a) {
                             empty++;
                         }
                         count2++;
-                        if (count >= a.w || sum >= a.T) done = true;
+                        if (count >= a.w || sum >= Tb) done = true;
                         // a traversal this long (mostly empty cells) runs faster in
                         // the staged kernels, which keep more queries in flight
                         else if (count2 >= a.pop_budget) { flag = 1; done = true; }
@@ -2155,11 +2168,12 @@ Plan make_plan(const sc_index *ix, const sc_search_params *p, int64_t nq) {
 }  // namespace
 
 static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int64_t nq,
-                         const sc_search_params *p, int32_t *d_ids, float *d_dists,
-                         int32_t *d_sums, sc_search_stats *st);
+                         const sc_search_params *p, const int32_t *d_T, int32_t *d_ids,
+                         float *d_dists, int32_t *d_sums, sc_search_stats *st);
 static int finish_flagged(sc_ctx *ctx, const sc_index *ix, const sc_search_params *p,
-                          const float *d_q, int64_t nq, const int32_t *flags, int nflag,
-                          int32_t *d_ids, float *d_dists, int32_t *d_sums, sc_search_stats *st);
+                          const float *d_q, const int32_t *d_T, int64_t nq, const int32_t *flags,
+                          int nflag, int32_t *d_ids, float *d_dists, int32_t *d_sums,
+                          sc_search_stats *st);
 
 // SC traversals that pop this many cells leave the fused kernel for the
 // staged kernels (SC_POP_BUDGET overrides it; a test hook).  Typical queries
@@ -2241,8 +2255,8 @@ static FusedConfig fused_config(const sc_index *ix, const sc_search_params *p) {
 // Fused persistent path for IVF/SC.  Returns SC_ERR_TOO_LARGE when its LDS
 // layout does not fit (the caller then uses the staged kernels).
 static int search_fused(sc_ctx *ctx, const sc_index *ix, const float *d_q, int64_t nq,
-                        const sc_search_params *p, int32_t *d_ids, float *d_dists,
-                        int32_t *d_sums, sc_search_stats *st) {
+                        const sc_search_params *p, const int32_t *d_T, int32_t *d_ids,
+                        float *d_dists, int32_t *d_sums, sc_search_stats *st) {
     const int kind = p->kind, kc = ix->kc;
     const FusedConfig F = fused_config(ix, p);
     const FusedLayout &Lay = F.lay;
@@ -2286,6 +2300,7 @@ static int search_fused(sc_ctx *ctx, const sc_index *ix, const float *d_q, int64
     fa.margin_scale = mixed_margin_scale();
     fa.nvec = (int)ix->n;
     fa.pop_budget = fused_pop_budget();
+    fa.Tq = d_T;
     if (st) SC_HIP(hipMemsetAsync(stats + 8, 0, 48, sm));
     hipEvent_t e0 = nullptr, e1 = nullptr, el = nullptr;
     if (st) {
@@ -2337,18 +2352,19 @@ static int search_fused(sc_ctx *ctx, const sc_index *ix, const float *d_q, int64
         st->ms_lut = ml;
         st->ms_scan = ms;
     }
-    return finish_flagged(ctx, ix, p, d_q, nq, flags, nflag, d_ids, d_dists, d_sums, st);
+    return finish_flagged(ctx, ix, p, d_q, d_T, nq, flags, nflag, d_ids, d_dists, d_sums, st);
 }
 
 // Finishes the queries the fused kernel flagged (on-chip capacity exceeded)
 // with the staged kernels and scatters their results into place.
 static int finish_flagged(sc_ctx *ctx, const sc_index *ix, const sc_search_params *p,
-                          const float *d_q, int64_t nq, const int32_t *flags, int nflag,
-                          int32_t *d_ids, float *d_dists, int32_t *d_sums, sc_search_stats *st) {
+                          const float *d_q, const int32_t *d_T, int64_t nq, const int32_t *flags,
+                          int nflag, int32_t *d_ids, float *d_dists, int32_t *d_sums,
+                          sc_search_stats *st) {
     hipStream_t sm = ctx->stream;
     if (nflag > 0) {
         // finish the flagged queries with the staged kernels
-        int32_t *list = nullptr, *sub_ids = nullptr, *sub_sums = nullptr;
+        int32_t *list = nullptr, *sub_ids = nullptr, *sub_sums = nullptr, *sub_T = nullptr;
         float *sub_q = nullptr, *sub_d = nullptr;
         int *dn = nullptr;
         bool ok = hipMalloc(&list, 4 * (size_t)nflag) == hipSuccess &&
@@ -2356,7 +2372,8 @@ static int finish_flagged(sc_ctx *ctx, const sc_index *ix, const sc_search_param
                   hipMalloc(&sub_q, 4 * (size_t)nflag * ix->dim) == hipSuccess &&
                   hipMalloc(&sub_ids, 4 * (size_t)nflag * p->R) == hipSuccess &&
                   hipMalloc(&sub_d, 4 * (size_t)nflag * p->R) == hipSuccess &&
-                  hipMalloc(&sub_sums, 4 * (size_t)nflag) == hipSuccess;
+                  hipMalloc(&sub_sums, 4 * (size_t)nflag) == hipSuccess &&
+                  (!d_T || hipMalloc(&sub_T, 4 * (size_t)nflag) == hipSuccess);
         int rc = ok ? SC_OK : SC_ERR_NOMEM;
         if (ok) {
             hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, sm, dn, 1);
@@ -2364,11 +2381,16 @@ static int finish_flagged(sc_ctx *ctx, const sc_index *ix, const sc_search_param
                                flags, nq, list, dn);
             hipLaunchKernelGGL(k_gather_rows, dim3((unsigned)sc_cdiv((int64_t)nflag * ix->dim, 256)),
                                dim3(256), 0, sm, d_q, ix->dim, list, nflag, sub_q);
+            if (d_T)   // the budgets travel with their queries (4-byte words, copied as is)
+                hipLaunchKernelGGL(k_gather_rows, dim3((unsigned)sc_cdiv(nflag, 256)), dim3(256),
+                                   0, sm, reinterpret_cast<const float *>(d_T), 1, list, nflag,
+                                   reinterpret_cast<float *>(sub_T));
             rc = hipGetLastError() == hipSuccess ? SC_OK : SC_ERR_HIP;
         }
         sc_search_stats sub{};
         if (rc == SC_OK)
-            rc = search_staged(ctx, ix, sub_q, nflag, p, sub_ids, sub_d, sub_sums, st ? &sub : nullptr);
+            rc = search_staged(ctx, ix, sub_q, nflag, p, sub_T, sub_ids, sub_d, sub_sums,
+                               st ? &sub : nullptr);
         if (rc == SC_OK) {
             hipLaunchKernelGGL(k_scatter_results, dim3((unsigned)sc_cdiv((int64_t)nflag * p->R, 256)),
                                dim3(256), 0, sm, list, nflag, p->R, sub_ids, sub_d, sub_sums,
@@ -2377,7 +2399,7 @@ static int finish_flagged(sc_ctx *ctx, const sc_index *ix, const sc_search_param
             (void)hipStreamSynchronize(sm);
         }
         for (void *b : {(void *)list, (void *)dn, (void *)sub_q, (void *)sub_ids, (void *)sub_d,
-                        (void *)sub_sums})
+                        (void *)sub_sums, (void *)sub_T})
             if (b) (void)hipFree(b);
         if (rc != SC_OK) {
             if (rc == SC_ERR_NOMEM) sc_set_error("sc_search: fallback allocation failed");
@@ -2395,9 +2417,9 @@ static int finish_flagged(sc_ctx *ctx, const sc_index *ix, const sc_search_param
     return SC_OK;
 }
 
-extern "C" int sc_search(sc_ctx *ctx, const sc_index *ix, const float *d_q, int64_t nq,
-                         const sc_search_params *p, int32_t *d_ids, float *d_dists,
-                         int32_t *d_sums, sc_search_stats *st) {
+static int search_impl(sc_ctx *ctx, const sc_index *ix, const float *d_q, int64_t nq,
+                       const sc_search_params *p, const int32_t *d_T, int32_t *d_ids,
+                       float *d_dists, int32_t *d_sums, sc_search_stats *st) {
     SC_TRY(sc_ctx_enter(ctx));
     if (!ix || !p || (nq > 0 && (!d_q || !d_ids || !d_dists)) || nq < 0) {
         sc_set_error("sc_search: bad arguments");
@@ -2441,18 +2463,34 @@ extern "C" int sc_search(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
     // candidate, so the scan is bound by those reads, not by the on-chip stages
     if (kind != SC_KIND_IMI && kind != SC_KIND_SC3 && p->form != SC_FORM_REAL &&
         !getenv("SC_NO_FUSED")) {
-        int rc = search_fused(ctx, ix, d_q, nq, p, d_ids, d_dists, d_sums, st);
+        int rc = search_fused(ctx, ix, d_q, nq, p, d_T, d_ids, d_dists, d_sums, st);
         if (rc != SC_ERR_TOO_LARGE) return rc;      // otherwise: staged kernels
         if (st) *st = sc_search_stats{};
     }
-    return search_staged(ctx, ix, d_q, nq, p, d_ids, d_dists, d_sums, st);
+    return search_staged(ctx, ix, d_q, nq, p, d_T, d_ids, d_dists, d_sums, st);
+}
+
+extern "C" int sc_search(sc_ctx *ctx, const sc_index *ix, const float *d_q, int64_t nq,
+                         const sc_search_params *p, int32_t *d_ids, float *d_dists,
+                         int32_t *d_sums, sc_search_stats *st) {
+    return search_impl(ctx, ix, d_q, nq, p, nullptr, d_ids, d_dists, d_sums, st);
+}
+
+extern "C" int sc_search_budgets(sc_ctx *ctx, const sc_index *ix, const float *d_q, int64_t nq,
+                                 const sc_search_params *p, const int32_t *d_T, int32_t *d_ids,
+                                 float *d_dists, int32_t *d_sums, sc_search_stats *st) {
+    if (nq > 0 && !d_T) {
+        sc_set_error("sc_search_budgets: no budgets");
+        return SC_ERR_ARG;
+    }
+    return search_impl(ctx, ix, d_q, nq, p, d_T, d_ids, d_dists, d_sums, st);
 }
 
 // The three-candidate traversal of a chunk's cn queries, in groups whose
 // global heaps (the reference's capacity w) and depth maps fit 1 GiB.
 static int traverse3(sc_ctx *ctx, const sc_search_params *p, int kc, const float *sv,
                      const int32_t *si, const int32_t *L, int32_t *cells, int32_t *clen,
-                     uint32_t *cinfo, int32_t *qinfo, int ccap, int64_t cn) {
+                     uint32_t *cinfo, int32_t *qinfo, int ccap, int64_t cn, const int32_t *Tq) {
     const size_t per = (size_t)p->w * 8 + (size_t)kc * kc * 2;
     const int64_t group =
         std::max<int64_t>(1, std::min<int64_t>(cn, (int64_t)(((size_t)1 << 30) / per)));
@@ -2473,7 +2511,7 @@ static int traverse3(sc_ctx *ctx, const sc_search_params *p, int kc, const float
             break;
         }
         Trav3Args ta{kc, p->w, p->T, ccap, sv, si, L, cells, clen, cinfo, qinfo, g0, gn, hv, hp,
-                     depth};
+                     depth, Tq};
         hipLaunchKernelGGL(k_traverse3, dim3((unsigned)sc_cdiv(gn, 64)), dim3(64), 0, ctx->stream,
                            ta);
         if (hipGetLastError() != hipSuccess) rc = SC_ERR_HIP;
@@ -2485,8 +2523,8 @@ static int traverse3(sc_ctx *ctx, const sc_search_params *p, int kc, const float
 }
 
 static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int64_t nq,
-                         const sc_search_params *p, int32_t *d_ids, float *d_dists,
-                         int32_t *d_sums, sc_search_stats *st) {
+                         const sc_search_params *p, const int32_t *d_T, int32_t *d_ids,
+                         float *d_dists, int32_t *d_sums, sc_search_stats *st) {
     const int kind = p->kind;
     const bool three = kind == SC_KIND_SC3;
     const int scan_kind = three ? SC_KIND_SC : kind;   // same ranking and scan as SC
@@ -2562,10 +2600,11 @@ static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
                            Q, ix->dim, scan_kind, ix->mc, kc, dqc, qs, sv, si, npow2);
         SC_CHECK_LAUNCH();
         mark();
+        const int32_t *Tq = d_T ? d_T + q0 : nullptr;      // this chunk's budgets
         TravArgs ta{kind, kc, p->w, p->T, P.ccap, sv, si, ix->L, cells, cinfo, qinfo,
-                    nullptr, cn, nullptr, nullptr, nullptr, clen};
+                    nullptr, cn, nullptr, nullptr, nullptr, clen, Tq};
         if (three) {
-            SC_TRY(traverse3(ctx, p, kc, sv, si, ix->L, cells, clen, cinfo, qinfo, P.ccap, cn));
+            SC_TRY(traverse3(ctx, p, kc, sv, si, ix->L, cells, clen, cinfo, qinfo, P.ccap, cn, Tq));
         } else if (wave) {
             const size_t lds = trav_wave_layout(lists, kr, hcap_w).total;
             hipLaunchKernelGGL(k_traverse_wave, dim3((unsigned)cn), dim3(kTravThreads), lds,
@@ -2681,6 +2720,7 @@ struct sc_plan {
     const sc_index *ix;
     sc_search_params p;
     const float *d_q;
+    const int32_t *d_T;            // per-query budgets or null
     int64_t nq;
     int32_t *d_ids, *d_sums;
     float *d_dists;
@@ -2727,9 +2767,9 @@ static int plan_capture(sc_plan *P) {
     return SC_OK;
 }
 
-extern "C" int sc_plan_create(sc_ctx *ctx, const sc_index *ix, const sc_search_params *p,
-                              const float *d_q, int64_t nq, int32_t *d_ids, float *d_dists,
-                              int32_t *d_sums, sc_plan **out) {
+static int plan_create(sc_ctx *ctx, const sc_index *ix, const sc_search_params *p,
+                       const float *d_q, const int32_t *d_T, int64_t nq, int32_t *d_ids,
+                       float *d_dists, int32_t *d_sums, sc_plan **out) {
     SC_TRY(sc_ctx_enter(ctx));
     if (!out || !p || nq < 1 || !d_q || !d_ids || !d_dists || !d_sums) {
         sc_set_error("sc_plan_create: bad arguments");
@@ -2741,7 +2781,7 @@ extern "C" int sc_plan_create(sc_ctx *ctx, const sc_index *ix, const sc_search_p
     sc_plan *P = new (std::nothrow) sc_plan();
     if (!P) return SC_ERR_NOMEM;
     *P = sc_plan{};
-    P->ctx = ctx; P->ix = ix; P->p = *p; P->d_q = d_q; P->nq = nq;
+    P->ctx = ctx; P->ix = ix; P->p = *p; P->d_q = d_q; P->d_T = d_T; P->nq = nq;
     P->d_ids = d_ids; P->d_dists = d_dists; P->d_sums = d_sums;
     const int kind = p->kind, kc = ix->kc;
     const FusedConfig F = fused_config(ix, p);
@@ -2772,6 +2812,7 @@ extern "C" int sc_plan_create(sc_ctx *ctx, const sc_index *ix, const sc_search_p
     fa.margin_scale = mixed_margin_scale();
     fa.nvec = (int)ix->n;
     fa.pop_budget = fused_pop_budget();
+    fa.Tq = d_T;
     P->mixed = p->form == SC_FORM_MIXED;
     for (int v = 3; v <= 5; ++v)
         if (hipFuncSetAttribute((const void *)fused_kernel(v, P->mixed),
@@ -2790,11 +2831,29 @@ extern "C" int sc_plan_create(sc_ctx *ctx, const sc_index *ix, const sc_search_p
     return SC_OK;
 }
 
+extern "C" int sc_plan_create(sc_ctx *ctx, const sc_index *ix, const sc_search_params *p,
+                              const float *d_q, int64_t nq, int32_t *d_ids, float *d_dists,
+                              int32_t *d_sums, sc_plan **out) {
+    return plan_create(ctx, ix, p, d_q, nullptr, nq, d_ids, d_dists, d_sums, out);
+}
+
+extern "C" int sc_plan_create_budgets(sc_ctx *ctx, const sc_index *ix,
+                                      const sc_search_params *p, const float *d_q,
+                                      const int32_t *d_T, int64_t nq, int32_t *d_ids,
+                                      float *d_dists, int32_t *d_sums, sc_plan **out) {
+    if (!d_T) {
+        sc_set_error("sc_plan_create_budgets: no budgets");
+        return SC_ERR_ARG;
+    }
+    return plan_create(ctx, ix, p, d_q, d_T, nq, d_ids, d_dists, d_sums, out);
+}
+
 extern "C" int sc_plan_execute(sc_plan *P, sc_search_stats *st) {
     if (!P) return SC_ERR_ARG;
     SC_TRY(sc_ctx_enter(P->ctx));
     if (!P->fused)
-        return sc_search(P->ctx, P->ix, P->d_q, P->nq, &P->p, P->d_ids, P->d_dists, P->d_sums, st);
+        return search_impl(P->ctx, P->ix, P->d_q, P->nq, &P->p, P->d_T, P->d_ids, P->d_dists,
+                           P->d_sums, st);
     if (st) *st = sc_search_stats{};
     hipStream_t sm = P->ctx->stream;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -2831,7 +2890,7 @@ extern "C" int sc_plan_execute(sc_plan *P, sc_search_stats *st) {
         st->reranked = (int64_t)h[6];
         st->ms_scan = ms;
     }
-    return finish_flagged(P->ctx, P->ix, &P->p, P->d_q, P->nq, P->flags, nflag, P->d_ids,
+    return finish_flagged(P->ctx, P->ix, &P->p, P->d_q, P->d_T, P->nq, P->flags, nflag, P->d_ids,
                           P->d_dists, P->d_sums, st);
 }
 

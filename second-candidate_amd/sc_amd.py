@@ -97,8 +97,12 @@ _SIGS = {
     "sc_index_export": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P]),
     "sc_search": (C.c_int, [_P, _P, _P, C.c_int64, C.POINTER(SearchParams), _P, _P, _P,
                             C.POINTER(SearchStats)]),
+    "sc_search_budgets": (C.c_int, [_P, _P, _P, C.c_int64, C.POINTER(SearchParams), _P, _P, _P,
+                                    _P, C.POINTER(SearchStats)]),
     "sc_plan_create": (C.c_int, [_P, _P, C.POINTER(SearchParams), _P, C.c_int64, _P, _P, _P,
                                  C.POINTER(_P)]),
+    "sc_plan_create_budgets": (C.c_int, [_P, _P, C.POINTER(SearchParams), _P, _P, C.c_int64,
+                                         _P, _P, _P, C.POINTER(_P)]),
     "sc_plan_execute": (C.c_int, [_P, C.POINTER(SearchStats)]),
     "sc_plan_autotune": (C.c_int, [_P, _P]),
     "sc_plan_destroy": (C.c_int, [_P]),
@@ -280,17 +284,24 @@ class Index:
         return dict(L=L, pid=pid, codes=codes, norm_c=nc_, norm_r=nr, dot_cr=dcr)
 
     def search(self, q, kind, R, w, T, M=0, form=FORM_PLAIN, ids=None, dists=None, sums=None,
-               stats=True):
-        """q: Dev [nq][dim].  Returns (ids Dev, dists Dev, sums Dev, SearchStats|None)."""
+               stats=True, budgets=None):
+        """q: Dev [nq][dim]; budgets: Dev [nq] int32 of per-query candidate
+        budgets (T is then the largest) or None.  Returns (ids Dev, dists Dev,
+        sums Dev, SearchStats|None)."""
         nq = q.shape[0]
         ids = ids if ids is not None else Dev(self.ctx, (nq, R), np.int32)
         dists = dists if dists is not None else Dev(self.ctx, (nq, R), np.float32)
         sums = sums if sums is not None else Dev(self.ctx, (nq,), np.int32)
         p = SearchParams(kind, form, R, w, T, M or self.kc)
         st = SearchStats()
-        _check("sc_search", lib().sc_search(self.ctx.h, self.h, q.ptr, nq, C.byref(p), ids.ptr,
-                                            dists.ptr, sums.ptr,
-                                            C.byref(st) if stats else None))
+        if budgets is None:
+            _check("sc_search", lib().sc_search(self.ctx.h, self.h, q.ptr, nq, C.byref(p), ids.ptr,
+                                                dists.ptr, sums.ptr,
+                                                C.byref(st) if stats else None))
+        else:
+            _check("sc_search_budgets", lib().sc_search_budgets(
+                self.ctx.h, self.h, q.ptr, nq, C.byref(p), budgets.ptr, ids.ptr, dists.ptr,
+                sums.ptr, C.byref(st) if stats else None))
         return ids, dists, sums, (st if stats else None)
 
     def close(self):
@@ -309,8 +320,10 @@ class Plan:
     """A batched search captured in a hipGraph (sc_plan_*): fixed queries and
     output buffers, replayed by execute()."""
 
-    def __init__(self, index, q, kind, R, w, T, M=0, form=FORM_PLAIN):
-        self.index, self.q = index, q
+    def __init__(self, index, q, kind, R, w, T, M=0, form=FORM_PLAIN, budgets=None):
+        """budgets: Dev [nq] int32 of per-query candidate budgets (T is then the
+        largest), or None."""
+        self.index, self.q, self.budgets = index, q, budgets
         nq = q.shape[0]
         ctx = index.ctx
         self.ids = Dev(ctx, (nq, R), np.int32)
@@ -318,9 +331,14 @@ class Plan:
         self.sums = Dev(ctx, (nq,), np.int32)
         self.params = SearchParams(kind, form, R, w, T, M or index.kc)
         self.h = _P()
-        _check("sc_plan_create", lib().sc_plan_create(
-            ctx.h, index.h, C.byref(self.params), q.ptr, nq, self.ids.ptr, self.dists.ptr,
-            self.sums.ptr, C.byref(self.h)))
+        if budgets is None:
+            _check("sc_plan_create", lib().sc_plan_create(
+                ctx.h, index.h, C.byref(self.params), q.ptr, nq, self.ids.ptr, self.dists.ptr,
+                self.sums.ptr, C.byref(self.h)))
+        else:
+            _check("sc_plan_create_budgets", lib().sc_plan_create_budgets(
+                ctx.h, index.h, C.byref(self.params), q.ptr, budgets.ptr, nq, self.ids.ptr,
+                self.dists.ptr, self.sums.ptr, C.byref(self.h)))
 
     def execute(self, stats=True):
         st = SearchStats()

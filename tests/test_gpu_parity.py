@@ -657,3 +657,43 @@ def test_exhaustive_pq_config(ctx):
     assert np.array_equal(ids.numpy(), oids)
     assert np.array_equal(_bits(dists.numpy()), _bits(od))
     ix.close()
+
+
+@pytest.mark.parametrize("kind,kc,nc_ok", [(S.KIND_SC, 16, True), (S.KIND_IMI, 16, True),
+                                           (S.KIND_IVFADC, 16, True), (S.KIND_SC3, 12, True)])
+def test_search_per_query_budgets(ctx, kind, kc, nc_ok):
+    """A batch whose queries carry their own candidate budgets (drawn from a
+    wide range, some above the batch maximum and some negative, which clamp)
+    returns for every query what a search of that query alone with its budget
+    returns in the oracle; eager calls and a hipGraph plan (SC)."""
+    ix, oix = _index_pair(ctx, kind, kc=kc)
+    nq = 90
+    rng = np.random.default_rng(int(kind) + 17)
+    q = sco.gen(sco.mix_d128(ncomp=11), sco.STREAM_QUERY, 0, nq, ix.dim)
+    Tmax = 3000
+    T = rng.integers(0, Tmax + 1, nq).astype(np.int32)
+    T[:3] = [-5, Tmax + 999, 0]
+    w = 16 if kind == S.KIND_IVFADC else 400
+    dq, dT = S.Dev.from_numpy(ctx, q), S.Dev.from_numpy(ctx, T)
+    okind = {S.KIND_IVFADC: sco.IVFADC, S.KIND_IMI: sco.IMI, S.KIND_SC: sco.SC,
+             S.KIND_SC3: sco.SC3}[kind]
+    want = [oix.search(okind, q[i], 100, w, int(min(max(T[i], 0), Tmax))) for i in range(nq)]
+    ids, dists, sums, _ = ix.search(dq, kind, 100, w, Tmax, budgets=dT)
+    gi, gd, gs = ids.numpy(), dists.numpy(), sums.numpy()
+    for i in range(nq):
+        oi, od, st, _ = want[i]
+        assert gs[i] == st.sum, i
+        assert np.array_equal(gi[i], oi), i
+        assert np.array_equal(_bits(gd[i]), _bits(od)), i
+    if kind == S.KIND_SC:
+        plan = S.Plan(ix, dq, kind, 100, w, Tmax, budgets=dT)
+        plan.execute(stats=False)
+        assert np.array_equal(plan.ids.numpy(), gi) and np.array_equal(plan.sums.numpy(), gs)
+        T2 = T[::-1].copy()                        # new budgets, same buffer
+        dT.copy_from(T2)
+        plan.execute(stats=False)
+        for i in range(0, nq, 7):
+            oi, od, st, _ = oix.search(okind, q[i], 100, w, int(min(max(T2[i], 0), Tmax)))
+            assert plan.sums.numpy()[i] == st.sum and np.array_equal(plan.ids.numpy()[i], oi)
+        plan.close()
+    ix.close()
