@@ -921,6 +921,16 @@ __device__ void scan_run_real(const ScanArgs &a, ScanShared &S, const float *qv,
             const float *y = a.raw + (size_t)a.pid[vidx] * dim;
             double acc = 0.0;
             int t = 0;
+            for (; t + 32 <= dim; t += 32) {     // 128 bytes of the row in flight
+                float yv[32];
+#pragma unroll
+                for (int j = 0; j < 32; ++j) yv[j] = y[t + j];
+#pragma unroll
+                for (int j = 0; j < 32; ++j) {
+                    const double d = (double)(qv[t + j] - yv[j]);
+                    acc = __fma_rn(d, d, acc);
+                }
+            }
             for (; t + 16 <= dim; t += 16) {
                 float yv[16];
 #pragma unroll
