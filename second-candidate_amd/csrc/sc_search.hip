@@ -21,6 +21,8 @@
 #include <new>
 #include <vector>
 
+#include <rocprim/device/device_radix_sort.hpp>
+
 #include "sc_internal.h"
 
 namespace {
@@ -2141,7 +2143,8 @@ struct Plan {
     int64_t chunk;      // queries per pass
     int ccap;
     size_t off_qs, off_dqc, off_dqr, off_sv, off_si, off_cells, off_clen, off_cinfo, off_qinfo,
-        off_list, off_stats, total;
+        off_list, off_stats, off_okey, off_okey2, off_oid, off_oid2, off_perm, off_otmp,
+        otmp_bytes, total;
 };
 
 Plan make_plan(const sc_index *ix, const sc_search_params *p, int64_t nq) {
@@ -2171,8 +2174,49 @@ Plan make_plan(const sc_index *ix, const sc_search_params *p, int64_t nq) {
     P.off_qinfo = take(32ull * chunk);
     P.off_list = take(4ull * chunk + 4);
     P.off_stats = take(8ull * 8);
+    // scan order: (first cell, query) pairs sorted, then the permutation
+    P.off_okey = take(4ull * chunk);
+    P.off_okey2 = take(4ull * chunk);
+    P.off_oid = take(4ull * chunk);
+    P.off_oid2 = take(4ull * chunk);
+    P.off_perm = take(4ull * chunk);
+    size_t tb = 0;
+    (void)rocprim::radix_sort_pairs(nullptr, tb, (uint32_t *)nullptr, (uint32_t *)nullptr,
+                                    (int32_t *)nullptr, (int32_t *)nullptr, (size_t)chunk, 0u,
+                                    32u, (hipStream_t)0);
+    P.otmp_bytes = tb;
+    P.off_otmp = take(tb);
     P.total = o;
     return P;
+}
+
+// The staged scan's workgroup order.  Each query's first cell (its nearest
+// coarse centroid, or pair of them) is its key; queries sorted by key share
+// table rows, and the XCD-aware permutation gives each XCD (workgroup i runs
+// on XCD i mod nx) a contiguous range of the sorted queries, so that the rows
+// they share stay in that XCD's L2.  Results do not depend on the order.
+__global__ void k_scan_keys(const uint32_t *__restrict__ cinfo, const int32_t *__restrict__ qinfo,
+                            int64_t nq, int ccap, uint32_t *__restrict__ keys,
+                            int32_t *__restrict__ ids) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    keys[q] = qinfo[q * 8] > 0 ? cinfo[(size_t)q * ccap] : 0u;
+    ids[q] = (int32_t)q;
+}
+__global__ void k_xcd_order(const int32_t *__restrict__ sorted, int64_t n, int nx,
+                            int32_t *__restrict__ perm) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t x = i % nx, j = i / nx;
+    int64_t off = 0;                                  // workgroups on the XCDs before x
+    for (int64_t y = 0; y < x; ++y) off += (n - y + nx - 1) / nx;
+    perm[i] = sorted[off + j];
+}
+
+// SC_SCAN_ORDER (tuning hook): 0 launch order, 1 sorted, 2 sorted per XCD.
+static int scan_order_mode() {
+    if (const char *e = getenv("SC_SCAN_ORDER")) return atoi(e);
+    return 2;
 }
 
 }  // namespace
@@ -2635,6 +2679,23 @@ static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
         const size_t lds = sizeof(ScanShared) +
                            sizeof(float) * std::max<size_t>((size_t)ix->mp * ix->kp,
                                                             p->form == SC_FORM_REAL ? ix->dim : 0);
+        const int order = scan_order_mode();
+        if (order > 0 && cn > 1) {
+            uint32_t *okey = (uint32_t *)(s + P.off_okey), *okey2 = (uint32_t *)(s + P.off_okey2);
+            int32_t *oid = (int32_t *)(s + P.off_oid), *oid2 = (int32_t *)(s + P.off_oid2);
+            int32_t *perm = (int32_t *)(s + P.off_perm);
+            hipLaunchKernelGGL(k_scan_keys, dim3((unsigned)sc_cdiv(cn, 256)), dim3(256), 0,
+                               ctx->stream, cinfo, qinfo, cn, P.ccap, okey, oid);
+            SC_CHECK_LAUNCH();
+            size_t tb = P.otmp_bytes;
+            SC_HIP(rocprim::radix_sort_pairs(s + P.off_otmp, tb, okey, okey2, oid, oid2,
+                                             (size_t)cn, 0u, 32u, ctx->stream));
+            const int nx = order == 2 && ctx->n_cu % 32 == 0 ? ctx->n_cu / 32 : 1;
+            hipLaunchKernelGGL(k_xcd_order, dim3((unsigned)sc_cdiv(cn, 256)), dim3(256), 0,
+                               ctx->stream, oid2, cn, nx, perm);
+            SC_CHECK_LAUNCH();
+            sa.qlist = perm;
+        }
         if (hcap_eff > 0 || three) {
             hipLaunchKernelGGL(k_scan_topk, dim3((unsigned)cn), dim3(kScanThreads), lds,
                                ctx->stream, sa);

@@ -366,6 +366,29 @@ def test_search_pop_budget_handoff(ctx, monkeypatch, form):
     ix.close()
 
 
+@pytest.mark.parametrize("kind", [S.KIND_IVFADC, S.KIND_IMI, S.KIND_SC])
+def test_search_scan_order_invariant(ctx, kind, monkeypatch):
+    """The staged scan's workgroup order (launch order, sorted by first
+    cell, sorted and split over the XCDs) changes no output bit, in the
+    plain and the real-distance forms."""
+    monkeypatch.setenv("SC_NO_FUSED", "1")
+    ix, oix = _index_pair(ctx, kind)
+    raw = sco.gen(sco.mix_d128(ncomp=11), sco.STREAM_BASE, 0, oix.pid.size, ix.dim)
+    ix.attach_raw(S.Dev.from_numpy(ctx, raw))
+    q = sco.gen(sco.mix_d128(ncomp=11), sco.STREAM_QUERY, 0, 300, ix.dim)
+    dq = S.Dev.from_numpy(ctx, q)
+    for form in (S.FORM_PLAIN, S.FORM_REAL):
+        out = {}
+        for order in ("0", "1", "2"):
+            monkeypatch.setenv("SC_SCAN_ORDER", order)
+            ids, dists, sums, _ = ix.search(dq, kind, 50, 8, 800, form=form)
+            out[order] = (ids.numpy(), dists.numpy().view(np.uint32), sums.numpy())
+        for order in ("1", "2"):
+            for a, b in zip(out["0"], out[order]):
+                assert np.array_equal(a, b), (form, order)
+    ix.close()
+
+
 def test_search_exhausted_grid(ctx):
     ix, oix = _index_pair(ctx, S.KIND_SC, n=300, kc=4)
     st = _compare_search(ctx, ix, oix, S.KIND_SC, 400, 1000, 10 ** 6, nq=20)
