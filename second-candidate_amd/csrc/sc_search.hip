@@ -2203,6 +2203,17 @@ __global__ void k_scan_keys(const uint32_t *__restrict__ cinfo, const int32_t *_
     keys[q] = qinfo[q * 8] > 0 ? cinfo[(size_t)q * ccap] : 0u;
     ids[q] = (int32_t)q;
 }
+// Before the one-workgroup traversal: a query's first cell is the pair of
+// its rank-0 centroids (IMI: one per list; SC: nearest and second nearest).
+__global__ void k_trav_keys(const int32_t *__restrict__ si, int64_t nq, int lists, int kc,
+                            uint32_t *__restrict__ keys, int32_t *__restrict__ ids) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    const int32_t *r = si + (size_t)q * lists * kc;
+    const int32_t b = lists > 1 ? r[kc] : (kc > 1 ? r[1] : 0);
+    keys[q] = (uint32_t)r[0] * (uint32_t)kc + (uint32_t)b;
+    ids[q] = (int32_t)q;
+}
 __global__ void k_xcd_order(const int32_t *__restrict__ sorted, int64_t n, int nx,
                             int32_t *__restrict__ perm) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -2217,6 +2228,12 @@ __global__ void k_xcd_order(const int32_t *__restrict__ sorted, int64_t n, int n
 static int scan_order_mode() {
     if (const char *e = getenv("SC_SCAN_ORDER")) return atoi(e);
     return 2;
+}
+// SC_TRAV_ORDER (tuning hook): 1 (default) orders the one-workgroup traversal
+// as well, by the queries' rank-0 centroids; 0 orders only the scan.
+static int trav_order_mode() {
+    if (const char *e = getenv("SC_TRAV_ORDER")) return atoi(e);
+    return 1;
 }
 
 }  // namespace
@@ -2657,6 +2674,34 @@ static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
         const int32_t *Tq = d_T ? d_T + q0 : nullptr;      // this chunk's budgets
         TravArgs ta{kind, kc, p->w, p->T, P.ccap, sv, si, ix->L, cells, cinfo, qinfo,
                     nullptr, cn, nullptr, nullptr, nullptr, clen, Tq};
+        // workgroup order of the traversal and the scan (results do not depend on it)
+        const int order = scan_order_mode();
+        int32_t *perm = (int32_t *)(s + P.off_perm);
+        bool ordered = false;
+        auto sort_order = [&](bool from_ranks) -> int {
+            uint32_t *okey = (uint32_t *)(s + P.off_okey), *okey2 = (uint32_t *)(s + P.off_okey2);
+            int32_t *oid = (int32_t *)(s + P.off_oid), *oid2 = (int32_t *)(s + P.off_oid2);
+            if (from_ranks)
+                hipLaunchKernelGGL(k_trav_keys, dim3((unsigned)sc_cdiv(cn, 256)), dim3(256), 0,
+                                   ctx->stream, si, cn, lists, kc, okey, oid);
+            else
+                hipLaunchKernelGGL(k_scan_keys, dim3((unsigned)sc_cdiv(cn, 256)), dim3(256), 0,
+                                   ctx->stream, cinfo, qinfo, cn, P.ccap, okey, oid);
+            SC_CHECK_LAUNCH();
+            size_t tb = P.otmp_bytes;
+            SC_HIP(rocprim::radix_sort_pairs(s + P.off_otmp, tb, okey, okey2, oid, oid2,
+                                             (size_t)cn, 0u, 32u, ctx->stream));
+            const int nx = order == 2 && ctx->n_cu % 32 == 0 ? ctx->n_cu / 32 : 1;
+            hipLaunchKernelGGL(k_xcd_order, dim3((unsigned)sc_cdiv(cn, 256)), dim3(256), 0,
+                               ctx->stream, oid2, cn, nx, perm);
+            SC_CHECK_LAUNCH();
+            ordered = true;
+            return SC_OK;
+        };
+        if (order > 0 && cn > 1 && wave && trav_order_mode()) {
+            SC_TRY(sort_order(true));
+            ta.qlist = perm;
+        }
         if (three) {
             SC_TRY(traverse3(ctx, p, kc, sv, si, ix->L, cells, clen, cinfo, qinfo, P.ccap, cn, Tq));
         } else if (wave) {
@@ -2679,21 +2724,8 @@ static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
         const size_t lds = sizeof(ScanShared) +
                            sizeof(float) * std::max<size_t>((size_t)ix->mp * ix->kp,
                                                             p->form == SC_FORM_REAL ? ix->dim : 0);
-        const int order = scan_order_mode();
         if (order > 0 && cn > 1) {
-            uint32_t *okey = (uint32_t *)(s + P.off_okey), *okey2 = (uint32_t *)(s + P.off_okey2);
-            int32_t *oid = (int32_t *)(s + P.off_oid), *oid2 = (int32_t *)(s + P.off_oid2);
-            int32_t *perm = (int32_t *)(s + P.off_perm);
-            hipLaunchKernelGGL(k_scan_keys, dim3((unsigned)sc_cdiv(cn, 256)), dim3(256), 0,
-                               ctx->stream, cinfo, qinfo, cn, P.ccap, okey, oid);
-            SC_CHECK_LAUNCH();
-            size_t tb = P.otmp_bytes;
-            SC_HIP(rocprim::radix_sort_pairs(s + P.off_otmp, tb, okey, okey2, oid, oid2,
-                                             (size_t)cn, 0u, 32u, ctx->stream));
-            const int nx = order == 2 && ctx->n_cu % 32 == 0 ? ctx->n_cu / 32 : 1;
-            hipLaunchKernelGGL(k_xcd_order, dim3((unsigned)sc_cdiv(cn, 256)), dim3(256), 0,
-                               ctx->stream, oid2, cn, nx, perm);
-            SC_CHECK_LAUNCH();
+            if (!ordered) SC_TRY(sort_order(false));
             sa.qlist = perm;
         }
         if (hcap_eff > 0 || three) {
