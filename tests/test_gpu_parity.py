@@ -60,8 +60,7 @@ ENC_CASES = [  # dim, mc, kc, mp, kp, nc, n
 @pytest.mark.parametrize("case", ENC_CASES)
 def test_encode_matches_oracle(ctx, case, prec):
     dim, mc, kc, mp, kp, nc, n = case
-    if prec == S.PREC_BF16 and ((dim // mc) % 16 or (dim // mp) % 16):
-        pytest.skip("the BF16 form needs block lengths that are multiples of 16")
+    bf16_shape_ok = (dim // mc) % 16 == 0 and (dim // mp) % 16 == 0
     rng = np.random.default_rng(sum(case))
     mix = sco.mix_d960() if dim == 960 else sco.mix_d128(ncomp=7)
     x = sco.gen(mix, sco.STREAM_TRAIN, 0, n, dim)
@@ -71,8 +70,15 @@ def test_encode_matches_oracle(ctx, case, prec):
     bsc = dim // mc
     if kc > 1:
         x[:5, :bsc] = cq[0, 1]                    # exact ties on the coarse distance
-    cid, codes, st = S.encode(ctx, S.Dev.from_numpy(ctx, x), S.Dev.from_numpy(ctx, cq),
-                              S.Dev.from_numpy(ctx, pq), mc, kc, mp, kp, nc, prec)
+    args = (ctx, S.Dev.from_numpy(ctx, x), S.Dev.from_numpy(ctx, cq),
+            S.Dev.from_numpy(ctx, pq), mc, kc, mp, kp, nc, prec)
+    if prec == S.PREC_BF16 and not bf16_shape_ok:
+        # the BF16 form needs block lengths that are multiples of 16: refused
+        # before any launch, with the reason
+        with pytest.raises(S.ScError, match="multiples of 16"):
+            S.encode(*args)
+        return
+    cid, codes, st = S.encode(*args)
     ocid, ocodes = sco.encode(x, cq, pq, mc, kc, mp, kp, nc)
     assert np.array_equal(cid.numpy(), ocid)
     assert np.array_equal(codes.numpy(), ocodes)
