@@ -44,7 +44,11 @@ extern "C" {
 #define SC_PREC_FP32 0    /* the reference's arithmetic everywhere */
 #define SC_PREC_BF16 1    /* MFMA bf16 distance GEMM + exact re-check */
 #define SC_FORM_PLAIN 0   /* FP32 LUT scan */
-#define SC_FORM_MIXED 1   /* FP16 LUT scan + exact FP32 re-rank */
+#define SC_FORM_MIXED 1   /* FP16 LUT scan + exact FP32 re-rank: ids and
+                             distances bit-identical to SC_FORM_PLAIN.
+                             Kinds without a mixed scan (SC_KIND_SC3,
+                             SC_KIND_IMI) refuse it with SC_ERR_ARG rather
+                             than run the plain scan */
 #define SC_FORM_REAL 2    /* the reference's real_dist = true: every gathered
                              candidate ranked by its exact distance to its raw
                              vector (include/query.h:307-312,

@@ -387,8 +387,7 @@ int sco_heap4_pop(sco_heap4 *h, int32_t *a, int32_t *b, int32_t *c, int32_t *d) 
 
 /* Single-id heap of search_ivfadc: pq_insert/pq_pop (sc_utilities.h:229-300),
  * same array algorithm. */
-typedef struct { float *v; int32_t *id; int32_t pos, cap; } heap1;
-static void heap1_insert(heap1 *h, float v, int32_t id) {
+void sco_heap1_insert(sco_heap1 *h, float v, int32_t id) {
     if (h->pos >= h->cap || h->pos < 0) return;
     int32_t i = h->pos++;
     h->v[i] = v; h->id[i] = id;
@@ -401,7 +400,8 @@ static void heap1_insert(heap1 *h, float v, int32_t id) {
         i = j; j = i >> 1;
     }
 }
-static int32_t heap1_pop(heap1 *h) {
+int32_t sco_heap1_pop(sco_heap1 *h) {
+    if (h->pos <= 0) return -1;
     int32_t top = h->id[0];
     int32_t last = --h->pos;
     h->v[0] = h->v[last]; h->id[0] = h->id[last];
@@ -512,11 +512,11 @@ int sco_search(const sco_index *ix, int32_t kind, const float *q, int32_t R,
     int32_t count = 0, count2 = 0, sum = 0, e = 0, exhausted = 0;
     if (kind == SCO_IVFADC) {
         /* query.h:232-259 */
-        heap1 h = {hv, hb, 0, kc};
-        for (int32_t i = 0; i < kc; ++i) heap1_insert(&h, q_sum + diff_qc[i], i);
+        sco_heap1 h = {hv, hb, 0, kc};
+        for (int32_t i = 0; i < kc; ++i) sco_heap1_insert(&h, q_sum + diff_qc[i], i);
         h.pos = kc;
         for (int32_t i = 0; i < w; ++i) {
-            int32_t bid = heap1_pop(&h);
+            int32_t bid = sco_heap1_pop(&h);
             int32_t l = cell_len(ix->L, bid);
             prebuck[count] = bid; s1[count] = bid; s2[count] = 0; count++;
             sum += l;

@@ -142,6 +142,16 @@ void sco_heap4_insert(sco_heap4 *h, float v, int32_t a, int32_t b, int32_t c,
                       int32_t d);
 int sco_heap4_pop(sco_heap4 *h, int32_t *a, int32_t *b, int32_t *c, int32_t *d);
 
+/* Single-id heap of search_ivfadc (include/sc_utilities.h:229-300); pop
+ * returns the id, or -1 on an empty heap. */
+typedef struct {
+    float *v;
+    int32_t *id;
+    int32_t pos, cap;
+} sco_heap1;
+void sco_heap1_insert(sco_heap1 *h, float v, int32_t id);
+int32_t sco_heap1_pop(sco_heap1 *h);
+
 /* Canonical top-R over a candidate list (dist ASC, position ASC): writes the
  * R selected positions (sorted) to pos_out; returns min(R, n). */
 int32_t sco_topr(const float *d, int32_t n, int32_t R, int32_t *pos_out);

@@ -53,6 +53,11 @@ class Heap4(C.Structure):
                 ("cap", C.c_int32)]
 
 
+class Heap1(C.Structure):
+    _fields_ = [("v", C.c_void_p), ("id", C.c_void_p), ("pos", C.c_int32),
+                ("cap", C.c_int32)]
+
+
 def _load():
     if not os.path.exists(LIB_PATH):
         raise RuntimeError("oracle/liboracle.so missing: run `make -C oracle`")
@@ -83,6 +88,9 @@ def _load():
                                     C.c_int32, C.c_int32, _i32p, _f32p, C.c_void_p]
     lib.sco_heap4_insert.argtypes = [C.POINTER(Heap4), C.c_float] + [C.c_int32] * 4
     lib.sco_heap4_pop.argtypes = [C.POINTER(Heap4)] + [C.POINTER(C.c_int32)] * 4
+    lib.sco_heap1_insert.argtypes = [C.POINTER(Heap1), C.c_float, C.c_int32]
+    lib.sco_heap1_pop.argtypes = [C.POINTER(Heap1)]
+    lib.sco_heap1_pop.restype = C.c_int32
     lib.sco_topr.argtypes = [_f32p, C.c_int32, C.c_int32, _i32p]
     lib.sco_topr.restype = C.c_int32
     lib.sco_bruteforce_nn1.argtypes = [_f32p, C.c_int64, _f32p, C.c_int64,
@@ -355,6 +363,7 @@ def ref():
         r.scref_heap1_insert.argtypes = [_f32p, _i32p, C.c_float, C.c_int,
                                          C.POINTER(C.c_int32), C.c_int]
         r.scref_heap1_pop.argtypes = [_f32p, _i32p, C.POINTER(C.c_int32)]
+        r.scref_heap1_pop.restype = C.c_int
         r.scref_heap6_insert.argtypes = [_f32p] + [_i32p] * 6 + [C.c_float] + [C.c_int] * 6 + \
             [C.POINTER(C.c_int32), C.c_int]
         r.scref_heap6_pop.argtypes = [_f32p] + [_i32p] * 6 + [_i32p, C.POINTER(C.c_int32)]

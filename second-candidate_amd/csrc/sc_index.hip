@@ -289,8 +289,12 @@ extern "C" int sc_index_load(sc_ctx *ctx, const int32_t *h_L, int64_t ncells,
                              int32_t dim, const float *h_cq, int32_t mc, int32_t kc,
                              const float *h_pq, int32_t mp, int32_t kp, int32_t nc,
                              sc_index **out) {
-    SC_TRY(sc_ctx_enter(ctx));
-    if (!out || !h_L || !h_cq || !h_pq || (n > 0 && (!h_pid || !h_codes))) return SC_ERR_ARG;
+    // host arguments are validated before any device work (and without a
+    // device), so a malformed file is refused wherever it is loaded
+    if (!out || !h_L || !h_cq || !h_pq || (n > 0 && (!h_pid || !h_codes))) {
+        sc_set_error("index_load: null argument");
+        return SC_ERR_ARG;
+    }
     *out = nullptr;
     int64_t want;
     SC_TRY(check_shape(n, dim, mc, kc, mp, kp, nc, &want));
@@ -299,10 +303,27 @@ extern "C" int sc_index_load(sc_ctx *ctx, const int32_t *h_L, int64_t ncells,
                      (long long)want);
         return SC_ERR_ARG;
     }
+    // The kernels index codes and ids through L[] and pid[] without bounds
+    // checks, so a malformed inverted file is refused here, before upload.
     if (h_L[ncells - 1] != n) {
         sc_set_error("index_load: L[last]=%d != n=%lld", h_L[ncells - 1], (long long)n);
         return SC_ERR_ARG;
     }
+    for (int64_t c = 0; c < ncells; ++c) {
+        const int32_t prev = c > 0 ? h_L[c - 1] : 0;
+        if (h_L[c] < prev) {
+            sc_set_error("index_load: L is not a non-decreasing inclusive prefix from 0 "
+                         "(L[%lld]=%d after %d)", (long long)c, h_L[c], prev);
+            return SC_ERR_ARG;
+        }
+    }
+    for (int64_t i = 0; i < n; ++i)
+        if (h_pid[i] < 0 || h_pid[i] >= n) {
+            sc_set_error("index_load: pid[%lld]=%d outside [0, %lld)", (long long)i, h_pid[i],
+                         (long long)n);
+            return SC_ERR_ARG;
+        }
+    SC_TRY(sc_ctx_enter(ctx));
     sc_index *ix;
     SC_TRY(alloc_index(ctx, n, dim, mc, kc, mp, kp, nc, ncells, &ix));
     const size_t bsc = dim / mc, bsp = dim / mp;

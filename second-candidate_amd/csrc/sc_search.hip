@@ -2517,8 +2517,16 @@ static int search_impl(sc_ctx *ctx, const sc_index *ix, const float *d_q, int64_
                      p->w, ix->kc);
         return SC_ERR_HEAP_EMPTY;
     }
-    if ((kind == SC_KIND_SC && ix->kc < 2) || (kind == SC_KIND_SC3 && ix->kc < 3))
+    if ((kind == SC_KIND_SC && ix->kc < 2) || (kind == SC_KIND_SC3 && ix->kc < 3)) {
+        sc_set_error("sc_search: kind %d needs kc >= %d candidates per list (kc=%d)", kind,
+                     kind == SC_KIND_SC3 ? 3 : 2, ix->kc);
         return SC_ERR_ARG;
+    }
+    if (p->form == SC_FORM_MIXED && (kind == SC_KIND_SC3 || kind == SC_KIND_IMI)) {
+        sc_set_error("sc_search: SC_FORM_MIXED is not available for kind %d; "
+                     "use SC_FORM_PLAIN", kind);
+        return SC_ERR_ARG;
+    }
     if (ix->kc > 8192 || ix->mp * ix->kp > 8192) {
         sc_set_error("sc_search: kc=%d / mp*kp=%d exceed the on-chip ranking limits", ix->kc,
                      ix->mp * ix->kp);

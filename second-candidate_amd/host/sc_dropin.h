@@ -656,9 +656,18 @@ protected:
         const unsigned char *m =
             (const unsigned char *)mmap(nullptr, fsz, PROT_READ, MAP_PRIVATE, fd, 0);
         if (m == MAP_FAILED) detail::die("Cannot map the file");
-        const unsigned char *t = m;
+        // every read is bounded by the file size and every cell by N, so a
+        // truncated or mismatched file ends with a message, not an overrun
+        const unsigned char *t = m, *end = m + fsz;
+        auto bad = [&](const char *why) {
+            munmap((void *)m, fsz);
+            close(fd);
+            detail::die(why);
+        };
+        if (fsz < 8) bad("Encoded data file too short for its header");
         std::memcpy(&not_empty, t, 4); t += 4;
         std::memcpy(&config.N, t, 4); t += 4;
+        if (config.N < 0) bad("Encoded data file: negative N");
         ::operator delete(L);
         ::operator delete(pid);
         ::operator delete(codes);
@@ -668,15 +677,21 @@ protected:
         size_t base = 0;
         for (size_t i = 0; i < cells; ++i) {
             int l;
+            if ((size_t)(end - t) < 4) bad("Encoded data file truncated (cell length)");
             std::memcpy(&l, t, 4); t += 4;
+            if (l < 0 || base + (size_t)l > (size_t)config.N)
+                bad("Encoded data file: cell lengths exceed N");
             L[i] = l + (i > 0 ? L[i - 1] : 0);
             if (l > 0) {
+                const size_t need = (size_t)l * (4 + (size_t)config.mp);
+                if ((size_t)(end - t) < need) bad("Encoded data file truncated (cell body)");
                 std::memcpy(pid + base, t, (size_t)l * 4); t += (size_t)l * 4;
                 std::memcpy(codes + base * config.mp, t, (size_t)l * config.mp);
                 t += (size_t)l * config.mp;
                 base += (size_t)l;
             }
         }
+        if (base != (size_t)config.N) bad("Encoded data file: cell lengths do not sum to N");
         munmap((void *)m, fsz);
         close(fd);
         if (index_) sc_index_destroy(index_);

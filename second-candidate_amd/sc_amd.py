@@ -15,6 +15,8 @@ LIB_PATH = os.path.join(HERE, "lib", "libsc_amd.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "sc_hip.h")
 
 SC_OK = 0
+SC_ERR_ARG, SC_ERR_NOMEM, SC_ERR_CONFIG, SC_ERR_HIP = -1, -2, -3, -4
+SC_ERR_TOO_LARGE, SC_ERR_NO_DEVICE, SC_ERR_HEAP_EMPTY = -5, -6, -7
 KIND_IVFADC, KIND_IMI, KIND_SC, KIND_SC3 = 0, 1, 2, 3
 PREC_FP32, PREC_BF16 = 0, 1
 FORM_PLAIN, FORM_MIXED, FORM_REAL = 0, 1, 2

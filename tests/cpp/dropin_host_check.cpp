@@ -6,6 +6,10 @@
 // bit with the reference's SimpleCluster::distortion
 // (lib/simple-cluster/include/k-means.h:444-464) applied part by part.
 // usage: dropin_host_check            exit 0 = all checks passed
+//        dropin_host_check edat CQ PQ FILE
+//            PQQuery::load_codebooks + load_encoded_data on FILE: a malformed
+//            inverted file must end the process with a message before any
+//            device work (the caller checks the exit status and message)
 #include <cstdio>
 #include <cstring>
 #include <vector>
@@ -33,7 +37,14 @@ public:
 uint32_t lcg(uint32_t &s) { return s = s * 1664525u + 1013904223u; }
 }  // namespace
 
-int main() {
+int main(int argc, char **argv) {
+    if (argc == 5 && std::strcmp(argv[1], "edat") == 0) {
+        SC::PQQuery q;
+        q.load_codebooks(argv[2], argv[3], false);
+        q.load_encoded_data(argv[4], false);
+        std::printf("loaded\n");
+        return 0;
+    }
     int fails = 0;
     const int cases[][4] = {{32, 2, 16, 5000}, {128, 8, 64, 3000}, {48, 1, 7, 20000}};
     for (const auto &cs : cases) {

@@ -33,3 +33,35 @@ def test_status_codes_match_header():
     assert re.search(r"#define SC_OK 0\b", text)
     assert re.search(r"#define SC_KIND_SC 2\b", text)
     assert sc_amd.KIND_SC == 2 and sc_amd.FORM_MIXED == 1 and sc_amd.PREC_BF16 == 1
+
+
+def test_index_load_refuses_malformed_inverted_file():
+    """sc_index_load validates L and pid on the host before any device work:
+    L must be a non-decreasing inclusive prefix ending at n and every pid in
+    [0, n); otherwise SC_ERR_ARG with the reason (no GPU needed to reach it)."""
+    import numpy as np
+    L = sc_amd.lib()
+    kc, mp, kp, dim = 4, 2, 4, 8
+    cq = np.zeros((kc, dim), np.float32)
+    pq = np.zeros((mp, kp, dim // mp), np.float32)
+    out = C.c_void_p()
+
+    def load(Lc, pid):
+        Lc = np.ascontiguousarray(Lc, np.int32)
+        pid = np.ascontiguousarray(pid, np.int32)
+        codes = np.zeros((pid.size, mp), np.uint8)
+        rc = L.sc_index_load(None, Lc.ctypes.data, Lc.size, pid.ctypes.data, codes.ctypes.data,
+                             pid.size, dim, cq.ctypes.data, 1, kc, pq.ctypes.data, mp, kp, 1,
+                             C.byref(out))
+        return rc, L.sc_last_error().decode()
+
+    rc, msg = load([2, 1, 3, 5], [0, 1, 2, 3, 4])
+    assert rc == sc_amd.SC_ERR_ARG and "non-decreasing" in msg
+    rc, msg = load([2, 2, 3, 4], [0, 1, 2, 3, 4])
+    assert rc == sc_amd.SC_ERR_ARG and "L[last]" in msg
+    rc, msg = load([2, 2, 3, 5], [0, 1, 7, 3, 4])
+    assert rc == sc_amd.SC_ERR_ARG and "pid[2]=7" in msg
+    rc, msg = load([2, 2, 3, 5], [0, 1, -1, 3, 4])
+    assert rc == sc_amd.SC_ERR_ARG and "pid[2]=-1" in msg
+    rc, msg = load([2, 2, 3, 5], [0, 1, 2, 3, 4])       # well formed: stops at the context
+    assert rc == sc_amd.SC_ERR_ARG and "null context" in msg

@@ -183,7 +183,7 @@ SEARCH_CASES = [  # kind, R, w, T
     (S.KIND_SC, 100, 500, 3000),
     (S.KIND_SC, 100, 4, 100000),         # cell budget binds
     (S.KIND_SC, 300, 500, 40),           # fewer candidates than R
-    (S.KIND_SC, 7, 500, 0),              # T = 0: one cell
+    (S.KIND_SC, 7, 500, 0),              # T = 0: the loop test fails first, no cell
 ]
 
 
@@ -204,7 +204,12 @@ def _compare_search(ctx, ix, oix, kind, R, w, T, nq=150, form=S.FORM_PLAIN):
 def test_search_matches_oracle(ctx, case):
     kind, R, w, T = case
     ix, oix = _index_pair(ctx, kind)
-    _compare_search(ctx, ix, oix, kind, R, w, T)
+    st = _compare_search(ctx, ix, oix, kind, R, w, T)
+    if T == 0:
+        # include/sc_query.h:130 tests `sum < T` before every pop: with T = 0
+        # nothing is popped, no candidate is gathered and every result slot
+        # is empty
+        assert st.candidates == 0 and st.cells == 0 and st.visited == 0
     ix.close()
 
 
@@ -251,7 +256,7 @@ SC3_CASES = [  # kc, R, w, T
 def test_search_sc3_matches_oracle(ctx, case):
     """SCQuery::search_mr_ivf3: the (nearest, second, third) index built on
     the GPU equals the oracle's; ids, distances and candidate counts of the
-    plain and mixed forms equal the oracle's search."""
+    plain form equal the oracle's search; the mixed form is refused."""
     kc, R, w, T = case
     ix, oix = _index_pair(ctx, S.KIND_SC3, kc=kc)
     e = ix.export()
@@ -260,7 +265,10 @@ def test_search_sc3_matches_oracle(ctx, case):
     st = _compare_search(ctx, ix, oix, S.KIND_SC3, R, w, T, nq=80)
     if kc == 4 and T == 100000:
         assert st.visited == 80 * 64
-    _compare_search(ctx, ix, oix, S.KIND_SC3, R, w, T, nq=80, form=S.FORM_MIXED)
+    # the three-candidate kind has no mixed form: refused, not run plain
+    with pytest.raises(S.ScError, match="not available") as e:
+        _compare_search(ctx, ix, oix, S.KIND_SC3, R, w, T, nq=80, form=S.FORM_MIXED)
+    assert e.value.rc == S.SC_ERR_ARG
     with pytest.raises(S.ScError):
         ix.search(S.Dev.from_numpy(ctx, np.zeros((1, ix.dim), np.float32)), S.KIND_SC, R, w, T)
     ix.close()

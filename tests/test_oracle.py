@@ -223,6 +223,77 @@ def test_heap4_matches_reference_heap():
                 assert np.array_equal(x[:n], y[:n])
 
 
+@ref_missing
+@pytest.mark.ref
+def test_heap1_matches_reference_heap():
+    """The single-id heap of search_ivfadc (pq_insert/pq_pop,
+    include/sc_utilities.h:229-300) against the reference's compiled heap:
+    random insert/pop sequences with many ties, drops past capacity and pops
+    of an emptied heap; the arrays must agree after every step."""
+    import ctypes as C
+    rng = np.random.default_rng(14)
+    L, R = sco.lib(), sco.ref()
+    for trial in range(60):
+        cap = int(rng.integers(2, 48))
+        ov = np.zeros(cap, np.float32); oi = np.zeros(cap, np.int32)
+        rv = np.zeros(cap, np.float32); ri = np.zeros(cap, np.int32)
+        h = sco.Heap1(ov.ctypes.data, oi.ctypes.data, 0, cap)
+        rpos = C.c_int32(0)
+        for step in range(400):
+            if rng.random() < 0.55:
+                v = float(rng.integers(0, 5)) if trial % 2 else float(rng.standard_normal())
+                i = int(rng.integers(0, 10 ** 6))
+                L.sco_heap1_insert(C.byref(h), v, i)
+                R.scref_heap1_insert(rv, ri, v, i, C.byref(rpos), cap)
+            else:
+                assert L.sco_heap1_pop(C.byref(h)) == R.scref_heap1_pop(rv, ri, C.byref(rpos))
+            assert h.pos == rpos.value
+            n = h.pos
+            assert np.array_equal(ov[:n], rv[:n]) and np.array_equal(oi[:n], ri[:n])
+
+
+def _replay_ivfadc(ix, q, w, T):
+    """search_ivfadc steps 1-2 (include/query.h:225-259) driven by the
+    reference's compiled pq_insert/pq_pop: every coarse score inserted in
+    centroid order into a heap of capacity kc, then up to w pops, stopping
+    once the budget T is reached."""
+    import ctypes as C
+    R = sco.ref()
+    dqc, _ = sco.lut(q, ix.cq, ix.pq, ix.norm_c, ix.norm_r, ix.dim, ix.mc, ix.kc, ix.mp, ix.kp)
+    kc = ix.kc
+    acc = np.float32(0)
+    for t in q:
+        acc = np.float32(acc + np.float32(t * t))
+    hv = np.zeros(kc, np.float32); hid = np.zeros(kc, np.int32)
+    pos = C.c_int32(0)
+    for i in range(kc):
+        R.scref_heap1_insert(hv, hid, np.float32(acc + dqc[i]), i, C.byref(pos), kc)
+    cells, total = [], 0
+    for _ in range(w):
+        bid = R.scref_heap1_pop(hv, hid, C.byref(pos))
+        cells.append(bid)
+        total += int(ix.L[bid] - ix.L[bid - 1]) if bid > 0 else int(ix.L[0])
+        if total >= T:
+            break
+    return cells, total
+
+
+@ref_missing
+@pytest.mark.ref
+def test_ivfadc_traversal_replay_with_reference_heap():
+    """The oracle's IVFADC traversal equals the replay on the reference's own
+    single-id heap, over budgets that stop early, late and at w."""
+    for kc in (8, 37):
+        ix, x, _, _ = _make_index(sco.IVFADC, n=3000, kc=kc)
+        q = sco.gen(sco.mix_d128(ncomp=9), sco.STREAM_QUERY, 0, 10, ix.dim)
+        for qi in range(10):
+            for w, T in ((1, 10 ** 9), (kc // 2, 700), (kc, 10 ** 9), (kc, 1)):
+                ids, dists, st, ex = ix.search(sco.IVFADC, q[qi], R=15, w=w, T=T, trace=True)
+                cells, total = _replay_ivfadc(ix, q[qi], w, T)
+                assert list(ex["cells"]) == cells
+                assert st.sum == total
+
+
 def test_heap_lone_child_is_not_compared():
     """Known answer of the array heap (children of i are {2i, 2i+1}; the
     sift-down stops while j + 1 < size, so a lone last child is never
@@ -637,6 +708,71 @@ def test_kmeans_pp_and_lloyd_properties():
         if prev is not None:
             assert obj <= prev + 1e-6 * prev
         prev = obj
+
+
+def _kpp_replay(x, k, seed, lagged):
+    """k-means++ seeding restated with numpy (the Philox stream checked by the
+    Random123 vectors above): first seed u64(round 0) mod n; then weights
+    w_i = floor(D2_i * 2^16), pivot p = u64(round c) mod W, pick = the first i
+    whose inclusive prefix sum of w exceeds p (W = 0: u64 mod n); D2 folds in
+    the newest seed, or the previous one when lagged (k-means.h:244-250)."""
+    def u64(rnd, which):
+        o = sco.philox([rnd, 0, 0xA5, which], [seed, 1])
+        return (int(o[1]) << 32) | int(o[0])
+    n = x.shape[0]
+    idx = [u64(0, 0) % n]
+    d2 = _s1_closed(x, x[idx[0]][None, :])
+    for c in range(1, k):
+        w = np.ldexp(d2, 16).astype(np.uint64)
+        W = int(w.sum(dtype=np.uint64))
+        r = u64(c, 1)
+        if W == 0:
+            pick = r % n
+        else:
+            pick = int(np.searchsorted(np.cumsum(w, dtype=np.uint64), np.uint64(r % W),
+                                       side="right"))
+        idx.append(pick)
+        d2 = np.minimum(d2, _s1_closed(x, x[idx[c - 1] if lagged else idx[c]][None, :]))
+    return np.array(idx, np.int64)
+
+
+@pytest.mark.parametrize("lagged", [0, 1])
+def test_kmeans_pp_sampler_exact_replay(lagged):
+    """The oracle's seeds equal an independent numpy replay of the defined
+    sampler, pick for pick, incl. duplicated points (D2 = 0 weights)."""
+    m = sco.mix_d128(ncomp=5)
+    x = sco.gen(m, sco.STREAM_TRAIN, 0, 700, 8)
+    x[100:140] = x[5]                     # duplicates: zero weight once x[5] is a seed
+    for seed in (1, 7, 20151001):
+        for k in (2, 9, 40):
+            want = _kpp_replay(x, k, seed, lagged)
+            assert np.array_equal(sco.kmeans_pp(x, k, seed=seed, lagged=lagged), want)
+    # all points identical: W = 0 at every round, picks fall back to u64 mod n
+    z = np.full((13, 4), 3.0, np.float32)
+    assert np.array_equal(sco.kmeans_pp(z, 6, seed=3, lagged=lagged), _kpp_replay(z, 6, 3, lagged))
+
+
+def test_kmeans_pp_second_pick_is_d2_weighted():
+    """Distribution of the second seed over many seeds: P(j | first f) must be
+    D2(j, f) / sum D2 (k-means.h:216-230), not D / sum D: on the points
+    {0, 1, 3, 7} the two rules differ by up to 0.19 in probability."""
+    x = np.array([[0.0], [1.0], [3.0], [7.0]], np.float32)
+    counts = np.zeros((4, 4))
+    for seed in range(4000):
+        f, s = sco.kmeans_pp(x, 2, seed=seed + 11)
+        counts[f, s] += 1
+    for f in range(4):
+        tot = counts[f].sum()
+        assert tot > 600
+        d2 = (x[:, 0].astype(np.float64) - x[f, 0]) ** 2
+        p = d2 / d2.sum()
+        assert counts[f, f] == 0
+        # 5 sigma of a binomial count around the D^2 expectation
+        sig = np.sqrt(tot * p * (1 - p)) + 1
+        assert np.all(np.abs(counts[f] - tot * p) <= 5 * sig), (f, counts[f], tot * p)
+        pd = np.abs(x[:, 0].astype(np.float64) - x[f, 0])
+        pd = pd / pd.sum()
+        assert np.max(np.abs(counts[f] / tot - pd)) > 0.05   # D weighting would be rejected
 
 
 def test_residual_second_nearest():
