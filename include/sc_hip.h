@@ -262,6 +262,17 @@ int sc_residual(sc_ctx *ctx, float *d_x, int64_t n, int32_t dim,
  * Synchronises the context stream. */
 int sc_groundtruth_nn1(sc_ctx *ctx, const float *d_base, int64_t n, const float *d_q,
                        int64_t nq, int32_t dim, int32_t *d_ids, double *d_dists);
+/* sc_groundtruth_nn1 without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc).  The
+ * per-call partial results live in a fixed 16 MiB device array of the library
+ * (one per GPU, held until the process ends), so nq * chunks * 12 bytes must
+ * fit: nq is limited to 1398101 queries per call (SC_ERR_ARG beyond), and
+ * fewer base chunks are used for large nq.  Results are identical to
+ * sc_groundtruth_nn1.  Every call on one GPU shares that array: calls that
+ * overlap on one GPU (other threads, other streams) race. */
+int sc_groundtruth_nn1_noalloc(sc_ctx *ctx, const float *d_base, int64_t n, const float *d_q,
+                               int64_t nq, int32_t dim, int32_t *d_ids, double *d_dists);
 
 #ifdef __cplusplus
 }

@@ -70,54 +70,96 @@ __global__ void k_nn1_merge(const double *__restrict__ pd, const int32_t *__rest
     if (dists) dists[i] = bd;
 }
 
+// Work split of one call: enough (query tile, chunk) workgroups to fill the
+// chip several times, with at most max_partials partial winners in all.
+// Returns the number of chunks (0 when even one chunk per query exceeds
+// max_partials) and the chunk length in *chunk.
+template <int TK>
+int64_t nn1_chunks(int64_t n, int64_t nq, int64_t qtiles, int64_t max_partials, int64_t *chunk) {
+    int64_t want = std::max<int64_t>(1, sc_cdiv(4096, qtiles));
+    want = std::min<int64_t>(want, max_partials / nq);
+    if (want < 1) return 0;
+    int64_t c = std::max<int64_t>(TK, sc_cdiv(n, want));
+    c = sc_cdiv(c, TK) * TK;
+    *chunk = c;
+    return sc_cdiv(n, c);
+}
+
+template <int TV, int TK>
+size_t nn1_lds(int dim) {
+    return (size_t)dim * TV * 4 + kTT * TK * 4 + (TK / 4) * TV * sizeof(Best2) + TV * 4 + TV * 8;
+}
+
+// Launches the partial and merge kernels over pd/pk ([nq][nchunks] partial
+// winners) and waits for them.
 template <int TV, int TK>
 int launch_nn1(sc_ctx *ctx, const float *base, int64_t n, const float *q, int64_t nq, int dim,
-               int32_t *ids, double *dists) {
+               int64_t chunk, int nchunks, double *pd, int32_t *pk, int32_t *ids,
+               double *dists) {
     constexpr int NT = (TV / 4) * (TK / 4);
-    const size_t lds = (size_t)dim * TV * 4 + kTT * TK * 4 + (TK / 4) * TV * sizeof(Best2) +
-                       TV * 4 + TV * 8;
-    if (lds > 160 * 1024) {
-        sc_set_error("sc_groundtruth_nn1: dim %d too large for the exact tile", dim);
-        return SC_ERR_ARG;
-    }
-    // enough (query tile, chunk) workgroups to fill the chip several times
-    const int64_t qtiles = sc_cdiv(nq, TV);
-    int64_t want = std::max<int64_t>(1, sc_cdiv(4096, qtiles));
-    int64_t chunk = std::max<int64_t>(TK, sc_cdiv(n, want));
-    chunk = sc_cdiv(chunk, TK) * TK;
-    const int nchunks = (int)sc_cdiv(n, chunk);
+    const size_t lds = nn1_lds<TV, TK>(dim);
     hipStream_t sm = ctx->stream;
-    double *pd = nullptr;
-    int32_t *pk = nullptr;
-    const size_t np = (size_t)nq * nchunks;
-    if (hipMalloc(&pd, 8 * np) != hipSuccess || hipMalloc(&pk, 4 * np) != hipSuccess) {
-        (void)hipFree(pd);
-        sc_set_error("sc_groundtruth_nn1: allocation failed");
-        return SC_ERR_NOMEM;
-    }
     auto kern = k_nn1_partial<TV, TK>;
     int rc = SC_OK;
     if (hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)lds) != hipSuccess)
         rc = SC_ERR_HIP;
     if (rc == SC_OK) {
-        hipLaunchKernelGGL(kern, dim3((unsigned)qtiles, (unsigned)nchunks), dim3(NT), lds, sm,
-                           base, n, q, nq, dim, chunk, nchunks, pd, pk);
+        hipLaunchKernelGGL(kern, dim3((unsigned)sc_cdiv(nq, TV), (unsigned)nchunks), dim3(NT),
+                           lds, sm, base, n, q, nq, dim, chunk, nchunks, pd, pk);
         hipLaunchKernelGGL(k_nn1_merge, dim3((unsigned)sc_cdiv(nq, 256)), dim3(256), 0, sm, pd,
                            pk, nq, nchunks, ids, dists);
         if (hipGetLastError() != hipSuccess) rc = SC_ERR_HIP;
     }
-    (void)hipStreamSynchronize(sm);
-    (void)hipFree(pd);
-    (void)hipFree(pk);
+    if (hipStreamSynchronize(sm) != hipSuccess) rc = SC_ERR_HIP;
     if (rc != SC_OK) sc_set_error("sc_groundtruth_nn1: launch failed");
     return rc;
 }
 
-}  // namespace
+// fixed: partial winners go to the context's fixed scratch (no allocation)
+// instead of device memory allocated for the call.
+template <int TV, int TK>
+int run_nn1(sc_ctx *ctx, const float *base, int64_t n, const float *q, int64_t nq, int dim,
+            int32_t *ids, double *dists, bool fixed) {
+    if (nn1_lds<TV, TK>(dim) > 160 * 1024) {
+        sc_set_error("sc_groundtruth_nn1: dim %d too large for the exact tile", dim);
+        return SC_ERR_ARG;
+    }
+    constexpr int64_t kPartial = 12;                 // one double + one int32
+    const int64_t qtiles = sc_cdiv(nq, TV);
+    const int64_t max_partials =
+        fixed ? (int64_t)(ctx->fixed_scratch_bytes / kPartial) : INT64_MAX / kPartial;
+    int64_t chunk = 0;
+    const int64_t nchunks = nn1_chunks<TK>(n, nq, qtiles, max_partials, &chunk);
+    if (nchunks < 1) {
+        sc_set_error("sc_groundtruth_nn1_noalloc: %lld queries exceed the fixed scratch "
+                     "(%zu bytes: at most %lld queries per call)", (long long)nq,
+                     ctx->fixed_scratch_bytes, (long long)max_partials);
+        return SC_ERR_ARG;
+    }
+    const size_t np = (size_t)nq * (size_t)nchunks;
+    double *pd = nullptr;
+    int32_t *pk = nullptr;
+    if (fixed) {
+        pd = reinterpret_cast<double *>(ctx->fixed_scratch);
+        pk = reinterpret_cast<int32_t *>(pd + np);
+        return launch_nn1<TV, TK>(ctx, base, n, q, nq, dim, chunk, (int)nchunks, pd, pk, ids,
+                                  dists);
+    }
+    if (hipMalloc(&pd, 8 * np) != hipSuccess || hipMalloc(&pk, 4 * np) != hipSuccess) {
+        (void)hipFree(pd);
+        sc_set_error("sc_groundtruth_nn1: allocation failed");
+        return SC_ERR_NOMEM;
+    }
+    const int rc =
+        launch_nn1<TV, TK>(ctx, base, n, q, nq, dim, chunk, (int)nchunks, pd, pk, ids, dists);
+    (void)hipFree(pd);
+    (void)hipFree(pk);
+    return rc;
+}
 
-extern "C" int sc_groundtruth_nn1(sc_ctx *ctx, const float *d_base, int64_t n, const float *d_q,
-                                  int64_t nq, int32_t dim, int32_t *d_ids, double *d_dists) {
+int groundtruth(sc_ctx *ctx, const float *d_base, int64_t n, const float *d_q, int64_t nq,
+                int32_t dim, int32_t *d_ids, double *d_dists, bool fixed) {
     SC_TRY(sc_ctx_enter(ctx));
     if (!d_base || !d_q || !d_ids || n < 1 || n > INT32_MAX || nq < 0 || dim < 1 ||
         dim > 4096) {
@@ -126,6 +168,23 @@ extern "C" int sc_groundtruth_nn1(sc_ctx *ctx, const float *d_base, int64_t n, c
         return SC_ERR_ARG;
     }
     if (nq == 0) return SC_OK;
-    if (dim <= 256) return launch_nn1<64, 64>(ctx, d_base, n, d_q, nq, dim, d_ids, d_dists);
-    return launch_nn1<16, 256>(ctx, d_base, n, d_q, nq, dim, d_ids, d_dists);
+    if (dim <= 256) return run_nn1<64, 64>(ctx, d_base, n, d_q, nq, dim, d_ids, d_dists, fixed);
+    return run_nn1<16, 256>(ctx, d_base, n, d_q, nq, dim, d_ids, d_dists, fixed);
+}
+
+}  // namespace
+
+extern "C" int sc_groundtruth_nn1(sc_ctx *ctx, const float *d_base, int64_t n, const float *d_q,
+                                  int64_t nq, int32_t dim, int32_t *d_ids, double *d_dists) {
+    return groundtruth(ctx, d_base, n, d_q, nq, dim, d_ids, d_dists, false);
+}
+
+// The same computation with the partial winners in the library's fixed
+// scratch: no device memory is allocated, so callers that cannot allocate
+// (code under graph capture, serving loops) can call it.  The scratch is
+// shared by every call on one GPU: calls that overlap on one GPU race.
+extern "C" int sc_groundtruth_nn1_noalloc(sc_ctx *ctx, const float *d_base, int64_t n,
+                                          const float *d_q, int64_t nq, int32_t dim,
+                                          int32_t *d_ids, double *d_dists) {
+    return groundtruth(ctx, d_base, n, d_q, nq, dim, d_ids, d_dists, true);
 }

@@ -12,6 +12,8 @@ struct sc_ctx {
     void *scratch;          // growable device scratch for temporaries
     size_t scratch_bytes;
     int n_cu;               // compute units
+    void *fixed_scratch;    // this device's copy of the library's static scratch
+    size_t fixed_scratch_bytes;
 };
 
 struct sc_index {

@@ -9,6 +9,14 @@
 
 static thread_local char g_err[512];
 
+// Fixed device scratch for the allocation-free entry points (partial results
+// of one call).  Every GPU the library runs on holds its own copy for the
+// life of the process.  Host code never passes this name to a kernel: the
+// device address of this GPU's copy is looked up once per context (below)
+// and kept in sc_ctx::fixed_scratch.
+constexpr size_t kFixedScratchBytes = (size_t)16 << 20;   // 16 MiB
+static __device__ __align__(256) unsigned char scratch[kFixedScratchBytes];
+
 void sc_set_error(const char *fmt, ...) {
     va_list ap;
     va_start(ap, fmt);
@@ -50,6 +58,16 @@ extern "C" int sc_ctx_create(int device, sc_ctx **out) {
     c->scratch = nullptr;
     c->scratch_bytes = 0;
     c->n_cu = prop.multiProcessorCount;
+    // this device's copy of the static scratch (the device is current here)
+    c->fixed_scratch = nullptr;
+    c->fixed_scratch_bytes = 0;
+    hipError_t se = hipGetSymbolAddress(&c->fixed_scratch, HIP_SYMBOL(scratch));
+    if (se != hipSuccess || !c->fixed_scratch) {
+        delete c;
+        sc_set_error("hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(se));
+        return SC_ERR_HIP;
+    }
+    c->fixed_scratch_bytes = kFixedScratchBytes;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         sc_set_error("hipStreamCreate failed");

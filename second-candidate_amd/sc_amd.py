@@ -95,6 +95,8 @@ _SIGS = {
     "sc_encode_general": (C.c_int, [_P, _P, C.c_int64, C.c_int32, _P, C.c_int32, C.c_int32, _P,
                                     C.c_int32, C.c_int32, C.c_int32, _P, _P]),
     "sc_groundtruth_nn1": (C.c_int, [_P, _P, C.c_int64, _P, C.c_int64, C.c_int32, _P, _P]),
+    "sc_groundtruth_nn1_noalloc": (C.c_int, [_P, _P, C.c_int64, _P, C.c_int64, C.c_int32, _P,
+                                             _P]),
     "sc_index_shape": (C.c_int, [_P, _P]),
     "sc_index_export": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P]),
     "sc_search": (C.c_int, [_P, _P, _P, C.c_int64, C.POINTER(SearchParams), _P, _P, _P,
