@@ -39,6 +39,21 @@ extern "C" {
 #define SC_KIND_IMI 1     /* MultiQuery::search_multi2 include/multi_query.h:62-273 */
 #define SC_KIND_SC 2      /* SCQuery::search_mr_ivf    include/sc_query.h:60-256 */
 #define SC_KIND_SC3 3     /* SCQuery::search_mr_ivf3   include/sc_query.h:265-475 */
+#define SC_KIND_SCIMI 4   /* (nearest, second) cells of two half-blocks
+                             (mc = 2, nc = 2): the cell key of
+                             src/sc_encoder.cpp:39-50 over a sparse directory
+                             of the non-empty cells (the reference's dense
+                             kc^4 array aborts, src/sc_encoder.cpp:26-33, and
+                             it has no search for it).  Cells are taken in
+                             order of score = (v0[n0] + v0[s0]) +
+                             (v1[n1] + v1[s1]) -- each half's SC pair score
+                             (include/sc_query.h:113-166) summed as the IMI
+                             sums its lists (include/multi_query.h:175-203),
+                             v_h[j] = q_h + diff_qc[h*kc + j] -- ties by cell
+                             key, while count < w and sum < T; each cell is
+                             scanned as search_multi2 scans
+                             (include/multi_query.h:209-262) with the nearest
+                             centroid of each half. */
 
 /* ---- precision / forms ------------------------------------------------- */
 #define SC_PREC_FP32 0    /* the reference's arithmetic everywhere */
@@ -47,8 +62,8 @@ extern "C" {
 #define SC_FORM_MIXED 1   /* FP16 LUT scan + exact FP32 re-rank: ids and
                              distances bit-identical to SC_FORM_PLAIN.
                              Kinds without a mixed scan (SC_KIND_SC3,
-                             SC_KIND_IMI) refuse it with SC_ERR_ARG rather
-                             than run the plain scan */
+                             SC_KIND_IMI, SC_KIND_SCIMI) refuse it with
+                             SC_ERR_ARG rather than run the plain scan */
 #define SC_FORM_REAL 2    /* the reference's real_dist = true: every gathered
                              candidate ranked by its exact distance to its raw
                              vector (include/query.h:307-312,
@@ -136,7 +151,11 @@ int sc_encode_general(sc_ctx *ctx, const float *d_x, int64_t n, int32_t dim,
  * Dense cells: kc^mc (nc=1), kc^2 (nc=2, mc=1) or kc^3 (nc=3, mc=1, kc <= 1024;
  * d_cid in sc_encode_general's layout [n][3]; SCEncoder::distribution with
  * the per-vector cid stride mc*nc that src/sc_encoder.cpp:41-59 gets wrong
- * for nc != 2). */
+ * for nc != 2).  nc=2 with mc=2 (two halves, SC_KIND_SCIMI; kc <= 65535)
+ * keeps only the non-empty cells: a sparse directory of 64-bit keys
+ * uid_0*kc^2 + uid_1, uid_h = nearest_h*kc + second_h, ascending, with L
+ * the inclusive prefix over those cells (sc_index_shape reports their
+ * number, sc_index_cell_keys their keys). */
 int sc_index_build(sc_ctx *ctx, const uint16_t *d_cid, const uint8_t *d_codes,
                    int64_t n, int32_t dim, const float *d_cq, int32_t mc,
                    int32_t kc, const float *d_pq, int32_t mp, int32_t kp,
@@ -148,6 +167,16 @@ int sc_index_load(sc_ctx *ctx, const int32_t *h_L, int64_t ncells,
                   int32_t dim, const float *h_cq, int32_t mc, int32_t kc,
                   const float *h_pq, int32_t mp, int32_t kp, int32_t nc,
                   sc_index **out);
+/* The two-halves index (mc = 2, nc = 2) from host arrays: keys [ncells]
+ * strictly ascending below kc^4, L [ncells] the inclusive prefix over those
+ * (non-empty) cells, pid [n], codes [n][mp] in cell order; h_cq
+ * [2][kc][dim/2].  Validated on the host before any device work. */
+int sc_index_load_sparse(sc_ctx *ctx, const uint64_t *h_ckeys, const int32_t *h_L,
+                         int64_t ncells, const int32_t *h_pid, const uint8_t *h_codes,
+                         int64_t n, int32_t dim, const float *h_cq, int32_t kc,
+                         const float *h_pq, int32_t mp, int32_t kp, sc_index **out);
+/* The sparse directory's keys (two-halves index only), [ncells]. Synchronous. */
+int sc_index_cell_keys(sc_ctx *ctx, const sc_index *ix, uint64_t *h_keys);
 int sc_index_destroy(sc_index *ix);
 /* Gives the index the raw base vectors d_raw [n][dim] (float, in vector id
  * order) for SC_FORM_REAL searches: PQQuery::load_data

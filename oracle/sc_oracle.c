@@ -284,6 +284,32 @@ int sco_build_dense(const uint64_t *keys, int64_t n, int64_t ncells,
     return 0;
 }
 
+/* Sparse buckets: the same members and order as sco_build_dense, with only
+ * the non-empty cells kept, each named by its key (a stable counting of a
+ * sorted copy: keys ascending, ids ascending within a key). */
+typedef struct { uint64_t k; int64_t i; } key_id;
+static int cmp_key_id(const void *a, const void *b) {
+    const key_id *x = (const key_id *)a, *y = (const key_id *)b;
+    if (x->k != y->k) return x->k < y->k ? -1 : 1;
+    return (x->i > y->i) - (x->i < y->i);
+}
+int64_t sco_build_sparse(const uint64_t *keys, int64_t n, const uint8_t *codes, int32_t mp,
+                         uint64_t *ckeys, int32_t *L, int32_t *pid, uint8_t *codes_sorted) {
+    key_id *t = (key_id *)malloc(sizeof(key_id) * ((size_t)n + 1));
+    if (!t) return SCO_ERR_NOMEM;
+    for (int64_t i = 0; i < n; ++i) { t[i].k = keys[i]; t[i].i = i; }
+    qsort(t, (size_t)n, sizeof(key_id), cmp_key_id);
+    int64_t nc = 0;
+    for (int64_t p = 0; p < n; ++p) {
+        if (p == 0 || t[p].k != t[p - 1].k) ckeys[nc++] = t[p].k;
+        L[nc - 1] = (int32_t)(p + 1);
+        pid[p] = (int32_t)t[p].i;
+        memcpy(codes_sorted + (size_t)p * mp, codes + (size_t)t[p].i * mp, (size_t)mp);
+    }
+    free(t);
+    return nc;
+}
+
 /* PQQuery::pre_compute1 (include/query.h:91-149): sequential FP32 sums. */
 void sco_tables(const float *cq, const float *pq, int32_t dim, int32_t mc,
                 int32_t kc, int32_t mp, int32_t kp, float *norm_c,
@@ -459,6 +485,69 @@ static inline int32_t cell_len(const int32_t *L, int64_t bid) {
     return bid > 0 ? L[bid] - L[bid - 1] : L[0];
 }
 
+/* ---------------------------------------------------------------------- */
+/* Two-halves dense-partition search (SCO_SCIMI).  The reference defines the */
+/* cell (src/sc_encoder.cpp:39-50: uid_h = nearest_h*kc + second_h, key =   */
+/* uid_0*kc^2 + uid_1) but neither builds (26-33 abort) nor searches it.     */
+/* This build's definition:                                                  */
+/*  1. per half h: q_h = sequential FP32 sum of the half's squares, q_sum =  */
+/*     q_0 + q_1, v_h[j] = q_h + diff_qc[h*kc + j] (include/multi_query.h:   */
+/*     102-122);                                                             */
+/*  2. every non-empty cell gets the score its rank pairs would have in the  */
+/*     multi-sequences: per half the SC pair score v_h[nearest] +            */
+/*     v_h[second] (include/sc_query.h:113-124, 151-166), and the IMI sum of */
+/*     the two (include/multi_query.h:175-203):                              */
+/*        score = (v_0[n_0] + v_0[s_0]) + (v_1[n_1] + v_1[s_1])   (FP32);     */
+/*  3. cells are taken in (score ASC, key ASC) order while count < w and    */
+/*     sum < T (search_mr_ivf's loop test, include/sc_query.h:130, w counts  */
+/*     non-empty cells) -- the order a multi-sequence with an exact priority */
+/*     queue would visit them in, without walking the ~kc^4 empty cells;    */
+/*  4. the IMI scan (include/multi_query.h:209-262) with h3 = n_0, h4 = n_1: */
+/*     each vector's residual is against its nearest centroid of each half   */
+/*     (include/sc_encoder.h:159-163).                                       */
+/* ---------------------------------------------------------------------- */
+typedef struct { float s; uint64_t key; int64_t c; } cell_rank;
+static int cmp_cell_rank(const void *a, const void *b) {
+    const cell_rank *x = (const cell_rank *)a, *y = (const cell_rank *)b;
+    if (x->s < y->s) return -1;
+    if (x->s > y->s) return 1;
+    return (x->key > y->key) - (x->key < y->key);
+}
+
+/* Steps 2-3 for one query (v [2][kc] by centroid id); cells/s1/s2 [w] get the
+ * directory index and the two nearest centroids of each taken cell. */
+static int scimi_rank(const sco_index *ix, const float *v, int32_t w, int32_t T,
+                      int32_t *prebuck, int32_t *s1, int32_t *s2, int32_t *count_o,
+                      int32_t *sum_o) {
+    const int64_t kc = ix->kc, kc2 = kc * kc;
+    cell_rank *r = (cell_rank *)malloc(sizeof(cell_rank) * ((size_t)ix->ncells + 1));
+    if (!r) return SCO_ERR_NOMEM;
+    for (int64_t c = 0; c < ix->ncells; ++c) {
+        const uint64_t key = ix->ckeys[c];
+        const int64_t u0 = (int64_t)(key / (uint64_t)kc2), u1 = (int64_t)(key % (uint64_t)kc2);
+        const float a = v[u0 / kc] + v[u0 % kc];
+        const float b = v[kc + u1 / kc] + v[kc + u1 % kc];
+        r[c].s = a + b;
+        r[c].key = key;
+        r[c].c = c;
+    }
+    qsort(r, (size_t)ix->ncells, sizeof(cell_rank), cmp_cell_rank);
+    int32_t count = 0, sum = 0;
+    for (int64_t i = 0; i < ix->ncells && count < w && sum < T; ++i) {
+        const int64_t c = r[i].c;
+        const uint64_t key = r[i].key;
+        prebuck[count] = (int32_t)c;
+        s1[count] = (int32_t)(key / (uint64_t)kc2 / (uint64_t)kc);
+        s2[count] = (int32_t)(key % (uint64_t)kc2 / (uint64_t)kc);
+        count++;
+        sum += cell_len(ix->L, c);
+    }
+    free(r);
+    *count_o = count;
+    *sum_o = sum;
+    return 0;
+}
+
 int sco_search(const sco_index *ix, int32_t kind, const float *q, int32_t R,
                int32_t w, int32_t T, int32_t M, int32_t *ids, float *dists,
                sco_stats *st, int32_t *cells_out, int32_t *s1_out,
@@ -472,6 +561,8 @@ int sco_search(const sco_index *ix, int32_t kind, const float *q, int32_t R,
     if (kind == SCO_SC && mc != 1) return SCO_ERR_CONFIG;
     if (kind == SCO_SC3 && (mc != 1 || kc > 1024 || ix->ncells < (int64_t)kc * kc * kc))
         return SCO_ERR_CONFIG;
+    if (kind == SCO_SCIMI && (mc != 2 || !ix->ckeys || kc < 2)) return SCO_ERR_CONFIG;
+    if (kind != SCO_SCIMI && ix->ckeys) return SCO_ERR_CONFIG;
     if (kind == SCO_IVFADC && w > kc) return SCO_ERR_ARG; /* pq_pop on empty -> exit */
 
     float *diff_qc = (float *)malloc(sizeof(float) * (size_t)mc * kc);
@@ -486,16 +577,19 @@ int sco_search(const sco_index *ix, int32_t kind, const float *q, int32_t R,
     int32_t *s1 = (int32_t *)malloc(sizeof(int32_t) * (size_t)w);
     int32_t *s2 = (int32_t *)malloc(sizeof(int32_t) * (size_t)w);
     unsigned char *trav = NULL;
-    if (kind != SCO_IVFADC)
+    if (kind != SCO_IVFADC && kind != SCO_SCIMI)
         trav = (unsigned char *)calloc((size_t)kc * kc * (kind == SCO_SC3 ? (size_t)kc : 1), 1);
     int rc = 0;
     if (!diff_qc || !diff_qr || !v || !tmp || !stmp || !hv || !hb || !prebuck ||
-        !s1 || !s2 || (kind != SCO_IVFADC && !trav)) { rc = SCO_ERR_NOMEM; goto done; }
+        !s1 || !s2 || (kind != SCO_IVFADC && kind != SCO_SCIMI && !trav)) {
+        rc = SCO_ERR_NOMEM;
+        goto done;
+    }
 
     sco_lut(q, ix->cq, ix->pq, ix->norm_c, ix->norm_r, dim, mc, kc, mp, kp, diff_qc, diff_qr);
 
     float q_sum = 0.0f, qh[2] = {0.0f, 0.0f};
-    if (kind == SCO_IMI) {
+    if (kind == SCO_IMI || kind == SCO_SCIMI) {
         /* multi_query.h:102-112 */
         int32_t bsc = dim / mc;
         for (int32_t i = 0; i < mc; ++i) {
@@ -558,6 +652,13 @@ int sco_search(const sco_index *ix, int32_t kind, const float *q, int32_t R,
                     sco_heap4_insert(&h, v2[h1] + v3[h2 + 1], h1, h2 + 1, t1[h1], t2[h2 + 1]);
             }
         }
+    } else if (kind == SCO_SCIMI) {
+        for (int32_t i = 0; i < mc; ++i)
+            for (int32_t j = 0; j < kc; ++j) v[i * kc + j] = qh[i] + diff_qc[i * kc + j];
+        rc = scimi_rank(ix, v, w, T, prebuck, s1, s2, &count, &sum);
+        if (rc) goto done;
+        count2 = count;
+        exhausted = count < w && sum < T;   /* every non-empty cell taken */
     } else if (kind == SCO_SC3) {
         /* sc_query.h:296-402: (nearest, second, third) cells over one coarse
          * list; the heap holds rank triples, capacity w */
@@ -635,7 +736,7 @@ int sco_search(const sco_index *ix, int32_t kind, const float *q, int32_t R,
         int32_t l = cell_len(ix->L, bid);
         int64_t start = bid > 0 ? ix->L[bid - 1] : 0;
         const uint8_t *cc = ix->codes + (size_t)start * mp;
-        if (kind == SCO_IMI) {
+        if (kind == SCO_IMI || kind == SCO_SCIMI) {
             int32_t h3 = s1[c], h4 = s2[c], half = mp >> 1;
             float d0 = (q_sum + diff_qc[h3]) + diff_qc[kc + h4];
             const float *r1 = ix->dot_cr + (size_t)h3 * bs;

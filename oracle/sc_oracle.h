@@ -76,6 +76,13 @@ int sco_build_dense(const uint64_t *keys, int64_t n, int64_t ncells,
                     const uint8_t *codes, int32_t mp, int32_t *L, int32_t *pid,
                     uint8_t *codes_sorted);
 
+/* Sparse inverted file: the distinct keys ascending (ckeys [ncells]), their
+ * inclusive prefix L [ncells], ids ascending within a cell (pid [n]) and the
+ * codes in cell order.  ckeys/L need room for n entries; returns ncells (or a
+ * negative error). */
+int64_t sco_build_sparse(const uint64_t *keys, int64_t n, const uint8_t *codes, int32_t mp,
+                         uint64_t *ckeys, int32_t *L, int32_t *pid, uint8_t *codes_sorted);
+
 /* PQQuery::pre_compute1 tables. */
 void sco_tables(const float *cq, const float *pq, int32_t dim, int32_t mc,
                 int32_t kc, int32_t mp, int32_t kp, float *norm_c,
@@ -94,6 +101,9 @@ typedef struct {
     const int32_t *L;    /* inclusive prefix, [ncells] */
     const int32_t *pid;  /* [N] */
     const uint8_t *codes;/* [N][mp], cell order */
+    const uint64_t *ckeys; /* sparse directory: the key of each of the ncells
+                              non-empty cells, ascending (NULL: dense cells,
+                              cell = key) */
 } sco_index;
 
 typedef struct {
@@ -109,6 +119,14 @@ typedef struct {
 #define SCO_IMI    1  /* MultiQuery::search_multi2 (include/multi_query.h:62-273) */
 #define SCO_SC     2  /* SCQuery::search_mr_ivf    (include/sc_query.h:60-256) */
 #define SCO_SC3    3  /* SCQuery::search_mr_ivf3   (include/sc_query.h:265-475) */
+/* (nearest, second) cells of two half-blocks (mc = 2, nc = 2): the cell key of
+ * src/sc_encoder.cpp:39-50 in 64 bits over a sparse directory; non-empty cells
+ * are taken in order of the sum of the halves' SC pair scores
+ * (include/sc_query.h:113-166 per half, include/multi_query.h:175-203 across
+ * them), then scanned as the IMI scans (include/multi_query.h:209-262).  The
+ * reference builds no such index (src/sc_encoder.cpp:26-33 aborts) and has no
+ * such search; the definition is stated above scimi_rank in sc_oracle.c. */
+#define SCO_SCIMI  4
 
 /* One query.  ids/dists: [R] (padded with -1 / +inf when sum < R, in which
  * case the first `sum` entries keep traversal order like the reference).

@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 REF_PATH = os.path.join(HERE, "_ref", "libscref.so")
 
-IVFADC, IMI, SC, SC3 = 0, 1, 2, 3
+IVFADC, IMI, SC, SC3, SCIMI = 0, 1, 2, 3, 4
 
 _f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
 _i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
@@ -38,7 +38,7 @@ class Index(C.Structure):
                 ("cq", C.c_void_p), ("pq", C.c_void_p), ("norm_c", C.c_void_p),
                 ("norm_r", C.c_void_p), ("dot_cr", C.c_void_p),
                 ("ncells", C.c_int64), ("L", C.c_void_p), ("pid", C.c_void_p),
-                ("codes", C.c_void_p)]
+                ("codes", C.c_void_p), ("ckeys", C.c_void_p)]
 
 
 class Stats(C.Structure):
@@ -75,6 +75,9 @@ def _load():
                                   C.c_int32, _u64p]
     lib.sco_build_dense.argtypes = [_u64p, C.c_int64, C.c_int64, _u8p, C.c_int32,
                                     _i32p, _i32p, _u8p]
+    lib.sco_build_sparse.argtypes = [_u64p, C.c_int64, _u8p, C.c_int32, _u64p, _i32p, _i32p,
+                                     _u8p]
+    lib.sco_build_sparse.restype = C.c_int64
     lib.sco_tables.argtypes = [_f32p, _f32p] + [C.c_int32] * 5 + [_f32p] * 3
     lib.sco_lut.argtypes = [_f32p] * 5 + [C.c_int32] * 5 + [_f32p] * 2
     lib.sco_search.argtypes = [C.POINTER(Index), C.c_int32, _f32p, C.c_int32,
@@ -202,6 +205,21 @@ def build_dense(keys, ncells, codes):
     return L, pid, cs
 
 
+def build_sparse(keys, codes):
+    """Non-empty cells only: (ckeys ascending, L inclusive prefix, pid, codes
+    in cell order)."""
+    n, mp = codes.shape
+    ck = np.zeros(max(n, 1), np.uint64)
+    L = np.zeros(max(n, 1), np.int32)
+    pid = np.zeros(n, np.int32)
+    cs = np.zeros((n, mp), np.uint8)
+    nc = lib().sco_build_sparse(np.ascontiguousarray(keys, np.uint64), n,
+                                np.ascontiguousarray(codes), mp, ck, L, pid, cs)
+    if nc < 0:
+        raise ValueError("sco_build_sparse rc=%d" % nc)
+    return ck[:nc].copy(), L[:nc].copy(), pid, cs
+
+
 def tables(cq, pq, dim, mc, kc, mp, kp):
     nc_ = np.zeros(mc * kc, np.float32)
     nr = np.zeros(mp * kp, np.float32)
@@ -223,8 +241,11 @@ def lut(q, cq, pq, norm_c, norm_r, dim, mc, kc, mp, kp):
 class OracleIndex:
     """Dense-cell index held in numpy arrays (kept alive with the struct)."""
 
-    def __init__(self, dim, mc, kc, mp, kp, cq, pq, L, pid, codes_sorted):
+    def __init__(self, dim, mc, kc, mp, kp, cq, pq, L, pid, codes_sorted, ckeys=None):
+        """ckeys: the sparse directory's keys (one per entry of L), or None for
+        dense cells."""
         self.dim, self.mc, self.kc, self.mp, self.kp = dim, mc, kc, mp, kp
+        self.ckeys = None if ckeys is None else np.ascontiguousarray(ckeys, np.uint64)
         self.cq = np.ascontiguousarray(cq, np.float32)
         self.pq = np.ascontiguousarray(pq, np.float32)
         self.norm_c, self.norm_r, self.dot_cr = tables(self.cq, self.pq, dim, mc, kc, mp, kp)
@@ -234,7 +255,8 @@ class OracleIndex:
         p = lambda a: a.ctypes.data
         self.s = Index(dim, mc, kc, mp, kp, p(self.cq), p(self.pq), p(self.norm_c),
                        p(self.norm_r), p(self.dot_cr), self.L.size, p(self.L),
-                       p(self.pid), p(self.codes))
+                       p(self.pid), p(self.codes),
+                       None if self.ckeys is None else p(self.ckeys))
 
     def search(self, kind, q, R, w, T, M=None, trace=False):
         M = self.kc if M is None else M

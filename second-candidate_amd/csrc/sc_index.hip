@@ -10,8 +10,10 @@
 #include <cstring>
 
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
+#include <cmath>
 
 #include "sc_internal.h"
 
@@ -50,6 +52,44 @@ __global__ void k_dense_prefix(const uint64_t *__restrict__ skeys, int64_t n, in
         if (skeys[mid] <= (uint64_t)c) lo = mid + 1; else hi = mid;
     }
     L[c] = (int32_t)lo;
+}
+
+// Sparse directory of the two-halves index: heads[i] = 1 where sorted key i
+// starts a new cell.
+__global__ void k_heads(const uint64_t *__restrict__ skeys, int64_t n, int32_t *__restrict__ h) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    h[i] = (i == 0 || skeys[i] != skeys[i - 1]) ? 1 : 0;
+}
+// From the inclusive scan of the heads (cell number + 1 of every sorted
+// entry): each cell's key and the inclusive prefix L (one past its last entry).
+__global__ void k_sparse_fill(const uint64_t *__restrict__ skeys, const int32_t *__restrict__ cno,
+                              int64_t n, uint64_t *__restrict__ ckeys, int32_t *__restrict__ L) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t c = cno[i] - 1;
+    if (i == 0 || skeys[i] != skeys[i - 1]) ckeys[c] = skeys[i];
+    if (i == n - 1 || skeys[i + 1] != skeys[i]) L[c] = (int32_t)(i + 1);
+}
+// Ranking records: the two (nearest, second) pairs of each cell and its size.
+__global__ void k_cell_records(const uint64_t *__restrict__ ckeys, const int32_t *__restrict__ L,
+                               int64_t ncells, int kc, int words, uint32_t *__restrict__ rec) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncells) return;
+    const uint64_t k2 = (uint64_t)kc * (uint64_t)kc, key = ckeys[c];
+    const uint64_t u0 = key / k2, u1 = key - u0 * k2;
+    const uint32_t n0 = (uint32_t)(u0 / kc), s0 = (uint32_t)(u0 % kc);
+    const uint32_t n1 = (uint32_t)(u1 / kc), s1 = (uint32_t)(u1 % kc);
+    const uint32_t size = (uint32_t)(L[c] - (c > 0 ? L[c - 1] : 0));
+    if (words == 2) {
+        rec[2 * c] = n0 | (s0 << 8) | (n1 << 16) | (s1 << 24);
+        rec[2 * c + 1] = size;
+    } else {
+        rec[4 * c] = n0 | (s0 << 16);
+        rec[4 * c + 1] = n1 | (s1 << 16);
+        rec[4 * c + 2] = size;
+        rec[4 * c + 3] = 0;
+    }
 }
 
 // out = max |x| over x[0..n) as float bits (non-negative floats order like
@@ -132,11 +172,18 @@ int alloc_index(sc_ctx *ctx, int64_t n, int32_t dim, int32_t mc, int32_t kc, int
               hipMalloc(&ix->norm_c, sizeof(float) * mc * kc) == hipSuccess &&
               hipMalloc(&ix->norm_r, sizeof(float) * mp * kp) == hipSuccess &&
               hipMalloc(&ix->dot_cr, sizeof(float) * (size_t)kc * mp * kp) == hipSuccess &&
-              hipMalloc(&ix->L, sizeof(int32_t) * ncells) == hipSuccess &&
+              hipMalloc(&ix->L, sizeof(int32_t) * (ncells > 0 ? ncells : 1)) == hipSuccess &&
               hipMalloc(&ix->pid, sizeof(int32_t) * (n > 0 ? n : 1)) == hipSuccess &&
               hipMalloc(&ix->codes, (size_t)(n > 0 ? n : 1) * mp) == hipSuccess &&
               hipMalloc(&ix->cqT, sizeof(float) * mc * kc * bsc) == hipSuccess &&
               hipMalloc(&ix->pqT, sizeof(float) * mp * kp * bsp) == hipSuccess;
+    if (ok && mc == 2 && nc == 2) {      // two halves: sparse directory
+        ix->sparse = true;
+        ix->crec_words = kc <= 256 ? 2 : 4;
+        ok = hipMalloc(&ix->ckeys, sizeof(uint64_t) * (ncells > 0 ? ncells : 1)) == hipSuccess &&
+             hipMalloc(&ix->crec, sizeof(uint32_t) * ix->crec_words * (ncells > 0 ? ncells : 1)) ==
+                 hipSuccess;
+    }
     if (!ok) {
         sc_index_destroy(ix);
         sc_set_error("index allocation failed (n=%lld cells=%lld)", (long long)n,
@@ -156,6 +203,17 @@ int check_shape(int64_t n, int32_t dim, int32_t mc, int32_t kc, int32_t mp, int3
                      (long long)n, dim, mc, kc, mp, kp, nc);
         return SC_ERR_ARG;
     }
+    if (mc == 2 && nc == 2) {
+        // two halves: (nearest, second) per half, 64-bit keys below kc^4 < 2^64,
+        // only the non-empty cells are stored (their number is known after
+        // the sort)
+        if (kc < 2 || kc > 65535) {
+            sc_set_error("index: two-halves cells need 2 <= kc <= 65535 (kc=%d)", kc);
+            return SC_ERR_ARG;
+        }
+        *ncells = 0;
+        return SC_OK;
+    }
     double cells = 1.0;
     for (int j = 0; j < mc * nc; ++j) cells *= kc;
     if (cells >= (double)INT32_MAX) {
@@ -170,12 +228,20 @@ int check_shape(int64_t n, int32_t dim, int32_t mc, int32_t kc, int32_t mp, int3
 int finish_index(sc_index *ix) {
     sc_ctx *ctx = ix->ctx;
     SC_TRY(sc_launch_tables(ix));
+    if (ix->sparse && ix->ncells > 0) {
+        hipLaunchKernelGGL(k_cell_records, dim3((unsigned)sc_cdiv(ix->ncells, 256)), dim3(256), 0,
+                           ctx->stream, ix->ckeys, ix->L, ix->ncells, ix->kc, ix->crec_words,
+                           ix->crec);
+        SC_CHECK_LAUNCH();
+    }
     int32_t *dmax, h[3] = {0, 0, 0};
     SC_TRY(sc_scratch(ctx, 3 * sizeof(int32_t), (void **)&dmax));
     SC_HIP(hipMemsetAsync(dmax, 0, 3 * sizeof(int32_t), ctx->stream));
-    hipLaunchKernelGGL(k_cell_max, dim3((unsigned)sc_cdiv(ix->ncells, 256)), dim3(256), 0,
-                       ctx->stream, ix->L, ix->ncells, dmax);
-    SC_CHECK_LAUNCH();
+    if (ix->ncells > 0) {
+        hipLaunchKernelGGL(k_cell_max, dim3((unsigned)sc_cdiv(ix->ncells, 256)), dim3(256), 0,
+                           ctx->stream, ix->L, ix->ncells, dmax);
+        SC_CHECK_LAUNCH();
+    }
     const int64_t ndot = (int64_t)ix->kc * ix->mp * ix->kp;
     hipLaunchKernelGGL(k_absmax, dim3((unsigned)std::min<int64_t>(1024, sc_cdiv(ndot, 256))),
                        dim3(256), 0, ctx->stream, ix->dot_cr, ndot, (uint32_t *)(dmax + 2));
@@ -224,6 +290,88 @@ int sc_launch_tables(sc_index *ix) {
     return SC_OK;
 }
 
+// The two-halves index (mc = 2, nc = 2): the keys of src/sc_encoder.cpp:39-50
+// (uid_h = nearest_h*kc + second_h, key = uid_0*kc^2 + uid_1) in 64 bits, a
+// stable radix sort of (key, id) -- ids ascending within a cell, as
+// Bucket::push_back leaves them (src/encoder.cpp:182-186) -- and only the
+// non-empty cells kept, in key order (the reference's dense kc^4 array,
+// src/sc_encoder.cpp:26-33, does not fit).
+static int build_sparse(sc_ctx *ctx, const uint16_t *d_cid, const uint8_t *d_codes, int64_t n,
+                        int32_t dim, const float *d_cq, int32_t kc, const float *d_pq, int32_t mp,
+                        int32_t kp, sc_index **out) {
+    const int mc = 2, nc = 2;
+    int bits = 1;
+    {
+        const double cells = (double)kc * kc * kc * kc;
+        while (bits < 64 && std::ldexp(1.0, bits) < cells) ++bits;
+    }
+    // scratch: keys, sorted keys, ids, sorted ids, cell numbers, sort/scan temp
+    size_t sort_bytes = 0, scan_bytes = 0;
+    if (n > 0) {
+        SC_HIP(rocprim::radix_sort_pairs(nullptr, sort_bytes, (uint64_t *)nullptr,
+                                         (uint64_t *)nullptr, (int32_t *)nullptr,
+                                         (int32_t *)nullptr, (size_t)n, 0u, (unsigned)bits,
+                                         ctx->stream));
+        SC_HIP(rocprim::inclusive_scan(nullptr, scan_bytes, (int32_t *)nullptr,
+                                       (int32_t *)nullptr, (size_t)n, rocprim::plus<int32_t>(),
+                                       ctx->stream));
+    }
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t o_keys2 = al((size_t)n * 8), o_ids = o_keys2 + al((size_t)n * 8),
+                 o_ids2 = o_ids + al((size_t)n * 4), o_cno = o_ids2 + al((size_t)n * 4),
+                 o_heads = o_cno + al((size_t)n * 4), o_tmp = o_heads + al((size_t)n * 4),
+                 total = o_tmp + al(std::max(sort_bytes, scan_bytes));
+    unsigned char *s;
+    SC_TRY(sc_scratch(ctx, total, (void **)&s));
+    uint64_t *keys = (uint64_t *)s, *skeys = (uint64_t *)(s + o_keys2);
+    int32_t *ids = (int32_t *)(s + o_ids), *sids = (int32_t *)(s + o_ids2);
+    int32_t *cno = (int32_t *)(s + o_cno), *heads = (int32_t *)(s + o_heads);
+    int64_t ncells = 0;
+    if (n > 0) {
+        hipLaunchKernelGGL(k_keys, dim3((unsigned)sc_cdiv(n, 256)), dim3(256), 0, ctx->stream,
+                           d_cid, n, mc, kc, nc, keys, ids);
+        SC_CHECK_LAUNCH();
+        size_t tb = sort_bytes;
+        SC_HIP(rocprim::radix_sort_pairs(s + o_tmp, tb, keys, skeys, ids, sids, (size_t)n, 0u,
+                                         (unsigned)bits, ctx->stream));
+        hipLaunchKernelGGL(k_heads, dim3((unsigned)sc_cdiv(n, 256)), dim3(256), 0, ctx->stream,
+                           skeys, n, heads);
+        SC_CHECK_LAUNCH();
+        tb = scan_bytes;
+        SC_HIP(rocprim::inclusive_scan(s + o_tmp, tb, heads, cno, (size_t)n,
+                                       rocprim::plus<int32_t>(), ctx->stream));
+        int32_t h = 0;
+        SC_HIP(hipMemcpyAsync(&h, cno + (n - 1), 4, hipMemcpyDeviceToHost, ctx->stream));
+        SC_HIP(hipStreamSynchronize(ctx->stream));
+        ncells = h;
+    }
+    sc_index *ix;
+    SC_TRY(alloc_index(ctx, n, dim, mc, kc, mp, kp, nc, ncells, &ix));
+    auto fail = [&](int rc) { sc_index_destroy(ix); return rc; };
+    const size_t bsc = dim / mc, bsp = dim / mp;
+    if (hipMemcpyAsync(ix->cq, d_cq, sizeof(float) * mc * kc * bsc, hipMemcpyDeviceToDevice,
+                       ctx->stream) != hipSuccess ||
+        hipMemcpyAsync(ix->pq, d_pq, sizeof(float) * mp * kp * bsp, hipMemcpyDeviceToDevice,
+                       ctx->stream) != hipSuccess) {
+        sc_set_error("index: codebook copy failed");
+        return fail(SC_ERR_HIP);
+    }
+    if (n > 0) {
+        if (hipMemcpyAsync(ix->pid, sids, sizeof(int32_t) * n, hipMemcpyDeviceToDevice,
+                           ctx->stream) != hipSuccess)
+            return fail(SC_ERR_HIP);
+        hipLaunchKernelGGL(k_sparse_fill, dim3((unsigned)sc_cdiv(n, 256)), dim3(256), 0,
+                           ctx->stream, skeys, cno, n, ix->ckeys, ix->L);
+        hipLaunchKernelGGL(k_gather_codes, dim3((unsigned)sc_cdiv(n * mp, 256)), dim3(256), 0,
+                           ctx->stream, d_codes, ix->pid, n, mp, ix->codes);
+        if (hipGetLastError() != hipSuccess) return fail(SC_ERR_HIP);
+    }
+    int rc = finish_index(ix);
+    if (rc) return fail(rc);
+    *out = ix;
+    return SC_OK;
+}
+
 extern "C" int sc_index_build(sc_ctx *ctx, const uint16_t *d_cid, const uint8_t *d_codes,
                               int64_t n, int32_t dim, const float *d_cq, int32_t mc, int32_t kc,
                               const float *d_pq, int32_t mp, int32_t kp, int32_t nc,
@@ -233,6 +381,8 @@ extern "C" int sc_index_build(sc_ctx *ctx, const uint16_t *d_cid, const uint8_t 
     *out = nullptr;
     int64_t ncells;
     SC_TRY(check_shape(n, dim, mc, kc, mp, kp, nc, &ncells));
+    if (mc == 2 && nc == 2)
+        return build_sparse(ctx, d_cid, d_codes, n, dim, d_cq, kc, d_pq, mp, kp, out);
     sc_index *ix;
     SC_TRY(alloc_index(ctx, n, dim, mc, kc, mp, kp, nc, ncells, &ix));
     const size_t bsc = dim / mc, bsp = dim / mp;
@@ -284,6 +434,8 @@ extern "C" int sc_index_build(sc_ctx *ctx, const uint16_t *d_cid, const uint8_t 
     return SC_OK;
 }
 
+static int check_lists(const int32_t *h_L, int64_t ncells, const int32_t *h_pid, int64_t n);
+
 extern "C" int sc_index_load(sc_ctx *ctx, const int32_t *h_L, int64_t ncells,
                              const int32_t *h_pid, const uint8_t *h_codes, int64_t n,
                              int32_t dim, const float *h_cq, int32_t mc, int32_t kc,
@@ -298,6 +450,11 @@ extern "C" int sc_index_load(sc_ctx *ctx, const int32_t *h_L, int64_t ncells,
     *out = nullptr;
     int64_t want;
     SC_TRY(check_shape(n, dim, mc, kc, mp, kp, nc, &want));
+    if (mc == 2 && nc == 2) {
+        sc_set_error("index_load: a two-halves index (mc = 2, nc = 2) is loaded with "
+                     "sc_index_load_sparse (cell keys + L over the non-empty cells)");
+        return SC_ERR_ARG;
+    }
     if (ncells != want) {
         sc_set_error("index_load: %lld cells given, shape needs %lld", (long long)ncells,
                      (long long)want);
@@ -305,24 +462,7 @@ extern "C" int sc_index_load(sc_ctx *ctx, const int32_t *h_L, int64_t ncells,
     }
     // The kernels index codes and ids through L[] and pid[] without bounds
     // checks, so a malformed inverted file is refused here, before upload.
-    if (h_L[ncells - 1] != n) {
-        sc_set_error("index_load: L[last]=%d != n=%lld", h_L[ncells - 1], (long long)n);
-        return SC_ERR_ARG;
-    }
-    for (int64_t c = 0; c < ncells; ++c) {
-        const int32_t prev = c > 0 ? h_L[c - 1] : 0;
-        if (h_L[c] < prev) {
-            sc_set_error("index_load: L is not a non-decreasing inclusive prefix from 0 "
-                         "(L[%lld]=%d after %d)", (long long)c, h_L[c], prev);
-            return SC_ERR_ARG;
-        }
-    }
-    for (int64_t i = 0; i < n; ++i)
-        if (h_pid[i] < 0 || h_pid[i] >= n) {
-            sc_set_error("index_load: pid[%lld]=%d outside [0, %lld)", (long long)i, h_pid[i],
-                         (long long)n);
-            return SC_ERR_ARG;
-        }
+    SC_TRY(check_lists(h_L, ncells, h_pid, n));
     SC_TRY(sc_ctx_enter(ctx));
     sc_index *ix;
     SC_TRY(alloc_index(ctx, n, dim, mc, kc, mp, kp, nc, ncells, &ix));
@@ -345,6 +485,104 @@ extern "C" int sc_index_load(sc_ctx *ctx, const int32_t *h_L, int64_t ncells,
     return SC_OK;
 }
 
+// Host-side checks shared by the loaders: L a non-decreasing inclusive prefix
+// from 0 ending at n, every pid in [0, n).
+static int check_lists(const int32_t *h_L, int64_t ncells, const int32_t *h_pid, int64_t n) {
+    if (ncells > 0 && h_L[ncells - 1] != n) {
+        sc_set_error("index_load: L[last]=%d != n=%lld", h_L[ncells - 1], (long long)n);
+        return SC_ERR_ARG;
+    }
+    if (ncells == 0 && n != 0) {
+        sc_set_error("index_load: no cells for n=%lld vectors", (long long)n);
+        return SC_ERR_ARG;
+    }
+    for (int64_t c = 0; c < ncells; ++c) {
+        const int32_t prev = c > 0 ? h_L[c - 1] : 0;
+        if (h_L[c] < prev) {
+            sc_set_error("index_load: L is not a non-decreasing inclusive prefix from 0 "
+                         "(L[%lld]=%d after %d)", (long long)c, h_L[c], prev);
+            return SC_ERR_ARG;
+        }
+    }
+    for (int64_t i = 0; i < n; ++i)
+        if (h_pid[i] < 0 || h_pid[i] >= n) {
+            sc_set_error("index_load: pid[%lld]=%d outside [0, %lld)", (long long)i, h_pid[i],
+                         (long long)n);
+            return SC_ERR_ARG;
+        }
+    return SC_OK;
+}
+
+extern "C" int sc_index_load_sparse(sc_ctx *ctx, const uint64_t *h_ckeys, const int32_t *h_L,
+                                    int64_t ncells, const int32_t *h_pid, const uint8_t *h_codes,
+                                    int64_t n, int32_t dim, const float *h_cq, int32_t kc,
+                                    const float *h_pq, int32_t mp, int32_t kp, sc_index **out) {
+    const int mc = 2, nc = 2;
+    if (!out || !h_cq || !h_pq || ncells < 0 ||
+        (ncells > 0 && (!h_ckeys || !h_L)) || (n > 0 && (!h_pid || !h_codes))) {
+        sc_set_error("index_load_sparse: null argument");
+        return SC_ERR_ARG;
+    }
+    *out = nullptr;
+    int64_t unused;
+    SC_TRY(check_shape(n, dim, mc, kc, mp, kp, nc, &unused));
+    const double kc4 = (double)kc * kc * kc * kc;
+    for (int64_t c = 0; c < ncells; ++c) {
+        if ((double)h_ckeys[c] >= kc4 || (c > 0 && h_ckeys[c] <= h_ckeys[c - 1])) {
+            sc_set_error("index_load_sparse: cell keys must ascend strictly below kc^4 "
+                         "(key[%lld]=%llu)", (long long)c, (unsigned long long)h_ckeys[c]);
+            return SC_ERR_ARG;
+        }
+        if (h_L[c] <= (c > 0 ? h_L[c - 1] : 0)) {
+            sc_set_error("index_load_sparse: cell %lld is empty or L decreases", (long long)c);
+            return SC_ERR_ARG;
+        }
+    }
+    SC_TRY(check_lists(h_L, ncells, h_pid, n));
+    SC_TRY(sc_ctx_enter(ctx));
+    sc_index *ix;
+    SC_TRY(alloc_index(ctx, n, dim, mc, kc, mp, kp, nc, ncells, &ix));
+    const size_t bsc = dim / mc, bsp = dim / mp;
+    bool ok =
+        hipMemcpyAsync(ix->cq, h_cq, sizeof(float) * mc * kc * bsc, hipMemcpyHostToDevice,
+                       ctx->stream) == hipSuccess &&
+        hipMemcpyAsync(ix->pq, h_pq, sizeof(float) * mp * kp * bsp, hipMemcpyHostToDevice,
+                       ctx->stream) == hipSuccess &&
+        (ncells == 0 ||
+         (hipMemcpyAsync(ix->L, h_L, sizeof(int32_t) * ncells, hipMemcpyHostToDevice,
+                         ctx->stream) == hipSuccess &&
+          hipMemcpyAsync(ix->ckeys, h_ckeys, sizeof(uint64_t) * ncells, hipMemcpyHostToDevice,
+                         ctx->stream) == hipSuccess)) &&
+        (n == 0 || (hipMemcpyAsync(ix->pid, h_pid, sizeof(int32_t) * n, hipMemcpyHostToDevice,
+                                   ctx->stream) == hipSuccess &&
+                    hipMemcpyAsync(ix->codes, h_codes, (size_t)n * mp, hipMemcpyHostToDevice,
+                                   ctx->stream) == hipSuccess));
+    if (!ok) {
+        sc_index_destroy(ix);
+        sc_set_error("index_load_sparse: copy failed");
+        return SC_ERR_HIP;
+    }
+    int rc = finish_index(ix);
+    if (rc) { sc_index_destroy(ix); return rc; }
+    *out = ix;
+    return SC_OK;
+}
+
+extern "C" int sc_index_cell_keys(sc_ctx *ctx, const sc_index *ix, uint64_t *h_keys) {
+    SC_TRY(sc_ctx_enter(ctx));
+    if (!ix || !h_keys || !ix->sparse) {
+        sc_set_error("sc_index_cell_keys: the index has no sparse directory (dense cells: "
+                     "cell = key)");
+        return SC_ERR_ARG;
+    }
+    if (ix->ncells > 0) {
+        SC_HIP(hipMemcpyAsync(h_keys, ix->ckeys, sizeof(uint64_t) * ix->ncells,
+                              hipMemcpyDeviceToHost, ctx->stream));
+        SC_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    return SC_OK;
+}
+
 extern "C" int sc_index_attach_raw(sc_index *ix, const float *d_raw, int64_t n) {
     if (!ix || (d_raw && n != ix->n)) {
         sc_set_error("sc_index_attach_raw: the raw vectors must match the index (n=%lld)",
@@ -361,8 +599,8 @@ extern "C" int sc_index_destroy(sc_index *ix) {
         (void)hipSetDevice(ix->ctx->device);
         (void)hipStreamSynchronize(ix->ctx->stream);
     }
-    void *bufs[] = {ix->cq,  ix->pq,  ix->norm_c, ix->norm_r, ix->dot_cr,
-                    ix->L,   ix->pid, ix->codes,  ix->cqT,    ix->pqT};
+    void *bufs[] = {ix->cq,  ix->pq,  ix->norm_c, ix->norm_r, ix->dot_cr, ix->L,
+                    ix->pid, ix->codes, ix->cqT,  ix->pqT,    ix->ckeys,  ix->crec};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     delete ix;

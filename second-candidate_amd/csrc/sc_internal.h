@@ -30,6 +30,14 @@ struct sc_index {
     int64_t nonempty;               // non-empty cells
     float dot_absmax;               // max |dot_cr| (an index statistic)
     const float *raw;               // raw base vectors [n][dim] by id (not owned) or null
+    // Two-halves (mc = 2, nc = 2) index: a sparse directory of the non-empty
+    // cells (ncells of them), in key order; L is the inclusive prefix over it.
+    bool sparse;
+    uint64_t *ckeys;                // [ncells] keys ascending
+    uint32_t *crec;                 // [ncells][crec_words] ranking records:
+                                    // kc <= 256: {n0 | s0<<8 | n1<<16 | s1<<24, size}
+                                    // else {n0 | s0<<16, n1 | s1<<16, size, 0}
+    int crec_words;                 // 2 or 4
 };
 
 void sc_set_error(const char *fmt, ...);
@@ -101,3 +109,22 @@ __host__ __device__ inline float sc_key_float(uint32_t k) {
 
 // Internal launches shared across translation units.
 int sc_launch_tables(sc_index *ix);
+
+// Two-halves cell ranking (sc_rank.hip) of the queries [0, nq): cells taken
+// in (score ASC, key ASC) order while count < w and sum < T.  Writes qs
+// [nq][4] = {q_sum, q_0, q_1, 0}, the cell lists (cells: starts, clen:
+// sizes, cinfo: n0 | n1 << 16; [nq][ccap]) and qinfo [nq][8] = {count, sum,
+// count, 0, exhausted, overflow, error, 0}.  Queries whose cells do not fit
+// the on-chip list are finished by a device-wide sort in the same call.
+struct RankCellsArgs {
+    const float *Q;          // [nq][dim]
+    const float *dqc;        // [nq][2 kc]
+    float *qs;
+    int32_t *cells, *clen;
+    uint32_t *cinfo;
+    int32_t *qinfo;
+    const int32_t *Tq;       // per-query budgets or null
+    int64_t nq;
+    int w, T, ccap;
+};
+int sc_rank_cells(sc_ctx *ctx, const sc_index *ix, const RankCellsArgs &a);

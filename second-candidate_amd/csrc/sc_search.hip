@@ -2506,8 +2506,8 @@ static int search_impl(sc_ctx *ctx, const sc_index *ix, const float *d_q, int64_
     if ((kind == SC_KIND_IVFADC && (ix->mc != 1 || ix->nc != 1)) ||
         (kind == SC_KIND_IMI && (ix->mc != 2 || ix->nc != 1)) ||
         (kind == SC_KIND_SC && (ix->mc != 1 || ix->nc != 2)) ||
-        (kind == SC_KIND_SC3 && (ix->mc != 1 || ix->nc != 3 || ix->kc > 1024)) || kind < 0 ||
-        kind > 3) {
+        (kind == SC_KIND_SC3 && (ix->mc != 1 || ix->nc != 3 || ix->kc > 1024)) ||
+        (kind == SC_KIND_SCIMI && !ix->sparse) || kind < 0 || kind > 4) {
         sc_set_error("sc_search: kind %d does not match the index (mc=%d nc=%d)", kind, ix->mc,
                      ix->nc);
         return SC_ERR_CONFIG;
@@ -2522,7 +2522,8 @@ static int search_impl(sc_ctx *ctx, const sc_index *ix, const float *d_q, int64_
                      kind == SC_KIND_SC3 ? 3 : 2, ix->kc);
         return SC_ERR_ARG;
     }
-    if (p->form == SC_FORM_MIXED && (kind == SC_KIND_SC3 || kind == SC_KIND_IMI)) {
+    if (p->form == SC_FORM_MIXED &&
+        (kind == SC_KIND_SC3 || kind == SC_KIND_IMI || kind == SC_KIND_SCIMI)) {
         sc_set_error("sc_search: SC_FORM_MIXED is not available for kind %d; "
                      "use SC_FORM_PLAIN", kind);
         return SC_ERR_ARG;
@@ -2540,8 +2541,8 @@ static int search_impl(sc_ctx *ctx, const sc_index *ix, const float *d_q, int64_
     if (nq == 0) return SC_OK;
     // real distances run on the staged kernels: the raw vectors are read per
     // candidate, so the scan is bound by those reads, not by the on-chip stages
-    if (kind != SC_KIND_IMI && kind != SC_KIND_SC3 && p->form != SC_FORM_REAL &&
-        !getenv("SC_NO_FUSED")) {
+    if (kind != SC_KIND_IMI && kind != SC_KIND_SC3 && kind != SC_KIND_SCIMI &&
+        p->form != SC_FORM_REAL && !getenv("SC_NO_FUSED")) {
         int rc = search_fused(ctx, ix, d_q, nq, p, d_T, d_ids, d_dists, d_sums, st);
         if (rc != SC_ERR_TOO_LARGE) return rc;      // otherwise: staged kernels
         if (st) *st = sc_search_stats{};
@@ -2606,7 +2607,9 @@ static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
                          float *d_dists, int32_t *d_sums, sc_search_stats *st) {
     const int kind = p->kind;
     const bool three = kind == SC_KIND_SC3;
-    const int scan_kind = three ? SC_KIND_SC : kind;   // same ranking and scan as SC
+    const bool halves = kind == SC_KIND_SCIMI;
+    // SC3 ranks and scans as SC; the two-halves kind scans as the IMI
+    const int scan_kind = three ? SC_KIND_SC : (halves ? SC_KIND_IMI : kind);
     const Plan P = make_plan(ix, p, nq);
     unsigned char *s;
     SC_TRY(sc_scratch(ctx, P.total, (void **)&s));
@@ -2637,7 +2640,7 @@ static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
     // needs more is redone by the global-heap lane kernel)
     int hcap_w = 0;
     const int kr = std::min(kc, kTravRanks);
-    if (kind != SC_KIND_IVFADC && !three && kc <= 65535) {
+    if (kind != SC_KIND_IVFADC && !three && !halves && kc <= 65535) {
         hcap_w = std::min(hcap_ref, 256);
         if (const char *e = getenv("SC_HEAP_ONCHIP")) {
             int v = atoi(e);
@@ -2675,9 +2678,12 @@ static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
                                ix->norm_r, ix->mc, kc, ix->mp, ix->kp, dqc, dqr);
         SC_CHECK_LAUNCH();
         mark();
-        hipLaunchKernelGGL(k_rank, dim3((unsigned)cn), dim3(256), (size_t)npow2 * 8, ctx->stream,
-                           Q, ix->dim, scan_kind, ix->mc, kc, dqc, qs, sv, si, npow2);
-        SC_CHECK_LAUNCH();
+        if (!halves) {      // the two-halves ranking computes its own query terms
+            hipLaunchKernelGGL(k_rank, dim3((unsigned)cn), dim3(256), (size_t)npow2 * 8,
+                               ctx->stream, Q, ix->dim, scan_kind, ix->mc, kc, dqc, qs, sv, si,
+                               npow2);
+            SC_CHECK_LAUNCH();
+        }
         mark();
         const int32_t *Tq = d_T ? d_T + q0 : nullptr;      // this chunk's budgets
         TravArgs ta{kind, kc, p->w, p->T, P.ccap, sv, si, ix->L, cells, cinfo, qinfo,
@@ -2710,7 +2716,10 @@ static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
             SC_TRY(sort_order(true));
             ta.qlist = perm;
         }
-        if (three) {
+        if (halves) {
+            RankCellsArgs ra{Q, dqc, qs, cells, clen, cinfo, qinfo, Tq, cn, p->w, p->T, P.ccap};
+            SC_TRY(sc_rank_cells(ctx, ix, ra));
+        } else if (three) {
             SC_TRY(traverse3(ctx, p, kc, sv, si, ix->L, cells, clen, cinfo, qinfo, P.ccap, cn, Tq));
         } else if (wave) {
             const size_t lds = trav_wave_layout(lists, kr, hcap_w).total;
@@ -2736,7 +2745,7 @@ static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
             if (!ordered) SC_TRY(sort_order(false));
             sa.qlist = perm;
         }
-        if (hcap_eff > 0 || three) {
+        if (hcap_eff > 0 || three || halves) {
             hipLaunchKernelGGL(k_scan_topk, dim3((unsigned)cn), dim3(kScanThreads), lds,
                                ctx->stream, sa);
             SC_CHECK_LAUNCH();
@@ -2745,7 +2754,7 @@ static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
         // Queries whose heap outgrew the on-chip capacity are traversed again
         // with a global-memory heap of the reference's capacity and scanned
         // again; the first scan's output for them is overwritten.
-        if (!three && hcap_eff < hcap_ref) {
+        if (!three && !halves && hcap_eff < hcap_ref) {
             int32_t nfb = 0;
             int32_t *dn = list + P.chunk;
             if (hcap_eff > 0) {
@@ -2898,8 +2907,8 @@ static int plan_create(sc_ctx *ctx, const sc_index *ix, const sc_search_params *
     const FusedConfig F = fused_config(ix, p);
     const FusedLayout &Lay = F.lay;
     const int hcap = F.hcap, ccap = F.ccap, npow2 = F.npow2;
-    P->fused = kind != SC_KIND_IMI && kind != SC_KIND_SC3 && p->form != SC_FORM_REAL &&
-               Lay.total <= 150 * 1024;
+    P->fused = kind != SC_KIND_IMI && kind != SC_KIND_SC3 && kind != SC_KIND_SCIMI &&
+               p->form != SC_FORM_REAL && Lay.total <= 150 * 1024;
     if (!P->fused) { *out = P; return SC_OK; }
     P->lds = Lay.total;
     bool ok = hipMalloc(&P->counter, 256) == hipSuccess &&

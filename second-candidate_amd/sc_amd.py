@@ -17,7 +17,7 @@ HEADER = os.path.join(os.path.dirname(HERE), "include", "sc_hip.h")
 SC_OK = 0
 SC_ERR_ARG, SC_ERR_NOMEM, SC_ERR_CONFIG, SC_ERR_HIP = -1, -2, -3, -4
 SC_ERR_TOO_LARGE, SC_ERR_NO_DEVICE, SC_ERR_HEAP_EMPTY = -5, -6, -7
-KIND_IVFADC, KIND_IMI, KIND_SC, KIND_SC3 = 0, 1, 2, 3
+KIND_IVFADC, KIND_IMI, KIND_SC, KIND_SC3, KIND_SCIMI = 0, 1, 2, 3, 4
 PREC_FP32, PREC_BF16 = 0, 1
 FORM_PLAIN, FORM_MIXED, FORM_REAL = 0, 1, 2
 
@@ -90,6 +90,9 @@ _SIGS = {
     "sc_index_load": (C.c_int, [_P, _P, C.c_int64, _P, _P, C.c_int64, C.c_int32, _P,
                                 C.c_int32, C.c_int32, _P, C.c_int32, C.c_int32, C.c_int32,
                                 C.POINTER(_P)]),
+    "sc_index_load_sparse": (C.c_int, [_P, _P, _P, C.c_int64, _P, _P, C.c_int64, C.c_int32, _P,
+                                       C.c_int32, _P, C.c_int32, C.c_int32, C.POINTER(_P)]),
+    "sc_index_cell_keys": (C.c_int, [_P, _P, _P]),
     "sc_index_destroy": (C.c_int, [_P]),
     "sc_index_attach_raw": (C.c_int, [_P, _P, C.c_int64]),
     "sc_encode_general": (C.c_int, [_P, _P, C.c_int64, C.c_int32, _P, C.c_int32, C.c_int32, _P,
@@ -261,6 +264,29 @@ class Index:
             ctx.h, L.ctypes.data, L.size, pid.ctypes.data, codes.ctypes.data, pid.size, dim,
             cq.ctypes.data, mc, kc, pq.ctypes.data, mp, kp, nc, C.byref(h)))
         return cls(ctx, h, dim, mc, kc, mp, kp, nc)
+
+    @classmethod
+    def load_sparse(cls, ctx, ckeys, L, pid, codes, dim, cq, kc, pq, mp, kp):
+        """A two-halves index from host arrays (non-empty cells only)."""
+        h = _P()
+        ckeys = np.ascontiguousarray(ckeys, np.uint64)
+        L = np.ascontiguousarray(L, np.int32)
+        pid = np.ascontiguousarray(pid, np.int32)
+        codes = np.ascontiguousarray(codes, np.uint8)
+        cq = np.ascontiguousarray(cq, np.float32)
+        pq = np.ascontiguousarray(pq, np.float32)
+        _check("sc_index_load_sparse", lib().sc_index_load_sparse(
+            ctx.h, ckeys.ctypes.data, L.ctypes.data, L.size, pid.ctypes.data, codes.ctypes.data,
+            pid.size, dim, cq.ctypes.data, kc, pq.ctypes.data, mp, kp, C.byref(h)))
+        return cls(ctx, h, dim, 2, kc, mp, kp, 2)
+
+    def cell_keys(self):
+        """Keys of the sparse directory (two-halves index), ascending."""
+        ncells = int(self.shape()[1])
+        out = np.zeros(max(ncells, 1), np.uint64)
+        _check("sc_index_cell_keys", lib().sc_index_cell_keys(self.ctx.h, self.h,
+                                                              out.ctypes.data))
+        return out[:ncells]
 
     def attach_raw(self, raw):
         """Raw base vectors (Dev [n][dim], by id) for FORM_REAL searches; the

@@ -88,10 +88,14 @@ def test_encode_matches_oracle(ctx, case, prec):
 
 
 # ------------------------------------------------------------------ index + search
+OKIND = {S.KIND_IVFADC: sco.IVFADC, S.KIND_IMI: sco.IMI, S.KIND_SC: sco.SC,
+         S.KIND_SC3: sco.SC3, S.KIND_SCIMI: sco.SCIMI}
+
+
 def _index_pair(ctx, kind, n=6000, dim=64, kc=16, mp=8, kp=64, seed=3):
     rng = np.random.default_rng(seed)
-    mc = 2 if kind == S.KIND_IMI else 1
-    nc = {S.KIND_SC: 2, S.KIND_SC3: 3}.get(kind, 1)
+    mc = 2 if kind in (S.KIND_IMI, S.KIND_SCIMI) else 1
+    nc = {S.KIND_SC: 2, S.KIND_SC3: 3, S.KIND_SCIMI: 2}.get(kind, 1)
     x = sco.gen(sco.mix_d128(ncomp=11), sco.STREAM_BASE, 0, n, dim)
     cq, pq = _codebooks(rng, x, mc, kc, mp, kp, ties=False)
     dx, dcq, dpq = (S.Dev.from_numpy(ctx, a) for a in (x, cq, pq))
@@ -101,14 +105,19 @@ def _index_pair(ctx, kind, n=6000, dim=64, kc=16, mp=8, kp=64, seed=3):
     else:
         cid, codes, _ = S.encode(ctx, dx, dcq, dpq, mc, kc, mp, kp, nc)
         ocid, ocodes = sco.encode(x, cq, pq, mc, kc, mp, kp, nc)
+    assert np.array_equal(cid.numpy(), ocid) and np.array_equal(codes.numpy(), ocodes)
     ix = S.Index.build(ctx, cid, codes, dim, dcq, mc, kc, dpq, mp, kp, nc)
+    if kind == S.KIND_SCIMI:
+        ck, L, pid, cs = sco.build_sparse(sco.cell_keys(ocid, mc, kc, nc), ocodes)
+        return ix, sco.OracleIndex(dim, mc, kc, mp, kp, cq, pq, L, pid, cs, ckeys=ck)
     ncells = kc ** (mc * nc)
     L, pid, cs = sco.build_dense(sco.cell_keys(ocid, mc, kc, nc), ncells, ocodes)
     oix = sco.OracleIndex(dim, mc, kc, mp, kp, cq, pq, L, pid, cs)
     return ix, oix
 
 
-@pytest.mark.parametrize("kind", [S.KIND_IVFADC, S.KIND_IMI, S.KIND_SC, S.KIND_SC3])
+@pytest.mark.parametrize("kind", [S.KIND_IVFADC, S.KIND_IMI, S.KIND_SC, S.KIND_SC3,
+                                  S.KIND_SCIMI])
 def test_search_real_distances_match_oracle(ctx, kind):
     """SC_FORM_REAL (real_dist = true): candidates ranked by the exact S1
     distance to the raw vectors, bit-exact ids and distances against the
@@ -190,9 +199,7 @@ SEARCH_CASES = [  # kind, R, w, T
 def _compare_search(ctx, ix, oix, kind, R, w, T, nq=150, form=S.FORM_PLAIN):
     q = sco.gen(sco.mix_d128(ncomp=11), sco.STREAM_QUERY, 0, nq, ix.dim)
     ids, dists, sums, st = ix.search(S.Dev.from_numpy(ctx, q), kind, R, w, T, form=form)
-    okind = {S.KIND_IVFADC: sco.IVFADC, S.KIND_IMI: sco.IMI, S.KIND_SC: sco.SC,
-             S.KIND_SC3: sco.SC3}[kind]
-    oids, od, osum = oix.search_batch(okind, q, R, w, T)
+    oids, od, osum = oix.search_batch(OKIND[kind], q, R, w, T)
     assert np.array_equal(sums.numpy(), osum)
     assert np.array_equal(ids.numpy(), oids)
     assert np.array_equal(_bits(dists.numpy()), _bits(od))
@@ -210,6 +217,82 @@ def test_search_matches_oracle(ctx, case):
         # nothing is popped, no candidate is gathered and every result slot
         # is empty
         assert st.candidates == 0 and st.cells == 0 and st.visited == 0
+    ix.close()
+
+
+# ------------------------------------------------------------------ two halves (SC_KIND_SCIMI)
+SCIMI_CASES = [  # kc, R, w, T
+    (8, 100, 100000, 3000),
+    (8, 100, 5, 100000),         # the cell budget binds
+    (8, 300, 500, 40),           # fewer candidates than R
+    (8, 7, 500, 0),              # T = 0: no cell
+    (4, 50, 10 ** 6, 10 ** 6),   # every cell of the index taken (exhausted)
+    (16, 100, 100000, 2500),
+    (300, 100, 100000, 1500),    # kc > 256: 16-byte ranking records
+]
+
+
+@pytest.mark.parametrize("case", SCIMI_CASES)
+def test_search_scimi_matches_oracle(ctx, case):
+    """Two halves with (nearest, second) cells per half: the sparse
+    directory built on the GPU equals the oracle's (keys, L, pid, codes), and
+    ids, distances and candidate counts equal the oracle's search."""
+    kc, R, w, T = case
+    ix, oix = _index_pair(ctx, S.KIND_SCIMI, kc=kc)
+    e = ix.export()
+    assert np.array_equal(ix.cell_keys(), oix.ckeys)
+    assert np.array_equal(e["L"], oix.L) and np.array_equal(e["pid"], oix.pid)
+    assert np.array_equal(e["codes"], oix.codes)
+    assert np.array_equal(_bits(e["dot_cr"]), _bits(oix.dot_cr))
+    st = _compare_search(ctx, ix, oix, S.KIND_SCIMI, R, w, T, nq=100)
+    if T == 0:
+        assert st.candidates == 0 and st.cells == 0
+    if kc == 4:
+        assert st.exhausted == 100 and st.candidates == 100 * oix.pid.size
+    with pytest.raises(S.ScError) as ex:
+        ix.search(S.Dev.from_numpy(ctx, np.zeros((1, ix.dim), np.float32)), S.KIND_IMI, R, w, T)
+    assert ex.value.rc == S.SC_ERR_CONFIG
+    ix.close()
+
+
+@pytest.mark.parametrize("cap", ["16", "1"])
+def test_search_scimi_small_rank_list(ctx, cap, monkeypatch):
+    """An on-chip ranking list of 16 (or 1) cells: queries go through the
+    second histogram level and the device-wide sorted fallback; results must
+    not change."""
+    monkeypatch.setenv("SC_RANK_CAP", cap)
+    ix, oix = _index_pair(ctx, S.KIND_SCIMI, kc=8)
+    for R, w, T in ((100, 100000, 3000), (50, 7, 10 ** 6), (20, 10 ** 6, 10 ** 6)):
+        _compare_search(ctx, ix, oix, S.KIND_SCIMI, R, w, T, nq=60)
+    ix.close()
+
+
+def test_search_scimi_load_sparse_and_refusals(ctx):
+    """sc_index_load_sparse from the oracle's arrays gives the same search;
+    a dense load of a two-halves index, unordered keys, an empty cell and the
+    mixed form are refused."""
+    ix, oix = _index_pair(ctx, S.KIND_SCIMI, kc=8)
+    lx = S.Index.load_sparse(ctx, oix.ckeys, oix.L, oix.pid, oix.codes, oix.dim, oix.cq, oix.kc,
+                             oix.pq, oix.mp, oix.kp)
+    assert np.array_equal(lx.cell_keys(), oix.ckeys)
+    _compare_search(ctx, lx, oix, S.KIND_SCIMI, 100, 100000, 2000, nq=40)
+    bad = oix.ckeys.copy()
+    bad[[1, 2]] = bad[[2, 1]]
+    with pytest.raises(S.ScError, match="ascend"):
+        S.Index.load_sparse(ctx, bad, oix.L, oix.pid, oix.codes, oix.dim, oix.cq, oix.kc, oix.pq,
+                            oix.mp, oix.kp)
+    L2 = oix.L.copy()
+    L2[1] = L2[0]
+    with pytest.raises(S.ScError, match="empty"):
+        S.Index.load_sparse(ctx, oix.ckeys, L2, oix.pid, oix.codes, oix.dim, oix.cq, oix.kc,
+                            oix.pq, oix.mp, oix.kp)
+    with pytest.raises(S.ScError, match="load_sparse"):
+        S.Index.load(ctx, oix.L, oix.pid, oix.codes, oix.dim, oix.cq, 2, oix.kc, oix.pq, oix.mp,
+                     oix.kp, 2)
+    with pytest.raises(S.ScError, match="not available"):
+        ix.search(S.Dev.from_numpy(ctx, np.zeros((1, ix.dim), np.float32)), S.KIND_SCIMI, 10,
+                  10, 10, form=S.FORM_MIXED)
+    lx.close()
     ix.close()
 
 
@@ -380,7 +463,7 @@ def test_search_pop_budget_handoff(ctx, monkeypatch, form):
     ix.close()
 
 
-@pytest.mark.parametrize("kind", [S.KIND_IVFADC, S.KIND_IMI, S.KIND_SC])
+@pytest.mark.parametrize("kind", [S.KIND_IVFADC, S.KIND_IMI, S.KIND_SC, S.KIND_SCIMI])
 def test_search_scan_order_invariant(ctx, kind, monkeypatch):
     """The staged scan's workgroup order (launch order, sorted by first
     cell, sorted and split over the XCDs) changes no output bit, in the
@@ -696,8 +779,59 @@ def test_exhaustive_pq_config(ctx):
     ix.close()
 
 
+def test_full_size_scimi_index_and_search(ctx):
+    """1M x 128, two 64-dim halves with K=256 each and (nearest, second)
+    cells per half, PQ 8x256 (the dense-partition benchmark shape): the BF16
+    encode equals the FP32 encode on every row, sampled rows equal the
+    oracle's encode, the sparse directory holds exactly the vectors' keys,
+    and search (w=100k, T=10k, top-100) on 64 queries equals the oracle."""
+    n, dim, kc, mp, kp = 1_000_000, 128, 256, 8, 256
+    mix = S.mix_d128()
+    xt = S.gen_rows(ctx, mix, S.STREAM_TRAIN, 0, 50_000, dim)
+    th = xt.numpy()
+    cq = np.stack([S.kmeans(ctx, S.Dev.from_numpy(ctx, np.ascontiguousarray(th[:, h * 64:(h + 1) * 64])),
+                            kc, 3, seed=31 + h, want_labels=False)[0].numpy() for h in range(2)])
+    dcq = S.Dev.from_numpy(ctx, cq)
+    S.residual(ctx, xt, dcq, 2, kc, 0)
+    rh = xt.numpy()
+    pq = np.stack([S.kmeans(ctx, S.Dev.from_numpy(ctx, np.ascontiguousarray(rh[:, s * 16:(s + 1) * 16])),
+                            kp, 3, seed=41 + s, want_labels=False)[0].numpy() for s in range(mp)])
+    dpq = S.Dev.from_numpy(ctx, pq)
+    x = S.gen_rows(ctx, mix, S.STREAM_BASE, 0, n, dim)
+    cid, codes, _ = S.encode(ctx, x, dcq, dpq, 2, kc, mp, kp, 2)
+    cidh, codesh = cid.numpy(), codes.numpy()
+    bcid, bcodes, bst = S.encode(ctx, x, dcq, dpq, 2, kc, mp, kp, 2, S.PREC_BF16)
+    assert np.array_equal(bcid.numpy(), cidh) and np.array_equal(bcodes.numpy(), codesh)
+    print("two halves bf16: %d re-checked, %d exact rows" % (bst.rechecked, bst.fallback))
+    del bcid, bcodes
+    rows = np.random.default_rng(2).choice(n, 2000, replace=False)
+    xr = np.concatenate([sco.gen(sco.mix_d128(), sco.STREAM_BASE, int(r), 1, dim) for r in rows])
+    ocid, ocodes = sco.encode(xr, cq, pq, 2, kc, mp, kp, 2)
+    assert np.array_equal(cidh[rows], ocid) and np.array_equal(codesh[rows], ocodes)
+    ix = S.Index.build(ctx, cid, codes, dim, dcq, 2, kc, dpq, mp, kp, 2)
+    e = ix.export()
+    ck = ix.cell_keys()
+    vkeys = sco.cell_keys(cidh, 2, kc, 2)
+    uk, cnt = np.unique(vkeys, return_counts=True)
+    assert np.array_equal(ck, uk)                                  # exactly the vectors' cells
+    assert np.array_equal(np.diff(np.concatenate([[0], e["L"]])), cnt)
+    assert np.array_equal(vkeys[e["pid"]], np.repeat(uk, cnt))     # members in key order
+    assert np.all(np.diff(e["pid"])[np.diff(vkeys[e["pid"]]) == 0] > 0)   # ids ascend in a cell
+    assert np.array_equal(e["codes"], codesh[e["pid"]])
+    print("two halves: %d non-empty cells of %d" % (ck.size, kc ** 4))
+    oix = sco.OracleIndex(dim, 2, kc, mp, kp, cq, pq, e["L"], e["pid"], e["codes"], ckeys=ck)
+    q = sco.gen(sco.mix_d128(), sco.STREAM_QUERY, 0, 64, dim)
+    ids, dists, sums, st = ix.search(S.Dev.from_numpy(ctx, q), S.KIND_SCIMI, 100, 100000, 10000)
+    oids, od, osum = oix.search_batch(sco.SCIMI, q, 100, 100000, 10000)
+    assert np.array_equal(sums.numpy(), osum)
+    assert np.array_equal(ids.numpy(), oids)
+    assert np.array_equal(_bits(dists.numpy()), _bits(od))
+    ix.close()
+
+
 @pytest.mark.parametrize("kind,kc,nc_ok", [(S.KIND_SC, 16, True), (S.KIND_IMI, 16, True),
-                                           (S.KIND_IVFADC, 16, True), (S.KIND_SC3, 12, True)])
+                                           (S.KIND_IVFADC, 16, True), (S.KIND_SC3, 12, True),
+                                           (S.KIND_SCIMI, 8, True)])
 def test_search_per_query_budgets(ctx, kind, kc, nc_ok):
     """A batch whose queries carry their own candidate budgets (drawn from a
     wide range, some above the batch maximum and some negative, which clamp)
@@ -712,8 +846,7 @@ def test_search_per_query_budgets(ctx, kind, kc, nc_ok):
     T[:3] = [-5, Tmax + 999, 0]
     w = 16 if kind == S.KIND_IVFADC else 400
     dq, dT = S.Dev.from_numpy(ctx, q), S.Dev.from_numpy(ctx, T)
-    okind = {S.KIND_IVFADC: sco.IVFADC, S.KIND_IMI: sco.IMI, S.KIND_SC: sco.SC,
-             S.KIND_SC3: sco.SC3}[kind]
+    okind = OKIND[kind]
     want = [oix.search(okind, q[i], 100, w, int(min(max(T[i], 0), Tmax))) for i in range(nq)]
     ids, dists, sums, _ = ix.search(dq, kind, 100, w, Tmax, budgets=dT)
     gi, gd, gs = ids.numpy(), dists.numpy(), sums.numpy()

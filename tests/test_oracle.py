@@ -340,15 +340,18 @@ def test_recall_definition():
 # ------------------------------------------------------------------ search
 def _make_index(kind, n=3000, dim=32, kc=8, mp=4, kp=16, seed=7):
     rng = np.random.default_rng(seed)
-    mc = 2 if kind == sco.IMI else 1
+    mc = 2 if kind in (sco.IMI, sco.SCIMI) else 1
     x = sco.gen(sco.mix_d128(ncomp=9), sco.STREAM_BASE, 0, n, dim)
     cq, pq = _codebooks(rng, x, mc, kc, mp, kp, dup=False)
-    nc = {sco.SC: 2, sco.SC3: 3}.get(kind, 1)
+    nc = {sco.SC: 2, sco.SC3: 3, sco.SCIMI: 2}.get(kind, 1)
     if kind == sco.SC3:
         cid, codes = sco.encode_general(x, cq, pq, mc, kc, mp, kp, nc)
     else:
         cid, codes = sco.encode(x, cq, pq, mc, kc, mp, kp, nc)
     keys = sco.cell_keys(cid, mc, kc, nc)
+    if kind == sco.SCIMI:
+        ck, L, pid, cs = sco.build_sparse(keys, codes)
+        return sco.OracleIndex(dim, mc, kc, mp, kp, cq, pq, L, pid, cs, ckeys=ck), x, cid, codes
     ncells = {sco.IVFADC: kc, sco.SC3: kc ** 3}.get(kind, kc * kc)
     L, pid, cs = sco.build_dense(keys, ncells, codes)
     ix = sco.OracleIndex(dim, mc, kc, mp, kp, cq, pq, L, pid, cs)
@@ -358,7 +361,7 @@ def _make_index(kind, n=3000, dim=32, kc=8, mp=4, kp=16, seed=7):
 def test_adc_matches_fp64_reconstruction():
     """ADC value = ||q - c - r||^2 for the candidate's centroid c and PQ
     reconstruction r (FP64), within FP32 rounding; q = c + r gives ~0."""
-    for kind in (sco.IVFADC, sco.IMI, sco.SC, sco.SC3):
+    for kind in (sco.IVFADC, sco.IMI, sco.SC, sco.SC3, sco.SCIMI):
         ix, x, cid, codes = _make_index(kind)
         q = sco.gen(sco.mix_d128(ncomp=9), sco.STREAM_QUERY, 0, 3, ix.dim)
         for qi in range(3):
@@ -382,6 +385,84 @@ def test_adc_matches_fp64_reconstruction():
         ids, dists, st, ex = ix.search(kind, qq, R=1, w=ix.kc, T=10 ** 9, trace=True)
         pos = int(np.nonzero(ex["all_ids"] == vid)[0][0])
         assert abs(ex["all_dists"][pos]) <= 1e-5 * np.sum(qq.astype(np.float64) ** 2)
+
+
+# ------------------------------------------------------------------ two halves
+def test_sparse_directory_matches_dense_cells():
+    """The sparse directory lists exactly the non-empty dense cells: keys
+    ascending, the same sizes, members and codes in the same order."""
+    ix, x, cid, codes = _make_index(sco.SCIMI, n=2500, kc=5)
+    keys = sco.cell_keys(cid, 2, 5, 2)
+    assert keys.max() < 5 ** 4
+    L, pid, cs = sco.build_dense(keys, 5 ** 4, codes)
+    sizes = np.diff(np.concatenate([[0], L]))
+    assert np.array_equal(ix.ckeys, np.nonzero(sizes)[0].astype(np.uint64))
+    assert np.array_equal(np.diff(np.concatenate([[0], ix.L])), sizes[sizes > 0])
+    assert np.array_equal(ix.pid, pid) and np.array_equal(ix.codes, cs)
+    # the key is vector_base over the per-half (nearest, second) uids
+    # (src/sc_encoder.cpp:39-50): uid_h = n_h*kc + s_h, key = uid_0*kc^2 + uid_1
+    want = ((cid[:, 0].astype(np.uint64) * 5 + cid[:, 2]) * 25 +
+            cid[:, 1].astype(np.uint64) * 5 + cid[:, 3])
+    assert np.array_equal(keys, want)
+
+
+def _scimi_grid_order(ix, q):
+    """Brute force over the whole (n0, s0, n1, s1) grid: every cell's score
+    from the halves' SC pair scores in FP32, the grid sorted by (score, key),
+    the empty cells dropped."""
+    kc = ix.kc
+    dqc, _ = sco.lut(q, ix.cq, ix.pq, ix.norm_c, ix.norm_r, ix.dim, 2, kc, ix.mp, ix.kp)
+    bsc = ix.dim // 2
+    v = []
+    for h in range(2):
+        acc = np.float32(0)
+        for t in q[h * bsc:(h + 1) * bsc]:
+            acc = np.float32(acc + np.float32(t * t))
+        v.append((acc + dqc[h * kc:(h + 1) * kc]).astype(np.float32))
+    g = np.indices((kc, kc, kc, kc)).reshape(4, -1).astype(np.int64)
+    n0, s0, n1, s1 = g
+    score = ((v[0][n0] + v[0][s0]).astype(np.float32) +
+             (v[1][n1] + v[1][s1]).astype(np.float32)).astype(np.float32)
+    key = (n0 * kc + s0) * kc * kc + n1 * kc + s1
+    order = np.lexsort((key, score))
+    pos = {int(k): c for c, k in enumerate(ix.ckeys)}
+    return [pos[int(key[o])] for o in order if int(key[o]) in pos]
+
+
+def test_scimi_ranking_matches_grid_enumeration():
+    """Two-halves search: the cells taken are the prefix of the brute-force
+    grid order that stops at w cells or T candidates, whatever the budgets
+    (including every cell of the index and T = 0)."""
+    ix, x, cid, codes = _make_index(sco.SCIMI, n=3000, kc=6)
+    sizes = np.diff(np.concatenate([[0], ix.L]))
+    q = sco.gen(sco.mix_d128(ncomp=9), sco.STREAM_QUERY, 0, 8, ix.dim)
+    for qi in range(8):
+        order = _scimi_grid_order(ix, q[qi])
+        assert len(order) == ix.ckeys.size
+        for w, T in ((3, 10 ** 9), (50, 400), (10 ** 5, 1500), (10 ** 5, 10 ** 9), (9, 0)):
+            ids, dists, st, ex = ix.search(sco.SCIMI, q[qi], R=30, w=w, T=T, trace=True)
+            want, tot = [], 0
+            for c in order:
+                if len(want) >= w or tot >= T:
+                    break
+                want.append(c)
+                tot += int(sizes[c])
+            assert list(ex["cells"]) == want
+            assert st.sum == tot and st.ncells == len(want)
+            assert st.exhausted == (len(want) == len(order) and len(want) < w and tot < T)
+            # the scan's coarse terms use the nearest centroid of each half
+            k2 = ix.kc ** 2
+            assert np.array_equal(ex["s1"], (ix.ckeys[want] // k2 // ix.kc).astype(np.int32))
+            assert np.array_equal(ex["s2"], (ix.ckeys[want] % k2 // ix.kc).astype(np.int32))
+
+
+def test_scimi_refuses_dense_index_and_other_kinds_refuse_sparse():
+    ix, x, _, _ = _make_index(sco.SCIMI, n=500, kc=4)
+    with pytest.raises(ValueError):
+        ix.search(sco.IMI, x[0], R=5, w=10, T=10)
+    ixd, xd, _, _ = _make_index(sco.IMI, n=500, kc=4)
+    with pytest.raises(ValueError):
+        ixd.search(sco.SCIMI, xd[0], R=5, w=10, T=10)
 
 
 def _replay_traversal(ix, kind, q, w, T):
@@ -758,9 +839,10 @@ def test_kmeans_pp_second_pick_is_d2_weighted():
     {0, 1, 3, 7} the two rules differ by up to 0.19 in probability."""
     x = np.array([[0.0], [1.0], [3.0], [7.0]], np.float32)
     counts = np.zeros((4, 4))
+    idx = np.zeros(2, np.int64)
     for seed in range(4000):
-        f, s = sco.kmeans_pp(x, 2, seed=seed + 11)
-        counts[f, s] += 1
+        assert sco.lib().sco_kmeans_pp(x, 4, 1, 2, seed + 11, 0, idx, 1) == 0   # one thread
+        counts[idx[0], idx[1]] += 1
     for f in range(4):
         tot = counts[f].sum()
         assert tot > 600
