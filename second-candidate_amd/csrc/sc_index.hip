@@ -92,6 +92,66 @@ __global__ void k_cell_records(const uint64_t *__restrict__ ckeys, const int32_t
     }
 }
 
+// The halves' pairs: uid_h of every cell (h = 0: key / kc^2, h = 1: key % kc^2).
+__global__ void k_cell_uids(const uint64_t *__restrict__ ckeys, int64_t ncells, int kc,
+                            uint32_t *__restrict__ u0, uint32_t *__restrict__ u1) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncells) return;
+    const uint64_t k2 = (uint64_t)kc * (uint64_t)kc, key = ckeys[c];
+    u0[c] = (uint32_t)(key / k2);
+    u1[c] = (uint32_t)(key - (key / k2) * k2);
+}
+// Run heads of a sorted list.
+__global__ void k_heads32(const uint32_t *__restrict__ s, int64_t n, int32_t *__restrict__ h) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    h[i] = (i == 0 || s[i] != s[i - 1]) ? 1 : 0;
+}
+// The distinct values (from the inclusive scan of the heads), stored as
+// n | s << 16 for uid = n*kc + s.
+__global__ void k_unique_pairs(const uint32_t *__restrict__ s, const int32_t *__restrict__ no,
+                               int64_t n, int kc, uint32_t *__restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (i == 0 || s[i] != s[i - 1]) out[no[i] - 1] = (s[i] / kc) | ((s[i] % kc) << 16);
+}
+// Each cell's positions in the two halves' pair lists (binary search of the
+// ascending uid lists) and its size.
+__global__ void k_cell_pairs(const uint64_t *__restrict__ ckeys, const int32_t *__restrict__ L,
+                             int64_t ncells, int kc, const uint32_t *__restrict__ p0, int n0,
+                             const uint32_t *__restrict__ p1, int n1, uint32_t *__restrict__ out) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncells) return;
+    const uint64_t k2 = (uint64_t)kc * (uint64_t)kc, key = ckeys[c];
+    const uint32_t u[2] = {(uint32_t)(key / k2), (uint32_t)(key % k2)};
+    const uint32_t *p[2] = {p0, p1};
+    const int np[2] = {n0, n1};
+    uint32_t pos[2];
+    for (int h = 0; h < 2; ++h) {
+        int lo = 0, hi = np[h];
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            const uint32_t v = p[h][mid];
+            const uint32_t uv = (v & 0xFFFFu) * (uint32_t)kc + (v >> 16);
+            if (uv < u[h]) lo = mid + 1; else hi = mid;
+        }
+        pos[h] = (uint32_t)lo;
+    }
+    out[2 * c] = pos[0] | (pos[1] << 16);
+    out[2 * c + 1] = (uint32_t)(L[c] - (c > 0 ? L[c - 1] : 0));
+}
+
+// First cell of each half-0 pair (cells are in key order, so a pair's cells
+// are contiguous); gstart[n0] = ncells.
+__global__ void k_group_starts(const uint32_t *__restrict__ cpair, int64_t ncells, int n0,
+                               int32_t *__restrict__ gstart) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c > ncells) return;
+    if (c == ncells) { gstart[n0] = (int32_t)ncells; return; }
+    const uint32_t g = cpair[2 * c] & 0xFFFFu;
+    if (c == 0 || (cpair[2 * (c - 1)] & 0xFFFFu) != g) gstart[g] = (int32_t)c;
+}
+
 // out = max |x| over x[0..n) as float bits (non-negative floats order like
 // their bit patterns)
 __global__ void k_absmax(const float *__restrict__ x, int64_t n, uint32_t *out) {
@@ -225,6 +285,73 @@ int check_shape(int64_t n, int32_t dim, int32_t mc, int32_t kc, int32_t mp, int3
     return SC_OK;
 }
 
+// The halves' distinct pairs and each cell's positions in them (two-halves
+// index): the cell ranking then scores a cell with two table lookups.
+static int build_pair_lists(sc_index *ix) {
+    sc_ctx *ctx = ix->ctx;
+    const int64_t nc = ix->ncells;
+    size_t sort_bytes = 0, scan_bytes = 0;
+    SC_HIP(rocprim::radix_sort_keys(nullptr, sort_bytes, (uint32_t *)nullptr, (uint32_t *)nullptr,
+                                    (size_t)nc, 0u, 32u, ctx->stream));
+    SC_HIP(rocprim::inclusive_scan(nullptr, scan_bytes, (int32_t *)nullptr, (int32_t *)nullptr,
+                                   (size_t)nc, rocprim::plus<int32_t>(), ctx->stream));
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t o_u1 = al((size_t)nc * 4), o_s = o_u1 + al((size_t)nc * 4),
+                 o_h = o_s + al((size_t)nc * 4), o_no = o_h + al((size_t)nc * 4),
+                 o_tmp = o_no + al((size_t)nc * 4), total = o_tmp + al(std::max(sort_bytes, scan_bytes));
+    unsigned char *s;
+    SC_TRY(sc_scratch(ctx, total, (void **)&s));
+    uint32_t *u[2] = {(uint32_t *)s, (uint32_t *)(s + o_u1)};
+    uint32_t *srt = (uint32_t *)(s + o_s);
+    int32_t *heads = (int32_t *)(s + o_h), *no = (int32_t *)(s + o_no);
+    const unsigned g = (unsigned)sc_cdiv(nc, 256);
+    hipLaunchKernelGGL(k_cell_uids, dim3(g), dim3(256), 0, ctx->stream, ix->ckeys, nc, ix->kc, u[0],
+                       u[1]);
+    SC_CHECK_LAUNCH();
+    for (int h = 0; h < 2; ++h) {
+        // h = 0 is already ascending (cells are in key order); sort h = 1
+        const uint32_t *sorted = u[h];
+        if (h == 1) {
+            size_t tb = sort_bytes;
+            SC_HIP(rocprim::radix_sort_keys(s + o_tmp, tb, u[1], srt, (size_t)nc, 0u, 32u,
+                                            ctx->stream));
+            sorted = srt;
+        }
+        hipLaunchKernelGGL(k_heads32, dim3(g), dim3(256), 0, ctx->stream, sorted, nc, heads);
+        SC_CHECK_LAUNCH();
+        size_t tb = scan_bytes;
+        SC_HIP(rocprim::inclusive_scan(s + o_tmp, tb, heads, no, (size_t)nc,
+                                       rocprim::plus<int32_t>(), ctx->stream));
+        int32_t cnt = 0;
+        SC_HIP(hipMemcpyAsync(&cnt, no + (nc - 1), 4, hipMemcpyDeviceToHost, ctx->stream));
+        SC_HIP(hipStreamSynchronize(ctx->stream));
+        ix->nu[h] = cnt;
+        if (cnt > 65535) return SC_OK;          // too many for 16-bit positions: no tables
+        if (hipMalloc(&ix->upairs[h], 4 * (size_t)cnt) != hipSuccess) {
+            sc_set_error("index: pair list allocation failed");
+            return SC_ERR_NOMEM;
+        }
+        hipLaunchKernelGGL(k_unique_pairs, dim3(g), dim3(256), 0, ctx->stream, sorted, no, nc,
+                           ix->kc, ix->upairs[h]);
+        SC_CHECK_LAUNCH();
+    }
+    if (hipMalloc(&ix->cpair, 8 * (size_t)nc) != hipSuccess) {
+        sc_set_error("index: cell pair allocation failed");
+        return SC_ERR_NOMEM;
+    }
+    hipLaunchKernelGGL(k_cell_pairs, dim3(g), dim3(256), 0, ctx->stream, ix->ckeys, ix->L, nc,
+                       ix->kc, ix->upairs[0], ix->nu[0], ix->upairs[1], ix->nu[1], ix->cpair);
+    SC_CHECK_LAUNCH();
+    if (hipMalloc(&ix->gstart, 4 * ((size_t)ix->nu[0] + 1)) != hipSuccess) {
+        sc_set_error("index: group start allocation failed");
+        return SC_ERR_NOMEM;
+    }
+    hipLaunchKernelGGL(k_group_starts, dim3((unsigned)sc_cdiv(nc + 1, 256)), dim3(256), 0,
+                       ctx->stream, ix->cpair, nc, ix->nu[0], ix->gstart);
+    SC_CHECK_LAUNCH();
+    return SC_OK;
+}
+
 int finish_index(sc_index *ix) {
     sc_ctx *ctx = ix->ctx;
     SC_TRY(sc_launch_tables(ix));
@@ -233,6 +360,7 @@ int finish_index(sc_index *ix) {
                            ctx->stream, ix->ckeys, ix->L, ix->ncells, ix->kc, ix->crec_words,
                            ix->crec);
         SC_CHECK_LAUNCH();
+        SC_TRY(build_pair_lists(ix));
     }
     int32_t *dmax, h[3] = {0, 0, 0};
     SC_TRY(sc_scratch(ctx, 3 * sizeof(int32_t), (void **)&dmax));
@@ -599,8 +727,10 @@ extern "C" int sc_index_destroy(sc_index *ix) {
         (void)hipSetDevice(ix->ctx->device);
         (void)hipStreamSynchronize(ix->ctx->stream);
     }
-    void *bufs[] = {ix->cq,  ix->pq,  ix->norm_c, ix->norm_r, ix->dot_cr, ix->L,
-                    ix->pid, ix->codes, ix->cqT,  ix->pqT,    ix->ckeys,  ix->crec};
+    void *bufs[] = {ix->cq,    ix->pq,   ix->norm_c, ix->norm_r, ix->dot_cr,
+                    ix->L,     ix->pid,  ix->codes,  ix->cqT,    ix->pqT,
+                    ix->ckeys, ix->crec, ix->upairs[0], ix->upairs[1], ix->cpair,
+                    ix->gstart};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     delete ix;

@@ -38,6 +38,14 @@ struct sc_index {
                                     // kc <= 256: {n0 | s0<<8 | n1<<16 | s1<<24, size}
                                     // else {n0 | s0<<16, n1 | s1<<16, size, 0}
     int crec_words;                 // 2 or 4
+    // The distinct (nearest, second) pairs of each half that occur in the
+    // index (uid_h = n_h*kc + s_h, ascending), and per cell the positions of
+    // its two pairs in those lists {i0 | i1 << 16, size} (null when a half
+    // has more than 65535 pairs).
+    int32_t nu[2];
+    uint32_t *upairs[2];            // [nu[h]] n | s << 16
+    uint32_t *cpair;                // [ncells][2]
+    int32_t *gstart;                // [nu[0] + 1] first cell of each half-0 pair
 };
 
 void sc_set_error(const char *fmt, ...);
@@ -126,5 +134,7 @@ struct RankCellsArgs {
     const int32_t *Tq;       // per-query budgets or null
     int64_t nq;
     int w, T, ccap;
+    int32_t *flag_list;      // device scratch: [nq] queries for the device-wide path
+    int32_t *flag_count;     // device scratch: 1 counter
 };
 int sc_rank_cells(sc_ctx *ctx, const sc_index *ix, const RankCellsArgs &a);

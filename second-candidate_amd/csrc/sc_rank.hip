@@ -43,6 +43,10 @@ struct RankKernelArgs {
     const int32_t *L;
     const int32_t *qlist;   // fallback: the queries of this launch (null = all)
     int cap;                // on-chip list capacity (at most kCap)
+    // pair tables (k_rank_pairs)
+    int nu0, nu1;
+    const uint32_t *up0, *up1, *cpair;
+    const int32_t *gstart;
 };
 
 template <int W>
@@ -137,116 +141,52 @@ __device__ int find_bucket(const uint32_t *hcnt, const uint32_t *hwt, uint32_t p
     return b;
 }
 
+// Shared scalars of one workgroup's ranking.
+struct RankShared {
+    uint32_t wsum[8];
+    int b;
+    uint32_t pre[2];
+    float q[3];
+    int n, count, stop;
+    uint32_t sum, mink;
+    uint64_t stopkey;
+};
+
+// A cell's score, size and nearest centroids from its ranking record.
 template <int W>
-__global__ __launch_bounds__(kRankThreads) void k_rank_cells(RankKernelArgs k) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    uint32_t *hcnt = reinterpret_cast<uint32_t *>(smem);           // [kBuckets]
-    uint32_t *hwt = hcnt + kBuckets;                               // [kBuckets]
-    uint64_t *lkey = reinterpret_cast<uint64_t *>(smem);           // [cap], after the histograms
-    float *v = reinterpret_cast<float *>(smem + 8 * kBuckets);     // [2 kc]
-    __shared__ uint32_t wsum[8];
-    __shared__ int s_b;
-    __shared__ uint32_t s_pre[2];
-    __shared__ float s_q[3];
-    __shared__ int s_n, s_count;
-    __shared__ uint32_t s_sum;
+struct RecScorer {
+    const uint32_t *rec;
+    const float *v;
+    int kc;
+    __device__ __forceinline__ float score(int64_t c) const { return Rec<W>(rec, c).score(v, kc); }
+    __device__ __forceinline__ uint32_t size(int64_t c) const { return Rec<W>(rec, c).size; }
+    __device__ __forceinline__ uint32_t nearest(int64_t c) const {
+        const Rec<W> r(rec, c);
+        return r.n0 | (r.n1 << 16);
+    }
+};
+// The same from the per-query pair tables: st0[i] = v_0[n] + v_0[s] for the
+// i-th pair of half 0 (st1 likewise), so score = st0[i0] + st1[i1] performs
+// exactly the additions of Rec::score.
+struct PairScorer {
+    const uint32_t *cpair;
+    const float *st0, *st1;
+    const uint32_t *up0, *up1;
+    __device__ __forceinline__ float score(int64_t c) const {
+        const uint32_t p = cpair[2 * c];
+        return st0[p & 0xFFFFu] + st1[p >> 16];
+    }
+    __device__ __forceinline__ uint32_t size(int64_t c) const { return cpair[2 * c + 1]; }
+    __device__ __forceinline__ uint32_t nearest(int64_t c) const {
+        const uint32_t p = cpair[2 * c];
+        return (up0[p & 0xFFFFu] & 0xFFFFu) | ((up1[p >> 16] & 0xFFFFu) << 16);
+    }
+};
 
-    const RankCellsArgs &a = k.a;
-    const int tid = threadIdx.x, kc = k.kc;
-    const int64_t q = k.qlist ? k.qlist[blockIdx.x] : blockIdx.x;
-    const int T = query_budget(a.Tq, q, a.T), w = a.w;
-    if (tid == 0) {
-        // include/multi_query.h:102-112: per-half sequential sums, q_sum their sum
-        const float *qv = a.Q + (size_t)q * k.dim;
-        const int bsc = k.dim / 2;
-        float qsum = 0.0f, h[2];
-        for (int i = 0; i < 2; ++i) {
-            float d1 = 0.0f;
-            for (int j = 0; j < bsc; ++j) { const float t = qv[i * bsc + j]; d1 += t * t; }
-            qsum += d1;
-            h[i] = d1;
-        }
-        s_q[0] = qsum; s_q[1] = h[0]; s_q[2] = h[1];
-        float *qs = a.qs + (size_t)q * 4;
-        qs[0] = qsum; qs[1] = h[0]; qs[2] = h[1]; qs[3] = 0.0f;
-    }
-    for (int i = tid; i < kBuckets; i += kRankThreads) { hcnt[i] = 0; hwt[i] = 0; }
-    __syncthreads();
-    const float *dq = a.dqc + (size_t)q * 2 * kc;
-    for (int j = tid; j < 2 * kc; j += kRankThreads) v[j] = s_q[j < kc ? 1 : 2] + dq[j];
-    __syncthreads();
-
-    int32_t *qi = a.qinfo + (size_t)q * 8;
-    int32_t *cells = a.cells + (size_t)q * a.ccap;
-    int32_t *clen = a.clen + (size_t)q * a.ccap;
-    uint32_t *cinfo = a.cinfo + (size_t)q * a.ccap;
-    if (T <= 0 || k.ncells == 0) {          // the loop test fails before the first cell
-        if (tid == 0) {
-            qi[0] = 0; qi[1] = 0; qi[2] = 0; qi[3] = 0;
-            qi[4] = (k.ncells == 0 && T > 0) ? 1 : 0; qi[5] = 0; qi[6] = 0; qi[7] = 0;
-        }
-        return;
-    }
-
-    // level 1: the top 12 bits of the score key
-    for (int64_t c = tid; c < k.ncells; c += kRankThreads) {
-        const Rec<W> r(k.rec, c);
-        const uint32_t key = sc_float_key(r.score(v, kc));
-        atomicAdd(&hcnt[key >> 20], 1u);
-        atomicAdd(&hwt[key >> 20], r.size);
-    }
-    __syncthreads();
-    const int b1 = find_bucket(hcnt, hwt, 0u, 0u, w, T, wsum, &s_b, s_pre);
-    const uint32_t pre1_cnt = s_pre[0], pre1_w = s_pre[1];
-    const uint32_t b1cnt = b1 < kBuckets ? hcnt[b1] : 0u;
-    __syncthreads();
-    // cells to sort: those of the buckets below b1, plus bucket b1 (or its
-    // part up to the level-2 bucket b2)
-    int b2 = kBuckets - 1;
-    uint32_t need = b1 < kBuckets ? pre1_cnt + b1cnt : pre1_cnt;
-    bool overflow = false;
-    if (need > (uint32_t)k.cap && b1 < kBuckets) {
-        // level 2: the next 12 bits, inside bucket b1
-        for (int i = tid; i < kBuckets; i += kRankThreads) { hcnt[i] = 0; hwt[i] = 0; }
-        __syncthreads();
-        for (int64_t c = tid; c < k.ncells; c += kRankThreads) {
-            const Rec<W> r(k.rec, c);
-            const uint32_t key = sc_float_key(r.score(v, kc));
-            if ((int)(key >> 20) != b1) continue;
-            atomicAdd(&hcnt[(key >> 8) & 0xFFFu], 1u);
-            atomicAdd(&hwt[(key >> 8) & 0xFFFu], r.size);
-        }
-        __syncthreads();
-        const int f = find_bucket(hcnt, hwt, pre1_cnt, pre1_w, w, T, wsum, &s_b, s_pre);
-        b2 = f < kBuckets ? f : kBuckets - 1;
-        need = f < kBuckets ? s_pre[0] + hcnt[f] : s_pre[0];
-        __syncthreads();
-        overflow = need > (uint32_t)k.cap;
-    } else if (need > (uint32_t)k.cap) {
-        overflow = true;     // every cell is taken and there are too many of them
-    }
-    if (overflow) {
-        if (tid == 0) {
-            qi[0] = 0; qi[1] = 0; qi[2] = 0; qi[3] = 0; qi[4] = 0; qi[5] = 1; qi[6] = 0;
-            qi[7] = 0;
-        }
-        return;
-    }
-
-    // compact the cells up to (b1, b2) into LDS as (score key, cell) pairs
-    if (tid == 0) s_n = 0;
-    __syncthreads();
-    for (int64_t c = tid; c < k.ncells; c += kRankThreads) {
-        const Rec<W> r(k.rec, c);
-        const uint32_t key = sc_float_key(r.score(v, kc));
-        const int kb1 = (int)(key >> 20);
-        if (kb1 < b1 || (kb1 == b1 && (int)((key >> 8) & 0xFFFu) <= b2)) {
-            const int o = atomicAdd(&s_n, 1);
-            lkey[o] = ((uint64_t)key << 32) | (uint32_t)c;
-        }
-    }
-    __syncthreads();
-    const int n = s_n;
+// Bitonic sort of lkey[0 .. n) ascending (padded to a power of two >= 64 with
+// ~0; the list has room for it).  All threads call.
+__device__ void sort_list(uint64_t *lkey, int n) {
+    const int tid = threadIdx.x;
     int np2 = 64;
     while (np2 < n) np2 <<= 1;
     for (int i = n + tid; i < np2; i += kRankThreads) lkey[i] = ~(uint64_t)0;
@@ -262,38 +202,481 @@ __global__ __launch_bounds__(kRankThreads) void k_rank_cells(RankKernelArgs k) {
             __syncthreads();
         }
     }
+}
 
-    // stopping point: entry i is taken iff i < w and the sizes before it sum
-    // to less than T; the taken entries are a prefix
+// The stopping entry of a sorted list of n cells: the first i at which the
+// count reaches w or the size sum reaches T (n when none).  All threads call.
+template <class Sc>
+__device__ int stop_entry(const Sc &sc, const uint64_t *lkey, int n, int w, int T,
+                          RankShared &sh) {
+    const int tid = threadIdx.x;
     const int per = (n + kRankThreads - 1) / kRankThreads;
     const int i0 = min(n, tid * per), i1 = min(n, i0 + per);
     uint32_t local = 0;
-    for (int i = i0; i < i1; ++i) local += Rec<W>(k.rec, (int64_t)(lkey[i] & 0xFFFFFFFFu)).size;
+    for (int i = i0; i < i1; ++i) local += sc.size((int64_t)(lkey[i] & 0xFFFFFFFFu));
     uint32_t tot;
-    uint32_t run = block_excl_scan(local, wsum, &tot);
-    if (tid == 0) { s_count = 0; s_sum = 0; }
+    uint32_t run = block_excl_scan(local, sh.wsum, &tot);
+    if (tid == 0) sh.stop = n;
+    __syncthreads();
+    for (int i = i0; i < i1; ++i) {
+        run += sc.size((int64_t)(lkey[i] & 0xFFFFFFFFu));
+        if (i + 1 >= w || (int64_t)run >= T) {
+            atomicMin(&sh.stop, i);
+            break;
+        }
+    }
+    __syncthreads();
+    return sh.stop;
+}
+
+// Selection by score histograms (count and sizes per bucket of the score
+// key's top 12 bits, then the next 12 inside the crossing bucket): fills
+// lkey[0 .. sh.n) with every cell up to the stopping point, or returns false
+// when they do not fit the list.
+template <class Sc>
+__device__ bool hist_select(const Sc &sc, int64_t ncells, int w, int T, int cap, uint32_t *hcnt,
+                            uint32_t *hwt, uint64_t *lkey, RankShared &sh) {
+    const int tid = threadIdx.x;
+    for (int i = tid; i < kBuckets; i += kRankThreads) { hcnt[i] = 0; hwt[i] = 0; }
+    __syncthreads();
+    for (int64_t c = tid; c < ncells; c += kRankThreads) {
+        const uint32_t key = sc_float_key(sc.score(c));
+        atomicAdd(&hcnt[key >> 20], 1u);
+        atomicAdd(&hwt[key >> 20], sc.size(c));
+    }
+    __syncthreads();
+    const int b1 = find_bucket(hcnt, hwt, 0u, 0u, w, T, sh.wsum, &sh.b, sh.pre);
+    const uint32_t pre1_cnt = sh.pre[0], pre1_w = sh.pre[1];
+    const uint32_t b1cnt = b1 < kBuckets ? hcnt[b1] : 0u;
+    __syncthreads();
+    int b2 = kBuckets - 1;
+    uint32_t need = b1 < kBuckets ? pre1_cnt + b1cnt : pre1_cnt;
+    if (need > (uint32_t)cap && b1 < kBuckets) {
+        for (int i = tid; i < kBuckets; i += kRankThreads) { hcnt[i] = 0; hwt[i] = 0; }
+        __syncthreads();
+        for (int64_t c = tid; c < ncells; c += kRankThreads) {
+            const uint32_t key = sc_float_key(sc.score(c));
+            if ((int)(key >> 20) != b1) continue;
+            atomicAdd(&hcnt[(key >> 8) & 0xFFFu], 1u);
+            atomicAdd(&hwt[(key >> 8) & 0xFFFu], sc.size(c));
+        }
+        __syncthreads();
+        const int f = find_bucket(hcnt, hwt, pre1_cnt, pre1_w, w, T, sh.wsum, &sh.b, sh.pre);
+        b2 = f < kBuckets ? f : kBuckets - 1;
+        need = f < kBuckets ? sh.pre[0] + hcnt[f] : sh.pre[0];
+        __syncthreads();
+    }
+    if (need > (uint32_t)cap) return false;
+    if (tid == 0) sh.n = 0;
+    __syncthreads();
+    for (int64_t c = tid; c < ncells; c += kRankThreads) {
+        const uint32_t key = sc_float_key(sc.score(c));
+        const int kb1 = (int)(key >> 20);
+        if (kb1 < b1 || (kb1 == b1 && (int)((key >> 8) & 0xFFFu) <= b2)) {
+            const int o = atomicAdd(&sh.n, 1);
+            lkey[o] = ((uint64_t)key << 32) | (uint32_t)c;
+        }
+    }
+    __syncthreads();
+    return true;
+}
+
+// Sorts lkey[0 .. n) and writes the taken prefix to the query's cell list.
+template <class Sc>
+__device__ void emit_sorted(const Sc &sc, const RankKernelArgs &k, int64_t q, uint64_t *lkey, int n,
+                            int w, int T, RankShared &sh) {
+    const RankCellsArgs &a = k.a;
+    const int tid = threadIdx.x;
+    sort_list(lkey, n);
+    int32_t *cells = a.cells + (size_t)q * a.ccap;
+    int32_t *clen = a.clen + (size_t)q * a.ccap;
+    uint32_t *cinfo = a.cinfo + (size_t)q * a.ccap;
+    // entry i is taken iff i < w and the sizes before it sum to less than T;
+    // the taken entries are a prefix
+    const int per = (n + kRankThreads - 1) / kRankThreads;
+    const int i0 = min(n, tid * per), i1 = min(n, i0 + per);
+    uint32_t local = 0;
+    for (int i = i0; i < i1; ++i) local += sc.size((int64_t)(lkey[i] & 0xFFFFFFFFu));
+    uint32_t tot;
+    uint32_t run = block_excl_scan(local, sh.wsum, &tot);
+    if (tid == 0) { sh.count = 0; sh.sum = 0; }
     __syncthreads();
     for (int i = i0; i < i1; ++i) {
         const int64_t c = (int64_t)(lkey[i] & 0xFFFFFFFFu);
-        const Rec<W> r(k.rec, c);
+        const uint32_t size = sc.size(c);
         if (i < w && (int64_t)run < T) {
             if (i < a.ccap) {
                 cells[i] = c > 0 ? k.L[c - 1] : 0;
-                clen[i] = (int32_t)r.size;
-                cinfo[i] = r.n0 | (r.n1 << 16);
+                clen[i] = (int32_t)size;
+                cinfo[i] = sc.nearest(c);
             }
-            atomicMax(&s_count, i + 1);
-            atomicMax(&s_sum, run + r.size);
+            atomicMax(&sh.count, i + 1);
+            atomicMax(&sh.sum, run + size);
         }
-        run += r.size;
+        run += size;
     }
     __syncthreads();
     if (tid == 0) {
-        const int count = s_count;
-        qi[0] = count; qi[1] = (int32_t)s_sum; qi[2] = count; qi[3] = 0;
-        qi[4] = (count == k.ncells && count < w && (int64_t)s_sum < T) ? 1 : 0;
+        const int count = sh.count;
+        int32_t *qi = a.qinfo + (size_t)q * 8;
+        qi[0] = count; qi[1] = (int32_t)sh.sum; qi[2] = count; qi[3] = 0;
+        qi[4] = (count == k.ncells && count < w && (int64_t)sh.sum < T) ? 1 : 0;
         qi[5] = 0; qi[6] = count > a.ccap ? 2 : 0; qi[7] = 0;
     }
+}
+
+// The query terms of include/multi_query.h:102-112 (per-half sequential sums,
+// q_sum their sum) and v_h[j] = q_h + diff_qc[h*kc + j] in LDS.
+__device__ void query_terms(const RankKernelArgs &k, int64_t q, float *v, RankShared &sh) {
+    const RankCellsArgs &a = k.a;
+    const int tid = threadIdx.x, kc = k.kc;
+    // the query row goes to LDS first (all threads), so that the sequential
+    // sums below wait on LDS, not on one global load after another
+    float *qrow = v;                               // [dim] <= [2 kc] (checked by the host)
+    for (int j = tid; j < k.dim; j += kRankThreads) qrow[j] = a.Q[(size_t)q * k.dim + j];
+    __syncthreads();
+    if (tid == 0) {
+        const int bsc = k.dim / 2;
+        float qsum = 0.0f, h[2];
+        for (int i = 0; i < 2; ++i) {
+            float d1 = 0.0f;
+            for (int j = 0; j < bsc; ++j) { const float t = qrow[i * bsc + j]; d1 += t * t; }
+            qsum += d1;
+            h[i] = d1;
+        }
+        sh.q[0] = qsum; sh.q[1] = h[0]; sh.q[2] = h[1];
+        float *qs = a.qs + (size_t)q * 4;
+        qs[0] = qsum; qs[1] = h[0]; qs[2] = h[1]; qs[3] = 0.0f;
+    }
+    __syncthreads();
+    const float *dq = a.dqc + (size_t)q * 2 * kc;
+    for (int j = tid; j < 2 * kc; j += kRankThreads) v[j] = sh.q[j < kc ? 1 : 2] + dq[j];
+    __syncthreads();
+}
+
+__device__ bool empty_query(const RankKernelArgs &k, int64_t q, int T) {
+    if (T > 0 && k.ncells > 0) return false;
+    if (threadIdx.x == 0) {           // the loop test fails before the first cell
+        int32_t *qi = k.a.qinfo + (size_t)q * 8;
+        qi[0] = 0; qi[1] = 0; qi[2] = 0; qi[3] = 0;
+        qi[4] = (k.ncells == 0 && T > 0) ? 1 : 0; qi[5] = 0; qi[6] = 0; qi[7] = 0;
+    }
+    return true;
+}
+
+__device__ void flag_overflow(const RankKernelArgs &k, int64_t q) {
+    if (threadIdx.x == 0) {
+        int32_t *qi = k.a.qinfo + (size_t)q * 8;
+        qi[0] = 0; qi[1] = 0; qi[2] = 0; qi[3] = 0; qi[4] = 0; qi[5] = 1; qi[6] = 0; qi[7] = 0;
+    }
+}
+
+// LDS of the ranking kernels: region A (the histograms, or the cell list of
+// up to cap entries) of k.abytes, then v [2 kc], then the pair tables.
+__host__ __device__ inline size_t rank_abytes(int cap) {
+    int np2 = 64;
+    while (np2 < cap) np2 <<= 1;
+    return 8 * (size_t)(np2 > kBuckets ? np2 : kBuckets);
+}
+
+// Ranking from the cells' records (any index shape).
+template <int W>
+__global__ __launch_bounds__(kRankThreads) void k_rank_cells(RankKernelArgs k) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    uint32_t *hcnt = reinterpret_cast<uint32_t *>(smem);           // [kBuckets]
+    uint32_t *hwt = hcnt + kBuckets;                               // [kBuckets]
+    uint64_t *lkey = reinterpret_cast<uint64_t *>(smem);           // [cap], after the histograms
+    float *v = reinterpret_cast<float *>(smem + rank_abytes(k.cap));   // [2 kc]
+    __shared__ RankShared sh;
+    const int64_t q = k.qlist ? k.qlist[blockIdx.x] : blockIdx.x;
+    const int T = query_budget(k.a.Tq, q, k.a.T), w = k.a.w;
+    query_terms(k, q, v, sh);
+    if (empty_query(k, q, T)) return;
+    const RecScorer<W> sc{k.rec, v, k.kc};
+    if (!hist_select(sc, k.ncells, w, T, k.cap, hcnt, hwt, lkey, sh)) {
+        flag_overflow(k, q);
+        return;
+    }
+    emit_sorted(sc, k, q, lkey, sh.n, w, T, sh);
+}
+
+// Ranking from the pair tables, with a bound that skips most cells: the cells
+// of the half-0 pairs of least st0 whose cells together reach the budget form
+// a subset whose own stopping score thr bounds the full set's (a subset
+// reaches the budget no earlier); then only pairs whose least possible cell
+// score, st0[g] + min st1, is at most thr are visited, and only cells scoring
+// at most thr are listed.  When the subset is too large for the counting
+// order or the listed cells do not fit, the histogram selection runs instead.
+constexpr int kMaxPairs0 = 2048;     // half-0 pairs sorted on chip for the bound
+constexpr int kMaxPass = 512;        // pairs whose cells are visited by all threads at once
+
+// Every cell of the groups (half-0 pairs) gbeg/gpre describe is handed to f
+// (group slot, cell), all threads at once: gpre[j] is the number of cells in
+// the groups before j, gbeg[j] the first cell of group j.
+template <class F>
+__device__ void for_group_cells(const int32_t *gbeg, const int32_t *gpre, int ng, F f) {
+    const int total = gpre[ng];
+    for (int t = threadIdx.x; t < total; t += kRankThreads) {
+        int lo = 0, hi = ng;           // the last j with gpre[j] <= t
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (gpre[mid] <= t) lo = mid; else hi = mid;
+        }
+        f(lo, (int64_t)gbeg[lo] + (t - gpre[lo]));
+    }
+}
+
+// gpre from the group sizes gsz[0 .. ng) (ng <= kMaxPass).  All threads call.
+__device__ void group_prefix(const int32_t *gsz, int32_t *gpre, int ng, RankShared &sh) {
+    constexpr int per = kMaxPass / kRankThreads;
+    const int t = threadIdx.x;
+    uint32_t loc = 0;
+    for (int i = 0; i < per; ++i) {
+        const int j = t * per + i;
+        if (j < ng) loc += (uint32_t)gsz[j];
+    }
+    uint32_t tot;
+    uint32_t run = block_excl_scan(loc, sh.wsum, &tot);
+    for (int i = 0; i < per; ++i) {
+        const int j = t * per + i;
+        if (j < ng) { gpre[j] = (int32_t)run; run += (uint32_t)gsz[j]; }
+    }
+    if (t == 0) gpre[ng] = (int32_t)tot;
+    __syncthreads();
+}
+
+// Lists of at most kSmallList cells are ordered without sorting: entry i's
+// rank is the number of entries of smaller key and its position base the sum
+// of their sizes, counted over the whole list (every thread reads the same
+// entry at a time, an LDS broadcast).  The sizes sit in LDS after the keys.
+constexpr int kSmallList = 1024;
+
+template <class Sc>
+__device__ void small_ranks(const Sc &sc, const uint64_t *lkey, int n, uint32_t *lsz) {
+    for (int i = threadIdx.x; i < n; i += kRankThreads)
+        lsz[i] = sc.size((int64_t)(lkey[i] & 0xFFFFFFFFu));
+    __syncthreads();
+}
+
+// Rank (entries of smaller key) and position base (their sizes' sum) of
+// entry i of a small list, counting over all n entries.
+__device__ __forceinline__ void small_rank_of(const uint64_t *lkey, const uint32_t *lsz, int n,
+                                              uint64_t ki, int *r_out, uint32_t *base_out) {
+    int r = 0;
+    uint32_t base = 0;
+#pragma unroll 8
+    for (int j = 0; j < n; ++j) {
+        const bool before = lkey[j] < ki;
+        r += before;
+        base += before ? lsz[j] : 0u;
+    }
+    *r_out = r;
+    *base_out = base;
+}
+
+// The stopping entry of a small list: the key (score bits) of the entry at
+// which the count reaches w or the size sum reaches T, in *thr; returns n when
+// the list does not reach the budget (then *thr is undefined).
+template <class Sc>
+__device__ int small_stop(const Sc &sc, uint64_t *lkey, int n, int w, int T, RankShared &sh,
+                          uint32_t *thr) {
+    uint32_t *lsz = reinterpret_cast<uint32_t *>(lkey + kSmallList);
+    small_ranks(sc, lkey, n, lsz);
+    if (threadIdx.x == 0) sh.stopkey = ~(uint64_t)0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += kRankThreads) {
+        int r;
+        uint32_t base;
+        small_rank_of(lkey, lsz, n, lkey[i], &r, &base);
+        // the entry of least rank at which the budget is reached; its key is
+        // published with the rank in one 64-bit minimum (rank high, key low)
+        if (r + 1 >= w || (int64_t)base + lsz[i] >= T)
+            atomicMin(&sh.stopkey, ((uint64_t)(uint32_t)r << 32) | (uint32_t)(lkey[i] >> 32));
+    }
+    __syncthreads();
+    const uint64_t sk = sh.stopkey;
+    *thr = (uint32_t)sk;
+    return sk == ~(uint64_t)0 ? n : (int)(sk >> 32);
+}
+
+// emit_sorted for a small list: every entry's rank and position base by
+// counting, the taken ones written at their rank.
+template <class Sc>
+__device__ void emit_small(const Sc &sc, const RankKernelArgs &k, int64_t q, uint64_t *lkey, int n,
+                           int w, int T, RankShared &sh) {
+    const RankCellsArgs &a = k.a;
+    uint32_t *lsz = reinterpret_cast<uint32_t *>(lkey + kSmallList);
+    small_ranks(sc, lkey, n, lsz);
+    int32_t *cells = a.cells + (size_t)q * a.ccap;
+    int32_t *clen = a.clen + (size_t)q * a.ccap;
+    uint32_t *cinfo = a.cinfo + (size_t)q * a.ccap;
+    if (threadIdx.x == 0) { sh.count = 0; sh.sum = 0; }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += kRankThreads) {
+        const uint64_t ki = lkey[i];
+        int r;
+        uint32_t base;
+        small_rank_of(lkey, lsz, n, ki, &r, &base);
+        if (r < w && (int64_t)base < T) {
+            const int64_t c = (int64_t)(ki & 0xFFFFFFFFu);
+            if (r < a.ccap) {
+                cells[r] = c > 0 ? k.L[c - 1] : 0;
+                clen[r] = (int32_t)lsz[i];
+                cinfo[r] = sc.nearest(c);
+            }
+            atomicMax(&sh.count, r + 1);
+            atomicMax(&sh.sum, base + lsz[i]);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int count = sh.count;
+        int32_t *qi = a.qinfo + (size_t)q * 8;
+        qi[0] = count; qi[1] = (int32_t)sh.sum; qi[2] = count; qi[3] = 0;
+        qi[4] = (count == k.ncells && count < w && (int64_t)sh.sum < T) ? 1 : 0;
+        qi[5] = 0; qi[6] = count > a.ccap ? 2 : 0; qi[7] = 0;
+    }
+}
+
+__global__ __launch_bounds__(kRankThreads) void k_rank_pairs(RankKernelArgs k) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    uint32_t *hcnt = reinterpret_cast<uint32_t *>(smem);           // [kBuckets]
+    uint32_t *hwt = hcnt + kBuckets;                               // [kBuckets]
+    uint64_t *lkey = reinterpret_cast<uint64_t *>(smem);           // [cap]
+    float *v = reinterpret_cast<float *>(smem + rank_abytes(k.cap));   // [2 kc]
+    const int nu0 = k.nu0, nu1 = k.nu1;
+    float *st0 = v + max(2 * k.kc, k.dim);                         // [nu0]
+    float *st1 = st0 + nu0;                                        // [nu1]
+    __shared__ RankShared sh;
+    __shared__ int32_t gbeg[kMaxPass], gsz[kMaxPass], gpre[kMaxPass + 1];
+    __shared__ int s_np;
+    __shared__ unsigned long long s_acc;
+    const int tid = threadIdx.x;
+    const int64_t q = k.qlist ? k.qlist[blockIdx.x] : blockIdx.x;
+    const int T = query_budget(k.a.Tq, q, k.a.T), w = k.a.w;
+    query_terms(k, q, v, sh);
+    if (empty_query(k, q, T)) return;
+    const int kc = k.kc;
+    if (tid == 0) sh.mink = ~0u;
+    for (int i = tid; i < nu0; i += kRankThreads) {
+        const uint32_t p = k.up0[i];
+        st0[i] = v[p & 0xFFFFu] + v[p >> 16];
+    }
+    __syncthreads();
+    uint32_t mk = ~0u;
+    for (int i = tid; i < nu1; i += kRankThreads) {
+        const uint32_t p = k.up1[i];
+        const float s1 = v[kc + (p & 0xFFFFu)] + v[kc + (p >> 16)];
+        st1[i] = s1;
+        mk = min(mk, sc_float_key(s1));
+    }
+    for (int o = 32; o > 0; o >>= 1) mk = min(mk, (uint32_t)__shfl_xor(mk, o));
+    if ((tid & 63) == 0) atomicMin(&sh.mink, mk);
+    __syncthreads();
+    const PairScorer sc{k.cpair, st0, st1, k.up0, k.up1};
+    const float s1min = sc_key_float(sh.mink);
+
+    // the subset: the half-0 pairs of least st0 whose cells together reach
+    // the budget (a weighted histogram of st0 over the pairs, weighted by
+    // their cells' count and size), and their cells
+    for (int i = tid; i < kBuckets; i += kRankThreads) { hcnt[i] = 0; hwt[i] = 0; }
+    __syncthreads();
+    for (int g = tid; g < nu0; g += kRankThreads) {
+        const int c0 = k.gstart[g], c1 = k.gstart[g + 1];
+        const uint32_t b = sc_float_key(st0[g]) >> 20;
+        atomicAdd(&hcnt[b], (uint32_t)(c1 - c0));
+        atomicAdd(&hwt[b], (uint32_t)(k.L[c1 - 1] - (c0 > 0 ? k.L[c0 - 1] : 0)));
+    }
+    __syncthreads();
+    const int b0 = find_bucket(hcnt, hwt, 0u, 0u, w, T, sh.wsum, &sh.b, sh.pre);
+    // of the crossing bucket, only as many pairs as the budget needs (any of
+    // them: every subset that reaches the budget gives a valid bound)
+    const int64_t need_c = (int64_t)w - sh.pre[0], need_w = (int64_t)T - sh.pre[1];
+    __syncthreads();
+    if (tid == 0) { sh.n = 0; s_np = 0; s_acc = 0; }
+    __syncthreads();
+    for (int g = tid; g < nu0; g += kRankThreads) {
+        const int gb = (int)(sc_float_key(st0[g]) >> 20);
+        if (gb > b0) continue;
+        if (gb == b0) {
+            const int c0 = k.gstart[g], c1 = k.gstart[g + 1];
+            const uint32_t gw = (uint32_t)(k.L[c1 - 1] - (c0 > 0 ? k.L[c0 - 1] : 0));
+            // one 64-bit counter (cells high, sizes low: neither sum reaches
+            // 2^32), so that every pair sees one consistent running total
+            const unsigned long long old =
+                atomicAdd(&s_acc, ((unsigned long long)(c1 - c0) << 32) | gw);
+            if ((int64_t)(old >> 32) >= need_c || (int64_t)(old & 0xFFFFFFFFull) >= need_w)
+                continue;
+        }
+        const int o = atomicAdd(&s_np, 1);
+        if (o < kMaxPass) {
+            gbeg[o] = k.gstart[g];
+            gsz[o] = k.gstart[g + 1] - gbeg[o];
+        }
+    }
+    __syncthreads();
+    int e = kSmallList, nsub = 0;
+    uint32_t thr = 0;
+    if (s_np <= kMaxPass) {
+        const int np0 = s_np;
+        group_prefix(gsz, gpre, np0, sh);
+        if (gpre[np0] <= kSmallList) {
+            for_group_cells(gbeg, gpre, np0, [&](int, int64_t c) {
+                const int o = atomicAdd(&sh.n, 1);
+                lkey[o] = ((uint64_t)sc_float_key(sc.score(c)) << 32) | (uint32_t)c;
+            });
+            __syncthreads();
+            nsub = sh.n;
+            e = small_stop(sc, lkey, nsub, w, T, sh, &thr);
+        }
+    }
+    __syncthreads();
+    bool done = false;
+    if (e < nsub) {
+        __syncthreads();
+        if (tid == 0) { sh.n = 0; s_np = 0; }
+        __syncthreads();
+        // the pairs that can hold a cell scoring at most thr
+        for (int g = tid; g < nu0; g += kRankThreads) {
+            if (sc_float_key(st0[g] + s1min) > thr) continue;
+            const int o = atomicAdd(&s_np, 1);
+            if (o < kMaxPass) {
+                gbeg[o] = k.gstart[g];
+                gsz[o] = k.gstart[g + 1] - gbeg[o];
+            }
+        }
+        __syncthreads();
+        const int np = s_np;
+        auto visit = [&](int64_t c) {
+            const uint32_t key = sc_float_key(sc.score(c));
+            if (key <= thr) {
+                const int o = atomicAdd(&sh.n, 1);
+                if (o < k.cap) lkey[o] = ((uint64_t)key << 32) | (uint32_t)c;
+            }
+        };
+        if (np <= kMaxPass) {
+            group_prefix(gsz, gpre, np, sh);
+            for_group_cells(gbeg, gpre, np, [&](int, int64_t c) { visit(c); });
+        } else {
+            for (int g = tid; g < nu0; g += kRankThreads) {
+                if (sc_float_key(st0[g] + s1min) > thr) continue;
+                for (int64_t c = k.gstart[g]; c < k.gstart[g + 1]; ++c) visit(c);
+            }
+        }
+        __syncthreads();
+        done = sh.n <= k.cap;
+    }
+    __syncthreads();
+    if (done) {
+        if (sh.n <= kSmallList) emit_small(sc, k, q, lkey, sh.n, w, T, sh);
+        else emit_sorted(sc, k, q, lkey, sh.n, w, T, sh);
+        return;
+    }
+    if (!hist_select(sc, k.ncells, w, T, k.cap, hcnt, hwt, lkey, sh)) {
+        flag_overflow(k, q);
+        return;
+    }
+    emit_sorted(sc, k, q, lkey, sh.n, w, T, sh);
 }
 
 // ---------------------------------------------------------------------------
@@ -398,38 +781,69 @@ int rank_cap() {
     return kCap;
 }
 
+// SC_RANK_PAIRS=0 (test hook): rank from the cell records only.
+bool use_pair_tables(const sc_index *ix) {
+    if (const char *e = getenv("SC_RANK_PAIRS"))
+        if (atoi(e) == 0) return false;
+    return ix->cpair && ix->gstart && ix->nu[0] <= kMaxPairs0 && ix->nu[0] + ix->nu[1] <= 8192;
+}
+
+// Launches the ranking of a.nq queries (or of the queries in qlist) with an
+// on-chip list of cap cells.
+template <int W>
+int launch_rank(sc_ctx *ctx, const sc_index *ix, RankKernelArgs k, const int32_t *qlist,
+                int64_t nq, int cap) {
+    k.qlist = qlist;
+    k.cap = cap;
+    const bool pairs = use_pair_tables(ix);
+    // v [2 kc] also holds the query row while its sums are taken
+    const size_t lds = rank_abytes(cap) + 4 * std::max<size_t>(2 * (size_t)ix->kc, ix->dim) +
+                       (pairs ? 4 * ((size_t)ix->nu[0] + ix->nu[1]) : 0);
+    if (lds > 150 * 1024) return SC_ERR_TOO_LARGE;
+    const void *kern = pairs ? (const void *)k_rank_pairs : (const void *)k_rank_cells<W>;
+    SC_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    if (pairs)
+        hipLaunchKernelGGL(k_rank_pairs, dim3((unsigned)nq), dim3(kRankThreads), lds, ctx->stream,
+                           k);
+    else
+        hipLaunchKernelGGL(k_rank_cells<W>, dim3((unsigned)nq), dim3(kRankThreads), lds,
+                           ctx->stream, k);
+    SC_CHECK_LAUNCH();
+    return SC_OK;
+}
+
+// The queries flagged as overflowing (qinfo[5]) into list; returns their count.
+int flagged(sc_ctx *ctx, const RankCellsArgs &a, int32_t *list, int32_t *dn, int32_t *nflag) {
+    SC_HIP(hipMemsetAsync(dn, 0, 4, ctx->stream));
+    hipLaunchKernelGGL(k_rank_flagged, dim3((unsigned)sc_cdiv(a.nq, 256)), dim3(256), 0,
+                       ctx->stream, a.qinfo, a.nq, list, dn);
+    SC_CHECK_LAUNCH();
+    SC_HIP(hipMemcpyAsync(nflag, dn, 4, hipMemcpyDeviceToHost, ctx->stream));
+    SC_HIP(hipStreamSynchronize(ctx->stream));
+    return SC_OK;
+}
+
 template <int W>
 int rank_cells(sc_ctx *ctx, const sc_index *ix, const RankCellsArgs &a) {
-    RankKernelArgs k{a, ix->kc, ix->dim, ix->ncells, ix->crec, ix->L, nullptr, rank_cap()};
-    const size_t lds = 8 * kBuckets + 8 * (size_t)ix->kc;
-    if (lds > 160 * 1024) {
+    RankKernelArgs k{a, ix->kc, ix->dim, ix->ncells, ix->crec, ix->L, nullptr, rank_cap(),
+                     ix->nu[0], ix->nu[1], ix->upairs[0], ix->upairs[1], ix->cpair, ix->gstart};
+    const int cap = rank_cap();
+    int rc = launch_rank<W>(ctx, ix, k, nullptr, a.nq, cap);
+    if (rc == SC_ERR_TOO_LARGE) {
         sc_set_error("sc_search: kc=%d too large for the on-chip cell ranking", ix->kc);
         return SC_ERR_ARG;
     }
-    auto kern = k_rank_cells<W>;
-    SC_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)lds));
-    hipLaunchKernelGGL(kern, dim3((unsigned)a.nq), dim3(kRankThreads), lds, ctx->stream, k);
-    SC_CHECK_LAUNCH();
-
-    // queries flagged for the device-wide path
-    int32_t *list = nullptr, *dn = nullptr;
-    const size_t lbytes = (size_t)a.nq * 4 + 256;
-    if (hipMalloc(&list, lbytes) != hipSuccess) {
-        sc_set_error("sc_search: ranking list allocation failed");
-        return SC_ERR_NOMEM;
-    }
-    dn = list + a.nq;
+    if (rc) return rc;
+    int32_t *list = a.flag_list, *dn = a.flag_count;
     int32_t nflag = 0;
-    int rc = SC_OK;
-    if (hipMemsetAsync(dn, 0, 4, ctx->stream) != hipSuccess) rc = SC_ERR_HIP;
-    if (rc == SC_OK) {
-        hipLaunchKernelGGL(k_rank_flagged, dim3((unsigned)sc_cdiv(a.nq, 256)), dim3(256), 0,
-                           ctx->stream, a.qinfo, a.nq, list, dn);
-        if (hipGetLastError() != hipSuccess ||
-            hipMemcpyAsync(&nflag, dn, 4, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
-            hipStreamSynchronize(ctx->stream) != hipSuccess)
-            rc = SC_ERR_HIP;
+    SC_TRY(flagged(ctx, a, list, dn, &nflag));
+    // queries whose cells did not fit: once more with a list four times as
+    // long (fewer workgroups per CU, only for them), when the LDS allows
+    if (nflag > 0 && cap == kCap) {
+        rc = launch_rank<W>(ctx, ix, k, list, nflag, 4 * kCap);
+        if (rc == SC_OK) SC_TRY(flagged(ctx, a, list, dn, &nflag));
+        else if (rc != SC_ERR_TOO_LARGE) return rc;
+        rc = SC_OK;
     }
     if (rc == SC_OK && nflag > 0) {
         // groups of queries whose keys (two buffers of ncells each) fit 1 GiB
@@ -477,8 +891,6 @@ int rank_cells(sc_ctx *ctx, const sc_index *ix, const RankCellsArgs &a) {
             if (rc == SC_ERR_HIP) sc_set_error("sc_search: ranking fallback launch failed");
         }
     }
-    (void)hipStreamSynchronize(ctx->stream);
-    (void)hipFree(list);
     return rc;
 }
 

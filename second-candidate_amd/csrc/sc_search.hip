@@ -30,7 +30,8 @@ namespace {
 constexpr int kScanThreads = 256;
 constexpr int kMaxFusedThreads = 512;   // fused workgroups run 256 or 512 threads
 constexpr int kBufCap = 2048;      // streaming top-R buffer (entries)
-constexpr int kRunCells = 256;     // cells staged per scan run
+constexpr int kRunCells = 256;     // cells staged per scan run (one per scan thread)
+static_assert(kRunCells == kScanThreads, "a scan run loads one cell per thread");
 constexpr int kSpec = 16;          // traversal pops run ahead of the L[] loads
 constexpr int kWin = 16;           // SC: cell sizes of the kWin x kWin nearest rank pairs
 
@@ -668,6 +669,7 @@ struct ScanArgs {
     const int32_t *clen;    // [nq][ccap] cell lengths (cells holds cell starts)
     const float *Q;         // SC_FORM_REAL: the queries [nq][dim]
     const float *raw;       // SC_FORM_REAL: raw base vectors [n][dim] by id, else null
+    int imi_mode;           // IMI scan: 0 byte-wise codes, 1 packed codes (mp = 8 or 16)
 };
 
 struct ScanShared {
@@ -676,6 +678,8 @@ struct ScanShared {
     int32_t c_start[kRunCells];  // cell start in pid/codes
     int32_t c_pref[kRunCells + 1];
     uint32_t c_info[kRunCells];
+    uint32_t scan_wsum[kScanThreads / 64];
+
     uint32_t hist[256];
     int nb;
     uint64_t thr;
@@ -959,6 +963,80 @@ __device__ void scan_run_real(const ScanArgs &a, ScanShared &S, const float *qv,
     }
 }
 
+// Exclusive prefix of one value per thread over a kScanThreads workgroup;
+// *total gets the sum.  All threads call it.
+__device__ uint32_t run_excl_scan(uint32_t x, uint32_t *wsum, uint32_t *total) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    uint32_t incl = x;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t t = __shfl_up(incl, o);
+        if (lane >= o) incl += t;
+    }
+    if (lane == 63) wsum[wv] = incl;
+    __syncthreads();
+    uint32_t base = 0, all = 0;
+    for (int i = 0; i < kScanThreads / 64; ++i) {
+        if (i < wv) base += wsum[i];
+        all += wsum[i];
+    }
+    *total = all;
+    return base + incl - x;
+}
+
+// The IMI sequence over a whole run in traversal order, the rows read from
+// dot_cr in global memory (L2), the codes loaded MPW words at a time.
+template <int MPW>
+__device__ void scan_run_imi_flat(const ScanArgs &a, ScanShared &S, const float *lut,
+                                  const float *dqc, float qsum, int rn, int rtotal, int base_pos) {
+    const int tid = threadIdx.x, NT = blockDim.x, U = kRound / NT;
+    const int kp = a.kp, kc = a.kc, bs = kp * 4 * MPW / a.mc;
+    int cur = 0;
+    for (int p0 = 0; p0 < rtotal; p0 += kRound) {
+        const bool full = S.nb > kBufCap - kRound;   // read by all before any append
+        __syncthreads();
+        if (full) compact(S, a.R);
+        uint32_t cw[4][MPW];
+        int vx[4], cl[4];
+        for (int u = 0; u < U; ++u) {
+            const int p = p0 + u * NT + tid;
+            vx[u] = -1;
+            if (p < rtotal) {
+                while (cur + 1 < rn && S.c_pref[cur + 1] <= p) ++cur;
+                vx[u] = S.c_start[cur] + (p - S.c_pref[cur]);
+                cl[u] = cur;
+                load_codes<MPW>(a.codes, vx[u], cw[u]);
+            }
+        }
+        for (int u = 0; u < U; ++u) {
+            if (vx[u] < 0) continue;
+            const uint32_t ci = S.c_info[cl[u]];
+            const int h3 = (int)(ci & 0xFFFFu), h4 = (int)(ci >> 16);
+            const float *r1 = a.dot_cr + (size_t)h3 * bs;
+            const float *r2 = a.dot_cr + (size_t)(kc + h4) * bs - (size_t)2 * MPW * kp;
+            float d = (qsum + dqc[h3]) + dqc[kc + h4];
+#pragma unroll
+            for (int w4 = 0; w4 < MPW; ++w4) {
+                const uint32_t wv = cw[u][w4];
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    const int k = w4 * 4 + b;
+                    const int i = k * kp + (int)((wv >> (8 * b)) & 255u);
+                    d += lut[i];
+                    d += k < 2 * MPW ? r1[i] : r2[i];
+                }
+            }
+            const int p = p0 + u * NT + tid;
+            const uint64_t key = ((uint64_t)sc_float_key(d) << 32) | (uint32_t)(base_pos + p);
+            if (key < S.thr) {
+                const int o = atomicAdd(&S.nb, 1);
+                S.key[o] = key;
+                S.idx[o] = vx[u];
+            }
+        }
+        __syncthreads();
+    }
+}
+
 __device__ void scan_query(const ScanArgs &a, ScanShared &S, float *lut, int64_t q,
                            const float *dqr, const float *dqc, float qsum,
                            const int32_t *cstart, const int32_t *clen, const uint32_t *cinfo,
@@ -967,6 +1045,7 @@ __device__ void scan_query(const ScanArgs &a, ScanShared &S, float *lut, int64_t
     const int mp = a.mp, kp = a.kp;
     const int lutn = mp * kp;
     const bool imi = a.kind == SC_KIND_IMI;
+    const int imi_mode = a.imi_mode;
     const bool real = a.raw != nullptr;   // lut holds the query vector then
     const int bs = kp * mp / a.mc;
     if (tid == 0) { S.nb = 0; S.thr = ~(uint64_t)0; }
@@ -980,23 +1059,29 @@ __device__ void scan_query(const ScanArgs &a, ScanShared &S, float *lut, int64_t
 
     int c0 = 0;
     while (c0 < ncell) {
-        // a run: consecutive cells sharing h3 (IVF/SC), or up to kRunCells (IMI)
-        if (tid == 0) {
-            int n = 0, acc = 0;
-            const uint32_t h3first = cinfo[c0] & 0xFFFFu;
-            while (c0 + n < ncell && n < kRunCells) {
-                const uint32_t ci = cinfo[c0 + n];
-                if (!imi && (ci & 0xFFFFu) != h3first) break;
-                S.c_start[n] = cstart[c0 + n];
-                S.c_pref[n] = acc;
-                S.c_info[n] = ci;
-                acc += clen[c0 + n];
-                ++n;
+        // a run: consecutive cells sharing h3 (IVF/SC), or up to kRunCells (IMI),
+        // loaded by all threads at once (one cell each) with a prefix sum of
+        // their sizes; the run ends before the first cell of another h3
+        {
+            const int navail = min(kRunCells, ncell - c0);
+            if (tid == 0) S.run_cells = navail;
+            __syncthreads();
+            const int i = tid;
+            uint32_t ln = 0;
+            if (i < navail) {
+                const uint32_t ci = cinfo[c0 + i];
+                S.c_start[i] = cstart[c0 + i];
+                S.c_info[i] = ci;
+                ln = (uint32_t)clen[c0 + i];
+                if (!imi && (ci & 0xFFFFu) != (cinfo[c0] & 0xFFFFu)) atomicMin(&S.run_cells, i);
             }
-            S.c_pref[n] = acc;
-            S.run_cells = n;
+            uint32_t tot;
+            const uint32_t ex = run_excl_scan(ln, S.scan_wsum, &tot);
+            if (i < navail) S.c_pref[i] = (int32_t)ex;
+            if (tid == 0) S.c_pref[navail] = (int32_t)tot;
         }
         __syncthreads();
+
         const int rn = S.run_cells;
         const int rtotal = S.c_pref[rn];
         float d0 = 0.0f;
@@ -1012,6 +1097,11 @@ __device__ void scan_query(const ScanArgs &a, ScanShared &S, float *lut, int64_t
         __syncthreads();
         if (real)
             scan_run_real(a, S, lut, rn, rtotal, base_pos);
+        else if (imi && imi_mode == 1 && mp == 8)
+            scan_run_imi_flat<2>(a, S, lut, dqc, qsum, rn, rtotal, base_pos);
+        else if (imi && imi_mode == 1 && mp == 16)
+            scan_run_imi_flat<4>(a, S, lut, dqc, qsum, rn, rtotal, base_pos);
+
         else if (!imi && mp == 8)
             scan_run_packed<2>(a, S, lut, d0, rn, rtotal, base_pos);
         else if (!imi && mp == 16)
@@ -1401,6 +1491,7 @@ struct GroupCells {
 template <int MPW, bool MIXED>
 __device__ __forceinline__ void scan_group(const ScanArgs &a, TopShared &T, uint64_t *key, int bufcap, const float *lut, float d0, int p0, int p1, int i0,
                            const GroupCells &G, float S, float twoE) {
+    constexpr int MW = MPW > 0 ? MPW : 1;
     const _Float16 *lut16 = reinterpret_cast<const _Float16 *>(lut);
     const int tid = threadIdx.x, NT = blockDim.x, U = kRound / NT;
     const int mp = a.mp, kp = a.kp;
@@ -1421,7 +1512,7 @@ __device__ __forceinline__ void scan_group(const ScanArgs &a, TopShared &T, uint
             else compact_top(T, key, a.R);
         }
         const uint64_t thr = T.thr;
-        uint32_t cw[4][MPW > 0 ? MPW : 1];
+        uint32_t cw[4][MW];
         int vx[4], ps[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
@@ -1453,7 +1544,7 @@ __device__ __forceinline__ void scan_group(const ScanArgs &a, TopShared &T, uint
                     float acc = 0.0f;
                     if (MPW > 0) {
 #pragma unroll
-                        for (int w4 = 0; w4 < (MPW > 0 ? MPW : 1); ++w4) {
+                        for (int w4 = 0; w4 < MW; ++w4) {
                             const uint32_t wv = cw[u][w4];
 #pragma unroll
                             for (int b = 0; b < 4; ++b)
@@ -1466,7 +1557,7 @@ __device__ __forceinline__ void scan_group(const ScanArgs &a, TopShared &T, uint
                     d = d0 + S * acc;
                 } else if (MPW > 0) {
 #pragma unroll
-                    for (int w4 = 0; w4 < (MPW > 0 ? MPW : 1); ++w4) {
+                    for (int w4 = 0; w4 < MW; ++w4) {
                         const uint32_t wv = cw[u][w4];
 #pragma unroll
                         for (int b = 0; b < 4; ++b)
@@ -2224,6 +2315,12 @@ __global__ void k_xcd_order(const int32_t *__restrict__ sorted, int64_t n, int n
     perm[i] = sorted[off + j];
 }
 
+// SC_IMI_SCAN (tuning hook): the IMI scan variant (see ScanArgs::imi_mode).
+static int imi_scan_mode() {
+    if (const char *e = getenv("SC_IMI_SCAN")) return atoi(e);
+    return 1;
+}
+
 // SC_SCAN_ORDER (tuning hook): 0 launch order, 1 sorted, 2 sorted per XCD.
 static int scan_order_mode() {
     if (const char *e = getenv("SC_SCAN_ORDER")) return atoi(e);
@@ -2717,7 +2814,8 @@ static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
             ta.qlist = perm;
         }
         if (halves) {
-            RankCellsArgs ra{Q, dqc, qs, cells, clen, cinfo, qinfo, Tq, cn, p->w, p->T, P.ccap};
+            RankCellsArgs ra{Q, dqc, qs, cells, clen, cinfo, qinfo, Tq, cn, p->w, p->T, P.ccap,
+                             list, list + P.chunk};
             SC_TRY(sc_rank_cells(ctx, ix, ra));
         } else if (three) {
             SC_TRY(traverse3(ctx, p, kc, sv, si, ix->L, cells, clen, cinfo, qinfo, P.ccap, cn, Tq));
@@ -2737,14 +2835,18 @@ static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
                     ix->dot_cr, ix->L, ix->pid, ix->codes, cells, cinfo, qinfo,
                     d_ids + (size_t)q0 * p->R, d_dists + (size_t)q0 * p->R,
                     d_sums ? d_sums + q0 : nullptr, nullptr, clen,
-                    Q, p->form == SC_FORM_REAL ? ix->raw : nullptr};
-        const size_t lds = sizeof(ScanShared) +
-                           sizeof(float) * std::max<size_t>((size_t)ix->mp * ix->kp,
-                                                            p->form == SC_FORM_REAL ? ix->dim : 0);
+                    Q, p->form == SC_FORM_REAL ? ix->raw : nullptr, imi_scan_mode()};
+        // LDS: the diff_qr table (or the query row of the real-distance form)
+        const size_t lds =
+            sizeof(ScanShared) +
+            sizeof(float) * std::max<size_t>((size_t)ix->mp * ix->kp,
+                                             p->form == SC_FORM_REAL ? ix->dim : 0);
         if (order > 0 && cn > 1) {
             if (!ordered) SC_TRY(sort_order(false));
             sa.qlist = perm;
         }
+        SC_HIP(hipFuncSetAttribute((const void *)k_scan_topk,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         if (hcap_eff > 0 || three || halves) {
             hipLaunchKernelGGL(k_scan_topk, dim3((unsigned)cn), dim3(kScanThreads), lds,
                                ctx->stream, sa);

@@ -232,6 +232,16 @@ SCIMI_CASES = [  # kc, R, w, T
 ]
 
 
+@pytest.mark.parametrize("case", SCIMI_CASES[:3])
+def test_search_scimi_record_ranking_matches_oracle(ctx, case, monkeypatch):
+    """The ranking from the cells' records alone (no pair tables)."""
+    monkeypatch.setenv("SC_RANK_PAIRS", "0")
+    kc, R, w, T = case
+    ix, oix = _index_pair(ctx, S.KIND_SCIMI, kc=kc)
+    _compare_search(ctx, ix, oix, S.KIND_SCIMI, R, w, T, nq=100)
+    ix.close()
+
+
 @pytest.mark.parametrize("case", SCIMI_CASES)
 def test_search_scimi_matches_oracle(ctx, case):
     """Two halves with (nearest, second) cells per half: the sparse
@@ -255,12 +265,14 @@ def test_search_scimi_matches_oracle(ctx, case):
     ix.close()
 
 
+@pytest.mark.parametrize("pairs", ["1", "0"])
 @pytest.mark.parametrize("cap", ["16", "1"])
-def test_search_scimi_small_rank_list(ctx, cap, monkeypatch):
+def test_search_scimi_small_rank_list(ctx, cap, pairs, monkeypatch):
     """An on-chip ranking list of 16 (or 1) cells: queries go through the
-    second histogram level and the device-wide sorted fallback; results must
-    not change."""
+    bounded subset, the second histogram level and the device-wide sorted
+    fallback, with and without the pair tables; results must not change."""
     monkeypatch.setenv("SC_RANK_CAP", cap)
+    monkeypatch.setenv("SC_RANK_PAIRS", pairs)
     ix, oix = _index_pair(ctx, S.KIND_SCIMI, kc=8)
     for R, w, T in ((100, 100000, 3000), (50, 7, 10 ** 6), (20, 10 ** 6, 10 ** 6)):
         _compare_search(ctx, ix, oix, S.KIND_SCIMI, R, w, T, nq=60)
