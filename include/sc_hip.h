@@ -61,9 +61,11 @@ extern "C" {
 #define SC_FORM_PLAIN 0   /* FP32 LUT scan */
 #define SC_FORM_MIXED 1   /* FP16 LUT scan + exact FP32 re-rank: ids and
                              distances bit-identical to SC_FORM_PLAIN.
-                             Kinds without a mixed scan (SC_KIND_SC3,
-                             SC_KIND_IMI, SC_KIND_SCIMI) refuse it with
-                             SC_ERR_ARG rather than run the plain scan */
+                             IVF/SC: FP16 combined tables; IMI and SCIMI
+                             (mp = 8 or 16): an FP16 diff_qr table plus each
+                             vector's stored dot_cr sum.  SC_KIND_SC3 has no
+                             mixed scan and refuses it with SC_ERR_ARG rather
+                             than run the plain scan */
 #define SC_FORM_REAL 2    /* the reference's real_dist = true: every gathered
                              candidate ranked by its exact distance to its raw
                              vector (include/query.h:307-312,

@@ -152,6 +152,37 @@ __global__ void k_group_starts(const uint32_t *__restrict__ cpair, int64_t ncell
     if (c == 0 || (cpair[2 * (c - 1)] & 0xFFFFu) != g) gstart[g] = (int32_t)c;
 }
 
+// Per vector p (cell order) of a two-block index, the sum of its dot_cr terms
+// in subspace order, e = sum_k dot_cr[row(k)][k][c_k] with row(k) the cell's
+// nearest centroid of the half subspace k lies in (include/multi_query.h:
+// 231-253 adds the same terms, interleaved with the query's).
+__global__ void k_vec_dot(const uint8_t *__restrict__ codes, const int32_t *__restrict__ L,
+                          int64_t ncells, const uint64_t *__restrict__ ckeys, int64_t n, int kc,
+                          int mp, int kp, const float *__restrict__ dot_cr, float *__restrict__ evec) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    int64_t lo = 0, hi = ncells;            // the cell of p: first c with L[c] > p
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if ((int64_t)L[mid] <= p) lo = mid + 1; else hi = mid;
+    }
+    int n0, n1;
+    if (ckeys) {
+        const uint64_t k2 = (uint64_t)kc * (uint64_t)kc, key = ckeys[lo];
+        n0 = (int)(key / k2 / (uint64_t)kc);
+        n1 = (int)(key % k2 / (uint64_t)kc);
+    } else {
+        n0 = (int)(lo / kc);
+        n1 = (int)(lo % kc);
+    }
+    const int half = mp >> 1, bs = kp * half;
+    const uint8_t *cc = codes + (size_t)p * mp;
+    float e = 0.0f;
+    for (int k = 0; k < half; ++k) e += dot_cr[(size_t)n0 * bs + k * kp + cc[k]];
+    for (int k = half; k < mp; ++k) e += dot_cr[(size_t)(kc + n1) * bs + (k - half) * kp + cc[k]];
+    evec[p] = e;
+}
+
 // out = max |x| over x[0..n) as float bits (non-negative floats order like
 // their bit patterns)
 __global__ void k_absmax(const float *__restrict__ x, int64_t n, uint32_t *out) {
@@ -374,6 +405,17 @@ int finish_index(sc_index *ix) {
     hipLaunchKernelGGL(k_absmax, dim3((unsigned)std::min<int64_t>(1024, sc_cdiv(ndot, 256))),
                        dim3(256), 0, ctx->stream, ix->dot_cr, ndot, (uint32_t *)(dmax + 2));
     SC_CHECK_LAUNCH();
+    if (ix->mc == 2 && ix->n > 0) {
+        if (hipMalloc(&ix->evec, sizeof(float) * (size_t)ix->n) != hipSuccess) {
+            sc_set_error("index: per-vector sums allocation failed");
+            return SC_ERR_NOMEM;
+        }
+        hipLaunchKernelGGL(k_vec_dot, dim3((unsigned)sc_cdiv(ix->n, 256)), dim3(256), 0,
+                           ctx->stream, ix->codes, ix->L, ix->ncells,
+                           ix->sparse ? ix->ckeys : nullptr, ix->n, ix->kc, ix->mp, ix->kp,
+                           ix->dot_cr, ix->evec);
+        SC_CHECK_LAUNCH();
+    }
     SC_HIP(hipMemcpyAsync(h, dmax, 3 * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
     SC_HIP(hipStreamSynchronize(ctx->stream));
     ix->max_cell = h[0];
@@ -730,7 +772,7 @@ extern "C" int sc_index_destroy(sc_index *ix) {
     void *bufs[] = {ix->cq,    ix->pq,   ix->norm_c, ix->norm_r, ix->dot_cr,
                     ix->L,     ix->pid,  ix->codes,  ix->cqT,    ix->pqT,
                     ix->ckeys, ix->crec, ix->upairs[0], ix->upairs[1], ix->cpair,
-                    ix->gstart};
+                    ix->gstart, ix->evec};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     delete ix;

@@ -46,6 +46,10 @@ struct sc_index {
     uint32_t *upairs[2];            // [nu[h]] n | s << 16
     uint32_t *cpair;                // [ncells][2]
     int32_t *gstart;                // [nu[0] + 1] first cell of each half-0 pair
+    // Two-block indexes (mc = 2): per vector in cell order, the sequential FP32
+    // sum of its dot_cr terms (the query-independent part of its IMI distance),
+    // for the mixed-form scan.
+    float *evec;                    // [n]
 };
 
 void sc_set_error(const char *fmt, ...);

@@ -30,6 +30,7 @@ namespace {
 constexpr int kScanThreads = 256;
 constexpr int kMaxFusedThreads = 512;   // fused workgroups run 256 or 512 threads
 constexpr int kBufCap = 2048;      // streaming top-R buffer (entries)
+constexpr int kRound = 1024;       // candidates a scan workgroup takes per round
 constexpr int kRunCells = 256;     // cells staged per scan run (one per scan thread)
 static_assert(kRunCells == kScanThreads, "a scan run loads one cell per thread");
 constexpr int kSpec = 16;          // traversal pops run ahead of the L[] loads
@@ -670,6 +671,14 @@ struct ScanArgs {
     const float *Q;         // SC_FORM_REAL: the queries [nq][dim]
     const float *raw;       // SC_FORM_REAL: raw base vectors [n][dim] by id, else null
     int imi_mode;           // IMI scan: 0 byte-wise codes, 1 packed codes (mp = 8 or 16)
+    // mixed form of the IMI scan (SC_FORM_MIXED): per-vector dot_cr sums, the
+    // index's max |dot_cr|, and qinfo (slot 7 flags a query for an exact redo)
+    const float *evec;
+    float dot_absmax;
+    int32_t *redo;
+    float margin_scale;     // test hook: widens the keep margin (1 = the bound)
+    const uint64_t *ckeys;  // two-halves directory keys (null: dense cells)
+    int64_t ncells;
 };
 
 struct ScanShared {
@@ -679,6 +688,9 @@ struct ScanShared {
     int32_t c_pref[kRunCells + 1];
     uint32_t c_info[kRunCells];
     uint32_t scan_wsum[kScanThreads / 64];
+    float c_d0[kRunCells];       // IMI runs: each cell's start term
+    int overflow;                // mixed: the kept set outgrew the buffer
+    float lscale, twoE;          // mixed: FP16 table scale, keep margin
 
     uint32_t hist[256];
     int nb;
@@ -757,6 +769,48 @@ __device__ void compact(ScanShared &S, int R) {
     __syncthreads();
 }
 
+// Mixed form: keeps every entry whose approximate distance is within twoE of
+// the R-th smallest (a true top-R candidate is never dropped, see the mixed
+// IMI scan); when they would not leave room for a scan round, the query is
+// marked for an exact redo and only the R smallest are kept.
+__device__ void compact_margin(ScanShared &S, int R, float twoE) {
+    const int nb = S.nb;
+    const uint64_t kth = select_kth(S, nb, R);
+    const float aR = sc_key_float((uint32_t)(kth >> 32));
+    const uint32_t kthr = sc_float_key(aR + twoE);
+    const uint64_t keep = ((uint64_t)kthr << 32) | 0xFFFFFFFFull;
+    __shared__ int s_cnt;
+    if (threadIdx.x == 0) s_cnt = 0;
+    __syncthreads();
+    int c = 0;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) c += S.key[i] <= keep;
+    atomicAdd(&s_cnt, c);
+    __syncthreads();
+    const int cnt = s_cnt;
+    if (cnt > kBufCap - kRound) {
+        if (threadIdx.x == 0) S.overflow = 1;
+        compact(S, R);
+        return;
+    }
+    __shared__ int s_w;
+    if (threadIdx.x == 0) s_w = 0;
+    __syncthreads();
+    uint64_t mk[kBufCap / kScanThreads];
+    int32_t mi[kBufCap / kScanThreads];
+    int m = 0;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x)
+        if (S.key[i] <= keep) { mk[m] = S.key[i]; mi[m] = S.idx[i]; ++m; }
+    __syncthreads();
+    for (int j = 0; j < m; ++j) {
+        const int o = atomicAdd(&s_w, 1);
+        S.key[o] = mk[j];
+        S.idx[o] = mi[j];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) { S.nb = cnt; S.thr = keep; }
+    __syncthreads();
+}
+
 // Sorts key/idx[0..n) ascending (n <= kBufCap) by bitonic sort.
 __device__ void sort_buffer(ScanShared &S, int n) {
     int np = 1;
@@ -788,7 +842,6 @@ __device__ void sort_buffer(ScanShared &S, int n) {
 // of them (positions tid, tid + blockDim, ...), walking a cell pointer that
 // only moves forward, so their code loads are independent and in flight
 // together.  Additions keep the reference's order (k ascending).
-constexpr int kRound = 1024;
 
 template <int MPW>   // 32-bit words of codes per candidate (mp = 4 * MPW)
 __device__ __forceinline__ void load_codes(const uint8_t *codes, int vidx, uint32_t *w) {
@@ -1037,6 +1090,141 @@ __device__ void scan_run_imi_flat(const ScanArgs &a, ScanShared &S, const float 
     }
 }
 
+// Mixed form of the IMI scan.  The exact distance E (include/multi_query.h:
+// 231-253) is d0 plus, for every subspace k, diff_qr[k][c_k] and dot_cr[row][k]
+// [c_k] added one at a time.  The dot_cr terms do not depend on the query, so
+// the index keeps their sequential FP32 sum per vector, e_v; the approximate
+// distance is
+//     A = (d0 + S * sum_k f16(diff_qr[k][c_k] / S)) + e_v
+// (8 FP16 LDS lookups and one coalesced load instead of 16 lookups, 8 of them
+// L2 gathers).  |A - E| <= E_q for the bound of mixed_bound(); every candidate
+// with A within 2 E_q of the running R-th smallest A is kept (compact_margin),
+// which contains the exact top-R, and the kept ones are re-ranked with E.
+template <int MPW>
+__device__ void scan_run_imi_mixed(const ScanArgs &a, ScanShared &S, const _Float16 *lut16,
+                                   int rn, int rtotal, int base_pos) {
+    const int tid = threadIdx.x, NT = blockDim.x, U = kRound / NT;
+    const int kp = a.kp;
+    const float scale = S.lscale, twoE = S.twoE;
+    int cur = 0;
+    for (int p0 = 0; p0 < rtotal; p0 += kRound) {
+        const bool full = S.nb > kBufCap - kRound;   // read by all before any append
+        __syncthreads();
+        if (full) compact_margin(S, a.R, twoE);
+        uint32_t cw[4][MPW];
+        int vx[4], cl[4];
+        float ev[4];
+        for (int u = 0; u < U; ++u) {
+            const int p = p0 + u * NT + tid;
+            vx[u] = -1;
+            if (p < rtotal) {
+                while (cur + 1 < rn && S.c_pref[cur + 1] <= p) ++cur;
+                vx[u] = S.c_start[cur] + (p - S.c_pref[cur]);
+                cl[u] = cur;
+                load_codes<MPW>(a.codes, vx[u], cw[u]);
+                ev[u] = a.evec[vx[u]];
+            }
+        }
+        for (int u = 0; u < U; ++u) {
+            if (vx[u] < 0) continue;
+            float acc = 0.0f;
+#pragma unroll
+            for (int w4 = 0; w4 < MPW; ++w4) {
+                const uint32_t wv = cw[u][w4];
+#pragma unroll
+                for (int b = 0; b < 4; ++b)
+                    acc += (float)lut16[(w4 * 4 + b) * kp + ((wv >> (8 * b)) & 255u)];
+            }
+            const float A = (S.c_d0[cl[u]] + scale * acc) + ev[u];
+            const int p = p0 + u * NT + tid;
+            const uint64_t key = ((uint64_t)sc_float_key(A) << 32) | (uint32_t)(base_pos + p);
+            if (key < S.thr) {
+                const int o = atomicAdd(&S.nb, 1);
+                S.key[o] = key;
+                S.idx[o] = vx[u];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Per query, before the mixed IMI scan: the FP16 table f16(diff_qr / S) in
+// LDS with S a power of two that keeps every entry below 2^14, and the keep
+// margin 2 E_q.  The bound on |A - E| (A, E as above; u = 2^-24):
+//   table rounding          2^-11 * sum_k |diff_qr_k|  +  mp * S * u (subnormals)
+//   sums of A and of e_v    (7 + 1 + 1 + 7) * u * P
+//   the 16 additions of E   16 * u * P
+// with P >= |d0| + sum_k |diff_qr_k| + sum_k |dot_cr_k| >= every partial sum,
+// |d0| <= |q_sum| + 2 max|diff_qc|; sum_k |.| <= mp * max|.|.  E_q is that
+// bound times 1.25 (room for the rounding of the bound itself).
+__device__ void mixed_setup(const ScanArgs &a, ScanShared &S, _Float16 *lut16, const float *dqr,
+                            const float *dqc, float qsum) {
+    const int tid = threadIdx.x, lutn = a.mp * a.kp;
+    __shared__ uint32_t s_m[2];
+    if (tid == 0) { s_m[0] = 0; s_m[1] = 0; }
+    __syncthreads();
+    uint32_t m0 = 0, m1 = 0;
+    for (int i = tid; i < lutn; i += blockDim.x) m0 = max(m0, __float_as_uint(fabsf(dqr[i])));
+    for (int i = tid; i < 2 * a.kc; i += blockDim.x) m1 = max(m1, __float_as_uint(fabsf(dqc[i])));
+    for (int o = 32; o > 0; o >>= 1) {
+        m0 = max(m0, (uint32_t)__shfl_xor(m0, o));
+        m1 = max(m1, (uint32_t)__shfl_xor(m1, o));
+    }
+    if ((tid & 63) == 0) { atomicMax(&s_m[0], m0); atomicMax(&s_m[1], m1); }
+    __syncthreads();
+    const float maxlut = __uint_as_float(s_m[0]), maxqc = __uint_as_float(s_m[1]);
+    int ex = 0;
+    (void)frexpf(maxlut, &ex);                     // maxlut < 2^ex
+    const float scale = maxlut > 0.0f ? ldexpf(1.0f, ex - 14) : 1.0f;
+    const float inv = 1.0f / scale;                // exact: a power of two
+    for (int i = tid; i < lutn; i += blockDim.x) lut16[i] = (_Float16)(dqr[i] * inv);
+    if (tid == 0) {
+        const double u = ldexp(1.0, -24), mp = a.mp;
+        const double P = (fabs((double)qsum) + 2.0 * maxqc) + mp * ((double)maxlut + a.dot_absmax);
+        const double E = ldexp(1.0, -11) * mp * maxlut + mp * scale * u + 32.0 * u * P;
+        S.lscale = scale;
+        S.twoE = (float)(2.0 * 1.25 * E * a.margin_scale);
+    }
+    __syncthreads();
+}
+
+// The exact IMI distance of the vector at index position vidx: its cell is
+// the first one whose inclusive prefix L exceeds vidx, the cell's nearest
+// centroids (h3, h4) come from its number (dense: h3*kc + h4) or its key
+// (two halves), and the operation sequence is include/multi_query.h:231-253.
+__device__ __forceinline__ float imi_exact(const ScanArgs &a, const float *dqr, const float *dqc,
+                                           float qsum, int vidx) {
+    const int mp = a.mp, kp = a.kp, kc = a.kc, half = mp >> 1, bs = kp * mp / a.mc;
+    int64_t lo = 0, hi = a.ncells;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (a.L[mid] <= vidx) lo = mid + 1; else hi = mid;
+    }
+    int h3, h4;
+    if (a.ckeys) {
+        const uint64_t k2 = (uint64_t)kc * (uint64_t)kc, key = a.ckeys[lo];
+        h3 = (int)(key / k2 / (uint64_t)kc);
+        h4 = (int)(key % k2 / (uint64_t)kc);
+    } else {
+        h3 = (int)(lo / kc);
+        h4 = (int)(lo % kc);
+    }
+    const float *r1 = a.dot_cr + (size_t)h3 * bs, *r2 = a.dot_cr + (size_t)(h4 + kc - 1) * bs;
+    const uint8_t *cc = a.codes + (size_t)vidx * mp;
+    float d = (qsum + dqc[h3]) + dqc[kc + h4];
+    for (int k = 0; k < half; ++k) {
+        const int b = k * kp + cc[k];
+        d += dqr[b];
+        d += r1[b];
+    }
+    for (int k = half; k < mp; ++k) {
+        const int b = k * kp + cc[k];
+        d += dqr[b];
+        d += r2[b];
+    }
+    return d;
+}
+
 __device__ void scan_query(const ScanArgs &a, ScanShared &S, float *lut, int64_t q,
                            const float *dqr, const float *dqc, float qsum,
                            const int32_t *cstart, const int32_t *clen, const uint32_t *cinfo,
@@ -1047,10 +1235,13 @@ __device__ void scan_query(const ScanArgs &a, ScanShared &S, float *lut, int64_t
     const bool imi = a.kind == SC_KIND_IMI;
     const int imi_mode = a.imi_mode;
     const bool real = a.raw != nullptr;   // lut holds the query vector then
+    const bool mixed = imi && a.evec && !real && (mp == 8 || mp == 16);
     const int bs = kp * mp / a.mc;
-    if (tid == 0) { S.nb = 0; S.thr = ~(uint64_t)0; }
+    if (tid == 0) { S.nb = 0; S.thr = ~(uint64_t)0; S.overflow = 0; }
     if (real)
         for (int i = tid; i < a.dim; i += blockDim.x) lut[i] = a.Q[(size_t)q * a.dim + i];
+    else if (mixed)
+        mixed_setup(a, S, reinterpret_cast<_Float16 *>(lut), dqr, dqc, qsum);
     else if (imi)
         for (int i = tid; i < lutn; i += blockDim.x) lut[i] = dqr[i];
     int lut_row = -1;     // h3 whose combined row is staged (IVF/SC)
@@ -1084,6 +1275,13 @@ __device__ void scan_query(const ScanArgs &a, ScanShared &S, float *lut, int64_t
 
         const int rn = S.run_cells;
         const int rtotal = S.c_pref[rn];
+        if (mixed) {
+            for (int i = tid; i < rn; i += blockDim.x) {
+                const uint32_t ci = S.c_info[i];
+                S.c_d0[i] = (qsum + dqc[ci & 0xFFFFu]) + dqc[a.kc + (ci >> 16)];
+            }
+            __syncthreads();
+        }
         float d0 = 0.0f;
         if (!imi && !real) {
             const int h3 = (int)(S.c_info[0] & 0xFFFFu);
@@ -1097,6 +1295,12 @@ __device__ void scan_query(const ScanArgs &a, ScanShared &S, float *lut, int64_t
         __syncthreads();
         if (real)
             scan_run_real(a, S, lut, rn, rtotal, base_pos);
+        else if (mixed && mp == 8)
+            scan_run_imi_mixed<2>(a, S, reinterpret_cast<const _Float16 *>(lut), rn, rtotal,
+                                  base_pos);
+        else if (mixed && mp == 16)
+            scan_run_imi_mixed<4>(a, S, reinterpret_cast<const _Float16 *>(lut), rn, rtotal,
+                                  base_pos);
         else if (imi && imi_mode == 1 && mp == 8)
             scan_run_imi_flat<2>(a, S, lut, dqc, qsum, rn, rtotal, base_pos);
         else if (imi && imi_mode == 1 && mp == 16)
@@ -1113,6 +1317,18 @@ __device__ void scan_query(const ScanArgs &a, ScanShared &S, float *lut, int64_t
         __syncthreads();
     }
     if (t_scanned && tid == 0) *t_scanned = wall_clock64();
+    if (mixed) {
+        if (S.overflow) {              // redone exactly by the caller
+            if (tid == 0 && a.redo) a.redo[q * 8 + 7] = 1;
+            return;
+        }
+        // exact re-rank of the kept candidates (their positions stay)
+        for (int i = tid; i < S.nb; i += blockDim.x) {
+            const float e = imi_exact(a, dqr, dqc, qsum, S.idx[i]);
+            S.key[i] = ((uint64_t)sc_float_key(e) << 32) | (S.key[i] & 0xFFFFFFFFull);
+        }
+        __syncthreads();
+    }
     // final selection
     const int R = a.R;
     int nout;
@@ -2202,6 +2418,12 @@ __global__ void k_scatter_results(const int32_t *__restrict__ list, int n, int R
     if (c == 0 && out_sums) out_sums[q] = sums[r];
 }
 
+__global__ void k_gather_redo(const int32_t *__restrict__ qinfo, int64_t nq,
+                              int32_t *__restrict__ list, int32_t *__restrict__ n) {
+    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    if (qinfo[q * 8 + 7]) list[atomicAdd(n, 1)] = (int32_t)q;
+}
 __global__ void k_gather_fallback(const int32_t *__restrict__ qinfo, int64_t nq,
                                    int32_t *__restrict__ list, int32_t *__restrict__ n) {
     int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -2619,8 +2841,7 @@ static int search_impl(sc_ctx *ctx, const sc_index *ix, const float *d_q, int64_
                      kind == SC_KIND_SC3 ? 3 : 2, ix->kc);
         return SC_ERR_ARG;
     }
-    if (p->form == SC_FORM_MIXED &&
-        (kind == SC_KIND_SC3 || kind == SC_KIND_IMI || kind == SC_KIND_SCIMI)) {
+    if (p->form == SC_FORM_MIXED && kind == SC_KIND_SC3) {
         sc_set_error("sc_search: SC_FORM_MIXED is not available for kind %d; "
                      "use SC_FORM_PLAIN", kind);
         return SC_ERR_ARG;
@@ -2835,7 +3056,10 @@ static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
                     ix->dot_cr, ix->L, ix->pid, ix->codes, cells, cinfo, qinfo,
                     d_ids + (size_t)q0 * p->R, d_dists + (size_t)q0 * p->R,
                     d_sums ? d_sums + q0 : nullptr, nullptr, clen,
-                    Q, p->form == SC_FORM_REAL ? ix->raw : nullptr, imi_scan_mode()};
+                    Q, p->form == SC_FORM_REAL ? ix->raw : nullptr, imi_scan_mode(),
+                    p->form == SC_FORM_MIXED && scan_kind == SC_KIND_IMI ? ix->evec : nullptr,
+                    ix->dot_absmax, qinfo, mixed_margin_scale(),
+                    ix->sparse ? ix->ckeys : nullptr, ix->ncells};
         // LDS: the diff_qr table (or the query row of the real-distance form)
         const size_t lds =
             sizeof(ScanShared) +
@@ -2851,6 +3075,25 @@ static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
             hipLaunchKernelGGL(k_scan_topk, dim3((unsigned)cn), dim3(kScanThreads), lds,
                                ctx->stream, sa);
             SC_CHECK_LAUNCH();
+        }
+        if (sa.evec) {
+            // mixed IMI scan: queries whose kept set outgrew the buffer are
+            // scanned again exactly (their first outputs are overwritten)
+            int32_t *dn = list + P.chunk, nredo = 0;
+            SC_HIP(hipMemsetAsync(dn, 0, 4, ctx->stream));
+            hipLaunchKernelGGL(k_gather_redo, dim3((unsigned)sc_cdiv(cn, 256)), dim3(256), 0,
+                               ctx->stream, qinfo, cn, list, dn);
+            SC_CHECK_LAUNCH();
+            SC_HIP(hipMemcpyAsync(&nredo, dn, 4, hipMemcpyDeviceToHost, ctx->stream));
+            SC_HIP(hipStreamSynchronize(ctx->stream));
+            if (nredo > 0) {
+                ScanArgs ra = sa;
+                ra.evec = nullptr;
+                ra.qlist = list;
+                hipLaunchKernelGGL(k_scan_topk, dim3((unsigned)nredo), dim3(kScanThreads), lds,
+                                   ctx->stream, ra);
+                SC_CHECK_LAUNCH();
+            }
         }
         mark();
         // Queries whose heap outgrew the on-chip capacity are traversed again

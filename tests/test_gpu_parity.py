@@ -255,6 +255,7 @@ def test_search_scimi_matches_oracle(ctx, case):
     assert np.array_equal(e["codes"], oix.codes)
     assert np.array_equal(_bits(e["dot_cr"]), _bits(oix.dot_cr))
     st = _compare_search(ctx, ix, oix, S.KIND_SCIMI, R, w, T, nq=100)
+    _compare_search(ctx, ix, oix, S.KIND_SCIMI, R, w, T, nq=100, form=S.FORM_MIXED)
     if T == 0:
         assert st.candidates == 0 and st.cells == 0
     if kc == 4:
@@ -281,8 +282,8 @@ def test_search_scimi_small_rank_list(ctx, cap, pairs, monkeypatch):
 
 def test_search_scimi_load_sparse_and_refusals(ctx):
     """sc_index_load_sparse from the oracle's arrays gives the same search;
-    a dense load of a two-halves index, unordered keys, an empty cell and the
-    mixed form are refused."""
+    a dense load of a two-halves index, unordered keys and an empty cell are
+    refused."""
     ix, oix = _index_pair(ctx, S.KIND_SCIMI, kc=8)
     lx = S.Index.load_sparse(ctx, oix.ckeys, oix.L, oix.pid, oix.codes, oix.dim, oix.cq, oix.kc,
                              oix.pq, oix.mp, oix.kp)
@@ -301,9 +302,6 @@ def test_search_scimi_load_sparse_and_refusals(ctx):
     with pytest.raises(S.ScError, match="load_sparse"):
         S.Index.load(ctx, oix.L, oix.pid, oix.codes, oix.dim, oix.cq, 2, oix.kc, oix.pq, oix.mp,
                      oix.kp, 2)
-    with pytest.raises(S.ScError, match="not available"):
-        ix.search(S.Dev.from_numpy(ctx, np.zeros((1, ix.dim), np.float32)), S.KIND_SCIMI, 10,
-                  10, 10, form=S.FORM_MIXED)
     lx.close()
     ix.close()
 
@@ -369,17 +367,42 @@ def test_search_sc3_matches_oracle(ctx, case):
     ix.close()
 
 
-@pytest.mark.parametrize("case", [c for c in SEARCH_CASES if c[0] != S.KIND_IMI])
+@pytest.mark.parametrize("case", SEARCH_CASES)
 @pytest.mark.parametrize("kc,kp", [(16, 64), (64, 64)])
 def test_search_mixed_form_matches_oracle(ctx, case, kc, kp, monkeypatch):
     """SC_FORM_MIXED (FP16 table scan + exact re-rank) returns the plain form's
     ids and distances bit for bit; kc = kp = 64 takes the precomputed-LUT path.
-    The pop budget is lifted so that long traversals stay in the fused kernel."""
+    The pop budget is lifted so that long traversals stay in the fused kernel.
+    The multi-index takes the staged mixed scan (per-vector dot_cr sums)."""
     monkeypatch.setenv("SC_POP_BUDGET", "1000000")
     kind, R, w, T = case
     ix, oix = _index_pair(ctx, kind, kc=kc, kp=kp)
     st = _compare_search(ctx, ix, oix, kind, R, w, T, form=S.FORM_MIXED)
-    assert st.reranked > 0 or st.candidates == 0     # the exact re-rank ran
+    if kind != S.KIND_IMI:
+        assert st.reranked > 0 or st.candidates == 0     # the exact re-rank ran
+    ix.close()
+
+
+@pytest.mark.parametrize("kind", [S.KIND_IMI, S.KIND_SCIMI])
+@pytest.mark.parametrize("mp,kp", [(8, 64), (16, 32), (4, 64)])
+def test_search_mixed_imi_scan_matches_oracle(ctx, kind, mp, kp):
+    """The mixed IMI-sequence scan (FP16 diff_qr table + per-vector dot_cr
+    sums, exact re-rank) equals the oracle's plain search for both
+    two-block kinds, mp = 8 and 16 (and mp = 4, which scans plain)."""
+    ix, oix = _index_pair(ctx, kind, kc=8 if kind == S.KIND_SCIMI else 16, mp=mp, kp=kp)
+    for R, w, T in ((100, 100000, 3000), (300, 500, 40), (10, 100000, 10 ** 6)):
+        _compare_search(ctx, ix, oix, kind, R, w, T, nq=80, form=S.FORM_MIXED)
+    ix.close()
+
+
+@pytest.mark.parametrize("kind", [S.KIND_IMI, S.KIND_SCIMI])
+def test_search_mixed_imi_overflow_redo(ctx, kind, monkeypatch):
+    """A margin widened a billion times keeps every candidate: the kept sets
+    outgrow the buffer, those queries are redone exactly, and the results do
+    not change."""
+    monkeypatch.setenv("SC_MIXED_MARGIN", "1e9")
+    ix, oix = _index_pair(ctx, kind, kc=8 if kind == S.KIND_SCIMI else 16)
+    _compare_search(ctx, ix, oix, kind, 50, 100000, 4000, nq=40, form=S.FORM_MIXED)
     ix.close()
 
 
@@ -833,11 +856,13 @@ def test_full_size_scimi_index_and_search(ctx):
     print("two halves: %d non-empty cells of %d" % (ck.size, kc ** 4))
     oix = sco.OracleIndex(dim, 2, kc, mp, kp, cq, pq, e["L"], e["pid"], e["codes"], ckeys=ck)
     q = sco.gen(sco.mix_d128(), sco.STREAM_QUERY, 0, 64, dim)
-    ids, dists, sums, st = ix.search(S.Dev.from_numpy(ctx, q), S.KIND_SCIMI, 100, 100000, 10000)
     oids, od, osum = oix.search_batch(sco.SCIMI, q, 100, 100000, 10000)
-    assert np.array_equal(sums.numpy(), osum)
-    assert np.array_equal(ids.numpy(), oids)
-    assert np.array_equal(_bits(dists.numpy()), _bits(od))
+    for form in (S.FORM_PLAIN, S.FORM_MIXED):
+        ids, dists, sums, st = ix.search(S.Dev.from_numpy(ctx, q), S.KIND_SCIMI, 100, 100000,
+                                         10000, form=form)
+        assert np.array_equal(sums.numpy(), osum)
+        assert np.array_equal(ids.numpy(), oids)
+        assert np.array_equal(_bits(dists.numpy()), _bits(od))
     ix.close()
 
 
