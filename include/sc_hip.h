@@ -144,6 +144,20 @@ int sc_encode_general(sc_ctx *ctx, const float *d_x, int64_t n, int32_t dim,
                       const float *d_cq, int32_t mc, int32_t kc, const float *d_pq,
                       int32_t mp, int32_t kp, int32_t nc, uint16_t *d_cid, uint8_t *d_codes);
 
+/* PQ codes against GIVEN coarse cells: Encoder::rencode<T>
+ * (include/encoder.h:202-309), which re-encodes the vectors of a loaded
+ * inverted file (Encoder::load_encoded_data, src/encoder.cpp:57-146) in its
+ * cell order.  Position i encodes raw row d_rows[i] of d_x [nx][dim]
+ * (d_rows NULL: row i, n <= nx) against the cells d_cid [n][mc]: float
+ * residual per coarse block, then Encoder::encode's PQ argmin (S1
+ * distance, first strict minimum).  d_codes [n][mp].  A row outside
+ * [0, nx) or a cell id >= kc returns SC_ERR_ARG (the reference reads out
+ * of bounds there).  Synchronises the context stream. */
+int sc_encode_given(sc_ctx *ctx, const float *d_x, int64_t nx, int32_t dim,
+                    const int32_t *d_rows, int64_t n, const uint16_t *d_cid,
+                    const float *d_cq, int32_t mc, int32_t kc, const float *d_pq,
+                    int32_t mp, int32_t kp, uint8_t *d_codes);
+
 /* ---- index ------------------------------------------------------------- */
 /* Builds the device-resident inverted file from encoder outputs:
  * Encoder::distribution (src/encoder.cpp:170-205) / SCEncoder::distribution

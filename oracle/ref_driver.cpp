@@ -7,7 +7,10 @@
 // vector_base (include/sc_utilities.h:156-185) and the array heaps
 // pq_insert/pq_pop, pq_insert4/pq_pop4, pq_insert6/pq_pop6
 // (include/sc_utilities.h:229-300, 469-675) and SimpleCluster::distortion
-// (lib/simple-cluster/include/k-means.h:444-464).  The reference's query classes and SCEncoder include <cblas.h>
+// (lib/simple-cluster/include/k-means.h:444-464) and the quantizers' residual
+// passes PQQuantizer / SCQuantizer::calc_residual_vector
+// (include/quantizer.h:456-510, include/sc_quantizer.h:72-133).  The
+// reference's query classes and SCEncoder include <cblas.h>
 // (OpenBLAS fork, not vendored) and are not built.
 //
 // Built by oracle/Makefile into oracle/_ref/libscref.so straight from the
@@ -18,6 +21,8 @@
 
 #include "encoder.h"
 #include "k-means.h"
+#include "quantizer.h"
+#include "sc_quantizer.h"
 #include "sc_utilities.h"
 #include "utilities.h"
 
@@ -134,6 +139,29 @@ int scref_heap6_pop(float *v, int32_t *h1, int32_t *h2, int32_t *h3, int32_t *h4
   out6[0] = a; out6[1] = b; out6[2] = c; out6[3] = d; out6[4] = e; out6[5] = f;
   *pos = p;
   return 1;
+}
+
+// calc_residual_vector of PQQuantizer<float> (second = 0) or
+// SCQuantizer<float> (second = 1), built with `part` partitions and one
+// coarse codebook read from cq_path, in place on x [n][dim].
+int scref_residual(float *x, int n, int dim, int part, const char *cq_path, int second) {
+  float *d = static_cast<float *>(::operator new(sizeof(float) * (size_t)n * dim));
+  std::memcpy(d, x, sizeof(float) * (size_t)n * dim);
+  char *names[1] = {const_cast<char *>(cq_path)};
+  if (second) {
+    SC::SCQuantizer<float> q(dim, part, 1, 0, 1, false);
+    q.load_data_mat(d, n);
+    q.load_codebooks(names, false);
+    q.calc_residual_vector(false);
+    std::memcpy(x, q.data, sizeof(float) * (size_t)n * dim);
+  } else {
+    SC::PQQuantizer<float> q(dim, part, 1, 0, 1, false);
+    q.load_data_mat(d, n);
+    q.load_codebooks(names, false);
+    q.calc_residual_vector(false);
+    std::memcpy(x, q.data, sizeof(float) * (size_t)n * dim);
+  }
+  return 0;
 }
 
 }  // extern "C"

@@ -172,6 +172,26 @@ def encode(x, cq, pq, mc, kc, mp, kp, nc):
     return cid, codes
 
 
+def encode_given(x, rows, cid, cq, pq, mc, kc, mp, kp):
+    """Encoder::rencode<T> (include/encoder.h:202-309): position i takes raw
+    row rows[i] of x, subtracts the centroid of its given cell cid[i][j] in
+    each coarse block (float subtraction, encoder.h:271-285) and is coded by
+    Encoder::encode's PQ argmin (encoder.h:287-303: S1 distance, first strict
+    minimum) -- here sco_encode over the residuals with one all-zero coarse
+    centroid (mc=1, kc=1), which leaves them unchanged (r - 0 = r)."""
+    x = np.ascontiguousarray(x, np.float32)
+    dim = x.shape[1]
+    bsc = dim // mc
+    rows = np.asarray(rows, np.int64)
+    cid = np.asarray(cid, np.int64).reshape(len(rows), mc)
+    cq = np.ascontiguousarray(cq, np.float32).reshape(mc, kc, bsc)
+    res = x[rows].copy()
+    for j in range(mc):
+        res[:, j * bsc:(j + 1) * bsc] -= cq[j][cid[:, j]]
+    _, codes = encode(res, np.zeros((1, 1, dim), np.float32), pq, 1, 1, mp, kp, 1)
+    return codes
+
+
 def encode_general(x, cq, pq, mc, kc, mp, kp, nc):
     """SCEncoder::encode (general nc): cid [n][mc*nc] (block-major)."""
     x = np.ascontiguousarray(x, np.float32)
@@ -389,6 +409,7 @@ def ref():
         r.scref_heap6_insert.argtypes = [_f32p] + [_i32p] * 6 + [C.c_float] + [C.c_int] * 6 + \
             [C.POINTER(C.c_int32), C.c_int]
         r.scref_heap6_pop.argtypes = [_f32p] + [_i32p] * 6 + [_i32p, C.POINTER(C.c_int32)]
+        r.scref_residual.argtypes = [_f32p, C.c_int, C.c_int, C.c_int, C.c_char_p, C.c_int]
         _REF = r
     return _REF
 

@@ -200,7 +200,6 @@ protected:
     float **cq;
     int k;
     int *kc, *mc;
-    int residual_which_ = 0;   // 0: nearest (PQQuantizer), 1: second (SCQuantizer)
 
 public:
     float *data;
@@ -349,33 +348,43 @@ public:
     }
 
     // quantizer.h:456-510 (nearest) / sc_quantizer.h:72-133 (second), on
-    // the GPU, in place on `data`.
-    inline void calc_residual_vector(bool verbose) {
-        (void)verbose;
+    // the GPU, in place on `data`; `which` 0 nearest, 1 second nearest.
+    // part_walk: the quantizer's `part` blocks of dim/part (SCQuantizer's
+    // walk); otherwise each coarse codebook's own mc blocks.
+    inline void residual_pass(int which, bool part_walk) {
         detail::Dev dx(data, (size_t)N * dim * sizeof(float));
         for (int k0 = 0; k0 < k; ++k0) {
             detail::Dev dc(cq[k0], (size_t)kc[k0] * dim * sizeof(float));
             detail::check(sc_residual(detail::ctx(), dx.as<float>(), N, dim, dc.as<float>(),
-                                      mc[k0], kc[k0], residual_which_),
+                                      part_walk ? part : mc[k0], kc[k0], which),
                           "sc_residual");
         }
         dx.get(data, (size_t)N * dim * sizeof(float));
     }
+    inline void calc_residual_vector(bool verbose) {
+        (void)verbose;
+        residual_pass(0, false);
+    }
 };
 
 // include/sc_quantizer.h:27-37: residuals against the second-nearest centre.
-// (The reference walks `part` blocks of dim/part there, sc_quantizer.h:98-102;
-// the drop-in uses the codebook's own mc blocks.)
+// By default this walks the reference's blocks (sc_quantizer.h:96-124): the
+// quantizer's `part` blocks of dim/part, over the coarse codebook's floats
+// read as [part][kc][dim/part] -- which is the codebook's own geometry only
+// when its mc equals part.  set_codebook_blocks(true) walks the codebook's
+// own mc blocks instead (the second-nearest centroid of each coarse block).
 template <typename DataType>
 class SCQuantizer : public PQQuantizer<DataType> {
+    bool codebook_blocks_ = false;
+
 public:
     SCQuantizer(int _dim, int _m, int _nsc, int _os, int _k, bool verbose)
-        : PQQuantizer<DataType>(_dim, _m, _nsc, _os, _k, verbose) {
-        this->residual_which_ = 1;
-    }
+        : PQQuantizer<DataType>(_dim, _m, _nsc, _os, _k, verbose) {}
     virtual ~SCQuantizer() {}
+    void set_codebook_blocks(bool on) { codebook_blocks_ = on; }
     inline void calc_residual_vector(bool verbose) {
-        PQQuantizer<DataType>::calc_residual_vector(verbose);
+        (void)verbose;
+        this->residual_pass(1, !codebook_blocks_);
     }
 };
 
@@ -528,8 +537,144 @@ public:
         write_ivf(fname);
     }
 
+    // src/encoder.cpp:57-146: reads an inverted file written by output() or
+    // output2(): per cell (all `size` of them, in cell order) its length, its
+    // ids and old_mp code bytes per vector (skipped).  Fills L with the cell
+    // lengths (not a prefix), pid with the ids in cell order and cid [N][mc]
+    // with each vector's cell digits (the cell index in base kc, most
+    // significant first).  A truncated or inconsistent file is an error here
+    // (the reference reads past the mapping).
+    void load_encoded_data(const char *filename, bool verbose) {
+        std::FILE *f = std::fopen(filename, "rb");
+        if (!f) {
+            if (verbose) std::cerr << "Cannot open the file" << std::endl;
+            std::exit(1);
+        }
+        std::vector<unsigned char> buf;
+        {
+            unsigned char tmp[1 << 16];
+            size_t got;
+            while ((got = std::fread(tmp, 1, sizeof tmp, f)) > 0) buf.insert(buf.end(), tmp, tmp + got);
+            std::fclose(f);
+        }
+        size_t at = 0;
+        auto take = [&](void *dst, size_t bytes) {
+            if (bytes > buf.size() - at) detail::die("load_encoded_data: truncated inverted file");
+            std::memcpy(dst, buf.data() + at, bytes);
+            at += bytes;
+        };
+        if (size <= 0 || config.mc <= 0 || config.kc <= 0)
+            detail::die("load_encoded_data: load the codebooks first");
+        if (old_mp < 0) detail::die("load_encoded_data: negative code length (set_mp)");
+        take(&non_empty_bucket, sizeof(int));
+        std::cout << "The number of non empty buckets: " << non_empty_bucket << std::endl;
+        take(&config.N, sizeof(int));
+        if (config.N < 0) detail::die("load_encoded_data: negative vector count");
+        ::operator delete(cid);
+        ::operator delete(L);
+        ::operator delete(pid);
+        cid = static_cast<ushort *>(::operator new((size_t)config.N * config.mc * sizeof(ushort) + 2));
+        L = static_cast<int *>(::operator new((size_t)size * sizeof(int)));
+        pid = static_cast<int *>(::operator new((size_t)config.N * sizeof(int) + 4));
+        std::cout << "Old size: " << old_mp << "[byte(s)]" << std::endl;
+        std::vector<ushort> c((size_t)config.mc);
+        size_t base = 0;
+        for (int i = 0; i < size; ++i) {
+            int l = 0;
+            take(&l, sizeof(int));
+            if (l < 0 || (size_t)l > (size_t)config.N - base)
+                detail::die("load_encoded_data: cell lengths exceed the vector count");
+            L[i] = l;
+            if (verbose) std::cout << "This bucket contains " << l << " vector(s)" << std::endl;
+            int64_t p = i;
+            for (int j = config.mc - 1; j >= 0; --j) {
+                c[(size_t)j] = (ushort)(p % config.kc);
+                p /= config.kc;
+            }
+            take(pid + base, (size_t)l * sizeof(int));
+            for (int v = 0; v < l; ++v)
+                for (int j = 0; j < config.mc; ++j) cid[(base + v) * config.mc + j] = c[(size_t)j];
+            base += (size_t)l;
+            if ((size_t)l * old_mp > buf.size() - at)
+                detail::die("load_encoded_data: truncated inverted file");
+            at += (size_t)l * old_mp;
+            if (verbose) std::cout << "Read " << l << " vector(s)" << std::endl;
+        }
+        if (base != (size_t)config.N)
+            detail::die("load_encoded_data: cell lengths do not add up to the vector count");
+        std::cout << "Read " << config.N << " data  from " << filename << std::endl;
+    }
+
+    // include/encoder.h:202-309: PQ codes of the raw vectors of `filename`
+    // in the loaded file's cell order -- position i is raw row pid[i],
+    // encoded against its loaded cell cid[i] (sc_encode_given).  The raw
+    // file must hold exactly the loaded file's N vectors.
+    template <typename DataType>
+    inline void rencode(const char *filename, int offset, bool verbose) {
+        const int nloaded = config.N;
+        float *x = nullptr;
+        config.N = detail::load_rows<DataType>(filename, x, offset, config.dim, verbose);
+        if (config.mc <= 0 || config.mp <= 0) {
+            std::cerr << "Nothing to do" << std::endl;
+            ::operator delete(x);
+            return;
+        }
+        if (!pid || !cid || config.N != nloaded) {
+            ::operator delete(x);
+            detail::die("rencode: load_encoded_data first, from a file of the same vectors");
+        }
+        const size_t n = (size_t)config.N;
+        detail::Dev dx(x, n * config.dim * sizeof(float));
+        ::operator delete(x);
+        detail::Dev drows(pid, n * sizeof(int32_t)), dcid(cid, n * config.mc * sizeof(ushort));
+        detail::Dev dcq(cq, (size_t)config.kc * config.dim * sizeof(float));
+        detail::Dev dpq(pq, (size_t)config.kp * config.dim * sizeof(float));
+        detail::Dev dcodes(n * config.mp);
+        detail::check(sc_encode_given(detail::ctx(), dx.as<float>(), (int64_t)n, config.dim,
+                                      drows.as<int32_t>(), (int64_t)n, dcid.as<uint16_t>(),
+                                      dcq.as<float>(), config.mc, config.kc, dpq.as<float>(),
+                                      config.mp, config.kp, dcodes.as<uint8_t>()),
+                      "sc_encode_given");
+        ::operator delete(codes);
+        codes = static_cast<unsigned char *>(::operator new(n * config.mp + 1));
+        dcodes.get(codes, n * config.mp);
+    }
+
+    // src/encoder.cpp:307-394 -> <path>/<prefix>_ivf.edat_ from the flat
+    // arrays load_encoded_data + rencode fill: per cell L[i], its ids and
+    // codes, then the lseek stretch byte (N*mp + (N+size+2)*4 + 1 bytes).
+    void output2(const char *db_path, const char *db_prefix, bool verbose) {
+        if (!L || !pid || !codes) detail::die("output2: nothing loaded");
+        char fname[600];
+        std::snprintf(fname, sizeof fname, "%s/%s_ivf.edat_", db_path, db_prefix);
+        FILE *f = std::fopen(fname, "wb");
+        if (!f) {
+            if (verbose) std::cerr << "Cannot open the file " << fname << std::endl;
+            std::exit(1);
+        }
+        int N = config.N;
+        std::fwrite(&non_empty_bucket, sizeof(int), 1, f);
+        std::fwrite(&N, sizeof(int), 1, f);
+        size_t base = 0;
+        for (int i = 0; i < size; ++i) {
+            const int l = L[i];
+            std::fwrite(&l, sizeof(int), 1, f);
+            std::fwrite(pid + base, sizeof(int), (size_t)l, f);
+            std::fwrite(codes + base * config.mp, 1, (size_t)l * config.mp, f);
+            base += (size_t)l;
+            if (verbose) std::cout << "Exported " << l << " vector(s)" << std::endl;
+        }
+        std::fputc(0, f);
+        const size_t f_size = (size_t)N * config.mp + (size_t)(N + size + 2) * sizeof(int);
+        std::fclose(f);
+        std::cout << "Read " << f_size << " byte(s) and wrote out " << f_size << " byte(s)"
+                  << std::endl;
+    }
+
     PQConfig get_config() { return config; }
-    void set_mp(int mp) { old_mp = config.mp; config.mp = mp; }
+    // src/encoder.cpp:396-398: the code length of the file load_encoded_data
+    // reads (config.mp, the length rencode writes, stays the codebook's).
+    void set_mp(int old) { old_mp = old; }
     void set_size(size_t s) { size = (int)s; }
     // src/encoder.cpp:152-168
     void statistic(bool detail_ = false) {
@@ -551,6 +696,8 @@ public:
     const ushort *cell_ids() const { return cid; }
     const unsigned char *pq_codes() const { return codes; }
     const std::vector<Bucket> &buckets() const { return ivf; }
+    const int *cell_lengths() const { return L; }
+    const int *ids() const { return pid; }
 };
 
 // include/sc_encoder.h:32-54, src/sc_encoder.cpp

@@ -97,6 +97,8 @@ _SIGS = {
     "sc_index_attach_raw": (C.c_int, [_P, _P, C.c_int64]),
     "sc_encode_general": (C.c_int, [_P, _P, C.c_int64, C.c_int32, _P, C.c_int32, C.c_int32, _P,
                                     C.c_int32, C.c_int32, C.c_int32, _P, _P]),
+    "sc_encode_given": (C.c_int, [_P, _P, C.c_int64, C.c_int32, _P, C.c_int64, _P, _P,
+                                  C.c_int32, C.c_int32, _P, C.c_int32, C.c_int32, _P]),
     "sc_groundtruth_nn1": (C.c_int, [_P, _P, C.c_int64, _P, C.c_int64, C.c_int32, _P, _P]),
     "sc_groundtruth_nn1_noalloc": (C.c_int, [_P, _P, C.c_int64, _P, C.c_int64, C.c_int32, _P,
                                              _P]),
@@ -402,6 +404,19 @@ def encode_general(ctx, x, cq, pq, mc, kc, mp, kp, nc):
     _check("sc_encode_general", lib().sc_encode_general(ctx.h, x.ptr, n, dim, cq.ptr, mc, kc,
                                                         pq.ptr, mp, kp, nc, cid.ptr, codes.ptr))
     return cid, codes
+
+
+def encode_given(ctx, x, cid, cq, pq, mc, kc, mp, kp, rows=None):
+    """Encoder::rencode: PQ codes (Dev [n][mp]) of raw rows `rows` (Dev int32
+    [n], or None: every row of x) against the given cells cid (Dev [n][mc])."""
+    nx, dim = x.shape
+    n = rows.shape[0] if rows is not None else nx
+    codes = Dev(ctx, (n, mp), np.uint8)
+    _check("sc_encode_given", lib().sc_encode_given(ctx.h, x.ptr, nx, dim,
+                                                    rows.ptr if rows is not None else None, n,
+                                                    cid.ptr, cq.ptr, mc, kc, pq.ptr, mp, kp,
+                                                    codes.ptr))
+    return codes
 
 
 def groundtruth_nn1(ctx, base, q):

@@ -10,6 +10,8 @@
 #include <string>
 #include <vector>
 
+#include <unistd.h>
+
 #include "../../oracle/sc_oracle.h"
 #include "../../second-candidate_amd/host/sc_dropin.h"
 
@@ -32,6 +34,29 @@ static void write_fvecs(const std::string &fn, const std::vector<float> &x, int 
     std::fclose(f);
 }
 
+// Rows of d bytes, each preceded by the int d (the .bvecs layout).
+static void write_bvecs(const std::string &fn, const std::vector<float> &x, int d) {
+    FILE *f = std::fopen(fn.c_str(), "wb");
+    size_t n = x.size() / d;
+    std::vector<unsigned char> row(d);
+    for (size_t i = 0; i < n; ++i) {
+        for (int j = 0; j < d; ++j) row[j] = (unsigned char)x[i * d + j];
+        std::fwrite(&d, 4, 1, f);
+        std::fwrite(row.data(), 1, d, f);
+    }
+    std::fclose(f);
+}
+
+static std::vector<unsigned char> slurp(const std::string &fn) {
+    std::vector<unsigned char> b;
+    FILE *f = std::fopen(fn.c_str(), "rb");
+    if (!f) return b;
+    int c;
+    while ((c = std::fgetc(f)) != EOF) b.push_back((unsigned char)c);
+    std::fclose(f);
+    return b;
+}
+
 static std::vector<float> gen(uint32_t stream, int64_t n, int d) {
     sco_mix m = {20151001u, 16, 8, 0, 24 * 256, 208 * 256, 8 * 256, 24 * 256, 255};
     std::vector<float> x((size_t)n * d);
@@ -47,6 +72,21 @@ static std::vector<float> read_ctr(const std::string &fn, int *k, int *m, int *d
     std::vector<float> v(cb, cb + (size_t)c.kc * c.dim);
     ::operator delete(cb);
     return v;
+}
+
+// The rows of an .fvecs file (4-byte header per row).
+static std::vector<float> xt_of(const std::string &fn, int d) {
+    FILE *f = std::fopen(fn.c_str(), "rb");
+    std::vector<float> x;
+    if (!f) return x;
+    std::vector<unsigned char> row(4 + (size_t)d * 4);
+    while (std::fread(row.data(), 1, row.size(), f) == row.size()) {
+        const size_t at = x.size();
+        x.resize(at + d);
+        std::memcpy(&x[at], row.data() + 4, (size_t)d * 4);
+    }
+    std::fclose(f);
+    return x;
 }
 
 // Trains cq (mc blocks, kc) and pq (8 x 256) on the residuals, like
@@ -78,7 +118,25 @@ static void train(const std::string &dir, const std::string &tag, const std::str
     std::string cpath = cfn + ".ctr_";
     char *names[1] = {&cpath[0]};
     cq->load_codebooks(names, false);
-    cq->calc_residual_vector(false);
+    // calc_residual_vector is not virtual (as in the reference): call the
+    // SCQuantizer's own, as test_sc_quantizer.cpp:51-61 does through an
+    // SCQuantizer pointer
+    if (second)
+        static_cast<SC::SCQuantizer<float> *>(cq)->calc_residual_vector(false);
+    else
+        cq->calc_residual_vector(false);
+    // residuals as the reference's quantizer computes them: PQQuantizer over
+    // the codebook's mc blocks; SCQuantizer over its 8 part blocks of dim/8
+    // (include/sc_quantizer.h:96-124), second nearest
+    {
+        int k_, m_, d_;
+        std::vector<float> cb = read_ctr(cpath, &k_, &m_, &d_);
+        std::vector<float> want(xt_of(train_fv, dim));
+        EXPECT(k_ == kc && m_ == mc && d_ == dim);
+        sco_residual(want.data(), (int64_t)want.size() / dim, dim, cb.data(), second ? 8 : mc,
+                     kc, second ? 1 : 0, 8);
+        EXPECT(std::memcmp(cq->data, want.data(), want.size() * sizeof(float)) == 0);
+    }
     cq->create_sub_quantizers(false);
     const std::string pfn = dir + "/" + tag + "_pq";
     cq->output(pfn.c_str(), false);
@@ -295,6 +353,95 @@ int main(int argc, char **argv) {
         o.pq = read_ctr(dir + "/imi_pq.ctr_", &k, &m, &d);
         oracle_index(o, xb, dim, 2, 16, 1);
         check_edat(dir + "/base_ivf.edat_", o, 20000);
+
+        // Encoder::statistic (src/encoder.cpp:152-168): ./ivf.txt lists every
+        // cell's length, then one line "id:code code ... " per vector
+        {
+            char cwd[4096];
+            EXPECT(getcwd(cwd, sizeof cwd) != nullptr);
+            EXPECT(chdir(dir.c_str()) == 0);
+            enc.statistic(false);
+            EXPECT(chdir(cwd) == 0);
+            std::string want;
+            int prev = 0;
+            char line[64];
+            for (size_t c = 0; c < o.L.size(); ++c) {
+                std::snprintf(line, sizeof line, "Bucket %zu:%d\n", c, o.L[c] - prev);
+                want += line;
+                for (int v = prev; v < o.L[c]; ++v) {
+                    std::snprintf(line, sizeof line, "%d:", o.pid[v]);
+                    want += line;
+                    for (int t = 0; t < 8; ++t) {
+                        std::snprintf(line, sizeof line, "%d ", (int)o.codes[(size_t)v * 8 + t]);
+                        want += line;
+                    }
+                    want += "\n";
+                }
+                prev = o.L[c];
+            }
+            const std::vector<unsigned char> got = slurp(dir + "/ivf.txt");
+            EXPECT(std::string(got.begin(), got.end()) == want);
+        }
+
+        // test_encoder.cpp:94-104: set_mp(8); load_encoded_data; rencode the
+        // raw vectors (as bytes) in the file's cell order; output2.
+        for (int variant = 0; variant < 2; ++variant) {
+            // variant 0: the encoded vectors themselves -> the same file;
+            // variant 1: other vectors of the same count, coded against the
+            // loaded cells (residual per block, then the PQ argmin)
+            std::vector<float> xr = xb;
+            if (variant == 1)
+                for (size_t i = 0; i < xr.size(); ++i) xr[i] = (float)(((int)xr[i] * 7 + 37) % 256);
+            write_bvecs(dir + "/base.bvecs", xr, dim);
+            SC::Encoder re;
+            re.load_codebooks((dir + "/imi_cq.ctr_").c_str(), (dir + "/imi_pq.ctr_").c_str(),
+                              false);
+            re.set_mp(8);
+            EXPECT(re.get_config().mp == 8);
+            re.load_encoded_data((dir + "/base_ivf.edat_").c_str(), false);
+            EXPECT(re.get_config().N == 20000);
+            // cells, ids and cell digits as the file holds them
+            const int kc = 16;
+            int prev = 0;
+            bool ok = true;
+            for (size_t c = 0; c < o.L.size(); ++c) {
+                ok = ok && re.cell_lengths()[c] == o.L[c] - prev;
+                for (int v = prev; v < o.L[c]; ++v)
+                    ok = ok && re.ids()[v] == o.pid[v] && re.cell_ids()[2 * v] == (int)(c / kc) &&
+                         re.cell_ids()[2 * v + 1] == (int)(c % kc);
+                prev = o.L[c];
+            }
+            EXPECT(ok);
+            re.rencode<unsigned char>((dir + "/base.bvecs").c_str(), 4, false);
+            EXPECT(re.get_config().N == 20000);
+            // expected codes: the residual against the loaded cells, coded by
+            // the oracle's encoder with one all-zero coarse centroid (x - 0 = x)
+            const int bs = dim / 2;
+            std::vector<float> res((size_t)20000 * dim), zero(dim, 0.0f);
+            for (int i = 0; i < 20000; ++i)
+                for (int j = 0; j < 2; ++j) {
+                    const int cell = re.cell_ids()[2 * i + j];
+                    for (int t = 0; t < bs; ++t)
+                        res[(size_t)i * dim + j * bs + t] =
+                            xr[(size_t)o.pid[i] * dim + j * bs + t] -
+                            o.cq[((size_t)j * kc + cell) * bs + t];
+                }
+            std::vector<uint16_t> zc(20000);
+            OracleIx o2;
+            o2.L = o.L;
+            o2.pid = o.pid;
+            o2.codes.resize((size_t)20000 * 8);
+            EXPECT(sco_encode(res.data(), 20000, dim, zero.data(), 1, 1, o.pq.data(), 8, 256, 1,
+                              zc.data(), o2.codes.data(), 8) == 0);
+            EXPECT(std::memcmp(re.pq_codes(), o2.codes.data(), o2.codes.size()) == 0);
+            re.output2(dir.c_str(), "base_r", false);
+            check_edat(dir + "/base_r_ivf.edat_", o2, 20000);
+            if (variant == 0)
+                EXPECT(slurp(dir + "/base_r_ivf.edat_") == slurp(dir + "/base_ivf.edat_"));
+            else
+                EXPECT(o2.codes != o.codes);
+        }
+
         SC::MultiQuery q;
         q.load_codebooks((dir + "/imi_cq.ctr_").c_str(), (dir + "/imi_pq.ctr_").c_str(), false);
         EXPECT(q.load_encoded_data((dir + "/base_ivf.edat_").c_str(), false) == 20000);

@@ -87,6 +87,45 @@ def test_encode_matches_oracle(ctx, case, prec):
               % (st.rechecked, st.fallback, n))
 
 
+GIVEN_CASES = [  # dim, mc, kc, mp, kp, n (raw rows), m (positions)
+    (128, 1, 256, 8, 256, 3001, 4000),
+    (128, 2, 16, 8, 256, 2000, 2000),
+    (960, 2, 32, 16, 256, 300, 500),
+    (64, 1, 300, 4, 100, 700, 1),
+]
+
+
+@pytest.mark.parametrize("case", GIVEN_CASES)
+def test_encode_given_matches_oracle(ctx, case):
+    """Encoder::rencode: codes of raw rows (permuted, repeated) against given
+    cells, bit-exact; rows=None codes every row in order; a row or cell out
+    of range is refused with the reason."""
+    dim, mc, kc, mp, kp, n, m = case
+    rng = np.random.default_rng(sum(case) + 5)
+    mix = sco.mix_d960() if dim == 960 else sco.mix_d128(ncomp=7)
+    x = sco.gen(mix, sco.STREAM_TRAIN, 0, n, dim)
+    cq, pq = _codebooks(rng, x, mc, kc, mp, kp, ties=kc > 2)
+    rows = rng.integers(0, n, m).astype(np.int32)
+    cid = rng.integers(0, kc, (m, mc)).astype(np.uint16)
+    dx, dcq, dpq = (S.Dev.from_numpy(ctx, a) for a in (x, cq, pq))
+    got = S.encode_given(ctx, dx, S.Dev.from_numpy(ctx, cid), dcq, dpq, mc, kc, mp, kp,
+                         rows=S.Dev.from_numpy(ctx, rows)).numpy()
+    assert np.array_equal(got, sco.encode_given(x, rows, cid, cq, pq, mc, kc, mp, kp))
+    own, codes, _ = S.encode(ctx, dx, dcq, dpq, mc, kc, mp, kp, 1)
+    assert np.array_equal(S.encode_given(ctx, dx, own, dcq, dpq, mc, kc, mp, kp).numpy(),
+                          codes.numpy())
+    bad_rows = rows.copy()
+    bad_rows[-1] = n
+    with pytest.raises(S.ScError, match="row index"):
+        S.encode_given(ctx, dx, S.Dev.from_numpy(ctx, cid), dcq, dpq, mc, kc, mp, kp,
+                       rows=S.Dev.from_numpy(ctx, bad_rows))
+    bad_cid = cid.copy()
+    bad_cid[0, mc - 1] = kc
+    with pytest.raises(S.ScError, match="cell id"):
+        S.encode_given(ctx, dx, S.Dev.from_numpy(ctx, bad_cid), dcq, dpq, mc, kc, mp, kp,
+                       rows=S.Dev.from_numpy(ctx, rows))
+
+
 # ------------------------------------------------------------------ index + search
 OKIND = {S.KIND_IVFADC: sco.IVFADC, S.KIND_IMI: sco.IMI, S.KIND_SC: sco.SC,
          S.KIND_SC3: sco.SC3, S.KIND_SCIMI: sco.SCIMI}

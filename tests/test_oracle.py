@@ -135,6 +135,30 @@ def test_encode_bruteforce(nc, integer):
         assert np.all(cid[:4, 0] == kc - 1) and np.all(cid[:4, mc] == 1)
 
 
+def test_encode_given_pins():
+    """Encoder::rencode's restatement: with each vector's own nearest cells
+    it reproduces Encoder::encode's codes (pinned above against brute force);
+    with permuted rows and other cells it equals brute force on the explicit
+    residual (ties planted in the PQ codebook)."""
+    rng = np.random.default_rng(77)
+    mc, kc, mp, kp, dim, n = 2, 12, 4, 16, 32, 300
+    x = sco.gen(sco.mix_d128(ncomp=5), sco.STREAM_TRAIN, 0, n, dim)
+    cq, pq = _codebooks(rng, x, mc, kc, mp, kp)
+    cid, codes = sco.encode(x, cq, pq, mc, kc, mp, kp, 1)
+    assert np.array_equal(sco.encode_given(x, np.arange(n), cid, cq, pq, mc, kc, mp, kp), codes)
+    rows = rng.integers(0, n, 500)
+    gcid = rng.integers(0, kc, (500, mc))
+    got = sco.encode_given(x, rows, gcid, cq, pq, mc, kc, mp, kp)
+    bsc, bsp = dim // mc, dim // mp
+    for i in range(500):
+        r = x[rows[i]].copy()
+        for j in range(mc):
+            r[j * bsc:(j + 1) * bsc] = r[j * bsc:(j + 1) * bsc] - cq[j, gcid[i, j]]
+        for s in range(mp):
+            d = _s1_closed(r[None, s * bsp:(s + 1) * bsp], pq[s])
+            assert got[i, s] == np.lexsort((np.arange(kp), d))[0]
+
+
 @ref_missing
 @pytest.mark.ref
 @pytest.mark.parametrize("mc,kc", [(1, 32), (2, 16)])
@@ -867,3 +891,35 @@ def test_residual_second_nearest():
     o = np.lexsort((np.tile(np.arange(10), (200, 1)), d), axis=1)
     assert np.array_equal(r0, x - cq[o[:, 0]])
     assert np.array_equal(r1, x - cq[o[:, 1]])
+
+
+@ref_missing
+@pytest.mark.ref
+@pytest.mark.parametrize("mc,part,second", [(1, 8, 1), (2, 4, 1), (1, 1, 1), (2, 8, 0),
+                                            (1, 8, 0)])
+def test_residual_matches_reference(tmp_path, mc, part, second):
+    """PQQuantizer / SCQuantizer::calc_residual_vector as the reference runs
+    them.  PQQuantizer walks the codebook's own mc blocks (which=0 with mc);
+    SCQuantizer walks the quantizer's `part` blocks of dim/part over the same
+    floats, reading the codebook as [part][kc][dim/part]
+    (include/sc_quantizer.h:96-124), and subtracts the second nearest of each
+    such block: which=1 with mc = part.  Exact ties are planted (duplicated
+    centroids, vectors equal to a centroid)."""
+    rng = np.random.default_rng(mc * 100 + part * 10 + second)
+    n, dim, kc = 700, 64, 12
+    x = sco.gen(sco.mix_d128(ncomp=4), sco.STREAM_TRAIN, 0, n, dim)
+    x[n // 2:] += rng.uniform(-0.5, 0.5, (n - n // 2, dim)).astype(np.float32)
+    bsc = dim // mc
+    cq = np.stack([x[rng.choice(n, kc, replace=False), j * bsc:(j + 1) * bsc]
+                   for j in range(mc)])                       # [mc][kc][bsc]
+    cq[:, kc - 1] = cq[:, 2]
+    flat = cq.reshape(-1)
+    x[:6] = flat[:dim]                                        # ties in the part walk too
+    path = str(tmp_path / "cq.ctr_")
+    sco.write_codebook(path, cq, kc, mc, dim)
+    want = x.copy()
+    sco.ref().scref_residual(want, n, dim, part, path.encode(), second)
+    got = sco.residual(x, flat, part if second else mc, kc, second)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    if second and part != mc:   # the codebook's own blocks give other residuals
+        assert not np.array_equal(sco.residual(x, flat, mc, kc, 1), want)
