@@ -10,11 +10,15 @@
 // exceeds p.  The whole round stays on the device.
 //
 // Lloyd.  Labels are the argmin of the float-rounded S1 distance with ties to
-// the later centroid (`min >= d`, k-means.h:602, 806); sums are FP64 (atomic
-// adds, exact for data on a fixed-point grid), counts int; an empty cluster
+// the later centroid (`min >= d`, k-means.h:602, 806); sums are FP64, each
+// accumulated over its centre's members in index order (the sequential
+// order, so the result is the same on every run and for any data), counts
+// int; an empty cluster
 // takes the point farthest from its own centroid (SINGLETON, k-means.h:
 // 845-877 with find_lonely 506-533); centres = (float)(sum / count).
 #include <float.h>
+
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include <vector>
 
@@ -170,7 +174,7 @@ __global__ void k_copy_seeds(const float *__restrict__ x, int d, const int64_t *
 template <int TV, int TK>
 __global__ __launch_bounds__((TV / 4) * (TK / 4)) void k_assign(
     const float *__restrict__ x, int64_t n, int d, const float *__restrict__ c, int k,
-    int32_t *__restrict__ labels, double *__restrict__ sums, int32_t *__restrict__ counts) {
+    int32_t *__restrict__ labels) {
     constexpr int NT = (TV / 4) * (TK / 4);
     extern __shared__ __align__(16) unsigned char smem[];
     float *xs = reinterpret_cast<float *>(smem);
@@ -186,28 +190,71 @@ __global__ __launch_bounds__((TV / 4) * (TK / 4)) void k_assign(
     }
     __syncthreads();
     tile_best<TV, TK, true, true>(xs, 0, d, c, k, cs, red, kb1, nullptr);
-    if (tid < nv) {
-        labels[v0 + tid] = kb1[tid];
-        atomicAdd(&counts[kb1[tid]], 1);
-    }
-    for (int e = tid; e < nv * d; e += NT) {
-        int v = e % nv, t = e / nv;
-        atomicAdd(&sums[(size_t)kb1[v] * d + t], (double)xs[(size_t)t * TV + v]);
-    }
+    if (tid < nv) labels[v0 + tid] = kb1[tid];
 }
 
-// The Lloyd sums and counts of given labels (the mixed form assigns first):
-// FP64 sums, exact for integer data in any order, like k_assign's.
-__global__ void k_accumulate(const float *__restrict__ x, int64_t n, int d,
-                             const int32_t *__restrict__ labels, double *__restrict__ sums,
-                             int32_t *__restrict__ counts) {
+// Deterministic Lloyd sums.  The points are ordered by label, stably (a
+// radix sort of (label, index) pairs keeps index order within a label), and
+// each (centre, dimension) sum is one FP64 accumulation over that centre's
+// members in index order: the order of a sequential pass over the points,
+// so every sum -- and every centre -- is bitwise the sequential one, with
+// no dependence on scheduling.
+__global__ void k_label_order(const int32_t *__restrict__ labels, int64_t n,
+                              int32_t *__restrict__ idx, int32_t *__restrict__ counts) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    idx[i] = (int32_t)i;
+    atomicAdd(&counts[labels[i]], 1);   // integer counts: any order is exact
+}
+
+// One thread per (centre j, dimension t); members of j are
+// order[start[j] .. start[j] + counts[j]).
+__global__ void k_member_sums(const float *__restrict__ x, int d, int k,
+                              const int32_t *__restrict__ order,
+                              const int32_t *__restrict__ start,
+                              const int32_t *__restrict__ counts, double *__restrict__ sums) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n * d) return;
-    const int64_t i = e / d;
-    const int t = (int)(e - i * d);
-    const int j = labels[i];
-    atomicAdd(&sums[(size_t)j * d + t], (double)x[e]);
-    if (t == 0) atomicAdd(&counts[j], 1);
+    if (e >= (int64_t)k * d) return;
+    const int j = (int)(e / d), t = (int)(e - (int64_t)j * d);
+    const int32_t *m = order + start[j];
+    const int cnt = counts[j];
+    double acc = 0.0;
+    int i = 0;
+    for (; i + 4 <= cnt; i += 4) {   // four independent loads, adds in order
+        const float a = x[(size_t)m[i] * d + t], b = x[(size_t)m[i + 1] * d + t];
+        const float c = x[(size_t)m[i + 2] * d + t], f = x[(size_t)m[i + 3] * d + t];
+        acc += (double)a;
+        acc += (double)b;
+        acc += (double)c;
+        acc += (double)f;
+    }
+    for (; i < cnt; ++i) acc += (double)x[(size_t)m[i] * d + t];
+    sums[e] = acc;
+}
+
+// start[j] = counts[0] + ... + counts[j-1], one workgroup.
+__global__ void k_count_starts(const int32_t *__restrict__ counts, int k,
+                               int32_t *__restrict__ start) {
+    __shared__ int32_t part[256];
+    __shared__ int32_t carry;
+    const int tid = threadIdx.x;
+    if (tid == 0) carry = 0;
+    __syncthreads();
+    for (int b = 0; b < k; b += 256) {
+        const int v = b + tid < k ? counts[b + tid] : 0;
+        part[tid] = v;
+        __syncthreads();
+        for (int o = 1; o < 256; o <<= 1) {
+            const int u = tid >= o ? part[tid - o] : 0;
+            __syncthreads();
+            part[tid] += u;
+            __syncthreads();
+        }
+        if (b + tid < k) start[b + tid] = carry + part[tid] - v;
+        __syncthreads();
+        if (tid == 255) carry += part[255];
+        __syncthreads();
+    }
 }
 
 // argmax over i of (float)S1(x_i, c[label_i]) (first index on ties):
@@ -309,12 +356,23 @@ extern "C" int sc_kmeans(sc_ctx *ctx, const float *d_x, int64_t n, int32_t d, in
         return SC_OK;
     }
     const int64_t nb = sc_cdiv(n, 256);
+    // label bits for the member sort
+    unsigned bits = 1;
+    while (bits < 31 && ((int64_t)1 << bits) < k) ++bits;
+    size_t sort_bytes = 0;
+    SC_HIP(rocprim::radix_sort_pairs(nullptr, sort_bytes, (uint32_t *)nullptr,
+                                     (uint32_t *)nullptr, (int32_t *)nullptr,
+                                     (int32_t *)nullptr, (size_t)n, 0u, bits, s));
     // scratch: d2 [n] f64, bsum [nb] u64, picks [k] i64, sums [k*d] f64,
-    // counts [k] i32, labels [n] (when the caller passes none), best u64
+    // counts [k] i32, labels [n] (when the caller passes none), best u64,
+    // member sort: idx [n], order [n], sorted labels [n], starts [k], temp
+    auto al = [](size_t o) { return (o + 255) & ~(size_t)255; };
     size_t o_d2 = 0, o_bs = o_d2 + 8 * (size_t)n, o_pk = o_bs + 8 * (size_t)nb,
            o_sum = o_pk + 8 * (size_t)k, o_cnt = o_sum + 8 * (size_t)k * d,
            o_lab = o_cnt + 4 * (size_t)k + 8, o_best = o_lab + 4 * (size_t)n + 8,
-           o_mix = (o_best + 16 + 255) & ~(size_t)255,
+           o_idx = al(o_best + 16), o_ord = al(o_idx + 4 * (size_t)n),
+           o_skey = al(o_ord + 4 * (size_t)n), o_start = al(o_skey + 4 * (size_t)n),
+           o_tmp = al(o_start + 4 * (size_t)k), o_mix = al(o_tmp + sort_bytes),
            total = o_mix + (mixed ? kmeans_bf16_scratch(k, d) : 0);
     unsigned char *S;
     SC_TRY(sc_scratch(ctx, total, (void **)&S));
@@ -324,6 +382,9 @@ extern "C" int sc_kmeans(sc_ctx *ctx, const float *d_x, int64_t n, int32_t d, in
     int32_t *counts = (int32_t *)(S + o_cnt);
     int32_t *labels = d_labels ? d_labels : (int32_t *)(S + o_lab);
     unsigned long long *best = (unsigned long long *)(S + o_best);
+    int32_t *idx = (int32_t *)(S + o_idx), *order = (int32_t *)(S + o_ord),
+            *start = (int32_t *)(S + o_start);
+    uint32_t *skey = (uint32_t *)(S + o_skey);
 
     // ---- seeding
     hipLaunchKernelGGL(k_first_seed, dim3(1), dim3(1), 0, s, n, seed, picks);
@@ -349,16 +410,23 @@ extern "C" int sc_kmeans(sc_ctx *ctx, const float *d_x, int64_t n, int32_t d, in
                                (int)lds));
     std::vector<int32_t> hcnt(k);
     for (int it = 0; it < iters; ++it) {
-        SC_HIP(hipMemsetAsync(sums, 0, 8 * (size_t)k * d, s));
         SC_HIP(hipMemsetAsync(counts, 0, 4 * (size_t)k, s));
         if (mixed) {   // BF16 MFMA distances, exact re-check: the same labels
             SC_TRY(kmeans_assign_bf16(ctx, d_x, n, d, d_centers, k, labels, S + o_mix));
-            hipLaunchKernelGGL(k_accumulate, dim3((unsigned)sc_cdiv(n * d, 256)), dim3(256), 0, s,
-                               d_x, n, d, labels, sums, counts);
         } else {
             hipLaunchKernelGGL(kern, dim3((unsigned)sc_cdiv(n, TV)), dim3((TV / 4) * (TK / 4)),
-                               lds, s, d_x, n, d, d_centers, k, labels, sums, counts);
+                               lds, s, d_x, n, d, d_centers, k, labels);
         }
+        SC_CHECK_LAUNCH();
+        hipLaunchKernelGGL(k_label_order, dim3((unsigned)nb), dim3(256), 0, s, labels, n, idx,
+                           counts);
+        SC_CHECK_LAUNCH();
+        size_t tb = sort_bytes;
+        SC_HIP(rocprim::radix_sort_pairs(S + o_tmp, tb, (const uint32_t *)labels, skey, idx,
+                                         order, (size_t)n, 0u, bits, s));
+        hipLaunchKernelGGL(k_count_starts, dim3(1), dim3(256), 0, s, counts, k, start);
+        hipLaunchKernelGGL(k_member_sums, dim3((unsigned)sc_cdiv((int64_t)k * d, 256)),
+                           dim3(256), 0, s, d_x, d, k, order, start, counts, sums);
         SC_CHECK_LAUNCH();
         SC_HIP(hipMemcpyAsync(hcnt.data(), counts, 4 * (size_t)k, hipMemcpyDeviceToHost, s));
         SC_HIP(hipStreamSynchronize(s));

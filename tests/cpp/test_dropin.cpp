@@ -5,6 +5,7 @@
 // (oracle/liboracle.so; test infrastructure).  Needs a GPU.
 //
 // usage: test_dropin <scratch dir>     exit 0 = all checks passed
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -274,6 +275,39 @@ int main(int argc, char **argv) {
             EXPECT(std::memcmp(&bres[(size_t)i * R], oid.data(), 4 * (size_t)R) == 0);
         }
         EXPECT(q.get_full_size() == 32 && q.get_size() > 0 && q.entropy(32 * 32) > 0.0);
+
+        // PQQuery::entropy (src/query.cpp:168-182) is the Shannon entropy in
+        // bits of the cell sizes: files with known size distributions give
+        // log2(8) = 3 (8 equal cells) and H(1/4, 3/4) = 2 - (3/4) log2 3.
+        for (int shape = 0; shape < 2; ++shape) {
+            const std::string efn = dir + "/shape_ivf.edat_";
+            FILE *f = std::fopen(efn.c_str(), "wb");
+            std::vector<int> len(32 * 32, 0);
+            if (shape == 0)
+                for (int c = 0; c < 8; ++c) len[(size_t)c * 97] = 2500;
+            else {
+                len[5] = 5000;
+                len[1000] = 15000;
+            }
+            int ne = shape == 0 ? 8 : 2, N = 20000, next = 0;
+            std::fwrite(&ne, 4, 1, f);
+            std::fwrite(&N, 4, 1, f);
+            std::vector<unsigned char> zc((size_t)N * 8, 0);
+            for (int c = 0; c < 32 * 32; ++c) {
+                std::fwrite(&len[(size_t)c], 4, 1, f);
+                for (int v = 0; v < len[(size_t)c]; ++v, ++next) std::fwrite(&next, 4, 1, f);
+                std::fwrite(zc.data(), 1, (size_t)len[(size_t)c] * 8, f);
+            }
+            std::fputc(0, f);
+            std::fclose(f);
+            SC::SCQuery qs;
+            qs.load_codebooks((dir + "/sc_cq.ctr_").c_str(), (dir + "/sc_pq.ctr_").c_str(),
+                              false);
+            EXPECT(qs.load_encoded_data(efn.c_str(), false) == 20000);
+            const double want = shape == 0 ? 3.0 : 2.0 - 0.75 * std::log2(3.0);
+            EXPECT(std::fabs(qs.entropy(32 * 32) - want) < 1e-12);
+            EXPECT(qs.get_size() == ne);
+        }
 
         // real_dist = true: raw vectors loaded after the index (query.h:83-86)
         q.load_data<float>((dir + "/base.fvecs").c_str(), 4, false);   // 4-byte row header

@@ -612,16 +612,23 @@ def test_kmeans_empty_cluster_repair(ctx):
     assert np.array_equal(_bits(c.numpy()), _bits(oc))
 
 
-def test_kmeans_fp64_error_on_fractional_data(ctx):
+@pytest.mark.parametrize("prec", [S.PREC_FP32, S.PREC_BF16])
+def test_kmeans_exact_on_fractional_data(ctx, prec):
+    """Fractional data (PQ codebooks train on residuals): the Lloyd sums run
+    over each centre's members in index order, the oracle's sequential order,
+    so labels and centres equal the oracle's bit for bit, and two runs agree
+    bit for bit."""
     rng = np.random.default_rng(4)
-    x = sco.gen(sco.mix_d128(ncomp=9), sco.STREAM_TRAIN, 0, 5000, 16)
+    x = sco.gen(sco.mix_d128(ncomp=9), sco.STREAM_TRAIN, 0, 20000, 16)
     x = x + rng.uniform(-0.5, 0.5, x.shape).astype(np.float32) / 3.0
-    c, lab = S.kmeans(ctx, S.Dev.from_numpy(ctx, x), 32, 3, seed=2)
-    oc, olab = sco.kmeans(x, 32, 3, seed=2)
-    # atomic FP64 sums may differ from the ordered oracle in the last bits
-    rel = np.abs(c.numpy().astype(np.float64) - oc) / np.maximum(1.0, np.abs(oc))
-    assert rel.max() <= 4e-7
-    assert np.mean(lab.numpy() == olab) >= 0.999
+    dx = S.Dev.from_numpy(ctx, x)
+    c, lab = S.kmeans(ctx, dx, 32, 6, seed=2, prec=prec)
+    c2, lab2 = S.kmeans(ctx, dx, 32, 6, seed=2, prec=prec)
+    oc, olab = sco.kmeans(x, 32, 6, seed=2)
+    assert np.array_equal(lab.numpy(), olab)
+    assert np.array_equal(_bits(c.numpy()), _bits(oc))
+    assert np.array_equal(_bits(c2.numpy()), _bits(c.numpy()))
+    assert np.array_equal(lab2.numpy(), lab.numpy())
 
 
 @pytest.mark.parametrize("which", [0, 1])
