@@ -679,6 +679,7 @@ struct ScanArgs {
     float margin_scale;     // test hook: widens the keep margin (1 = the bound)
     const uint64_t *ckeys;  // two-halves directory keys (null: dense cells)
     int64_t ncells;
+    const int32_t *qcount = nullptr;  // with qlist: workgroups at or past *qcount exit
 };
 
 struct ScanShared {
@@ -1371,6 +1372,7 @@ void k_scan_topk(ScanArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     ScanShared &S = *reinterpret_cast<ScanShared *>(smem);
     float *lut = reinterpret_cast<float *>(smem + sizeof(ScanShared));   // [mp*kp]
+    if (a.qcount && (int)blockIdx.x >= *a.qcount) return;   // the whole workgroup
     const int64_t q = a.qlist ? a.qlist[blockIdx.x] : blockIdx.x;
     scan_query(a, S, lut, q, a.dqr + (size_t)q * a.mp * a.kp, a.dqc + (size_t)q * a.mc * a.kc,
                a.qs[q * 4], a.cells + (size_t)q * a.ccap, a.clen + (size_t)q * a.ccap,
@@ -2431,11 +2433,24 @@ __global__ void k_gather_fallback(const int32_t *__restrict__ qinfo, int64_t nq,
     if (qinfo[q * 8 + 5]) list[atomicAdd(n, 1)] = (int32_t)q;
 }
 
+// Graph form of the heap fallback: flags[q] = 1 for the queries whose heap
+// outgrew the on-chip capacity (finished after the graph), counted in *n.
+__global__ void k_flag_fallback(const int32_t *__restrict__ qinfo, int64_t nq,
+                                int32_t *__restrict__ flags, int *__restrict__ n) {
+    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    const int f = qinfo[q * 8 + 5] != 0;
+    flags[q] = f;
+    if (f) atomicAdd(n, 1);
+}
+
+// skip_flagged: leave out the queries flagged for the heap fallback (their
+// statistics come from the search that finishes them).
 __global__ void k_reduce_stats(const int32_t *__restrict__ qinfo, int64_t nq,
-                               unsigned long long *__restrict__ out) {
+                               unsigned long long *__restrict__ out, int skip_flagged = 0) {
     int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     unsigned long long v[7] = {0, 0, 0, 0, 0, 0, 0};
-    if (q < nq) {
+    if (q < nq && !(skip_flagged && qinfo[q * 8 + 5])) {
         const int32_t *qi = qinfo + q * 8;
         v[0] = (unsigned long long)qi[1];
         v[1] = (unsigned long long)qi[0];
@@ -2557,9 +2572,21 @@ static int trav_order_mode() {
 
 }  // namespace
 
+// The staged search as a plan captures it in a graph: the plan's own scratch
+// (make_plan's layout), no host synchronisation and no stage events; the
+// queries whose heap outgrows the on-chip capacity are flagged (flags [nq],
+// counted in *nflag) for the plan to finish after the graph.
+struct StagedGraph {
+    unsigned char *scratch;
+    int32_t *flags;
+    int *nflag;
+    unsigned long long *stats;   // [8] statistics of the unflagged queries
+};
+
 static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int64_t nq,
                          const sc_search_params *p, const int32_t *d_T, int32_t *d_ids,
-                         float *d_dists, int32_t *d_sums, sc_search_stats *st);
+                         float *d_dists, int32_t *d_sums, sc_search_stats *st,
+                         const StagedGraph *g = nullptr);
 static int finish_flagged(sc_ctx *ctx, const sc_index *ix, const sc_search_params *p,
                           const float *d_q, const int32_t *d_T, int64_t nq, const int32_t *flags,
                           int nflag, int32_t *d_ids, float *d_dists, int32_t *d_sums,
@@ -2922,7 +2949,8 @@ static int traverse3(sc_ctx *ctx, const sc_search_params *p, int kc, const float
 
 static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int64_t nq,
                          const sc_search_params *p, const int32_t *d_T, int32_t *d_ids,
-                         float *d_dists, int32_t *d_sums, sc_search_stats *st) {
+                         float *d_dists, int32_t *d_sums, sc_search_stats *st,
+                         const StagedGraph *g) {
     const int kind = p->kind;
     const bool three = kind == SC_KIND_SC3;
     const bool halves = kind == SC_KIND_SCIMI;
@@ -2930,7 +2958,16 @@ static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
     const int scan_kind = three ? SC_KIND_SC : (halves ? SC_KIND_IMI : kind);
     const Plan P = make_plan(ix, p, nq);
     unsigned char *s;
-    SC_TRY(sc_scratch(ctx, P.total, (void **)&s));
+    if (g) {
+        if (three || halves) {
+            sc_set_error("sc_search: kind %d has no graph form", kind);
+            return SC_ERR_ARG;
+        }
+        s = g->scratch;
+        st = nullptr;
+    } else {
+        SC_TRY(sc_scratch(ctx, P.total, (void **)&s));
+    }
     float *qs = (float *)(s + P.off_qs), *dqc = (float *)(s + P.off_dqc),
           *dqr = (float *)(s + P.off_dqr), *sv = (float *)(s + P.off_sv);
     int32_t *si = (int32_t *)(s + P.off_si), *cells = (int32_t *)(s + P.off_cells);
@@ -2967,6 +3004,10 @@ static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
     }
     const bool wave = hcap_w > 0;
     const int hcap_eff = wave ? hcap_w : hcap;     // on-chip heap of the traversal used
+    if (g && hcap_eff == 0 && !three && !halves) {
+        sc_set_error("sc_search: no on-chip heap fits; no graph form");
+        return SC_ERR_TOO_LARGE;
+    }
     int64_t fallback_total = 0;
     // stage timing: five events per chunk on the context stream
     std::vector<hipEvent_t> ev;
@@ -3006,8 +3047,9 @@ static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
         const int32_t *Tq = d_T ? d_T + q0 : nullptr;      // this chunk's budgets
         TravArgs ta{kind, kc, p->w, p->T, P.ccap, sv, si, ix->L, cells, cinfo, qinfo,
                     nullptr, cn, nullptr, nullptr, nullptr, clen, Tq};
-        // workgroup order of the traversal and the scan (results do not depend on it)
-        const int order = scan_order_mode();
+        // workgroup order of the traversal and the scan (results do not depend
+        // on it); the graph form keeps query order (no sort in the graph)
+        const int order = g ? 0 : scan_order_mode();
         int32_t *perm = (int32_t *)(s + P.off_perm);
         bool ordered = false;
         auto sort_order = [&](bool from_ranks) -> int {
@@ -3069,14 +3111,29 @@ static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
             if (!ordered) SC_TRY(sort_order(false));
             sa.qlist = perm;
         }
-        SC_HIP(hipFuncSetAttribute((const void *)k_scan_topk,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        if (!g)   // a plan sets it before capture
+            SC_HIP(hipFuncSetAttribute((const void *)k_scan_topk,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         if (hcap_eff > 0 || three || halves) {
             hipLaunchKernelGGL(k_scan_topk, dim3((unsigned)cn), dim3(kScanThreads), lds,
                                ctx->stream, sa);
             SC_CHECK_LAUNCH();
         }
-        if (sa.evec) {
+        if (sa.evec && g) {
+            // graph form of the redo below: a workgroup per query of the
+            // chunk, those past the device-side count exit at once
+            int32_t *dn = list + P.chunk;
+            hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, ctx->stream, dn, 1);
+            hipLaunchKernelGGL(k_gather_redo, dim3((unsigned)sc_cdiv(cn, 256)), dim3(256), 0,
+                               ctx->stream, qinfo, cn, list, dn);
+            ScanArgs ra = sa;
+            ra.evec = nullptr;
+            ra.qlist = list;
+            ra.qcount = dn;
+            hipLaunchKernelGGL(k_scan_topk, dim3((unsigned)cn), dim3(kScanThreads), lds,
+                               ctx->stream, ra);
+            SC_CHECK_LAUNCH();
+        } else if (sa.evec) {
             // mixed IMI scan: queries whose kept set outgrew the buffer are
             // scanned again exactly (their first outputs are overwritten)
             int32_t *dn = list + P.chunk, nredo = 0;
@@ -3099,7 +3156,11 @@ static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
         // Queries whose heap outgrew the on-chip capacity are traversed again
         // with a global-memory heap of the reference's capacity and scanned
         // again; the first scan's output for them is overwritten.
-        if (!three && !halves && hcap_eff < hcap_ref) {
+        if (g && hcap_eff < hcap_ref) {
+            hipLaunchKernelGGL(k_flag_fallback, dim3((unsigned)sc_cdiv(cn, 256)), dim3(256), 0,
+                               ctx->stream, qinfo, cn, g->flags + q0, g->nflag);
+            SC_CHECK_LAUNCH();
+        } else if (!three && !halves && hcap_eff < hcap_ref) {
             int32_t nfb = 0;
             int32_t *dn = list + P.chunk;
             if (hcap_eff > 0) {
@@ -3130,6 +3191,7 @@ static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
                 hipError_t le = hipGetLastError();
                 ScanArgs fs = sa;
                 fs.qlist = hcap_eff > 0 ? list : nullptr;
+                fs.evec = nullptr;   // exact scan: the mixed form's redo does not follow
                 if (le == hipSuccess) {
                     hipLaunchKernelGGL(k_scan_topk, dim3((unsigned)nfb), dim3(kScanThreads), lds,
                                        ctx->stream, fs);
@@ -3144,9 +3206,9 @@ static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
                 fallback_total += nfb;
             }
         }
-        if (st) {
+        if (st || g) {
             hipLaunchKernelGGL(k_reduce_stats, dim3((unsigned)sc_cdiv(cn, 256)), dim3(256), 0,
-                               ctx->stream, qinfo, cn, stats);
+                               ctx->stream, qinfo, cn, g ? g->stats : stats, g ? 1 : 0);
             SC_CHECK_LAUNCH();
         }
     }
@@ -3200,7 +3262,43 @@ struct sc_plan {
     float *lut;                    // [nq][mc*kc + mp*kp] (tiled LUT kernel) or null
     hipGraph_t graph;
     hipGraphExec_t exec;
+    unsigned char *sbuf;           // staged graph: the staged kernels' scratch
 };
+
+// LDS of the staged scan kernel for this index and form.
+static size_t staged_scan_lds(const sc_index *ix, const sc_search_params *p) {
+    return sizeof(ScanShared) +
+           sizeof(float) * std::max<size_t>((size_t)ix->mp * ix->kp,
+                                            p->form == SC_FORM_REAL ? ix->dim : 0);
+}
+
+// Captures the staged kernels of the plan's batch (LUTs, coarse ranking,
+// traversal, scan, the mixed form's exact redo) in one graph over the plan's
+// own scratch.
+static int plan_capture_staged(sc_plan *P) {
+    if (P->exec) { (void)hipGraphExecDestroy(P->exec); P->exec = nullptr; }
+    if (P->graph) { (void)hipGraphDestroy(P->graph); P->graph = nullptr; }
+    SC_HIP(hipFuncSetAttribute((const void *)k_scan_topk,
+                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)staged_scan_lds(P->ix, &P->p)));
+    hipStream_t sm = P->ctx->stream;
+    SC_HIP(hipStreamBeginCapture(sm, hipStreamCaptureModeThreadLocal));
+    hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, sm, P->counter, 2);
+    hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, sm, (int *)P->stats, 16);
+    const StagedGraph g{P->sbuf, P->flags, P->counter + 1, P->stats};
+    int rc = search_staged(P->ctx, P->ix, P->d_q, P->nq, &P->p, P->d_T, P->d_ids, P->d_dists,
+                           P->d_sums, nullptr, &g);
+    hipError_t le = hipGetLastError();
+    hipError_t ce = hipStreamEndCapture(sm, &P->graph);
+    if (rc != SC_OK) return rc;
+    if (le != hipSuccess || ce != hipSuccess) {
+        sc_set_error("sc_plan: graph capture failed: %s",
+                     hipGetErrorString(le != hipSuccess ? le : ce));
+        return SC_ERR_HIP;
+    }
+    SC_HIP(hipGraphInstantiate(&P->exec, P->graph, nullptr, nullptr, 0));
+    return SC_OK;
+}
 
 static void plan_lut(sc_plan *P) {
     if (!P->lut) return;
@@ -3254,7 +3352,35 @@ static int plan_create(sc_ctx *ctx, const sc_index *ix, const sc_search_params *
     const int hcap = F.hcap, ccap = F.ccap, npow2 = F.npow2;
     P->fused = kind != SC_KIND_IMI && kind != SC_KIND_SC3 && kind != SC_KIND_SCIMI &&
                p->form != SC_FORM_REAL && Lay.total <= 150 * 1024;
-    if (!P->fused) { *out = P; return SC_OK; }
+    if (!P->fused) {
+        // the staged kernels in a graph (kinds whose staged path needs no
+        // host synchronisation); otherwise sc_plan_execute runs sc_search
+        if (kind != SC_KIND_SC3 && kind != SC_KIND_SCIMI && !getenv("SC_NO_PLAN_GRAPH")) {
+            const Plan SP = make_plan(ix, p, nq);
+            bool ok = hipMalloc(&P->counter, 256) == hipSuccess &&
+                      hipMalloc(&P->flags, 4 * (size_t)nq) == hipSuccess &&
+                      hipMalloc(&P->stats, 8 * 14) == hipSuccess &&
+                      hipMalloc(&P->sbuf, SP.total) == hipSuccess;
+            if (!ok) {
+                sc_plan_destroy(P);
+                sc_set_error("sc_plan_create: allocation failed");
+                return SC_ERR_NOMEM;
+            }
+            const int rc = plan_capture_staged(P);
+            if (rc == SC_ERR_TOO_LARGE) {   // no graph form: plain staged calls
+                for (void *b : {(void *)P->counter, (void *)P->flags, (void *)P->sbuf,
+                                (void *)P->stats})
+                    (void)hipFree(b);
+                P->counter = nullptr; P->flags = nullptr; P->sbuf = nullptr; P->stats = nullptr;
+                if (P->graph) { (void)hipGraphDestroy(P->graph); P->graph = nullptr; }
+            } else if (rc != SC_OK) {
+                sc_plan_destroy(P);
+                return rc;
+            }
+        }
+        *out = P;
+        return SC_OK;
+    }
     P->lds = Lay.total;
     bool ok = hipMalloc(&P->counter, 256) == hipSuccess &&
               hipMalloc(&P->flags, 4 * (size_t)nq) == hipSuccess &&
@@ -3316,11 +3442,41 @@ extern "C" int sc_plan_create_budgets(sc_ctx *ctx, const sc_index *ix,
 extern "C" int sc_plan_execute(sc_plan *P, sc_search_stats *st) {
     if (!P) return SC_ERR_ARG;
     SC_TRY(sc_ctx_enter(P->ctx));
-    if (!P->fused)
+    if (!P->fused && !P->exec)
         return search_impl(P->ctx, P->ix, P->d_q, P->nq, &P->p, P->d_T, P->d_ids, P->d_dists,
                            P->d_sums, st);
     if (st) *st = sc_search_stats{};
     hipStream_t sm = P->ctx->stream;
+    if (!P->fused) {   // the staged graph
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (st) {
+            SC_HIP(hipEventCreate(&e0));
+            SC_HIP(hipEventCreate(&e1));
+            SC_HIP(hipEventRecord(e0, sm));
+        }
+        SC_HIP(hipGraphLaunch(P->exec, sm));
+        int nflag = 0;
+        unsigned long long h[8] = {0};
+        if (st) SC_HIP(hipEventRecord(e1, sm));
+        SC_HIP(hipMemcpyAsync(&nflag, P->counter + 1, 4, hipMemcpyDeviceToHost, sm));
+        if (st) SC_HIP(hipMemcpyAsync(h, P->stats, 64, hipMemcpyDeviceToHost, sm));
+        SC_HIP(hipStreamSynchronize(sm));
+        if (st) {
+            float ms = 0.0f;
+            SC_HIP(hipEventElapsedTime(&ms, e0, e1));
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            st->queries = P->nq;
+            st->candidates = (int64_t)h[0];
+            st->cells = (int64_t)h[1];
+            st->visited = (int64_t)h[2];
+            st->exhausted = (int64_t)h[3];
+            st->reranked = (int64_t)h[6];
+            st->ms_scan = ms;
+        }
+        return finish_flagged(P->ctx, P->ix, &P->p, P->d_q, P->d_T, P->nq, P->flags, nflag,
+                              P->d_ids, P->d_dists, P->d_sums, st);
+    }
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (st) {
         SC_HIP(hipEventCreate(&e0));
@@ -3411,7 +3567,7 @@ extern "C" int sc_plan_destroy(sc_plan *P) {
     if (P->exec) (void)hipGraphExecDestroy(P->exec);
     if (P->graph) (void)hipGraphDestroy(P->graph);
     for (void *b : {(void *)P->counter, (void *)P->flags, (void *)P->qinfo, (void *)P->stats,
-                    (void *)P->lut})
+                    (void *)P->lut, (void *)P->sbuf})
         if (b) (void)hipFree(b);
     delete P;
     return SC_OK;

@@ -459,12 +459,13 @@ def test_search_mixed_form_overflow_redo(ctx, monkeypatch):
                                              (S.KIND_SC, 64, 64, S.FORM_PLAIN),
                                              (S.KIND_IVFADC, 64, 64, S.FORM_PLAIN),
                                              (S.KIND_IMI, 16, 64, S.FORM_PLAIN),
+                                             (S.KIND_IMI, 16, 64, S.FORM_MIXED),
                                              (S.KIND_SC, 64, 64, S.FORM_MIXED)])
 def test_search_plan_replays_match_oracle(ctx, kind, kc, kp, form):
     """A captured plan replayed several times, with eager searches and new
     query contents between the replays, and again after autotuning; every
     replay must equal the oracle (kc = kp = 64 takes the tiled LUT kernel
-    inside the graph; IMI runs the staged kernels)."""
+    inside the graph; IMI captures the staged kernels in its graph)."""
     ix, oix = _index_pair(ctx, kind, kc=kc, kp=kp)
     okind = {S.KIND_IVFADC: sco.IVFADC, S.KIND_IMI: sco.IMI, S.KIND_SC: sco.SC,
              S.KIND_SC3: sco.SC3}[kind]
@@ -491,13 +492,43 @@ def test_search_plan_replays_match_oracle(ctx, kind, kc, kp, form):
     ix.close()
 
 
-def test_search_heap_fallback_path(ctx, monkeypatch):
+@pytest.mark.parametrize("form", [S.FORM_PLAIN, S.FORM_MIXED])
+def test_search_plan_staged_graph_fallback(ctx, monkeypatch, form):
+    """IMI plan graph with a 2-entry on-chip heap: the graph flags the queries
+    whose heap outgrows it and sc_plan_execute finishes them after the graph;
+    every replay equals the oracle.  The mixed form's exact redo runs inside
+    the graph (an inflated margin makes it happen)."""
+    monkeypatch.setenv("SC_HEAP_ONCHIP", "2")
+    if form == S.FORM_MIXED:
+        monkeypatch.setenv("SC_MIXED_MARGIN", "1e9")
+    ix, oix = _index_pair(ctx, S.KIND_IMI)
+    R, w, T, nq = 50, 300, 2500, 70
+    q = sco.gen(sco.mix_d128(ncomp=11), sco.STREAM_QUERY, 0, nq, ix.dim)
+    dq = S.Dev.from_numpy(ctx, q)
+    plan = S.Plan(ix, dq, S.KIND_IMI, R, w, T, form=form)
+    oids, od, osum = oix.search_batch(sco.IMI, q, R, w, T)
+    for rep in range(3):
+        st = plan.execute(stats=True)
+        assert st.heap_fallback > 0
+        assert np.array_equal(plan.sums.numpy(), osum), rep
+        assert np.array_equal(plan.ids.numpy(), oids), rep
+        assert np.array_equal(_bits(plan.dists.numpy()), _bits(od)), rep
+        assert st.candidates == int(osum.sum())
+    plan.close()
+    ix.close()
+
+
+@pytest.mark.parametrize("form", [S.FORM_PLAIN, S.FORM_MIXED])
+def test_search_heap_fallback_path(ctx, monkeypatch, form):
     """A 2-entry on-chip heap forces every multi-sequence query onto the
-    global-memory heap of capacity w; results must not change."""
+    global-memory heap of capacity w; results must not change (the mixed
+    form with an inflated margin, so that its scans overflow as well)."""
+    if form == S.FORM_MIXED:
+        monkeypatch.setenv("SC_MIXED_MARGIN", "1e9")
     for kind in (S.KIND_IMI, S.KIND_SC):
         ix, oix = _index_pair(ctx, kind)
         monkeypatch.setenv("SC_HEAP_ONCHIP", "2")
-        st = _compare_search(ctx, ix, oix, kind, 50, 300, 2500, nq=60)
+        st = _compare_search(ctx, ix, oix, kind, 50, 300, 2500, nq=60, form=form)
         assert st.heap_fallback > 0
         monkeypatch.delenv("SC_HEAP_ONCHIP")
         ix.close()
@@ -919,7 +950,7 @@ def test_search_per_query_budgets(ctx, kind, kc, nc_ok):
     """A batch whose queries carry their own candidate budgets (drawn from a
     wide range, some above the batch maximum and some negative, which clamp)
     returns for every query what a search of that query alone with its budget
-    returns in the oracle; eager calls and a hipGraph plan (SC)."""
+    returns in the oracle; eager calls and a plan of every kind."""
     ix, oix = _index_pair(ctx, kind, kc=kc)
     nq = 90
     rng = np.random.default_rng(int(kind) + 17)
@@ -938,15 +969,15 @@ def test_search_per_query_budgets(ctx, kind, kc, nc_ok):
         assert gs[i] == st.sum, i
         assert np.array_equal(gi[i], oi), i
         assert np.array_equal(_bits(gd[i]), _bits(od)), i
-    if kind == S.KIND_SC:
-        plan = S.Plan(ix, dq, kind, 100, w, Tmax, budgets=dT)
-        plan.execute(stats=False)
-        assert np.array_equal(plan.ids.numpy(), gi) and np.array_equal(plan.sums.numpy(), gs)
-        T2 = T[::-1].copy()                        # new budgets, same buffer
-        dT.copy_from(T2)
-        plan.execute(stats=False)
-        for i in range(0, nq, 7):
-            oi, od, st, _ = oix.search(okind, q[i], 100, w, int(min(max(T2[i], 0), Tmax)))
-            assert plan.sums.numpy()[i] == st.sum and np.array_equal(plan.ids.numpy()[i], oi)
-        plan.close()
+    # plans: fused graph (SC, IVFADC), staged graph (IMI), staged calls
+    plan = S.Plan(ix, dq, kind, 100, w, Tmax, budgets=dT)
+    plan.execute(stats=False)
+    assert np.array_equal(plan.ids.numpy(), gi) and np.array_equal(plan.sums.numpy(), gs)
+    T2 = T[::-1].copy()                        # new budgets, same buffer
+    dT.copy_from(T2)
+    plan.execute(stats=False)
+    for i in range(0, nq, 7):
+        oi, od, st, _ = oix.search(okind, q[i], 100, w, int(min(max(T2[i], 0), Tmax)))
+        assert plan.sums.numpy()[i] == st.sum and np.array_equal(plan.ids.numpy()[i], oi)
+    plan.close()
     ix.close()
