@@ -858,48 +858,6 @@ __device__ __forceinline__ void load_codes(const uint8_t *codes, int vidx, uint3
     }
 }
 
-template <int MPW>
-__device__ void scan_run_packed(const ScanArgs &a, ScanShared &S, const float *lut, float d0,
-                                int rn, int rtotal, int base_pos) {
-    const int tid = threadIdx.x, NT = blockDim.x, U = kRound / NT;
-    const int kp = a.kp;
-    int cur = 0;
-    for (int p0 = 0; p0 < rtotal; p0 += kRound) {
-        const bool full = S.nb > kBufCap - kRound;   // read by all before any append
-        __syncthreads();
-        if (full) compact(S, a.R);
-        uint32_t cw[4][MPW];
-        int vx[4];
-        for (int u = 0; u < U; ++u) {
-            const int p = p0 + u * NT + tid;
-            vx[u] = -1;
-            if (p < rtotal) {
-                while (cur + 1 < rn && S.c_pref[cur + 1] <= p) ++cur;
-                vx[u] = S.c_start[cur] + (p - S.c_pref[cur]);
-                load_codes<MPW>(a.codes, vx[u], cw[u]);
-            }
-        }
-        for (int u = 0; u < U; ++u) {
-            if (vx[u] < 0) continue;
-            float d = d0;
-#pragma unroll
-            for (int w4 = 0; w4 < MPW; ++w4) {
-                const uint32_t wv = cw[u][w4];
-#pragma unroll
-                for (int b = 0; b < 4; ++b) d += lut[(w4 * 4 + b) * kp + ((wv >> (8 * b)) & 255u)];
-            }
-            const int p = p0 + u * NT + tid;
-            const uint64_t key = ((uint64_t)sc_float_key(d) << 32) | (uint32_t)(base_pos + p);
-            if (key < S.thr) {
-                const int o = atomicAdd(&S.nb, 1);
-                S.key[o] = key;
-                S.idx[o] = vx[u];
-            }
-        }
-        __syncthreads();
-    }
-}
-
 // Any mp; IMI sequences read dot_cr rows from global memory.
 __device__ void scan_run_generic(const ScanArgs &a, ScanShared &S, const float *lut,
                                  const float *dqc, float qsum, int rn, int rtotal, int base_pos) {
@@ -1037,40 +995,106 @@ __device__ uint32_t run_excl_scan(uint32_t x, uint32_t *wsum, uint32_t *total) {
     return base + incl - x;
 }
 
+// The cell of run position p (0 <= p < c_pref[rn]): the last cell c with
+// c_pref[c] <= p, which is the non-empty cell holding p.
+__device__ __forceinline__ int find_cell(const ScanShared &S, int rn, int p) {
+    int lo = 0, hi = rn - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (S.c_pref[mid] <= p) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// One round's candidates of a thread: positions p0 + u * NT + tid, their cell
+// (binary search over the run's prefix) and vector index, and their codes
+// (and, for the mixed form, e_v) loaded into registers.  A round's loads are
+// issued before the previous round is processed, so they are in flight
+// while it runs.
+template <int MPW, bool EV>
+struct RoundRegs {
+    uint32_t cw[4][MPW];
+    int vx[4], cl[4];
+    float ev[4];
+    __device__ __forceinline__ void fetch(const ScanArgs &a, const ScanShared &S, int rn,
+                                          int rtotal, int p0) {
+        const int NT = blockDim.x, U = kRound / NT;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int p = p0 + u * NT + threadIdx.x;
+            vx[u] = -1;
+            if (u < U && p < rtotal) {
+                cl[u] = find_cell(S, rn, p);
+                vx[u] = S.c_start[cl[u]] + (p - S.c_pref[cl[u]]);
+                load_codes<MPW>(a.codes, vx[u], cw[u]);
+                if (EV) ev[u] = a.evec[vx[u]];
+            }
+        }
+    }
+};
+
+// IVF / SC runs: one combined table row (diff_qr + dot_cr of the run's h3)
+// in LDS, d0 = q_sum + diff_qc[h3].
+template <int MPW>
+__device__ void scan_run_packed(const ScanArgs &a, ScanShared &S, const float *lut, float d0,
+                                int rn, int rtotal, int base_pos) {
+    const int tid = threadIdx.x, NT = blockDim.x;
+    const int kp = a.kp;
+    RoundRegs<MPW, false> cur, nxt;
+    cur.fetch(a, S, rn, rtotal, 0);
+    for (int p0 = 0; p0 < rtotal; p0 += kRound) {
+        if (p0 + kRound < rtotal) nxt.fetch(a, S, rn, rtotal, p0 + kRound);
+        const bool full = S.nb > kBufCap - kRound;   // read by all before any append
+        __syncthreads();
+        if (full) compact(S, a.R);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (cur.vx[u] < 0) continue;
+            float d = d0;
+#pragma unroll
+            for (int w4 = 0; w4 < MPW; ++w4) {
+                const uint32_t wv = cur.cw[u][w4];
+#pragma unroll
+                for (int b = 0; b < 4; ++b) d += lut[(w4 * 4 + b) * kp + ((wv >> (8 * b)) & 255u)];
+            }
+            const int p = p0 + u * NT + tid;
+            const uint64_t key = ((uint64_t)sc_float_key(d) << 32) | (uint32_t)(base_pos + p);
+            if (key < S.thr) {
+                const int o = atomicAdd(&S.nb, 1);
+                S.key[o] = key;
+                S.idx[o] = cur.vx[u];
+            }
+        }
+        __syncthreads();
+        cur = nxt;
+    }
+}
+
 // The IMI sequence over a whole run in traversal order, the rows read from
 // dot_cr in global memory (L2), the codes loaded MPW words at a time.
 template <int MPW>
 __device__ void scan_run_imi_flat(const ScanArgs &a, ScanShared &S, const float *lut,
                                   const float *dqc, float qsum, int rn, int rtotal, int base_pos) {
-    const int tid = threadIdx.x, NT = blockDim.x, U = kRound / NT;
+    const int tid = threadIdx.x, NT = blockDim.x;
     const int kp = a.kp, kc = a.kc, bs = kp * 4 * MPW / a.mc;
-    int cur = 0;
+    RoundRegs<MPW, false> cur, nxt;
+    cur.fetch(a, S, rn, rtotal, 0);
     for (int p0 = 0; p0 < rtotal; p0 += kRound) {
+        if (p0 + kRound < rtotal) nxt.fetch(a, S, rn, rtotal, p0 + kRound);
         const bool full = S.nb > kBufCap - kRound;   // read by all before any append
         __syncthreads();
         if (full) compact(S, a.R);
-        uint32_t cw[4][MPW];
-        int vx[4], cl[4];
-        for (int u = 0; u < U; ++u) {
-            const int p = p0 + u * NT + tid;
-            vx[u] = -1;
-            if (p < rtotal) {
-                while (cur + 1 < rn && S.c_pref[cur + 1] <= p) ++cur;
-                vx[u] = S.c_start[cur] + (p - S.c_pref[cur]);
-                cl[u] = cur;
-                load_codes<MPW>(a.codes, vx[u], cw[u]);
-            }
-        }
-        for (int u = 0; u < U; ++u) {
-            if (vx[u] < 0) continue;
-            const uint32_t ci = S.c_info[cl[u]];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (cur.vx[u] < 0) continue;
+            const uint32_t ci = S.c_info[cur.cl[u]];
             const int h3 = (int)(ci & 0xFFFFu), h4 = (int)(ci >> 16);
             const float *r1 = a.dot_cr + (size_t)h3 * bs;
             const float *r2 = a.dot_cr + (size_t)(kc + h4) * bs - (size_t)2 * MPW * kp;
             float d = (qsum + dqc[h3]) + dqc[kc + h4];
 #pragma unroll
             for (int w4 = 0; w4 < MPW; ++w4) {
-                const uint32_t wv = cw[u][w4];
+                const uint32_t wv = cur.cw[u][w4];
 #pragma unroll
                 for (int b = 0; b < 4; ++b) {
                     const int k = w4 * 4 + b;
@@ -1084,10 +1108,11 @@ __device__ void scan_run_imi_flat(const ScanArgs &a, ScanShared &S, const float 
             if (key < S.thr) {
                 const int o = atomicAdd(&S.nb, 1);
                 S.key[o] = key;
-                S.idx[o] = vx[u];
+                S.idx[o] = cur.vx[u];
             }
         }
         __syncthreads();
+        cur = nxt;
     }
 }
 
@@ -1104,48 +1129,38 @@ __device__ void scan_run_imi_flat(const ScanArgs &a, ScanShared &S, const float 
 template <int MPW>
 __device__ void scan_run_imi_mixed(const ScanArgs &a, ScanShared &S, const _Float16 *lut16,
                                    int rn, int rtotal, int base_pos) {
-    const int tid = threadIdx.x, NT = blockDim.x, U = kRound / NT;
+    const int tid = threadIdx.x, NT = blockDim.x;
     const int kp = a.kp;
     const float scale = S.lscale, twoE = S.twoE;
-    int cur = 0;
+    RoundRegs<MPW, true> cur, nxt;
+    cur.fetch(a, S, rn, rtotal, 0);
     for (int p0 = 0; p0 < rtotal; p0 += kRound) {
+        if (p0 + kRound < rtotal) nxt.fetch(a, S, rn, rtotal, p0 + kRound);
         const bool full = S.nb > kBufCap - kRound;   // read by all before any append
         __syncthreads();
         if (full) compact_margin(S, a.R, twoE);
-        uint32_t cw[4][MPW];
-        int vx[4], cl[4];
-        float ev[4];
-        for (int u = 0; u < U; ++u) {
-            const int p = p0 + u * NT + tid;
-            vx[u] = -1;
-            if (p < rtotal) {
-                while (cur + 1 < rn && S.c_pref[cur + 1] <= p) ++cur;
-                vx[u] = S.c_start[cur] + (p - S.c_pref[cur]);
-                cl[u] = cur;
-                load_codes<MPW>(a.codes, vx[u], cw[u]);
-                ev[u] = a.evec[vx[u]];
-            }
-        }
-        for (int u = 0; u < U; ++u) {
-            if (vx[u] < 0) continue;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (cur.vx[u] < 0) continue;
             float acc = 0.0f;
 #pragma unroll
             for (int w4 = 0; w4 < MPW; ++w4) {
-                const uint32_t wv = cw[u][w4];
+                const uint32_t wv = cur.cw[u][w4];
 #pragma unroll
                 for (int b = 0; b < 4; ++b)
                     acc += (float)lut16[(w4 * 4 + b) * kp + ((wv >> (8 * b)) & 255u)];
             }
-            const float A = (S.c_d0[cl[u]] + scale * acc) + ev[u];
+            const float A = (S.c_d0[cur.cl[u]] + scale * acc) + cur.ev[u];
             const int p = p0 + u * NT + tid;
             const uint64_t key = ((uint64_t)sc_float_key(A) << 32) | (uint32_t)(base_pos + p);
             if (key < S.thr) {
                 const int o = atomicAdd(&S.nb, 1);
                 S.key[o] = key;
-                S.idx[o] = vx[u];
+                S.idx[o] = cur.vx[u];
             }
         }
         __syncthreads();
+        cur = nxt;
     }
 }
 
