@@ -158,7 +158,8 @@ __global__ void k_group_starts(const uint32_t *__restrict__ cpair, int64_t ncell
 // 231-253 adds the same terms, interleaved with the query's).
 __global__ void k_vec_dot(const uint8_t *__restrict__ codes, const int32_t *__restrict__ L,
                           int64_t ncells, const uint64_t *__restrict__ ckeys, int64_t n, int kc,
-                          int mp, int kp, const float *__restrict__ dot_cr, float *__restrict__ evec) {
+                          int mp, int kp, const float *__restrict__ dot_cr, float *__restrict__ evec,
+                          uint32_t *__restrict__ vcell) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     int64_t lo = 0, hi = ncells;            // the cell of p: first c with L[c] > p
@@ -181,6 +182,7 @@ __global__ void k_vec_dot(const uint8_t *__restrict__ codes, const int32_t *__re
     for (int k = 0; k < half; ++k) e += dot_cr[(size_t)n0 * bs + k * kp + cc[k]];
     for (int k = half; k < mp; ++k) e += dot_cr[(size_t)(kc + n1) * bs + (k - half) * kp + cc[k]];
     evec[p] = e;
+    vcell[p] = (uint32_t)n0 | ((uint32_t)n1 << 16);
 }
 
 // out = max |x| over x[0..n) as float bits (non-negative floats order like
@@ -406,14 +408,15 @@ int finish_index(sc_index *ix) {
                        dim3(256), 0, ctx->stream, ix->dot_cr, ndot, (uint32_t *)(dmax + 2));
     SC_CHECK_LAUNCH();
     if (ix->mc == 2 && ix->n > 0) {
-        if (hipMalloc(&ix->evec, sizeof(float) * (size_t)ix->n) != hipSuccess) {
+        if (hipMalloc(&ix->evec, sizeof(float) * (size_t)ix->n) != hipSuccess ||
+            hipMalloc(&ix->vcell, sizeof(uint32_t) * (size_t)ix->n) != hipSuccess) {
             sc_set_error("index: per-vector sums allocation failed");
             return SC_ERR_NOMEM;
         }
         hipLaunchKernelGGL(k_vec_dot, dim3((unsigned)sc_cdiv(ix->n, 256)), dim3(256), 0,
                            ctx->stream, ix->codes, ix->L, ix->ncells,
                            ix->sparse ? ix->ckeys : nullptr, ix->n, ix->kc, ix->mp, ix->kp,
-                           ix->dot_cr, ix->evec);
+                           ix->dot_cr, ix->evec, ix->vcell);
         SC_CHECK_LAUNCH();
     }
     SC_HIP(hipMemcpyAsync(h, dmax, 3 * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
@@ -772,7 +775,7 @@ extern "C" int sc_index_destroy(sc_index *ix) {
     void *bufs[] = {ix->cq,    ix->pq,   ix->norm_c, ix->norm_r, ix->dot_cr,
                     ix->L,     ix->pid,  ix->codes,  ix->cqT,    ix->pqT,
                     ix->ckeys, ix->crec, ix->upairs[0], ix->upairs[1], ix->cpair,
-                    ix->gstart, ix->evec};
+                    ix->gstart, ix->evec, ix->vcell};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     delete ix;

@@ -50,6 +50,7 @@ struct sc_index {
     // sum of its dot_cr terms (the query-independent part of its IMI distance),
     // for the mixed-form scan.
     float *evec;                    // [n]
+    uint32_t *vcell;                // [n] (mc = 2): n0 | n1 << 16 of each position's cell
 };
 
 void sc_set_error(const char *fmt, ...);

@@ -680,6 +680,7 @@ struct ScanArgs {
     const uint64_t *ckeys;  // two-halves directory keys (null: dense cells)
     int64_t ncells;
     const int32_t *qcount = nullptr;  // with qlist: workgroups at or past *qcount exit
+    const uint32_t *vcell = nullptr;  // [n] n0 | n1 << 16 per position (mc = 2) or null
 };
 
 struct ScanShared {
@@ -1204,26 +1205,33 @@ __device__ void mixed_setup(const ScanArgs &a, ScanShared &S, _Float16 *lut16, c
     __syncthreads();
 }
 
-// The exact IMI distance of the vector at index position vidx: its cell is
-// the first one whose inclusive prefix L exceeds vidx, the cell's nearest
-// centroids (h3, h4) come from its number (dense: h3*kc + h4) or its key
-// (two halves), and the operation sequence is include/multi_query.h:231-253.
+// The exact IMI distance of the vector at index position vidx.  Its cell's
+// nearest centroids (h3, h4) come from the index's per-position record or,
+// without one, from the cell (the first whose inclusive prefix L exceeds
+// vidx) by its number (dense: h3*kc + h4) or its key (two halves); the
+// operation sequence is include/multi_query.h:231-253.
 __device__ __forceinline__ float imi_exact(const ScanArgs &a, const float *dqr, const float *dqc,
                                            float qsum, int vidx) {
     const int mp = a.mp, kp = a.kp, kc = a.kc, half = mp >> 1, bs = kp * mp / a.mc;
-    int64_t lo = 0, hi = a.ncells;
-    while (lo < hi) {
-        const int64_t mid = (lo + hi) >> 1;
-        if (a.L[mid] <= vidx) lo = mid + 1; else hi = mid;
-    }
     int h3, h4;
-    if (a.ckeys) {
-        const uint64_t k2 = (uint64_t)kc * (uint64_t)kc, key = a.ckeys[lo];
-        h3 = (int)(key / k2 / (uint64_t)kc);
-        h4 = (int)(key % k2 / (uint64_t)kc);
+    if (a.vcell) {                 // the index's per-position cell record
+        const uint32_t v = a.vcell[vidx];
+        h3 = (int)(v & 0xFFFFu);
+        h4 = (int)(v >> 16);
     } else {
-        h3 = (int)(lo / kc);
-        h4 = (int)(lo % kc);
+        int64_t lo = 0, hi = a.ncells;
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (a.L[mid] <= vidx) lo = mid + 1; else hi = mid;
+        }
+        if (a.ckeys) {
+            const uint64_t k2 = (uint64_t)kc * (uint64_t)kc, key = a.ckeys[lo];
+            h3 = (int)(key / k2 / (uint64_t)kc);
+            h4 = (int)(key % k2 / (uint64_t)kc);
+        } else {
+            h3 = (int)(lo / kc);
+            h4 = (int)(lo % kc);
+        }
     }
     const float *r1 = a.dot_cr + (size_t)h3 * bs, *r2 = a.dot_cr + (size_t)(h4 + kc - 1) * bs;
     const uint8_t *cc = a.codes + (size_t)vidx * mp;
@@ -3117,6 +3125,7 @@ static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
                     p->form == SC_FORM_MIXED && scan_kind == SC_KIND_IMI ? ix->evec : nullptr,
                     ix->dot_absmax, qinfo, mixed_margin_scale(),
                     ix->sparse ? ix->ckeys : nullptr, ix->ncells};
+        sa.vcell = ix->vcell;
         // LDS: the diff_qr table (or the query row of the real-distance form)
         const size_t lds =
             sizeof(ScanShared) +
