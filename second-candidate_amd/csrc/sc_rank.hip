@@ -230,20 +230,21 @@ __device__ int stop_entry(const Sc &sc, const uint64_t *lkey, int n, int w, int 
 }
 
 // Selection by score histograms (count and sizes per bucket of the score
-// key's top 12 bits, then the next 12 inside the crossing bucket): fills
-// lkey[0 .. sh.n) with every cell up to the stopping point, or returns false
-// when they do not fit the list.
-template <class Sc>
-__device__ bool hist_select(const Sc &sc, int64_t ncells, int w, int T, int cap, uint32_t *hcnt,
-                            uint32_t *hwt, uint64_t *lkey, RankShared &sh) {
+// key's top 12 bits, then the next 12 inside the crossing bucket) over the
+// cells `each` hands out (each(f) calls f(c) once per cell, spread over the
+// threads): fills lkey[0 .. sh.n) with every one of those cells up to the
+// stopping point, or returns false when they do not fit the list.
+template <class Sc, class Each>
+__device__ bool hist_select_each(const Sc &sc, Each each, int w, int T, int cap, uint32_t *hcnt,
+                                 uint32_t *hwt, uint64_t *lkey, RankShared &sh) {
     const int tid = threadIdx.x;
     for (int i = tid; i < kBuckets; i += kRankThreads) { hcnt[i] = 0; hwt[i] = 0; }
     __syncthreads();
-    for (int64_t c = tid; c < ncells; c += kRankThreads) {
+    each([&](int64_t c) {
         const uint32_t key = sc_float_key(sc.score(c));
         atomicAdd(&hcnt[key >> 20], 1u);
         atomicAdd(&hwt[key >> 20], sc.size(c));
-    }
+    });
     __syncthreads();
     const int b1 = find_bucket(hcnt, hwt, 0u, 0u, w, T, sh.wsum, &sh.b, sh.pre);
     const uint32_t pre1_cnt = sh.pre[0], pre1_w = sh.pre[1];
@@ -254,12 +255,12 @@ __device__ bool hist_select(const Sc &sc, int64_t ncells, int w, int T, int cap,
     if (need > (uint32_t)cap && b1 < kBuckets) {
         for (int i = tid; i < kBuckets; i += kRankThreads) { hcnt[i] = 0; hwt[i] = 0; }
         __syncthreads();
-        for (int64_t c = tid; c < ncells; c += kRankThreads) {
+        each([&](int64_t c) {
             const uint32_t key = sc_float_key(sc.score(c));
-            if ((int)(key >> 20) != b1) continue;
+            if ((int)(key >> 20) != b1) return;
             atomicAdd(&hcnt[(key >> 8) & 0xFFFu], 1u);
             atomicAdd(&hwt[(key >> 8) & 0xFFFu], sc.size(c));
-        }
+        });
         __syncthreads();
         const int f = find_bucket(hcnt, hwt, pre1_cnt, pre1_w, w, T, sh.wsum, &sh.b, sh.pre);
         b2 = f < kBuckets ? f : kBuckets - 1;
@@ -269,15 +270,61 @@ __device__ bool hist_select(const Sc &sc, int64_t ncells, int w, int T, int cap,
     if (need > (uint32_t)cap) return false;
     if (tid == 0) sh.n = 0;
     __syncthreads();
-    for (int64_t c = tid; c < ncells; c += kRankThreads) {
+    each([&](int64_t c) {
         const uint32_t key = sc_float_key(sc.score(c));
         const int kb1 = (int)(key >> 20);
         if (kb1 < b1 || (kb1 == b1 && (int)((key >> 8) & 0xFFFu) <= b2)) {
             const int o = atomicAdd(&sh.n, 1);
             lkey[o] = ((uint64_t)key << 32) | (uint32_t)c;
         }
-    }
+    });
     __syncthreads();
+    return true;
+}
+
+template <class Sc>
+__device__ bool hist_select(const Sc &sc, int64_t ncells, int w, int T, int cap, uint32_t *hcnt,
+                            uint32_t *hwt, uint64_t *lkey, RankShared &sh) {
+    auto all = [&](auto f) {
+        for (int64_t c = threadIdx.x; c < ncells; c += kRankThreads) f(c);
+    };
+    return hist_select_each(sc, all, w, T, cap, hcnt, hwt, lkey, sh);
+}
+
+// An upper bound of the stopping score key of the cells `each` hands out:
+// the key bits of the histogram buckets (top 12 bits, then the next 12) in
+// which the running count reaches w or the running size reaches T, the
+// remaining 8 bits set.  Returns false when those cells never reach the
+// budget.
+template <class Sc, class Each>
+__device__ bool hist_stop(const Sc &sc, Each each, int w, int T, uint32_t *hcnt, uint32_t *hwt,
+                          RankShared &sh, uint32_t *thr) {
+    const int tid = threadIdx.x;
+    for (int i = tid; i < kBuckets; i += kRankThreads) { hcnt[i] = 0; hwt[i] = 0; }
+    __syncthreads();
+    each([&](int64_t c) {
+        const uint32_t key = sc_float_key(sc.score(c));
+        atomicAdd(&hcnt[key >> 20], 1u);
+        atomicAdd(&hwt[key >> 20], sc.size(c));
+    });
+    __syncthreads();
+    const int b1 = find_bucket(hcnt, hwt, 0u, 0u, w, T, sh.wsum, &sh.b, sh.pre);
+    const uint32_t pre1_cnt = sh.pre[0], pre1_w = sh.pre[1];
+    __syncthreads();
+    if (b1 >= kBuckets) return false;
+    for (int i = tid; i < kBuckets; i += kRankThreads) { hcnt[i] = 0; hwt[i] = 0; }
+    __syncthreads();
+    each([&](int64_t c) {
+        const uint32_t key = sc_float_key(sc.score(c));
+        if ((int)(key >> 20) != b1) return;
+        atomicAdd(&hcnt[(key >> 8) & 0xFFFu], 1u);
+        atomicAdd(&hwt[(key >> 8) & 0xFFFu], sc.size(c));
+    });
+    __syncthreads();
+    const int b2 = find_bucket(hcnt, hwt, pre1_cnt, pre1_w, w, T, sh.wsum, &sh.b, sh.pre);
+    __syncthreads();
+    // the level-1 bucket reaches the budget, so some level-2 bucket in it does
+    *thr = ((uint32_t)b1 << 20) | ((uint32_t)min(b2, kBuckets - 1) << 8) | 0xFFu;
     return true;
 }
 
@@ -402,11 +449,13 @@ __global__ __launch_bounds__(kRankThreads) void k_rank_cells(RankKernelArgs k) {
 
 // Ranking from the pair tables, with a bound that skips most cells: the cells
 // of the half-0 pairs of least st0 whose cells together reach the budget form
-// a subset whose own stopping score thr bounds the full set's (a subset
-// reaches the budget no earlier); then only pairs whose least possible cell
+// a subset whose own stopping score bounds the full set's (a subset reaches
+// the budget no earlier); two histogram levels over the subset's cells give
+// that score to 8 bits, thr.  Then only pairs whose least possible cell
 // score, st0[g] + min st1, is at most thr are visited, and only cells scoring
-// at most thr are listed.  When the subset is too large for the counting
-// order or the listed cells do not fit, the histogram selection runs instead.
+// at most thr are listed; when more than the list holds, the histogram
+// selection runs over those pairs' cells, and over all cells when the subset
+// has too many pairs.
 constexpr int kMaxPairs0 = 2048;     // half-0 pairs sorted on chip for the bound
 constexpr int kMaxPass = 512;        // pairs whose cells are visited by all threads at once
 
@@ -445,11 +494,13 @@ __device__ void group_prefix(const int32_t *gsz, int32_t *gpre, int ng, RankShar
     __syncthreads();
 }
 
-// Lists of at most kSmallList cells are ordered without sorting: entry i's
+// Lists of at most kSmallEmit cells are ordered without sorting: entry i's
 // rank is the number of entries of smaller key and its position base the sum
 // of their sizes, counted over the whole list (every thread reads the same
-// entry at a time, an LDS broadcast).  The sizes sit in LDS after the keys.
+// entry at a time, an LDS broadcast).  The sizes sit in LDS after the keys
+// (room for kSmallList of them).
 constexpr int kSmallList = 1024;
+constexpr int kSmallEmit = 256;     // lists up to this long are emitted by counting
 
 template <class Sc>
 __device__ void small_ranks(const Sc &sc, const uint64_t *lkey, int n, uint32_t *lsz) {
@@ -472,31 +523,6 @@ __device__ __forceinline__ void small_rank_of(const uint64_t *lkey, const uint32
     }
     *r_out = r;
     *base_out = base;
-}
-
-// The stopping entry of a small list: the key (score bits) of the entry at
-// which the count reaches w or the size sum reaches T, in *thr; returns n when
-// the list does not reach the budget (then *thr is undefined).
-template <class Sc>
-__device__ int small_stop(const Sc &sc, uint64_t *lkey, int n, int w, int T, RankShared &sh,
-                          uint32_t *thr) {
-    uint32_t *lsz = reinterpret_cast<uint32_t *>(lkey + kSmallList);
-    small_ranks(sc, lkey, n, lsz);
-    if (threadIdx.x == 0) sh.stopkey = ~(uint64_t)0;
-    __syncthreads();
-    for (int i = threadIdx.x; i < n; i += kRankThreads) {
-        int r;
-        uint32_t base;
-        small_rank_of(lkey, lsz, n, lkey[i], &r, &base);
-        // the entry of least rank at which the budget is reached; its key is
-        // published with the rank in one 64-bit minimum (rank high, key low)
-        if (r + 1 >= w || (int64_t)base + lsz[i] >= T)
-            atomicMin(&sh.stopkey, ((uint64_t)(uint32_t)r << 32) | (uint32_t)(lkey[i] >> 32));
-    }
-    __syncthreads();
-    const uint64_t sk = sh.stopkey;
-    *thr = (uint32_t)sk;
-    return sk == ~(uint64_t)0 ? n : (int)(sk >> 32);
 }
 
 // emit_sorted for a small list: every entry's rank and position base by
@@ -615,25 +641,19 @@ __global__ __launch_bounds__(kRankThreads) void k_rank_pairs(RankKernelArgs k) {
         }
     }
     __syncthreads();
-    int e = kSmallList, nsub = 0;
+    // the subset's stopping score (to histogram resolution) bounds the full set's
     uint32_t thr = 0;
+    bool have = false;
     if (s_np <= kMaxPass) {
         const int np0 = s_np;
         group_prefix(gsz, gpre, np0, sh);
-        if (gpre[np0] <= kSmallList) {
-            for_group_cells(gbeg, gpre, np0, [&](int, int64_t c) {
-                const int o = atomicAdd(&sh.n, 1);
-                lkey[o] = ((uint64_t)sc_float_key(sc.score(c)) << 32) | (uint32_t)c;
-            });
-            __syncthreads();
-            nsub = sh.n;
-            e = small_stop(sc, lkey, nsub, w, T, sh, &thr);
-        }
+        auto sub = [&](auto f) {
+            for_group_cells(gbeg, gpre, np0, [&](int, int64_t c) { f(c); });
+        };
+        have = hist_stop(sc, sub, w, T, hcnt, hwt, sh, &thr);
     }
     __syncthreads();
-    bool done = false;
-    if (e < nsub) {
-        __syncthreads();
+    if (have) {
         if (tid == 0) { sh.n = 0; s_np = 0; }
         __syncthreads();
         // the pairs that can hold a cell scoring at most thr
@@ -647,29 +667,40 @@ __global__ __launch_bounds__(kRankThreads) void k_rank_pairs(RankKernelArgs k) {
         }
         __syncthreads();
         const int np = s_np;
-        auto visit = [&](int64_t c) {
+        if (np <= kMaxPass) group_prefix(gsz, gpre, np, sh);
+        auto cand = [&](auto f) {
+            if (np <= kMaxPass) {
+                for_group_cells(gbeg, gpre, np, [&](int, int64_t c) { f(c); });
+            } else {
+                for (int g = tid; g < nu0; g += kRankThreads) {
+                    if (sc_float_key(st0[g] + s1min) > thr) continue;
+                    for (int64_t c = k.gstart[g]; c < k.gstart[g + 1]; ++c) f(c);
+                }
+            }
+        };
+        cand([&](int64_t c) {
             const uint32_t key = sc_float_key(sc.score(c));
             if (key <= thr) {
                 const int o = atomicAdd(&sh.n, 1);
                 if (o < k.cap) lkey[o] = ((uint64_t)key << 32) | (uint32_t)c;
             }
-        };
-        if (np <= kMaxPass) {
-            group_prefix(gsz, gpre, np, sh);
-            for_group_cells(gbeg, gpre, np, [&](int, int64_t c) { visit(c); });
-        } else {
-            for (int g = tid; g < nu0; g += kRankThreads) {
-                if (sc_float_key(st0[g] + s1min) > thr) continue;
-                for (int64_t c = k.gstart[g]; c < k.gstart[g + 1]; ++c) visit(c);
-            }
-        }
+        });
         __syncthreads();
-        done = sh.n <= k.cap;
-    }
-    __syncthreads();
-    if (done) {
-        if (sh.n <= kSmallList) emit_small(sc, k, q, lkey, sh.n, w, T, sh);
-        else emit_sorted(sc, k, q, lkey, sh.n, w, T, sh);
+        const int n = sh.n;
+        if (n <= k.cap) {
+            if (n <= kSmallEmit) emit_small(sc, k, q, lkey, n, w, T, sh);
+            else emit_sorted(sc, k, q, lkey, n, w, T, sh);
+            return;
+        }
+        // more cells score at most thr than the list holds: the histogram
+        // selection over the candidate pairs' cells (every cell up to the
+        // stopping point is among them)
+        __syncthreads();
+        if (!hist_select_each(sc, cand, w, T, k.cap, hcnt, hwt, lkey, sh)) {
+            flag_overflow(k, q);
+            return;
+        }
+        emit_sorted(sc, k, q, lkey, sh.n, w, T, sh);
         return;
     }
     if (!hist_select(sc, k.ncells, w, T, k.cap, hcnt, hwt, lkey, sh)) {
