@@ -328,6 +328,121 @@ __device__ bool hist_stop(const Sc &sc, Each each, int w, int T, uint32_t *hcnt,
     return true;
 }
 
+// The score key of the stopping entry of the list lkey[0 .. n) (unsorted):
+// the key at which, in (score, cell) order, the count reaches w or the size
+// sum reaches T.  A radix select weighted by count and size, 8 key bits a
+// pass, from the highest bit in which the list's keys differ.  Returns false
+// when the list never reaches the budget.  All threads call.
+template <class Sc>
+__device__ bool list_stop(const Sc &sc, const uint64_t *lkey, int n, int w, int T, RankShared &sh,
+                          uint32_t *stop) {
+    __shared__ uint32_t hc[256], hw[256], s_and, s_or;
+    const int tid = threadIdx.x;
+    uint32_t a = ~0u, o = 0u;
+    for (int i = tid; i < n; i += kRankThreads) {
+        const uint32_t key = (uint32_t)(lkey[i] >> 32);
+        a &= key;
+        o |= key;
+    }
+    for (int m = 32; m > 0; m >>= 1) {
+        a &= (uint32_t)__shfl_xor(a, m);
+        o |= (uint32_t)__shfl_xor(o, m);
+    }
+    if (tid == 0) { s_and = ~0u; s_or = 0u; }
+    __syncthreads();
+    if ((tid & 63) == 0) { atomicAnd(&s_and, a); atomicOr(&s_or, o); }
+    __syncthreads();
+    const uint32_t diff = s_and ^ s_or;
+    uint32_t prefix = s_and & ~diff;      // the bits every key shares
+    if (diff == 0) {                      // one key value: stop there if reached
+        uint32_t cnt = 0, sz = 0;
+        for (int i = tid; i < n; i += kRankThreads) {
+            cnt += 1;
+            sz += sc.size((int64_t)(lkey[i] & 0xFFFFFFFFu));
+        }
+        uint32_t tc, ts;
+        (void)block_excl_scan(cnt, sh.wsum, &tc);
+        (void)block_excl_scan(sz, sh.wsum + 4, &ts);
+        *stop = prefix;
+        return (int64_t)tc >= w || (int64_t)ts >= T;
+    }
+    const int hb = 31 - __builtin_clz(diff);   // highest differing bit
+    uint32_t pre_cnt = 0, pre_w = 0;
+    for (int shift = (hb / 8) * 8; shift >= 0; shift -= 8) {
+        hc[tid] = 0;
+        hw[tid] = 0;
+        __syncthreads();
+        const uint32_t hi_mask = shift + 8 >= 32 ? 0u : ~0u << (shift + 8);
+        for (int i = tid; i < n; i += kRankThreads) {
+            const uint32_t key = (uint32_t)(lkey[i] >> 32);
+            if ((key & hi_mask) != (prefix & hi_mask)) continue;
+            const uint32_t b = (key >> shift) & 255u;
+            atomicAdd(&hc[b], 1u);
+            atomicAdd(&hw[b], sc.size((int64_t)(lkey[i] & 0xFFFFFFFFu)));
+        }
+        __syncthreads();
+        if (tid < 64) {   // wave 0: the first bin where the running totals reach the budget
+            uint32_t c4[4], w4[4], tc = 0, tw = 0;
+            for (int j = 0; j < 4; ++j) {
+                c4[j] = hc[4 * tid + j];
+                w4[j] = hw[4 * tid + j];
+                tc += c4[j];
+                tw += w4[j];
+            }
+            uint32_t ic = tc, iw = tw;
+            for (int m = 1; m < 64; m <<= 1) {
+                const uint32_t uc = __shfl_up(ic, m), uw = __shfl_up(iw, m);
+                if (tid >= m) { ic += uc; iw += uw; }
+            }
+            uint32_t cc = pre_cnt + ic - tc, cw = pre_w + iw - tw;
+            int found = -1;
+            uint32_t fc = 0, fw = 0;
+            for (int j = 0; j < 4; ++j) {
+                if (found < 0 && ((int64_t)(cc + c4[j]) >= w || (int64_t)(cw + w4[j]) >= T)) {
+                    found = 4 * tid + j;
+                    fc = cc;
+                    fw = cw;
+                }
+                cc += c4[j];
+                cw += w4[j];
+            }
+            // the lowest lane that found a bin publishes it
+            const unsigned long long fl = __ballot(found >= 0);
+            if (tid == 0) sh.b = -1;
+            if (fl != 0 && tid == __ffsll((long long)fl) - 1) {
+                sh.b = found;
+                sh.pre[0] = fc;
+                sh.pre[1] = fw;
+            }
+        }
+        __syncthreads();
+        const int b = sh.b;
+        if (b < 0) return false;
+        prefix = (prefix & hi_mask) | ((uint32_t)b << shift);
+        pre_cnt = sh.pre[0];
+        pre_w = sh.pre[1];
+        __syncthreads();
+    }
+    *stop = prefix;
+    return true;
+}
+
+// Moves the entries of score key at most `stop` to the front of lkey
+// (n <= kCap); returns their number.  All threads call.
+__device__ int list_keep(uint64_t *lkey, int n, uint32_t stop, RankShared &sh) {
+    const int tid = threadIdx.x;
+    constexpr int kPer = kCap / kRankThreads;
+    uint64_t mine[kPer];
+    int m = 0;
+    for (int i = tid; i < n; i += kRankThreads)
+        if ((uint32_t)(lkey[i] >> 32) <= stop) mine[m++] = lkey[i];
+    if (tid == 0) sh.n = 0;
+    __syncthreads();
+    for (int j = 0; j < m; ++j) lkey[atomicAdd(&sh.n, 1)] = mine[j];
+    __syncthreads();
+    return sh.n;
+}
+
 // Sorts lkey[0 .. n) and writes the taken prefix to the query's cell list.
 template <class Sc>
 __device__ void emit_sorted(const Sc &sc, const RankKernelArgs &k, int64_t q, uint64_t *lkey, int n,
@@ -686,8 +801,12 @@ __global__ __launch_bounds__(kRankThreads) void k_rank_pairs(RankKernelArgs k) {
             }
         });
         __syncthreads();
-        const int n = sh.n;
+        int n = sh.n;
         if (n <= k.cap) {
+            // only the entries up to the exact stopping key are ordered
+            uint32_t stop;
+            if (n > kSmallEmit && n <= kCap && list_stop(sc, lkey, n, w, T, sh, &stop))
+                n = list_keep(lkey, n, stop, sh);
             if (n <= kSmallEmit) emit_small(sc, k, q, lkey, n, w, T, sh);
             else emit_sorted(sc, k, q, lkey, n, w, T, sh);
             return;
