@@ -38,6 +38,13 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
+def recall_at(gt, ids, R):
+    """Fraction of queries whose true nearest neighbour gt[i] is among their
+    first R result ids (Evaluation::calc_recall, include/evaluation.h:60-92)."""
+    ids = np.asarray(ids)[:, :R]
+    return float(np.mean(np.any(ids == np.asarray(gt)[:, None], axis=1)))
+
+
 def lib_sha16():
     import sc_amd
     h = hashlib.sha256()
@@ -97,6 +104,8 @@ def main():
     ap.add_argument("--R", type=int, default=100)
     ap.add_argument("--w", type=int, default=100_000)
     ap.add_argument("--iters", type=int, default=10, help="Lloyd iterations (setup)")
+    ap.add_argument("--ntrain960", type=int, default=500_000,
+                    help="training vectors of the D=960 side line")
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="target CPU work of the oracle baseline (0 = skip)")
     ap.add_argument("--recall-sample", type=int, default=100,
@@ -131,7 +140,6 @@ def main():
         sys.exit(2)
 
     import sc_amd as S
-    import sco
 
     dim, mp, kp, kc = 128, 8, 256, args.kc
     ctx = S.Ctx(0)
@@ -198,23 +206,17 @@ def main():
                      np.array_equal(res["plain"]["sums"], res["mixed"]["sums"]))
     cand = res["plain"]["cand"]
 
-    # ---------------- recall@100: exact nearest neighbours on the GPU, checked
-    # against the oracle's brute force on a sample of the queries
+    # ---------------- recall@100: exact nearest neighbours on the GPU (checked
+    # against the oracle's brute force on a sample, in the CPU-baseline leg)
     qh = q.numpy()
     recall, gt, gt_info = None, None, None
     if args.recall_sample > 0:
         t0 = time.perf_counter()
         gt = S.groundtruth_nn1(ctx, x, q)[0].numpy()
         t_gt = time.perf_counter() - t0
-        nchk = min(args.recall_sample, args.nq)
-        xh = x.numpy()
-        gt_ok = bool(np.array_equal(gt[:nchk], sco.bruteforce_nn1(xh, qh[:nchk])))
-        del xh
-        recall = sco.recall(gt, res["plain"]["ids"], args.R)
-        gt_info = {"queries": int(args.nq), "seconds": round(t_gt, 4),
-                   "oracle_checked_queries": int(nchk), "matches_oracle": gt_ok}
-        log("recall@%d: %.4f (ground truth %.3fs, oracle agrees on %d: %s)"
-            % (args.R, recall, t_gt, nchk, gt_ok))
+        recall = recall_at(gt, res["plain"]["ids"], args.R)
+        gt_info = {"queries": int(args.nq), "seconds": round(t_gt, 4)}
+        log("recall@%d: %.4f (ground truth %.3fs)" % (args.R, recall, t_gt))
 
     # ---------------- QPS / recall over T (both forms)
     curve = []
@@ -227,7 +229,7 @@ def main():
                                              ids=ids, stats=False), ctx, 3, 1) * 1e3
                 row["qps_" + name] = round(args.nq / (ms * 1e-3), 1)
                 if form == S.FORM_PLAIN and gt is not None:
-                    row["recall_at_%d" % args.R] = round(sco.recall(gt, ids.numpy(), args.R), 4)
+                    row["recall_at_%d" % args.R] = round(recall_at(gt, ids.numpy(), args.R), 4)
             curve.append(row)
             log("T=%d: %s" % (tv, row))
 
@@ -259,6 +261,7 @@ def main():
     # ---------------- CPU baseline: the oracle on a bounded sample
     cpu = None
     if args.cpu_seconds > 0:
+        import sco   # the oracle: the CPU baseline and the checker of the GPU ground truth
         e = ix.export()
         ck = ix.cell_keys()
         oix = sco.OracleIndex(dim, 2, kc, mp, kp, cq, pq, e["L"], e["pid"], e["codes"], ckeys=ck)
@@ -285,11 +288,17 @@ def main():
                "ids_match_gpu": bool(match and match1)}
         log("cpu oracle: %.1f QPS on %d threads, %.1f QPS on 1 thread (%s); ids match %s"
             % (v_all, threads, v_one, cpu["cpu_model"], cpu["ids_match_gpu"]))
+        if gt is not None:   # the GPU ground truth, checked by the oracle's brute force
+            nchk = min(args.recall_sample, args.nq)
+            gt_ok = bool(np.array_equal(gt[:nchk], sco.bruteforce_nn1(x.numpy(), qh[:nchk])))
+            gt_info.update({"oracle_checked_queries": int(nchk), "matches_oracle": gt_ok})
+            cpu["ground_truth_checked"] = gt_ok
+            log("ground truth: oracle agrees on %d queries: %s" % (nchk, gt_ok))
 
     # ---------------- side lines
     side = {}
     if args.side:
-        side = side_lines(S, sco, ctx, mix, x, q, gt, args, dim, mp, kp)
+        side = side_lines(S, ctx, mix, x, q, gt, args, dim, mp, kp)
 
     encode_flops = 2.0 * args.n * (kc * dim + mp * kp * (dim // mp))
     out = {
@@ -341,7 +350,7 @@ def main():
     ctx.close()
 
 
-def side_lines(S, sco, ctx, mix, x, q, gt, args, dim, mp, kp):
+def side_lines(S, ctx, mix, x, q, gt, args, dim, mp, kp):
     """The other index families on the same base and queries."""
     out = {}
     nq = args.nq
@@ -372,7 +381,7 @@ def side_lines(S, sco, ctx, mix, x, q, gt, args, dim, mp, kp):
         if form == S.FORM_PLAIN:
             ids1 = pl.ids.numpy()
             if gt is not None:
-                row["recall_at_%d" % args.R] = round(sco.recall(gt, ids1, args.R), 4)
+                row["recall_at_%d" % args.R] = round(recall_at(gt, ids1, args.R), 4)
         else:
             row["identical"] = bool(np.array_equal(ids1, pl.ids.numpy()))
         pl.close()
@@ -397,7 +406,7 @@ def side_lines(S, sco, ctx, mix, x, q, gt, args, dim, mp, kp):
         if form == S.FORM_PLAIN:
             ref_ids = ids.numpy()
             if gt is not None:
-                row["recall_at_%d" % args.R] = round(sco.recall(gt, ref_ids, args.R), 4)
+                row["recall_at_%d" % args.R] = round(recall_at(gt, ref_ids, args.R), 4)
             st = iix.search(q, S.KIND_IMI, args.R, args.w, args.T, stats=True)[3]
             row["stage_ms"] = {"lut": round(st.ms_lut, 3), "rank": round(st.ms_rank, 3),
                                "traverse": round(st.ms_traverse, 3), "scan": round(st.ms_scan, 3)}
@@ -424,7 +433,7 @@ def side_lines(S, sco, ctx, mix, x, q, gt, args, dim, mp, kp):
     out["real_distance_two_halves"] = {
         "qps": round(nq / (ms * 1e-3), 1), "ms_per_step": round(ms, 3),
         "raw_gb_per_s": round(rcand * (dim * 4 + 4) / (ms * 1e-3) / 1e9, 1),
-        "recall_at_%d" % args.R: round(sco.recall(gt, ids.numpy(), args.R), 4) if gt is not None else None}
+        "recall_at_%d" % args.R: round(recall_at(gt, ids.numpy(), args.R), 4) if gt is not None else None}
     log("real distances: %s" % out["real_distance_two_halves"])
     ix.close()
 
@@ -451,7 +460,81 @@ def side_lines(S, sco, ctx, mix, x, q, gt, args, dim, mp, kp):
                            "lloyd_bf16_s": round(t_mx - t_seed, 4), "identical": same,
                            "lloyd_bf16_algorithmic_tflops": round(flops / max(t_mx - t_seed, 1e-9) / 1e12, 2)}
     log("k-means: %s" % out["kmeans_k4096"])
+    del xh64
+    out["imi_d960_budgets"] = d960_line(S, ctx, args)
     return out
+
+
+def d960_line(S, ctx, args):
+    """High-dimensional batched search: D=960 (32-component mixture on [0,1]),
+    IMI with two 480-dim halves of K=1024, PQ 16x256, 1M base, 20k queries
+    with per-query candidate budgets drawn from U[1k, 30k], each form run as
+    one replayed hipGraph plan (the staged kernels captured with their
+    budgets), recall@100 on a 2000-query sample."""
+    dim, kc, mp, kp, n, nq, R = 960, 1024, 16, 256, 1_000_000, 20_000, 100
+    mix = S.mix_d960()
+    t0 = time.perf_counter()
+    xt = S.gen_rows(ctx, mix, S.STREAM_TRAIN, 0, args.ntrain960, dim)
+    th = xt.numpy()
+    cq = np.stack([S.kmeans(ctx, S.Dev.from_numpy(ctx, np.ascontiguousarray(th[:, h * 480:(h + 1) * 480])),
+                            kc, args.iters, seed=41 + h, prec=S.PREC_BF16,
+                            want_labels=False)[0].numpy() for h in range(2)])
+    dcq = S.Dev.from_numpy(ctx, cq)
+    S.residual(ctx, xt, dcq, 2, kc, 0)
+    rh = xt.numpy()
+    pq = np.stack([S.kmeans(ctx, S.Dev.from_numpy(ctx, np.ascontiguousarray(rh[:, s * 60:(s + 1) * 60])),
+                            kp, args.iters, seed=51 + s, want_labels=False)[0].numpy()
+                   for s in range(mp)])
+    dpq = S.Dev.from_numpy(ctx, pq)
+    del xt, th, rh
+    ctx.sync()
+    t_train = time.perf_counter() - t0
+    x = S.gen_rows(ctx, mix, S.STREAM_BASE, 0, n, dim)
+    t0 = time.perf_counter()
+    cid, codes, _ = S.encode(ctx, x, dcq, dpq, 2, kc, mp, kp, 1)   # 60-dim PQ blocks: exact form
+    ctx.sync()
+    t_enc = time.perf_counter() - t0
+    ix = S.Index.build(ctx, cid, codes, dim, dcq, 2, kc, dpq, mp, kp, 1)
+    del cid, codes
+    q = S.gen_rows(ctx, mix, S.STREAM_QUERY, 0, nq, dim)
+    T = np.random.default_rng(960).integers(1000, 30001, nq).astype(np.int32)
+    dT = S.Dev.from_numpy(ctx, T)
+    w = 200_000
+    row = {"workload": "D=960 IMI, two 480-dim halves with K=1024 each, PQ 16x256, 1M base, "
+                       "20k queries with per-query budgets U[1k,30k], top-100, one hipGraph "
+                       "plan per form",
+           "train_s": round(t_train, 2), "train_vectors": args.ntrain960,
+           "encode_s_fp32_exact": round(t_enc, 3)}
+    ids = {}
+    for form, name in ((S.FORM_PLAIN, "plain"), (S.FORM_MIXED, "mixed")):
+        pl = S.Plan(ix, q, S.KIND_IMI, R, w, 30000, form=form, budgets=dT)
+        ms = timed(lambda: pl.execute(stats=False), ctx, args.steps, args.warmup) * 1e3
+        row["qps_" + name] = round(nq / (ms * 1e-3), 1)
+        row["ms_" + name] = round(ms, 3)
+        ids[name] = (pl.ids.numpy().copy(), pl.sums.numpy().copy())
+        pl.close()
+        st = ix.search(q, S.KIND_IMI, R, w, 30000, form=form, budgets=dT, stats=True)[3]
+        cand = float(ids[name][1].astype(np.int64).sum())
+        algo = cand * (mp + 4) + nq * dim * 4 + nq * R * 8
+        row["stage_ms_" + name] = {"lut": round(st.ms_lut, 3), "rank": round(st.ms_rank, 3),
+                                   "traverse": round(st.ms_traverse, 3),
+                                   "scan": round(st.ms_scan, 3)}
+        row["roofline_scan_" + name] = {
+            "bound": "hbm", "achieved": round(algo / (st.ms_scan * 1e-3) / 1e9, 1),
+            "peak": 8000.0, "unit": "GB/s",
+            "frac": round(algo / (st.ms_scan * 1e-3) / 8e12, 4),
+            "algorithmic_bytes_per_launch": int(algo)}
+    row["candidates_per_query"] = round(float(ids["plain"][1].mean()), 1)
+    row["mixed_over_plain"] = round(row["qps_mixed"] / row["qps_plain"], 3)
+    row["identical"] = bool(np.array_equal(ids["plain"][0], ids["mixed"][0]) and
+                            np.array_equal(ids["plain"][1], ids["mixed"][1]))
+    gs = 2000
+    gt = S.groundtruth_nn1(ctx, x, S.Dev.from_numpy(ctx, q.numpy()[:gs]))[0].numpy()
+    row["recall_at_100_sample"] = round(recall_at(gt, ids["mixed"][0][:gs], R), 4)
+    ix.close()
+    del x, q
+    log("D=960 IMI: %s" % row)
+    return row
 
 
 if __name__ == "__main__":

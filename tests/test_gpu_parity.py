@@ -864,6 +864,61 @@ def test_d960_imi_search(ctx):
     ix.close()
 
 
+def test_d960_imi_full_size_budget_plans(ctx):
+    """High-dimensional batched search at its full base size: D=960, IMI with
+    two 480-dim halves of K=1024, PQ 16x256, 1M base vectors.  Encode on
+    sampled rows vs the oracle; 2000 queries with budgets drawn from
+    U[1k, 30k] run as one hipGraph plan per form (identical ids, sums and
+    distance bits between the forms), and 24 sampled queries equal the
+    oracle's search with their own budget."""
+    n, dim, kc, mp, kp, R, w = 1_000_000, 960, 1024, 16, 256, 100, 200_000
+    mix = S.mix_d960()
+    xt = S.gen_rows(ctx, mix, S.STREAM_TRAIN, 0, 30_000, dim)
+    th = xt.numpy()
+    cq = np.stack([S.kmeans(ctx, S.Dev.from_numpy(ctx, np.ascontiguousarray(th[:, h * 480:(h + 1) * 480])),
+                            kc, 2, seed=41 + h, prec=S.PREC_BF16, want_labels=False)[0].numpy()
+                   for h in range(2)])
+    dcq = S.Dev.from_numpy(ctx, cq)
+    S.residual(ctx, xt, dcq, 2, kc, 0)
+    rh = xt.numpy()
+    pq = np.stack([S.kmeans(ctx, S.Dev.from_numpy(ctx, np.ascontiguousarray(rh[:, s * 60:(s + 1) * 60])),
+                            kp, 2, seed=51 + s, want_labels=False)[0].numpy() for s in range(mp)])
+    dpq = S.Dev.from_numpy(ctx, pq)
+    x = S.gen_rows(ctx, mix, S.STREAM_BASE, 0, n, dim)
+    cid, codes, _ = S.encode(ctx, x, dcq, dpq, 2, kc, mp, kp, 1)
+    rows = np.random.default_rng(7).choice(n, 200, replace=False)
+    xr = np.concatenate([sco.gen(sco.mix_d960(), sco.STREAM_BASE, int(r), 1, dim) for r in rows])
+    ocid, ocodes = sco.encode(xr, cq, pq, 2, kc, mp, kp, 1)
+    assert np.array_equal(cid.numpy()[rows], ocid) and np.array_equal(codes.numpy()[rows], ocodes)
+    del x
+    ix = S.Index.build(ctx, cid, codes, dim, dcq, 2, kc, dpq, mp, kp, 1)
+    del cid, codes
+    nq = 2000
+    q = S.gen_rows(ctx, mix, S.STREAM_QUERY, 0, nq, dim)
+    T = np.random.default_rng(960).integers(1000, 30001, nq).astype(np.int32)
+    dT = S.Dev.from_numpy(ctx, T)
+    got = {}
+    for form in (S.FORM_PLAIN, S.FORM_MIXED):
+        plan = S.Plan(ix, q, S.KIND_IMI, R, w, 30000, form=form, budgets=dT)
+        for _ in range(2):
+            plan.execute(stats=False)
+        got[form] = (plan.ids.numpy(), plan.dists.numpy(), plan.sums.numpy())
+        plan.close()
+    for a, b in zip(got[S.FORM_PLAIN], got[S.FORM_MIXED]):
+        assert np.array_equal(_bits(a) if a.dtype == np.float32 else a,
+                              _bits(b) if b.dtype == np.float32 else b)
+    e = ix.export()
+    oix = sco.OracleIndex(dim, 2, kc, mp, kp, cq, pq, e["L"], e["pid"], e["codes"])
+    qh = q.numpy()
+    gi, gd, gs = got[S.FORM_PLAIN]
+    for i in np.random.default_rng(3).choice(nq, 24, replace=False):
+        oi, od, st, _ = oix.search(sco.IMI, qh[i], R, w, int(T[i]))
+        assert gs[i] == st.sum, i
+        assert np.array_equal(gi[i], oi), i
+        assert np.array_equal(_bits(gd[i]), _bits(od)), i
+    ix.close()
+
+
 def test_exhaustive_pq_config(ctx):
     """Exhaustive PQ ADC (the reference's CPU configuration: 100k base vectors,
     PQ 8x256 trained on 50k, top-100) mapped onto search_ivfadc with one
