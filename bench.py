@@ -477,8 +477,8 @@ def d960_line(S, ctx, args):
     xt = S.gen_rows(ctx, mix, S.STREAM_TRAIN, 0, args.ntrain960, dim)
     th = xt.numpy()
     cq = np.stack([S.kmeans(ctx, S.Dev.from_numpy(ctx, np.ascontiguousarray(th[:, h * 480:(h + 1) * 480])),
-                            kc, args.iters, seed=41 + h, prec=S.PREC_BF16,
-                            want_labels=False)[0].numpy() for h in range(2)])
+                            kc, args.iters, seed=41 + h, want_labels=False)[0].numpy()
+                   for h in range(2)])
     dcq = S.Dev.from_numpy(ctx, cq)
     S.residual(ctx, xt, dcq, 2, kc, 0)
     rh = xt.numpy()

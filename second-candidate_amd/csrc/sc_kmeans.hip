@@ -220,13 +220,12 @@ __global__ void k_member_sums(const float *__restrict__ x, int d, int k,
     const int cnt = counts[j];
     double acc = 0.0;
     int i = 0;
-    for (; i + 4 <= cnt; i += 4) {   // four independent loads, adds in order
-        const float a = x[(size_t)m[i] * d + t], b = x[(size_t)m[i + 1] * d + t];
-        const float c = x[(size_t)m[i + 2] * d + t], f = x[(size_t)m[i + 3] * d + t];
-        acc += (double)a;
-        acc += (double)b;
-        acc += (double)c;
-        acc += (double)f;
+    for (; i + 16 <= cnt; i += 16) {   // sixteen independent loads, adds in order
+        float f[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) f[u] = x[(size_t)m[i + u] * d + t];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) acc += (double)f[u];
     }
     for (; i < cnt; ++i) acc += (double)x[(size_t)m[i] * d + t];
     sums[e] = acc;
@@ -308,8 +307,61 @@ __global__ void k_fill_small(const float *__restrict__ x, int64_t n, int d, int 
 }
 
 // Residuals: nearest (which=0) or second (which=1) by float-rounded S1 with
-// strict `>` (lowest index wins), subtracted in place.  One thread per
-// (vector, block).
+// strict `>` (lowest index wins), subtracted in place -- the two smallest by
+// (float distance, index).  Tiled form: TV vectors per workgroup against the
+// block's codebook in LDS chunks (the general encoder's float-rounded top-NC
+// tile), the chosen centroid subtracted from the vectors in global memory.
+template <int NC, int TV, int TK>
+__global__ __launch_bounds__((TV / 4) * (TK / 4)) void k_residual_tile(
+    float *__restrict__ x, int64_t n, int dim, const float *__restrict__ cq, int mc, int kc) {
+    constexpr int NT = (TV / 4) * (TK / 4);
+    extern __shared__ __align__(16) unsigned char smem[];
+    float *xs = reinterpret_cast<float *>(smem);                       // [dim][TV]
+    float *cs = xs + (size_t)dim * TV;                                  // [kTT][TK]
+    TopF<NC> *red = reinterpret_cast<TopF<NC> *>(cs + kTT * TK);        // [TK/4][TV]
+    int *kb = reinterpret_cast<int *>(red + (TK / 4) * TV);             // [TV][NC]
+    const int tid = threadIdx.x;
+    const int64_t v0 = (int64_t)blockIdx.x * TV;
+    const int nv = (int)min((int64_t)TV, n - v0);
+    for (int e = tid; e < TV * dim; e += NT) {
+        const int v = e / dim, t = e % dim;
+        xs[(size_t)t * TV + v] = v < nv ? x[(size_t)(v0 + v) * dim + t] : 0.0f;
+    }
+    __syncthreads();
+    const int bsc = dim / mc;
+    for (int j = 0; j < mc; ++j) {
+        const float *cb = cq + (size_t)j * kc * bsc;
+        tile_top<TV, TK, NC>(xs, j * bsc, bsc, cb, kc, cs, red, kb);
+        for (int e = tid; e < nv * bsc; e += NT) {
+            const int v = e / bsc, t = e % bsc;
+            const int c = max(kb[v * NC + NC - 1], 0);
+            x[(size_t)(v0 + v) * dim + j * bsc + t] = xs[(size_t)(j * bsc + t) * TV + v] -
+                                                      cb[(size_t)c * bsc + t];
+        }
+        __syncthreads();
+    }
+}
+
+template <int NC, int TV, int TK>
+size_t residual_lds(int dim) {
+    return (size_t)dim * TV * 4 + kTT * TK * 4 + (TK / 4) * TV * sizeof(TopF<NC>) +
+           (size_t)TV * NC * 4;
+}
+
+template <int NC, int TV, int TK>
+int launch_residual(sc_ctx *ctx, float *x, int64_t n, int dim, const float *cq, int mc, int kc) {
+    constexpr int NT = (TV / 4) * (TK / 4);
+    const size_t lds = residual_lds<NC, TV, TK>(dim);
+    auto kern = k_residual_tile<NC, TV, TK>;
+    SC_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)lds));
+    hipLaunchKernelGGL(kern, dim3((unsigned)sc_cdiv(n, TV)), dim3(NT), lds, ctx->stream, x, n, dim,
+                       cq, mc, kc);
+    SC_CHECK_LAUNCH();
+    return SC_OK;
+}
+
+// One thread per (vector, block): the form for vectors too long for the tile.
 __global__ void k_residual(float *__restrict__ x, int64_t n, int dim, const float *__restrict__ cq,
                            int mc, int kc, int which) {
     int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -470,6 +522,12 @@ extern "C" int sc_residual(sc_ctx *ctx, float *d_x, int64_t n, int32_t dim, cons
         return SC_ERR_ARG;
     }
     if (n == 0) return SC_OK;
+    if (dim <= 256)
+        return which ? launch_residual<2, 64, 64>(ctx, d_x, n, dim, d_cq, mc, kc)
+                     : launch_residual<1, 64, 64>(ctx, d_x, n, dim, d_cq, mc, kc);
+    if (residual_lds<2, 16, 256>(dim) <= 160 * 1024)
+        return which ? launch_residual<2, 16, 256>(ctx, d_x, n, dim, d_cq, mc, kc)
+                     : launch_residual<1, 16, 256>(ctx, d_x, n, dim, d_cq, mc, kc);
     hipLaunchKernelGGL(k_residual, dim3((unsigned)sc_cdiv(n * mc, 256)), dim3(256), 0,
                        ctx->stream, d_x, n, dim, d_cq, mc, kc, which);
     SC_CHECK_LAUNCH();

@@ -663,13 +663,23 @@ def test_kmeans_exact_on_fractional_data(ctx, prec):
 
 
 @pytest.mark.parametrize("which", [0, 1])
-def test_residual_matches_oracle(ctx, which):
-    x = sco.gen(sco.mix_d128(ncomp=6), sco.STREAM_TRAIN, 0, 3000, 32)
-    rng = np.random.default_rng(which)
-    cq = x[rng.choice(3000, 2 * 20, replace=False)].reshape(2, 20, 32)[:, :, :16].copy()
+@pytest.mark.parametrize("dim,mc,kc,n", [(32, 2, 20, 3000), (128, 8, 256, 2100),
+                                          (960, 2, 300, 700), (64, 1, 2, 500)])
+def test_residual_matches_oracle(ctx, which, dim, mc, kc, n):
+    """Residuals against the nearest / second-nearest centroid of each block
+    (float-rounded distances, lowest index on ties), with duplicated
+    centroids and vectors equal to a centroid planted."""
+    x = sco.gen(sco.mix_d960() if dim == 960 else sco.mix_d128(ncomp=6), sco.STREAM_TRAIN, 0, n,
+                dim)
+    rng = np.random.default_rng(which + dim)
+    bs = dim // mc
+    cq = np.stack([x[rng.choice(n, kc, replace=False), j * bs:(j + 1) * bs] for j in range(mc)])
+    if kc > 3:
+        cq[:, kc - 1] = cq[:, 1]
+    x[:3, :bs] = cq[0, 1]
     d = S.Dev.from_numpy(ctx, x)
-    S.residual(ctx, d, S.Dev.from_numpy(ctx, cq), 2, 20, which)
-    o = sco.residual(x, cq, 2, 20, which)
+    S.residual(ctx, d, S.Dev.from_numpy(ctx, cq), mc, kc, which)
+    o = sco.residual(x, cq, mc, kc, which)
     assert np.array_equal(_bits(d.numpy()), _bits(o))
 
 
