@@ -114,8 +114,10 @@ int sc_gen_rows(sc_ctx *ctx, const sc_mix *mix, uint32_t stream, int64_t row0,
 
 /* ---- encode ------------------------------------------------------------ */
 typedef struct {
-    int64_t rechecked;   /* rows re-evaluated exactly (SC_PREC_BF16) */
-    int64_t fallback;    /* rows whose candidate set overflowed -> full exact */
+    int64_t rechecked;   /* SC_PREC_BF16: codewords re-evaluated exactly, summed
+                            over rows and blocks */
+    int64_t fallback;    /* (row, block) pairs whose candidate set overflowed and
+                            were evaluated exactly over the whole codebook */
 } sc_encode_stats;
 
 /* Coarse assignment + residual + PQ codes for n vectors.

@@ -514,26 +514,85 @@ static int cmp_cell_rank(const void *a, const void *b) {
     return (x->key > y->key) - (x->key < y->key);
 }
 
+/* Directories of at least this many cells are ranked by selection (below);
+ * smaller ones by sorting every cell.  Settable for tests. */
+static int64_t scimi_select_min = (int64_t)1 << 20;
+void sco_scimi_set_select_min(int64_t n) { scimi_select_min = n; }
+
+static float scimi_score(const sco_index *ix, const float *v, uint64_t key) {
+    const int64_t kc = ix->kc, kc2 = kc * kc;
+    const int64_t u0 = (int64_t)(key / (uint64_t)kc2), u1 = (int64_t)(key % (uint64_t)kc2);
+    const float a = v[u0 / kc] + v[u0 % kc];
+    const float b = v[kc + u1 / kc] + v[kc + u1 % kc];
+    return a + b;
+}
+
+/* Order-preserving map of a float onto uint32 (-0.0 and +0.0 share a key). */
+static uint32_t fkey(float f) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    if (u == 0x80000000u) u = 0u;
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
 /* Steps 2-3 for one query (v [2][kc] by centroid id); cells/s1/s2 [w] get the
- * directory index and the two nearest centroids of each taken cell. */
+ * directory index and the two nearest centroids of each taken cell.
+ * Large directories: the taken cells are a prefix of the (score, key) order,
+ * so they all lie in the histogram buckets (top 16 bits of the score's
+ * order-preserving key) up to the first one at which the running count
+ * reaches w or the running size sum reaches T; only the cells of those
+ * buckets are sorted.  Small directories: every cell is sorted. */
 static int scimi_rank(const sco_index *ix, const float *v, int32_t w, int32_t T,
                       int32_t *prebuck, int32_t *s1, int32_t *s2, int32_t *count_o,
                       int32_t *sum_o) {
     const int64_t kc = ix->kc, kc2 = kc * kc;
-    cell_rank *r = (cell_rank *)malloc(sizeof(cell_rank) * ((size_t)ix->ncells + 1));
-    if (!r) return SCO_ERR_NOMEM;
-    for (int64_t c = 0; c < ix->ncells; ++c) {
-        const uint64_t key = ix->ckeys[c];
-        const int64_t u0 = (int64_t)(key / (uint64_t)kc2), u1 = (int64_t)(key % (uint64_t)kc2);
-        const float a = v[u0 / kc] + v[u0 % kc];
-        const float b = v[kc + u1 / kc] + v[kc + u1 % kc];
-        r[c].s = a + b;
-        r[c].key = key;
-        r[c].c = c;
+    int64_t nr = ix->ncells;
+    cell_rank *r;
+    if (ix->ncells >= scimi_select_min) {
+        enum { NB = 65536 };
+        uint32_t *keys = (uint32_t *)malloc(sizeof(uint32_t) * ((size_t)ix->ncells + 1));
+        int64_t *hc = (int64_t *)calloc(NB, sizeof(int64_t));
+        int64_t *hs = (int64_t *)calloc(NB, sizeof(int64_t));
+        if (!keys || !hc || !hs) { free(keys); free(hc); free(hs); return SCO_ERR_NOMEM; }
+        for (int64_t c = 0; c < ix->ncells; ++c) {
+            keys[c] = fkey(scimi_score(ix, v, ix->ckeys[c]));
+            hc[keys[c] >> 16] += 1;
+            hs[keys[c] >> 16] += cell_len(ix->L, c);
+        }
+        int64_t cc = 0, cs = 0;
+        uint32_t last = NB - 1;
+        for (uint32_t b = 0; b < NB; ++b) {
+            cc += hc[b];
+            cs += hs[b];
+            if (cc >= w || cs >= T) { last = b; break; }
+        }
+        nr = 0;
+        for (int64_t c = 0; c < ix->ncells; ++c) nr += (keys[c] >> 16) <= last;
+        r = (cell_rank *)malloc(sizeof(cell_rank) * ((size_t)nr + 1));
+        if (!r) { free(keys); free(hc); free(hs); return SCO_ERR_NOMEM; }
+        int64_t j = 0;
+        for (int64_t c = 0; c < ix->ncells; ++c) {
+            if ((keys[c] >> 16) > last) continue;
+            r[j].s = scimi_score(ix, v, ix->ckeys[c]);
+            r[j].key = ix->ckeys[c];
+            r[j].c = c;
+            ++j;
+        }
+        free(keys);
+        free(hc);
+        free(hs);
+    } else {
+        r = (cell_rank *)malloc(sizeof(cell_rank) * ((size_t)ix->ncells + 1));
+        if (!r) return SCO_ERR_NOMEM;
+        for (int64_t c = 0; c < ix->ncells; ++c) {
+            r[c].s = scimi_score(ix, v, ix->ckeys[c]);
+            r[c].key = ix->ckeys[c];
+            r[c].c = c;
+        }
     }
-    qsort(r, (size_t)ix->ncells, sizeof(cell_rank), cmp_cell_rank);
+    qsort(r, (size_t)nr, sizeof(cell_rank), cmp_cell_rank);
     int32_t count = 0, sum = 0;
-    for (int64_t i = 0; i < ix->ncells && count < w && sum < T; ++i) {
+    for (int64_t i = 0; i < nr && count < w && sum < T; ++i) {
         const int64_t c = r[i].c;
         const uint64_t key = r[i].key;
         prebuck[count] = (int32_t)c;

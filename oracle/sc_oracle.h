@@ -128,6 +128,11 @@ typedef struct {
  * such search; the definition is stated above scimi_rank in sc_oracle.c. */
 #define SCO_SCIMI  4
 
+/* SCO_SCIMI directories of at least n cells are ranked by selection (only
+ * the cells of the histogram buckets up to the budget are sorted), smaller
+ * ones by sorting every cell; both give the same cells.  Default 2^20. */
+void sco_scimi_set_select_min(int64_t n);
+
 /* One query.  ids/dists: [R] (padded with -1 / +inf when sum < R, in which
  * case the first `sum` entries keep traversal order like the reference).
  * Optional (may be NULL): cells [w] bid of scanned cells in order, s1/s2 [w],

@@ -87,6 +87,7 @@ def _load():
     lib.sco_search_batch.argtypes = [C.POINTER(Index), C.c_int32, _f32p, C.c_int64,
                                      C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                      _i32p, _f32p, C.c_void_p, C.c_int32]
+    lib.sco_scimi_set_select_min.argtypes = [C.c_int64]
     lib.sco_search_real.argtypes = [C.POINTER(Index), C.c_int32, _f32p, _f32p, C.c_int32,
                                     C.c_int32, C.c_int32, _i32p, _f32p, C.c_void_p]
     lib.sco_heap4_insert.argtypes = [C.POINTER(Heap4), C.c_float] + [C.c_int32] * 4

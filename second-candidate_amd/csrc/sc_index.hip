@@ -152,6 +152,50 @@ __global__ void k_group_starts(const uint32_t *__restrict__ cpair, int64_t ncell
     if (c == 0 || (cpair[2 * (c - 1)] & 0xFFFFu) != g) gstart[g] = (int32_t)c;
 }
 
+// Groups of the pruned ranking: the runs of cells sharing a half-0 pair
+// (uid_0 = key / kc^2; cells are in key order).  heads[c] = 1 where a run
+// starts.
+__global__ void k_group_heads(const uint64_t *__restrict__ ckeys, int64_t ncells, int kc,
+                              int32_t *__restrict__ heads) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncells) return;
+    const uint64_t k2 = (uint64_t)kc * (uint64_t)kc;
+    heads[c] = (c == 0 || ckeys[c] / k2 != ckeys[c - 1] / k2) ? 1 : 0;
+}
+// From the inclusive scan of the heads (gno = group number + 1 of each cell):
+// each group's pair and first cell, and each cell's half-1 record
+// n1 | s1 << 12 | min(size, 255) << 24 (kc <= 4096).
+__global__ void k_group_fill(const uint64_t *__restrict__ ckeys, const int32_t *__restrict__ L,
+                             const int32_t *__restrict__ gno, int64_t ncells, int kc,
+                             uint32_t *__restrict__ bg_pair, int32_t *__restrict__ bg_start,
+                             uint32_t *__restrict__ brec) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncells) return;
+    const uint64_t k2 = (uint64_t)kc * (uint64_t)kc, key = ckeys[c];
+    const uint32_t u0 = (uint32_t)(key / k2), u1 = (uint32_t)(key - (uint64_t)u0 * k2);
+    if (c == 0 || ckeys[c - 1] / k2 != u0) {
+        const int32_t g = gno[c] - 1;
+        bg_start[g] = (int32_t)c;
+        bg_pair[g] = (u0 / (uint32_t)kc) | ((u0 % (uint32_t)kc) << 16);
+    }
+    if (c == ncells - 1) bg_start[gno[c]] = (int32_t)ncells;
+    const uint32_t size = (uint32_t)(L[c] - (c > 0 ? L[c - 1] : 0));
+    brec[c] = (u1 / (uint32_t)kc) | ((u1 % (uint32_t)kc) << 12) | (min(size, 255u) << 24);
+}
+// bg_n0[n] = the first group whose nearest-0 centroid is at least n
+// (n in [0, kc]; bg_n0[kc] = ngroups).
+__global__ void k_group_n0(const uint32_t *__restrict__ bg_pair, int32_t ngroups, int kc,
+                           int32_t *__restrict__ bg_n0) {
+    const int n = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (n > kc) return;
+    int lo = 0, hi = ngroups;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if ((int)(bg_pair[mid] & 0xFFFFu) < n) lo = mid + 1; else hi = mid;
+    }
+    bg_n0[n] = lo;
+}
+
 // Per vector p (cell order) of a two-block index, the sum of its dot_cr terms
 // in subspace order, e = sum_k dot_cr[row(k)][k][c_k] with row(k) the cell's
 // nearest centroid of the half subspace k lies in (include/multi_query.h:
@@ -385,6 +429,46 @@ static int build_pair_lists(sc_index *ix) {
     return SC_OK;
 }
 
+// The groups and half-1 records of the pruned ranking (two halves, kc <= 4096).
+static int build_groups(sc_index *ix) {
+    sc_ctx *ctx = ix->ctx;
+    const int64_t nc = ix->ncells;
+    size_t scan_bytes = 0;
+    SC_HIP(rocprim::inclusive_scan(nullptr, scan_bytes, (int32_t *)nullptr, (int32_t *)nullptr,
+                                   (size_t)nc, rocprim::plus<int32_t>(), ctx->stream));
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t o_gno = al((size_t)nc * 4), o_tmp = o_gno + al((size_t)nc * 4),
+                 total = o_tmp + al(scan_bytes);
+    unsigned char *s;
+    SC_TRY(sc_scratch(ctx, total, (void **)&s));
+    int32_t *heads = (int32_t *)s, *gno = (int32_t *)(s + o_gno);
+    const unsigned g = (unsigned)sc_cdiv(nc, 256);
+    hipLaunchKernelGGL(k_group_heads, dim3(g), dim3(256), 0, ctx->stream, ix->ckeys, nc, ix->kc,
+                       heads);
+    SC_CHECK_LAUNCH();
+    size_t tb = scan_bytes;
+    SC_HIP(rocprim::inclusive_scan(s + o_tmp, tb, heads, gno, (size_t)nc,
+                                   rocprim::plus<int32_t>(), ctx->stream));
+    int32_t ng = 0;
+    SC_HIP(hipMemcpyAsync(&ng, gno + (nc - 1), 4, hipMemcpyDeviceToHost, ctx->stream));
+    SC_HIP(hipStreamSynchronize(ctx->stream));
+    ix->ngroups = ng;
+    if (hipMalloc(&ix->bg_pair, 4 * (size_t)ng) != hipSuccess ||
+        hipMalloc(&ix->bg_start, 4 * ((size_t)ng + 1)) != hipSuccess ||
+        hipMalloc(&ix->bg_n0, 4 * ((size_t)ix->kc + 1)) != hipSuccess ||
+        hipMalloc(&ix->brec, 4 * (size_t)nc) != hipSuccess) {
+        sc_set_error("index: group directory allocation failed");
+        return SC_ERR_NOMEM;
+    }
+    hipLaunchKernelGGL(k_group_fill, dim3(g), dim3(256), 0, ctx->stream, ix->ckeys, ix->L, gno, nc,
+                       ix->kc, ix->bg_pair, ix->bg_start, ix->brec);
+    SC_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_group_n0, dim3((unsigned)sc_cdiv((int64_t)ix->kc + 1, 256)), dim3(256), 0,
+                       ctx->stream, ix->bg_pair, ng, ix->kc, ix->bg_n0);
+    SC_CHECK_LAUNCH();
+    return SC_OK;
+}
+
 int finish_index(sc_index *ix) {
     sc_ctx *ctx = ix->ctx;
     SC_TRY(sc_launch_tables(ix));
@@ -394,6 +478,7 @@ int finish_index(sc_index *ix) {
                            ix->crec);
         SC_CHECK_LAUNCH();
         SC_TRY(build_pair_lists(ix));
+        if (ix->kc <= 4096) SC_TRY(build_groups(ix));
     }
     int32_t *dmax, h[3] = {0, 0, 0};
     SC_TRY(sc_scratch(ctx, 3 * sizeof(int32_t), (void **)&dmax));
@@ -775,7 +860,8 @@ extern "C" int sc_index_destroy(sc_index *ix) {
     void *bufs[] = {ix->cq,    ix->pq,   ix->norm_c, ix->norm_r, ix->dot_cr,
                     ix->L,     ix->pid,  ix->codes,  ix->cqT,    ix->pqT,
                     ix->ckeys, ix->crec, ix->upairs[0], ix->upairs[1], ix->cpair,
-                    ix->gstart, ix->evec, ix->vcell};
+                    ix->gstart, ix->evec, ix->vcell, ix->bg_n0, ix->bg_pair, ix->bg_start,
+                    ix->brec};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     delete ix;

@@ -46,6 +46,14 @@ struct sc_index {
     uint32_t *upairs[2];            // [nu[h]] n | s << 16
     uint32_t *cpair;                // [ncells][2]
     int32_t *gstart;                // [nu[0] + 1] first cell of each half-0 pair
+    // Large two-halves indexes (kc <= 4096): the half-0 pairs present
+    // ("groups") in key order, each the contiguous run of its cells, and per
+    // cell a compact record of its half-1 pair, for the pruned cell ranking.
+    int32_t ngroups;
+    int32_t *bg_n0;                 // [kc + 1] first group of each nearest-0 centroid
+    uint32_t *bg_pair;              // [ngroups] n0 | s0 << 16
+    int32_t *bg_start;              // [ngroups + 1] first cell of each group
+    uint32_t *brec;                 // [ncells] n1 | s1 << 12 | min(size, 255) << 24
     // Two-block indexes (mc = 2): per vector in cell order, the sequential FP32
     // sum of its dot_cr terms (the query-independent part of its IMI distance),
     // for the mixed-form scan.
@@ -141,5 +149,15 @@ struct RankCellsArgs {
     int w, T, ccap;
     int32_t *flag_list;      // device scratch: [nq] queries for the device-wide path
     int32_t *flag_count;     // device scratch: 1 counter
+    // Pruned ranking of large indexes (sc_rank_big_cap(ix) > 0): per query
+    // `big_cap` appended entries, their bucket-ordered copy and sizes, and
+    // four words of per-query state.
+    uint64_t *big_list, *big_sorted;
+    uint32_t *big_size, *big_aux;
+    int big_cap;
+    int64_t *overflowed;     // host, may be null: += queries that outgrew the first list
 };
 int sc_rank_cells(sc_ctx *ctx, const sc_index *ix, const RankCellsArgs &a);
+// Entries per query of the pruned ranking's scratch, or 0 when the index is
+// ranked by the other kernels.
+int sc_rank_big_cap(const sc_index *ix);

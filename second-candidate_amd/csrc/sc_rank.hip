@@ -830,6 +830,423 @@ __global__ __launch_bounds__(kRankThreads) void k_rank_pairs(RankKernelArgs k) {
 }
 
 // ---------------------------------------------------------------------------
+// Pruned ranking of large indexes (kc up to 4096, tens of millions of cells):
+// scoring every cell for every query is out of reach, so only the cells that
+// can lie before the stopping point are visited.
+//
+// Lower bounds (round-to-nearest addition is monotone in each operand):
+//   P_h = v_h[n_h] + v_h[s_h] >= LB_h = m_h1 + m_h2 (the half's two smallest
+//   v, as n_h != s_h), so score >= base = LB_0 + LB_1; the cells of a half-0
+//   pair score at least P_0 + LB_1, and those of a nearest-0 centroid n at
+//   least (v_0[n] + m_01) + LB_1.
+// Buckets: d = score - base >= 0 and its top 12 bits (exponent and four
+// mantissa bits) order like d, so like the score: a log-spaced histogram
+// whose buckets are 1/16 of their value wide, whatever the data's scale.
+//
+// k_rank_big (one workgroup per query): the nearest-0 centroids are ordered
+// by the bucket of their bound (a counting sort) and handed to the waves in
+// that order; a wave scores the cells of its centroid's pairs and appends
+// every cell whose bucket is at most the current cut to the query's list,
+// adding it to a histogram of counts and sizes.  The cut -- the bucket where
+// the histogram's running count reaches w or its running size sum reaches T
+// -- only falls as cells are added, and it never falls below the cut of the
+// full set of cells (a histogram over fewer cells reaches the budget no
+// earlier).  A centroid, pair or cell whose bound's bucket exceeds the cut
+// is skipped, so every cell whose bucket is at most the final cut has been
+// appended and counted: the histogram's running totals are exact up to it
+// and its crossing bucket is the full set's.
+//
+// k_rank_big_emit: the entries up to the final cut, counting-sorted by their
+// score key into fine buckets; each entry's rank and position base (the
+// sizes of the entries before it) in (score, cell) order are its bucket's
+// prefix plus a count over its own bucket; the taken ones (rank < w, base <
+// T) are written at their rank.
+//
+// A query whose list outgrows its capacity is run again starting from the
+// final cut of its first run (then only the cells up to that cut are
+// appended); if that still overflows, the device-wide sort finishes it.
+// ---------------------------------------------------------------------------
+constexpr int kBigCap = 32768;       // appended entries per query
+constexpr int kFine = 4096;          // fine buckets of the emit kernel
+
+struct BigArgs {
+    RankKernelArgs k;
+    const int32_t *bg_n0;
+    const uint32_t *bg_pair;
+    const int32_t *bg_start;
+    const uint32_t *brec;
+    const uint64_t *ckeys;
+    uint64_t *list, *sorted;
+    uint32_t *size, *aux;       // aux [nq][4]: appended, final bucket, base bits, overflow
+    int cap;
+    int rerun;                  // 1: start from the cut aux holds
+};
+
+__device__ __forceinline__ int dbucket(float s, float base) {
+    const uint32_t u = __float_as_uint(s - base);
+    return (u & 0x80000000u) ? 0 : (int)(u >> 19);
+}
+
+__device__ __forceinline__ uint32_t big_size(const BigArgs &b, int64_t c, uint32_t r) {
+    const uint32_t sz = r >> 24;
+    return sz < 255u ? sz : (uint32_t)(b.k.L[c] - (c > 0 ? b.k.L[c - 1] : 0));
+}
+
+// The two smallest values of x[0 .. n) (ascending; equal when the least
+// value occurs twice).  All threads call; every thread gets the result.
+__device__ void two_smallest(const float *x, int n, float *scratch8, float *lo, float *hi) {
+    const int tid = threadIdx.x;
+    float a = __builtin_huge_valf(), b = __builtin_huge_valf();
+    for (int i = tid; i < n; i += kRankThreads) {
+        const float t = x[i];
+        if (t < a) { b = a; a = t; } else if (t < b) { b = t; }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const float a2 = __shfl_xor(a, o), b2 = __shfl_xor(b, o);
+        const float na = fminf(a, a2), nb = fminf(fmaxf(a, a2), fminf(b, b2));
+        a = na;
+        b = nb;
+    }
+    __syncthreads();
+    if ((tid & 63) == 0) { scratch8[2 * (tid >> 6)] = a; scratch8[2 * (tid >> 6) + 1] = b; }
+    __syncthreads();
+    a = scratch8[0];
+    b = scratch8[1];
+    for (int i = 1; i < kRankThreads / 64; ++i) {
+        const float a2 = scratch8[2 * i], b2 = scratch8[2 * i + 1];
+        const float na = fminf(a, a2), nb = fminf(fmaxf(a, a2), fminf(b, b2));
+        a = na;
+        b = nb;
+    }
+    *lo = a;
+    *hi = b;
+    __syncthreads();
+}
+
+// The crossing bucket of the histograms, read by one wave while other waves
+// may still add to them.  Every value read is at most the count (or size
+// sum) of all the cells of its bucket, so the bucket found is never below
+// the full set's crossing bucket.  kBuckets when the totals read never reach
+// the budget.
+__device__ int wave_find_bucket(const uint32_t *hcnt, const uint32_t *hmass, int w, int T) {
+    constexpr int per = kBuckets / 64;
+    const int lane = threadIdx.x & 63;
+    uint32_t c = 0, m = 0;
+    for (int i = 0; i < per; ++i) {
+        c += hcnt[lane * per + i];
+        m += hmass[lane * per + i];
+    }
+    uint32_t ic = c, im = m;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t t = __shfl_up(ic, o), u = __shfl_up(im, o);
+        if (lane >= o) { ic += t; im += u; }
+    }
+    const unsigned long long hit = __ballot((int64_t)ic >= w || (int64_t)im >= T);
+    if (hit == 0) return kBuckets;
+    const int l0 = __ffsll((long long)hit) - 1;
+    uint32_t cc = __shfl(ic - c, l0), mm = __shfl(im - m, l0);
+    int found = l0 * per + per - 1;
+    if (lane == l0) {
+        for (int i = 0; i < per; ++i) {
+            cc += hcnt[l0 * per + i];
+            mm += hmass[l0 * per + i];
+            if ((int64_t)cc >= w || (int64_t)mm >= T) { found = l0 * per + i; break; }
+        }
+    }
+    return __shfl(found, l0);
+}
+
+// LDS of k_rank_big: the histograms [2][kBuckets], v [max(2 kc, dim)], the
+// centroid order [kc] (u16).
+__host__ __device__ inline size_t big_lds(int kc, int dim) {
+    const size_t nv = (size_t)(2 * kc > dim ? 2 * kc : dim);
+    return 8 * (size_t)kBuckets + 4 * nv + ((2 * (size_t)kc + 15) & ~(size_t)15);
+}
+
+__global__ __launch_bounds__(kRankThreads) void k_rank_big(BigArgs b) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const RankKernelArgs &k = b.k;
+    const int kc = k.kc;
+    uint32_t *hcnt = reinterpret_cast<uint32_t *>(smem);          // [kBuckets]
+    uint32_t *hmass = hcnt + kBuckets;                            // [kBuckets]
+    float *v = reinterpret_cast<float *>(hmass + kBuckets);       // [max(2 kc, dim)]
+    uint16_t *ord = reinterpret_cast<uint16_t *>(v + max(2 * kc, k.dim));   // [kc]
+    __shared__ RankShared sh;
+    __shared__ float s_min[8];
+    __shared__ int s_pos, s_bcur, s_napp;
+    __shared__ uint32_t s_cnt, s_mass;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int64_t q = k.qlist ? k.qlist[blockIdx.x] : blockIdx.x;
+    const int T = query_budget(k.a.Tq, q, k.a.T), w = k.a.w;
+    uint32_t *aux = b.aux + (size_t)q * 4;
+    const int start_cut = b.rerun ? (int)aux[1] : kBuckets;
+    query_terms(k, q, v, sh);
+    if (T <= 0 || k.ncells == 0) return;       // the emit kernel writes the empty result
+    float m0a, m0b, m1a, m1b;
+    two_smallest(v, kc, s_min, &m0a, &m0b);
+    two_smallest(v + kc, kc, s_min, &m1a, &m1b);
+    const float LB1 = m1a + m1b, base = (m0a + m0b) + LB1;
+
+    // the nearest-0 centroids in order of their bound's bucket
+    for (int i = tid; i < kBuckets; i += kRankThreads) hcnt[i] = 0;
+    __syncthreads();
+    for (int n = tid; n < kc; n += kRankThreads)
+        atomicAdd(&hcnt[dbucket((v[n] + m0a) + LB1, base)], 1u);
+    __syncthreads();
+    {
+        constexpr int per = kBuckets / kRankThreads;
+        uint32_t loc = 0;
+        for (int i = 0; i < per; ++i) loc += hcnt[tid * per + i];
+        uint32_t tot;
+        uint32_t run = block_excl_scan(loc, sh.wsum, &tot);
+        for (int i = 0; i < per; ++i) {
+            const uint32_t c = hcnt[tid * per + i];
+            hcnt[tid * per + i] = run;
+            run += c;
+        }
+    }
+    __syncthreads();
+    for (int n = tid; n < kc; n += kRankThreads)
+        ord[atomicAdd(&hcnt[dbucket((v[n] + m0a) + LB1, base)], 1u)] = (uint16_t)n;
+    __syncthreads();
+    for (int i = tid; i < kBuckets; i += kRankThreads) { hcnt[i] = 0; hmass[i] = 0; }
+    if (tid == 0) { s_pos = 0; s_bcur = start_cut; s_napp = 0; s_cnt = 0; s_mass = 0; }
+    __syncthreads();
+
+    uint64_t *list = b.list + (size_t)q * b.cap;
+    const float *v1 = v + kc;
+    volatile int *bcut = &s_bcur;
+    uint32_t since_cnt = 0, since_mass = 0;       // this wave's additions since its last cut
+    for (;;) {
+        int i = 0;
+        if (lane == 0) i = atomicAdd(&s_pos, 1);
+        i = __shfl(i, 0);
+        if (i >= kc) break;
+        const int n0 = ord[i];
+        const float a0 = v[n0];
+        // one value of the cut per wave (another wave may lower it meanwhile)
+        int bcur = __shfl(*bcut, 0);
+        if (dbucket((a0 + m0a) + LB1, base) > bcur) break;   // so are the centroids after it
+        const int g0 = b.bg_n0[n0], g1 = b.bg_n0[n0 + 1];
+        uint32_t add_cnt = 0, add_mass = 0;                 // this lane's additions
+        for (int gb = g0; gb < g1; gb += 64) {
+            const int g = gb + lane;
+            float p0 = 0.0f;
+            int c0 = 0, c1 = 0;
+            bool ok = false;
+            if (g < g1) {
+                p0 = a0 + v[b.bg_pair[g] >> 16];
+                ok = dbucket(p0 + LB1, base) <= bcur;
+                if (ok) { c0 = b.bg_start[g]; c1 = b.bg_start[g + 1]; }
+            }
+            unsigned long long gm = __ballot(ok);
+            while (gm) {
+                const int j = __ffsll((long long)gm) - 1;
+                gm &= gm - 1;
+                const float P0 = __shfl(p0, j);
+                const int cb = __shfl(c0, j), ce = __shfl(c1, j);
+                bcur = __shfl(*bcut, 0);
+                for (int cs = cb; cs < ce; cs += 64) {
+                    const int c = cs + lane;
+                    bool adm = false;
+                    uint32_t key = 0;
+                    if (c < ce) {
+                        const uint32_t r = b.brec[c];
+                        const float s = P0 + (v1[r & 0xFFFu] + v1[(r >> 12) & 0xFFFu]);
+                        const int bk = dbucket(s, base);
+                        if (bk <= bcur) {
+                            adm = true;
+                            key = sc_float_key(s);
+                            const uint32_t sz = big_size(b, c, r);
+                            atomicAdd(&hcnt[bk], 1u);
+                            atomicAdd(&hmass[bk], sz);
+                            add_cnt += 1;
+                            add_mass += sz;
+                        }
+                    }
+                    const unsigned long long am = __ballot(adm);
+                    if (am) {
+                        int slot0 = 0;
+                        if (lane == 0) slot0 = atomicAdd(&s_napp, (int)__popcll(am));
+                        slot0 = __shfl(slot0, 0);
+                        if (adm) {
+                            const int slot =
+                                slot0 + (int)__popcll(am & ((1ull << lane) - 1ull));
+                            if (slot < b.cap) list[slot] = ((uint64_t)key << 32) | (uint32_t)c;
+                        }
+                    }
+                }
+            }
+        }
+        // this centroid's additions, then a tighter cut when enough was added
+        for (int o = 32; o > 0; o >>= 1) {
+            add_cnt += __shfl_xor(add_cnt, o);
+            add_mass += __shfl_xor(add_mass, o);
+        }
+        if (add_cnt > 0) {
+            uint32_t tc = 0, tm = 0;
+            if (lane == 0) {
+                tc = atomicAdd(&s_cnt, add_cnt) + add_cnt;
+                tm = atomicAdd(&s_mass, add_mass) + add_mass;
+            }
+            tc = __shfl(tc, 0);
+            tm = __shfl(tm, 0);
+            since_cnt += add_cnt;
+            since_mass += add_mass;
+            const bool reach = (int64_t)tc >= w || (int64_t)tm >= T;
+            const bool enough = __shfl(*bcut, 0) == kBuckets || (int64_t)since_mass * 16 >= T ||
+                                (int64_t)since_cnt * 16 >= w;
+            if (reach && enough) {
+                const int f = wave_find_bucket(hcnt, hmass, w, T);
+                if (lane == 0 && f < kBuckets) atomicMin(&s_bcur, f);
+                since_cnt = 0;
+                since_mass = 0;
+            }
+        }
+    }
+    __syncthreads();
+    // the exact crossing bucket of the final histograms
+    const int bf = find_bucket(hcnt, hmass, 0u, 0u, w, T, sh.wsum, &sh.b, sh.pre);
+    if (tid == 0) {
+        aux[0] = (uint32_t)s_napp;
+        aux[1] = (uint32_t)bf;
+        aux[2] = __float_as_uint(base);
+        aux[3] = s_napp > b.cap ? 1u : 0u;
+    }
+}
+
+__global__ void k_big_flagged(const uint32_t *__restrict__ aux, int64_t nq,
+                              int32_t *__restrict__ list, int32_t *__restrict__ n) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < nq && aux[q * 4 + 3]) list[atomicAdd(n, 1)] = (int32_t)q;
+}
+
+__global__ __launch_bounds__(kRankThreads) void k_rank_big_emit(BigArgs b) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    uint32_t *fstart = reinterpret_cast<uint32_t *>(smem);     // [kFine + 1]
+    uint32_t *fmass = fstart + kFine + 1;                      // [kFine]
+    uint32_t *ffill = fmass + kFine;                           // [kFine]
+    const RankKernelArgs &k = b.k;
+    const RankCellsArgs &a = k.a;
+    __shared__ RankShared sh;
+    __shared__ uint32_t s_lo, s_hi, s_sum;
+    __shared__ int s_count;
+    const int tid = threadIdx.x;
+    const int64_t q = blockIdx.x;
+    const int T = query_budget(a.Tq, q, a.T), w = a.w;
+    if (empty_query(k, q, T)) return;
+    const uint32_t *aux = b.aux + (size_t)q * 4;
+    if (aux[3]) {                      // still over capacity: the device-wide path
+        flag_overflow(k, q);
+        return;
+    }
+    const int n = (int)aux[0];
+    const int bf = (int)aux[1];
+    const float base = __uint_as_float(aux[2]);
+    const uint64_t *list = b.list + (size_t)q * b.cap;
+    uint64_t *S = b.sorted + (size_t)q * b.cap;
+    uint32_t *SZ = b.size + (size_t)q * b.cap;
+    // the key range of the entries up to the cut
+    uint32_t lo = ~0u, hi = 0u;
+    for (int i = tid; i < n; i += kRankThreads) {
+        const uint32_t key = (uint32_t)(list[i] >> 32);
+        if (dbucket(sc_key_float(key), base) <= bf) { lo = min(lo, key); hi = max(hi, key); }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        lo = min(lo, (uint32_t)__shfl_xor(lo, o));
+        hi = max(hi, (uint32_t)__shfl_xor(hi, o));
+    }
+    if (tid == 0) { s_lo = ~0u; s_hi = 0u; s_count = 0; s_sum = 0; }
+    for (int i = tid; i < kFine; i += kRankThreads) { fstart[i] = 0; fmass[i] = 0; ffill[i] = 0; }
+    __syncthreads();
+    if ((tid & 63) == 0) { atomicMin(&s_lo, lo); atomicMax(&s_hi, hi); }
+    __syncthreads();
+    lo = s_lo;
+    hi = s_hi;
+    int shift = 0;
+    while (lo <= hi && ((hi - lo) >> shift) >= (uint32_t)kFine) ++shift;
+    const int64_t k2 = (int64_t)k.kc * k.kc;
+    // fine histogram (counts, sizes) of the kept entries
+    for (int i = tid; i < n; i += kRankThreads) {
+        const uint64_t e = list[i];
+        const uint32_t key = (uint32_t)(e >> 32);
+        if (dbucket(sc_key_float(key), base) > bf) continue;
+        const int64_t c = (int64_t)(uint32_t)e;
+        const int fb = (int)((key - lo) >> shift);
+        atomicAdd(&fstart[fb], 1u);
+        atomicAdd(&fmass[fb], big_size(b, c, b.brec[c]));
+    }
+    __syncthreads();
+    {
+        constexpr int per = kFine / kRankThreads;
+        uint32_t lc = 0, lm = 0;
+        for (int i = 0; i < per; ++i) { lc += fstart[tid * per + i]; lm += fmass[tid * per + i]; }
+        uint32_t tc, tm;
+        uint32_t rc = block_excl_scan(lc, sh.wsum, &tc);
+        uint32_t rm = block_excl_scan(lm, sh.wsum + 4, &tm);
+        for (int i = 0; i < per; ++i) {
+            const uint32_t c = fstart[tid * per + i], m = fmass[tid * per + i];
+            fstart[tid * per + i] = rc;
+            fmass[tid * per + i] = rm;
+            rc += c;
+            rm += m;
+        }
+        if (tid == kRankThreads - 1) fstart[kFine] = rc;
+    }
+    __syncthreads();
+    // scatter into fine-bucket order
+    for (int i = tid; i < n; i += kRankThreads) {
+        const uint64_t e = list[i];
+        const uint32_t key = (uint32_t)(e >> 32);
+        if (dbucket(sc_key_float(key), base) > bf) continue;
+        const int64_t c = (int64_t)(uint32_t)e;
+        const int fb = (int)((key - lo) >> shift);
+        const uint32_t slot = fstart[fb] + atomicAdd(&ffill[fb], 1u);
+        S[slot] = e;
+        SZ[slot] = big_size(b, c, b.brec[c]);
+    }
+    __syncthreads();
+    const int nk = (int)fstart[kFine];
+    int32_t *cells = a.cells + (size_t)q * a.ccap;
+    int32_t *clen = a.clen + (size_t)q * a.ccap;
+    uint32_t *cinfo = a.cinfo + (size_t)q * a.ccap;
+    for (int i = tid; i < nk; i += kRankThreads) {
+        const uint64_t e = S[i];
+        const int fb = (int)(((uint32_t)(e >> 32) - lo) >> shift);
+        const int j0 = (int)fstart[fb], j1 = j0 + (int)ffill[fb];
+        int r = 0;
+        uint32_t p = 0;
+        for (int j = j0; j < j1; ++j) {
+            const bool before = S[j] < e;
+            r += before;
+            p += before ? SZ[j] : 0u;
+        }
+        const int rank = j0 + r;
+        const uint32_t pos = fmass[fb] + p, sz = SZ[i];
+        if (rank < w && (int64_t)pos < T) {
+            if (rank < a.ccap) {
+                const int64_t c = (int64_t)(uint32_t)e;
+                const uint64_t key = b.ckeys[c];
+                const int64_t u0 = (int64_t)(key / (uint64_t)k2), u1 = (int64_t)(key % (uint64_t)k2);
+                cells[rank] = c > 0 ? k.L[c - 1] : 0;
+                clen[rank] = (int32_t)sz;
+                cinfo[rank] = (uint32_t)(u0 / k.kc) | ((uint32_t)(u1 / k.kc) << 16);
+            }
+            atomicMax(&s_count, rank + 1);
+            atomicMax(&s_sum, pos + sz);
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        const int count = s_count;
+        int32_t *qi = a.qinfo + (size_t)q * 8;
+        qi[0] = count; qi[1] = (int32_t)s_sum; qi[2] = count; qi[3] = 0;
+        qi[4] = (count == k.ncells && count < w && (int64_t)s_sum < T) ? 1 : 0;
+        qi[5] = 0; qi[6] = count > a.ccap ? 2 : 0; qi[7] = 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Fallback for queries whose cells do not fit on chip: every cell's
 // (score key << 32 | cell) into global memory, a segmented radix sort per
 // query, and a walk of the sorted keys.
@@ -973,23 +1390,77 @@ int flagged(sc_ctx *ctx, const RankCellsArgs &a, int32_t *list, int32_t *dn, int
     return SC_OK;
 }
 
+// The pruned ranking of all a.nq queries: k_rank_big, a second run of the
+// queries over capacity, and k_rank_big_emit.  Queries still over capacity
+// are flagged (qinfo[5]) for the device-wide path.
+int rank_big(sc_ctx *ctx, const sc_index *ix, const RankKernelArgs &k, const RankCellsArgs &a) {
+    BigArgs b{k,           ix->bg_n0,    ix->bg_pair, ix->bg_start, ix->brec, ix->ckeys,
+              a.big_list,  a.big_sorted, a.big_size,  a.big_aux,    a.big_cap, 0};
+    b.k.qlist = nullptr;
+    const size_t lds = big_lds(ix->kc, ix->dim);
+    if (lds > 150 * 1024) {
+        sc_set_error("sc_search: kc=%d too large for the pruned cell ranking", ix->kc);
+        return SC_ERR_ARG;
+    }
+    SC_HIP(hipFuncSetAttribute((const void *)k_rank_big,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(k_rank_big, dim3((unsigned)a.nq), dim3(kRankThreads), lds, ctx->stream, b);
+    SC_CHECK_LAUNCH();
+    int32_t *list = a.flag_list, *dn = a.flag_count;
+    int32_t nflag = 0;
+    SC_HIP(hipMemsetAsync(dn, 0, 4, ctx->stream));
+    hipLaunchKernelGGL(k_big_flagged, dim3((unsigned)sc_cdiv(a.nq, 256)), dim3(256), 0,
+                       ctx->stream, a.big_aux, a.nq, list, dn);
+    SC_CHECK_LAUNCH();
+    SC_HIP(hipMemcpyAsync(&nflag, dn, 4, hipMemcpyDeviceToHost, ctx->stream));
+    SC_HIP(hipStreamSynchronize(ctx->stream));
+    if (a.overflowed) *a.overflowed += nflag;
+    if (nflag > 0) {
+        BigArgs r = b;
+        r.k.qlist = list;
+        r.rerun = 1;
+        hipLaunchKernelGGL(k_rank_big, dim3((unsigned)nflag), dim3(kRankThreads), lds,
+                           ctx->stream, r);
+        SC_CHECK_LAUNCH();
+    }
+    const size_t lds2 = 4 * (3 * (size_t)kFine + 1);
+    hipLaunchKernelGGL(k_rank_big_emit, dim3((unsigned)a.nq), dim3(kRankThreads), lds2,
+                       ctx->stream, b);
+    SC_CHECK_LAUNCH();
+    return SC_OK;
+}
+
+// SC_RANK_BIG (test hook): 1 ranks every index that has the group directory
+// with the pruned ranking, 0 never uses it.
+int big_mode() {
+    if (const char *e = getenv("SC_RANK_BIG")) return atoi(e) ? 1 : 0;
+    return -1;
+}
+
 template <int W>
 int rank_cells(sc_ctx *ctx, const sc_index *ix, const RankCellsArgs &a) {
     RankKernelArgs k{a, ix->kc, ix->dim, ix->ncells, ix->crec, ix->L, nullptr, rank_cap(),
                      ix->nu[0], ix->nu[1], ix->upairs[0], ix->upairs[1], ix->cpair, ix->gstart};
     const int cap = rank_cap();
-    int rc = launch_rank<W>(ctx, ix, k, nullptr, a.nq, cap);
-    if (rc == SC_ERR_TOO_LARGE) {
-        sc_set_error("sc_search: kc=%d too large for the on-chip cell ranking", ix->kc);
-        return SC_ERR_ARG;
-    }
-    if (rc) return rc;
     int32_t *list = a.flag_list, *dn = a.flag_count;
     int32_t nflag = 0;
-    SC_TRY(flagged(ctx, a, list, dn, &nflag));
+    int rc = SC_OK;
+    if (a.big_cap > 0 && a.big_list) {
+        SC_TRY(rank_big(ctx, ix, k, a));
+        SC_TRY(flagged(ctx, a, list, dn, &nflag));
+    } else {
+        rc = launch_rank<W>(ctx, ix, k, nullptr, a.nq, cap);
+        if (rc == SC_ERR_TOO_LARGE) {
+            sc_set_error("sc_search: kc=%d too large for the on-chip cell ranking", ix->kc);
+            return SC_ERR_ARG;
+        }
+        if (rc) return rc;
+        SC_TRY(flagged(ctx, a, list, dn, &nflag));
+        if (a.overflowed) *a.overflowed += nflag;
+    }
     // queries whose cells did not fit: once more with a list four times as
     // long (fewer workgroups per CU, only for them), when the LDS allows
-    if (nflag > 0 && cap == kCap) {
+    if (nflag > 0 && cap == kCap && !(a.big_cap > 0 && a.big_list)) {
         rc = launch_rank<W>(ctx, ix, k, list, nflag, 4 * kCap);
         if (rc == SC_OK) SC_TRY(flagged(ctx, a, list, dn, &nflag));
         else if (rc != SC_ERR_TOO_LARGE) return rc;
@@ -1045,6 +1516,19 @@ int rank_cells(sc_ctx *ctx, const sc_index *ix, const RankCellsArgs &a) {
 }
 
 }  // namespace
+
+int sc_rank_big_cap(const sc_index *ix) {
+    if (!ix->sparse || !ix->brec || !ix->bg_n0 || ix->kc > 4096 || ix->ncells <= 0) return 0;
+    const int mode = big_mode();
+    if (mode == 0) return 0;
+    if (mode < 0 && use_pair_tables(ix)) return 0;
+    if (big_lds(ix->kc, ix->dim) > 150 * 1024) return 0;
+    if (const char *e = getenv("SC_RANK_BIG_CAP")) {   // test hook: a smaller list
+        const int v = atoi(e);
+        if (v >= 1) return std::min(v, kBigCap);
+    }
+    return kBigCap;
+}
 
 int sc_rank_cells(sc_ctx *ctx, const sc_index *ix, const RankCellsArgs &a) {
     if (a.nq <= 0) return SC_OK;

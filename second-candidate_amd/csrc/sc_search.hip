@@ -2495,7 +2495,8 @@ struct Plan {
     int ccap;
     size_t off_qs, off_dqc, off_dqr, off_sv, off_si, off_cells, off_clen, off_cinfo, off_qinfo,
         off_list, off_stats, off_okey, off_okey2, off_oid, off_oid2, off_perm, off_otmp,
-        otmp_bytes, total;
+        otmp_bytes, total, off_big_list, off_big_sorted, off_big_size, off_big_aux;
+    int big_cap;        // entries per query of the pruned ranking (0: not used)
 };
 
 Plan make_plan(const sc_index *ix, const sc_search_params *p, int64_t nq) {
@@ -2504,11 +2505,15 @@ Plan make_plan(const sc_index *ix, const sc_search_params *p, int64_t nq) {
     int64_t ccap = p->kind == SC_KIND_IVFADC ? p->w : std::min<int64_t>(p->w, (int64_t)p->T + 1);
     if (ccap < 1) ccap = 1;
     P.ccap = (int)ccap;
+    // the pruned ranking of large two-halves indexes: its per-query lists
+    const int big = p->kind == SC_KIND_SCIMI ? sc_rank_big_cap(ix) : 0;
+    P.big_cap = big;
     const size_t per = 16 + 4ull * ix->mc * ix->kc + 4ull * ix->mp * ix->kp +
-                       8ull * lists * ix->kc + 12ull * ccap + 32 + 4;
+                       8ull * lists * ix->kc + 12ull * ccap + 32 + 4 + 20ull * big + 16;
     // up to 4 GiB of per-query scratch (cell lists sized for the budget T),
-    // so that a batch of ten thousand queries runs as one pass
-    int64_t chunk = (int64_t)((size_t)4 << 30) / (int64_t)per;
+    // so that a batch of ten thousand queries runs as one pass (8 GiB with
+    // the pruned ranking's lists)
+    int64_t chunk = (int64_t)((size_t)(big ? 8 : 4) << 30) / (int64_t)per;
     chunk = std::max<int64_t>(1, std::min<int64_t>(chunk, std::max<int64_t>(nq, 1)));
     chunk = std::min<int64_t>(chunk, 65536);
     P.chunk = chunk;
@@ -2531,6 +2536,10 @@ Plan make_plan(const sc_index *ix, const sc_search_params *p, int64_t nq) {
     P.off_oid = take(4ull * chunk);
     P.off_oid2 = take(4ull * chunk);
     P.off_perm = take(4ull * chunk);
+    P.off_big_list = take(8ull * big * chunk);
+    P.off_big_sorted = take(8ull * big * chunk);
+    P.off_big_size = take(4ull * big * chunk);
+    P.off_big_aux = take(16ull * chunk);
     size_t tb = 0;
     (void)rocprim::radix_sort_pairs(nullptr, tb, (uint32_t *)nullptr, (uint32_t *)nullptr,
                                     (int32_t *)nullptr, (int32_t *)nullptr, (size_t)chunk, 0u,
@@ -3102,6 +3111,14 @@ static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
         if (halves) {
             RankCellsArgs ra{Q, dqc, qs, cells, clen, cinfo, qinfo, Tq, cn, p->w, p->T, P.ccap,
                              list, list + P.chunk};
+            if (P.big_cap > 0) {
+                ra.big_list = (uint64_t *)(s + P.off_big_list);
+                ra.big_sorted = (uint64_t *)(s + P.off_big_sorted);
+                ra.big_size = (uint32_t *)(s + P.off_big_size);
+                ra.big_aux = (uint32_t *)(s + P.off_big_aux);
+                ra.big_cap = P.big_cap;
+            }
+            ra.overflowed = &fallback_total;
             SC_TRY(sc_rank_cells(ctx, ix, ra));
         } else if (three) {
             SC_TRY(traverse3(ctx, p, kc, sv, si, ix->L, cells, clen, cinfo, qinfo, P.ccap, cn, Tq));

@@ -212,10 +212,24 @@ class Dev:
     def offset(self, elems):
         return _P(self.ptr.value + int(elems) * self.dtype.itemsize)
 
+    def rows(self, r0, n):
+        """Rows [r0, r0 + n) of the buffer as a Dev that does not own its
+        memory (no copy; it keeps this buffer alive)."""
+        inner = self.shape[1:]
+        per = int(np.prod(inner)) if inner else 1
+        if r0 < 0 or n < 0 or r0 + n > self.shape[0]:
+            raise ValueError("rows: [%d, %d) outside %d rows" % (r0, r0 + n, self.shape[0]))
+        v = Dev.__new__(Dev)
+        v.ctx, v.dtype, v.shape = self.ctx, self.dtype, (int(n),) + tuple(inner)
+        v.nbytes = int(n) * per * self.dtype.itemsize
+        v.ptr = self.offset(int(r0) * per)
+        v._owner = self
+        return v
+
     def free(self):
-        if self.ptr:
+        if self.ptr and getattr(self, "_owner", None) is None:
             lib().sc_free(self.ctx.h, self.ptr)
-            self.ptr = _P()
+        self.ptr = _P()
 
     def __del__(self):
         try:
@@ -239,6 +253,30 @@ def encode(ctx, x, cq, pq, mc, kc, mp, kp, nc, prec=PREC_FP32):
     _check("sc_encode", lib().sc_encode(ctx.h, x.ptr, n, dim, cq.ptr, mc, kc, pq.ptr, mp, kp,
                                         nc, prec, cid.ptr, codes.ptr, C.byref(st)))
     return cid, codes, st
+
+
+def encode_stream(ctx, fill, n, dim, cq, pq, mc, kc, mp, kp, nc, prec=PREC_FP32,
+                  chunk=1_000_000):
+    """Encodes n rows produced a chunk at a time: fill(row0, m, buf) writes
+    rows [row0, row0 + m) into buf (Dev [chunk][dim] float32); only one chunk
+    of rows is ever resident.  Returns (cid Dev [n][mc*nc], codes Dev
+    [n][mp], codewords re-checked exactly, (row, block) pairs evaluated over
+    the whole codebook)."""
+    cid = Dev(ctx, (n, mc * nc), np.uint16)
+    codes = Dev(ctx, (n, mp), np.uint8)
+    buf = Dev(ctx, (max(1, min(chunk, n)), dim), np.float32)
+    st = EncodeStats()
+    rech = exact = 0
+    for r0 in range(0, n, chunk):
+        m = min(chunk, n - r0)
+        fill(r0, m, buf)
+        _check("sc_encode", lib().sc_encode(ctx.h, buf.ptr, m, dim, cq.ptr, mc, kc, pq.ptr, mp,
+                                            kp, nc, prec, cid.offset(r0 * mc * nc),
+                                            codes.offset(r0 * mp), C.byref(st)))
+        rech += st.rechecked
+        exact += st.fallback
+    buf.free()
+    return cid, codes, rech, exact
 
 
 class Index:

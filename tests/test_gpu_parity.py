@@ -319,6 +319,50 @@ def test_search_scimi_small_rank_list(ctx, cap, pairs, monkeypatch):
     ix.close()
 
 
+@pytest.mark.parametrize("case", SCIMI_CASES)
+def test_search_scimi_pruned_ranking_matches_oracle(ctx, case, monkeypatch):
+    """The pruned ranking of large indexes (centroids visited in order of
+    their bound, a falling histogram cut, fine-bucket emission), forced onto
+    the small indexes: ids, distances and candidate counts equal the
+    oracle's, in both forms."""
+    monkeypatch.setenv("SC_RANK_BIG", "1")
+    kc, R, w, T = case
+    ix, oix = _index_pair(ctx, S.KIND_SCIMI, kc=kc)
+    st = _compare_search(ctx, ix, oix, S.KIND_SCIMI, R, w, T, nq=100)
+    _compare_search(ctx, ix, oix, S.KIND_SCIMI, R, w, T, nq=100, form=S.FORM_MIXED)
+    if T == 0:
+        assert st.candidates == 0 and st.cells == 0
+    if kc == 4:
+        assert st.exhausted == 100 and st.candidates == 100 * oix.pid.size
+    ix.close()
+
+
+@pytest.mark.parametrize("cap", ["64", "1"])
+def test_search_scimi_pruned_ranking_overflow(ctx, cap, monkeypatch):
+    """A pruned-ranking list of 64 (or 1) entries per query: queries are run
+    again from their first run's cut, and those still over capacity go to
+    the device-wide sort; results must not change.  Coarse codebooks with
+    duplicated centroids give many equal cell scores."""
+    monkeypatch.setenv("SC_RANK_BIG", "1")
+    monkeypatch.setenv("SC_RANK_BIG_CAP", cap)
+    rng = np.random.default_rng(9)
+    n, dim, kc, mp, kp = 6000, 64, 16, 8, 64
+    x = sco.gen(sco.mix_d128(ncomp=11), sco.STREAM_BASE, 0, n, dim)
+    cq, pq = _codebooks(rng, x, 2, kc, mp, kp, ties=True)
+    dx, dcq, dpq = (S.Dev.from_numpy(ctx, a) for a in (x, cq, pq))
+    cid, codes, _ = S.encode(ctx, dx, dcq, dpq, 2, kc, mp, kp, 2)
+    ocid, ocodes = sco.encode(x, cq, pq, 2, kc, mp, kp, 2)
+    assert np.array_equal(cid.numpy(), ocid) and np.array_equal(codes.numpy(), ocodes)
+    ix = S.Index.build(ctx, cid, codes, dim, dcq, 2, kc, dpq, mp, kp, 2)
+    ck, L, pid, cs = sco.build_sparse(sco.cell_keys(ocid, 2, kc, 2), ocodes)
+    oix = sco.OracleIndex(dim, 2, kc, mp, kp, cq, pq, L, pid, cs, ckeys=ck)
+    for R, w, T in ((100, 100000, 3000), (50, 7, 10 ** 6), (20, 10 ** 6, 10 ** 6), (100, 40, 200)):
+        st = _compare_search(ctx, ix, oix, S.KIND_SCIMI, R, w, T, nq=60)
+        if T == 3000:   # every query's first list outgrows the capacity
+            assert st.heap_fallback == 60
+    ix.close()
+
+
 def test_search_scimi_load_sparse_and_refusals(ctx):
     """sc_index_load_sparse from the oracle's arrays gives the same search;
     a dense load of a two-halves index, unordered keys and an empty cell are
@@ -1045,4 +1089,77 @@ def test_search_per_query_budgets(ctx, kind, kc, nc_ok):
         oi, od, st, _ = oix.search(okind, q[i], 100, w, int(min(max(T2[i], 0), Tmax)))
         assert plan.sums.numpy()[i] == st.sum and np.array_equal(plan.ids.numpy()[i], oi)
     plan.close()
+    ix.close()
+
+
+def test_two_halves_k4096_100m_base(ctx):
+    """The large dense-partition shape: two 64-dim halves with K=4096
+    (nearest, second) cells each, PQ 8x256, a 100M-vector base generated and
+    encoded a 1M-vector chunk at a time (BF16 MFMA distances with the exact
+    re-check), 8e8 bytes of codes.  Sampled rows' cell ids and codes equal
+    the oracle's encode and sit in their key's cell; search (w=100k, T=10k,
+    top-100) of 32 queries served in batches of 1, 4 and 27 equals the
+    oracle (the pruned ranking on the GPU against the oracle's selection
+    ranking), in both forms."""
+    n, dim, kc, mp, kp = 100_000_000, 128, 4096, 8, 256
+    mix = S.mix_d128()
+    # codebooks from 1M training vectors, 4 Lloyd iterations each
+    xt = S.gen_rows(ctx, mix, S.STREAM_TRAIN, 0, 1_000_000, dim)
+    th = xt.numpy()
+    cq = np.stack([S.kmeans(ctx, S.Dev.from_numpy(ctx, np.ascontiguousarray(th[:, h * 64:(h + 1) * 64])),
+                            kc, 4, seed=51 + h, prec=S.PREC_BF16, want_labels=False)[0].numpy()
+                   for h in range(2)])
+    dcq = S.Dev.from_numpy(ctx, cq)
+    S.residual(ctx, xt, dcq, 2, kc, 0)
+    rh = xt.numpy()
+    pq = np.stack([S.kmeans(ctx, S.Dev.from_numpy(ctx, np.ascontiguousarray(rh[:, s * 16:(s + 1) * 16])),
+                            kp, 4, seed=61 + s, prec=S.PREC_BF16, want_labels=False)[0].numpy()
+                   for s in range(mp)])
+    dpq = S.Dev.from_numpy(ctx, pq)
+    xt.free()
+    del th, rh
+
+    def fill(r0, m, buf):
+        S.gen_rows(ctx, mix, S.STREAM_BASE, r0, m, dim, out=buf)
+
+    cid, codes, rech, exact = S.encode_stream(ctx, fill, n, dim, dcq, dpq, 2, kc, mp, kp, 2,
+                                              prec=S.PREC_BF16)
+    print("100M encode: %d codewords re-checked, %d exact (row, block) pairs" % (rech, exact))
+    rows = np.random.default_rng(5).choice(n, 1500, replace=False)
+    xr = np.concatenate([sco.gen(sco.mix_d128(), sco.STREAM_BASE, int(r), 1, dim) for r in rows])
+    ocid, ocodes = sco.encode(xr, cq, pq, 2, kc, mp, kp, 2)
+    cidh, codesh = cid.numpy(), codes.numpy()
+    assert np.array_equal(cidh[rows], ocid) and np.array_equal(codesh[rows], ocodes)
+    ix = S.Index.build(ctx, cid, codes, dim, dcq, 2, kc, dpq, mp, kp, 2)
+    cid.free()
+    codes.free()
+    e = ix.export()
+    ck = ix.cell_keys()
+    L, pid = e["L"], e["pid"]
+    assert ck.size > 0 and np.all(np.diff(ck) > 0) and int(L[-1]) == n
+    assert np.all(np.diff(L) > 0) and L[0] > 0
+    keys = sco.cell_keys(np.ascontiguousarray(cidh[rows]), 2, kc, 2)
+    c = np.searchsorted(ck, keys)
+    assert np.array_equal(ck[c], keys)
+    for r, ci in zip(rows, c):
+        lo = int(L[ci - 1]) if ci > 0 else 0
+        assert r in pid[lo:int(L[ci])]
+        assert np.array_equal(e["codes"][lo + int(np.nonzero(pid[lo:int(L[ci])] == r)[0][0])],
+                              codesh[r])
+    del cidh, codesh
+    print("100M base: %d non-empty cells" % ck.size)
+    oix = sco.OracleIndex(dim, 2, kc, mp, kp, cq, pq, L, pid, e["codes"], ckeys=ck)
+    del e
+    q = sco.gen(sco.mix_d128(), sco.STREAM_QUERY, 0, 32, dim)
+    oids, od, osum = oix.search_batch(sco.SCIMI, q, 100, 100000, 10000)
+    dq = S.Dev.from_numpy(ctx, q)
+    for form in (S.FORM_PLAIN, S.FORM_MIXED):
+        out = [ix.search(dq.rows(a, b - a), S.KIND_SCIMI, 100, 100000, 10000, form=form)
+               for a, b in ((0, 1), (1, 5), (5, 32))]
+        ids = np.concatenate([o[0].numpy() for o in out])
+        dists = np.concatenate([o[1].numpy() for o in out])
+        sums = np.concatenate([o[2].numpy() for o in out])
+        assert np.array_equal(sums, osum)
+        assert np.array_equal(ids, oids)
+        assert np.array_equal(_bits(dists), _bits(od))
     ix.close()

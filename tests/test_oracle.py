@@ -453,10 +453,20 @@ def _scimi_grid_order(ix, q):
     return [pos[int(key[o])] for o in order if int(key[o]) in pos]
 
 
-def test_scimi_ranking_matches_grid_enumeration():
+@pytest.fixture
+def scimi_select(request):
+    """Ranks every two-halves directory by selection (histogram buckets up to
+    the budget, then a sort of those cells only) or by sorting every cell."""
+    sco.lib().sco_scimi_set_select_min(1 if request.param else 1 << 20)
+    yield request.param
+    sco.lib().sco_scimi_set_select_min(1 << 20)
+
+
+@pytest.mark.parametrize("scimi_select", [False, True], indirect=True)
+def test_scimi_ranking_matches_grid_enumeration(scimi_select):
     """Two-halves search: the cells taken are the prefix of the brute-force
     grid order that stops at w cells or T candidates, whatever the budgets
-    (including every cell of the index and T = 0)."""
+    (including every cell of the index and T = 0), by either ranking path."""
     ix, x, cid, codes = _make_index(sco.SCIMI, n=3000, kc=6)
     sizes = np.diff(np.concatenate([[0], ix.L]))
     q = sco.gen(sco.mix_d128(ncomp=9), sco.STREAM_QUERY, 0, 8, ix.dim)
