@@ -224,7 +224,7 @@ typedef struct {
     int64_t queries;
     int64_t candidates;   /* sum over queries of gathered candidates */
     int64_t cells;        /* non-empty cells scanned */
-    int64_t visited;      /* cells popped */
+    int64_t visited;      /* cells popped (two halves: cells scored by the ranking) */
     int64_t reranked;     /* candidates re-ranked exactly (SC_FORM_MIXED) */
     int64_t heap_fallback;/* queries whose heap exceeded on-chip capacity */
     int64_t exhausted;    /* queries whose heap emptied before the budget */

@@ -156,6 +156,7 @@ struct RankCellsArgs {
     uint32_t *big_size, *big_aux;
     int big_cap;
     int64_t *overflowed;     // host, may be null: += queries that outgrew the first list
+    unsigned long long *stats;   // device, may be null: [2] += cells scored
 };
 int sc_rank_cells(sc_ctx *ctx, const sc_index *ix, const RankCellsArgs &a);
 // Entries per query of the pruned ranking's scratch, or 0 when the index is

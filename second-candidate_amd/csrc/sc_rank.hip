@@ -78,6 +78,7 @@ __device__ __forceinline__ int query_budget(const int32_t *Tq, int64_t q, int T)
 
 // Exclusive prefix of one value per thread over the workgroup (256 threads);
 // *total gets the sum.  All threads call it.
+template <int NT = kRankThreads>
 __device__ uint32_t block_excl_scan(uint32_t x, uint32_t *wsum, uint32_t *total) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     uint32_t incl = x;
@@ -89,7 +90,7 @@ __device__ uint32_t block_excl_scan(uint32_t x, uint32_t *wsum, uint32_t *total)
     if (lane == 63) wsum[wv] = incl;
     __syncthreads();
     uint32_t base = 0, all = 0;
-    for (int i = 0; i < kRankThreads / 64; ++i) {
+    for (int i = 0; i < NT / 64; ++i) {
         if (i < wv) base += wsum[i];
         all += wsum[i];
     }
@@ -101,16 +102,17 @@ __device__ uint32_t block_excl_scan(uint32_t x, uint32_t *wsum, uint32_t *total)
 // reaches T, given the counts/sizes before the histogram (pre_cnt, pre_w).
 // Returns kBuckets when no bucket does; *bcnt/*bw get the totals before the
 // found bucket (or of all buckets).
+template <int NT = kRankThreads>
 __device__ int find_bucket(const uint32_t *hcnt, const uint32_t *hwt, uint32_t pre_cnt,
                            uint32_t pre_w, int w, int T, uint32_t *wsum, int *s_b,
                            uint32_t *s_pre) {
-    constexpr int per = kBuckets / kRankThreads;
+    constexpr int per = kBuckets / NT;
     const int t = threadIdx.x;
     uint32_t c = 0, s = 0;
     for (int i = 0; i < per; ++i) { c += hcnt[t * per + i]; s += hwt[t * per + i]; }
     uint32_t tc, ts;
-    const uint32_t bc = block_excl_scan(c, wsum, &tc);
-    const uint32_t bs = block_excl_scan(s, wsum + 4, &ts);
+    const uint32_t bc = block_excl_scan<NT>(c, wsum, &tc);
+    const uint32_t bs = block_excl_scan<NT>(s, wsum + NT / 64, &ts);
     if (t == 0) *s_b = kBuckets;
     __syncthreads();
     uint32_t cc = pre_cnt + bc, cw = pre_w + bs;
@@ -143,7 +145,7 @@ __device__ int find_bucket(const uint32_t *hcnt, const uint32_t *hwt, uint32_t p
 
 // Shared scalars of one workgroup's ranking.
 struct RankShared {
-    uint32_t wsum[8];
+    uint32_t wsum[32];      // two block scans of up to 16 waves
     int b;
     uint32_t pre[2];
     float q[3];
@@ -489,13 +491,14 @@ __device__ void emit_sorted(const Sc &sc, const RankKernelArgs &k, int64_t q, ui
 
 // The query terms of include/multi_query.h:102-112 (per-half sequential sums,
 // q_sum their sum) and v_h[j] = q_h + diff_qc[h*kc + j] in LDS.
+template <int NT = kRankThreads>
 __device__ void query_terms(const RankKernelArgs &k, int64_t q, float *v, RankShared &sh) {
     const RankCellsArgs &a = k.a;
     const int tid = threadIdx.x, kc = k.kc;
     // the query row goes to LDS first (all threads), so that the sequential
     // sums below wait on LDS, not on one global load after another
     float *qrow = v;                               // [dim] <= [2 kc] (checked by the host)
-    for (int j = tid; j < k.dim; j += kRankThreads) qrow[j] = a.Q[(size_t)q * k.dim + j];
+    for (int j = tid; j < k.dim; j += NT) qrow[j] = a.Q[(size_t)q * k.dim + j];
     __syncthreads();
     if (tid == 0) {
         const int bsc = k.dim / 2;
@@ -512,7 +515,7 @@ __device__ void query_terms(const RankKernelArgs &k, int64_t q, float *v, RankSh
     }
     __syncthreads();
     const float *dq = a.dqc + (size_t)q * 2 * kc;
-    for (int j = tid; j < 2 * kc; j += kRankThreads) v[j] = sh.q[j < kc ? 1 : 2] + dq[j];
+    for (int j = tid; j < 2 * kc; j += NT) v[j] = sh.q[j < kc ? 1 : 2] + dq[j];
     __syncthreads();
 }
 
@@ -867,6 +870,7 @@ __global__ __launch_bounds__(kRankThreads) void k_rank_pairs(RankKernelArgs k) {
 // appended); if that still overflows, the device-wide sort finishes it.
 // ---------------------------------------------------------------------------
 constexpr int kBigCap = 32768;       // appended entries per query
+constexpr int kBigThreads = 1024;    // threads of the pruned ranking's workgroups
 constexpr int kFine = 4096;          // fine buckets of the emit kernel
 
 struct BigArgs {
@@ -894,10 +898,11 @@ __device__ __forceinline__ uint32_t big_size(const BigArgs &b, int64_t c, uint32
 
 // The two smallest values of x[0 .. n) (ascending; equal when the least
 // value occurs twice).  All threads call; every thread gets the result.
-__device__ void two_smallest(const float *x, int n, float *scratch8, float *lo, float *hi) {
+template <int NT>
+__device__ void two_smallest(const float *x, int n, float *scratch, float *lo, float *hi) {
     const int tid = threadIdx.x;
     float a = __builtin_huge_valf(), b = __builtin_huge_valf();
-    for (int i = tid; i < n; i += kRankThreads) {
+    for (int i = tid; i < n; i += NT) {
         const float t = x[i];
         if (t < a) { b = a; a = t; } else if (t < b) { b = t; }
     }
@@ -908,12 +913,12 @@ __device__ void two_smallest(const float *x, int n, float *scratch8, float *lo, 
         b = nb;
     }
     __syncthreads();
-    if ((tid & 63) == 0) { scratch8[2 * (tid >> 6)] = a; scratch8[2 * (tid >> 6) + 1] = b; }
+    if ((tid & 63) == 0) { scratch[2 * (tid >> 6)] = a; scratch[2 * (tid >> 6) + 1] = b; }
     __syncthreads();
-    a = scratch8[0];
-    b = scratch8[1];
-    for (int i = 1; i < kRankThreads / 64; ++i) {
-        const float a2 = scratch8[2 * i], b2 = scratch8[2 * i + 1];
+    a = scratch[0];
+    b = scratch[1];
+    for (int i = 1; i < NT / 64; ++i) {
+        const float a2 = scratch[2 * i], b2 = scratch[2 * i + 1];
         const float na = fminf(a, a2), nb = fminf(fmaxf(a, a2), fminf(b, b2));
         a = na;
         b = nb;
@@ -924,18 +929,16 @@ __device__ void two_smallest(const float *x, int n, float *scratch8, float *lo, 
 }
 
 // The crossing bucket of the histograms, read by one wave while other waves
-// may still add to them.  Every value read is at most the count (or size
-// sum) of all the cells of its bucket, so the bucket found is never below
-// the full set's crossing bucket.  kBuckets when the totals read never reach
-// the budget.
-__device__ int wave_find_bucket(const uint32_t *hcnt, const uint32_t *hmass, int w, int T) {
-    constexpr int per = kBuckets / 64;
+// may still add to them: the 64 super-buckets (64 buckets each) locate it,
+// then the buckets of that super-bucket.  A cell is added to its bucket
+// before its super-bucket, and every value read is at most the count (or
+// size sum) of all the cells of its bucket, so the bucket found is never
+// below the full set's crossing bucket.  kBuckets when the totals read never
+// reach the budget.
+__device__ int wave_find_bucket(const uint32_t *hcnt, const uint32_t *hmass,
+                                const uint32_t *scnt, const uint32_t *smass, int w, int T) {
     const int lane = threadIdx.x & 63;
-    uint32_t c = 0, m = 0;
-    for (int i = 0; i < per; ++i) {
-        c += hcnt[lane * per + i];
-        m += hmass[lane * per + i];
-    }
+    const uint32_t c = scnt[lane], m = smass[lane];
     uint32_t ic = c, im = m;
     for (int o = 1; o < 64; o <<= 1) {
         const uint32_t t = __shfl_up(ic, o), u = __shfl_up(im, o);
@@ -944,16 +947,16 @@ __device__ int wave_find_bucket(const uint32_t *hcnt, const uint32_t *hmass, int
     const unsigned long long hit = __ballot((int64_t)ic >= w || (int64_t)im >= T);
     if (hit == 0) return kBuckets;
     const int l0 = __ffsll((long long)hit) - 1;
-    uint32_t cc = __shfl(ic - c, l0), mm = __shfl(im - m, l0);
-    int found = l0 * per + per - 1;
-    if (lane == l0) {
-        for (int i = 0; i < per; ++i) {
-            cc += hcnt[l0 * per + i];
-            mm += hmass[l0 * per + i];
-            if ((int64_t)cc >= w || (int64_t)mm >= T) { found = l0 * per + i; break; }
-        }
+    const uint32_t pc = __shfl(ic - c, l0), pm = __shfl(im - m, l0);
+    uint32_t fc = hcnt[l0 * 64 + lane], fm = hmass[l0 * 64 + lane];
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t t = __shfl_up(fc, o), u = __shfl_up(fm, o);
+        if (lane >= o) { fc += t; fm += u; }
     }
-    return __shfl(found, l0);
+    const unsigned long long hit2 =
+        __ballot((int64_t)(pc + fc) >= w || (int64_t)(pm + fm) >= T);
+    if (hit2 == 0) return kBuckets;
+    return l0 * 64 + __ffsll((long long)hit2) - 1;
 }
 
 // LDS of k_rank_big: the histograms [2][kBuckets], v [max(2 kc, dim)], the
@@ -963,7 +966,7 @@ __host__ __device__ inline size_t big_lds(int kc, int dim) {
     return 8 * (size_t)kBuckets + 4 * nv + ((2 * (size_t)kc + 15) & ~(size_t)15);
 }
 
-__global__ __launch_bounds__(kRankThreads) void k_rank_big(BigArgs b) {
+__global__ __launch_bounds__(kBigThreads, 8) void k_rank_big(BigArgs b) {
     extern __shared__ __align__(16) unsigned char smem[];
     const RankKernelArgs &k = b.k;
     const int kc = k.kc;
@@ -972,33 +975,35 @@ __global__ __launch_bounds__(kRankThreads) void k_rank_big(BigArgs b) {
     float *v = reinterpret_cast<float *>(hmass + kBuckets);       // [max(2 kc, dim)]
     uint16_t *ord = reinterpret_cast<uint16_t *>(v + max(2 * kc, k.dim));   // [kc]
     __shared__ RankShared sh;
-    __shared__ float s_min[8];
+    __shared__ float s_min[2 * kBigThreads / 64];
     __shared__ int s_pos, s_bcur, s_napp;
     __shared__ uint32_t s_cnt, s_mass;
+    __shared__ uint32_t scnt[64], smass[64];      // super-buckets: 64 buckets each
+    __shared__ unsigned long long s_scored;
     const int tid = threadIdx.x, lane = tid & 63;
     const int64_t q = k.qlist ? k.qlist[blockIdx.x] : blockIdx.x;
     const int T = query_budget(k.a.Tq, q, k.a.T), w = k.a.w;
     uint32_t *aux = b.aux + (size_t)q * 4;
     const int start_cut = b.rerun ? (int)aux[1] : kBuckets;
-    query_terms(k, q, v, sh);
+    query_terms<kBigThreads>(k, q, v, sh);
     if (T <= 0 || k.ncells == 0) return;       // the emit kernel writes the empty result
     float m0a, m0b, m1a, m1b;
-    two_smallest(v, kc, s_min, &m0a, &m0b);
-    two_smallest(v + kc, kc, s_min, &m1a, &m1b);
+    two_smallest<kBigThreads>(v, kc, s_min, &m0a, &m0b);
+    two_smallest<kBigThreads>(v + kc, kc, s_min, &m1a, &m1b);
     const float LB1 = m1a + m1b, base = (m0a + m0b) + LB1;
 
     // the nearest-0 centroids in order of their bound's bucket
-    for (int i = tid; i < kBuckets; i += kRankThreads) hcnt[i] = 0;
+    for (int i = tid; i < kBuckets; i += kBigThreads) hcnt[i] = 0;
     __syncthreads();
-    for (int n = tid; n < kc; n += kRankThreads)
+    for (int n = tid; n < kc; n += kBigThreads)
         atomicAdd(&hcnt[dbucket((v[n] + m0a) + LB1, base)], 1u);
     __syncthreads();
     {
-        constexpr int per = kBuckets / kRankThreads;
+        constexpr int per = kBuckets / kBigThreads;
         uint32_t loc = 0;
         for (int i = 0; i < per; ++i) loc += hcnt[tid * per + i];
         uint32_t tot;
-        uint32_t run = block_excl_scan(loc, sh.wsum, &tot);
+        uint32_t run = block_excl_scan<kBigThreads>(loc, sh.wsum, &tot);
         for (int i = 0; i < per; ++i) {
             const uint32_t c = hcnt[tid * per + i];
             hcnt[tid * per + i] = run;
@@ -1006,107 +1011,175 @@ __global__ __launch_bounds__(kRankThreads) void k_rank_big(BigArgs b) {
         }
     }
     __syncthreads();
-    for (int n = tid; n < kc; n += kRankThreads)
+    for (int n = tid; n < kc; n += kBigThreads)
         ord[atomicAdd(&hcnt[dbucket((v[n] + m0a) + LB1, base)], 1u)] = (uint16_t)n;
     __syncthreads();
-    for (int i = tid; i < kBuckets; i += kRankThreads) { hcnt[i] = 0; hmass[i] = 0; }
-    if (tid == 0) { s_pos = 0; s_bcur = start_cut; s_napp = 0; s_cnt = 0; s_mass = 0; }
+    for (int i = tid; i < kBuckets; i += kBigThreads) { hcnt[i] = 0; hmass[i] = 0; }
+    if (tid < 64) { scnt[tid] = 0; smass[tid] = 0; }
+    if (tid == 0) {
+        s_pos = 0; s_bcur = start_cut; s_napp = 0; s_cnt = 0; s_mass = 0; s_scored = 0;
+    }
     __syncthreads();
 
     uint64_t *list = b.list + (size_t)q * b.cap;
     const float *v1 = v + kc;
     volatile int *bcut = &s_bcur;
     uint32_t since_cnt = 0, since_mass = 0;       // this wave's additions since its last cut
-    for (;;) {
-        int i = 0;
-        if (lane == 0) i = atomicAdd(&s_pos, 1);
-        i = __shfl(i, 0);
-        if (i >= kc) break;
-        const int n0 = ord[i];
-        const float a0 = v[n0];
-        // one value of the cut per wave (another wave may lower it meanwhile)
-        int bcur = __shfl(*bcut, 0);
-        if (dbucket((a0 + m0a) + LB1, base) > bcur) break;   // so are the centroids after it
-        const int g0 = b.bg_n0[n0], g1 = b.bg_n0[n0 + 1];
-        uint32_t add_cnt = 0, add_mass = 0;                 // this lane's additions
-        for (int gb = g0; gb < g1; gb += 64) {
-            const int g = gb + lane;
-            float p0 = 0.0f;
-            int c0 = 0, c1 = 0;
-            bool ok = false;
-            if (g < g1) {
-                p0 = a0 + v[b.bg_pair[g] >> 16];
-                ok = dbucket(p0 + LB1, base) <= bcur;
-                if (ok) { c0 = b.bg_start[g]; c1 = b.bg_start[g + 1]; }
+    // Appends the admitted lanes' entries (one LDS atomic per wave).
+    auto append = [&](bool adm, uint32_t key, int c) {
+        const unsigned long long am = __ballot(adm);
+        if (am) {
+            int slot0 = 0;
+            if (lane == 0) slot0 = atomicAdd(&s_napp, (int)__popcll(am));
+            slot0 = __shfl(slot0, 0);
+            if (adm) {
+                const int slot = slot0 + (int)__popcll(am & ((1ull << lane) - 1ull));
+                if (slot < b.cap) list[slot] = ((uint64_t)key << 32) | (uint32_t)c;
             }
-            unsigned long long gm = __ballot(ok);
-            while (gm) {
-                const int j = __ffsll((long long)gm) - 1;
-                gm &= gm - 1;
-                const float P0 = __shfl(p0, j);
-                const int cb = __shfl(c0, j), ce = __shfl(c1, j);
-                bcur = __shfl(*bcut, 0);
-                for (int cs = cb; cs < ce; cs += 64) {
-                    const int c = cs + lane;
-                    bool adm = false;
-                    uint32_t key = 0;
-                    if (c < ce) {
-                        const uint32_t r = b.brec[c];
-                        const float s = P0 + (v1[r & 0xFFFu] + v1[(r >> 12) & 0xFFFu]);
-                        const int bk = dbucket(s, base);
-                        if (bk <= bcur) {
-                            adm = true;
-                            key = sc_float_key(s);
-                            const uint32_t sz = big_size(b, c, r);
-                            atomicAdd(&hcnt[bk], 1u);
-                            atomicAdd(&hmass[bk], sz);
-                            add_cnt += 1;
-                            add_mass += sz;
+        }
+    };
+    constexpr int kU = 4;                          // cell loads in flight per lane
+    constexpr int kWarm = 8;                       // centroids of the warm-up phase
+    // Phase 0 (warm-up, only without a given cut): the best kWarm centroids
+    // counted into the histograms without appending, for a first cut; the
+    // histograms are then cleared and phase 1 starts over from that cut with
+    // appends, so no cell is counted twice.
+    for (int phase = start_cut < kBuckets ? 1 : 0; phase < 2; ++phase) {
+        const bool app = phase == 1;
+        const int lim = app ? kc : min(kWarm, kc);
+        for (;;) {
+            int i = 0;
+            if (lane == 0) i = atomicAdd(&s_pos, 1);
+            i = __shfl(i, 0);
+            if (i >= lim) break;
+            const int n0 = ord[i];
+            const float a0 = v[n0];
+            // one value of the cut per wave (another wave may lower it meanwhile)
+            int bcur = __shfl(*bcut, 0);
+            if (dbucket((a0 + m0a) + LB1, base) > bcur) break;   // so are the centroids after it
+            const int g0 = b.bg_n0[n0], g1 = b.bg_n0[n0 + 1];
+            for (int gb = g0; gb < g1; gb += 64) {
+                // this round's pairs (one per lane) that pass the bound
+                const int g = gb + lane;
+                float p0 = 0.0f;
+                int c0 = 0, c1 = 0;
+                bool ok = false;
+                if (g < g1) {
+                    p0 = a0 + v[b.bg_pair[g] >> 16];
+                    ok = dbucket(p0 + LB1, base) <= bcur;
+                    if (ok) { c0 = b.bg_start[g]; c1 = b.bg_start[g + 1]; }
+                }
+                unsigned long long gm = __ballot(ok);
+                // the pairs one after another, all lanes on a pair's cells; the
+                // first kU * 64 cells of the next pair are loaded while the
+                // current pair is scored
+                uint32_t rn[kU];
+                int jn = gm ? __ffsll((long long)gm) - 1 : -1;
+                if (jn >= 0) {
+                    gm &= gm - 1;
+                    const int cb = __shfl(c0, jn), ce = __shfl(c1, jn);
+#pragma unroll
+                    for (int u = 0; u < kU; ++u) {
+                        const int c = cb + u * 64 + lane;
+                        rn[u] = c < ce ? b.brec[c] : 0u;
+                    }
+                }
+                while (jn >= 0) {
+                    const int j = jn;
+                    const float P0 = __shfl(p0, j);
+                    const int cb = __shfl(c0, j), ce = __shfl(c1, j);
+                    uint32_t r[kU];
+#pragma unroll
+                    for (int u = 0; u < kU; ++u) r[u] = rn[u];
+                    jn = gm ? __ffsll((long long)gm) - 1 : -1;
+                    if (jn >= 0) {
+                        gm &= gm - 1;
+                        const int nb = __shfl(c0, jn), ne = __shfl(c1, jn);
+#pragma unroll
+                        for (int u = 0; u < kU; ++u) {
+                            const int c = nb + u * 64 + lane;
+                            rn[u] = c < ne ? b.brec[c] : 0u;
                         }
                     }
-                    const unsigned long long am = __ballot(adm);
-                    if (am) {
-                        int slot0 = 0;
-                        if (lane == 0) slot0 = atomicAdd(&s_napp, (int)__popcll(am));
-                        slot0 = __shfl(slot0, 0);
-                        if (adm) {
-                            const int slot =
-                                slot0 + (int)__popcll(am & ((1ull << lane) - 1ull));
-                            if (slot < b.cap) list[slot] = ((uint64_t)key << 32) | (uint32_t)c;
+                    if (lane == 0) atomicAdd(&s_scored, (unsigned long long)(ce - cb));
+                    uint32_t add_cnt = 0, add_mass = 0;         // this lane's additions
+                    for (int cs = cb; cs < ce; cs += 64 * kU) {
+                        if (cs > cb) {                          // a pair of more than kU*64 cells
+#pragma unroll
+                            for (int u = 0; u < kU; ++u) {
+                                const int c = cs + u * 64 + lane;
+                                r[u] = c < ce ? b.brec[c] : 0u;
+                            }
+                        }
+#pragma unroll
+                        for (int u = 0; u < kU; ++u) {
+                            if (cs + u * 64 >= ce) break;       // uniform
+                            const int c = cs + u * 64 + lane;
+                            bool adm = false;
+                            uint32_t key = 0;
+                            if (c < ce) {
+                                const float sc =
+                                    P0 + (v1[r[u] & 0xFFFu] + v1[(r[u] >> 12) & 0xFFFu]);
+                                const int bk = dbucket(sc, base);
+                                if (bk <= bcur) {
+                                    adm = true;
+                                    key = sc_float_key(sc);
+                                    const uint32_t sz = big_size(b, c, r[u]);
+                                    atomicAdd(&hcnt[bk], 1u);
+                                    atomicAdd(&hmass[bk], sz);
+                                    atomicAdd(&scnt[bk >> 6], 1u);
+                                    atomicAdd(&smass[bk >> 6], sz);
+                                    add_cnt += 1;
+                                    add_mass += sz;
+                                }
+                            }
+                            if (app) append(adm, key, c);
                         }
                     }
+                    // this pair's additions, then a tighter cut when enough was added
+                    if (!__ballot(add_cnt > 0)) continue;
+                    for (int o = 32; o > 0; o >>= 1) {
+                        add_cnt += __shfl_xor(add_cnt, o);
+                        add_mass += __shfl_xor(add_mass, o);
+                    }
+                    uint32_t tc = 0, tm = 0;
+                    if (lane == 0) {
+                        tc = atomicAdd(&s_cnt, add_cnt) + add_cnt;
+                        tm = atomicAdd(&s_mass, add_mass) + add_mass;
+                    }
+                    tc = __shfl(tc, 0);
+                    tm = __shfl(tm, 0);
+                    since_cnt += add_cnt;
+                    since_mass += add_mass;
+                    const bool reach = (int64_t)tc >= w || (int64_t)tm >= T;
+                    const bool enough = bcur == kBuckets || (int64_t)since_mass * 32 >= T ||
+                                        (int64_t)since_cnt * 32 >= w;
+                    if (reach && enough) {
+                        const int f = wave_find_bucket(hcnt, hmass, scnt, smass, w, T);
+                        if (lane == 0 && f < kBuckets) atomicMin(&s_bcur, f);
+                        since_cnt = 0;
+                        since_mass = 0;
+                    }
+                    bcur = __shfl(*bcut, 0);
                 }
             }
         }
-        // this centroid's additions, then a tighter cut when enough was added
-        for (int o = 32; o > 0; o >>= 1) {
-            add_cnt += __shfl_xor(add_cnt, o);
-            add_mass += __shfl_xor(add_mass, o);
-        }
-        if (add_cnt > 0) {
-            uint32_t tc = 0, tm = 0;
-            if (lane == 0) {
-                tc = atomicAdd(&s_cnt, add_cnt) + add_cnt;
-                tm = atomicAdd(&s_mass, add_mass) + add_mass;
-            }
-            tc = __shfl(tc, 0);
-            tm = __shfl(tm, 0);
-            since_cnt += add_cnt;
-            since_mass += add_mass;
-            const bool reach = (int64_t)tc >= w || (int64_t)tm >= T;
-            const bool enough = __shfl(*bcut, 0) == kBuckets || (int64_t)since_mass * 16 >= T ||
-                                (int64_t)since_cnt * 16 >= w;
-            if (reach && enough) {
-                const int f = wave_find_bucket(hcnt, hmass, w, T);
-                if (lane == 0 && f < kBuckets) atomicMin(&s_bcur, f);
-                since_cnt = 0;
-                since_mass = 0;
-            }
+        if (phase == 0) {
+            // the warm-up's cut stays (its cells are a subset of all cells);
+            // its counts are cleared before the cells are visited again
+            __syncthreads();
+            for (int i = tid; i < kBuckets; i += kBigThreads) { hcnt[i] = 0; hmass[i] = 0; }
+            if (tid < 64) { scnt[tid] = 0; smass[tid] = 0; }
+            if (tid == 0) { s_pos = 0; s_cnt = 0; s_mass = 0; }
+            since_cnt = 0;
+            since_mass = 0;
+            __syncthreads();
         }
     }
     __syncthreads();
     // the exact crossing bucket of the final histograms
-    const int bf = find_bucket(hcnt, hmass, 0u, 0u, w, T, sh.wsum, &sh.b, sh.pre);
+    const int bf = find_bucket<kBigThreads>(hcnt, hmass, 0u, 0u, w, T, sh.wsum, &sh.b, sh.pre);
+    if (tid == 0 && k.a.stats) atomicAdd(k.a.stats + 2, s_scored);
     if (tid == 0) {
         aux[0] = (uint32_t)s_napp;
         aux[1] = (uint32_t)bf;
@@ -1121,7 +1194,7 @@ __global__ void k_big_flagged(const uint32_t *__restrict__ aux, int64_t nq,
     if (q < nq && aux[q * 4 + 3]) list[atomicAdd(n, 1)] = (int32_t)q;
 }
 
-__global__ __launch_bounds__(kRankThreads) void k_rank_big_emit(BigArgs b) {
+__global__ __launch_bounds__(kBigThreads) void k_rank_big_emit(BigArgs b) {
     extern __shared__ __align__(16) unsigned char smem[];
     uint32_t *fstart = reinterpret_cast<uint32_t *>(smem);     // [kFine + 1]
     uint32_t *fmass = fstart + kFine + 1;                      // [kFine]
@@ -1148,7 +1221,7 @@ __global__ __launch_bounds__(kRankThreads) void k_rank_big_emit(BigArgs b) {
     uint32_t *SZ = b.size + (size_t)q * b.cap;
     // the key range of the entries up to the cut
     uint32_t lo = ~0u, hi = 0u;
-    for (int i = tid; i < n; i += kRankThreads) {
+    for (int i = tid; i < n; i += kBigThreads) {
         const uint32_t key = (uint32_t)(list[i] >> 32);
         if (dbucket(sc_key_float(key), base) <= bf) { lo = min(lo, key); hi = max(hi, key); }
     }
@@ -1157,7 +1230,7 @@ __global__ __launch_bounds__(kRankThreads) void k_rank_big_emit(BigArgs b) {
         hi = max(hi, (uint32_t)__shfl_xor(hi, o));
     }
     if (tid == 0) { s_lo = ~0u; s_hi = 0u; s_count = 0; s_sum = 0; }
-    for (int i = tid; i < kFine; i += kRankThreads) { fstart[i] = 0; fmass[i] = 0; ffill[i] = 0; }
+    for (int i = tid; i < kFine; i += kBigThreads) { fstart[i] = 0; fmass[i] = 0; ffill[i] = 0; }
     __syncthreads();
     if ((tid & 63) == 0) { atomicMin(&s_lo, lo); atomicMax(&s_hi, hi); }
     __syncthreads();
@@ -1167,7 +1240,7 @@ __global__ __launch_bounds__(kRankThreads) void k_rank_big_emit(BigArgs b) {
     while (lo <= hi && ((hi - lo) >> shift) >= (uint32_t)kFine) ++shift;
     const int64_t k2 = (int64_t)k.kc * k.kc;
     // fine histogram (counts, sizes) of the kept entries
-    for (int i = tid; i < n; i += kRankThreads) {
+    for (int i = tid; i < n; i += kBigThreads) {
         const uint64_t e = list[i];
         const uint32_t key = (uint32_t)(e >> 32);
         if (dbucket(sc_key_float(key), base) > bf) continue;
@@ -1178,12 +1251,12 @@ __global__ __launch_bounds__(kRankThreads) void k_rank_big_emit(BigArgs b) {
     }
     __syncthreads();
     {
-        constexpr int per = kFine / kRankThreads;
+        constexpr int per = kFine / kBigThreads;
         uint32_t lc = 0, lm = 0;
         for (int i = 0; i < per; ++i) { lc += fstart[tid * per + i]; lm += fmass[tid * per + i]; }
         uint32_t tc, tm;
-        uint32_t rc = block_excl_scan(lc, sh.wsum, &tc);
-        uint32_t rm = block_excl_scan(lm, sh.wsum + 4, &tm);
+        uint32_t rc = block_excl_scan<kBigThreads>(lc, sh.wsum, &tc);
+        uint32_t rm = block_excl_scan<kBigThreads>(lm, sh.wsum + kBigThreads / 64, &tm);
         for (int i = 0; i < per; ++i) {
             const uint32_t c = fstart[tid * per + i], m = fmass[tid * per + i];
             fstart[tid * per + i] = rc;
@@ -1191,11 +1264,11 @@ __global__ __launch_bounds__(kRankThreads) void k_rank_big_emit(BigArgs b) {
             rc += c;
             rm += m;
         }
-        if (tid == kRankThreads - 1) fstart[kFine] = rc;
+        if (tid == kBigThreads - 1) fstart[kFine] = rc;
     }
     __syncthreads();
     // scatter into fine-bucket order
-    for (int i = tid; i < n; i += kRankThreads) {
+    for (int i = tid; i < n; i += kBigThreads) {
         const uint64_t e = list[i];
         const uint32_t key = (uint32_t)(e >> 32);
         if (dbucket(sc_key_float(key), base) > bf) continue;
@@ -1210,7 +1283,7 @@ __global__ __launch_bounds__(kRankThreads) void k_rank_big_emit(BigArgs b) {
     int32_t *cells = a.cells + (size_t)q * a.ccap;
     int32_t *clen = a.clen + (size_t)q * a.ccap;
     uint32_t *cinfo = a.cinfo + (size_t)q * a.ccap;
-    for (int i = tid; i < nk; i += kRankThreads) {
+    for (int i = tid; i < nk; i += kBigThreads) {
         const uint64_t e = S[i];
         const int fb = (int)(((uint32_t)(e >> 32) - lo) >> shift);
         const int j0 = (int)fstart[fb], j1 = j0 + (int)ffill[fb];
@@ -1404,7 +1477,7 @@ int rank_big(sc_ctx *ctx, const sc_index *ix, const RankKernelArgs &k, const Ran
     }
     SC_HIP(hipFuncSetAttribute((const void *)k_rank_big,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(k_rank_big, dim3((unsigned)a.nq), dim3(kRankThreads), lds, ctx->stream, b);
+    hipLaunchKernelGGL(k_rank_big, dim3((unsigned)a.nq), dim3(kBigThreads), lds, ctx->stream, b);
     SC_CHECK_LAUNCH();
     int32_t *list = a.flag_list, *dn = a.flag_count;
     int32_t nflag = 0;
@@ -1419,12 +1492,12 @@ int rank_big(sc_ctx *ctx, const sc_index *ix, const RankKernelArgs &k, const Ran
         BigArgs r = b;
         r.k.qlist = list;
         r.rerun = 1;
-        hipLaunchKernelGGL(k_rank_big, dim3((unsigned)nflag), dim3(kRankThreads), lds,
+        hipLaunchKernelGGL(k_rank_big, dim3((unsigned)nflag), dim3(kBigThreads), lds,
                            ctx->stream, r);
         SC_CHECK_LAUNCH();
     }
     const size_t lds2 = 4 * (3 * (size_t)kFine + 1);
-    hipLaunchKernelGGL(k_rank_big_emit, dim3((unsigned)a.nq), dim3(kRankThreads), lds2,
+    hipLaunchKernelGGL(k_rank_big_emit, dim3((unsigned)a.nq), dim3(kBigThreads), lds2,
                        ctx->stream, b);
     SC_CHECK_LAUNCH();
     return SC_OK;

@@ -3119,6 +3119,7 @@ static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
                 ra.big_cap = P.big_cap;
             }
             ra.overflowed = &fallback_total;
+            ra.stats = st ? stats : nullptr;
             SC_TRY(sc_rank_cells(ctx, ix, ra));
         } else if (three) {
             SC_TRY(traverse3(ctx, p, kc, sv, si, ix->L, cells, clen, cinfo, qinfo, P.ccap, cn, Tq));
