@@ -116,6 +116,13 @@ def main():
     ap.add_argument("--side", type=int, default=1,
                     help="1: side lines (one-block SC index, K=1024 multi-index, real distances, "
                          "k-means)")
+    ap.add_argument("--large", type=int, default=1,
+                    help="1: the large-index side line (10M training vectors, K=4096 per half, "
+                         "100M base)")
+    ap.add_argument("--n-large", type=int, default=100_000_000)
+    ap.add_argument("--ntrain-large", type=int, default=10_000_000)
+    ap.add_argument("--iters-large", type=int, default=20)
+    ap.add_argument("--recall-large", type=int, default=1000)
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="only run a plain device copy of this many MiB (HBM counter scale)")
     args = ap.parse_args()
@@ -299,6 +306,8 @@ def main():
     side = {}
     if args.side:
         side = side_lines(S, ctx, mix, x, q, gt, args, dim, mp, kp)
+    if args.large:
+        side["large_two_halves_k4096"] = large_line(S, ctx, args)
 
     encode_flops = 2.0 * args.n * (kc * dim + mp * kp * (dim // mp))
     out = {
@@ -534,6 +543,146 @@ def d960_line(S, ctx, args):
     ix.close()
     del x, q
     log("D=960 IMI: %s" % row)
+    return row
+
+
+def large_line(S, ctx, args):
+    """The large dense-partition index on one GPU: D=128 mixture, codebooks
+    trained on 10M vectors (k-means++ then 20 Lloyd iterations per codebook,
+    the assignment on BF16 MFMA with the exact re-check, FP64 centre sums),
+    two 64-dim halves with K=4096 (nearest, second) cells each, PQ 8x256, a
+    100M-vector base generated and encoded a 1M-vector chunk at a time (8e8
+    bytes of codes, beyond the 256 MB MALL), 10k queries served in batches of
+    4096, 64 and 1 (2 x (4096 + 14 x 64 + 8 x 1)), T=10k, top-100, both forms;
+    recall@100 against exact nearest neighbours over the whole base (chunked)
+    on a sample of the queries."""
+    dim, kc, mp, kp, R, T, w = 128, 4096, 8, 256, 100, 10_000, 100_000
+    n, ntrain, nq = args.n_large, args.ntrain_large, 10_000
+    mix = S.mix_d128()
+    row = {"workload": "two 64-dim halves, K=4096 (nearest, second) per half, PQ 8x256, "
+                       "%d base streamed in 1M chunks, %d training vectors, %d queries in "
+                       "batches of 4096/64/1, T=%d, w=%d, top-%d" % (n, ntrain, nq, T, w, R)}
+    # ---- training (untimed setup; reported)
+    t0 = time.perf_counter()
+    xt = S.gen_rows(ctx, mix, S.STREAM_TRAIN, 0, ntrain, dim)
+    th = xt.numpy()
+    t_km = 0.0
+    cq = []
+    for h in range(2):
+        xh = S.Dev.from_numpy(ctx, np.ascontiguousarray(th[:, h * 64:(h + 1) * 64]))
+        ctx.sync()
+        t1 = time.perf_counter()
+        c, _ = S.kmeans(ctx, xh, kc, args.iters_large, seed=71 + h, prec=S.PREC_BF16,
+                        want_labels=False)
+        ctx.sync()
+        t_km += time.perf_counter() - t1
+        cq.append(c.numpy())
+        xh.free()
+    cq = np.stack(cq)
+    dcq = S.Dev.from_numpy(ctx, cq)
+    S.residual(ctx, xt, dcq, 2, kc, 0)
+    rh = xt.numpy()
+    xt.free()
+    pq = np.stack([S.kmeans(ctx, S.Dev.from_numpy(ctx, np.ascontiguousarray(rh[:, s * 16:(s + 1) * 16])),
+                            kp, args.iters_large, seed=81 + s, prec=S.PREC_BF16,
+                            want_labels=False)[0].numpy() for s in range(mp)])
+    dpq = S.Dev.from_numpy(ctx, pq)
+    del th, rh
+    ctx.sync()
+    row["train_s"] = round(time.perf_counter() - t0, 2)
+    row["coarse_kmeans_s"] = round(t_km, 2)
+    log("large: trained in %.1fs (coarse k-means %.1fs)" % (row["train_s"], t_km))
+    # ---- streamed encode and build
+    def fill(r0, m, buf):
+        S.gen_rows(ctx, mix, S.STREAM_BASE, r0, m, dim, out=buf)
+    t0 = time.perf_counter()
+    cid, codes, rech, exact = S.encode_stream(ctx, fill, n, dim, dcq, dpq, 2, kc, mp, kp, 2,
+                                              prec=S.PREC_BF16)
+    ctx.sync()
+    t_enc = time.perf_counter() - t0
+    enc_flops = 2.0 * n * (kc * dim + mp * kp * (dim // mp))
+    row["encode_s"] = round(t_enc, 2)
+    row["encode_algorithmic_tflops_bf16"] = round(enc_flops / t_enc / 1e12, 2)
+    row["encode_rechecked_per_vector"] = round(rech / n, 2)
+    t0 = time.perf_counter()
+    ix = S.Index.build(ctx, cid, codes, dim, dcq, 2, kc, dpq, mp, kp, 2)
+    ctx.sync()
+    row["build_s"] = round(time.perf_counter() - t0, 3)
+    cid.free()
+    codes.free()
+    row["non_empty_cells"] = int(ix.shape()[1])
+    log("large: encoded %d in %.1fs, %d cells" % (n, t_enc, row["non_empty_cells"]))
+    # ---- queries in mixed batch sizes, both forms
+    q = S.gen_rows(ctx, mix, S.STREAM_QUERY, 0, nq, dim)
+    sched = ([4096] + [64] * 14 + [1] * 8) * 2
+    assert sum(sched) == nq
+    ids = S.Dev(ctx, (nq, R), np.int32)
+    dists = S.Dev(ctx, (nq, R), np.float32)
+    sums = S.Dev(ctx, (nq,), np.int32)
+    res = {}
+    for form, name in ((S.FORM_PLAIN, "plain"), (S.FORM_MIXED, "mixed")):
+        def serve():
+            lat = {}
+            q0 = 0
+            for bs in sched:
+                t1 = time.perf_counter()
+                ix.search(q.rows(q0, bs), S.KIND_SCIMI, R, w, T, form=form, ids=ids.rows(q0, bs),
+                          dists=dists.rows(q0, bs), sums=sums.rows(q0, bs), stats=False)
+                ctx.sync()
+                lat.setdefault(bs, []).append(time.perf_counter() - t1)
+                q0 += bs
+            return lat
+        serve()                      # warm-up
+        ctx.sync()
+        t0 = time.perf_counter()
+        lat = serve()
+        dt = time.perf_counter() - t0
+        st = ix.search(q, S.KIND_SCIMI, R, w, T, form=form, stats=True)[3]
+        res[name] = (ids.numpy().copy(), dists.numpy().copy(), sums.numpy().copy())
+        cand = float(res[name][2].astype(np.int64).sum())
+        algo = cand * (mp + 4) + nq * dim * 4 + nq * R * 8
+        row["qps_" + name] = round(nq / dt, 1)
+        row["batch_ms_" + name] = {str(bs): round(1e3 * float(np.median(v)), 3)
+                                   for bs, v in sorted(lat.items())}
+        row["stage_ms_" + name + "_one_batch_of_10k"] = {
+            "lut": round(st.ms_lut, 3), "rank": round(st.ms_traverse, 3),
+            "scan": round(st.ms_scan, 3)}
+        row["roofline_scan_" + name] = {
+            "bound": "hbm", "achieved": round(algo / (st.ms_scan * 1e-3) / 1e9, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(algo / (st.ms_scan * 1e-3) / 8e12, 4),
+            "algorithmic_bytes_per_launch": int(algo)}
+        row["rank_cells_scored_per_query"] = round(st.visited / nq, 1)
+        row["rank_gb_per_s"] = round(st.visited * 4 / (st.ms_traverse * 1e-3) / 1e9, 1)
+        log("large %s: %.0f QPS, %s" % (name, nq / dt, row["batch_ms_" + name]))
+    row["mixed_over_plain"] = round(row["qps_mixed"] / row["qps_plain"], 3)
+    row["identical"] = bool(all(np.array_equal(a.view(np.uint32) if a.dtype == np.float32 else a,
+                                               b.view(np.uint32) if b.dtype == np.float32 else b)
+                                for a, b in zip(res["plain"], res["mixed"])))
+    row["candidates_per_query"] = round(float(res["plain"][2].mean()), 1)
+    # ---- recall@100 on a sample: exact nearest neighbours over the whole base,
+    # one regenerated 1M chunk at a time (ties to the lowest id)
+    gs = args.recall_large
+    if gs > 0:
+        qs = S.Dev.from_numpy(ctx, q.numpy()[:gs])
+        best_d = np.full(gs, np.inf)
+        best_i = np.full(gs, -1, np.int64)
+        buf = S.Dev(ctx, (1_000_000, dim), np.float32)
+        t0 = time.perf_counter()
+        for r0 in range(0, n, 1_000_000):
+            m = min(1_000_000, n - r0)
+            S.gen_rows(ctx, mix, S.STREAM_BASE, r0, m, dim, out=buf)
+            gi, gd = S.groundtruth_nn1(ctx, buf.rows(0, m), qs)
+            gi, gd = gi.numpy(), gd.numpy()
+            better = gd < best_d
+            best_d[better] = gd[better]
+            best_i[better] = gi[better] + r0
+        row["recall_at_100_sample"] = round(recall_at(best_i, res["mixed"][0][:gs], R), 4)
+        row["recall_queries"] = gs
+        row["groundtruth_s"] = round(time.perf_counter() - t0, 2)
+        buf.free()
+    ix.close()
+    log("large: %s" % row)
     return row
 
 
