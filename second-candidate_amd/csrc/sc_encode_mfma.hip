@@ -53,14 +53,22 @@ __device__ __forceinline__ float bf16_val(unsigned short b) {
 // Per-block codebook preparation: shifted, split BF16 codewords, FP64 norms
 // of the shifted codewords and their maximum per block.
 // mean[b][t] = round(mean_k c[b][k][t]) when `shift`, else 0.
-__global__ void k_cb_mean(const float *__restrict__ cb, int nb, int K, int blen, int shift,
-                          float *__restrict__ mean) {
-    int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= nb * blen) return;
-    int b = e / blen, t = e % blen;
+// One workgroup per (block, dim): strided FP64 partial sums, then a fixed
+// tree, so the mean does not depend on scheduling.
+__global__ __launch_bounds__(256) void k_cb_mean(const float *__restrict__ cb, int nb, int K,
+                                                 int blen, int shift, float *__restrict__ mean) {
+    __shared__ double part[256];
+    const int e = blockIdx.x;                 // b * blen + t
+    const int b = e / blen, t = e % blen;
     double s = 0.0;
-    for (int k = 0; k < K; ++k) s += cb[((size_t)b * K + k) * blen + t];
-    mean[e] = shift ? (float)rint(s / K) : 0.0f;
+    for (int k = threadIdx.x; k < K; k += 256) s += cb[((size_t)b * K + k) * blen + t];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) mean[e] = shift ? (float)rint(part[0] / K) : 0.0f;
 }
 
 __global__ void k_cb_norms(const float *__restrict__ cb, int nb, int K, int blen,
@@ -341,7 +349,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8))) void
     const uint4 *__restrict__ pfrag,
     const double *__restrict__ pn2, const unsigned long long *__restrict__ pmax,
     const float *__restrict__ pmean, uint16_t *__restrict__ cid, uint8_t *__restrict__ codes,
-    unsigned long long *__restrict__ stats) {
+    unsigned long long *__restrict__ stats, int coarse_given) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int ld = dim + 1;
     float *xs = reinterpret_cast<float *>(smem);                          // [16][dim+1]
@@ -372,14 +380,18 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8))) void
         CbView cv{cq + (size_t)j * kc * bsc, cfrag + (size_t)j * ((kc + 15) / 16) * (bsc / 16) * 64,
                   cn2 + (size_t)j * kc,
                   __builtin_bit_cast(double, cmax[j]), cmean + (size_t)j * bsc, kc, bsc};
-        wave_best<NC == 2, NC == 2>(xs, ld, j * bsc, nr, cv, cnt, ck, cd, rown, sm, o1,
-                                    NC == 2 ? o2 : nullptr, s_wst);
-        if (l < nr) {
-            if (NC == 2) {
-                cid[(size_t)(v0 + l) * 2 * mc + j] = (uint16_t)o1[l];
-                cid[(size_t)(v0 + l) * 2 * mc + mc + j] = (uint16_t)o2[l];
-            } else {
-                cid[(size_t)(v0 + l) * mc + j] = (uint16_t)o1[l];
+        if (coarse_given) {          // the nearest centroids are in cid already
+            if (l < nr) o1[l] = cid[(size_t)(v0 + l) * NC * mc + j];
+        } else {
+            wave_best<NC == 2, NC == 2>(xs, ld, j * bsc, nr, cv, cnt, ck, cd, rown, sm, o1,
+                                        NC == 2 ? o2 : nullptr, s_wst);
+            if (l < nr) {
+                if (NC == 2) {
+                    cid[(size_t)(v0 + l) * 2 * mc + j] = (uint16_t)o1[l];
+                    cid[(size_t)(v0 + l) * 2 * mc + mc + j] = (uint16_t)o2[l];
+                } else {
+                    cid[(size_t)(v0 + l) * mc + j] = (uint16_t)o1[l];
+                }
             }
         }
         __syncthreads();
@@ -447,6 +459,312 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8))) void
     }
 }
 
+// ---------------------------------------------------------------------------
+// Tiled assignment (block lengths 32 and 64): the same approximate distances,
+// bound and exact re-check as wave_best, organised for the matrix cores.
+// A 256-thread workgroup takes 128 rows (32 per wave, two 16-row tiles); each
+// wave keeps its rows' BF16 hi/lo A operands in registers for the whole
+// block, and the codebook streams through LDS once per workgroup, 128
+// codewords (eight 16-codeword tiles) at a time, in the operand order of
+// v_mfma_f32_16x16x32_bf16 (lane l holds A[l % 16][8 (l / 16) + j] and
+// B[8 (l / 16) + j][l % 16] for j < 8, and receives C[4 (l / 16) + i][l % 16]
+// for i < 4).  Every B fragment read from LDS feeds both row tiles.
+// ---------------------------------------------------------------------------
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+constexpr int kTRows = 128;        // rows per workgroup
+constexpr int kTWaveRows = 32;     // rows per wave (two 16-row tiles)
+constexpr int kTChunk = 8;         // codeword tiles (of 16) per LDS chunk
+constexpr int kTCand = 24;         // exact re-check slots per row
+
+// Fragments of a block for the tiled kernel: [KTp][KS][2][64] uint4 with
+// KTp = KT rounded up to a multiple of kTChunk (zero past K), KS = blen / 32;
+// lane l of (tile kt, step ks) holds codeword k = 16 kt + l % 16, dims
+// t = 32 ks + 8 (l / 16) + j, j < 8: the BF16 hi parts ([0]) and lo parts ([1]).
+__global__ void k_cb_frag32(const float *__restrict__ cb, int nb, int K, int blen,
+                            const float *__restrict__ mean, uint4 *__restrict__ frag) {
+    const int KTp = ((K + 15) / 16 + kTChunk - 1) / kTChunk * kTChunk, KS = blen / 32;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)nb * KTp * KS * 64) return;
+    const int l = (int)(e % 64);
+    const int64_t f = e / 64;
+    const int ks = (int)(f % KS), kt = (int)((f / KS) % KTp), b = (int)(f / ((int64_t)KS * KTp));
+    const int k = 16 * kt + (l & 15), t = 32 * ks + 8 * (l >> 4);
+    uint32_t hw[4], lw[4];
+    for (int j = 0; j < 8; j += 2) {
+        unsigned short h[2], lo[2];
+        for (int u = 0; u < 2; ++u) {
+            const float v = k < K ? cb[((size_t)b * K + k) * blen + t + j + u] -
+                                        mean[(size_t)b * blen + t + j + u]
+                                  : 0.0f;
+            h[u] = bf16_bits(v);
+            lo[u] = bf16_bits(v - bf16_val(h[u]));
+        }
+        hw[j / 2] = h[0] | ((uint32_t)h[1] << 16);
+        lw[j / 2] = lo[0] | ((uint32_t)lo[1] << 16);
+    }
+    uint4 *o = frag + ((((size_t)b * KTp + kt) * KS + ks) * 2) * 64 + l;
+    o[0] = make_uint4(hw[0], hw[1], hw[2], hw[3]);
+    o[64] = make_uint4(lw[0], lw[1], lw[2], lw[3]);
+}
+
+struct TileOut {
+    uint16_t *cid;          // encode: cid[row * stride + off1] (and + off2 for the second)
+    int stride, off1, off2;
+    int32_t *labels;        // k-means labels (cid == null)
+};
+
+// One coarse block of the rows [128 blockIdx.x, +128): best one (TOP2 false)
+// or two (TOP2 true) codewords of the block's codebook, the order of
+// wave_best (DESC, ROUNDF as there).  KS = blen / 32 (1 or 2).
+template <int KS, bool TOP2, bool DESC, bool ROUNDF>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_assign_tiles(
+    const float *__restrict__ x, int64_t n, int ldx, int t0, const float *__restrict__ c,
+    const uint4 *__restrict__ frag, const double *__restrict__ cn2,
+    const unsigned long long *__restrict__ cmaxb, const float *__restrict__ mean, int K,
+    TileOut out, unsigned long long *__restrict__ stats) {
+    constexpr int blen = 32 * KS;
+    constexpr int kChunkU4 = kTChunk * KS * 2 * 64;          // uint4 per chunk
+    // the codeword chunk; after the last chunk, the re-check distances
+    constexpr int kBufU4 = kChunkU4 > kTRows * kTCand / 2 ? kChunkU4 : kTRows * kTCand / 2;
+    __shared__ uint4 bufB[kBufU4];
+    __shared__ float c2s[kTChunk * 16];
+    __shared__ int cnt[kTRows];
+    __shared__ int ck[kTRows * kTCand];
+    __shared__ float sm[blen];
+    __shared__ unsigned s_wst[2];
+    const int tid = threadIdx.x, l = tid & 63, wv = tid >> 6;
+    const int64_t r0 = (int64_t)blockIdx.x * kTRows + wv * kTWaveRows;   // this wave's rows
+    const int nr = (int)max((int64_t)0, min((int64_t)kTWaveRows, n - r0));
+    for (int t = tid; t < blen; t += 256) sm[t] = mean[t];
+    for (int i = tid; i < kTRows; i += 256) cnt[i] = 0;
+    if (tid == 0) { s_wst[0] = 0u; s_wst[1] = 0u; }
+    __syncthreads();
+    int *wcnt = cnt + wv * kTWaveRows;
+    int *wck = ck + wv * kTWaveRows * kTCand;
+    // A operands (shifted, split) and FP64 norms of this wave's rows
+    bf16x8 ah[2][KS], al[2][KS];
+    double nrm[2];
+    bool exact_local = true;
+#pragma unroll
+    for (int rt = 0; rt < 2; ++rt) {
+        const int rr = rt * 16 + (l & 15);
+        const bool ok = rr < nr;
+        const float *xr = x + (size_t)(r0 + (ok ? rr : 0)) * ldx + t0;
+        double s = 0.0;
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int t = 32 * ks + 8 * (l >> 4) + j;
+                const float v = ok ? xr[t] - sm[t] : 0.0f;
+                const unsigned short h = bf16_bits(v);
+                ah[rt][ks][j] = __builtin_bit_cast(__bf16, h);
+                al[rt][ks][j] = __builtin_bit_cast(__bf16, bf16_bits(v - bf16_val(h)));
+                exact_local &= (bf16_val(h) == v);
+                s += (double)v * (double)v;
+            }
+        }
+        s += __shfl_xor(s, 16);
+        s += __shfl_xor(s, 32);
+        nrm[rt] = s;                      // the norm of row rt * 16 + l % 16
+    }
+    const bool x_exact = __all(exact_local);
+    // this lane's C rows: rt * 16 + 4 (l / 16) + i
+    const int col = l & 15, q4 = 4 * (l >> 4);
+    float rf[2][4], E[2][4], m1[2][4], m2[2][4];
+    const double u = 1.0 / 8388608.0, nn = 3.0 * blen + 8.0, gam = nn * u / (1.0 - nn * u);
+    const double C = __builtin_bit_cast(double, cmaxb[0]);
+#pragma unroll
+    for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const double rn = __shfl(nrm[rt], q4 + i);
+            const double X = sqrt(rn);
+            const double e = 2.0 * (2.0 * X * C * ((3.0 / 65536.0 + gam) * (1.0 + 1.0 / 128.0)) +
+                                    (X + C) * (X + C) / 4194304.0) +
+                             (X + C) * (X + C) / 4194304.0 + 1e-6;
+            E[rt][i] = nextafterf((float)e, INFINITY);     // rounded up: still a bound
+            rf[rt][i] = (float)rn;
+            m1[rt][i] = INFINITY;
+            m2[rt][i] = INFINITY;
+        }
+    const int KT = (K + 15) / 16;
+    const int KTp = (KT + kTChunk - 1) / kTChunk * kTChunk;
+    for (int kc0 = 0; kc0 < KTp; kc0 += kTChunk) {
+        __syncthreads();                                      // the previous chunk is consumed
+        const uint4 *src = frag + (size_t)kc0 * KS * 2 * 64;
+        for (int e = tid; e < kChunkU4; e += 256) bufB[e] = src[e];
+        for (int e = tid; e < kTChunk * 16; e += 256) {
+            const int k = 16 * kc0 + e;
+            c2s[e] = k < K ? (float)cn2[k] : 0.0f;
+        }
+        __syncthreads();
+        f32x4 acc[kTChunk][2];
+#pragma unroll
+        for (int g = 0; g < kTChunk; ++g) {
+            acc[g][0] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+            acc[g][1] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        }
+#pragma unroll
+        for (int g = 0; g < kTChunk; ++g) {
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks) {
+                const uint4 fh = bufB[((g * KS + ks) * 2) * 64 + l];
+                const uint4 fl = bufB[((g * KS + ks) * 2 + 1) * 64 + l];
+                const bf16x8 bh = __builtin_bit_cast(bf16x8, fh);
+                const bf16x8 bl = __builtin_bit_cast(bf16x8, fl);
+#pragma unroll
+                for (int rt = 0; rt < 2; ++rt) {
+                    acc[g][rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rt][ks], bh, acc[g][rt], 0, 0, 0);
+                    acc[g][rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rt][ks], bl, acc[g][rt], 0, 0, 0);
+                    if (!x_exact)
+                        acc[g][rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[rt][ks], bh, acc[g][rt], 0, 0, 0);
+                }
+            }
+        }
+        // d~ minus the row's norm (the same for all of a row's codewords, so
+        // it changes neither the order nor the differences the threshold
+        // uses; fewer roundings keep E a bound) in place, +inf past K; the
+        // chunk's two smallest per row, the running pair and the threshold,
+        // then the candidates
+        const bool full = 16 * (kc0 + kTChunk) <= K;          // uniform
+#pragma unroll
+        for (int rt = 0; rt < 2; ++rt) {
+            float thr[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                float g1 = INFINITY, g2 = INFINITY;
+#pragma unroll
+                for (int g = 0; g < kTChunk; ++g) {
+                    const float d0 = c2s[g * 16 + col] - 2.0f * acc[g][rt][i];
+                    const float d = full || 16 * (kc0 + g) + col < K ? d0 : INFINITY;
+                    acc[g][rt][i] = d;
+                    if (d < g1) { g2 = g1; g1 = d; } else if (d < g2) { g2 = d; }
+                }
+                for (int o = 1; o < 16; o <<= 1) {
+                    const float a1 = __shfl_xor(g1, o), a2 = __shfl_xor(g2, o);
+                    const float lo1 = fminf(g1, a1), hi1 = fmaxf(g1, a1);
+                    g2 = fminf(hi1, fminf(g2, a2));
+                    g1 = lo1;
+                }
+                const float lo1 = fminf(m1[rt][i], g1), hi1 = fmaxf(m1[rt][i], g1);
+                m2[rt][i] = fminf(hi1, fminf(m2[rt][i], g2));
+                m1[rt][i] = lo1;
+                const double mref = (double)(TOP2 ? m2[rt][i] : m1[rt][i]);
+                // ROUNDF: the slack for float-rounded distances scales with
+                // the full distance, row norm included
+                thr[i] = nextafterf((float)(mref + 2.0 * (double)E[rt][i] +
+                                            (ROUNDF ? (fabs(mref + (double)rf[rt][i]) +
+                                                       (double)E[rt][i]) *
+                                                          (1.0 / 4194304.0)
+                                                    : 0.0)),
+                                    INFINITY);
+            }
+#pragma unroll
+            for (int g = 0; g < kTChunk; ++g) {
+                const int k = 16 * (kc0 + g) + col;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int row = rt * 16 + q4 + i;
+                    if (acc[g][rt][i] <= thr[i] && row < nr && (full || k < K)) {
+                        const int sl = atomicAdd(&wcnt[row], 1);
+                        if (sl < kTCand) wck[row * kTCand + sl] = k;
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // exact re-check: the candidates' S1 distances (FP64) into the chunk
+    // buffer, then each row's best one or two under the reference's order
+    double *cd = reinterpret_cast<double *>(bufB) + wv * kTWaveRows * kTCand;
+    int total = 0;
+    for (int r = 0; r < nr; ++r) total += wcnt[r] <= kTCand ? wcnt[r] : 0;
+    for (int p = l; p < total; p += 64) {
+        int r = 0, sp = p;
+        while (true) {
+            const int cc = wcnt[r] <= kTCand ? wcnt[r] : 0;
+            if (sp < cc) break;
+            sp -= cc;
+            ++r;
+        }
+        const int k = wck[r * kTCand + sp];
+        const double e = s1(x + (size_t)(r0 + r) * ldx + t0, c + (size_t)k * blen, blen);
+        cd[r * kTCand + sp] = ROUNDF ? (double)(float)e : e;
+    }
+    __syncthreads();
+    if (l < nr) {
+        double d1 = INFINITY, d2 = INFINITY;
+        int k1 = -1, k2 = -1;
+        const int cc = wcnt[l];
+        auto push = [&](double d, int k) {
+            if (precedes<DESC>(d, k, d1, k1)) { d2 = d1; k2 = k1; d1 = d; k1 = k; }
+            else if (precedes<DESC>(d, k, d2, k2)) { d2 = d; k2 = k; }
+        };
+        const float *xr = x + (size_t)(r0 + l) * ldx + t0;
+        if (cc <= kTCand) {
+            for (int sp = 0; sp < cc; ++sp) push(cd[l * kTCand + sp], wck[l * kTCand + sp]);
+            atomicAdd(&s_wst[0], (unsigned)cc);
+        } else {
+            for (int k = 0; k < K; ++k) {
+                const double e = s1(xr, c + (size_t)k * blen, blen);
+                push(ROUNDF ? (double)(float)e : e, k);
+            }
+            atomicAdd(&s_wst[1], 1u);
+        }
+        const int64_t row = r0 + l;
+        if (out.cid) {
+            out.cid[(size_t)row * out.stride + out.off1] = (uint16_t)k1;
+            if (TOP2) out.cid[(size_t)row * out.stride + out.off2] = (uint16_t)k2;
+        } else {
+            out.labels[row] = k1;
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        if (s_wst[0]) atomicAdd(&stats[0], (unsigned long long)s_wst[0]);
+        if (s_wst[1]) atomicAdd(&stats[1], (unsigned long long)s_wst[1]);
+    }
+}
+
+template <bool TOP2, bool DESC, bool ROUNDF>
+int launch_tiles(sc_ctx *ctx, const float *x, int64_t n, int ldx, int t0, int blen,
+                 const float *c, const uint4 *frag, const double *cn2,
+                 const unsigned long long *cmax, const float *mean, int K, TileOut out,
+                 unsigned long long *stats) {
+    const unsigned g = (unsigned)sc_cdiv(n, kTRows);
+    if (blen == 64)
+        hipLaunchKernelGGL((k_assign_tiles<2, TOP2, DESC, ROUNDF>), dim3(g), dim3(256), 0,
+                           ctx->stream, x, n, ldx, t0, c, frag, cn2, cmax, mean, K, out, stats);
+    else
+        hipLaunchKernelGGL((k_assign_tiles<1, TOP2, DESC, ROUNDF>), dim3(g), dim3(256), 0,
+                           ctx->stream, x, n, ldx, t0, c, frag, cn2, cmax, mean, K, out, stats);
+    SC_CHECK_LAUNCH();
+    return SC_OK;
+}
+
+// Blocks the tiled kernel takes: lengths 32 and 64 (codebooks of any size).
+inline bool tiles_ok(int blen) {
+    if (const char *e = getenv("SC_ASSIGN_TILES"))   // test hook: 0 = the per-wave kernel
+        if (atoi(e) == 0) return false;
+    return blen == 32 || blen == 64;
+}
+
+int prep_frag32(sc_ctx *ctx, const float *cb, int nb, int K, int blen, const float *mean,
+                unsigned char *base, size_t &off, uint4 **frag) {
+    const int KTp = ((K + 15) / 16 + kTChunk - 1) / kTChunk * kTChunk;
+    const int64_t nf = (int64_t)nb * KTp * (blen / 32) * 64;
+    unsigned char *p = base ? base + off : nullptr;
+    off = (off + sizeof(uint4) * 2 * (size_t)nf + 255) & ~(size_t)255;
+    *frag = (uint4 *)p;
+    if (!base) return SC_OK;
+    hipLaunchKernelGGL(k_cb_frag32, dim3((unsigned)sc_cdiv(nf, 256)), dim3(256), 0, ctx->stream,
+                       cb, nb, K, blen, mean, *frag);
+    SC_CHECK_LAUNCH();
+    return SC_OK;
+}
+
 int prep_codebook(sc_ctx *ctx, const float *cb, int nb, int K, int blen, int shift,
                   unsigned char *base, size_t &off, float **mean, uint4 **frag, double **cn2,
                   unsigned long long **cmax) {
@@ -458,8 +776,8 @@ int prep_codebook(sc_ctx *ctx, const float *cb, int nb, int K, int blen, int shi
     *cmax = (unsigned long long *)take(8ull * nb);
     if (!base) return SC_OK;   // sizing pass
     SC_HIP(hipMemsetAsync(*cmax, 0, 8ull * nb, ctx->stream));
-    hipLaunchKernelGGL(k_cb_mean, dim3((unsigned)sc_cdiv((int64_t)nb * blen, 256)), dim3(256), 0,
-                       ctx->stream, cb, nb, K, blen, shift, *mean);
+    hipLaunchKernelGGL(k_cb_mean, dim3((unsigned)(nb * blen)), dim3(256), 0, ctx->stream, cb, nb,
+                       K, blen, shift, *mean);
     hipLaunchKernelGGL(k_cb_norms, dim3((unsigned)sc_cdiv((int64_t)nb * K, 256)), dim3(256), 0,
                        ctx->stream, cb, nb, K, blen, *mean, *cn2, *cmax);
     hipLaunchKernelGGL(k_cb_frag, dim3((unsigned)sc_cdiv(nfrag, 256)), dim3(256), 0, ctx->stream,
@@ -473,11 +791,12 @@ int prep_codebook(sc_ctx *ctx, const float *cb, int nb, int K, int blen, int shi
 // Scratch of kmeans_assign_bf16: the prepared centres and two statistics.
 size_t kmeans_bf16_scratch(int k, int d) {
     float *mean;
-    uint4 *frag;
+    uint4 *frag, *frag32;
     double *cn2;
     unsigned long long *cmax;
     size_t off = 0;
     prep_codebook(nullptr, nullptr, 1, k, d, 1, nullptr, off, &mean, &frag, &cn2, &cmax);
+    if (tiles_ok(d)) prep_frag32(nullptr, nullptr, 1, k, d, nullptr, nullptr, off, &frag32);
     return off + 256;
 }
 
@@ -489,8 +808,14 @@ int kmeans_assign_bf16(sc_ctx *ctx, const float *x, int64_t n, int d, const floa
     unsigned long long *cmax;
     size_t off = 0;
     SC_TRY(prep_codebook(ctx, c, 1, k, d, 1, scratch, off, &mean, &frag, &cn2, &cmax));
+    uint4 *frag32 = nullptr;
+    const bool tiles = tiles_ok(d);
+    if (tiles) SC_TRY(prep_frag32(ctx, c, 1, k, d, mean, scratch, off, &frag32));
     unsigned long long *stats = (unsigned long long *)(scratch + off);
     SC_HIP(hipMemsetAsync(stats, 0, 16, ctx->stream));
+    if (tiles)
+        return launch_tiles<false, true, true>(ctx, x, n, d, 0, d, c, frag32, cn2, cmax, mean, k,
+                                               TileOut{nullptr, 0, 0, 0, labels}, stats);
     const size_t lds = encode_lds(d);
     SC_HIP(hipFuncSetAttribute((const void *)k_assign_mfma,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -514,16 +839,40 @@ int sc_encode_bf16(sc_ctx *ctx, const float *x, int64_t n, int dim, const float 
     double *cn2, *pn2;
     unsigned long long *cmax, *pmax, *stats;
     size_t off = 0;
+    const bool tiles = tiles_ok(bsc);
+    uint4 *cfrag32 = nullptr;
     prep_codebook(ctx, cq, mc, kc, bsc, 1, nullptr, off, &cmean, &cfrag, &cn2, &cmax);
     prep_codebook(ctx, pq, mp, kp, bsp, 0, nullptr, off, &pmean, &pfrag, &pn2, &pmax);
+    if (tiles) prep_frag32(ctx, cq, mc, kc, bsc, nullptr, nullptr, off, &cfrag32);
     const size_t total = off + 256;
     unsigned char *s;
     SC_TRY(sc_scratch(ctx, total, (void **)&s));
     off = 0;
     SC_TRY(prep_codebook(ctx, cq, mc, kc, bsc, 1, s, off, &cmean, &cfrag, &cn2, &cmax));
     SC_TRY(prep_codebook(ctx, pq, mp, kp, bsp, 0, s, off, &pmean, &pfrag, &pn2, &pmax));
+    if (tiles) SC_TRY(prep_frag32(ctx, cq, mc, kc, bsc, cmean, s, off, &cfrag32));
     stats = (unsigned long long *)(s + off);
     SC_HIP(hipMemsetAsync(stats, 0, 16, ctx->stream));
+    if (tiles && n > 0) {
+        // the coarse blocks on the tiled kernel; the per-wave kernel then
+        // takes the residuals and the PQ blocks
+        const int KTp = ((kc + 15) / 16 + kTChunk - 1) / kTChunk * kTChunk;
+        for (int j = 0; j < mc; ++j) {
+            const uint4 *fj = cfrag32 + (size_t)j * KTp * (bsc / 32) * 2 * 64;
+            TileOut o{cid, nc * mc, j, mc + j, nullptr};
+            if (nc == 2)
+                SC_TRY((launch_tiles<true, true, false>(ctx, x, n, dim, j * bsc, bsc,
+                                                         cq + (size_t)j * kc * bsc, fj,
+                                                         cn2 + (size_t)j * kc, cmax + j,
+                                                         cmean + (size_t)j * bsc, kc, o, stats)));
+            else
+                SC_TRY((launch_tiles<false, false, false>(ctx, x, n, dim, j * bsc, bsc,
+                                                           cq + (size_t)j * kc * bsc, fj,
+                                                           cn2 + (size_t)j * kc, cmax + j,
+                                                           cmean + (size_t)j * bsc, kc, o, stats)));
+        }
+    }
+    const int given = tiles ? 1 : 0;
     const size_t lds = encode_lds(dim);
     const unsigned blocks = (unsigned)sc_cdiv(n, kRows);
     if (nc == 2) {
@@ -531,13 +880,13 @@ int sc_encode_bf16(sc_ctx *ctx, const float *x, int64_t n, int dim, const float 
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         hipLaunchKernelGGL(k_encode_mfma<2>, dim3(blocks), dim3(64), lds, ctx->stream, x, n, dim,
                            cq, mc, kc, pq, mp, kp, cfrag, cn2, cmax, cmean, pfrag, pn2,
-                           pmax, pmean, cid, codes, stats);
+                           pmax, pmean, cid, codes, stats, given);
     } else {
         SC_HIP(hipFuncSetAttribute((const void *)k_encode_mfma<1>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         hipLaunchKernelGGL(k_encode_mfma<1>, dim3(blocks), dim3(64), lds, ctx->stream, x, n, dim,
                            cq, mc, kc, pq, mp, kp, cfrag, cn2, cmax, cmean, pfrag, pn2,
-                           pmax, pmean, cid, codes, stats);
+                           pmax, pmean, cid, codes, stats, given);
     }
     SC_CHECK_LAUNCH();
     if (st) {
