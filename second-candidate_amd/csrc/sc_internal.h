@@ -155,6 +155,7 @@ struct RankCellsArgs {
     uint64_t *big_list, *big_sorted;
     uint32_t *big_size, *big_aux;
     int big_cap;
+    uint32_t *big_split;     // [nq][kSplitWords] split-query state (small batches) or null
     int64_t *overflowed;     // host, may be null: += queries that outgrew the first list
     unsigned long long *stats;   // device, may be null: [2] += cells scored
 };

@@ -884,7 +884,18 @@ struct BigArgs {
     uint32_t *size, *aux;       // aux [nq][4]: appended, final bucket, base bits, overflow
     int cap;
     int rerun;                  // 1: start from the cut aux holds
+    // A query split over `parts` workgroups (small batches): workgroup p
+    // takes the nearest-0 centroids n with n % parts == p; they share the
+    // query's cut, list counter and (at the end) histograms through split
+    // [nq][kSplitWords] = {cut, appended, finished, 0, counts [kBuckets],
+    // sizes [kBuckets]}.
+    int parts;
+    uint32_t *split;
+    // 0: the whole ranking; 1: only the warm-up, whose cut goes to split;
+    // 2: a query's share (of `parts`) from the cut in split
+    int mode;
 };
+constexpr int kSplitWords = 4 + 2 * 4096;
 
 __device__ __forceinline__ int dbucket(float s, float base) {
     const uint32_t u = __float_as_uint(s - base);
@@ -981,10 +992,13 @@ __global__ __launch_bounds__(kBigThreads, 8) void k_rank_big(BigArgs b) {
     __shared__ uint32_t scnt[64], smass[64];      // super-buckets: 64 buckets each
     __shared__ unsigned long long s_scored;
     const int tid = threadIdx.x, lane = tid & 63;
-    const int64_t q = k.qlist ? k.qlist[blockIdx.x] : blockIdx.x;
+    const int P = b.mode == 2 ? b.parts : 1, part = (int)(blockIdx.x % (unsigned)P);
+    const int64_t q = k.qlist ? k.qlist[blockIdx.x / P] : blockIdx.x / P;
     const int T = query_budget(k.a.Tq, q, k.a.T), w = k.a.w;
     uint32_t *aux = b.aux + (size_t)q * 4;
-    const int start_cut = b.rerun ? (int)aux[1] : kBuckets;
+    uint32_t *sp = b.mode != 0 ? b.split + (size_t)q * kSplitWords : nullptr;
+    const int start_cut = b.rerun ? (int)aux[1] : b.mode == 2 ? (int)sp[0] : kBuckets;
+    __shared__ int s_nord, s_last;
     query_terms<kBigThreads>(k, q, v, sh);
     if (T <= 0 || k.ncells == 0) return;       // the emit kernel writes the empty result
     float m0a, m0b, m1a, m1b;
@@ -996,7 +1010,7 @@ __global__ __launch_bounds__(kBigThreads, 8) void k_rank_big(BigArgs b) {
     for (int i = tid; i < kBuckets; i += kBigThreads) hcnt[i] = 0;
     __syncthreads();
     for (int n = tid; n < kc; n += kBigThreads)
-        atomicAdd(&hcnt[dbucket((v[n] + m0a) + LB1, base)], 1u);
+        if (n % P == part) atomicAdd(&hcnt[dbucket((v[n] + m0a) + LB1, base)], 1u);
     __syncthreads();
     {
         constexpr int per = kBuckets / kBigThreads;
@@ -1009,10 +1023,12 @@ __global__ __launch_bounds__(kBigThreads, 8) void k_rank_big(BigArgs b) {
             hcnt[tid * per + i] = run;
             run += c;
         }
+        if (tid == 0) s_nord = (int)tot;
     }
     __syncthreads();
+    const int nord = s_nord;          // this workgroup's centroids
     for (int n = tid; n < kc; n += kBigThreads)
-        ord[atomicAdd(&hcnt[dbucket((v[n] + m0a) + LB1, base)], 1u)] = (uint16_t)n;
+        if (n % P == part) ord[atomicAdd(&hcnt[dbucket((v[n] + m0a) + LB1, base)], 1u)] = (uint16_t)n;
     __syncthreads();
     for (int i = tid; i < kBuckets; i += kBigThreads) { hcnt[i] = 0; hmass[i] = 0; }
     if (tid < 64) { scnt[tid] = 0; smass[tid] = 0; }
@@ -1030,7 +1046,9 @@ __global__ __launch_bounds__(kBigThreads, 8) void k_rank_big(BigArgs b) {
         const unsigned long long am = __ballot(adm);
         if (am) {
             int slot0 = 0;
-            if (lane == 0) slot0 = atomicAdd(&s_napp, (int)__popcll(am));
+            if (lane == 0)
+                slot0 = b.mode == 2 ? (int)atomicAdd(sp + 1, (uint32_t)__popcll(am))
+                                    : atomicAdd(&s_napp, (int)__popcll(am));
             slot0 = __shfl(slot0, 0);
             if (adm) {
                 const int slot = slot0 + (int)__popcll(am & ((1ull << lane) - 1ull));
@@ -1044,9 +1062,10 @@ __global__ __launch_bounds__(kBigThreads, 8) void k_rank_big(BigArgs b) {
     // counted into the histograms without appending, for a first cut; the
     // histograms are then cleared and phase 1 starts over from that cut with
     // appends, so no cell is counted twice.
-    for (int phase = start_cut < kBuckets ? 1 : 0; phase < 2; ++phase) {
+    const int phase0 = start_cut < kBuckets || b.mode == 2 ? 1 : 0;
+    for (int phase = phase0; phase < (b.mode == 1 ? 1 : 2); ++phase) {
         const bool app = phase == 1;
-        const int lim = app ? kc : min(kWarm, kc);
+        const int lim = app ? nord : min(kWarm, nord);
         for (;;) {
             int i = 0;
             if (lane == 0) i = atomicAdd(&s_pos, 1);
@@ -1054,7 +1073,12 @@ __global__ __launch_bounds__(kBigThreads, 8) void k_rank_big(BigArgs b) {
             if (i >= lim) break;
             const int n0 = ord[i];
             const float a0 = v[n0];
-            // one value of the cut per wave (another wave may lower it meanwhile)
+            // one value of the cut per wave (another wave or workgroup may
+            // lower it meanwhile)
+            if (b.mode == 2 && lane == 0) {
+                const int gc = (int)__hip_atomic_load(sp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (gc < *bcut) atomicMin(&s_bcur, gc);
+            }
             int bcur = __shfl(*bcut, 0);
             if (dbucket((a0 + m0a) + LB1, base) > bcur) break;   // so are the centroids after it
             const int g0 = b.bg_n0[n0], g1 = b.bg_n0[n0 + 1];
@@ -1156,13 +1180,28 @@ __global__ __launch_bounds__(kBigThreads, 8) void k_rank_big(BigArgs b) {
                                         (int64_t)since_cnt * 32 >= w;
                     if (reach && enough) {
                         const int f = wave_find_bucket(hcnt, hmass, scnt, smass, w, T);
-                        if (lane == 0 && f < kBuckets) atomicMin(&s_bcur, f);
+                        if (lane == 0 && f < kBuckets) {
+                            atomicMin(&s_bcur, f);
+                            if (b.mode == 2) atomicMin(reinterpret_cast<int *>(sp), f);
+                        }
                         since_cnt = 0;
                         since_mass = 0;
                     }
                     bcur = __shfl(*bcut, 0);
                 }
             }
+        }
+        if (b.mode == 1) {
+            // warm-up only: its exact crossing bucket (a subset's, so a bound)
+            // is the starting cut of the query's workgroups
+            __syncthreads();
+            const int wf = find_bucket<kBigThreads>(hcnt, hmass, 0u, 0u, w, T, sh.wsum, &sh.b,
+                                                    sh.pre);
+            if (tid == 0) {
+                atomicMin(reinterpret_cast<int *>(sp), min(wf, s_bcur));
+                if (k.a.stats) atomicAdd(k.a.stats + 2, s_scored);
+            }
+            return;
         }
         if (phase == 0) {
             // the warm-up's cut stays (its cells are a subset of all cells);
@@ -1177,15 +1216,43 @@ __global__ __launch_bounds__(kBigThreads, 8) void k_rank_big(BigArgs b) {
         }
     }
     __syncthreads();
+    if (tid == 0 && k.a.stats) atomicAdd(k.a.stats + 2, s_scored);
+    if (b.mode == 2) {
+        // the workgroups' histograms (disjoint cells) summed into the
+        // query's; the last workgroup to finish takes the crossing bucket
+        uint32_t *gc = sp + 4, *gm = sp + 4 + kBuckets;
+        for (int i = tid; i < kBuckets; i += kBigThreads) {
+            if (hcnt[i]) atomicAdd(gc + i, hcnt[i]);
+            if (hmass[i]) atomicAdd(gm + i, hmass[i]);
+        }
+        __threadfence();
+        __syncthreads();
+        if (tid == 0) s_last = atomicAdd(sp + 2, 1u) == (uint32_t)(P - 1);
+        __syncthreads();
+        if (!s_last) return;
+        __threadfence();
+        for (int i = tid; i < kBuckets; i += kBigThreads) {
+            hcnt[i] = __hip_atomic_load(gc + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            hmass[i] = __hip_atomic_load(gm + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (tid == 0) s_napp = (int)__hip_atomic_load(sp + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+    }
     // the exact crossing bucket of the final histograms
     const int bf = find_bucket<kBigThreads>(hcnt, hmass, 0u, 0u, w, T, sh.wsum, &sh.b, sh.pre);
-    if (tid == 0 && k.a.stats) atomicAdd(k.a.stats + 2, s_scored);
     if (tid == 0) {
         aux[0] = (uint32_t)s_napp;
         aux[1] = (uint32_t)bf;
         aux[2] = __float_as_uint(base);
         aux[3] = s_napp > b.cap ? 1u : 0u;
     }
+}
+
+// Resets the split state of the queries: cut = none, counters and histograms 0.
+__global__ void k_split_reset(uint32_t *__restrict__ split, int64_t words) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= words) return;
+    split[i] = (i % kSplitWords) == 0 ? (uint32_t)kBuckets : 0u;
 }
 
 __global__ void k_big_flagged(const uint32_t *__restrict__ aux, int64_t nq,
@@ -1468,8 +1535,24 @@ int flagged(sc_ctx *ctx, const RankCellsArgs &a, int32_t *list, int32_t *dn, int
 // are flagged (qinfo[5]) for the device-wide path.
 int rank_big(sc_ctx *ctx, const sc_index *ix, const RankKernelArgs &k, const RankCellsArgs &a) {
     BigArgs b{k,           ix->bg_n0,    ix->bg_pair, ix->bg_start, ix->brec, ix->ckeys,
-              a.big_list,  a.big_sorted, a.big_size,  a.big_aux,    a.big_cap, 0};
+              a.big_list,  a.big_sorted, a.big_size,  a.big_aux,    a.big_cap, 0,
+              1,           nullptr,      0};
     b.k.qlist = nullptr;
+    // small batches: each query over several workgroups (about two per CU)
+    if (a.big_split && a.nq * 2 < (int64_t)ctx->n_cu) {
+        b.parts = (int)std::min<int64_t>(64, std::max<int64_t>(1, 2 * ctx->n_cu / a.nq));
+        if (const char *e = getenv("SC_RANK_PARTS")) {   // test hook
+            const int v = atoi(e);
+            if (v >= 1) b.parts = std::min(v, 64);
+        }
+        b.split = a.big_split;
+        if (b.parts > 1) {
+            const int64_t words = a.nq * (int64_t)kSplitWords;
+            hipLaunchKernelGGL(k_split_reset, dim3((unsigned)sc_cdiv(words, 256)), dim3(256), 0,
+                               ctx->stream, b.split, words);
+            SC_CHECK_LAUNCH();
+        }
+    }
     const size_t lds = big_lds(ix->kc, ix->dim);
     if (lds > 150 * 1024) {
         sc_set_error("sc_search: kc=%d too large for the pruned cell ranking", ix->kc);
@@ -1477,7 +1560,18 @@ int rank_big(sc_ctx *ctx, const sc_index *ix, const RankKernelArgs &k, const Ran
     }
     SC_HIP(hipFuncSetAttribute((const void *)k_rank_big,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(k_rank_big, dim3((unsigned)a.nq), dim3(kBigThreads), lds, ctx->stream, b);
+    if (b.parts > 1) {
+        // the warm-up on one workgroup per query, then the query's shares
+        BigArgs wu = b;
+        wu.mode = 1;
+        hipLaunchKernelGGL(k_rank_big, dim3((unsigned)a.nq), dim3(kBigThreads), lds, ctx->stream,
+                           wu);
+        SC_CHECK_LAUNCH();
+        b.mode = 2;
+    }
+    hipLaunchKernelGGL(k_rank_big, dim3((unsigned)(a.nq * (b.mode == 2 ? b.parts : 1))),
+                       dim3(kBigThreads), lds,
+                       ctx->stream, b);
     SC_CHECK_LAUNCH();
     int32_t *list = a.flag_list, *dn = a.flag_count;
     int32_t nflag = 0;
@@ -1492,6 +1586,9 @@ int rank_big(sc_ctx *ctx, const sc_index *ix, const RankKernelArgs &k, const Ran
         BigArgs r = b;
         r.k.qlist = list;
         r.rerun = 1;
+        r.parts = 1;              // from the exact cut: one workgroup per query
+        r.split = nullptr;
+        r.mode = 0;
         hipLaunchKernelGGL(k_rank_big, dim3((unsigned)nflag), dim3(kBigThreads), lds,
                            ctx->stream, r);
         SC_CHECK_LAUNCH();

@@ -319,13 +319,17 @@ def test_search_scimi_small_rank_list(ctx, cap, pairs, monkeypatch):
     ix.close()
 
 
+@pytest.mark.parametrize("parts", ["1", "3", "auto"])
 @pytest.mark.parametrize("case", SCIMI_CASES)
-def test_search_scimi_pruned_ranking_matches_oracle(ctx, case, monkeypatch):
+def test_search_scimi_pruned_ranking_matches_oracle(ctx, case, parts, monkeypatch):
     """The pruned ranking of large indexes (centroids visited in order of
     their bound, a falling histogram cut, fine-bucket emission), forced onto
-    the small indexes: ids, distances and candidate counts equal the
-    oracle's, in both forms."""
+    the small indexes, with each query on one workgroup, on three, or on as
+    many as the batch size gives: ids, distances and candidate counts equal
+    the oracle's, in both forms."""
     monkeypatch.setenv("SC_RANK_BIG", "1")
+    if parts != "auto":
+        monkeypatch.setenv("SC_RANK_PARTS", parts)
     kc, R, w, T = case
     ix, oix = _index_pair(ctx, S.KIND_SCIMI, kc=kc)
     st = _compare_search(ctx, ix, oix, S.KIND_SCIMI, R, w, T, nq=100)
