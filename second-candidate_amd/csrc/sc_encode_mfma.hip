@@ -728,6 +728,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8))) voi
     }
 }
 
+int prep_codebook(sc_ctx *ctx, const float *cb, int nb, int K, int blen, int shift,
+                  unsigned char *base, size_t &off, float **mean, uint4 **frag, double **cn2,
+                  unsigned long long **cmax) {
+    auto take = [&](size_t bytes) { unsigned char *p = base + off; off = (off + bytes + 255) & ~(size_t)255; return p; };
+    const int64_t nfrag = (int64_t)nb * ((K + 15) / 16) * (blen / 16) * 64;
+    *mean = (float *)take(sizeof(float) * nb * blen);
+    *frag = (uint4 *)take(sizeof(uint4) * (size_t)nfrag);
+    *cn2 = (double *)take(8ull * nb * K);
+    *cmax = (unsigned long long *)take(8ull * nb);
+    if (!base) return SC_OK;   // sizing pass
+    SC_HIP(hipMemsetAsync(*cmax, 0, 8ull * nb, ctx->stream));
+    hipLaunchKernelGGL(k_cb_mean, dim3((unsigned)(nb * blen)), dim3(256), 0, ctx->stream, cb, nb,
+                       K, blen, shift, *mean);
+    hipLaunchKernelGGL(k_cb_norms, dim3((unsigned)sc_cdiv((int64_t)nb * K, 256)), dim3(256), 0,
+                       ctx->stream, cb, nb, K, blen, *mean, *cn2, *cmax);
+    hipLaunchKernelGGL(k_cb_frag, dim3((unsigned)sc_cdiv(nfrag, 256)), dim3(256), 0, ctx->stream,
+                       cb, nb, K, blen, *mean, *frag);
+    SC_CHECK_LAUNCH();
+    return SC_OK;
+}
+
 template <bool TOP2, bool DESC, bool ROUNDF>
 int launch_tiles(sc_ctx *ctx, const float *x, int64_t n, int ldx, int t0, int blen,
                  const float *c, const uint4 *frag, const double *cn2,
@@ -761,27 +782,6 @@ int prep_frag32(sc_ctx *ctx, const float *cb, int nb, int K, int blen, const flo
     if (!base) return SC_OK;
     hipLaunchKernelGGL(k_cb_frag32, dim3((unsigned)sc_cdiv(nf, 256)), dim3(256), 0, ctx->stream,
                        cb, nb, K, blen, mean, *frag);
-    SC_CHECK_LAUNCH();
-    return SC_OK;
-}
-
-int prep_codebook(sc_ctx *ctx, const float *cb, int nb, int K, int blen, int shift,
-                  unsigned char *base, size_t &off, float **mean, uint4 **frag, double **cn2,
-                  unsigned long long **cmax) {
-    auto take = [&](size_t bytes) { unsigned char *p = base + off; off = (off + bytes + 255) & ~(size_t)255; return p; };
-    const int64_t nfrag = (int64_t)nb * ((K + 15) / 16) * (blen / 16) * 64;
-    *mean = (float *)take(sizeof(float) * nb * blen);
-    *frag = (uint4 *)take(sizeof(uint4) * (size_t)nfrag);
-    *cn2 = (double *)take(8ull * nb * K);
-    *cmax = (unsigned long long *)take(8ull * nb);
-    if (!base) return SC_OK;   // sizing pass
-    SC_HIP(hipMemsetAsync(*cmax, 0, 8ull * nb, ctx->stream));
-    hipLaunchKernelGGL(k_cb_mean, dim3((unsigned)(nb * blen)), dim3(256), 0, ctx->stream, cb, nb,
-                       K, blen, shift, *mean);
-    hipLaunchKernelGGL(k_cb_norms, dim3((unsigned)sc_cdiv((int64_t)nb * K, 256)), dim3(256), 0,
-                       ctx->stream, cb, nb, K, blen, *mean, *cn2, *cmax);
-    hipLaunchKernelGGL(k_cb_frag, dim3((unsigned)sc_cdiv(nfrag, 256)), dim3(256), 0, ctx->stream,
-                       cb, nb, K, blen, *mean, *frag);
     SC_CHECK_LAUNCH();
     return SC_OK;
 }

@@ -2540,8 +2540,8 @@ Plan make_plan(const sc_index *ix, const sc_search_params *p, int64_t nq) {
     P.off_big_sorted = take(8ull * big * chunk);
     P.off_big_size = take(4ull * big * chunk);
     P.off_big_aux = take(16ull * chunk);
-    // split-query state for batches of fewer queries than CUs (4 + 2 x 4096 words each)
-    P.off_big_split = take(big && chunk < 256 ? 4ull * (4 + 2 * 4096) * chunk : 0);
+    // split-query state for batches of a few queries (4 + 2 x 4096 words each)
+    P.off_big_split = take(big && chunk <= 4 ? 4ull * (4 + 2 * 4096) * chunk : 0);
     size_t tb = 0;
     (void)rocprim::radix_sort_pairs(nullptr, tb, (uint32_t *)nullptr, (uint32_t *)nullptr,
                                     (int32_t *)nullptr, (int32_t *)nullptr, (size_t)chunk, 0u,
@@ -3119,7 +3119,7 @@ static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
                 ra.big_size = (uint32_t *)(s + P.off_big_size);
                 ra.big_aux = (uint32_t *)(s + P.off_big_aux);
                 ra.big_cap = P.big_cap;
-                if (P.chunk < 256) ra.big_split = (uint32_t *)(s + P.off_big_split);
+                if (P.chunk <= 4) ra.big_split = (uint32_t *)(s + P.off_big_split);
             }
             ra.overflowed = &fallback_total;
             ra.stats = st ? stats : nullptr;

@@ -41,13 +41,14 @@ ENC_SRC = os.path.join(ROOT, "second-candidate_amd", "csrc", "sc_encode_mfma.hip
 
 def generate_encode(path):
     """The device code of sc_encode_mfma.hip (its anonymous namespace up to
-    the host-side codebook preparation) with the MFMA builtin emulated."""
+    the host-side codebook preparation) with the MFMA builtins emulated."""
     src = open(ENC_SRC).read()
     a, b = src.index("namespace {"), src.index("int prep_codebook(")
     body = src[a:b]
     body = body.replace("extern __shared__ __align__(16) unsigned char smem[];",
                         "unsigned char *smem = g_smem;")
     body = body.replace("__builtin_amdgcn_mfma_f32_16x16x16bf16_1k", "emu_mfma_bf16")
+    body = body.replace("__builtin_amdgcn_mfma_f32_16x16x32_bf16", "emu_mfma32_bf16")
     out = ['#include "shim.h"', "#include <vector>", "extern unsigned char *g_smem;", body,
            "} // namespace", open(os.path.join(HERE, "driver_encode.inc")).read()]
     open(path, "w").write("\n".join(out))

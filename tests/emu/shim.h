@@ -128,6 +128,26 @@ template <class A, class C> inline C emu_mfma_bf16(A a, A b, C c, int, int, int)
     return r;
 }
 
+// v_mfma_f32_16x16x32_bf16 of one wave: lane l holds A[l % 16][8 (l / 16) + j]
+// and B[8 (l / 16) + j][l % 16] for j < 8 and receives C[4 (l / 16) + i][l % 16]
+// (the layout the device probe confirms); FP32 sums in k order.
+extern float g_mfma32_a[16][64][8], g_mfma32_b[16][64][8];
+template <class A, class C> inline C emu_mfma32_bf16(A a, A b, C c, int, int, int) {
+    const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+    for (int j = 0; j < 8; ++j) { g_mfma32_a[w][l][j] = (float)a[j]; g_mfma32_b[w][l][j] = (float)b[j]; }
+    g_wave_bar[w].wait("mfma32");
+    C r = c;
+    for (int i = 0; i < 4; ++i) {
+        const int row = 4 * (l / 16) + i, col = l % 16;
+        float s = r[i];
+        for (int k = 0; k < 32; ++k)
+            s += g_mfma32_a[w][row + 16 * (k / 8)][k % 8] * g_mfma32_b[w][col + 16 * (k / 8)][k % 8];
+        r[i] = s;
+    }
+    g_wave_bar[w].wait("mfma32b");
+    return r;
+}
+
 #define SC_KIND_IVFADC 0
 #define SC_KIND_IMI 1
 #define SC_KIND_SC 2

@@ -2,10 +2,12 @@
 
 The device code of second-candidate_amd/csrc/sc_encode_mfma.hip is compiled
 for the host against tests/emu/shim.h, whose wave-collective MFMA follows the
-v_mfma_f32_16x16x16bf16_1k operand layout (FP32 sums; any summation order is
-inside the kernel's error bound).  Each case runs the codebook preparation
-kernels and one emulated wave per 16 vectors, then compares every cell id and
-code with the oracle's encoder bit for bit.  AddressSanitizer checks every
+v_mfma_f32_16x16x16bf16_1k and v_mfma_f32_16x16x32_bf16 operand layouts (FP32
+sums; any summation order is inside the kernel's error bound).  Each case runs
+the codebook preparation kernels, the tiled coarse assignment (256-thread
+workgroups of 128 rows) where the library uses it, and one emulated wave per
+16 vectors, then compares every cell id and code with the oracle's encoder bit
+for bit.  AddressSanitizer checks every
 access against exact-size buffers (LDS included); ThreadSanitizer checks the
 wave's LDS traffic.
 """
@@ -24,12 +26,15 @@ CXX = os.environ.get("EMU_CXX", "/opt/rocm/llvm/bin/clang++")
 pytestmark = pytest.mark.skipif(not shutil.which(CXX) and not os.path.exists(CXX),
                                 reason="no clang++ for the host emulation")
 
-CASES = [  # n dim mc kc mp kp nc noisy ties
-    ("sc top-2, K=256, planted ties", "96 128 1 256 8 256 2 1 1"),
-    ("ivf top-1, K=256", "48 128 1 256 8 256 1 1 1"),
-    ("imi two halves, K=64", "48 128 2 64 8 256 1 1 1"),
-    ("sc K=1000: several groups, partial tile", "40 64 1 1000 4 256 2 1 1"),
-    ("D=960, integer data", "20 960 2 64 15 256 1 0 0"),
+CASES = [  # n dim mc kc mp kp nc noisy ties tiles
+    ("sc top-2, K=256, planted ties", "96 128 1 256 8 256 2 1 1 1"),
+    ("ivf top-1, K=256", "48 128 1 256 8 256 1 1 1 1"),
+    ("imi two halves, K=64, per-wave kernel", "48 128 2 64 8 256 1 1 1 0"),
+    ("imi two halves, K=64, tiled", "150 128 2 64 8 256 1 1 1 1"),
+    ("two halves top-2, K=300 (partial chunk), tiled, planted ties", "140 128 2 300 8 256 2 1 1 1"),
+    ("32-dim halves top-2, K=128, tiled", "130 64 2 128 4 256 2 1 0 1"),
+    ("sc K=1000: several groups, partial tile", "40 64 1 1000 4 256 2 1 1 1"),
+    ("D=960, integer data", "20 960 2 64 15 256 1 0 0 1"),
 ]
 
 

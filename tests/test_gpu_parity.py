@@ -334,6 +334,9 @@ def test_search_scimi_pruned_ranking_matches_oracle(ctx, case, parts, monkeypatc
     ix, oix = _index_pair(ctx, S.KIND_SCIMI, kc=kc)
     st = _compare_search(ctx, ix, oix, S.KIND_SCIMI, R, w, T, nq=100)
     _compare_search(ctx, ix, oix, S.KIND_SCIMI, R, w, T, nq=100, form=S.FORM_MIXED)
+    # batches of a few queries (each query shared over several workgroups)
+    for nq in (1, 4):
+        _compare_search(ctx, ix, oix, S.KIND_SCIMI, R, w, T, nq=nq)
     if T == 0:
         assert st.candidates == 0 and st.cells == 0
     if kc == 4:
