@@ -123,6 +123,12 @@ def main():
     ap.add_argument("--ntrain-large", type=int, default=10_000_000)
     ap.add_argument("--iters-large", type=int, default=20)
     ap.add_argument("--recall-large", type=int, default=1000)
+    ap.add_argument("--large-train", choices=("kmeans", "sample"), default="kmeans",
+                    help="codebooks of the large line: k-means on the training vectors, or "
+                         "training vectors drawn at random (profiling runs)")
+    ap.add_argument("--encode-only", type=int, default=0,
+                    help="K > 0: only time the encode of the 1M base with two 64-dim halves of "
+                         "K codewords drawn from the training vectors (both forms), then exit")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="only run a plain device copy of this many MiB (HBM counter scale)")
     args = ap.parse_args()
@@ -139,6 +145,10 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes": n, "launches": 3}), flush=True)
+        return
+
+    if args.encode_only > 0:
+        encode_only(args)
         return
 
     if args.gpus != 1:
@@ -546,6 +556,50 @@ def d960_line(S, ctx, args):
     return row
 
 
+def encode_only(args):
+    """Encode timing: the 1M-vector D=128 base, two 64-dim halves with K
+    codewords each drawn from 200k training vectors, PQ 8x256 (sub-vectors of
+    the training residuals), both forms three times; one JSON line."""
+    import sc_amd as S
+    ctx = S.Ctx(0)
+    mix = S.mix_d128()
+    dim, mp, kp, kc, n = 128, 8, 256, args.encode_only, 1_000_000
+    rng = np.random.default_rng(5)
+    xt = S.gen_rows(ctx, mix, S.STREAM_TRAIN, 0, 200_000, dim)
+    th = xt.numpy()
+    cq = np.stack([th[rng.choice(th.shape[0], kc, replace=False), h * 64:(h + 1) * 64]
+                   for h in range(2)]).astype(np.float32)
+    dcq = S.Dev.from_numpy(ctx, cq)
+    S.residual(ctx, xt, dcq, 2, kc, 0)
+    rh = xt.numpy()
+    pq = np.stack([rh[rng.choice(rh.shape[0], kp, replace=False), s * 16:(s + 1) * 16]
+                   for s in range(mp)]).astype(np.float32)
+    dpq = S.Dev.from_numpy(ctx, pq)
+    x = S.gen_rows(ctx, mix, S.STREAM_BASE, 0, n, dim)
+    out = {"workload": "encode 1M x 128, two 64-dim halves, K=%d (nearest, second), PQ 8x256"
+                       % kc}
+    res = {}
+    for prec, name in ((S.PREC_BF16, "bf16_mfma_recheck"), (S.PREC_FP32, "fp32_exact")):
+        best = None
+        for _ in range(3):
+            ctx.sync()
+            t0 = time.perf_counter()
+            cid, codes, st = S.encode(ctx, x, dcq, dpq, 2, kc, mp, kp, 2, prec)
+            ctx.sync()
+            dt = time.perf_counter() - t0
+            best = dt if best is None else min(best, dt)
+        res[name] = (cid.numpy(), codes.numpy())
+        out[name + "_ms"] = round(best * 1e3, 3)
+        if prec == S.PREC_BF16:
+            out["rechecked_per_vector"] = round(st.rechecked / n, 2)
+    flops = 2.0 * n * (kc * dim + mp * kp * (dim // mp))
+    out["algorithmic_tflops_bf16"] = round(flops / (out["bf16_mfma_recheck_ms"] * 1e-3) / 1e12, 2)
+    out["identical"] = bool(all(np.array_equal(a, b) for a, b in
+                                zip(res["bf16_mfma_recheck"], res["fp32_exact"])))
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
 def large_line(S, ctx, args):
     """The large dense-partition index on one GPU: D=128 mixture, codebooks
     trained on 10M vectors (k-means++ then 20 Lloyd iterations per codebook,
@@ -568,7 +622,14 @@ def large_line(S, ctx, args):
     th = xt.numpy()
     t_km = 0.0
     cq = []
+    rng = np.random.default_rng(71)
+    sample = args.large_train == "sample"
+    row["codebooks"] = ("k-means" if not sample else
+                        "training vectors drawn at random (profiling run)")
     for h in range(2):
+        if sample:
+            cq.append(th[rng.choice(ntrain, kc, replace=False), h * 64:(h + 1) * 64].copy())
+            continue
         xh = S.Dev.from_numpy(ctx, np.ascontiguousarray(th[:, h * 64:(h + 1) * 64]))
         ctx.sync()
         t1 = time.perf_counter()
@@ -583,9 +644,13 @@ def large_line(S, ctx, args):
     S.residual(ctx, xt, dcq, 2, kc, 0)
     rh = xt.numpy()
     xt.free()
-    pq = np.stack([S.kmeans(ctx, S.Dev.from_numpy(ctx, np.ascontiguousarray(rh[:, s * 16:(s + 1) * 16])),
-                            kp, args.iters_large, seed=81 + s, prec=S.PREC_BF16,
-                            want_labels=False)[0].numpy() for s in range(mp)])
+    if sample:
+        pq = np.stack([rh[rng.choice(ntrain, kp, replace=False), s * 16:(s + 1) * 16]
+                       for s in range(mp)]).astype(np.float32)
+    else:
+        pq = np.stack([S.kmeans(ctx, S.Dev.from_numpy(ctx, np.ascontiguousarray(rh[:, s * 16:(s + 1) * 16])),
+                                kp, args.iters_large, seed=81 + s, prec=S.PREC_BF16,
+                                want_labels=False)[0].numpy() for s in range(mp)])
     dpq = S.Dev.from_numpy(ctx, pq)
     del th, rh
     ctx.sync()
