@@ -155,6 +155,8 @@ struct RankCellsArgs {
     uint64_t *big_list, *big_sorted;
     uint32_t *big_size, *big_aux;
     int big_cap;
+    int retry_cap;           // > 0: big_* hold scratch for that many re-ranked queries
+                             // (indexed by slot) although big_cap == 0
     uint32_t *big_split;     // [nq][kSplitWords] split-query state (small batches) or null
     int64_t *overflowed;     // host, may be null: += queries that outgrew the first list
     unsigned long long *stats;   // device, may be null: [2] += cells scored
@@ -163,3 +165,6 @@ int sc_rank_cells(sc_ctx *ctx, const sc_index *ix, const RankCellsArgs &a);
 // Entries per query of the pruned ranking's scratch, or 0 when the index is
 // ranked by the other kernels.
 int sc_rank_big_cap(const sc_index *ix);
+// Entries per query of the pruned ranking's scratch when it re-ranks the
+// queries whose cells overflow the pair-table ranking's on-chip list, or 0.
+int sc_rank_retry_cap(const sc_index *ix);

@@ -894,6 +894,10 @@ struct BigArgs {
     // 0: the whole ranking; 1: only the warm-up, whose cut goes to split;
     // 2: a query's share (of `parts`) from the cut in split
     int mode;
+    // 1: the per-query scratch (aux, list, sorted, size) is indexed by the
+    // query's slot in k.qlist rather than by the query (a re-run of a few
+    // queries of a batch with scratch for those few only)
+    int by_slot;
 };
 constexpr int kSplitWords = 4 + 2 * 4096;
 
@@ -993,9 +997,11 @@ __global__ __launch_bounds__(kBigThreads, 8) void k_rank_big(BigArgs b) {
     __shared__ unsigned long long s_scored;
     const int tid = threadIdx.x, lane = tid & 63;
     const int P = b.mode == 2 ? b.parts : 1, part = (int)(blockIdx.x % (unsigned)P);
-    const int64_t q = k.qlist ? k.qlist[blockIdx.x / P] : blockIdx.x / P;
+    const int64_t slot = blockIdx.x / P;
+    const int64_t q = k.qlist ? k.qlist[slot] : slot;
+    const int64_t si = b.by_slot ? slot : q;        // the query's scratch
     const int T = query_budget(k.a.Tq, q, k.a.T), w = k.a.w;
-    uint32_t *aux = b.aux + (size_t)q * 4;
+    uint32_t *aux = b.aux + (size_t)si * 4;
     uint32_t *sp = b.mode != 0 ? b.split + (size_t)q * kSplitWords : nullptr;
     const int start_cut = b.rerun ? (int)aux[1] : b.mode == 2 ? (int)sp[0] : kBuckets;
     __shared__ int s_nord, s_last;
@@ -1037,7 +1043,7 @@ __global__ __launch_bounds__(kBigThreads, 8) void k_rank_big(BigArgs b) {
     }
     __syncthreads();
 
-    uint64_t *list = b.list + (size_t)q * b.cap;
+    uint64_t *list = b.list + (size_t)si * b.cap;
     const float *v1 = v + kc;
     volatile int *bcut = &s_bcur;
     uint32_t since_cnt = 0, since_mass = 0;       // this wave's additions since its last cut
@@ -1272,10 +1278,12 @@ __global__ __launch_bounds__(kBigThreads) void k_rank_big_emit(BigArgs b) {
     __shared__ uint32_t s_lo, s_hi, s_sum;
     __shared__ int s_count;
     const int tid = threadIdx.x;
-    const int64_t q = blockIdx.x;
+    const int64_t slot = blockIdx.x;
+    const int64_t q = k.qlist ? k.qlist[slot] : slot;
+    const int64_t si = b.by_slot ? slot : q;        // the query's scratch
     const int T = query_budget(a.Tq, q, a.T), w = a.w;
     if (empty_query(k, q, T)) return;
-    const uint32_t *aux = b.aux + (size_t)q * 4;
+    const uint32_t *aux = b.aux + (size_t)si * 4;
     if (aux[3]) {                      // still over capacity: the device-wide path
         flag_overflow(k, q);
         return;
@@ -1283,9 +1291,9 @@ __global__ __launch_bounds__(kBigThreads) void k_rank_big_emit(BigArgs b) {
     const int n = (int)aux[0];
     const int bf = (int)aux[1];
     const float base = __uint_as_float(aux[2]);
-    const uint64_t *list = b.list + (size_t)q * b.cap;
-    uint64_t *S = b.sorted + (size_t)q * b.cap;
-    uint32_t *SZ = b.size + (size_t)q * b.cap;
+    const uint64_t *list = b.list + (size_t)si * b.cap;
+    uint64_t *S = b.sorted + (size_t)si * b.cap;
+    uint32_t *SZ = b.size + (size_t)si * b.cap;
     // the key range of the entries up to the cut
     uint32_t lo = ~0u, hi = 0u;
     for (int i = tid; i < n; i += kBigThreads) {
@@ -1536,7 +1544,7 @@ int flagged(sc_ctx *ctx, const RankCellsArgs &a, int32_t *list, int32_t *dn, int
 int rank_big(sc_ctx *ctx, const sc_index *ix, const RankKernelArgs &k, const RankCellsArgs &a) {
     BigArgs b{k,           ix->bg_n0,    ix->bg_pair, ix->bg_start, ix->brec, ix->ckeys,
               a.big_list,  a.big_sorted, a.big_size,  a.big_aux,    a.big_cap, 0,
-              1,           nullptr,      0};
+              1,           nullptr,      0,           0};
     b.k.qlist = nullptr;
     // batches of a few queries: each query over several workgroups (about
     // two per CU).  The shares start from a warm-up's cut but tighten only
@@ -1632,9 +1640,32 @@ int rank_cells(sc_ctx *ctx, const sc_index *ix, const RankCellsArgs &a) {
         SC_TRY(flagged(ctx, a, list, dn, &nflag));
         if (a.overflowed) *a.overflowed += nflag;
     }
-    // queries whose cells did not fit: once more with a list four times as
-    // long (fewer workgroups per CU, only for them), when the LDS allows
-    if (nflag > 0 && cap == kCap && !(a.big_cap > 0 && a.big_list)) {
+    // queries whose cells did not fit the on-chip list: the pruned ranking
+    // over them (scratch for a.retry_cap queries at a time) where the index
+    // has its directory; else once more with a list four times as long
+    // (fewer workgroups per CU, only for them), when the LDS allows
+    if (nflag > 0 && !(a.big_cap > 0 && a.big_list) && a.retry_cap > 0 && ix->brec &&
+        ix->bg_n0 && big_lds(ix->kc, ix->dim) <= 150 * 1024) {
+        const size_t lds = big_lds(ix->kc, ix->dim);
+        SC_HIP(hipFuncSetAttribute((const void *)k_rank_big,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        for (int32_t g0 = 0; g0 < nflag; g0 += a.retry_cap) {
+            const int32_t gn = std::min<int32_t>(a.retry_cap, nflag - g0);
+            BigArgs b{k, ix->bg_n0, ix->bg_pair, ix->bg_start, ix->brec, ix->ckeys,
+                      a.big_list, a.big_sorted, a.big_size, a.big_aux, kBigCap, 0,
+                      1, nullptr, 0, 1};
+            b.k.qlist = list + g0;
+            hipLaunchKernelGGL(k_rank_big, dim3((unsigned)gn), dim3(kBigThreads), lds,
+                               ctx->stream, b);
+            SC_CHECK_LAUNCH();
+            // a query still over the list's capacity is flagged for the
+            // device-wide sort by the emit kernel
+            hipLaunchKernelGGL(k_rank_big_emit, dim3((unsigned)gn), dim3(kBigThreads),
+                               4 * (3 * (size_t)kFine + 1), ctx->stream, b);
+            SC_CHECK_LAUNCH();
+        }
+        SC_TRY(flagged(ctx, a, list, dn, &nflag));
+    } else if (nflag > 0 && cap == kCap && !(a.big_cap > 0 && a.big_list)) {
         rc = launch_rank<W>(ctx, ix, k, list, nflag, 4 * kCap);
         if (rc == SC_OK) SC_TRY(flagged(ctx, a, list, dn, &nflag));
         else if (rc != SC_ERR_TOO_LARGE) return rc;
@@ -1701,6 +1732,12 @@ int sc_rank_big_cap(const sc_index *ix) {
         const int v = atoi(e);
         if (v >= 1) return std::min(v, kBigCap);
     }
+    return kBigCap;
+}
+
+int sc_rank_retry_cap(const sc_index *ix) {
+    if (!ix->sparse || !ix->brec || !ix->bg_n0 || ix->kc > 4096 || ix->ncells <= 0) return 0;
+    if (big_mode() == 0 || big_lds(ix->kc, ix->dim) > 150 * 1024) return 0;
     return kBigCap;
 }
 

@@ -2497,6 +2497,7 @@ struct Plan {
         off_list, off_stats, off_okey, off_okey2, off_oid, off_oid2, off_perm, off_otmp,
         otmp_bytes, total, off_big_list, off_big_sorted, off_big_size, off_big_aux, off_big_split;
     int big_cap;        // entries per query of the pruned ranking (0: not used)
+    int retry_cap;      // queries the pruned ranking can re-rank per pass (big_cap == 0)
 };
 
 Plan make_plan(const sc_index *ix, const sc_search_params *p, int64_t nq) {
@@ -2536,9 +2537,14 @@ Plan make_plan(const sc_index *ix, const sc_search_params *p, int64_t nq) {
     P.off_oid = take(4ull * chunk);
     P.off_oid2 = take(4ull * chunk);
     P.off_perm = take(4ull * chunk);
-    P.off_big_list = take(8ull * big * chunk);
-    P.off_big_sorted = take(8ull * big * chunk);
-    P.off_big_size = take(4ull * big * chunk);
+    // without the pruned ranking, scratch for it to re-rank up to 512 queries
+    // of a pass whose cells overflow the pair-table ranking
+    const int rcap = p->kind == SC_KIND_SCIMI && !big ? sc_rank_retry_cap(ix) : 0;
+    P.retry_cap = rcap ? (int)std::min<int64_t>(chunk, 512) : 0;
+    const size_t bigq = big ? (size_t)big * chunk : (size_t)rcap * P.retry_cap;
+    P.off_big_list = take(8ull * bigq);
+    P.off_big_sorted = take(8ull * bigq);
+    P.off_big_size = take(4ull * bigq);
     P.off_big_aux = take(16ull * chunk);
     // split-query state for batches of a few queries (4 + 2 x 4096 words each)
     P.off_big_split = take(big && chunk <= 4 ? 4ull * (4 + 2 * 4096) * chunk : 0);
@@ -3113,12 +3119,13 @@ static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
         if (halves) {
             RankCellsArgs ra{Q, dqc, qs, cells, clen, cinfo, qinfo, Tq, cn, p->w, p->T, P.ccap,
                              list, list + P.chunk};
-            if (P.big_cap > 0) {
+            if (P.big_cap > 0 || P.retry_cap > 0) {
                 ra.big_list = (uint64_t *)(s + P.off_big_list);
                 ra.big_sorted = (uint64_t *)(s + P.off_big_sorted);
                 ra.big_size = (uint32_t *)(s + P.off_big_size);
                 ra.big_aux = (uint32_t *)(s + P.off_big_aux);
                 ra.big_cap = P.big_cap;
+                ra.retry_cap = P.retry_cap;
                 if (P.chunk <= 4) ra.big_split = (uint32_t *)(s + P.off_big_split);
             }
             ra.overflowed = &fallback_total;

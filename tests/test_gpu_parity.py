@@ -305,14 +305,19 @@ def test_search_scimi_matches_oracle(ctx, case):
     ix.close()
 
 
+@pytest.mark.parametrize("big", ["auto", "0"])
 @pytest.mark.parametrize("pairs", ["1", "0"])
 @pytest.mark.parametrize("cap", ["16", "1"])
-def test_search_scimi_small_rank_list(ctx, cap, pairs, monkeypatch):
+def test_search_scimi_small_rank_list(ctx, cap, pairs, big, monkeypatch):
     """An on-chip ranking list of 16 (or 1) cells: queries go through the
-    bounded subset, the second histogram level and the device-wide sorted
-    fallback, with and without the pair tables; results must not change."""
+    bounded subset and the second histogram level, then the overflowing ones
+    through the pruned ranking (or, without it, a longer list and the
+    device-wide sorted fallback), with and without the pair tables; results
+    must not change."""
     monkeypatch.setenv("SC_RANK_CAP", cap)
     monkeypatch.setenv("SC_RANK_PAIRS", pairs)
+    if big == "0":
+        monkeypatch.setenv("SC_RANK_BIG", "0")
     ix, oix = _index_pair(ctx, S.KIND_SCIMI, kc=8)
     for R, w, T in ((100, 100000, 3000), (50, 7, 10 ** 6), (20, 10 ** 6, 10 ** 6)):
         _compare_search(ctx, ix, oix, S.KIND_SCIMI, R, w, T, nq=60)
