@@ -32,8 +32,16 @@
 namespace {
 
 constexpr int kRankThreads = 256;
-constexpr int kBuckets = 4096;     // 12 bits of the score key per level
-constexpr int kCap = 4096;         // cells sorted on chip
+constexpr int kBuckets = 4096;     // histogram buckets of the pruned ranking (below)
+// The pair-table and record rankings: score-key histograms of 2048 buckets,
+// 11 bits of the key per level (level 1: key >> 21, level 2: the next 11),
+// and an on-chip list of 2048 cells (16 KB together), so that four
+// workgroups fit a CU; queries whose cells do not fit go to the pruned
+// ranking (or, without it, a list four times as long).
+constexpr int kPB = 2048;
+constexpr int kL1 = 21, kL2 = 10;
+constexpr uint32_t kL2Mask = 0x7FFu, kLowBits = 0x3FFu;
+constexpr int kCap = 2048;         // cells sorted on chip
 
 struct RankKernelArgs {
     RankCellsArgs a;
@@ -100,20 +108,20 @@ __device__ uint32_t block_excl_scan(uint32_t x, uint32_t *wsum, uint32_t *total)
 
 // First bucket at which the running count reaches w or the running size sum
 // reaches T, given the counts/sizes before the histogram (pre_cnt, pre_w).
-// Returns kBuckets when no bucket does; *bcnt/*bw get the totals before the
+// Returns NB when no bucket does; *bcnt/*bw get the totals before the
 // found bucket (or of all buckets).
-template <int NT = kRankThreads>
+template <int NT = kRankThreads, int NB = kPB>
 __device__ int find_bucket(const uint32_t *hcnt, const uint32_t *hwt, uint32_t pre_cnt,
                            uint32_t pre_w, int w, int T, uint32_t *wsum, int *s_b,
                            uint32_t *s_pre) {
-    constexpr int per = kBuckets / NT;
+    constexpr int per = NB / NT;
     const int t = threadIdx.x;
     uint32_t c = 0, s = 0;
     for (int i = 0; i < per; ++i) { c += hcnt[t * per + i]; s += hwt[t * per + i]; }
     uint32_t tc, ts;
     const uint32_t bc = block_excl_scan<NT>(c, wsum, &tc);
     const uint32_t bs = block_excl_scan<NT>(s, wsum + NT / 64, &ts);
-    if (t == 0) *s_b = kBuckets;
+    if (t == 0) *s_b = NB;
     __syncthreads();
     uint32_t cc = pre_cnt + bc, cw = pre_w + bs;
     for (int i = 0; i < per; ++i) {
@@ -129,13 +137,13 @@ __device__ int find_bucket(const uint32_t *hcnt, const uint32_t *hwt, uint32_t p
     __syncthreads();
     const int b = *s_b;
     // the owner of the found bucket publishes the totals before it
-    if (b < kBuckets && t == b / per) {
+    if (b < NB && t == b / per) {
         uint32_t pc = pre_cnt + bc, pw = pre_w + bs;
         for (int i = t * per; i < b; ++i) { pc += hcnt[i]; pw += hwt[i]; }
         s_pre[0] = pc;
         s_pre[1] = pw;
     }
-    if (b == kBuckets && t == 0) {
+    if (b == NB && t == 0) {
         s_pre[0] = pre_cnt + tc;
         s_pre[1] = pre_w + ts;
     }
@@ -240,33 +248,33 @@ template <class Sc, class Each>
 __device__ bool hist_select_each(const Sc &sc, Each each, int w, int T, int cap, uint32_t *hcnt,
                                  uint32_t *hwt, uint64_t *lkey, RankShared &sh) {
     const int tid = threadIdx.x;
-    for (int i = tid; i < kBuckets; i += kRankThreads) { hcnt[i] = 0; hwt[i] = 0; }
+    for (int i = tid; i < kPB; i += kRankThreads) { hcnt[i] = 0; hwt[i] = 0; }
     __syncthreads();
     each([&](int64_t c) {
         const uint32_t key = sc_float_key(sc.score(c));
-        atomicAdd(&hcnt[key >> 20], 1u);
-        atomicAdd(&hwt[key >> 20], sc.size(c));
+        atomicAdd(&hcnt[key >> kL1], 1u);
+        atomicAdd(&hwt[key >> kL1], sc.size(c));
     });
     __syncthreads();
     const int b1 = find_bucket(hcnt, hwt, 0u, 0u, w, T, sh.wsum, &sh.b, sh.pre);
     const uint32_t pre1_cnt = sh.pre[0], pre1_w = sh.pre[1];
-    const uint32_t b1cnt = b1 < kBuckets ? hcnt[b1] : 0u;
+    const uint32_t b1cnt = b1 < kPB ? hcnt[b1] : 0u;
     __syncthreads();
-    int b2 = kBuckets - 1;
-    uint32_t need = b1 < kBuckets ? pre1_cnt + b1cnt : pre1_cnt;
-    if (need > (uint32_t)cap && b1 < kBuckets) {
-        for (int i = tid; i < kBuckets; i += kRankThreads) { hcnt[i] = 0; hwt[i] = 0; }
+    int b2 = kPB - 1;
+    uint32_t need = b1 < kPB ? pre1_cnt + b1cnt : pre1_cnt;
+    if (need > (uint32_t)cap && b1 < kPB) {
+        for (int i = tid; i < kPB; i += kRankThreads) { hcnt[i] = 0; hwt[i] = 0; }
         __syncthreads();
         each([&](int64_t c) {
             const uint32_t key = sc_float_key(sc.score(c));
-            if ((int)(key >> 20) != b1) return;
-            atomicAdd(&hcnt[(key >> 8) & 0xFFFu], 1u);
-            atomicAdd(&hwt[(key >> 8) & 0xFFFu], sc.size(c));
+            if ((int)(key >> kL1) != b1) return;
+            atomicAdd(&hcnt[(key >> kL2) & kL2Mask], 1u);
+            atomicAdd(&hwt[(key >> kL2) & kL2Mask], sc.size(c));
         });
         __syncthreads();
         const int f = find_bucket(hcnt, hwt, pre1_cnt, pre1_w, w, T, sh.wsum, &sh.b, sh.pre);
-        b2 = f < kBuckets ? f : kBuckets - 1;
-        need = f < kBuckets ? sh.pre[0] + hcnt[f] : sh.pre[0];
+        b2 = f < kPB ? f : kPB - 1;
+        need = f < kPB ? sh.pre[0] + hcnt[f] : sh.pre[0];
         __syncthreads();
     }
     if (need > (uint32_t)cap) return false;
@@ -274,8 +282,8 @@ __device__ bool hist_select_each(const Sc &sc, Each each, int w, int T, int cap,
     __syncthreads();
     each([&](int64_t c) {
         const uint32_t key = sc_float_key(sc.score(c));
-        const int kb1 = (int)(key >> 20);
-        if (kb1 < b1 || (kb1 == b1 && (int)((key >> 8) & 0xFFFu) <= b2)) {
+        const int kb1 = (int)(key >> kL1);
+        if (kb1 < b1 || (kb1 == b1 && (int)((key >> kL2) & kL2Mask) <= b2)) {
             const int o = atomicAdd(&sh.n, 1);
             lkey[o] = ((uint64_t)key << 32) | (uint32_t)c;
         }
@@ -302,31 +310,31 @@ template <class Sc, class Each>
 __device__ bool hist_stop(const Sc &sc, Each each, int w, int T, uint32_t *hcnt, uint32_t *hwt,
                           RankShared &sh, uint32_t *thr) {
     const int tid = threadIdx.x;
-    for (int i = tid; i < kBuckets; i += kRankThreads) { hcnt[i] = 0; hwt[i] = 0; }
+    for (int i = tid; i < kPB; i += kRankThreads) { hcnt[i] = 0; hwt[i] = 0; }
     __syncthreads();
     each([&](int64_t c) {
         const uint32_t key = sc_float_key(sc.score(c));
-        atomicAdd(&hcnt[key >> 20], 1u);
-        atomicAdd(&hwt[key >> 20], sc.size(c));
+        atomicAdd(&hcnt[key >> kL1], 1u);
+        atomicAdd(&hwt[key >> kL1], sc.size(c));
     });
     __syncthreads();
     const int b1 = find_bucket(hcnt, hwt, 0u, 0u, w, T, sh.wsum, &sh.b, sh.pre);
     const uint32_t pre1_cnt = sh.pre[0], pre1_w = sh.pre[1];
     __syncthreads();
-    if (b1 >= kBuckets) return false;
-    for (int i = tid; i < kBuckets; i += kRankThreads) { hcnt[i] = 0; hwt[i] = 0; }
+    if (b1 >= kPB) return false;
+    for (int i = tid; i < kPB; i += kRankThreads) { hcnt[i] = 0; hwt[i] = 0; }
     __syncthreads();
     each([&](int64_t c) {
         const uint32_t key = sc_float_key(sc.score(c));
-        if ((int)(key >> 20) != b1) return;
-        atomicAdd(&hcnt[(key >> 8) & 0xFFFu], 1u);
-        atomicAdd(&hwt[(key >> 8) & 0xFFFu], sc.size(c));
+        if ((int)(key >> kL1) != b1) return;
+        atomicAdd(&hcnt[(key >> kL2) & kL2Mask], 1u);
+        atomicAdd(&hwt[(key >> kL2) & kL2Mask], sc.size(c));
     });
     __syncthreads();
     const int b2 = find_bucket(hcnt, hwt, pre1_cnt, pre1_w, w, T, sh.wsum, &sh.b, sh.pre);
     __syncthreads();
     // the level-1 bucket reaches the budget, so some level-2 bucket in it does
-    *thr = ((uint32_t)b1 << 20) | ((uint32_t)min(b2, kBuckets - 1) << 8) | 0xFFu;
+    *thr = ((uint32_t)b1 << kL1) | ((uint32_t)min(b2, kPB - 1) << kL2) | kLowBits;
     return true;
 }
 
@@ -541,15 +549,15 @@ __device__ void flag_overflow(const RankKernelArgs &k, int64_t q) {
 __host__ __device__ inline size_t rank_abytes(int cap) {
     int np2 = 64;
     while (np2 < cap) np2 <<= 1;
-    return 8 * (size_t)(np2 > kBuckets ? np2 : kBuckets);
+    return 8 * (size_t)(np2 > kPB ? np2 : kPB);
 }
 
 // Ranking from the cells' records (any index shape).
 template <int W>
 __global__ __launch_bounds__(kRankThreads) void k_rank_cells(RankKernelArgs k) {
     extern __shared__ __align__(16) unsigned char smem[];
-    uint32_t *hcnt = reinterpret_cast<uint32_t *>(smem);           // [kBuckets]
-    uint32_t *hwt = hcnt + kBuckets;                               // [kBuckets]
+    uint32_t *hcnt = reinterpret_cast<uint32_t *>(smem);           // [kPB]
+    uint32_t *hwt = hcnt + kPB;                                    // [kPB]
     uint64_t *lkey = reinterpret_cast<uint64_t *>(smem);           // [cap], after the histograms
     float *v = reinterpret_cast<float *>(smem + rank_abytes(k.cap));   // [2 kc]
     __shared__ RankShared sh;
@@ -575,7 +583,7 @@ __global__ __launch_bounds__(kRankThreads) void k_rank_cells(RankKernelArgs k) {
 // selection runs over those pairs' cells, and over all cells when the subset
 // has too many pairs.
 constexpr int kMaxPairs0 = 2048;     // half-0 pairs sorted on chip for the bound
-constexpr int kMaxPass = 512;        // pairs whose cells are visited by all threads at once
+constexpr int kMaxPass = 256;        // pairs whose cells are visited by all threads at once
 
 // Every cell of the groups (half-0 pairs) gbeg/gpre describe is handed to f
 // (group slot, cell), all threads at once: gpre[j] is the number of cells in
@@ -684,8 +692,8 @@ __device__ void emit_small(const Sc &sc, const RankKernelArgs &k, int64_t q, uin
 
 __global__ __launch_bounds__(kRankThreads) void k_rank_pairs(RankKernelArgs k) {
     extern __shared__ __align__(16) unsigned char smem[];
-    uint32_t *hcnt = reinterpret_cast<uint32_t *>(smem);           // [kBuckets]
-    uint32_t *hwt = hcnt + kBuckets;                               // [kBuckets]
+    uint32_t *hcnt = reinterpret_cast<uint32_t *>(smem);           // [kPB]
+    uint32_t *hwt = hcnt + kPB;                                    // [kPB]
     uint64_t *lkey = reinterpret_cast<uint64_t *>(smem);           // [cap]
     float *v = reinterpret_cast<float *>(smem + rank_abytes(k.cap));   // [2 kc]
     const int nu0 = k.nu0, nu1 = k.nu1;
@@ -723,11 +731,11 @@ __global__ __launch_bounds__(kRankThreads) void k_rank_pairs(RankKernelArgs k) {
     // the subset: the half-0 pairs of least st0 whose cells together reach
     // the budget (a weighted histogram of st0 over the pairs, weighted by
     // their cells' count and size), and their cells
-    for (int i = tid; i < kBuckets; i += kRankThreads) { hcnt[i] = 0; hwt[i] = 0; }
+    for (int i = tid; i < kPB; i += kRankThreads) { hcnt[i] = 0; hwt[i] = 0; }
     __syncthreads();
     for (int g = tid; g < nu0; g += kRankThreads) {
         const int c0 = k.gstart[g], c1 = k.gstart[g + 1];
-        const uint32_t b = sc_float_key(st0[g]) >> 20;
+        const uint32_t b = sc_float_key(st0[g]) >> kL1;
         atomicAdd(&hcnt[b], (uint32_t)(c1 - c0));
         atomicAdd(&hwt[b], (uint32_t)(k.L[c1 - 1] - (c0 > 0 ? k.L[c0 - 1] : 0)));
     }
@@ -740,7 +748,7 @@ __global__ __launch_bounds__(kRankThreads) void k_rank_pairs(RankKernelArgs k) {
     if (tid == 0) { sh.n = 0; s_np = 0; s_acc = 0; }
     __syncthreads();
     for (int g = tid; g < nu0; g += kRankThreads) {
-        const int gb = (int)(sc_float_key(st0[g]) >> 20);
+        const int gb = (int)(sc_float_key(st0[g]) >> kL1);
         if (gb > b0) continue;
         if (gb == b0) {
             const int c0 = k.gstart[g], c1 = k.gstart[g + 1];
@@ -1201,7 +1209,7 @@ __global__ __launch_bounds__(kBigThreads, 8) void k_rank_big(BigArgs b) {
             // warm-up only: its exact crossing bucket (a subset's, so a bound)
             // is the starting cut of the query's workgroups
             __syncthreads();
-            const int wf = find_bucket<kBigThreads>(hcnt, hmass, 0u, 0u, w, T, sh.wsum, &sh.b,
+            const int wf = find_bucket<kBigThreads, kBuckets>(hcnt, hmass, 0u, 0u, w, T, sh.wsum, &sh.b,
                                                     sh.pre);
             if (tid == 0) {
                 atomicMin(reinterpret_cast<int *>(sp), min(wf, s_bcur));
@@ -1245,7 +1253,7 @@ __global__ __launch_bounds__(kBigThreads, 8) void k_rank_big(BigArgs b) {
         __syncthreads();
     }
     // the exact crossing bucket of the final histograms
-    const int bf = find_bucket<kBigThreads>(hcnt, hmass, 0u, 0u, w, T, sh.wsum, &sh.b, sh.pre);
+    const int bf = find_bucket<kBigThreads, kBuckets>(hcnt, hmass, 0u, 0u, w, T, sh.wsum, &sh.b, sh.pre);
     if (tid == 0) {
         aux[0] = (uint32_t)s_napp;
         aux[1] = (uint32_t)bf;
