@@ -700,6 +700,8 @@ struct ScanShared {
     uint64_t sel_prefix;
     int sel_rem;
     int run_cells;
+    uint32_t b_lo, b_hi;         // select_bound: the refined range of score keys
+    int b_below, b_cnt;          // entries below it, and in it
 };
 
 // Radix select: the R-th smallest key among key[0..nb).  All threads call.
@@ -711,6 +713,7 @@ __device__ uint64_t select_kth(ScanShared &S, int nb, int R) {
         for (int b = threadIdx.x; b < 256; b += blockDim.x) S.hist[b] = 0;
         __syncthreads();
         const uint64_t prefix = S.sel_prefix;
+        const uint32_t rem = (uint32_t)S.sel_rem;   // read by all before the update below
         for (int i = threadIdx.x; i < nb; i += blockDim.x) {
             uint64_t k = S.key[i];
             if ((k & mask) == prefix) atomicAdd(&S.hist[(k >> shift) & 255u], 1u);
@@ -727,7 +730,6 @@ __device__ uint64_t select_kth(ScanShared &S, int nb, int R) {
                 if (l >= o) incl += t;
             }
             uint32_t excl = incl - tot;
-            const uint32_t rem = (uint32_t)S.sel_rem;
             if (excl < rem && rem <= incl) {
                 uint32_t c[4] = {c0, c1, c2, c3};
                 uint32_t acc = excl;
@@ -771,46 +773,153 @@ __device__ void compact(ScanShared &S, int R) {
     __syncthreads();
 }
 
-// Mixed form: keeps every entry whose approximate distance is within twoE of
-// the R-th smallest (a true top-R candidate is never dropped, see the mixed
-// IMI scan); when they would not leave room for a scan round, the query is
-// marked for an exact redo and only the R smallest are kept.
-__device__ void compact_margin(ScanShared &S, int R, float twoE) {
-    const int nb = S.nb;
-    const uint64_t kth = select_kth(S, nb, R);
-    const float aR = sc_key_float((uint32_t)(kth >> 32));
-    const uint32_t kthr = sc_float_key(aR + twoE);
-    const uint64_t keep = ((uint64_t)kthr << 32) | 0xFFFFFFFFull;
-    __shared__ int s_cnt;
-    if (threadIdx.x == 0) s_cnt = 0;
+// An upper bound on the R-th smallest score (the high key word) among
+// key[0..nb) (R <= nb), at most one histogram bucket above it: the score
+// keys' range [min, max] is cut into 256 equal buckets, the bucket holding
+// the R-th smallest is found and, while it holds more than 32 entries, cut
+// again.  Unlike a radix select on fixed bits, the buckets follow the data,
+// so the histogram's LDS atomics spread over many addresses.  *n_le gets
+// the number of entries whose score key is at most the bound.  All threads
+// call.
+__device__ uint32_t select_bound(ScanShared &S, int nb, int R, int *n_le) {
+    const int tid = threadIdx.x;
+    if (tid == 0) { S.b_lo = ~0u; S.b_hi = 0u; S.b_below = 0; }
     __syncthreads();
-    int c = 0;
-    for (int i = threadIdx.x; i < nb; i += blockDim.x) c += S.key[i] <= keep;
-    atomicAdd(&s_cnt, c);
-    __syncthreads();
-    const int cnt = s_cnt;
-    if (cnt > kBufCap - kRound) {
-        if (threadIdx.x == 0) S.overflow = 1;
-        compact(S, R);
-        return;
+    uint32_t mn = ~0u, mx = 0u;
+    for (int i = tid; i < nb; i += blockDim.x) {
+        const uint32_t k = (uint32_t)(S.key[i] >> 32);
+        mn = min(mn, k);
+        mx = max(mx, k);
     }
-    __shared__ int s_w;
-    if (threadIdx.x == 0) s_w = 0;
+    for (int o = 32; o > 0; o >>= 1) {
+        mn = min(mn, (uint32_t)__shfl_xor(mn, o));
+        mx = max(mx, (uint32_t)__shfl_xor(mx, o));
+    }
+    if ((tid & 63) == 0) { atomicMin(&S.b_lo, mn); atomicMax(&S.b_hi, mx); }
     __syncthreads();
-    uint64_t mk[kBufCap / kScanThreads];
-    int32_t mi[kBufCap / kScanThreads];
-    int m = 0;
-    for (int i = threadIdx.x; i < nb; i += blockDim.x)
-        if (S.key[i] <= keep) { mk[m] = S.key[i]; mi[m] = S.idx[i]; ++m; }
+    for (int pass = 0; pass < 4; ++pass) {
+        const uint32_t lo = S.b_lo, hi = S.b_hi, width = hi - lo;
+        const int below = S.b_below;   // read by all before the update below
+        const int shift = width < 256u ? 0 : 32 - __clz(width) - 8;   // (width >> shift) < 256
+        for (int b = tid; b < 256; b += blockDim.x) S.hist[b] = 0;
+        __syncthreads();
+        for (int i = tid; i < nb; i += blockDim.x) {
+            const uint32_t k = (uint32_t)(S.key[i] >> 32);
+            if (k >= lo && k <= hi) atomicAdd(&S.hist[(k - lo) >> shift], 1u);
+        }
+        __syncthreads();
+        if (tid < 64) {
+            // wave 0: inclusive scan of the 256 bins (4 per lane), then the
+            // bin where the count reaches R
+            const int l = tid;
+            const uint32_t c0 = S.hist[4 * l], c1 = S.hist[4 * l + 1], c2 = S.hist[4 * l + 2],
+                           c3 = S.hist[4 * l + 3];
+            const uint32_t tot = c0 + c1 + c2 + c3;
+            uint32_t incl = tot;
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t t = __shfl_up(incl, o);
+                if (l >= o) incl += t;
+            }
+            const uint32_t rem = (uint32_t)(R - below), excl = incl - tot;
+            if (excl < rem && rem <= incl) {
+                const uint32_t c[4] = {c0, c1, c2, c3};
+                uint32_t acc = excl;
+                for (int b = 0; b < 4; ++b) {
+                    if (rem <= acc + c[b]) {
+                        const uint32_t blo = lo + ((uint32_t)(4 * l + b) << shift);
+                        const uint32_t bw = shift == 0 ? 0u : (1u << shift) - 1u;
+                        S.b_lo = blo;
+                        S.b_hi = hi - blo < bw ? hi : blo + bw;
+                        S.b_below = below + (int)acc;
+                        S.b_cnt = (int)c[b];
+                        break;
+                    }
+                    acc += c[b];
+                }
+            }
+        }
+        __syncthreads();
+        if (S.b_cnt <= 32 || S.b_lo == S.b_hi) break;
+    }
+    *n_le = S.b_below + S.b_cnt;
+    const uint32_t ub = S.b_hi;
     __syncthreads();
-    for (int j = 0; j < m; ++j) {
-        const int o = atomicAdd(&s_w, 1);
-        S.key[o] = mk[j];
-        S.idx[o] = mi[j];
+    return ub;
+}
+
+__device__ uint32_t run_excl_scan(uint32_t x, uint32_t *wsum, uint32_t *total);
+
+// Keeps the entries whose score key is at most `keep` (a score key, high
+// word) at the front of key/idx, when there are at most min(limit,
+// kBufCap / 2) of them; returns false (nothing moved) otherwise.  One
+// prefix over the workgroup of each thread's kept count (its entries are
+// positions tid + j * kScanThreads) gives the slots.  The entries of the
+// first half are in registers before its barrier; the second half's are
+// read after it, and no slot (all below kBufCap / 2) overwrites them.  All
+// threads call.
+__device__ bool keep_at_most(ScanShared &S, uint32_t keep, int limit) {
+    constexpr int J = kBufCap / kScanThreads, H = J / 2;
+    const int nb = S.nb, tid = threadIdx.x;
+    uint64_t mk[H];
+    int32_t mi[H];
+    uint32_t bits = 0;
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        const int i = tid + j * kScanThreads;
+        uint64_t k = ~0ull;
+        if (i < nb) k = S.key[i];
+        if (j < H) {
+            mk[j] = k;
+            mi[j] = i < nb ? S.idx[i] : 0;
+        }
+        if (i < nb && (uint32_t)(k >> 32) <= keep) bits |= 1u << j;
+    }
+    uint32_t total;
+    uint32_t o = run_excl_scan((uint32_t)__popc(bits), S.scan_wsum, &total);
+    if ((int)total > limit || total > (uint32_t)(kBufCap / 2)) {
+        __syncthreads();
+        return false;
+    }
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        if ((bits >> j) & 1u) {
+            if (j < H) {
+                S.key[o] = mk[j];
+                S.idx[o] = mi[j];
+            } else {
+                const int i = tid + j * kScanThreads;
+                const uint64_t k = S.key[i];
+                const int32_t x = S.idx[i];
+                S.key[o] = k;
+                S.idx[o] = x;
+            }
+            ++o;
+        }
     }
     __syncthreads();
-    if (threadIdx.x == 0) { S.nb = cnt; S.thr = keep; }
+    if (tid == 0) { S.nb = (int)total; S.thr = ((uint64_t)keep << 32) | 0xFFFFFFFFull; }
     __syncthreads();
+    return true;
+}
+
+// Shrinks the buffer (R < nb) to the entries that can still be among the R
+// smallest: score at most a bound on the R-th smallest score, plus `twoE`
+// in the mixed form (every entry within 2 E_q of the R-th smallest
+// approximate distance, see the mixed IMI scan).  Dropped entries have R
+// kept ones of smaller key, so the final top-R is unchanged.  When more
+// than min(limit, kBufCap / 2) entries would stay, the plain form keeps the
+// R smallest keys exactly; the mixed form, during the scan (limit <
+// kBufCap), marks the query for an exact redo and does the same, and after
+// it leaves the buffer as it is.  Not inlined: the scan rounds keep the
+// next round's loads in registers across it.
+__device__ __noinline__ void compact_bound(ScanShared &S, int R, float twoE, bool margin, int limit) {
+    int n_le;
+    const uint32_t ub = select_bound(S, S.nb, R, &n_le);
+    const uint32_t keep = margin ? sc_float_key(sc_key_float(ub) + twoE) : ub;
+    if (keep_at_most(S, keep, limit)) return;
+    if (margin && limit >= kBufCap) return;
+    if (margin && threadIdx.x == 0) S.overflow = 1;
+    compact(S, R);
 }
 
 // Sorts key/idx[0..n) ascending (n <= kBufCap) by bitonic sort.
@@ -872,7 +981,7 @@ __device__ void scan_run_generic(const ScanArgs &a, ScanShared &S, const float *
     for (int p0 = 0; p0 < rtotal; p0 += kRound) {
         const bool full = S.nb > kBufCap - kRound;   // read by all before any append
         __syncthreads();
-        if (full) compact(S, a.R);
+        if (full) compact_bound(S, a.R, 0.0f, false, kBufCap - kRound);
         for (int u = 0; u < U; ++u) {
             const int p = p0 + u * NT + tid;
             if (p >= rtotal) continue;
@@ -931,7 +1040,7 @@ __device__ void scan_run_real(const ScanArgs &a, ScanShared &S, const float *qv,
     for (int p0 = 0; p0 < rtotal; p0 += kRound) {
         const bool full = S.nb > kBufCap - kRound;   // read by all before any append
         __syncthreads();
-        if (full) compact(S, a.R);
+        if (full) compact_bound(S, a.R, 0.0f, false, kBufCap - kRound);
         for (int u = 0; u < U; ++u) {
             const int p = p0 + u * NT + tid;
             if (p >= rtotal) continue;
@@ -1047,7 +1156,7 @@ __device__ void scan_run_packed(const ScanArgs &a, ScanShared &S, const float *l
         if (p0 + kRound < rtotal) nxt.fetch(a, S, rn, rtotal, p0 + kRound);
         const bool full = S.nb > kBufCap - kRound;   // read by all before any append
         __syncthreads();
-        if (full) compact(S, a.R);
+        if (full) compact_bound(S, a.R, 0.0f, false, kBufCap - kRound);
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             if (cur.vx[u] < 0) continue;
@@ -1084,7 +1193,7 @@ __device__ void scan_run_imi_flat(const ScanArgs &a, ScanShared &S, const float 
         if (p0 + kRound < rtotal) nxt.fetch(a, S, rn, rtotal, p0 + kRound);
         const bool full = S.nb > kBufCap - kRound;   // read by all before any append
         __syncthreads();
-        if (full) compact(S, a.R);
+        if (full) compact_bound(S, a.R, 0.0f, false, kBufCap - kRound);
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             if (cur.vx[u] < 0) continue;
@@ -1125,7 +1234,7 @@ __device__ void scan_run_imi_flat(const ScanArgs &a, ScanShared &S, const float 
 //     A = (d0 + S * sum_k f16(diff_qr[k][c_k] / S)) + e_v
 // (8 FP16 LDS lookups and one coalesced load instead of 16 lookups, 8 of them
 // L2 gathers).  |A - E| <= E_q for the bound of mixed_bound(); every candidate
-// with A within 2 E_q of the running R-th smallest A is kept (compact_margin),
+// with A within 2 E_q of the running R-th smallest A is kept (compact_bound),
 // which contains the exact top-R, and the kept ones are re-ranked with E.
 template <int MPW>
 __device__ void scan_run_imi_mixed(const ScanArgs &a, ScanShared &S, const _Float16 *lut16,
@@ -1139,7 +1248,7 @@ __device__ void scan_run_imi_mixed(const ScanArgs &a, ScanShared &S, const _Floa
         if (p0 + kRound < rtotal) nxt.fetch(a, S, rn, rtotal, p0 + kRound);
         const bool full = S.nb > kBufCap - kRound;   // read by all before any append
         __syncthreads();
-        if (full) compact_margin(S, a.R, twoE);
+        if (full) compact_bound(S, a.R, twoE, true, kBufCap - kRound);
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             if (cur.vx[u] < 0) continue;
@@ -1346,6 +1455,9 @@ __device__ void scan_query(const ScanArgs &a, ScanShared &S, float *lut, int64_t
             if (tid == 0 && a.redo) a.redo[q * 8 + 7] = 1;
             return;
         }
+        // only the entries within the margin of the final R-th smallest
+        // approximate distance can be in the exact top-R
+        if (sum >= a.R && S.nb > a.R) compact_bound(S, a.R, S.twoE, true, kBufCap);
         // exact re-rank of the kept candidates (their positions stay)
         for (int i = tid; i < S.nb; i += blockDim.x) {
             const float e = imi_exact(a, dqr, dqc, qsum, S.idx[i]);
@@ -1353,13 +1465,13 @@ __device__ void scan_query(const ScanArgs &a, ScanShared &S, float *lut, int64_t
         }
         __syncthreads();
     }
-    // final selection
+    // final selection: the R smallest keys, ascending
     const int R = a.R;
     int nout;
     if (sum >= R) {
-        if (S.nb > R) compact(S, R);
+        if (S.nb > R) compact_bound(S, R, 0.0f, false, kBufCap);
         nout = R;
-        sort_buffer(S, R);
+        sort_buffer(S, S.nb);
     }
     int32_t *ids = a.ids + (size_t)q * R;
     float *dists = a.dists + (size_t)q * R;

@@ -3,10 +3,10 @@
 The device code of second-candidate_amd/csrc/sc_search.hip (its anonymous
 namespace up to the host-side plan code) is compiled as C++ against shim.h and
 linked with the oracle library and driver.inc.  Usage:
-    python make_emu.py OUT_BINARY SANITIZER [encode|trav3|travw]   (SANITIZER: address | thread)
+    python make_emu.py OUT_BINARY SANITIZER [encode|trav3|travw|scan]   (SANITIZER: address | thread)
 With `encode`, the MFMA encode kernel of sc_encode_mfma.hip instead; with
 `trav3` / `travw`, the three-candidate / one-workgroup-per-query traversal
-kernels with their own drivers.
+kernels, and with `scan`, the per-query scan kernel, with their own drivers.
 """
 import os
 import re
@@ -54,7 +54,7 @@ def generate_encode(path):
     open(path, "w").write("\n".join(out))
 
 
-def build(out, sanitizer, cxx=None, encode=False, trav3=False, travw=False):
+def build(out, sanitizer, cxx=None, encode=False, trav3=False, travw=False, scan=False):
     cxx = cxx or os.environ.get("EMU_CXX", "/opt/rocm/llvm/bin/clang++")
     cpp = out + ".cpp"
     if encode:
@@ -63,6 +63,8 @@ def build(out, sanitizer, cxx=None, encode=False, trav3=False, travw=False):
         generate(cpp, "driver_trav3.inc")
     elif travw:
         generate(cpp, "driver_travw.inc")
+    elif scan:
+        generate(cpp, "driver_scan.inc")
     else:
         generate(cpp)
     cmd = [cxx, "-std=c++17", "-O1", "-g", "-fsanitize=" + sanitizer, "-fno-omit-frame-pointer",
@@ -75,4 +77,4 @@ def build(out, sanitizer, cxx=None, encode=False, trav3=False, travw=False):
 if __name__ == "__main__":
     mode = sys.argv[3] if len(sys.argv) > 3 else ""
     build(sys.argv[1], sys.argv[2], encode=mode == "encode", trav3=mode == "trav3",
-          travw=mode == "travw")
+          travw=mode == "travw", scan=mode == "scan")

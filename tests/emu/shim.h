@@ -25,6 +25,7 @@
 #define __device__
 #define __host__
 #define __forceinline__ inline
+#define __noinline__ __attribute__((noinline))
 #define __launch_bounds__(...)
 #define amdgpu_waves_per_eu(...) unused
 #define __restrict__
@@ -45,6 +46,8 @@ inline int32_t __float_as_int(float f) { int32_t u; memcpy(&u, &f, 4); return u;
 inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
 inline int __ffsll(long long x) { return __builtin_ffsll(x); }
 inline int __clzll(long long x) { return x == 0 ? 64 : __builtin_clzll((unsigned long long)x); }
+inline int __clz(int x) { return x == 0 ? 32 : __builtin_clz((unsigned)x); }
+inline int __popc(unsigned x) { return __builtin_popcount(x); }
 inline uint64_t wall_clock64() { return 0; }
 using std::min; using std::max;
 

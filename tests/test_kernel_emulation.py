@@ -121,3 +121,50 @@ def test_traverse_wave_kernel_emulation(travw, name, args):
     assert "ERROR: AddressSanitizer" not in log, log[-4000:]
     assert r.returncode == 0, log[-4000:]
     assert ("OK: 0 of %s queries differ" % args.split()[5]) in log, log[-4000:]
+
+
+# The per-query scan kernel: n kc R w T nq mixed [margin_scale].  More than
+# kRound candidates per query exercise the buffer compaction (bounded
+# select, register-held move), runs of more than kRunCells cells the run
+# staging, T below R the position-ordered output, and a widened margin the
+# mixed form's exact-redo flag.
+SCAN_CASES = [
+    ("plain, compactions", "8000 32 100 1000 3000 4 0"),
+    ("mixed, compactions", "8000 32 100 1000 3000 4 1"),
+    ("mixed, margin x40", "8000 32 100 1000 3000 4 1 40"),
+    ("mixed, margin x2000: redo flags", "8000 32 100 1000 3000 4 1 2000"),
+    ("plain, fewer candidates than R", "8000 32 100 300 60 4 0"),
+    ("mixed, fewer candidates than R", "8000 32 100 300 60 4 1"),
+    ("plain, R 8", "8000 32 8 2000 5000 3 0"),
+    ("mixed, R 1000", "8000 32 1000 2000 5000 3 1"),
+]
+
+
+@pytest.fixture(scope="module", params=["thread", "address"])
+def scan(request, tmp_path_factory):
+    out = str(tmp_path_factory.mktemp("emus") / ("scan_" + request.param))
+    make_emu.build(out, request.param, CXX, scan=True)
+    return request.param, out
+
+
+@pytest.mark.parametrize("name,args", SCAN_CASES, ids=[c[0] for c in SCAN_CASES])
+def test_scan_kernel_emulation(scan, name, args):
+    """The per-query ADC scan and top-R kernel (256 host threads per
+    workgroup) under ThreadSanitizer and AddressSanitizer, against the
+    oracle's search over the same traced cells: identical ids, distances and
+    candidate counts; queries flagged for an exact redo are not compared
+    (and with the bound's own margin none is)."""
+    san, binary = scan
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0", TSAN_OPTIONS="report_signal_unsafe=0")
+    r = subprocess.run([binary] + args.split(), capture_output=True, text=True, timeout=600,
+                       env=env)
+    log = r.stdout + r.stderr
+    assert "WARNING: ThreadSanitizer" not in log, log[-4000:]
+    assert "ERROR: AddressSanitizer" not in log, log[-4000:]
+    assert r.returncode == 0, log[-4000:]
+    nq = args.split()[5]
+    assert ("OK: 0 of %s queries differ" % nq) in log, log[-4000:]
+    if len(args.split()) < 8:
+        assert "(0 flagged" in log, log[-4000:]
+    if "redo flags" in name:
+        assert ("(%s flagged" % nq) in log, log[-4000:]
