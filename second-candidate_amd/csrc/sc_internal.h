@@ -95,6 +95,11 @@ int kmeans_assign_bf16(sc_ctx *ctx, const float *x, int64_t n, int d, const floa
 // Sets the context's device current on the calling thread.
 int sc_ctx_enter(sc_ctx *ctx);
 
+// hipFuncAttributeMaxDynamicSharedMemorySize of kern on the current device
+// raised to at least `bytes` (one runtime call per kernel, device and new
+// maximum).
+int sc_set_max_lds(const void *kern, size_t bytes);
+
 static inline int64_t sc_cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 // Philox4x32-10 (Salmon et al., SC'11), device and host.
@@ -160,8 +165,21 @@ struct RankCellsArgs {
     uint32_t *big_split;     // [nq][kSplitWords] split-query state (small batches) or null
     int64_t *overflowed;     // host, may be null: += queries that outgrew the first list
     unsigned long long *stats;   // device, may be null: [2] += cells scored
+    // Deferred finish (pair-table ranking with the pruned re-ranking of the
+    // queries that outgrow its list), when both are set: no host
+    // synchronisation; flag_count[0] gets the number of queries that outgrew
+    // the first list, defer_list / defer_count[0] the queries still
+    // unranked after the re-ranking of the first retry_cap of them (their
+    // qinfo says zero cells), *deferred (host) is set to 1 and `overflowed`
+    // is left to the caller.  sc_rank_finish completes them.
+    int32_t *defer_list = nullptr, *defer_count = nullptr;
+    int *deferred = nullptr;
 };
 int sc_rank_cells(sc_ctx *ctx, const sc_index *ix, const RankCellsArgs &a);
+// Ranks the n queries of list (device) that a deferred sc_rank_cells left
+// unranked, with the same arguments.
+int sc_rank_finish(sc_ctx *ctx, const sc_index *ix, const RankCellsArgs &a, const int32_t *list,
+                   int32_t n);
 // Entries per query of the pruned ranking's scratch, or 0 when the index is
 // ranked by the other kernels.
 int sc_rank_big_cap(const sc_index *ix);

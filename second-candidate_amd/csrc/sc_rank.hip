@@ -906,6 +906,8 @@ struct BigArgs {
     // query's slot in k.qlist rather than by the query (a re-run of a few
     // queries of a batch with scratch for those few only)
     int by_slot;
+    // with k.qlist: workgroups at or past *qcount (device) exit at once
+    const int32_t *qcount = nullptr;
 };
 constexpr int kSplitWords = 4 + 2 * 4096;
 
@@ -1003,6 +1005,7 @@ __global__ __launch_bounds__(kBigThreads, 8) void k_rank_big(BigArgs b) {
     __shared__ uint32_t s_cnt, s_mass;
     __shared__ uint32_t scnt[64], smass[64];      // super-buckets: 64 buckets each
     __shared__ unsigned long long s_scored;
+    if (b.qcount && (int)blockIdx.x >= *b.qcount) return;   // the whole workgroup
     const int tid = threadIdx.x, lane = tid & 63;
     const int P = b.mode == 2 ? b.parts : 1, part = (int)(blockIdx.x % (unsigned)P);
     const int64_t slot = blockIdx.x / P;
@@ -1285,6 +1288,7 @@ __global__ __launch_bounds__(kBigThreads) void k_rank_big_emit(BigArgs b) {
     __shared__ RankShared sh;
     __shared__ uint32_t s_lo, s_hi, s_sum;
     __shared__ int s_count;
+    if (b.qcount && (int)blockIdx.x >= *b.qcount) return;   // the whole workgroup
     const int tid = threadIdx.x;
     const int64_t slot = blockIdx.x;
     const int64_t q = k.qlist ? k.qlist[slot] : slot;
@@ -1524,7 +1528,7 @@ int launch_rank(sc_ctx *ctx, const sc_index *ix, RankKernelArgs k, const int32_t
                        (pairs ? 4 * ((size_t)ix->nu[0] + ix->nu[1]) : 0);
     if (lds > 150 * 1024) return SC_ERR_TOO_LARGE;
     const void *kern = pairs ? (const void *)k_rank_pairs : (const void *)k_rank_cells<W>;
-    SC_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    SC_TRY(sc_set_max_lds(kern, lds));
     if (pairs)
         hipLaunchKernelGGL(k_rank_pairs, dim3((unsigned)nq), dim3(kRankThreads), lds, ctx->stream,
                            k);
@@ -1578,8 +1582,7 @@ int rank_big(sc_ctx *ctx, const sc_index *ix, const RankKernelArgs &k, const Ran
         sc_set_error("sc_search: kc=%d too large for the pruned cell ranking", ix->kc);
         return SC_ERR_ARG;
     }
-    SC_HIP(hipFuncSetAttribute((const void *)k_rank_big,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    SC_TRY(sc_set_max_lds((const void *)k_rank_big, lds));
     if (b.parts > 1) {
         // the warm-up on one workgroup per query, then the query's shares
         BigArgs wu = b;
@@ -1628,9 +1631,100 @@ int big_mode() {
 }
 
 template <int W>
+RankKernelArgs rank_args(const sc_index *ix, const RankCellsArgs &a) {
+    return RankKernelArgs{a, ix->kc, ix->dim, ix->ncells, ix->crec, ix->L, nullptr, rank_cap(),
+                          ix->nu[0], ix->nu[1], ix->upairs[0], ix->upairs[1], ix->cpair,
+                          ix->gstart};
+}
+
+// The pruned re-ranking is available for queries whose cells outgrow the
+// on-chip list (scratch for a.retry_cap queries at a time).
+bool retry_ok(const sc_index *ix, const RankCellsArgs &a) {
+    return !(a.big_cap > 0 && a.big_list) && a.retry_cap > 0 && a.big_list && ix->brec &&
+           ix->bg_n0 && big_lds(ix->kc, ix->dim) <= 150 * 1024;
+}
+
+BigArgs retry_args(const sc_index *ix, const RankKernelArgs &k, const RankCellsArgs &a) {
+    return BigArgs{k,         ix->bg_n0,    ix->bg_pair, ix->bg_start, ix->brec, ix->ckeys,
+                   a.big_list, a.big_sorted, a.big_size,  a.big_aux,    kBigCap,  0,
+                   1,          nullptr,      0,           1};
+}
+
+// The pruned re-ranking of the n queries of qlist, a.retry_cap at a time; a
+// query still over the list's capacity is flagged for the device-wide sort
+// by the emit kernel.
+int retry_groups(sc_ctx *ctx, const sc_index *ix, const RankKernelArgs &k,
+                 const RankCellsArgs &a, const int32_t *qlist, int32_t n) {
+    const size_t lds = big_lds(ix->kc, ix->dim);
+    SC_TRY(sc_set_max_lds((const void *)k_rank_big, lds));
+    for (int32_t g0 = 0; g0 < n; g0 += a.retry_cap) {
+        const int32_t gn = std::min<int32_t>(a.retry_cap, n - g0);
+        BigArgs b = retry_args(ix, k, a);
+        b.k.qlist = qlist + g0;
+        hipLaunchKernelGGL(k_rank_big, dim3((unsigned)gn), dim3(kBigThreads), lds, ctx->stream,
+                           b);
+        SC_CHECK_LAUNCH();
+        hipLaunchKernelGGL(k_rank_big_emit, dim3((unsigned)gn), dim3(kBigThreads),
+                           4 * (3 * (size_t)kFine + 1), ctx->stream, b);
+        SC_CHECK_LAUNCH();
+    }
+    return SC_OK;
+}
+
+// The device-wide ranking of the nflag queries of list: every cell's key,
+// a segmented sort per query and the walk along it.
+template <int W>
+int device_wide(sc_ctx *ctx, const sc_index *ix, const RankKernelArgs &k, const int32_t *list,
+                int32_t nflag) {
+    int rc = SC_OK;
+    // groups of queries whose keys (two buffers of ncells each) fit 1 GiB
+    const size_t per = (size_t)ix->ncells * 16;
+    const int64_t group = std::max<int64_t>(1, std::min<int64_t>(nflag, ((size_t)1 << 30) / per));
+    size_t tb = 0;
+    (void)rocprim::segmented_radix_sort_keys(nullptr, tb, (uint64_t *)nullptr,
+                                             (uint64_t *)nullptr, (unsigned)(group * ix->ncells),
+                                             (unsigned)group, (int64_t *)nullptr,
+                                             (int64_t *)nullptr + 1, 0u, 64u, ctx->stream);
+    unsigned char *g = nullptr;
+    const size_t kb = per / 2 * (size_t)group;
+    const size_t total = 2 * kb + 8 * (size_t)(group + 1) + tb + 512;
+    if (hipMalloc(&g, total) != hipSuccess) {
+        sc_set_error("sc_search: ranking fallback (%zu bytes) could not be allocated", total);
+        return SC_ERR_NOMEM;
+    }
+    uint64_t *keys = (uint64_t *)g, *skeys = (uint64_t *)(g + kb);
+    int64_t *off = (int64_t *)(g + 2 * kb);
+    unsigned char *tmp = g + 2 * kb + 8 * (size_t)(group + 1);
+    tmp = (unsigned char *)(((uintptr_t)tmp + 255) & ~(uintptr_t)255);
+    for (int64_t g0 = 0; g0 < nflag && rc == SC_OK; g0 += group) {
+        const int64_t gn = std::min<int64_t>(group, nflag - g0);
+        RankKernelArgs fk = k;
+        const unsigned gx = (unsigned)std::min<int64_t>(64, sc_cdiv(ix->ncells, 256));
+        hipLaunchKernelGGL(k_rank_keys<W>, dim3(gx, (unsigned)gn), dim3(256),
+                           8 * (size_t)ix->kc, ctx->stream, fk, list + g0, keys);
+        hipLaunchKernelGGL(k_segment_offsets, dim3((unsigned)sc_cdiv(gn + 1, 256)), dim3(256), 0,
+                           ctx->stream, gn, ix->ncells, off);
+        if (hipGetLastError() != hipSuccess) { rc = SC_ERR_HIP; break; }
+        size_t tb2 = tb;
+        if (rocprim::segmented_radix_sort_keys(tmp, tb2, keys, skeys, (unsigned)(gn * ix->ncells),
+                                               (unsigned)gn, off, off + 1, 0u, 64u,
+                                               ctx->stream) != hipSuccess) {
+            rc = SC_ERR_HIP;
+            break;
+        }
+        hipLaunchKernelGGL(k_rank_walk<W>, dim3((unsigned)gn), dim3(kRankThreads), 0,
+                           ctx->stream, fk, list + g0, skeys);
+        if (hipGetLastError() != hipSuccess) rc = SC_ERR_HIP;
+    }
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(g);
+    if (rc == SC_ERR_HIP) sc_set_error("sc_search: ranking fallback launch failed");
+    return rc;
+}
+
+template <int W>
 int rank_cells(sc_ctx *ctx, const sc_index *ix, const RankCellsArgs &a) {
-    RankKernelArgs k{a, ix->kc, ix->dim, ix->ncells, ix->crec, ix->L, nullptr, rank_cap(),
-                     ix->nu[0], ix->nu[1], ix->upairs[0], ix->upairs[1], ix->cpair, ix->gstart};
+    const RankKernelArgs k = rank_args<W>(ix, a);
     const int cap = rank_cap();
     int32_t *list = a.flag_list, *dn = a.flag_count;
     int32_t nflag = 0;
@@ -1645,33 +1739,42 @@ int rank_cells(sc_ctx *ctx, const sc_index *ix, const RankCellsArgs &a) {
             return SC_ERR_ARG;
         }
         if (rc) return rc;
+        if (retry_ok(ix, a) && a.defer_list && a.defer_count && a.deferred) {
+            // no host synchronisation: the flagged queries listed on the
+            // device, the first retry_cap of them re-ranked by workgroups that
+            // read the count (the others exit at once), those still flagged
+            // listed for the caller
+            SC_HIP(hipMemsetAsync(dn, 0, 4, ctx->stream));
+            hipLaunchKernelGGL(k_rank_flagged, dim3((unsigned)sc_cdiv(a.nq, 256)), dim3(256), 0,
+                               ctx->stream, a.qinfo, a.nq, list, dn);
+            SC_CHECK_LAUNCH();
+            const size_t lds = big_lds(ix->kc, ix->dim);
+            SC_TRY(sc_set_max_lds((const void *)k_rank_big, lds));
+            BigArgs b = retry_args(ix, k, a);
+            b.k.qlist = list;
+            b.qcount = dn;
+            const unsigned grid = (unsigned)std::min<int64_t>(a.nq, a.retry_cap);
+            hipLaunchKernelGGL(k_rank_big, dim3(grid), dim3(kBigThreads), lds, ctx->stream, b);
+            SC_CHECK_LAUNCH();
+            hipLaunchKernelGGL(k_rank_big_emit, dim3(grid), dim3(kBigThreads),
+                               4 * (3 * (size_t)kFine + 1), ctx->stream, b);
+            SC_CHECK_LAUNCH();
+            SC_HIP(hipMemsetAsync(a.defer_count, 0, 4, ctx->stream));
+            hipLaunchKernelGGL(k_rank_flagged, dim3((unsigned)sc_cdiv(a.nq, 256)), dim3(256), 0,
+                               ctx->stream, a.qinfo, a.nq, a.defer_list, a.defer_count);
+            SC_CHECK_LAUNCH();
+            *a.deferred = 1;
+            return SC_OK;
+        }
         SC_TRY(flagged(ctx, a, list, dn, &nflag));
         if (a.overflowed) *a.overflowed += nflag;
     }
     // queries whose cells did not fit the on-chip list: the pruned ranking
-    // over them (scratch for a.retry_cap queries at a time) where the index
-    // has its directory; else once more with a list four times as long
-    // (fewer workgroups per CU, only for them), when the LDS allows
-    if (nflag > 0 && !(a.big_cap > 0 && a.big_list) && a.retry_cap > 0 && ix->brec &&
-        ix->bg_n0 && big_lds(ix->kc, ix->dim) <= 150 * 1024) {
-        const size_t lds = big_lds(ix->kc, ix->dim);
-        SC_HIP(hipFuncSetAttribute((const void *)k_rank_big,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        for (int32_t g0 = 0; g0 < nflag; g0 += a.retry_cap) {
-            const int32_t gn = std::min<int32_t>(a.retry_cap, nflag - g0);
-            BigArgs b{k, ix->bg_n0, ix->bg_pair, ix->bg_start, ix->brec, ix->ckeys,
-                      a.big_list, a.big_sorted, a.big_size, a.big_aux, kBigCap, 0,
-                      1, nullptr, 0, 1};
-            b.k.qlist = list + g0;
-            hipLaunchKernelGGL(k_rank_big, dim3((unsigned)gn), dim3(kBigThreads), lds,
-                               ctx->stream, b);
-            SC_CHECK_LAUNCH();
-            // a query still over the list's capacity is flagged for the
-            // device-wide sort by the emit kernel
-            hipLaunchKernelGGL(k_rank_big_emit, dim3((unsigned)gn), dim3(kBigThreads),
-                               4 * (3 * (size_t)kFine + 1), ctx->stream, b);
-            SC_CHECK_LAUNCH();
-        }
+    // over them where the index has its directory; else once more with a
+    // list four times as long (fewer workgroups per CU, only for them), when
+    // the LDS allows
+    if (nflag > 0 && retry_ok(ix, a)) {
+        SC_TRY(retry_groups(ctx, ix, k, a, list, nflag));
         SC_TRY(flagged(ctx, a, list, dn, &nflag));
     } else if (nflag > 0 && cap == kCap && !(a.big_cap > 0 && a.big_list)) {
         rc = launch_rank<W>(ctx, ix, k, list, nflag, 4 * kCap);
@@ -1679,53 +1782,18 @@ int rank_cells(sc_ctx *ctx, const sc_index *ix, const RankCellsArgs &a) {
         else if (rc != SC_ERR_TOO_LARGE) return rc;
         rc = SC_OK;
     }
-    if (rc == SC_OK && nflag > 0) {
-        // groups of queries whose keys (two buffers of ncells each) fit 1 GiB
-        const size_t per = (size_t)ix->ncells * 16;
-        const int64_t group = std::max<int64_t>(1, std::min<int64_t>(nflag, ((size_t)1 << 30) / per));
-        size_t tb = 0;
-        (void)rocprim::segmented_radix_sort_keys(nullptr, tb, (uint64_t *)nullptr,
-                                                 (uint64_t *)nullptr, (unsigned)(group * ix->ncells),
-                                                 (unsigned)group, (int64_t *)nullptr,
-                                                 (int64_t *)nullptr + 1, 0u, 64u, ctx->stream);
-        unsigned char *g = nullptr;
-        const size_t kb = per / 2 * (size_t)group;
-        const size_t total = 2 * kb + 8 * (size_t)(group + 1) + tb + 512;
-        if (hipMalloc(&g, total) != hipSuccess) {
-            rc = SC_ERR_NOMEM;
-            sc_set_error("sc_search: ranking fallback (%zu bytes) could not be allocated", total);
-        } else {
-            uint64_t *keys = (uint64_t *)g, *skeys = (uint64_t *)(g + kb);
-            int64_t *off = (int64_t *)(g + 2 * kb);
-            unsigned char *tmp = g + 2 * kb + 8 * (size_t)(group + 1);
-            tmp = (unsigned char *)(((uintptr_t)tmp + 255) & ~(uintptr_t)255);
-            for (int64_t g0 = 0; g0 < nflag && rc == SC_OK; g0 += group) {
-                const int64_t gn = std::min<int64_t>(group, nflag - g0);
-                RankKernelArgs fk = k;
-                const unsigned gx = (unsigned)std::min<int64_t>(64, sc_cdiv(ix->ncells, 256));
-                hipLaunchKernelGGL(k_rank_keys<W>, dim3(gx, (unsigned)gn), dim3(256),
-                                   8 * (size_t)ix->kc, ctx->stream, fk, list + g0, keys);
-                hipLaunchKernelGGL(k_segment_offsets, dim3((unsigned)sc_cdiv(gn + 1, 256)),
-                                   dim3(256), 0, ctx->stream, gn, ix->ncells, off);
-                if (hipGetLastError() != hipSuccess) { rc = SC_ERR_HIP; break; }
-                size_t tb2 = tb;
-                if (rocprim::segmented_radix_sort_keys(tmp, tb2, keys, skeys,
-                                                       (unsigned)(gn * ix->ncells), (unsigned)gn,
-                                                       off, off + 1, 0u, 64u,
-                                                       ctx->stream) != hipSuccess) {
-                    rc = SC_ERR_HIP;
-                    break;
-                }
-                hipLaunchKernelGGL(k_rank_walk<W>, dim3((unsigned)gn), dim3(kRankThreads), 0,
-                                   ctx->stream, fk, list + g0, skeys);
-                if (hipGetLastError() != hipSuccess) rc = SC_ERR_HIP;
-            }
-            (void)hipStreamSynchronize(ctx->stream);
-            (void)hipFree(g);
-            if (rc == SC_ERR_HIP) sc_set_error("sc_search: ranking fallback launch failed");
-        }
-    }
+    if (nflag > 0) rc = device_wide<W>(ctx, ix, k, list, nflag);
     return rc;
+}
+
+template <int W>
+int rank_finish(sc_ctx *ctx, const sc_index *ix, const RankCellsArgs &a, const int32_t *qlist,
+                int32_t n) {
+    const RankKernelArgs k = rank_args<W>(ix, a);
+    int32_t nflag = 0;
+    if (n > 0 && retry_ok(ix, a)) SC_TRY(retry_groups(ctx, ix, k, a, qlist, n));
+    SC_TRY(flagged(ctx, a, a.flag_list, a.flag_count, &nflag));
+    return nflag > 0 ? device_wide<W>(ctx, ix, k, a.flag_list, nflag) : SC_OK;
 }
 
 }  // namespace
@@ -1756,4 +1824,11 @@ int sc_rank_cells(sc_ctx *ctx, const sc_index *ix, const RankCellsArgs &a) {
         return SC_ERR_CONFIG;
     }
     return ix->crec_words == 2 ? rank_cells<2>(ctx, ix, a) : rank_cells<4>(ctx, ix, a);
+}
+
+int sc_rank_finish(sc_ctx *ctx, const sc_index *ix, const RankCellsArgs &a, const int32_t *list,
+                   int32_t n) {
+    if (a.nq <= 0) return SC_OK;
+    return ix->crec_words == 2 ? rank_finish<2>(ctx, ix, a, list, n)
+                               : rank_finish<4>(ctx, ix, a, list, n);
 }

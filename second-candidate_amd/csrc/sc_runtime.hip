@@ -3,7 +3,10 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <map>
+#include <mutex>
 #include <new>
+#include <utility>
 
 #include "sc_internal.h"
 
@@ -28,6 +31,21 @@ extern "C" const char *sc_last_error(void) { return g_err; }
 
 extern "C" const char *sc_version(void) {
     return "libsc_amd gfx950 (-ffp-contract=off, no packed fp32)";
+}
+
+int sc_set_max_lds(const void *kern, size_t bytes) {
+    // the attribute is set once per kernel and device, to the largest size
+    // asked for so far (setting it costs a runtime call per launch otherwise)
+    static std::mutex mu;
+    static std::map<std::pair<const void *, int>, size_t> set;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) dev = -1;
+    std::lock_guard<std::mutex> lock(mu);
+    size_t &cur = set[{kern, dev}];
+    if (bytes <= cur) return SC_OK;
+    SC_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    cur = bytes;
+    return SC_OK;
 }
 
 int sc_ctx_enter(sc_ctx *ctx) {

@@ -2641,7 +2641,7 @@ Plan make_plan(const sc_index *ix, const sc_search_params *p, int64_t nq) {
     P.off_clen = take(4ull * ccap * chunk);
     P.off_cinfo = take(4ull * ccap * chunk);
     P.off_qinfo = take(32ull * chunk);
-    P.off_list = take(4ull * chunk + 4);
+    P.off_list = take(8ull * chunk + 16);   // two query lists, then counters
     P.off_stats = take(8ull * 8);
     // scan order: (first cell, query) pairs sorted, then the permutation
     P.off_okey = take(4ull * chunk);
@@ -3126,6 +3126,7 @@ static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
     int32_t *clen = (int32_t *)(s + P.off_clen);
     uint32_t *cinfo = (uint32_t *)(s + P.off_cinfo);
     int32_t *qinfo = (int32_t *)(s + P.off_qinfo), *list = (int32_t *)(s + P.off_list);
+    int32_t *list2 = list + P.chunk, *counters = list + 2 * P.chunk;   // [P.chunk] each, then 4
     unsigned long long *stats = (unsigned long long *)(s + P.off_stats);
     if (st) SC_HIP(hipMemsetAsync(stats, 0, 64, ctx->stream));
 
@@ -3228,9 +3229,10 @@ static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
             SC_TRY(sort_order(true));
             ta.qlist = perm;
         }
+        int deferred = 0;
+        RankCellsArgs ra{Q, dqc, qs, cells, clen, cinfo, qinfo, Tq, cn, p->w, p->T, P.ccap,
+                         list, counters};
         if (halves) {
-            RankCellsArgs ra{Q, dqc, qs, cells, clen, cinfo, qinfo, Tq, cn, p->w, p->T, P.ccap,
-                             list, list + P.chunk};
             if (P.big_cap > 0 || P.retry_cap > 0) {
                 ra.big_list = (uint64_t *)(s + P.off_big_list);
                 ra.big_sorted = (uint64_t *)(s + P.off_big_sorted);
@@ -3242,6 +3244,11 @@ static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
             }
             ra.overflowed = &fallback_total;
             ra.stats = st ? stats : nullptr;
+            if (P.retry_cap > 0 && P.big_cap == 0) {
+                ra.defer_list = list2;
+                ra.defer_count = counters + 1;
+                ra.deferred = &deferred;
+            }
             SC_TRY(sc_rank_cells(ctx, ix, ra));
         } else if (three) {
             SC_TRY(traverse3(ctx, p, kc, sv, si, ix->L, cells, clen, cinfo, qinfo, P.ccap, cn, Tq));
@@ -3276,43 +3283,45 @@ static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
             sa.qlist = perm;
         }
         if (!g)   // a plan sets it before capture
-            SC_HIP(hipFuncSetAttribute((const void *)k_scan_topk,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            SC_TRY(sc_set_max_lds((const void *)k_scan_topk, lds));
         if (hcap_eff > 0 || three || halves) {
             hipLaunchKernelGGL(k_scan_topk, dim3((unsigned)cn), dim3(kScanThreads), lds,
                                ctx->stream, sa);
             SC_CHECK_LAUNCH();
         }
-        if (sa.evec && g) {
-            // graph form of the redo below: a workgroup per query of the
-            // chunk, those past the device-side count exit at once
-            int32_t *dn = list + P.chunk;
+        if (sa.evec) {
+            // mixed IMI scan: queries whose kept set outgrew the buffer are
+            // scanned again exactly (their first outputs are overwritten), a
+            // workgroup per query of the chunk, those past the device-side
+            // count exiting at once (no host synchronisation)
+            int32_t *dn = counters + 2;
             hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, ctx->stream, dn, 1);
             hipLaunchKernelGGL(k_gather_redo, dim3((unsigned)sc_cdiv(cn, 256)), dim3(256), 0,
                                ctx->stream, qinfo, cn, list, dn);
-            ScanArgs ra = sa;
-            ra.evec = nullptr;
-            ra.qlist = list;
-            ra.qcount = dn;
+            ScanArgs rs = sa;
+            rs.evec = nullptr;
+            rs.qlist = list;
+            rs.qcount = dn;
             hipLaunchKernelGGL(k_scan_topk, dim3((unsigned)cn), dim3(kScanThreads), lds,
-                               ctx->stream, ra);
+                               ctx->stream, rs);
             SC_CHECK_LAUNCH();
-        } else if (sa.evec) {
-            // mixed IMI scan: queries whose kept set outgrew the buffer are
-            // scanned again exactly (their first outputs are overwritten)
-            int32_t *dn = list + P.chunk, nredo = 0;
-            SC_HIP(hipMemsetAsync(dn, 0, 4, ctx->stream));
-            hipLaunchKernelGGL(k_gather_redo, dim3((unsigned)sc_cdiv(cn, 256)), dim3(256), 0,
-                               ctx->stream, qinfo, cn, list, dn);
-            SC_CHECK_LAUNCH();
-            SC_HIP(hipMemcpyAsync(&nredo, dn, 4, hipMemcpyDeviceToHost, ctx->stream));
+        }
+        if (deferred) {
+            // the one host synchronisation of the chunk: queries the ranking
+            // left unranked (past the re-ranking's capacity, or over its
+            // list) are ranked now and scanned again exactly (their first,
+            // empty outputs are overwritten)
+            int32_t h[2] = {0, 0};
+            SC_HIP(hipMemcpyAsync(h, counters, 8, hipMemcpyDeviceToHost, ctx->stream));
             SC_HIP(hipStreamSynchronize(ctx->stream));
-            if (nredo > 0) {
-                ScanArgs ra = sa;
-                ra.evec = nullptr;
-                ra.qlist = list;
-                hipLaunchKernelGGL(k_scan_topk, dim3((unsigned)nredo), dim3(kScanThreads), lds,
-                                   ctx->stream, ra);
+            fallback_total += h[0];
+            if (h[1] > 0) {
+                SC_TRY(sc_rank_finish(ctx, ix, ra, list2, h[1]));
+                ScanArgs fs = sa;
+                fs.evec = nullptr;
+                fs.qlist = list2;
+                hipLaunchKernelGGL(k_scan_topk, dim3((unsigned)h[1]), dim3(kScanThreads), lds,
+                                   ctx->stream, fs);
                 SC_CHECK_LAUNCH();
             }
         }
@@ -3326,7 +3335,7 @@ static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
             SC_CHECK_LAUNCH();
         } else if (!three && !halves && hcap_eff < hcap_ref) {
             int32_t nfb = 0;
-            int32_t *dn = list + P.chunk;
+            int32_t *dn = counters;
             if (hcap_eff > 0) {
                 SC_HIP(hipMemsetAsync(dn, 0, 4, ctx->stream));
                 hipLaunchKernelGGL(k_gather_fallback, dim3((unsigned)sc_cdiv(cn, 256)), dim3(256),
@@ -3442,9 +3451,7 @@ static size_t staged_scan_lds(const sc_index *ix, const sc_search_params *p) {
 static int plan_capture_staged(sc_plan *P) {
     if (P->exec) { (void)hipGraphExecDestroy(P->exec); P->exec = nullptr; }
     if (P->graph) { (void)hipGraphDestroy(P->graph); P->graph = nullptr; }
-    SC_HIP(hipFuncSetAttribute((const void *)k_scan_topk,
-                               hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)staged_scan_lds(P->ix, &P->p)));
+    SC_TRY(sc_set_max_lds((const void *)k_scan_topk, staged_scan_lds(P->ix, &P->p)));
     hipStream_t sm = P->ctx->stream;
     SC_HIP(hipStreamBeginCapture(sm, hipStreamCaptureModeThreadLocal));
     hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, sm, P->counter, 2);

@@ -324,6 +324,21 @@ def test_search_scimi_small_rank_list(ctx, cap, pairs, big, monkeypatch):
     ix.close()
 
 
+@pytest.mark.parametrize("form", ["plain", "mixed"])
+def test_search_scimi_rerank_past_capacity(ctx, form, monkeypatch):
+    """An on-chip ranking list of one cell and 1200 queries in one batch:
+    more queries outgrow it than the pruned re-ranking takes in its
+    device-driven pass (512 workgroups that read the count), so the rest are
+    ranked after the batch's one host synchronisation and scanned again
+    exactly; ids, distances and counts must equal the oracle's."""
+    monkeypatch.setenv("SC_RANK_CAP", "1")
+    ix, oix = _index_pair(ctx, S.KIND_SCIMI, kc=8)
+    f = S.FORM_MIXED if form == "mixed" else S.FORM_PLAIN
+    st = _compare_search(ctx, ix, oix, S.KIND_SCIMI, 100, 100000, 3000, nq=1200, form=f)
+    assert st.heap_fallback == 1200
+    ix.close()
+
+
 @pytest.mark.parametrize("parts", ["1", "3", "auto"])
 @pytest.mark.parametrize("case", SCIMI_CASES)
 def test_search_scimi_pruned_ranking_matches_oracle(ctx, case, parts, monkeypatch):
