@@ -152,6 +152,15 @@ __global__ void k_group_starts(const uint32_t *__restrict__ cpair, int64_t ncell
     if (c == 0 || (cpair[2 * (c - 1)] & 0xFFFFu) != g) gstart[g] = (int32_t)c;
 }
 
+// Vectors in the cells of each half-0 pair (from the inclusive prefix L).
+__global__ void k_group_vecs(const int32_t *__restrict__ gstart, const int32_t *__restrict__ L,
+                             int n0, int32_t *__restrict__ gvec) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n0) return;
+    const int32_t c0 = gstart[g], c1 = gstart[g + 1];
+    gvec[g] = L[c1 - 1] - (c0 > 0 ? L[c0 - 1] : 0);
+}
+
 // Groups of the pruned ranking: the runs of cells sharing a half-0 pair
 // (uid_0 = key / kc^2; cells are in key order).  heads[c] = 1 where a run
 // starts.
@@ -425,6 +434,13 @@ static int build_pair_lists(sc_index *ix) {
     }
     hipLaunchKernelGGL(k_group_starts, dim3((unsigned)sc_cdiv(nc + 1, 256)), dim3(256), 0,
                        ctx->stream, ix->cpair, nc, ix->nu[0], ix->gstart);
+    SC_CHECK_LAUNCH();
+    if (hipMalloc(&ix->gvec, 4 * ((size_t)ix->nu[0] + 1)) != hipSuccess) {
+        sc_set_error("index: group size allocation failed");
+        return SC_ERR_NOMEM;
+    }
+    hipLaunchKernelGGL(k_group_vecs, dim3((unsigned)sc_cdiv(ix->nu[0], 256)), dim3(256), 0,
+                       ctx->stream, ix->gstart, ix->L, ix->nu[0], ix->gvec);
     SC_CHECK_LAUNCH();
     return SC_OK;
 }
@@ -860,8 +876,8 @@ extern "C" int sc_index_destroy(sc_index *ix) {
     void *bufs[] = {ix->cq,    ix->pq,   ix->norm_c, ix->norm_r, ix->dot_cr,
                     ix->L,     ix->pid,  ix->codes,  ix->cqT,    ix->pqT,
                     ix->ckeys, ix->crec, ix->upairs[0], ix->upairs[1], ix->cpair,
-                    ix->gstart, ix->evec, ix->vcell, ix->bg_n0, ix->bg_pair, ix->bg_start,
-                    ix->brec};
+                    ix->gstart, ix->gvec, ix->evec, ix->vcell, ix->bg_n0, ix->bg_pair,
+                    ix->bg_start, ix->brec};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     delete ix;

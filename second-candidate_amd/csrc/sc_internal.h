@@ -46,6 +46,7 @@ struct sc_index {
     uint32_t *upairs[2];            // [nu[h]] n | s << 16
     uint32_t *cpair;                // [ncells][2]
     int32_t *gstart;                // [nu[0] + 1] first cell of each half-0 pair
+    int32_t *gvec;                  // [nu[0]] vectors in each half-0 pair's cells
     // Large two-halves indexes (kc <= 4096): the half-0 pairs present
     // ("groups") in key order, each the contiguous run of its cells, and per
     // cell a compact record of its half-1 pair, for the pruned cell ranking.

@@ -55,6 +55,7 @@ struct RankKernelArgs {
     int nu0, nu1;
     const uint32_t *up0, *up1, *cpair;
     const int32_t *gstart;
+    const int32_t *gvec;    // vectors in each half-0 pair's cells
 };
 
 template <int W>
@@ -301,43 +302,6 @@ __device__ bool hist_select(const Sc &sc, int64_t ncells, int w, int T, int cap,
     return hist_select_each(sc, all, w, T, cap, hcnt, hwt, lkey, sh);
 }
 
-// An upper bound of the stopping score key of the cells `each` hands out:
-// the key bits of the histogram buckets (top 12 bits, then the next 12) in
-// which the running count reaches w or the running size reaches T, the
-// remaining 8 bits set.  Returns false when those cells never reach the
-// budget.
-template <class Sc, class Each>
-__device__ bool hist_stop(const Sc &sc, Each each, int w, int T, uint32_t *hcnt, uint32_t *hwt,
-                          RankShared &sh, uint32_t *thr) {
-    const int tid = threadIdx.x;
-    for (int i = tid; i < kPB; i += kRankThreads) { hcnt[i] = 0; hwt[i] = 0; }
-    __syncthreads();
-    each([&](int64_t c) {
-        const uint32_t key = sc_float_key(sc.score(c));
-        atomicAdd(&hcnt[key >> kL1], 1u);
-        atomicAdd(&hwt[key >> kL1], sc.size(c));
-    });
-    __syncthreads();
-    const int b1 = find_bucket(hcnt, hwt, 0u, 0u, w, T, sh.wsum, &sh.b, sh.pre);
-    const uint32_t pre1_cnt = sh.pre[0], pre1_w = sh.pre[1];
-    __syncthreads();
-    if (b1 >= kPB) return false;
-    for (int i = tid; i < kPB; i += kRankThreads) { hcnt[i] = 0; hwt[i] = 0; }
-    __syncthreads();
-    each([&](int64_t c) {
-        const uint32_t key = sc_float_key(sc.score(c));
-        if ((int)(key >> kL1) != b1) return;
-        atomicAdd(&hcnt[(key >> kL2) & kL2Mask], 1u);
-        atomicAdd(&hwt[(key >> kL2) & kL2Mask], sc.size(c));
-    });
-    __syncthreads();
-    const int b2 = find_bucket(hcnt, hwt, pre1_cnt, pre1_w, w, T, sh.wsum, &sh.b, sh.pre);
-    __syncthreads();
-    // the level-1 bucket reaches the budget, so some level-2 bucket in it does
-    *thr = ((uint32_t)b1 << kL1) | ((uint32_t)min(b2, kPB - 1) << kL2) | kLowBits;
-    return true;
-}
-
 // The score key of the stopping entry of the list lkey[0 .. n) (unsorted):
 // the key at which, in (score, cell) order, the count reaches w or the size
 // sum reaches T.  A radix select weighted by count and size, 8 key bits a
@@ -348,11 +312,22 @@ __device__ bool list_stop(const Sc &sc, const uint64_t *lkey, int n, int w, int 
                           uint32_t *stop) {
     __shared__ uint32_t hc[256], hw[256], s_and, s_or;
     const int tid = threadIdx.x;
+    // this thread's entries (tid + j * kRankThreads, n <= kCap): their sizes
+    // are read from global memory once, for every pass below
+    constexpr int kPer = kCap / kRankThreads;
+    uint32_t szr[kPer];
     uint32_t a = ~0u, o = 0u;
-    for (int i = tid; i < n; i += kRankThreads) {
-        const uint32_t key = (uint32_t)(lkey[i] >> 32);
-        a &= key;
-        o |= key;
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+        const int i = tid + j * kRankThreads;
+        szr[j] = 0;
+        if (i < n) {
+            const uint64_t e = lkey[i];
+            const uint32_t key = (uint32_t)(e >> 32);
+            a &= key;
+            o |= key;
+            szr[j] = sc.size((int64_t)(e & 0xFFFFFFFFu));
+        }
     }
     for (int m = 32; m > 0; m >>= 1) {
         a &= (uint32_t)__shfl_xor(a, m);
@@ -366,9 +341,12 @@ __device__ bool list_stop(const Sc &sc, const uint64_t *lkey, int n, int w, int 
     uint32_t prefix = s_and & ~diff;      // the bits every key shares
     if (diff == 0) {                      // one key value: stop there if reached
         uint32_t cnt = 0, sz = 0;
-        for (int i = tid; i < n; i += kRankThreads) {
-            cnt += 1;
-            sz += sc.size((int64_t)(lkey[i] & 0xFFFFFFFFu));
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            if (tid + j * kRankThreads < n) {
+                cnt += 1;
+                sz += szr[j];
+            }
         }
         uint32_t tc, ts;
         (void)block_excl_scan(cnt, sh.wsum, &tc);
@@ -383,12 +361,15 @@ __device__ bool list_stop(const Sc &sc, const uint64_t *lkey, int n, int w, int 
         hw[tid] = 0;
         __syncthreads();
         const uint32_t hi_mask = shift + 8 >= 32 ? 0u : ~0u << (shift + 8);
-        for (int i = tid; i < n; i += kRankThreads) {
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            const int i = tid + j * kRankThreads;
+            if (i >= n) continue;
             const uint32_t key = (uint32_t)(lkey[i] >> 32);
             if ((key & hi_mask) != (prefix & hi_mask)) continue;
             const uint32_t b = (key >> shift) & 255u;
             atomicAdd(&hc[b], 1u);
-            atomicAdd(&hw[b], sc.size((int64_t)(lkey[i] & 0xFFFFFFFFu)));
+            atomicAdd(&hw[b], szr[j]);
         }
         __syncthreads();
         if (tid < 64) {   // wave 0: the first bin where the running totals reach the budget
@@ -467,6 +448,59 @@ __device__ void emit_sorted(const Sc &sc, const RankKernelArgs &k, int64_t q, ui
     // the taken entries are a prefix
     const int per = (n + kRankThreads - 1) / kRankThreads;
     const int i0 = min(n, tid * per), i1 = min(n, i0 + per);
+    constexpr int kPer = kCap / kRankThreads;
+    if (per <= kPer) {
+        // each entry's size, first position and nearest centroids loaded
+        // together, once, and kept in registers
+        uint32_t szr[kPer], nr[kPer];
+        int32_t str[kPer];
+        uint32_t local = 0;
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            const int i = i0 + j;
+            szr[j] = 0; nr[j] = 0; str[j] = 0;
+            if (i < i1) {
+                const int64_t c = (int64_t)(lkey[i] & 0xFFFFFFFFu);
+                szr[j] = sc.size(c);
+                str[j] = c > 0 ? k.L[c - 1] : 0;
+                nr[j] = sc.nearest(c);
+                local += szr[j];
+            }
+        }
+        uint32_t tot;
+        uint32_t run = block_excl_scan(local, sh.wsum, &tot);
+        if (tid == 0) { sh.count = 0; sh.sum = 0; }
+        __syncthreads();
+        int cnt = 0;
+        uint32_t sum = 0;
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            const int i = i0 + j;
+            if (i < i1 && i < w && (int64_t)run < T) {
+                if (i < a.ccap) {
+                    cells[i] = str[j];
+                    clen[i] = (int32_t)szr[j];
+                    cinfo[i] = nr[j];
+                }
+                cnt = i + 1;
+                sum = run + szr[j];
+            }
+            run += szr[j];
+        }
+        if (cnt > 0) {
+            atomicMax(&sh.count, cnt);
+            atomicMax(&sh.sum, sum);
+        }
+        __syncthreads();
+        if (tid == 0) {
+            const int count = sh.count;
+            int32_t *qi = a.qinfo + (size_t)q * 8;
+            qi[0] = count; qi[1] = (int32_t)sh.sum; qi[2] = count; qi[3] = 0;
+            qi[4] = (count == k.ncells && count < w && (int64_t)sh.sum < T) ? 1 : 0;
+            qi[5] = 0; qi[6] = count > a.ccap ? 2 : 0; qi[7] = 0;
+        }
+        return;
+    }
     uint32_t local = 0;
     for (int i = i0; i < i1; ++i) local += sc.size((int64_t)(lkey[i] & 0xFFFFFFFFu));
     uint32_t tot;
@@ -506,6 +540,15 @@ __device__ void query_terms(const RankKernelArgs &k, int64_t q, float *v, RankSh
     // the query row goes to LDS first (all threads), so that the sequential
     // sums below wait on LDS, not on one global load after another
     float *qrow = v;                               // [dim] <= [2 kc] (checked by the host)
+    const float *dq = a.dqc + (size_t)q * 2 * kc;
+    // the first entries of this thread's diff_qc, loaded with the query row
+    constexpr int kPre = 4;
+    float dpre[kPre];
+#pragma unroll
+    for (int i = 0; i < kPre; ++i) {
+        const int j = tid + i * NT;
+        dpre[i] = j < 2 * kc ? dq[j] : 0.0f;
+    }
     for (int j = tid; j < k.dim; j += NT) qrow[j] = a.Q[(size_t)q * k.dim + j];
     __syncthreads();
     if (tid == 0) {
@@ -522,8 +565,12 @@ __device__ void query_terms(const RankKernelArgs &k, int64_t q, float *v, RankSh
         qs[0] = qsum; qs[1] = h[0]; qs[2] = h[1]; qs[3] = 0.0f;
     }
     __syncthreads();
-    const float *dq = a.dqc + (size_t)q * 2 * kc;
-    for (int j = tid; j < 2 * kc; j += NT) v[j] = sh.q[j < kc ? 1 : 2] + dq[j];
+#pragma unroll
+    for (int i = 0; i < kPre; ++i) {
+        const int j = tid + i * NT;
+        if (j < 2 * kc) v[j] = sh.q[j < kc ? 1 : 2] + dpre[i];
+    }
+    for (int j = tid + kPre * NT; j < 2 * kc; j += NT) v[j] = sh.q[j < kc ? 1 : 2] + dq[j];
     __syncthreads();
 }
 
@@ -690,6 +737,77 @@ __device__ void emit_small(const Sc &sc, const RankKernelArgs &k, int64_t q, uin
     }
 }
 
+// An upper bound of the stopping score key of the cells of the groups
+// gbeg/gpre describe (ng of them): the key bits of the histogram buckets
+// (top 11 bits, then the next 11) in which the running count reaches w or
+// the running size reaches T, the remaining bits set.  Returns false when
+// those cells never reach the budget.  A thread keeps the keys and sizes of
+// its first kR cells in registers for the second level.  All threads call.
+template <class Sc>
+__device__ bool subset_stop(const Sc &sc, const int32_t *gbeg, const int32_t *gpre, int ng, int w,
+                            int T, uint32_t *hcnt, uint32_t *hwt, RankShared &sh, uint32_t *thr) {
+    constexpr int kR = 8;
+    const int tid = threadIdx.x;
+    const int total = gpre[ng];
+    auto cell_of = [&](int t) -> int64_t {   // the last group j with gpre[j] <= t
+        int lo = 0, hi = ng;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (gpre[mid] <= t) lo = mid; else hi = mid;
+        }
+        return (int64_t)gbeg[lo] + (t - gpre[lo]);
+    };
+    uint32_t kr[kR], sr[kR];
+    for (int i = tid; i < kPB; i += kRankThreads) { hcnt[i] = 0; hwt[i] = 0; }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kR; ++j) {
+        const int t = tid + j * kRankThreads;
+        kr[j] = ~0u;
+        sr[j] = 0;
+        if (t < total) {
+            const int64_t c = cell_of(t);
+            kr[j] = sc_float_key(sc.score(c));
+            sr[j] = sc.size(c);
+            atomicAdd(&hcnt[kr[j] >> kL1], 1u);
+            atomicAdd(&hwt[kr[j] >> kL1], sr[j]);
+        }
+    }
+    for (int t = tid + kR * kRankThreads; t < total; t += kRankThreads) {
+        const int64_t c = cell_of(t);
+        const uint32_t key = sc_float_key(sc.score(c));
+        atomicAdd(&hcnt[key >> kL1], 1u);
+        atomicAdd(&hwt[key >> kL1], sc.size(c));
+    }
+    __syncthreads();
+    const int b1 = find_bucket(hcnt, hwt, 0u, 0u, w, T, sh.wsum, &sh.b, sh.pre);
+    const uint32_t pre1_cnt = sh.pre[0], pre1_w = sh.pre[1];
+    __syncthreads();
+    if (b1 >= kPB) return false;
+    for (int i = tid; i < kPB; i += kRankThreads) { hcnt[i] = 0; hwt[i] = 0; }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kR; ++j) {
+        if (kr[j] != ~0u && (int)(kr[j] >> kL1) == b1) {
+            atomicAdd(&hcnt[(kr[j] >> kL2) & kL2Mask], 1u);
+            atomicAdd(&hwt[(kr[j] >> kL2) & kL2Mask], sr[j]);
+        }
+    }
+    for (int t = tid + kR * kRankThreads; t < total; t += kRankThreads) {
+        const int64_t c = cell_of(t);
+        const uint32_t key = sc_float_key(sc.score(c));
+        if ((int)(key >> kL1) != b1) continue;
+        atomicAdd(&hcnt[(key >> kL2) & kL2Mask], 1u);
+        atomicAdd(&hwt[(key >> kL2) & kL2Mask], sc.size(c));
+    }
+    __syncthreads();
+    const int b2 = find_bucket(hcnt, hwt, pre1_cnt, pre1_w, w, T, sh.wsum, &sh.b, sh.pre);
+    __syncthreads();
+    // the level-1 bucket reaches the budget, so some level-2 bucket in it does
+    *thr = ((uint32_t)b1 << kL1) | ((uint32_t)min(b2, kPB - 1) << kL2) | kLowBits;
+    return true;
+}
+
 __global__ __launch_bounds__(kRankThreads) void k_rank_pairs(RankKernelArgs k) {
     extern __shared__ __align__(16) unsigned char smem[];
     uint32_t *hcnt = reinterpret_cast<uint32_t *>(smem);           // [kPB]
@@ -737,7 +855,7 @@ __global__ __launch_bounds__(kRankThreads) void k_rank_pairs(RankKernelArgs k) {
         const int c0 = k.gstart[g], c1 = k.gstart[g + 1];
         const uint32_t b = sc_float_key(st0[g]) >> kL1;
         atomicAdd(&hcnt[b], (uint32_t)(c1 - c0));
-        atomicAdd(&hwt[b], (uint32_t)(k.L[c1 - 1] - (c0 > 0 ? k.L[c0 - 1] : 0)));
+        atomicAdd(&hwt[b], (uint32_t)k.gvec[g]);
     }
     __syncthreads();
     const int b0 = find_bucket(hcnt, hwt, 0u, 0u, w, T, sh.wsum, &sh.b, sh.pre);
@@ -752,7 +870,7 @@ __global__ __launch_bounds__(kRankThreads) void k_rank_pairs(RankKernelArgs k) {
         if (gb > b0) continue;
         if (gb == b0) {
             const int c0 = k.gstart[g], c1 = k.gstart[g + 1];
-            const uint32_t gw = (uint32_t)(k.L[c1 - 1] - (c0 > 0 ? k.L[c0 - 1] : 0));
+            const uint32_t gw = (uint32_t)k.gvec[g];
             // one 64-bit counter (cells high, sizes low: neither sum reaches
             // 2^32), so that every pair sees one consistent running total
             const unsigned long long old =
@@ -773,10 +891,7 @@ __global__ __launch_bounds__(kRankThreads) void k_rank_pairs(RankKernelArgs k) {
     if (s_np <= kMaxPass) {
         const int np0 = s_np;
         group_prefix(gsz, gpre, np0, sh);
-        auto sub = [&](auto f) {
-            for_group_cells(gbeg, gpre, np0, [&](int, int64_t c) { f(c); });
-        };
-        have = hist_stop(sc, sub, w, T, hcnt, hwt, sh, &thr);
+        have = subset_stop(sc, gbeg, gpre, np0, w, T, hcnt, hwt, sh, &thr);
     }
     __syncthreads();
     if (have) {
@@ -1512,7 +1627,8 @@ int rank_cap() {
 bool use_pair_tables(const sc_index *ix) {
     if (const char *e = getenv("SC_RANK_PAIRS"))
         if (atoi(e) == 0) return false;
-    return ix->cpair && ix->gstart && ix->nu[0] <= kMaxPairs0 && ix->nu[0] + ix->nu[1] <= 8192;
+    return ix->cpair && ix->gstart && ix->gvec && ix->nu[0] <= kMaxPairs0 &&
+           ix->nu[0] + ix->nu[1] <= 8192;
 }
 
 // Launches the ranking of a.nq queries (or of the queries in qlist) with an
@@ -1634,7 +1750,7 @@ template <int W>
 RankKernelArgs rank_args(const sc_index *ix, const RankCellsArgs &a) {
     return RankKernelArgs{a, ix->kc, ix->dim, ix->ncells, ix->crec, ix->L, nullptr, rank_cap(),
                           ix->nu[0], ix->nu[1], ix->upairs[0], ix->upairs[1], ix->cpair,
-                          ix->gstart};
+                          ix->gstart, ix->gvec};
 }
 
 // The pruned re-ranking is available for queries whose cells outgrow the
