@@ -14,7 +14,14 @@ struct sc_ctx {
     int n_cu;               // compute units
     void *fixed_scratch;    // this device's copy of the library's static scratch
     size_t fixed_scratch_bytes;
+    // a second stream (default priority) for work that overlaps the context
+    // stream's, and two events ordering them; created on first use
+    hipStream_t side;
+    hipEvent_t ev_fork, ev_join;
 };
+
+// The context's second stream and its events (created on first use).
+int sc_ctx_side(sc_ctx *ctx);
 
 struct sc_index {
     sc_ctx *ctx;
@@ -175,6 +182,13 @@ struct RankCellsArgs {
     // is left to the caller.  sc_rank_finish completes them.
     int32_t *defer_list = nullptr, *defer_count = nullptr;
     int *deferred = nullptr;
+    // With deferral: skip [nq] (device) gets 1 for the queries that outgrew
+    // the first list and 0 for the others, and their re-ranking runs on the
+    // context's second stream (after ev_fork on the context stream, ending
+    // with ev_join), so that the others can be scanned meanwhile; the caller
+    // waits for ev_join before it reads the re-ranked queries' lists or the
+    // deferred count.
+    int32_t *skip = nullptr;
 };
 int sc_rank_cells(sc_ctx *ctx, const sc_index *ix, const RankCellsArgs &a);
 // Ranks the n queries of list (device) that a deferred sc_rank_cells left

@@ -1603,9 +1603,13 @@ __global__ __launch_bounds__(kRankThreads) void k_rank_walk(RankKernelArgs k, co
 }
 
 __global__ void k_rank_flagged(const int32_t *__restrict__ qinfo, int64_t nq,
-                               int32_t *__restrict__ list, int32_t *__restrict__ n) {
+                               int32_t *__restrict__ list, int32_t *__restrict__ n,
+                               int32_t *__restrict__ skip) {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q < nq && qinfo[q * 8 + 5]) list[atomicAdd(n, 1)] = (int32_t)q;
+    if (q >= nq) return;
+    const int32_t f = qinfo[q * 8 + 5];
+    if (skip) skip[q] = f ? 1 : 0;
+    if (f) list[atomicAdd(n, 1)] = (int32_t)q;
 }
 
 __global__ void k_segment_offsets(int64_t nseg, int64_t len, int64_t *off) {
@@ -1659,7 +1663,7 @@ int launch_rank(sc_ctx *ctx, const sc_index *ix, RankKernelArgs k, const int32_t
 int flagged(sc_ctx *ctx, const RankCellsArgs &a, int32_t *list, int32_t *dn, int32_t *nflag) {
     SC_HIP(hipMemsetAsync(dn, 0, 4, ctx->stream));
     hipLaunchKernelGGL(k_rank_flagged, dim3((unsigned)sc_cdiv(a.nq, 256)), dim3(256), 0,
-                       ctx->stream, a.qinfo, a.nq, list, dn);
+                       ctx->stream, a.qinfo, a.nq, list, dn, nullptr);
     SC_CHECK_LAUNCH();
     SC_HIP(hipMemcpyAsync(nflag, dn, 4, hipMemcpyDeviceToHost, ctx->stream));
     SC_HIP(hipStreamSynchronize(ctx->stream));
@@ -1862,23 +1866,32 @@ int rank_cells(sc_ctx *ctx, const sc_index *ix, const RankCellsArgs &a) {
             // listed for the caller
             SC_HIP(hipMemsetAsync(dn, 0, 4, ctx->stream));
             hipLaunchKernelGGL(k_rank_flagged, dim3((unsigned)sc_cdiv(a.nq, 256)), dim3(256), 0,
-                               ctx->stream, a.qinfo, a.nq, list, dn);
+                               ctx->stream, a.qinfo, a.nq, list, dn, a.skip);
             SC_CHECK_LAUNCH();
+            // with skip flags, the re-ranking runs on the second stream
+            hipStream_t rs = ctx->stream;
+            if (a.skip) {
+                SC_TRY(sc_ctx_side(ctx));
+                SC_HIP(hipEventRecord(ctx->ev_fork, ctx->stream));
+                SC_HIP(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
+                rs = ctx->side;
+            }
             const size_t lds = big_lds(ix->kc, ix->dim);
             SC_TRY(sc_set_max_lds((const void *)k_rank_big, lds));
             BigArgs b = retry_args(ix, k, a);
             b.k.qlist = list;
             b.qcount = dn;
             const unsigned grid = (unsigned)std::min<int64_t>(a.nq, a.retry_cap);
-            hipLaunchKernelGGL(k_rank_big, dim3(grid), dim3(kBigThreads), lds, ctx->stream, b);
+            hipLaunchKernelGGL(k_rank_big, dim3(grid), dim3(kBigThreads), lds, rs, b);
             SC_CHECK_LAUNCH();
             hipLaunchKernelGGL(k_rank_big_emit, dim3(grid), dim3(kBigThreads),
-                               4 * (3 * (size_t)kFine + 1), ctx->stream, b);
+                               4 * (3 * (size_t)kFine + 1), rs, b);
             SC_CHECK_LAUNCH();
-            SC_HIP(hipMemsetAsync(a.defer_count, 0, 4, ctx->stream));
+            SC_HIP(hipMemsetAsync(a.defer_count, 0, 4, rs));
             hipLaunchKernelGGL(k_rank_flagged, dim3((unsigned)sc_cdiv(a.nq, 256)), dim3(256), 0,
-                               ctx->stream, a.qinfo, a.nq, a.defer_list, a.defer_count);
+                               rs, a.qinfo, a.nq, a.defer_list, a.defer_count, nullptr);
             SC_CHECK_LAUNCH();
+            if (a.skip) SC_HIP(hipEventRecord(ctx->ev_join, ctx->side));
             *a.deferred = 1;
             return SC_OK;
         }

@@ -95,10 +95,24 @@ extern "C" int sc_ctx_create(int device, sc_ctx **out) {
     return SC_OK;
 }
 
+int sc_ctx_side(sc_ctx *ctx) {
+    if (ctx->side) return SC_OK;
+    SC_HIP(hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
+    SC_HIP(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
+    SC_HIP(hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
+    return SC_OK;
+}
+
 extern "C" int sc_ctx_destroy(sc_ctx *ctx) {
     if (!ctx) return SC_OK;
     SC_TRY(sc_ctx_enter(ctx));
     (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->side) {
+        (void)hipStreamSynchronize(ctx->side);
+        (void)hipStreamDestroy(ctx->side);
+    }
+    if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+    if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
     if (ctx->scratch) (void)hipFree(ctx->scratch);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;

@@ -681,6 +681,7 @@ struct ScanArgs {
     int64_t ncells;
     const int32_t *qcount = nullptr;  // with qlist: workgroups at or past *qcount exit
     const uint32_t *vcell = nullptr;  // [n] n0 | n1 << 16 per position (mc = 2) or null
+    const int32_t *skip = nullptr;    // [nq] or null: queries flagged 1 are left to a later scan
 };
 
 struct ScanShared {
@@ -1509,6 +1510,7 @@ void k_scan_topk(ScanArgs a) {
     float *lut = reinterpret_cast<float *>(smem + sizeof(ScanShared));   // [mp*kp]
     if (a.qcount && (int)blockIdx.x >= *a.qcount) return;   // the whole workgroup
     const int64_t q = a.qlist ? a.qlist[blockIdx.x] : blockIdx.x;
+    if (a.skip && a.skip[q]) return;                         // the whole workgroup
     scan_query(a, S, lut, q, a.dqr + (size_t)q * a.mp * a.kp, a.dqc + (size_t)q * a.mc * a.kc,
                a.qs[q * 4], a.cells + (size_t)q * a.ccap, a.clen + (size_t)q * a.ccap,
                a.cinfo + (size_t)q * a.ccap,
@@ -2606,7 +2608,7 @@ struct Plan {
     int64_t chunk;      // queries per pass
     int ccap;
     size_t off_qs, off_dqc, off_dqr, off_sv, off_si, off_cells, off_clen, off_cinfo, off_qinfo,
-        off_list, off_stats, off_okey, off_okey2, off_oid, off_oid2, off_perm, off_otmp,
+        off_list, off_skip, off_stats, off_okey, off_okey2, off_oid, off_oid2, off_perm, off_otmp,
         otmp_bytes, total, off_big_list, off_big_sorted, off_big_size, off_big_aux, off_big_split;
     int big_cap;        // entries per query of the pruned ranking (0: not used)
     int retry_cap;      // queries the pruned ranking can re-rank per pass (big_cap == 0)
@@ -2642,6 +2644,7 @@ Plan make_plan(const sc_index *ix, const sc_search_params *p, int64_t nq) {
     P.off_cinfo = take(4ull * ccap * chunk);
     P.off_qinfo = take(32ull * chunk);
     P.off_list = take(8ull * chunk + 16);   // two query lists, then counters
+    P.off_skip = take(4ull * chunk);
     P.off_stats = take(8ull * 8);
     // scan order: (first cell, query) pairs sorted, then the permutation
     P.off_okey = take(4ull * chunk);
@@ -3248,6 +3251,8 @@ static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
                 ra.defer_list = list2;
                 ra.defer_count = counters + 1;
                 ra.deferred = &deferred;
+                // the re-ranking overlaps the scan of the other queries
+                if (!g) ra.skip = (int32_t *)(s + P.off_skip);
             }
             SC_TRY(sc_rank_cells(ctx, ix, ra));
         } else if (three) {
@@ -3285,8 +3290,21 @@ static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
         if (!g)   // a plan sets it before capture
             SC_TRY(sc_set_max_lds((const void *)k_scan_topk, lds));
         if (hcap_eff > 0 || three || halves) {
+            ScanArgs first = sa;
+            first.skip = deferred ? ra.skip : nullptr;
             hipLaunchKernelGGL(k_scan_topk, dim3((unsigned)cn), dim3(kScanThreads), lds,
-                               ctx->stream, sa);
+                               ctx->stream, first);
+            SC_CHECK_LAUNCH();
+        }
+        if (deferred && ra.skip) {
+            // the re-ranked queries, once their ranking (second stream) is done:
+            // a workgroup per query of the chunk, those past the count exit
+            SC_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
+            ScanArgs later = sa;
+            later.qlist = list;
+            later.qcount = counters;
+            hipLaunchKernelGGL(k_scan_topk, dim3((unsigned)cn), dim3(kScanThreads), lds,
+                               ctx->stream, later);
             SC_CHECK_LAUNCH();
         }
         if (sa.evec) {
