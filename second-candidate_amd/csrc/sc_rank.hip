@@ -50,7 +50,7 @@ struct RankKernelArgs {
     const uint32_t *rec;
     const int32_t *L;
     const int32_t *qlist;   // fallback: the queries of this launch (null = all)
-    int cap;                // on-chip list capacity (at most kCap)
+    int cap;                // on-chip list capacity (at most 2 kCap; 4 kCap for a retry)
     // pair tables (k_rank_pairs)
     int nu0, nu1;
     const uint32_t *up0, *up1, *cpair;
@@ -312,9 +312,9 @@ __device__ bool list_stop(const Sc &sc, const uint64_t *lkey, int n, int w, int 
                           uint32_t *stop) {
     __shared__ uint32_t hc[256], hw[256], s_and, s_or;
     const int tid = threadIdx.x;
-    // this thread's entries (tid + j * kRankThreads, n <= kCap): their sizes
-    // are read from global memory once, for every pass below
-    constexpr int kPer = kCap / kRankThreads;
+    // this thread's entries (tid + j * kRankThreads, n <= 2 kCap): their
+    // sizes are read from global memory once, for every pass below
+    constexpr int kPer = 2 * kCap / kRankThreads;
     uint32_t szr[kPer];
     uint32_t a = ~0u, o = 0u;
 #pragma unroll
@@ -419,10 +419,10 @@ __device__ bool list_stop(const Sc &sc, const uint64_t *lkey, int n, int w, int 
 }
 
 // Moves the entries of score key at most `stop` to the front of lkey
-// (n <= kCap); returns their number.  All threads call.
+// (n <= 2 kCap); returns their number.  All threads call.
 __device__ int list_keep(uint64_t *lkey, int n, uint32_t stop, RankShared &sh) {
     const int tid = threadIdx.x;
-    constexpr int kPer = kCap / kRankThreads;
+    constexpr int kPer = 2 * kCap / kRankThreads;
     uint64_t mine[kPer];
     int m = 0;
     for (int i = tid; i < n; i += kRankThreads)
@@ -737,6 +737,43 @@ __device__ void emit_small(const Sc &sc, const RankKernelArgs &k, int64_t q, uin
     }
 }
 
+// An upper bound of the stopping score key of the cells `each` hands out
+// (each(f) calls f(c) once per cell, spread over the threads): the key bits
+// of the histogram buckets (top 11 bits, then the next 11) in which the
+// running count reaches w or the running size reaches T, the remaining bits
+// set.  Returns false when those cells never reach the budget.
+template <class Sc, class Each>
+__device__ bool hist_stop(const Sc &sc, Each each, int w, int T, uint32_t *hcnt, uint32_t *hwt,
+                          RankShared &sh, uint32_t *thr) {
+    const int tid = threadIdx.x;
+    for (int i = tid; i < kPB; i += kRankThreads) { hcnt[i] = 0; hwt[i] = 0; }
+    __syncthreads();
+    each([&](int64_t c) {
+        const uint32_t key = sc_float_key(sc.score(c));
+        atomicAdd(&hcnt[key >> kL1], 1u);
+        atomicAdd(&hwt[key >> kL1], sc.size(c));
+    });
+    __syncthreads();
+    const int b1 = find_bucket(hcnt, hwt, 0u, 0u, w, T, sh.wsum, &sh.b, sh.pre);
+    const uint32_t pre1_cnt = sh.pre[0], pre1_w = sh.pre[1];
+    __syncthreads();
+    if (b1 >= kPB) return false;
+    for (int i = tid; i < kPB; i += kRankThreads) { hcnt[i] = 0; hwt[i] = 0; }
+    __syncthreads();
+    each([&](int64_t c) {
+        const uint32_t key = sc_float_key(sc.score(c));
+        if ((int)(key >> kL1) != b1) return;
+        atomicAdd(&hcnt[(key >> kL2) & kL2Mask], 1u);
+        atomicAdd(&hwt[(key >> kL2) & kL2Mask], sc.size(c));
+    });
+    __syncthreads();
+    const int b2 = find_bucket(hcnt, hwt, pre1_cnt, pre1_w, w, T, sh.wsum, &sh.b, sh.pre);
+    __syncthreads();
+    // the level-1 bucket reaches the budget, so some level-2 bucket in it does
+    *thr = ((uint32_t)b1 << kL1) | ((uint32_t)min(b2, kPB - 1) << kL2) | kLowBits;
+    return true;
+}
+
 // An upper bound of the stopping score key of the cells of the groups
 // gbeg/gpre describe (ng of them): the key bits of the histogram buckets
 // (top 11 bits, then the next 11) in which the running count reaches w or
@@ -821,6 +858,7 @@ __global__ __launch_bounds__(kRankThreads) void k_rank_pairs(RankKernelArgs k) {
     __shared__ int32_t gbeg[kMaxPass], gsz[kMaxPass], gpre[kMaxPass + 1];
     __shared__ int s_np;
     __shared__ unsigned long long s_acc;
+    __shared__ uint32_t s_sub[kMaxPairs0 / 32];   // the subset's pairs, one bit each
     const int tid = threadIdx.x;
     const int64_t q = k.qlist ? k.qlist[blockIdx.x] : blockIdx.x;
     const int T = query_budget(k.a.Tq, q, k.a.T), w = k.a.w;
@@ -864,6 +902,7 @@ __global__ __launch_bounds__(kRankThreads) void k_rank_pairs(RankKernelArgs k) {
     const int64_t need_c = (int64_t)w - sh.pre[0], need_w = (int64_t)T - sh.pre[1];
     __syncthreads();
     if (tid == 0) { sh.n = 0; s_np = 0; s_acc = 0; }
+    for (int i = tid; i < kMaxPairs0 / 32; i += kRankThreads) s_sub[i] = 0;
     __syncthreads();
     for (int g = tid; g < nu0; g += kRankThreads) {
         const int gb = (int)(sc_float_key(st0[g]) >> kL1);
@@ -883,6 +922,7 @@ __global__ __launch_bounds__(kRankThreads) void k_rank_pairs(RankKernelArgs k) {
             gbeg[o] = k.gstart[g];
             gsz[o] = k.gstart[g + 1] - gbeg[o];
         }
+        atomicOr(&s_sub[g >> 5], 1u << (g & 31));
     }
     __syncthreads();
     // the subset's stopping score (to histogram resolution) bounds the full set's
@@ -892,6 +932,17 @@ __global__ __launch_bounds__(kRankThreads) void k_rank_pairs(RankKernelArgs k) {
         const int np0 = s_np;
         group_prefix(gsz, gpre, np0, sh);
         have = subset_stop(sc, gbeg, gpre, np0, w, T, hcnt, hwt, sh, &thr);
+    } else {
+        // more pairs than the list holds: every cell, those of the marked
+        // pairs taken (consecutive threads read consecutive cells, no thread
+        // walks a long pair alone)
+        auto sub = [&](auto f) {
+            for (int64_t c = tid; c < k.ncells; c += kRankThreads) {
+                const uint32_t g = k.cpair[2 * c] & 0xFFFFu;
+                if ((s_sub[g >> 5] >> (g & 31)) & 1u) f(c);
+            }
+        };
+        have = hist_stop(sc, sub, w, T, hcnt, hwt, sh, &thr);
     }
     __syncthreads();
     if (have) {
@@ -913,9 +964,12 @@ __global__ __launch_bounds__(kRankThreads) void k_rank_pairs(RankKernelArgs k) {
             if (np <= kMaxPass) {
                 for_group_cells(gbeg, gpre, np, [&](int, int64_t c) { f(c); });
             } else {
-                for (int g = tid; g < nu0; g += kRankThreads) {
-                    if (sc_float_key(st0[g] + s1min) > thr) continue;
-                    for (int64_t c = k.gstart[g]; c < k.gstart[g + 1]; ++c) f(c);
+                // more pairs than the list holds: every cell whose pair can
+                // hold a cell scoring at most thr, consecutive threads on
+                // consecutive cells
+                for (int64_t c = tid; c < k.ncells; c += kRankThreads) {
+                    const uint32_t g = k.cpair[2 * c] & 0xFFFFu;
+                    if (sc_float_key(st0[g] + s1min) <= thr) f(c);
                 }
             }
         };
@@ -931,7 +985,7 @@ __global__ __launch_bounds__(kRankThreads) void k_rank_pairs(RankKernelArgs k) {
         if (n <= k.cap) {
             // only the entries up to the exact stopping key are ordered
             uint32_t stop;
-            if (n > kSmallEmit && n <= kCap && list_stop(sc, lkey, n, w, T, sh, &stop))
+            if (n > kSmallEmit && n <= 2 * kCap && list_stop(sc, lkey, n, w, T, sh, &stop))
                 n = list_keep(lkey, n, stop, sh);
             if (n <= kSmallEmit) emit_small(sc, k, q, lkey, n, w, T, sh);
             else emit_sorted(sc, k, q, lkey, n, w, T, sh);
@@ -1619,12 +1673,16 @@ __global__ void k_segment_offsets(int64_t nseg, int64_t len, int64_t *off) {
 
 // SC_RANK_CAP (test hook): a smaller on-chip list, so that tests reach the
 // level-2 refinement and the device-wide fallback.
-int rank_cap() {
+int rank_cap(const sc_index *ix, const RankCellsArgs &a) {
     if (const char *e = getenv("SC_RANK_CAP")) {
         const int v = atoi(e);
         if (v >= 1) return std::min(v, kCap);
     }
-    return kCap;
+    // twice the list when the budget takes, on average, more than a third
+    // of it (T over the vectors per non-empty cell): large budgets would
+    // otherwise send many queries to the re-ranking
+    const double per = ix->ncells > 0 ? std::max(1.0, (double)ix->n / (double)ix->ncells) : 1.0;
+    return 3.0 * (double)a.T / per > (double)kCap ? 2 * kCap : kCap;
 }
 
 // SC_RANK_PAIRS=0 (test hook): rank from the cell records only.
@@ -1752,7 +1810,7 @@ int big_mode() {
 
 template <int W>
 RankKernelArgs rank_args(const sc_index *ix, const RankCellsArgs &a) {
-    return RankKernelArgs{a, ix->kc, ix->dim, ix->ncells, ix->crec, ix->L, nullptr, rank_cap(),
+    return RankKernelArgs{a, ix->kc, ix->dim, ix->ncells, ix->crec, ix->L, nullptr, rank_cap(ix, a),
                           ix->nu[0], ix->nu[1], ix->upairs[0], ix->upairs[1], ix->cpair,
                           ix->gstart, ix->gvec};
 }
@@ -1845,7 +1903,7 @@ int device_wide(sc_ctx *ctx, const sc_index *ix, const RankKernelArgs &k, const 
 template <int W>
 int rank_cells(sc_ctx *ctx, const sc_index *ix, const RankCellsArgs &a) {
     const RankKernelArgs k = rank_args<W>(ix, a);
-    const int cap = rank_cap();
+    const int cap = rank_cap(ix, a);
     int32_t *list = a.flag_list, *dn = a.flag_count;
     int32_t nflag = 0;
     int rc = SC_OK;
