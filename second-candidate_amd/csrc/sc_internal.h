@@ -184,9 +184,9 @@ struct RankCellsArgs {
     int *deferred = nullptr;
     // With deferral: skip [nq] (device) gets 1 for the queries that outgrew
     // the first list and 0 for the others, and their re-ranking runs on the
-    // context's second stream (after ev_fork on the context stream, ending
-    // with ev_join), so that the others can be scanned meanwhile; the caller
-    // waits for ev_join before it reads the re-ranked queries' lists or the
+    // context's second stream (after ev_fork on the context stream), so that
+    // the others can be scanned meanwhile; the caller continues on the
+    // second stream and joins it to the context stream before it reads the
     // deferred count.
     int32_t *skip = nullptr;
 };

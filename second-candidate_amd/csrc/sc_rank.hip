@@ -1949,7 +1949,6 @@ int rank_cells(sc_ctx *ctx, const sc_index *ix, const RankCellsArgs &a) {
             hipLaunchKernelGGL(k_rank_flagged, dim3((unsigned)sc_cdiv(a.nq, 256)), dim3(256), 0,
                                rs, a.qinfo, a.nq, a.defer_list, a.defer_count, nullptr);
             SC_CHECK_LAUNCH();
-            if (a.skip) SC_HIP(hipEventRecord(ctx->ev_join, ctx->side));
             *a.deferred = 1;
             return SC_OK;
         }

@@ -3297,15 +3297,17 @@ static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
             SC_CHECK_LAUNCH();
         }
         if (deferred && ra.skip) {
-            // the re-ranked queries, once their ranking (second stream) is done:
-            // a workgroup per query of the chunk, those past the count exit
-            SC_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
+            // the re-ranked queries on the second stream, after their ranking
+            // and alongside the others' scan: a workgroup per query of the
+            // chunk, those past the count exit; then the streams join
             ScanArgs later = sa;
             later.qlist = list;
             later.qcount = counters;
             hipLaunchKernelGGL(k_scan_topk, dim3((unsigned)cn), dim3(kScanThreads), lds,
-                               ctx->stream, later);
+                               ctx->side, later);
             SC_CHECK_LAUNCH();
+            SC_HIP(hipEventRecord(ctx->ev_join, ctx->side));
+            SC_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
         }
         if (sa.evec) {
             // mixed IMI scan: queries whose kept set outgrew the buffer are
