@@ -29,8 +29,10 @@ namespace {
 
 constexpr int kScanThreads = 256;
 constexpr int kMaxFusedThreads = 512;   // fused workgroups run 256 or 512 threads
-constexpr int kBufCap = 2048;      // streaming top-R buffer (entries)
-constexpr int kRound = 1024;       // candidates a scan workgroup takes per round
+constexpr int kBufCap = 1536;      // scan: streaming top-R buffer (entries)
+constexpr int kScanRound = 512;    // scan: candidates a workgroup takes per round
+constexpr int kScanSlots = kScanRound / kScanThreads;   // ... per thread
+constexpr int kRound = 1024;       // fused search: candidates a workgroup takes per round
 constexpr int kRunCells = 256;     // cells staged per scan run (one per scan thread)
 static_assert(kRunCells == kScanThreads, "a scan run loads one cell per thread");
 constexpr int kSpec = 16;          // traversal pops run ahead of the L[] loads
@@ -950,7 +952,7 @@ __device__ void sort_buffer(ScanShared &S, int n) {
 // candidates in all.  dqr/dqc and the cell arrays may live in LDS or global
 // memory.  All threads of the workgroup call it; `lut` is LDS of mp*kp floats.
 //
-// Each round covers kRound candidates: a thread takes U = kRound / blockDim
+// Each round covers kScanRound candidates: a thread takes U = kScanRound / blockDim
 // of them (positions tid, tid + blockDim, ...), walking a cell pointer that
 // only moves forward, so their code loads are independent and in flight
 // together.  Additions keep the reference's order (k ascending).
@@ -972,17 +974,17 @@ __device__ __forceinline__ void load_codes(const uint8_t *codes, int vidx, uint3
 // Any mp; IMI sequences read dot_cr rows from global memory.
 __device__ void scan_run_generic(const ScanArgs &a, ScanShared &S, const float *lut,
                                  const float *dqc, float qsum, int rn, int rtotal, int base_pos) {
-    const int tid = threadIdx.x, NT = blockDim.x, U = kRound / NT;
+    const int tid = threadIdx.x, NT = blockDim.x, U = kScanRound / NT;
     const int mp = a.mp, kp = a.kp, kc = a.kc, half = mp >> 1;
     const bool imi = a.kind == SC_KIND_IMI;
     const int bs = kp * mp / a.mc;
     int cur = 0, held = -1;       // the cell whose start term and table rows are held
     float d0 = 0.0f;
     const float *r1 = nullptr, *r2 = nullptr;
-    for (int p0 = 0; p0 < rtotal; p0 += kRound) {
-        const bool full = S.nb > kBufCap - kRound;   // read by all before any append
+    for (int p0 = 0; p0 < rtotal; p0 += kScanRound) {
+        const bool full = S.nb > kBufCap - kScanRound;   // read by all before any append
         __syncthreads();
-        if (full) compact_bound(S, a.R, 0.0f, false, kBufCap - kRound);
+        if (full) compact_bound(S, a.R, 0.0f, false, kBufCap - kScanRound);
         for (int u = 0; u < U; ++u) {
             const int p = p0 + u * NT + tid;
             if (p >= rtotal) continue;
@@ -1035,13 +1037,13 @@ __device__ void scan_run_generic(const ScanArgs &a, ScanShared &S, const float *
 // vector is read 16 values at a time so its loads overlap.
 __device__ void scan_run_real(const ScanArgs &a, ScanShared &S, const float *qv, int rn,
                               int rtotal, int base_pos) {
-    const int tid = threadIdx.x, NT = blockDim.x, U = kRound / NT;
+    const int tid = threadIdx.x, NT = blockDim.x, U = kScanRound / NT;
     const int dim = a.dim;
     int cur = 0;
-    for (int p0 = 0; p0 < rtotal; p0 += kRound) {
-        const bool full = S.nb > kBufCap - kRound;   // read by all before any append
+    for (int p0 = 0; p0 < rtotal; p0 += kScanRound) {
+        const bool full = S.nb > kBufCap - kScanRound;   // read by all before any append
         __syncthreads();
-        if (full) compact_bound(S, a.R, 0.0f, false, kBufCap - kRound);
+        if (full) compact_bound(S, a.R, 0.0f, false, kBufCap - kScanRound);
         for (int u = 0; u < U; ++u) {
             const int p = p0 + u * NT + tid;
             if (p >= rtotal) continue;
@@ -1124,14 +1126,14 @@ __device__ __forceinline__ int find_cell(const ScanShared &S, int rn, int p) {
 // while it runs.
 template <int MPW, bool EV>
 struct RoundRegs {
-    uint32_t cw[4][MPW];
-    int vx[4], cl[4];
-    float ev[4];
+    uint32_t cw[kScanSlots][MPW];
+    int vx[kScanSlots], cl[kScanSlots];
+    float ev[kScanSlots];
     __device__ __forceinline__ void fetch(const ScanArgs &a, const ScanShared &S, int rn,
                                           int rtotal, int p0) {
-        const int NT = blockDim.x, U = kRound / NT;
+        const int NT = blockDim.x, U = kScanRound / NT;
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < kScanSlots; ++u) {
             const int p = p0 + u * NT + threadIdx.x;
             vx[u] = -1;
             if (u < U && p < rtotal) {
@@ -1153,13 +1155,13 @@ __device__ void scan_run_packed(const ScanArgs &a, ScanShared &S, const float *l
     const int kp = a.kp;
     RoundRegs<MPW, false> cur, nxt;
     cur.fetch(a, S, rn, rtotal, 0);
-    for (int p0 = 0; p0 < rtotal; p0 += kRound) {
-        if (p0 + kRound < rtotal) nxt.fetch(a, S, rn, rtotal, p0 + kRound);
-        const bool full = S.nb > kBufCap - kRound;   // read by all before any append
+    for (int p0 = 0; p0 < rtotal; p0 += kScanRound) {
+        if (p0 + kScanRound < rtotal) nxt.fetch(a, S, rn, rtotal, p0 + kScanRound);
+        const bool full = S.nb > kBufCap - kScanRound;   // read by all before any append
         __syncthreads();
-        if (full) compact_bound(S, a.R, 0.0f, false, kBufCap - kRound);
+        if (full) compact_bound(S, a.R, 0.0f, false, kBufCap - kScanRound);
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < kScanSlots; ++u) {
             if (cur.vx[u] < 0) continue;
             float d = d0;
 #pragma unroll
@@ -1190,13 +1192,13 @@ __device__ void scan_run_imi_flat(const ScanArgs &a, ScanShared &S, const float 
     const int kp = a.kp, kc = a.kc, bs = kp * 4 * MPW / a.mc;
     RoundRegs<MPW, false> cur, nxt;
     cur.fetch(a, S, rn, rtotal, 0);
-    for (int p0 = 0; p0 < rtotal; p0 += kRound) {
-        if (p0 + kRound < rtotal) nxt.fetch(a, S, rn, rtotal, p0 + kRound);
-        const bool full = S.nb > kBufCap - kRound;   // read by all before any append
+    for (int p0 = 0; p0 < rtotal; p0 += kScanRound) {
+        if (p0 + kScanRound < rtotal) nxt.fetch(a, S, rn, rtotal, p0 + kScanRound);
+        const bool full = S.nb > kBufCap - kScanRound;   // read by all before any append
         __syncthreads();
-        if (full) compact_bound(S, a.R, 0.0f, false, kBufCap - kRound);
+        if (full) compact_bound(S, a.R, 0.0f, false, kBufCap - kScanRound);
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < kScanSlots; ++u) {
             if (cur.vx[u] < 0) continue;
             const uint32_t ci = S.c_info[cur.cl[u]];
             const int h3 = (int)(ci & 0xFFFFu), h4 = (int)(ci >> 16);
@@ -1245,13 +1247,13 @@ __device__ void scan_run_imi_mixed(const ScanArgs &a, ScanShared &S, const _Floa
     const float scale = S.lscale, twoE = S.twoE;
     RoundRegs<MPW, true> cur, nxt;
     cur.fetch(a, S, rn, rtotal, 0);
-    for (int p0 = 0; p0 < rtotal; p0 += kRound) {
-        if (p0 + kRound < rtotal) nxt.fetch(a, S, rn, rtotal, p0 + kRound);
-        const bool full = S.nb > kBufCap - kRound;   // read by all before any append
+    for (int p0 = 0; p0 < rtotal; p0 += kScanRound) {
+        if (p0 + kScanRound < rtotal) nxt.fetch(a, S, rn, rtotal, p0 + kScanRound);
+        const bool full = S.nb > kBufCap - kScanRound;   // read by all before any append
         __syncthreads();
-        if (full) compact_bound(S, a.R, twoE, true, kBufCap - kRound);
+        if (full) compact_bound(S, a.R, twoE, true, kBufCap - kScanRound);
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < kScanSlots; ++u) {
             if (cur.vx[u] < 0) continue;
             float acc = 0.0f;
 #pragma unroll
@@ -1503,7 +1505,7 @@ __device__ void scan_query(const ScanArgs &a, ScanShared &S, float *lut, int64_t
 // Four waves per SIMD: the register budget that lets the four workgroups per
 // CU the LDS allows run together (unconstrained, the compiler's allocation
 // holds it to two; a few cold-path values spill).
-__global__ __launch_bounds__(kScanThreads) __attribute__((amdgpu_waves_per_eu(4, 8)))
+__global__ __launch_bounds__(kScanThreads) __attribute__((amdgpu_waves_per_eu(5, 8)))
 void k_scan_topk(ScanArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     ScanShared &S = *reinterpret_cast<ScanShared *>(smem);
@@ -2997,7 +2999,7 @@ static int search_impl(sc_ctx *ctx, const sc_index *ix, const float *d_q, int64_
         sc_set_error("sc_search: bad arguments");
         return SC_ERR_ARG;
     }
-    if (p->R < 1 || p->R > kBufCap - kRound || p->w < 1 || p->T < 0 ||
+    if (p->R < 1 || p->R > kBufCap - kScanRound || p->w < 1 || p->T < 0 ||
         (p->form != SC_FORM_PLAIN && p->form != SC_FORM_MIXED && p->form != SC_FORM_REAL)) {
         sc_set_error("sc_search: bad parameters (R=%d w=%d T=%d form=%d)", p->R, p->w, p->T,
                      p->form);

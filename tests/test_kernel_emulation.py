@@ -127,7 +127,8 @@ def test_traverse_wave_kernel_emulation(travw, name, args):
 # kRound candidates per query exercise the buffer compaction (bounded
 # select, register-held move), runs of more than kRunCells cells the run
 # staging, T below R the position-ordered output, and a widened margin the
-# mixed form's exact-redo flag.
+# mixed form's exact-redo flag (as does an R whose kept set outgrows the
+# half of the buffer a compaction may keep).
 SCAN_CASES = [
     ("plain, compactions", "8000 32 100 1000 3000 4 0"),
     ("mixed, compactions", "8000 32 100 1000 3000 4 1"),
@@ -136,7 +137,8 @@ SCAN_CASES = [
     ("plain, fewer candidates than R", "8000 32 100 300 60 4 0"),
     ("mixed, fewer candidates than R", "8000 32 100 300 60 4 1"),
     ("plain, R 8", "8000 32 8 2000 5000 3 0"),
-    ("mixed, R 1000", "8000 32 1000 2000 5000 3 1"),
+    ("mixed, R 700", "8000 32 700 2000 5000 3 1"),
+    ("mixed, R 1000: redo flags", "8000 32 1000 2000 5000 3 1"),
 ]
 
 
@@ -164,7 +166,7 @@ def test_scan_kernel_emulation(scan, name, args):
     assert r.returncode == 0, log[-4000:]
     nq = args.split()[5]
     assert ("OK: 0 of %s queries differ" % nq) in log, log[-4000:]
-    if len(args.split()) < 8:
+    if len(args.split()) < 8 and "redo flags" not in name:
         assert "(0 flagged" in log, log[-4000:]
     if "redo flags" in name:
         assert ("(%s flagged" % nq) in log, log[-4000:]
