@@ -2633,6 +2633,10 @@ Plan make_plan(const sc_index *ix, const sc_search_params *p, int64_t nq) {
     int64_t chunk = (int64_t)((size_t)(big ? 8 : 4) << 30) / (int64_t)per;
     chunk = std::max<int64_t>(1, std::min<int64_t>(chunk, std::max<int64_t>(nq, 1)));
     chunk = std::min<int64_t>(chunk, 65536);
+    if (const char *e = getenv("SC_SEARCH_CHUNK")) {   // test hook: smaller passes
+        const long long v = atoll(e);
+        if (v >= 1) chunk = std::min<int64_t>(chunk, v);
+    }
     P.chunk = chunk;
     size_t o = 0;
     auto take = [&](size_t bytes) { size_t r = o; o = (o + bytes + 255) & ~(size_t)255; return r; };

@@ -339,6 +339,21 @@ def test_search_scimi_rerank_past_capacity(ctx, form, monkeypatch):
     ix.close()
 
 
+@pytest.mark.parametrize("form", ["plain", "mixed"])
+def test_search_scimi_small_passes(ctx, form, monkeypatch):
+    """1200 queries in passes of 256 with an on-chip ranking list of 16
+    cells: every pass re-ranks its overflowing queries on the second stream
+    while the others are scanned, and the passes reuse the same scratch and
+    events; ids, distances and counts must equal the oracle's."""
+    monkeypatch.setenv("SC_SEARCH_CHUNK", "256")
+    monkeypatch.setenv("SC_RANK_CAP", "16")
+    ix, oix = _index_pair(ctx, S.KIND_SCIMI, kc=8)
+    f = S.FORM_MIXED if form == "mixed" else S.FORM_PLAIN
+    for R, w, T in ((100, 100000, 3000), (20, 10 ** 6, 10 ** 6)):
+        _compare_search(ctx, ix, oix, S.KIND_SCIMI, R, w, T, nq=1200, form=f)
+    ix.close()
+
+
 @pytest.mark.parametrize("parts", ["1", "3", "auto"])
 @pytest.mark.parametrize("case", SCIMI_CASES)
 def test_search_scimi_pruned_ranking_matches_oracle(ctx, case, parts, monkeypatch):
