@@ -212,7 +212,7 @@ __global__ void k_group_n0(const uint32_t *__restrict__ bg_pair, int32_t ngroups
 __global__ void k_vec_dot(const uint8_t *__restrict__ codes, const int32_t *__restrict__ L,
                           int64_t ncells, const uint64_t *__restrict__ ckeys, int64_t n, int kc,
                           int mp, int kp, const float *__restrict__ dot_cr, float *__restrict__ evec,
-                          uint32_t *__restrict__ vcell) {
+                          uint32_t *__restrict__ vcell, uint4 *__restrict__ mrec) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     int64_t lo = 0, hi = ncells;            // the cell of p: first c with L[c] > p
@@ -236,6 +236,10 @@ __global__ void k_vec_dot(const uint8_t *__restrict__ codes, const int32_t *__re
     for (int k = half; k < mp; ++k) e += dot_cr[(size_t)(kc + n1) * bs + (k - half) * kp + cc[k]];
     evec[p] = e;
     vcell[p] = (uint32_t)n0 | ((uint32_t)n1 << 16);
+    if (mrec) {   // mp == 8
+        const uint2 c = *reinterpret_cast<const uint2 *>(cc);
+        mrec[p] = make_uint4(c.x, c.y, __float_as_uint(e), vcell[p]);
+    }
 }
 
 // out = max |x| over x[0..n) as float bits (non-negative floats order like
@@ -510,14 +514,15 @@ int finish_index(sc_index *ix) {
     SC_CHECK_LAUNCH();
     if (ix->mc == 2 && ix->n > 0) {
         if (hipMalloc(&ix->evec, sizeof(float) * (size_t)ix->n) != hipSuccess ||
-            hipMalloc(&ix->vcell, sizeof(uint32_t) * (size_t)ix->n) != hipSuccess) {
+            hipMalloc(&ix->vcell, sizeof(uint32_t) * (size_t)ix->n) != hipSuccess ||
+            (ix->mp == 8 && hipMalloc(&ix->mrec, sizeof(uint4) * (size_t)ix->n) != hipSuccess)) {
             sc_set_error("index: per-vector sums allocation failed");
             return SC_ERR_NOMEM;
         }
         hipLaunchKernelGGL(k_vec_dot, dim3((unsigned)sc_cdiv(ix->n, 256)), dim3(256), 0,
                            ctx->stream, ix->codes, ix->L, ix->ncells,
                            ix->sparse ? ix->ckeys : nullptr, ix->n, ix->kc, ix->mp, ix->kp,
-                           ix->dot_cr, ix->evec, ix->vcell);
+                           ix->dot_cr, ix->evec, ix->vcell, ix->mp == 8 ? ix->mrec : nullptr);
         SC_CHECK_LAUNCH();
     }
     SC_HIP(hipMemcpyAsync(h, dmax, 3 * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
@@ -876,8 +881,8 @@ extern "C" int sc_index_destroy(sc_index *ix) {
     void *bufs[] = {ix->cq,    ix->pq,   ix->norm_c, ix->norm_r, ix->dot_cr,
                     ix->L,     ix->pid,  ix->codes,  ix->cqT,    ix->pqT,
                     ix->ckeys, ix->crec, ix->upairs[0], ix->upairs[1], ix->cpair,
-                    ix->gstart, ix->gvec, ix->evec, ix->vcell, ix->bg_n0, ix->bg_pair,
-                    ix->bg_start, ix->brec};
+                    ix->gstart, ix->gvec, ix->evec, ix->vcell, ix->mrec, ix->bg_n0,
+                    ix->bg_pair, ix->bg_start, ix->brec};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     delete ix;

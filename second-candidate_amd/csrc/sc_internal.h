@@ -67,6 +67,9 @@ struct sc_index {
     // for the mixed-form scan.
     float *evec;                    // [n]
     uint32_t *vcell;                // [n] (mc = 2): n0 | n1 << 16 of each position's cell
+    // (mc = 2, mp = 8) per position one 16-byte record for the mixed scan:
+    // {codes bytes 0-3, codes bytes 4-7, e_v bits, vcell}, one load each
+    uint4 *mrec;                    // [n] or null
 };
 
 void sc_set_error(const char *fmt, ...);

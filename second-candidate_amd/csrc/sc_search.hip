@@ -684,6 +684,7 @@ struct ScanArgs {
     const int32_t *qcount = nullptr;  // with qlist: workgroups at or past *qcount exit
     const uint32_t *vcell = nullptr;  // [n] n0 | n1 << 16 per position (mc = 2) or null
     const int32_t *skip = nullptr;    // [nq] or null: queries flagged 1 are left to a later scan
+    const uint4 *mrec = nullptr;      // mixed form, mp = 8: {codes, e_v, vcell} per position
 };
 
 struct ScanShared {
@@ -1139,8 +1140,15 @@ struct RoundRegs {
             if (u < U && p < rtotal) {
                 cl[u] = find_cell(S, rn, p);
                 vx[u] = S.c_start[cl[u]] + (p - S.c_pref[cl[u]]);
-                load_codes<MPW>(a.codes, vx[u], cw[u]);
-                if (EV) ev[u] = a.evec[vx[u]];
+                if (EV && MPW == 2 && a.mrec) {   // codes and e_v in one 16-byte load
+                    const uint4 r = a.mrec[vx[u]];
+                    cw[u][0] = r.x;
+                    cw[u][MPW - 1] = r.y;
+                    ev[u] = __uint_as_float(r.z);
+                } else {
+                    load_codes<MPW>(a.codes, vx[u], cw[u]);
+                    if (EV) ev[u] = a.evec[vx[u]];
+                }
             }
         }
     }
@@ -1325,6 +1333,25 @@ __device__ void mixed_setup(const ScanArgs &a, ScanShared &S, _Float16 *lut16, c
 __device__ __forceinline__ float imi_exact(const ScanArgs &a, const float *dqr, const float *dqc,
                                            float qsum, int vidx) {
     const int mp = a.mp, kp = a.kp, kc = a.kc, half = mp >> 1, bs = kp * mp / a.mc;
+    if (a.mrec && mp == 8) {       // the position's 16-byte record: codes, e_v, cell
+        const uint4 r = a.mrec[vidx];
+        const int h3 = (int)(r.w & 0xFFFFu), h4 = (int)(r.w >> 16);
+        const float *r1 = a.dot_cr + (size_t)h3 * bs, *r2 = a.dot_cr + (size_t)(h4 + kc - 1) * bs;
+        float d = (qsum + dqc[h3]) + dqc[kc + h4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int b = k * kp + (int)((r.x >> (8 * k)) & 255u);
+            d += dqr[b];
+            d += r1[b];
+        }
+#pragma unroll
+        for (int k = 4; k < 8; ++k) {
+            const int b = k * kp + (int)((r.y >> (8 * (k - 4))) & 255u);
+            d += dqr[b];
+            d += r2[b];
+        }
+        return d;
+    }
     int h3, h4;
     if (a.vcell) {                 // the index's per-position cell record
         const uint32_t v = a.vcell[vidx];
@@ -3288,6 +3315,7 @@ static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
                     ix->dot_absmax, qinfo, mixed_margin_scale(),
                     ix->sparse ? ix->ckeys : nullptr, ix->ncells};
         sa.vcell = ix->vcell;
+        if (sa.evec) sa.mrec = ix->mrec;
         // LDS: the diff_qr table (or the query row of the real-distance form)
         const size_t lds =
             sizeof(ScanShared) +
