@@ -1301,7 +1301,19 @@ __device__ void mixed_setup(const ScanArgs &a, ScanShared &S, _Float16 *lut16, c
     if (tid == 0) { s_m[0] = 0; s_m[1] = 0; }
     __syncthreads();
     uint32_t m0 = 0, m1 = 0;
-    for (int i = tid; i < lutn; i += blockDim.x) m0 = max(m0, __float_as_uint(fabsf(dqr[i])));
+    // the table (up to 16 entries per thread) read once, kept in registers
+    // for the conversion below
+    constexpr int kHeld = 16;
+    float held[kHeld];
+    const bool hold = lutn <= kHeld * (int)blockDim.x;
+#pragma unroll
+    for (int j = 0; j < kHeld; ++j) {
+        const int i = tid + j * (int)blockDim.x;
+        held[j] = hold && i < lutn ? dqr[i] : 0.0f;
+        m0 = max(m0, __float_as_uint(fabsf(held[j])));
+    }
+    if (!hold)
+        for (int i = tid; i < lutn; i += blockDim.x) m0 = max(m0, __float_as_uint(fabsf(dqr[i])));
     for (int i = tid; i < 2 * a.kc; i += blockDim.x) m1 = max(m1, __float_as_uint(fabsf(dqc[i])));
     for (int o = 32; o > 0; o >>= 1) {
         m0 = max(m0, (uint32_t)__shfl_xor(m0, o));
@@ -1314,7 +1326,15 @@ __device__ void mixed_setup(const ScanArgs &a, ScanShared &S, _Float16 *lut16, c
     (void)frexpf(maxlut, &ex);                     // maxlut < 2^ex
     const float scale = maxlut > 0.0f ? ldexpf(1.0f, ex - 14) : 1.0f;
     const float inv = 1.0f / scale;                // exact: a power of two
-    for (int i = tid; i < lutn; i += blockDim.x) lut16[i] = (_Float16)(dqr[i] * inv);
+    if (hold) {
+#pragma unroll
+        for (int j = 0; j < kHeld; ++j) {
+            const int i = tid + j * (int)blockDim.x;
+            if (i < lutn) lut16[i] = (_Float16)(held[j] * inv);
+        }
+    } else {
+        for (int i = tid; i < lutn; i += blockDim.x) lut16[i] = (_Float16)(dqr[i] * inv);
+    }
     if (tid == 0) {
         const double u = ldexp(1.0, -24), mp = a.mp;
         const double P = (fabs((double)qsum) + 2.0 * maxqc) + mp * ((double)maxlut + a.dot_absmax);
