@@ -1506,8 +1506,10 @@ __device__ void scan_query(const ScanArgs &a, ScanShared &S, float *lut, int64_t
             return;
         }
         // only the entries within the margin of the final R-th smallest
-        // approximate distance can be in the exact top-R
-        if (sum >= a.R && S.nb > a.R) compact_bound(S, a.R, S.twoE, true, kBufCap);
+        // approximate distance can be in the exact top-R (a filter worth
+        // its passes only when it spares threads a second re-rank)
+        if (sum >= a.R && S.nb > a.R && S.nb > kScanThreads)
+            compact_bound(S, a.R, S.twoE, true, kBufCap);
         // exact re-rank of the kept candidates (their positions stay)
         for (int i = tid; i < S.nb; i += blockDim.x) {
             const float e = imi_exact(a, dqr, dqc, qsum, S.idx[i]);
@@ -1519,7 +1521,8 @@ __device__ void scan_query(const ScanArgs &a, ScanShared &S, float *lut, int64_t
     const int R = a.R;
     int nout;
     if (sum >= R) {
-        if (S.nb > R) compact_bound(S, R, 0.0f, false, kBufCap);
+        // (a buffer of at most a sort's 256 entries is sorted whole)
+        if (S.nb > R && S.nb > 256) compact_bound(S, R, 0.0f, false, kBufCap);
         nout = R;
         sort_buffer(S, S.nb);
     }
