@@ -1077,6 +1077,9 @@ struct BigArgs {
     int by_slot;
     // with k.qlist: workgroups at or past *qcount (device) exit at once
     const int32_t *qcount = nullptr;
+    // 1 (mode 0 only): the ranking workgroup emits its query's cell list
+    // itself (no emission launch; its LDS must hold the emission's)
+    int emit = 0;
 };
 constexpr int kSplitWords = 4 + 2 * 4096;
 
@@ -1160,6 +1163,145 @@ __host__ __device__ inline size_t big_lds(int kc, int dim) {
     return 8 * (size_t)kBuckets + 4 * nv + ((2 * (size_t)kc + 15) & ~(size_t)15);
 }
 
+// LDS of the emission: fine-bucket starts, sizes and fill counters.
+constexpr size_t big_emit_lds() { return 4 * (3 * (size_t)kFine + 1); }
+
+// The emission of the query of workgroup slot `slot` from its pruned-ranking
+// scratch: the entries up to the final cut counting-sorted by fine buckets,
+// ordered within their bucket, and the taken prefix written as its cell
+// list.  smem holds big_emit_lds() bytes.  All threads call.
+__device__ void big_emit(const BigArgs &b, int64_t slot, unsigned char *smem) {
+    uint32_t *fstart = reinterpret_cast<uint32_t *>(smem);     // [kFine + 1]
+    uint32_t *fmass = fstart + kFine + 1;                      // [kFine]
+    uint32_t *ffill = fmass + kFine;                           // [kFine]
+    const RankKernelArgs &k = b.k;
+    const RankCellsArgs &a = k.a;
+    __shared__ RankShared sh;
+    __shared__ uint32_t s_lo, s_hi, s_sum;
+    __shared__ int s_count;
+    const int tid = threadIdx.x;
+    const int64_t q = k.qlist ? k.qlist[slot] : slot;
+    const int64_t si = b.by_slot ? slot : q;        // the query's scratch
+    const int T = query_budget(a.Tq, q, a.T), w = a.w;
+    if (empty_query(k, q, T)) return;
+    const uint32_t *aux = b.aux + (size_t)si * 4;
+    if (aux[3]) {                      // still over capacity: the device-wide path
+        flag_overflow(k, q);
+        return;
+    }
+    const int n = (int)aux[0];
+    const int bf = (int)aux[1];
+    const float base = __uint_as_float(aux[2]);
+    const uint64_t *list = b.list + (size_t)si * b.cap;
+    uint64_t *S = b.sorted + (size_t)si * b.cap;
+    uint32_t *SZ = b.size + (size_t)si * b.cap;
+    // the key range of the entries up to the cut
+    uint32_t lo = ~0u, hi = 0u;
+    for (int i = tid; i < n; i += kBigThreads) {
+        const uint32_t key = (uint32_t)(list[i] >> 32);
+        if (dbucket(sc_key_float(key), base) <= bf) { lo = min(lo, key); hi = max(hi, key); }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        lo = min(lo, (uint32_t)__shfl_xor(lo, o));
+        hi = max(hi, (uint32_t)__shfl_xor(hi, o));
+    }
+    if (tid == 0) { s_lo = ~0u; s_hi = 0u; s_count = 0; s_sum = 0; }
+    for (int i = tid; i < kFine; i += kBigThreads) { fstart[i] = 0; fmass[i] = 0; ffill[i] = 0; }
+    __syncthreads();
+    if ((tid & 63) == 0) { atomicMin(&s_lo, lo); atomicMax(&s_hi, hi); }
+    __syncthreads();
+    lo = s_lo;
+    hi = s_hi;
+    int shift = 0;
+    while (lo <= hi && ((hi - lo) >> shift) >= (uint32_t)kFine) ++shift;
+    const int64_t k2 = (int64_t)k.kc * k.kc;
+    // fine histogram (counts, sizes) of the kept entries
+    for (int i = tid; i < n; i += kBigThreads) {
+        const uint64_t e = list[i];
+        const uint32_t key = (uint32_t)(e >> 32);
+        if (dbucket(sc_key_float(key), base) > bf) continue;
+        const int64_t c = (int64_t)(uint32_t)e;
+        const int fb = (int)((key - lo) >> shift);
+        atomicAdd(&fstart[fb], 1u);
+        atomicAdd(&fmass[fb], big_size(b, c, b.brec[c]));
+    }
+    __syncthreads();
+    {
+        constexpr int per = kFine / kBigThreads;
+        uint32_t lc = 0, lm = 0;
+        for (int i = 0; i < per; ++i) { lc += fstart[tid * per + i]; lm += fmass[tid * per + i]; }
+        uint32_t tc, tm;
+        uint32_t rc = block_excl_scan<kBigThreads>(lc, sh.wsum, &tc);
+        uint32_t rm = block_excl_scan<kBigThreads>(lm, sh.wsum + kBigThreads / 64, &tm);
+        for (int i = 0; i < per; ++i) {
+            const uint32_t c = fstart[tid * per + i], m = fmass[tid * per + i];
+            fstart[tid * per + i] = rc;
+            fmass[tid * per + i] = rm;
+            rc += c;
+            rm += m;
+        }
+        if (tid == kBigThreads - 1) fstart[kFine] = rc;
+    }
+    __syncthreads();
+    // scatter into fine-bucket order
+    for (int i = tid; i < n; i += kBigThreads) {
+        const uint64_t e = list[i];
+        const uint32_t key = (uint32_t)(e >> 32);
+        if (dbucket(sc_key_float(key), base) > bf) continue;
+        const int64_t c = (int64_t)(uint32_t)e;
+        const int fb = (int)((key - lo) >> shift);
+        const uint32_t slot = fstart[fb] + atomicAdd(&ffill[fb], 1u);
+        S[slot] = e;
+        SZ[slot] = big_size(b, c, b.brec[c]);
+    }
+    __syncthreads();
+    const int nk = (int)fstart[kFine];
+    int32_t *cells = a.cells + (size_t)q * a.ccap;
+    int32_t *clen = a.clen + (size_t)q * a.ccap;
+    uint32_t *cinfo = a.cinfo + (size_t)q * a.ccap;
+    for (int i = tid; i < nk; i += kBigThreads) {
+        const uint64_t e = S[i];
+        const int fb = (int)(((uint32_t)(e >> 32) - lo) >> shift);
+        const int j0 = (int)fstart[fb], j1 = j0 + (int)ffill[fb];
+        int r = 0;
+        uint32_t p = 0;
+        for (int j = j0; j < j1; ++j) {
+            const bool before = S[j] < e;
+            r += before;
+            p += before ? SZ[j] : 0u;
+        }
+        const int rank = j0 + r;
+        const uint32_t pos = fmass[fb] + p, sz = SZ[i];
+        if (rank < w && (int64_t)pos < T) {
+            if (rank < a.ccap) {
+                const int64_t c = (int64_t)(uint32_t)e;
+                const uint64_t key = b.ckeys[c];
+                const int64_t u0 = (int64_t)(key / (uint64_t)k2), u1 = (int64_t)(key % (uint64_t)k2);
+                cells[rank] = c > 0 ? k.L[c - 1] : 0;
+                clen[rank] = (int32_t)sz;
+                cinfo[rank] = (uint32_t)(u0 / k.kc) | ((uint32_t)(u1 / k.kc) << 16);
+            }
+            atomicMax(&s_count, rank + 1);
+            atomicMax(&s_sum, pos + sz);
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        const int count = s_count;
+        int32_t *qi = a.qinfo + (size_t)q * 8;
+        qi[0] = count; qi[1] = (int32_t)s_sum; qi[2] = count; qi[3] = 0;
+        qi[4] = (count == k.ncells && count < w && (int64_t)s_sum < T) ? 1 : 0;
+        qi[5] = 0; qi[6] = count > a.ccap ? 2 : 0; qi[7] = 0;
+    }
+}
+
+__global__ __launch_bounds__(kBigThreads) void k_rank_big_emit(BigArgs b) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    if (b.qcount && (int)blockIdx.x >= *b.qcount) return;   // the whole workgroup
+    big_emit(b, blockIdx.x, smem);
+}
+
+
 __global__ __launch_bounds__(kBigThreads, 8) void k_rank_big(BigArgs b) {
     extern __shared__ __align__(16) unsigned char smem[];
     const RankKernelArgs &k = b.k;
@@ -1186,7 +1328,10 @@ __global__ __launch_bounds__(kBigThreads, 8) void k_rank_big(BigArgs b) {
     const int start_cut = b.rerun ? (int)aux[1] : b.mode == 2 ? (int)sp[0] : kBuckets;
     __shared__ int s_nord, s_last;
     query_terms<kBigThreads>(k, q, v, sh);
-    if (T <= 0 || k.ncells == 0) return;       // the emit kernel writes the empty result
+    if (T <= 0 || k.ncells == 0) {             // the emission writes the empty result
+        if (b.emit && b.mode == 0) big_emit(b, slot, smem);
+        return;
+    }
     float m0a, m0b, m1a, m1b;
     two_smallest<kBigThreads>(v, kc, s_min, &m0a, &m0b);
     two_smallest<kBigThreads>(v + kc, kc, s_min, &m1a, &m1b);
@@ -1432,6 +1577,10 @@ __global__ __launch_bounds__(kBigThreads, 8) void k_rank_big(BigArgs b) {
         aux[2] = __float_as_uint(base);
         aux[3] = s_napp > b.cap ? 1u : 0u;
     }
+    if (b.emit && b.mode == 0) {
+        __syncthreads();   // this workgroup's list and aux writes, visible to all its threads
+        big_emit(b, slot, smem);
+    }
 }
 
 // Resets the split state of the queries: cut = none, counters and histograms 0.
@@ -1445,134 +1594,6 @@ __global__ void k_big_flagged(const uint32_t *__restrict__ aux, int64_t nq,
                               int32_t *__restrict__ list, int32_t *__restrict__ n) {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q < nq && aux[q * 4 + 3]) list[atomicAdd(n, 1)] = (int32_t)q;
-}
-
-__global__ __launch_bounds__(kBigThreads) void k_rank_big_emit(BigArgs b) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    uint32_t *fstart = reinterpret_cast<uint32_t *>(smem);     // [kFine + 1]
-    uint32_t *fmass = fstart + kFine + 1;                      // [kFine]
-    uint32_t *ffill = fmass + kFine;                           // [kFine]
-    const RankKernelArgs &k = b.k;
-    const RankCellsArgs &a = k.a;
-    __shared__ RankShared sh;
-    __shared__ uint32_t s_lo, s_hi, s_sum;
-    __shared__ int s_count;
-    if (b.qcount && (int)blockIdx.x >= *b.qcount) return;   // the whole workgroup
-    const int tid = threadIdx.x;
-    const int64_t slot = blockIdx.x;
-    const int64_t q = k.qlist ? k.qlist[slot] : slot;
-    const int64_t si = b.by_slot ? slot : q;        // the query's scratch
-    const int T = query_budget(a.Tq, q, a.T), w = a.w;
-    if (empty_query(k, q, T)) return;
-    const uint32_t *aux = b.aux + (size_t)si * 4;
-    if (aux[3]) {                      // still over capacity: the device-wide path
-        flag_overflow(k, q);
-        return;
-    }
-    const int n = (int)aux[0];
-    const int bf = (int)aux[1];
-    const float base = __uint_as_float(aux[2]);
-    const uint64_t *list = b.list + (size_t)si * b.cap;
-    uint64_t *S = b.sorted + (size_t)si * b.cap;
-    uint32_t *SZ = b.size + (size_t)si * b.cap;
-    // the key range of the entries up to the cut
-    uint32_t lo = ~0u, hi = 0u;
-    for (int i = tid; i < n; i += kBigThreads) {
-        const uint32_t key = (uint32_t)(list[i] >> 32);
-        if (dbucket(sc_key_float(key), base) <= bf) { lo = min(lo, key); hi = max(hi, key); }
-    }
-    for (int o = 32; o > 0; o >>= 1) {
-        lo = min(lo, (uint32_t)__shfl_xor(lo, o));
-        hi = max(hi, (uint32_t)__shfl_xor(hi, o));
-    }
-    if (tid == 0) { s_lo = ~0u; s_hi = 0u; s_count = 0; s_sum = 0; }
-    for (int i = tid; i < kFine; i += kBigThreads) { fstart[i] = 0; fmass[i] = 0; ffill[i] = 0; }
-    __syncthreads();
-    if ((tid & 63) == 0) { atomicMin(&s_lo, lo); atomicMax(&s_hi, hi); }
-    __syncthreads();
-    lo = s_lo;
-    hi = s_hi;
-    int shift = 0;
-    while (lo <= hi && ((hi - lo) >> shift) >= (uint32_t)kFine) ++shift;
-    const int64_t k2 = (int64_t)k.kc * k.kc;
-    // fine histogram (counts, sizes) of the kept entries
-    for (int i = tid; i < n; i += kBigThreads) {
-        const uint64_t e = list[i];
-        const uint32_t key = (uint32_t)(e >> 32);
-        if (dbucket(sc_key_float(key), base) > bf) continue;
-        const int64_t c = (int64_t)(uint32_t)e;
-        const int fb = (int)((key - lo) >> shift);
-        atomicAdd(&fstart[fb], 1u);
-        atomicAdd(&fmass[fb], big_size(b, c, b.brec[c]));
-    }
-    __syncthreads();
-    {
-        constexpr int per = kFine / kBigThreads;
-        uint32_t lc = 0, lm = 0;
-        for (int i = 0; i < per; ++i) { lc += fstart[tid * per + i]; lm += fmass[tid * per + i]; }
-        uint32_t tc, tm;
-        uint32_t rc = block_excl_scan<kBigThreads>(lc, sh.wsum, &tc);
-        uint32_t rm = block_excl_scan<kBigThreads>(lm, sh.wsum + kBigThreads / 64, &tm);
-        for (int i = 0; i < per; ++i) {
-            const uint32_t c = fstart[tid * per + i], m = fmass[tid * per + i];
-            fstart[tid * per + i] = rc;
-            fmass[tid * per + i] = rm;
-            rc += c;
-            rm += m;
-        }
-        if (tid == kBigThreads - 1) fstart[kFine] = rc;
-    }
-    __syncthreads();
-    // scatter into fine-bucket order
-    for (int i = tid; i < n; i += kBigThreads) {
-        const uint64_t e = list[i];
-        const uint32_t key = (uint32_t)(e >> 32);
-        if (dbucket(sc_key_float(key), base) > bf) continue;
-        const int64_t c = (int64_t)(uint32_t)e;
-        const int fb = (int)((key - lo) >> shift);
-        const uint32_t slot = fstart[fb] + atomicAdd(&ffill[fb], 1u);
-        S[slot] = e;
-        SZ[slot] = big_size(b, c, b.brec[c]);
-    }
-    __syncthreads();
-    const int nk = (int)fstart[kFine];
-    int32_t *cells = a.cells + (size_t)q * a.ccap;
-    int32_t *clen = a.clen + (size_t)q * a.ccap;
-    uint32_t *cinfo = a.cinfo + (size_t)q * a.ccap;
-    for (int i = tid; i < nk; i += kBigThreads) {
-        const uint64_t e = S[i];
-        const int fb = (int)(((uint32_t)(e >> 32) - lo) >> shift);
-        const int j0 = (int)fstart[fb], j1 = j0 + (int)ffill[fb];
-        int r = 0;
-        uint32_t p = 0;
-        for (int j = j0; j < j1; ++j) {
-            const bool before = S[j] < e;
-            r += before;
-            p += before ? SZ[j] : 0u;
-        }
-        const int rank = j0 + r;
-        const uint32_t pos = fmass[fb] + p, sz = SZ[i];
-        if (rank < w && (int64_t)pos < T) {
-            if (rank < a.ccap) {
-                const int64_t c = (int64_t)(uint32_t)e;
-                const uint64_t key = b.ckeys[c];
-                const int64_t u0 = (int64_t)(key / (uint64_t)k2), u1 = (int64_t)(key % (uint64_t)k2);
-                cells[rank] = c > 0 ? k.L[c - 1] : 0;
-                clen[rank] = (int32_t)sz;
-                cinfo[rank] = (uint32_t)(u0 / k.kc) | ((uint32_t)(u1 / k.kc) << 16);
-            }
-            atomicMax(&s_count, rank + 1);
-            atomicMax(&s_sum, pos + sz);
-        }
-    }
-    __syncthreads();
-    if (tid == 0) {
-        const int count = s_count;
-        int32_t *qi = a.qinfo + (size_t)q * 8;
-        qi[0] = count; qi[1] = (int32_t)s_sum; qi[2] = count; qi[3] = 0;
-        qi[4] = (count == k.ncells && count < w && (int64_t)s_sum < T) ? 1 : 0;
-        qi[5] = 0; qi[6] = count > a.ccap ? 2 : 0; qi[7] = 0;
-    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1755,12 +1776,15 @@ int rank_big(sc_ctx *ctx, const sc_index *ix, const RankKernelArgs &k, const Ran
             SC_CHECK_LAUNCH();
         }
     }
-    const size_t lds = big_lds(ix->kc, ix->dim);
-    if (lds > 150 * 1024) {
+    const size_t lds0 = big_lds(ix->kc, ix->dim);
+    if (lds0 > 150 * 1024) {
         sc_set_error("sc_search: kc=%d too large for the pruned cell ranking", ix->kc);
         return SC_ERR_ARG;
     }
-    SC_TRY(sc_set_max_lds((const void *)k_rank_big, lds));
+    // one workgroup per query (no split): it emits its query's list itself
+    const size_t lds = b.parts > 1 ? lds0 : std::max(lds0, big_emit_lds());
+    b.emit = b.parts > 1 ? 0 : 1;
+    SC_TRY(sc_set_max_lds((const void *)k_rank_big, std::max(lds0, big_emit_lds())));
     if (b.parts > 1) {
         // the warm-up on one workgroup per query, then the query's shares
         BigArgs wu = b;
@@ -1790,14 +1814,16 @@ int rank_big(sc_ctx *ctx, const sc_index *ix, const RankKernelArgs &k, const Ran
         r.parts = 1;              // from the exact cut: one workgroup per query
         r.split = nullptr;
         r.mode = 0;
-        hipLaunchKernelGGL(k_rank_big, dim3((unsigned)nflag), dim3(kBigThreads), lds,
-                           ctx->stream, r);
+        r.emit = 1;
+        hipLaunchKernelGGL(k_rank_big, dim3((unsigned)nflag), dim3(kBigThreads),
+                           std::max(lds0, big_emit_lds()), ctx->stream, r);
         SC_CHECK_LAUNCH();
     }
-    const size_t lds2 = 4 * (3 * (size_t)kFine + 1);
-    hipLaunchKernelGGL(k_rank_big_emit, dim3((unsigned)a.nq), dim3(kBigThreads), lds2,
-                       ctx->stream, b);
-    SC_CHECK_LAUNCH();
+    if (!b.emit) {   // the split queries' lists, from their last workgroups' histograms
+        hipLaunchKernelGGL(k_rank_big_emit, dim3((unsigned)a.nq), dim3(kBigThreads),
+                           big_emit_lds(), ctx->stream, b);
+        SC_CHECK_LAUNCH();
+    }
     return SC_OK;
 }
 
@@ -1833,17 +1859,15 @@ BigArgs retry_args(const sc_index *ix, const RankKernelArgs &k, const RankCellsA
 // by the emit kernel.
 int retry_groups(sc_ctx *ctx, const sc_index *ix, const RankKernelArgs &k,
                  const RankCellsArgs &a, const int32_t *qlist, int32_t n) {
-    const size_t lds = big_lds(ix->kc, ix->dim);
+    const size_t lds = std::max(big_lds(ix->kc, ix->dim), big_emit_lds());
     SC_TRY(sc_set_max_lds((const void *)k_rank_big, lds));
     for (int32_t g0 = 0; g0 < n; g0 += a.retry_cap) {
         const int32_t gn = std::min<int32_t>(a.retry_cap, n - g0);
         BigArgs b = retry_args(ix, k, a);
         b.k.qlist = qlist + g0;
+        b.emit = 1;
         hipLaunchKernelGGL(k_rank_big, dim3((unsigned)gn), dim3(kBigThreads), lds, ctx->stream,
                            b);
-        SC_CHECK_LAUNCH();
-        hipLaunchKernelGGL(k_rank_big_emit, dim3((unsigned)gn), dim3(kBigThreads),
-                           4 * (3 * (size_t)kFine + 1), ctx->stream, b);
         SC_CHECK_LAUNCH();
     }
     return SC_OK;
@@ -1934,16 +1958,14 @@ int rank_cells(sc_ctx *ctx, const sc_index *ix, const RankCellsArgs &a) {
                 SC_HIP(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
                 rs = ctx->side;
             }
-            const size_t lds = big_lds(ix->kc, ix->dim);
+            const size_t lds = std::max(big_lds(ix->kc, ix->dim), big_emit_lds());
             SC_TRY(sc_set_max_lds((const void *)k_rank_big, lds));
             BigArgs b = retry_args(ix, k, a);
             b.k.qlist = list;
             b.qcount = dn;
+            b.emit = 1;
             const unsigned grid = (unsigned)std::min<int64_t>(a.nq, a.retry_cap);
             hipLaunchKernelGGL(k_rank_big, dim3(grid), dim3(kBigThreads), lds, rs, b);
-            SC_CHECK_LAUNCH();
-            hipLaunchKernelGGL(k_rank_big_emit, dim3(grid), dim3(kBigThreads),
-                               4 * (3 * (size_t)kFine + 1), rs, b);
             SC_CHECK_LAUNCH();
             SC_HIP(hipMemsetAsync(a.defer_count, 0, 4, rs));
             hipLaunchKernelGGL(k_rank_flagged, dim3((unsigned)sc_cdiv(a.nq, 256)), dim3(256), 0,
