@@ -309,8 +309,8 @@ __device__ bool hist_select(const Sc &sc, int64_t ncells, int w, int T, int cap,
 // when the list never reaches the budget.  All threads call.
 template <class Sc>
 __device__ bool list_stop(const Sc &sc, const uint64_t *lkey, int n, int w, int T, RankShared &sh,
-                          uint32_t *stop) {
-    __shared__ uint32_t hc[256], hw[256], s_and, s_or;
+                          uint32_t *stop, uint32_t *hc, uint32_t *hw) {
+    __shared__ uint32_t s_and, s_or;
     const int tid = threadIdx.x;
     // this thread's entries (tid + j * kRankThreads, n <= 2 kCap): their
     // sizes are read from global memory once, for every pass below
@@ -593,6 +593,13 @@ __device__ void flag_overflow(const RankKernelArgs &k, int64_t q) {
 
 // LDS of the ranking kernels: region A (the histograms, or the cell list of
 // up to cap entries) of k.abytes, then v [2 kc], then the pair tables.
+// Floats of the v region of the ranking kernels: v [2 kc] (the query row
+// [dim] first), and at least 512 (the pair ranking's list select reuses it).
+__host__ __device__ inline int rank_vwords(int kc, int dim) {
+    const int m = 2 * kc > dim ? 2 * kc : dim;
+    return m > 512 ? m : 512;
+}
+
 __host__ __device__ inline size_t rank_abytes(int cap) {
     int np2 = 64;
     while (np2 < cap) np2 <<= 1;
@@ -648,22 +655,27 @@ __device__ void for_group_cells(const int32_t *gbeg, const int32_t *gpre, int ng
     }
 }
 
-// gpre from the group sizes gsz[0 .. ng) (ng <= kMaxPass).  All threads call.
-__device__ void group_prefix(const int32_t *gsz, int32_t *gpre, int ng, RankShared &sh) {
+// gp[0 .. ng) holds the group sizes on entry and their exclusive prefix on
+// exit, gp[ng] the total (ng <= kMaxPass).  All threads call.
+__device__ void group_prefix(int32_t *gp, int ng, RankShared &sh) {
     constexpr int per = kMaxPass / kRankThreads;
     const int t = threadIdx.x;
+    int32_t sz[per];
     uint32_t loc = 0;
+#pragma unroll
     for (int i = 0; i < per; ++i) {
         const int j = t * per + i;
-        if (j < ng) loc += (uint32_t)gsz[j];
+        sz[i] = j < ng ? gp[j] : 0;
+        loc += (uint32_t)sz[i];
     }
     uint32_t tot;
-    uint32_t run = block_excl_scan(loc, sh.wsum, &tot);
+    uint32_t run = block_excl_scan(loc, sh.wsum, &tot);   // (its barrier orders the reads above)
+#pragma unroll
     for (int i = 0; i < per; ++i) {
         const int j = t * per + i;
-        if (j < ng) { gpre[j] = (int32_t)run; run += (uint32_t)gsz[j]; }
+        if (j < ng) { gp[j] = (int32_t)run; run += (uint32_t)sz[i]; }
     }
-    if (t == 0) gpre[ng] = (int32_t)tot;
+    if (t == 0) gp[ng] = (int32_t)tot;
     __syncthreads();
 }
 
@@ -852,10 +864,10 @@ __global__ __launch_bounds__(kRankThreads) void k_rank_pairs(RankKernelArgs k) {
     uint64_t *lkey = reinterpret_cast<uint64_t *>(smem);           // [cap]
     float *v = reinterpret_cast<float *>(smem + rank_abytes(k.cap));   // [2 kc]
     const int nu0 = k.nu0, nu1 = k.nu1;
-    float *st0 = v + max(2 * k.kc, k.dim);                         // [nu0]
+    float *st0 = v + rank_vwords(k.kc, k.dim);                     // [nu0]
     float *st1 = st0 + nu0;                                        // [nu1]
     __shared__ RankShared sh;
-    __shared__ int32_t gbeg[kMaxPass], gsz[kMaxPass], gpre[kMaxPass + 1];
+    __shared__ int32_t gbeg[kMaxPass], gpre[kMaxPass + 1];   // gpre: sizes, then their prefix
     __shared__ int s_np;
     __shared__ unsigned long long s_acc;
     __shared__ uint32_t s_sub[kMaxPairs0 / 32];   // the subset's pairs, one bit each
@@ -920,7 +932,7 @@ __global__ __launch_bounds__(kRankThreads) void k_rank_pairs(RankKernelArgs k) {
         const int o = atomicAdd(&s_np, 1);
         if (o < kMaxPass) {
             gbeg[o] = k.gstart[g];
-            gsz[o] = k.gstart[g + 1] - gbeg[o];
+            gpre[o] = k.gstart[g + 1] - gbeg[o];
         }
         atomicOr(&s_sub[g >> 5], 1u << (g & 31));
     }
@@ -930,7 +942,7 @@ __global__ __launch_bounds__(kRankThreads) void k_rank_pairs(RankKernelArgs k) {
     bool have = false;
     if (s_np <= kMaxPass) {
         const int np0 = s_np;
-        group_prefix(gsz, gpre, np0, sh);
+        group_prefix(gpre, np0, sh);
         have = subset_stop(sc, gbeg, gpre, np0, w, T, hcnt, hwt, sh, &thr);
     } else {
         // more pairs than the list holds: every cell, those of the marked
@@ -954,12 +966,12 @@ __global__ __launch_bounds__(kRankThreads) void k_rank_pairs(RankKernelArgs k) {
             const int o = atomicAdd(&s_np, 1);
             if (o < kMaxPass) {
                 gbeg[o] = k.gstart[g];
-                gsz[o] = k.gstart[g + 1] - gbeg[o];
+                gpre[o] = k.gstart[g + 1] - gbeg[o];
             }
         }
         __syncthreads();
         const int np = s_np;
-        if (np <= kMaxPass) group_prefix(gsz, gpre, np, sh);
+        if (np <= kMaxPass) group_prefix(gpre, np, sh);
         auto cand = [&](auto f) {
             if (np <= kMaxPass) {
                 for_group_cells(gbeg, gpre, np, [&](int, int64_t c) { f(c); });
@@ -985,7 +997,11 @@ __global__ __launch_bounds__(kRankThreads) void k_rank_pairs(RankKernelArgs k) {
         if (n <= k.cap) {
             // only the entries up to the exact stopping key are ordered
             uint32_t stop;
-            if (n > kSmallEmit && n <= 2 * kCap && list_stop(sc, lkey, n, w, T, sh, &stop))
+            // (the select's two 256-bin histograms in v, free once the pair
+            // scores are taken)
+            uint32_t *hc = reinterpret_cast<uint32_t *>(v);
+            if (n > kSmallEmit && n <= 2 * kCap &&
+                list_stop(sc, lkey, n, w, T, sh, &stop, hc, hc + 256))
                 n = list_keep(lkey, n, stop, sh);
             if (n <= kSmallEmit) emit_small(sc, k, q, lkey, n, w, T, sh);
             else emit_sorted(sc, k, q, lkey, n, w, T, sh);
@@ -1723,7 +1739,7 @@ int launch_rank(sc_ctx *ctx, const sc_index *ix, RankKernelArgs k, const int32_t
     k.cap = cap;
     const bool pairs = use_pair_tables(ix);
     // v [2 kc] also holds the query row while its sums are taken
-    const size_t lds = rank_abytes(cap) + 4 * std::max<size_t>(2 * (size_t)ix->kc, ix->dim) +
+    const size_t lds = rank_abytes(cap) + 4 * (size_t)rank_vwords(ix->kc, ix->dim) +
                        (pairs ? 4 * ((size_t)ix->nu[0] + ix->nu[1]) : 0);
     if (lds > 150 * 1024) return SC_ERR_TOO_LARGE;
     const void *kern = pairs ? (const void *)k_rank_pairs : (const void *)k_rank_cells<W>;
