@@ -211,6 +211,7 @@ def main():
         res[name] = {"ms": ms, "qps": args.nq / (ms * 1e-3), "ids": ids.numpy(),
                      "dists": dists.numpy(), "sums": sums.numpy(),
                      "scan_ms": float(np.mean([s.ms_scan for s in sts])),
+                     "scan_kernel_ms": float(np.mean([s.ms_scan_kernel for s in sts])),
                      "rank_ms": float(np.mean([s.ms_traverse for s in sts])),
                      "lut_ms": float(np.mean([s.ms_lut for s in sts])),
                      "cand": int(sts[-1].candidates), "cells": int(sts[-1].cells)}
@@ -266,12 +267,15 @@ def main():
             traffic_src = os.path.relpath(TRAFFIC_JSON, ROOT)
 
     def roof(name):
-        ach = scan_bytes / (res[name]["scan_ms"] * 1e-3) / 1e9
+        # the scan launches alone (HIP events around them on the context
+        # stream; the scan of the few re-ranked queries overlaps the main one)
+        kms = res[name]["scan_kernel_ms"] or res[name]["scan_ms"]
+        ach = scan_bytes / (kms * 1e-3) / 1e9
         return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4),
                 "traffic": traffic.get(name) if traffic else None,
                 "kernel": "k_scan_topk (%s IMI scan)" % name,
-                "kernel_ms": round(res[name]["scan_ms"], 4),
+                "kernel_ms": round(kms, 4),
                 "algorithmic_bytes_per_launch": int(scan_bytes),
                 "traffic_source": traffic_src}
 
@@ -712,10 +716,12 @@ def large_line(S, ctx, args):
         row["stage_ms_" + name + "_one_batch_of_10k"] = {
             "lut": round(st.ms_lut, 3), "rank": round(st.ms_traverse, 3),
             "scan": round(st.ms_scan, 3)}
+        kms = st.ms_scan_kernel or st.ms_scan   # the scan launches alone
         row["roofline_scan_" + name] = {
-            "bound": "hbm", "achieved": round(algo / (st.ms_scan * 1e-3) / 1e9, 1),
+            "bound": "hbm", "achieved": round(algo / (kms * 1e-3) / 1e9, 1),
             "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(algo / (st.ms_scan * 1e-3) / 8e12, 4),
+            "frac": round(algo / (kms * 1e-3) / 8e12, 4),
+            "kernel_ms": round(kms, 3),
             "algorithmic_bytes_per_launch": int(algo)}
         row["rank_cells_scored_per_query"] = round(st.visited / nq, 1)
         row["rank_gb_per_s"] = round(st.visited * 4 / (st.ms_traverse * 1e-3) / 1e9, 1)

@@ -234,6 +234,10 @@ typedef struct {
      * the number of workgroups (milliseconds): 0 LUT load, 1 coarse sort,
      * 2 traversal, 3 scan, 4 final selection and output */
     double fused_phase_ms[5];
+    /* staged kinds: the scan kernels alone (HIP events around the scan
+     * launches of each pass, with the scan of re-ranked queries that
+     * overlaps it), milliseconds; 0 for the fused kernel */
+    double ms_scan_kernel;
 } sc_search_stats;
 
 /* Batched search of nq queries d_q [nq][dim]; outputs d_ids [nq][R],

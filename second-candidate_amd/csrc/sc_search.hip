@@ -3225,8 +3225,9 @@ static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
         return SC_ERR_TOO_LARGE;
     }
     int64_t fallback_total = 0;
-    // stage timing: five events per chunk on the context stream
-    std::vector<hipEvent_t> ev;
+    // stage timing: five events per chunk on the context stream, and a pair
+    // around each chunk's scan launches
+    std::vector<hipEvent_t> ev, kev;
     auto mark = [&]() {
         if (!st) return;
         hipEvent_t e;
@@ -3350,6 +3351,11 @@ static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
         }
         if (!g)   // a plan sets it before capture
             SC_TRY(sc_set_max_lds((const void *)k_scan_topk, lds));
+        hipEvent_t k0 = nullptr, k1 = nullptr;   // around the scan launches (stats)
+        if (st && hipEventCreate(&k0) == hipSuccess) {
+            (void)hipEventRecord(k0, ctx->stream);
+            kev.push_back(k0);
+        }
         if (hcap_eff > 0 || three || halves) {
             ScanArgs first = sa;
             first.skip = deferred ? ra.skip : nullptr;
@@ -3369,6 +3375,10 @@ static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
             SC_CHECK_LAUNCH();
             SC_HIP(hipEventRecord(ctx->ev_join, ctx->side));
             SC_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
+        }
+        if (k0 && hipEventCreate(&k1) == hipSuccess) {
+            (void)hipEventRecord(k1, ctx->stream);
+            kev.push_back(k1);
         }
         if (sa.evec) {
             // mixed IMI scan: queries whose kept set outgrew the buffer are
@@ -3484,7 +3494,14 @@ static int search_staged(sc_ctx *ctx, const sc_index *ix, const float *d_q, int6
                 if (hipEventElapsedTime(&ms, ev[c + k], ev[c + k + 1]) == hipSuccess) acc[k] += ms;
             }
         st->ms_lut = acc[0]; st->ms_rank = acc[1]; st->ms_traverse = acc[2]; st->ms_scan = acc[3];
+        double ks = 0.0;
+        for (size_t c = 0; c + 1 < kev.size(); c += 2) {
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, kev[c], kev[c + 1]) == hipSuccess) ks += ms;
+        }
+        st->ms_scan_kernel = ks;
         for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+        for (hipEvent_t e : kev) (void)hipEventDestroy(e);
         if (h[4]) {
             sc_set_error("sc_search: %llu queries hit an internal traversal error", h[4]);
             return SC_ERR_HIP;

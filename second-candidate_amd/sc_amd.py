@@ -51,7 +51,8 @@ class SearchStats(C.Structure):
                 ("reranked", C.c_int64), ("heap_fallback", C.c_int64),
                 ("exhausted", C.c_int64), ("ms_lut", C.c_double),
                 ("ms_rank", C.c_double), ("ms_traverse", C.c_double),
-                ("ms_scan", C.c_double), ("fused_phase_ms", C.c_double * 5)]
+                ("ms_scan", C.c_double), ("fused_phase_ms", C.c_double * 5),
+                ("ms_scan_kernel", C.c_double)]
 
 
 def mix_d128(seed=20151001, ncomp=64):
