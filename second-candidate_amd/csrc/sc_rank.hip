@@ -107,6 +107,33 @@ __device__ uint32_t block_excl_scan(uint32_t x, uint32_t *wsum, uint32_t *total)
     return base + incl - x;
 }
 
+// The exclusive prefixes of two values per thread at once (one pair of
+// barriers); wsum holds 2 NT / 64 words.  All threads call it.
+template <int NT = kRankThreads>
+__device__ void block_excl_scan2(uint32_t x, uint32_t y, uint32_t *wsum, uint32_t *ex,
+                                 uint32_t *ey, uint32_t *tx, uint32_t *ty) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    uint32_t ix = x, iy = y;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t a = __shfl_up(ix, o), b = __shfl_up(iy, o);
+        if (lane >= o) { ix += a; iy += b; }
+    }
+    __syncthreads();
+    if (lane == 63) { wsum[wv] = ix; wsum[NT / 64 + wv] = iy; }
+    __syncthreads();
+    uint32_t bx = 0, by = 0, ax = 0, ay = 0;
+    for (int i = 0; i < NT / 64; ++i) {
+        const uint32_t u = wsum[i], v = wsum[NT / 64 + i];
+        if (i < wv) { bx += u; by += v; }
+        ax += u;
+        ay += v;
+    }
+    *ex = bx + ix - x;
+    *ey = by + iy - y;
+    *tx = ax;
+    *ty = ay;
+}
+
 // First bucket at which the running count reaches w or the running size sum
 // reaches T, given the counts/sizes before the histogram (pre_cnt, pre_w).
 // Returns NB when no bucket does; *bcnt/*bw get the totals before the
@@ -119,9 +146,8 @@ __device__ int find_bucket(const uint32_t *hcnt, const uint32_t *hwt, uint32_t p
     const int t = threadIdx.x;
     uint32_t c = 0, s = 0;
     for (int i = 0; i < per; ++i) { c += hcnt[t * per + i]; s += hwt[t * per + i]; }
-    uint32_t tc, ts;
-    const uint32_t bc = block_excl_scan<NT>(c, wsum, &tc);
-    const uint32_t bs = block_excl_scan<NT>(s, wsum + NT / 64, &ts);
+    uint32_t tc, ts, bc, bs;
+    block_excl_scan2<NT>(c, s, wsum, &bc, &bs, &tc, &ts);
     if (t == 0) *s_b = NB;
     __syncthreads();
     uint32_t cc = pre_cnt + bc, cw = pre_w + bs;
@@ -1246,9 +1272,8 @@ __device__ void big_emit(const BigArgs &b, int64_t slot, unsigned char *smem) {
         constexpr int per = kFine / kBigThreads;
         uint32_t lc = 0, lm = 0;
         for (int i = 0; i < per; ++i) { lc += fstart[tid * per + i]; lm += fmass[tid * per + i]; }
-        uint32_t tc, tm;
-        uint32_t rc = block_excl_scan<kBigThreads>(lc, sh.wsum, &tc);
-        uint32_t rm = block_excl_scan<kBigThreads>(lm, sh.wsum + kBigThreads / 64, &tm);
+        uint32_t tc, tm, rc, rm;
+        block_excl_scan2<kBigThreads>(lc, lm, sh.wsum, &rc, &rm, &tc, &tm);
         for (int i = 0; i < per; ++i) {
             const uint32_t c = fstart[tid * per + i], m = fmass[tid * per + i];
             fstart[tid * per + i] = rc;
