@@ -1500,11 +1500,15 @@ __global__ __launch_bounds__(kBigThreads, 8) void k_rank_big(BigArgs b) {
                     if (lane == 0) atomicAdd(&s_scored, (unsigned long long)(ce - cb));
                     uint32_t add_cnt = 0, add_mass = 0;         // this lane's additions
                     for (int cs = cb; cs < ce; cs += 64 * kU) {
-                        if (cs > cb) {                          // a pair of more than kU*64 cells
+                        // a pair of more than kU*64 cells: its next kU*64 are
+                        // loaded while these are scored
+                        const bool more = cs + 64 * kU < ce;    // uniform
+                        uint32_t rx[kU];
+                        if (more) {
 #pragma unroll
                             for (int u = 0; u < kU; ++u) {
-                                const int c = cs + u * 64 + lane;
-                                r[u] = c < ce ? b.brec[c] : 0u;
+                                const int c = cs + (kU + u) * 64 + lane;
+                                rx[u] = c < ce ? b.brec[c] : 0u;
                             }
                         }
 #pragma unroll
@@ -1530,6 +1534,10 @@ __global__ __launch_bounds__(kBigThreads, 8) void k_rank_big(BigArgs b) {
                                 }
                             }
                             if (app) append(adm, key, c);
+                        }
+                        if (more) {
+#pragma unroll
+                            for (int u = 0; u < kU; ++u) r[u] = rx[u];
                         }
                     }
                     // this pair's additions, then a tighter cut when enough was added
